@@ -1,0 +1,17 @@
+"""Loader for the native library (ctypes).  There is no fallback: if the
+library has not been built, loading fails loudly."""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liblocarna_amd.so")
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("liblocarna_amd.so not built; run __graft_entry__.build()")
+        _lib = ctypes.CDLL(LIB_PATH)
+    return _lib

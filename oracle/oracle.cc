@@ -1,0 +1,968 @@
+// CPU oracle for the LocARNA pairwise alignment path -- TEST INFRASTRUCTURE.
+//
+// A direct restatement of the reference algorithm (LocARNA 2.0.1), written
+// for clarity rather than speed: 64-bit scores with the reference's
+// infinity encoding, one dense M matrix reused by all arc-match groups,
+// recursive traceback.  Only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg may use it; the product (locarna_amd/) never does.
+//
+// Citations are file:line into the reference source tree (src/LocARNA/...).
+#include "oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <limits>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <atomic>
+#include <vector>
+
+namespace orc {
+
+typedef long long sc_t;
+
+// infty_int.cc:7-23: finite range [min/5, -min/5); normalized -inf = min/5*2
+const sc_t LMIN = std::numeric_limits<sc_t>::min();
+const sc_t MIN_FINITE = LMIN / 5;
+const sc_t NEG_INF = LMIN / 5 * 2;
+inline bool is_neg_inf(sc_t v) { return v < MIN_FINITE; }
+inline sc_t normalize(sc_t v) { return v < MIN_FINITE ? NEG_INF : v; }  // infty_int.hh:334-341
+
+struct failure : std::runtime_error {
+    explicit failure(const std::string &s) : std::runtime_error(s) {}
+};
+
+template <class T>
+struct Mat {
+    size_t r = 0, c = 0;
+    std::vector<T> v;
+    void resize(size_t r_, size_t c_, T x) { r = r_; c = c_; v.assign(r * c, x); }
+    T &operator()(size_t i, size_t j) { return v[i * c + j]; }
+    const T &operator()(size_t i, size_t j) const { return v[i * c + j]; }
+};
+
+// ---------------------------------------------------------------- RIBOSUM
+// ribosum.cc:42-197, ribosum.hh:384-419 (lower triangular, symmetric fill)
+struct Ribosum {
+    double bm[4][4];
+    double am[16][16];
+    double base_nonstruct[4];
+    double basepair[4][4];
+    double basematch[4][4];
+    double arcmatch[16][16];
+    static int idx(char c) {
+        switch (c) { case 'A': return 0; case 'C': return 1; case 'G': return 2; case 'U': return 3; }
+        return -1;
+    }
+    // ribosum.cc:325-331
+    double basematch_score_corrected(char i, char j) const {
+        double bi = base_nonstruct[idx(i)], bj = base_nonstruct[idx(j)];
+        return log(basematch[idx(i)][idx(j)] / (bi * bj)) / log(2);
+    }
+};
+
+static void read_tri(std::istream &in, int n, double *m) {
+    std::string line;
+    while (std::getline(in, line) && line == "") {}
+    for (int i = 0; i < n; ++i) {
+        if (!std::getline(in, line)) throw failure("ribosum: truncated matrix");
+        std::istringstream ls(line);
+        std::string name;
+        ls >> name;
+        for (int j = 0; j <= i; ++j) {
+            double x;
+            ls >> x;
+            m[i * n + j] = m[j * n + i] = x;
+        }
+    }
+}
+
+static void read_block(std::istream &in, const std::string &header, int n, double *m) {
+    std::string line;
+    auto blank = [](const std::string &s) { return s.find_first_not_of(' ') == std::string::npos; };
+    while (std::getline(in, line) && blank(line)) {}
+    if (line != header) throw failure("ribosum: expected header " + header);
+    for (int i = 0; i < n; ++i) in >> m[i];
+}
+
+Ribosum read_ribosum(const std::string &path) {
+    std::ifstream in(path);
+    if (!in) throw failure("cannot read ribosum file " + path);
+    Ribosum r;
+    std::string line;
+    std::getline(in, line);  // name
+    read_tri(in, 4, &r.bm[0][0]);
+    std::getline(in, line);
+    std::getline(in, line);
+    read_tri(in, 16, &r.am[0][0]);
+    std::getline(in, line);
+    std::getline(in, line);
+    double base_probs[4];
+    read_block(in, "BASE FREQUENCIES", 4, base_probs);
+    read_block(in, "BASE NONSTRUCTURAL FREQUENCIES", 4, r.base_nonstruct);
+    read_block(in, "BASE PAIR FREQUENCIES", 16, &r.basepair[0][0]);
+    read_block(in, "BASE MATCH FREQUENCIES", 16, &r.basematch[0][0]);
+    read_block(in, "BASE PAIR MATCH FREQUENCIES", 256, &r.arcmatch[0][0]);
+    return r;
+}
+
+// ---------------------------------------------------------------- RnaData
+// rna_data.cc:984-1124 (pp 2.0 reader), aux.cc:65-70 (normalize symbols)
+struct RnaData {
+    std::string name;
+    std::string seq;  // 1-based via seq[i-1]
+    std::map<std::pair<int, int>, double> arc_probs;
+    double p_bpcut;
+    int length() const { return (int)seq.size(); }
+    double arc_prob(int i, int j) const {
+        auto it = arc_probs.find({i, j});
+        return it == arc_probs.end() ? 0.0 : it->second;
+    }
+    // rna_data.cc:713-732
+    double prob_paired_upstream(int i) const {
+        double p = 0.0;
+        for (int j = i + 1; j <= length(); j++) p += arc_prob(i, j);
+        return p;
+    }
+    double prob_paired_downstream(int i) const {
+        double p = 0.0;
+        for (int j = 1; j < i; j++) p += arc_prob(j, i);
+        return p;
+    }
+};
+
+static bool get_nonempty_line(std::istream &in, std::string &line) {
+    while (std::getline(in, line)) {
+        if (line.length() > 0 && !isspace((unsigned char)line[0])) {
+            size_t e = line.find_last_not_of(" \t");
+            line = line.substr(0, e + 1);
+            return true;
+        }
+    }
+    return false;
+}
+
+RnaData read_pp(const std::string &text, double min_prob, long long max_bp_span) {
+    std::istringstream in(text);
+    RnaData r;
+    r.p_bpcut = min_prob;
+    std::string line;
+    std::getline(in, line);
+    if (line.compare(0, 5, "#PP 2") != 0) throw failure("not a pp 2.0 file");
+    // sequence block: "name seq" lines until #END; other '#' lines are annotations
+    std::vector<std::string> names;
+    std::map<std::string, std::string> seqs;
+    while (get_nonempty_line(in, line)) {
+        if (line.compare(0, 4, "#END") == 0) break;
+        if (line[0] == '#') continue;  // anchor / structure annotation lines
+        std::istringstream ls(line);
+        std::string name, s;
+        ls >> name >> s;
+        if (!seqs.count(name)) names.push_back(name);
+        seqs[name] += s;
+    }
+    if (names.size() != 1) throw failure("oracle supports single-sequence pp input");
+    r.name = names[0];
+    r.seq = seqs[r.name];
+    for (auto &c : r.seq) {
+        c = (char)toupper((unsigned char)c);
+        if (c == 'T') c = 'U';
+    }
+    if (!get_nonempty_line(in, line) || line != "#SECTION BASEPAIRS")
+        throw failure("Expected base pair section header.");
+    size_t max_span = max_bp_span < 0 ? std::numeric_limits<size_t>::max() : (size_t)max_bp_span;
+    while (get_nonempty_line(in, line)) {
+        if (line[0] == '#') {
+            if (line.compare(0, 4, "#END") == 0) break;
+            if (line.compare(0, 6, "#BPCUT") == 0) {
+                std::istringstream ls(line);
+                std::string dummy;
+                double p;
+                ls >> dummy >> p;
+                if (ls.fail()) throw failure("Cannot parse line \"" + line + "\" in base pairs section.");
+                r.p_bpcut = std::max(p, r.p_bpcut);
+            }
+            continue;
+        }
+        std::istringstream ls(line);
+        size_t i, j;
+        double p;
+        ls >> i >> j >> p;
+        if (ls.fail()) throw failure("Cannot parse line \"" + line + "\" in base pairs section.");
+        if (!(1 <= i && i < j && j <= r.seq.size()))
+            throw failure("Invalid indices in PP input line \"" + line + "\".");
+        if (p <= r.p_bpcut || (j - i + 1) > max_span) continue;  // rna_data.cc:1078
+        r.arc_probs[{(int)i, (int)j}] = p;
+    }
+    return r;
+}
+
+// ---------------------------------------------------------------- BasePairs
+// basepairs.cc:156-220
+struct Arc {
+    int idx, left, right;
+};
+struct BasePairs {
+    int len;
+    std::vector<Arc> arcs;
+    std::vector<std::vector<Arc>> left_adj, right_adj;  // with sentinels
+    BasePairs(const RnaData &rna, double min_prob) : len(rna.length()) {
+        left_adj.resize(len + 1);
+        right_adj.resize(len + 1);
+        for (int i = len - 3; i >= 1; i--) {
+            for (int j = i + 3; j <= len; j++) {
+                double p = rna.arc_prob(i, j);
+                if (p >= min_prob) {
+                    Arc a{(int)arcs.size(), i, j};
+                    arcs.push_back(a);
+                    left_adj[i].push_back(a);
+                    right_adj[j].push_back(a);
+                }
+            }
+        }
+        for (auto &x : left_adj)
+            std::sort(x.begin(), x.end(), [](const Arc &a, const Arc &b) { return a.right < b.right; });
+        for (auto &x : right_adj)
+            std::sort(x.begin(), x.end(), [](const Arc &a, const Arc &b) { return a.left > b.left; });
+        for (auto &x : left_adj) x.push_back(Arc{0, len + 1, len + 1});
+        for (auto &x : right_adj) x.push_back(Arc{0, 0, 0});
+    }
+};
+
+// ---------------------------------------------------------------- band
+// trace_controller.hh:328-336; trace_controller.cc:406-440 (no max-diff),
+// :565-597 (restrict by trace probabilities)
+struct Band {
+    int lenA, lenB;
+    std::vector<size_t> min_col, max_col;
+    Band(int a, int b) : lenA(a), lenB(b), min_col(a + 1, 0), max_col(a + 1, (size_t)b) {}
+    bool is_valid(size_t i, size_t j) const { return min_col[i] <= j && j <= max_col[i]; }
+    bool is_valid_match(size_t i, size_t j) const { return is_valid(i, j) && is_valid(i - 1, j - 1); }
+    Band reverse() const {  // trace_controller.cc:320-338
+        Band r(lenA, lenB);
+        for (int i = 0; i <= lenA; ++i) {
+            r.max_col[lenA - i] = lenB - min_col[i];
+            r.min_col[lenA - i] = lenB - max_col[i];
+        }
+        return r;
+    }
+};
+
+// stral_score.cc:10-98
+struct StralScore {
+    std::string seqA, seqB;
+    const double (*sim)[4];
+    double struct_weight, indel_opening, indel;
+    std::vector<double> p_upA, p_downA, p_upB, p_downB;
+    StralScore(const RnaData &a, const RnaData &b, const double (*sim_)[4], double sw, double io, double ind)
+        : seqA(a.seq), seqB(b.seq), sim(sim_), struct_weight(sw), indel_opening(io), indel(ind) {
+        init(a, p_upA, p_downA);
+        init(b, p_upB, p_downB);
+    }
+    static void init(const RnaData &r, std::vector<double> &up, std::vector<double> &down) {
+        int n = r.length();
+        up.assign(n + 1, 0.0);
+        down.assign(n + 1, 0.0);
+        for (int i = 1; i <= n; i++) {
+            up[i] = r.prob_paired_upstream(i);
+            down[i] = r.prob_paired_downstream(i);
+        }
+    }
+    double sigma(size_t i, size_t j) const {
+        int pairs = 0;
+        double seq_score = 0;
+        int a = Ribosum::idx(seqA[i - 1]), b = Ribosum::idx(seqB[j - 1]);
+        if (a >= 0 && b >= 0) {
+            seq_score += sim[a][b];
+            pairs++;
+        }
+        if (pairs != 0) seq_score /= pairs;
+        return struct_weight * (sqrt(p_downA[i] * p_downB[j]) + sqrt(p_upA[i] * p_upB[j])) + seq_score;
+    }
+    void reverse() {
+        std::reverse(seqA.begin(), seqA.end());
+        std::reverse(seqB.begin(), seqB.end());
+        std::reverse(p_upA.begin() + 1, p_upA.end());
+        std::reverse(p_downA.begin() + 1, p_downA.end());
+        std::reverse(p_upB.begin() + 1, p_upB.end());
+        std::reverse(p_downB.begin() + 1, p_downB.end());
+        std::swap(p_upA, p_downA);
+        std::swap(p_upB, p_downB);
+    }
+};
+
+// edge_probs.icc:7-162 (PFGotoh), :212-268 (PFTraceProbs); no free end gaps,
+// global (locarna's band never uses the local variant for global alignment;
+// flag_local = sequ_local).
+template <class pf_t>
+struct PFGotoh {
+    int lenA, lenB;
+    double temp;
+    bool local;
+    Mat<pf_t> zM, zA, zB, zMr, zAr, zBr;
+    pf_t z;
+    void pf_gotoh(Mat<pf_t> &zMl, Mat<pf_t> &zAl, Mat<pf_t> &zBl, const Band &tc, const StralScore &score) {
+        double g_open = exp(score.indel_opening / temp);
+        double g_ext = exp(score.indel / temp);
+        zMl.resize(lenA + 1, lenB + 1, 0);
+        zAl.resize(lenA + 1, lenB + 1, 0);
+        zBl.resize(lenA + 1, lenB + 1, 0);
+        if (tc.is_valid(0, 0)) zMl(0, 0) = local ? 0 : 1;
+        if (lenA > 0 && tc.is_valid(1, 0)) zAl(1, 0) = g_open * g_ext;
+        if (lenB > 0 && tc.is_valid(0, 1)) zBl(0, 1) = g_open * g_ext;
+        for (int i = 2; i <= lenA; i++) {
+            if (tc.min_col[i] > 0) break;
+            zAl(i, 0) = zAl(i - 1, 0) * g_ext;
+        }
+        for (size_t j = std::max(tc.min_col[0], (size_t)2); j <= std::min(tc.max_col[0], (size_t)lenB); j++)
+            zBl(0, j) = zBl(0, j - 1) * g_ext;
+        for (int i = 1; i <= lenA; i++) {
+            for (size_t j = std::max(tc.min_col[i], (size_t)1); j <= std::min(tc.max_col[i], (size_t)lenB); j++) {
+                double match_score_ij = score.sigma(i, j);
+                double match_ij = exp(match_score_ij / temp);
+                zMl(i, j) = zMl(i - 1, j - 1) * match_ij + zAl(i - 1, j - 1) * match_ij +
+                            zBl(i - 1, j - 1) * match_ij + (local ? match_ij : 0);
+                zAl(i, j) = zAl(i - 1, j) * g_ext + zMl(i - 1, j) * g_open * g_ext + zBl(i - 1, j) * g_open * g_ext;
+                zBl(i, j) = zBl(i, j - 1) * g_ext + zMl(i, j - 1) * g_open * g_ext + zAl(i, j - 1) * g_open * g_ext;
+            }
+        }
+    }
+    PFGotoh(const RnaData &a, const RnaData &b, const Band &tc, const double (*sim)[4], double gap_opening,
+            double gap_extension, double sw, double temp_, bool local_)
+        : lenA(a.length()), lenB(b.length()), temp(temp_), local(local_) {
+        StralScore score(a, b, sim, sw, gap_opening, gap_extension);
+        pf_gotoh(zM, zA, zB, tc, score);
+        score.reverse();
+        pf_gotoh(zMr, zAr, zBr, tc.reverse(), score);
+        if (local) {
+            z = 1;
+            for (int i = 0; i <= lenA; i++)
+                for (int j = 0; j <= lenB; j++) z += zM(i, j);
+        } else {
+            z = zM(lenA, lenB) + zA(lenA, lenB) + zB(lenA, lenB);
+        }
+    }
+    bool fail() const { return z == (pf_t)0 || std::isnan((double)z) || std::isinf((double)z); }
+};
+
+template <class pf_t>
+Mat<double> trace_probs(const RnaData &a, const RnaData &b, const Band &tc, const double (*sim)[4],
+                        double gap_opening, double gap_extension, double sw, double temp, bool local) {
+    PFGotoh<pf_t> g(a, b, tc, sim, gap_opening, gap_extension, sw, temp, local);
+    Mat<double> probs;
+    int lenA = a.length(), lenB = b.length();
+    probs.resize(lenA + 1, lenB + 1, 0.0);
+    double locality_add = local ? 1 : 0;
+    double g_open = exp(gap_opening / temp);
+    for (int i = 0; i <= lenA; i++) {
+        for (size_t j = std::max(tc.min_col[i], (size_t)0); j <= std::min(tc.max_col[i], (size_t)lenB); j++) {
+            size_t ri = lenA - i, rj = lenB - j;
+            pf_t z_ij = g.zM(i, j) * (g.zMr(ri, rj) + g.zAr(ri, rj) + g.zBr(ri, rj) + locality_add) +
+                        g.zA(i, j) * (g.zMr(ri, rj) + g.zAr(ri, rj) / g_open + g.zBr(ri, rj)) +
+                        g.zB(i, j) * (g.zMr(ri, rj) + g.zAr(ri, rj) + g.zBr(ri, rj) / g_open);
+            probs(i, j) = z_ij / g.z;
+        }
+    }
+    return probs;
+}
+
+// trace_controller.cc:565-597
+void restrict_by_trace_probabilities(Band &tc, const Mat<double> &probs, double min_prob) {
+    int rows = tc.lenA;
+    for (int i = 0; i <= rows; ++i) {
+        size_t new_min = tc.max_col[i];
+        size_t new_max = tc.min_col[i];
+        for (size_t j = tc.min_col[i]; j <= tc.max_col[i]; ++j) {
+            if (probs(i, j) >= min_prob) {
+                new_min = std::min(new_min, j);
+                new_max = std::max(new_max, j);
+            }
+        }
+        tc.min_col[i] = std::max(tc.min_col[i], new_min);
+        tc.max_col[i] = std::min(tc.max_col[i], new_max);
+    }
+    size_t max_col = 0;
+    for (int i = 0; i <= rows; ++i) {
+        tc.max_col[i] = std::max(tc.max_col[i], max_col);
+        max_col = tc.max_col[i];
+    }
+    size_t min_col = tc.max_col[rows];
+    for (int i = rows + 1; i > 0;) {
+        --i;
+        tc.min_col[i] = std::min(tc.min_col[i], min_col);
+        min_col = tc.min_col[i];
+    }
+}
+
+// ---------------------------------------------------------------- ArcMatches
+// arc_matches.cc:19-48, 130-188, 76-110, 313-355
+struct ArcMatch {
+    int idx;
+    Arc a, b;
+};
+struct ArcMatches {
+    int lenA, lenB;
+    std::vector<ArcMatch> ams;
+    Mat<std::vector<int>> left_lists, right_lists;
+    ArcMatches(const BasePairs &A, const BasePairs &B, const Band &tc) : lenA(A.len), lenB(B.len) {
+        left_lists.resize(lenA + 1, lenB + 1, {});
+        right_lists.resize(lenA + 1, lenB + 1, {});
+        size_t max_length_diff = std::max(lenA, lenB), max_diff_at_am = std::max(lenA, lenB);
+        for (const Arc &a : A.arcs) {
+            for (const Arc &b : B.arcs) {
+                bool valid = tc.is_valid_match(a.left, b.left) && tc.is_valid_match(a.right, b.right) &&
+                             ((size_t)abs((a.right - a.left) - (b.right - b.left)) <= max_length_diff) &&
+                             (size_t)a.left <= b.left + max_diff_at_am && (size_t)b.left <= a.left + max_diff_at_am &&
+                             (size_t)a.right <= b.right + max_diff_at_am && (size_t)b.right <= a.right + max_diff_at_am;
+                if (!valid) continue;
+                int idx = (int)ams.size();
+                ams.push_back(ArcMatch{idx, a, b});
+                left_lists(a.left, b.left).push_back(idx);
+                right_lists(a.right, b.right).push_back(idx);
+            }
+        }
+        for (int i = 1; i <= lenA; i++)
+            for (int j = 1; j <= lenB; j++) {
+                auto &l = right_lists(i, j);
+                std::sort(l.begin(), l.end(), [&](int x, int y) {
+                    const ArcMatch &u = ams[x], &v = ams[y];
+                    return u.a.left > v.a.left || (u.a.left == v.a.left && u.b.left > v.b.left);
+                });
+            }
+    }
+    void max_right_ends(int al, int bl, int *max_ar, int *max_br) const {
+        for (int x : left_lists(al, bl)) {
+            *max_ar = std::max(*max_ar, ams[x].a.right);
+            *max_br = std::max(*max_br, ams[x].b.right);
+        }
+    }
+};
+
+// ---------------------------------------------------------------- Scoring
+// scoring.cc:28-52, 140-198 (sigma), 200-265 (weights), 271-311 (gap),
+// 368-438 (ribosum arc-match term), 440-554 (arcmatch)
+struct Scoring {
+    const oracle_params &p;
+    const Ribosum *rib;
+    const RnaData &A, &B;
+    const BasePairs &bpsA, &bpsB;
+    Mat<sc_t> sigma_tab;
+    std::vector<sc_t> gapA_tab, gapB_tab, weightsA, weightsB;
+    static sc_t round2score(double d) { return (sc_t)((d < 0) ? (d - 0.5) : (d + 0.5)); }  // scoring.hh:384-387
+    Scoring(const oracle_params &p_, const Ribosum *r, const RnaData &a, const RnaData &b, const BasePairs &ba,
+            const BasePairs &bb)
+        : p(p_), rib(r), A(a), B(b), bpsA(ba), bpsB(bb) {
+        int lenA = A.length(), lenB = B.length();
+        sigma_tab.resize(lenA + 1, lenB + 1, 0);
+        for (int i = 1; i <= lenA; ++i)
+            for (int j = 1; j <= lenB; ++j) sigma_tab(i, j) = sigma_(i, j);
+        gapA_tab.assign(lenA + 1, 0);
+        gapB_tab.assign(lenB + 1, 0);
+        for (int i = 1; i <= lenA; i++) gapA_tab[i] = round2score((1 - 0.0f) * (sc_t)p.indel);
+        for (int i = 1; i <= lenB; i++) gapB_tab[i] = round2score((1 - 0.0f) * (sc_t)p.indel);
+        double peA = p.exp_prob > 0 ? p.exp_prob : 1.0 / (2.0 * lenA);  // aux.hh:216-220
+        double peB = p.exp_prob > 0 ? p.exp_prob : 1.0 / (2.0 * lenB);
+        weights(A, bpsA, peA, weightsA);
+        weights(B, bpsB, peB, weightsB);
+    }
+    sc_t sigma_(int ia, int ib) const {
+        char cA = A.seq[ia - 1], cB = B.seq[ib - 1];
+        sc_t score = 0;
+        if (rib && Ribosum::idx(cA) >= 0 && Ribosum::idx(cB) >= 0) {
+            score += round2score(100.0 * rib->basematch_score_corrected(cA, cB));
+        } else if (cA != 'N' && cB != 'N') {
+            score += (cA == cB) ? p.match : p.mismatch;
+        }
+        return round2score(score / (int)(1 * 1));
+    }
+    sc_t probToWeight(double prob, double pe) const {
+        return round2score(round((sc_t)p.struct_weight * (1 - log(prob) / log(pe))));
+    }
+    void weights(const RnaData &r, const BasePairs &bps, double pe, std::vector<sc_t> &w) {
+        w.resize(bps.arcs.size());
+        for (size_t i = 0; i < bps.arcs.size(); ++i)
+            w[i] = probToWeight(r.arc_prob(bps.arcs[i].left, bps.arcs[i].right), pe);
+    }
+    sc_t riboX_arcmatch_score(const Arc &a, const Arc &b) const {
+        char al = A.seq[a.left - 1], ar = A.seq[a.right - 1], bl = B.seq[b.left - 1], br = B.seq[b.right - 1];
+        double score = 0;
+        int considered = 0;
+        if (Ribosum::idx(al) >= 0 && Ribosum::idx(ar) >= 0 && Ribosum::idx(bl) >= 0 && Ribosum::idx(br) >= 0) {
+            considered++;
+            double pam = rib->arcmatch[Ribosum::idx(al) * 4 + Ribosum::idx(ar)][Ribosum::idx(bl) * 4 + Ribosum::idx(br)];
+            double pa = rib->basepair[Ribosum::idx(al)][Ribosum::idx(ar)];
+            double pb = rib->basepair[Ribosum::idx(bl)][Ribosum::idx(br)];
+            score += log(pam / (pa * pb)) / log(2);
+        }
+        if (considered == 0) return 0;
+        return round2score(100.0 * score / considered);
+    }
+    sc_t arcmatch(const ArcMatch &am) const {
+        sc_t seq_contrib = 0;
+        if (p.tau != 0) {
+            if (rib) seq_contrib = riboX_arcmatch_score(am.a, am.b);
+            else seq_contrib = sigma_tab(am.a.left, am.b.left) + sigma_tab(am.a.right, am.b.right);
+        }
+        return ((sc_t)p.tau * seq_contrib) / 100 + (weightsA[am.a.idx] + weightsB[am.b.idx]);
+    }
+};
+
+// ---------------------------------------------------------------- inputs
+struct Inputs {
+    oracle_params p;
+    std::unique_ptr<Ribosum> rib;
+    RnaData A, B;
+    std::unique_ptr<BasePairs> bpsA, bpsB;
+    std::unique_ptr<Band> band;
+    std::unique_ptr<ArcMatches> am;
+    std::unique_ptr<Scoring> sc;
+};
+
+// src/locarna.cc:398-703 and src/locarna_p.cc:300-477 (parameter plumbing)
+std::unique_ptr<Inputs> prepare(const char *ppA, const char *ppB, const oracle_params &p, const char *rib_path) {
+    auto in = std::make_unique<Inputs>();
+    in->p = p;
+    if (p.use_ribosum) in->rib.reset(new Ribosum(read_ribosum(rib_path)));
+    in->A = read_pp(ppA, p.min_prob, p.max_bp_span);
+    in->B = read_pp(ppB, p.min_prob, p.max_bp_span);
+    int lenA = in->A.length(), lenB = in->B.length();
+    in->band.reset(new Band(lenA, lenB));
+    if (p.min_trace_prob > 0.0) {
+        // main_helper.icc:371-426: basematch scores = ribosum bm matrix (or
+        // match/mismatch); gap costs, struct weight, temperature divided by 100
+        double simm[4][4];
+        if (in->rib) {
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) simm[i][j] = in->rib->bm[i][j];
+        } else {
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) simm[i][j] = i == j ? p.match : p.mismatch;
+        }
+        Mat<double> probs;
+        if (p.band_extended)
+            probs = trace_probs<long double>(in->A, in->B, *in->band, simm, p.indel_opening / 100.0, p.indel / 100.0,
+                                             p.struct_weight / 100.0, p.temperature_alipf / 100.0, p.sequ_local);
+        else
+            probs = trace_probs<double>(in->A, in->B, *in->band, simm, p.indel_opening / 100.0, p.indel / 100.0,
+                                        p.struct_weight / 100.0, p.temperature_alipf / 100.0, p.sequ_local);
+        restrict_by_trace_probabilities(*in->band, probs, p.min_trace_prob);
+    }
+    in->bpsA.reset(new BasePairs(in->A, p.min_prob));
+    in->bpsB.reset(new BasePairs(in->B, p.min_prob));
+    in->am.reset(new ArcMatches(*in->bpsA, *in->bpsB, *in->band));
+    in->sc.reset(new Scoring(p, in->rib.get(), in->A, in->B, *in->bpsA, *in->bpsB));
+    return in;
+}
+
+// ---------------------------------------------------------------- Aligner
+// aligner.cc (state E_NO_NO only: no structure locality, no anchors, no
+// lonely-pair or stacking variants, no free end gaps)
+struct Aligner {
+    Inputs &in;
+    const Band &tc;
+    const ArcMatches &ams;
+    const Scoring &sc;
+    const BasePairs &bpsA, &bpsB;
+    int lenA, lenB;
+    sc_t alpha;
+    Mat<sc_t> M;
+    std::vector<sc_t> E;
+    sc_t F;
+    Mat<sc_t> Dmat;
+    std::vector<sc_t> am_score;
+    int max_i = 0, max_j = 0;
+    std::vector<std::pair<int, int>> edges;
+    std::string structA, structB;
+    long long cnt_G = 0, cnt_C = 0, cnt_K = 0;
+    bool counting = false;
+
+    explicit Aligner(Inputs &in_)
+        : in(in_), tc(*in_.band), ams(*in_.am), sc(*in_.sc), bpsA(*in_.bpsA), bpsB(*in_.bpsB),
+          lenA(in_.A.length()), lenB(in_.B.length()), alpha(in_.p.indel_opening) {
+        M.resize(lenA + 1, lenB + 1, 0);
+        E.assign(lenB + 1, 0);
+        Dmat.resize(bpsA.arcs.size(), bpsB.arcs.size(), NEG_INF);  // aligner.cc:122-123
+        am_score.resize(ams.ams.size());
+        for (size_t k = 0; k < ams.ams.size(); ++k) am_score[k] = sc.arcmatch(ams.ams[k]);
+        structA.assign(lenA, '.');
+        structB.assign(lenB, '.');
+    }
+    sc_t D(const Arc &a, const Arc &b) const { return Dmat(a.idx, b.idx); }
+    sc_t &D(const ArcMatch &am) { return Dmat(am.a.idx, am.b.idx); }
+
+    // aligner.cc:153-233
+    sc_t align_noex(size_t al, size_t bl, size_t i, size_t j) {
+        sc_t max_score = M(i - 1, j - 1) + sc.sigma_tab(i, j);
+        E[j] = std::max(E[j] + sc.gapA_tab[i], M(i - 1, j) + sc.gapA_tab[i] + alpha);
+        max_score = std::max(max_score, E[j]);
+        F = std::max(F + sc.gapB_tab[j], M(i, j - 1) + sc.gapB_tab[j] + alpha);
+        max_score = std::max(max_score, F);
+        const auto &adjlA = bpsA.right_adj[i];
+        const auto &adjlB = bpsB.right_adj[j];
+        for (auto arcA = adjlA.begin(); (size_t)arcA->left > al; ++arcA) {
+            for (auto arcB = adjlB.begin(); (size_t)arcB->left > bl; ++arcB) {
+                sc_t ns = M(arcA->left - 1, arcB->left - 1) + D(*arcA, *arcB);
+                if (ns <= max_score) continue;
+                max_score = ns;
+            }
+        }
+        if (counting) {
+            cnt_C++;
+            for (int x : ams.right_lists(i, j))
+                if ((size_t)ams.ams[x].a.left > al && (size_t)ams.ams[x].b.left > bl) cnt_K++;
+        }
+        return normalize(max_score);
+    }
+
+    // aligner.cc:264-368 (exclA = exclB = false)
+    void init_state(size_t al, size_t ar, size_t bl, size_t br, bool globalA, bool globalB) {
+        for (size_t j = bl; j < br; j++) E[j] = NEG_INF;
+        M(al, bl) = 0;
+        sc_t indel_score = globalA ? alpha : 0;
+        size_t i;
+        for (i = al + 1; i < ar; i++) {
+            if (tc.min_col[i] > bl) break;
+            if (!is_neg_inf(indel_score) && globalA) indel_score += sc.gapA_tab[i];
+            M(i, bl) = indel_score;
+        }
+        for (; i < ar; i++) M(i, tc.min_col[i] - 1) = NEG_INF;
+        indel_score = globalB ? alpha : 0;
+        size_t j;
+        for (j = bl + 1; j < std::min(br, tc.max_col[al] + 1); j++) {
+            if (!is_neg_inf(indel_score) && globalB) indel_score += sc.gapB_tab[j];
+            M(al, j) = indel_score;
+        }
+        for (i = al + 1; i < ar; i++)
+            for (; j < std::min(br, tc.max_col[i] + 1); ++j) M(i - 1, j) = NEG_INF;
+    }
+
+    // aligner.cc:373-437
+    void align_in_arcmatch(size_t al, size_t ar, size_t bl, size_t br) {
+        init_state(al, ar, bl, br, true, true);
+        for (size_t i = al + 1; i < ar; i++) {
+            F = NEG_INF;
+            size_t min_col = std::max(bl + 1, tc.min_col[i]);
+            size_t max_col = std::min(br - 1, tc.max_col[i]);
+            for (size_t j = min_col; j <= max_col; j++) M(i, j) = align_noex(al, bl, i, j);
+        }
+    }
+
+    // aligner.cc:574-610
+    void fill_D_entries(int al, int bl) {
+        for (int x : ams.left_lists(al, bl)) {
+            const ArcMatch &am = ams.ams[x];
+            D(am) = M(am.a.right - 1, am.b.right - 1) + am_score[x];
+        }
+    }
+
+    // aligner.cc:659-732
+    void align_D() {
+        for (size_t al = lenA + 1; al > 1;) {
+            al--;
+            size_t max_bl = std::min((size_t)lenB, tc.max_col[al]);
+            size_t min_bl = std::max((size_t)1, tc.min_col[al]);
+            for (size_t bl = max_bl + 1; bl > min_bl;) {
+                bl--;
+                if (!tc.is_valid_match(al, bl)) continue;
+                int max_ar = (int)al, max_br = (int)bl;
+                ams.max_right_ends((int)al, (int)bl, &max_ar, &max_br);
+                if ((int)al == max_ar || (int)bl == max_br) continue;
+                if (counting) cnt_G++;
+                align_in_arcmatch(al, max_ar, bl, max_br);
+                fill_D_entries((int)al, (int)bl);
+            }
+        }
+    }
+
+    // aligner.cc:736-819 (free end gaps "----")
+    sc_t align_top_level_global() {
+        init_state(0, lenA + 1, 0, lenB + 1, true, true);
+        for (size_t i = 1; i <= (size_t)lenA; i++) {
+            F = NEG_INF;
+            size_t min_col = std::max((size_t)1, tc.min_col[i]);
+            size_t max_col = std::min((size_t)lenB, tc.max_col[i]);
+            for (size_t j = min_col; j <= max_col; j++) M(i, j) = align_noex(0, 0, i, j);
+        }
+        max_i = lenA;
+        max_j = lenB;
+        return M(lenA, lenB);
+    }
+
+    // aligner.cc:823-880 (no anchors: left anchor (lenA+1,lenB+1), right (0,0))
+    sc_t align_top_level_locally() {
+        sc_t max_score = 0;
+        max_i = 0;
+        max_j = 0;
+        init_state(0, lenA + 1, 0, lenB + 1, false, false);
+        for (size_t i = 1; i <= (size_t)lenA; i++) {
+            F = NEG_INF;
+            size_t min_col = std::max((size_t)1, tc.min_col[i]);
+            size_t max_col = std::min((size_t)lenB, tc.max_col[i]);
+            for (size_t j = min_col; j <= max_col; j++) {
+                M(i, j) = align_noex(0, 0, i, j);
+                M(i, j) = std::max((sc_t)0, M(i, j));
+                if (max_score < M(i, j)) {
+                    max_score = M(i, j);
+                    max_i = (int)i;
+                    max_j = (int)j;
+                }
+            }
+        }
+        return max_score;
+    }
+
+    sc_t align() {
+        align_D();
+        return in.p.sequ_local ? align_top_level_locally() : align_top_level_global();
+    }
+
+    // ---- traceback: aligner.cc:967-1028, 1082-1229, 1233-1343, 1345-1358
+    void append(int a, int b) { edges.push_back({a, b}); }
+
+    void trace_arcmatch(const ArcMatch &am) {
+        align_in_arcmatch(am.a.left, am.a.right, am.b.left, am.b.right);
+        trace_in_arcmatch(am.a.left, am.a.right - 1, am.b.left, am.b.right - 1, false);
+    }
+
+    void trace_noex(size_t oal, size_t i, size_t obl, size_t j, bool tl) {
+        if (tc.is_valid(i - 1, j - 1) && M(i, j) == M(i - 1, j - 1) + sc.sigma_tab(i, j)) {
+            trace_in_arcmatch(oal, i - 1, obl, j - 1, tl);
+            append((int)i, (int)j);
+            return;
+        }
+        if (alpha == 0) {
+            if (tc.is_valid(i - 1, j) && M(i, j) == M(i - 1, j) + sc.gapA_tab[i]) {
+                trace_in_arcmatch(oal, i - 1, obl, j, tl);
+                append((int)i, -1);
+                return;
+            }
+            if (tc.is_valid(i, j - 1) && M(i, j) == M(i, j - 1) + sc.gapB_tab[j]) {
+                trace_in_arcmatch(oal, i, obl, j - 1, tl);
+                append(-1, (int)j);
+                return;
+            }
+        } else {
+            sc_t gap_cost = alpha;
+            for (size_t k = 1; i >= oal + k; k++) {
+                if (!tc.is_valid(i - k, j)) break;
+                gap_cost += sc.gapA_tab[i - k + 1];
+                if (M(i, j) == M(i - k, j) + gap_cost) {
+                    trace_in_arcmatch(oal, i - k, obl, j, tl);
+                    for (size_t l = k; l > 0; l--) append((int)(i - l + 1), -1);
+                    return;
+                }
+            }
+            gap_cost = alpha;
+            for (size_t k = 1; j >= obl + k; k++) {
+                if (!tc.is_valid(i, j - k)) break;
+                gap_cost += sc.gapB_tab[j - k + 1];
+                if (M(i, j) == M(i, j - k) + gap_cost) {
+                    trace_in_arcmatch(oal, i, obl, j - k, tl);
+                    for (size_t l = k; l > 0; l--) append(-1, (int)(j - l + 1));
+                    return;
+                }
+            }
+        }
+        if (!tc.is_valid(i - 1, j - 1)) return;
+        for (int x : ams.right_lists(i, j)) {
+            const ArcMatch &am = ams.ams[x];
+            if ((size_t)am.a.left <= oal || (size_t)am.b.left <= obl) continue;
+            size_t al = am.a.left, bl = am.b.left;
+            if (M(i, j) == M(al - 1, bl - 1) + D(am)) {
+                trace_in_arcmatch(oal, al - 1, obl, bl - 1, tl);
+                structA[al - 1] = '(';
+                structA[i - 1] = ')';
+                structB[bl - 1] = '(';
+                structB[j - 1] = ')';
+                append((int)al, (int)bl);
+                trace_arcmatch(am);
+                append((int)i, (int)j);
+                return;
+            }
+        }
+    }
+
+    void trace_in_arcmatch(size_t al, size_t i, size_t bl, size_t j, bool tl) {
+        if (tl && in.p.sequ_local && M(i, j) == 0) return;
+        if (i <= al) {
+            if (!(tl && in.p.sequ_local))
+                for (size_t k = bl + 1; k <= j; k++) append(-1, (int)k);
+            return;
+        }
+        if (j <= bl) {
+            if (!(tl && in.p.sequ_local))
+                for (size_t k = al + 1; k <= i; k++) append((int)k, -1);
+            return;
+        }
+        trace_noex(al, i, bl, j, tl);
+    }
+
+    void trace() {
+        edges.clear();
+        trace_in_arcmatch(0, max_i, 0, max_j, true);
+    }
+};
+
+}  // namespace orc
+
+// ================================================================ C API
+struct oracle_result {
+    std::unique_ptr<orc::Inputs> in;
+    long long score = 0;
+    double pf = 0;
+    std::vector<std::pair<int, int>> edges;
+    std::string sA, sB;
+    std::vector<long long> D;
+    std::vector<double> am_prob, bm_prob, pfD, pfDprime;
+    long long counters[6] = {0, 0, 0, 0, 0, 0};
+};
+
+static void set_err(char *err, int errlen, const char *msg) {
+    if (err && errlen > 0) {
+        std::strncpy(err, msg, errlen - 1);
+        err[errlen - 1] = 0;
+    }
+}
+
+extern "C" oracle_result *oracle_align(const char *ppA, const char *ppB, const oracle_params *p,
+                                       const char *ribosum_path, int do_trace, char *err, int errlen) {
+    try {
+        auto *r = new oracle_result();
+        r->in = orc::prepare(ppA, ppB, *p, ribosum_path);
+        orc::Aligner al(*r->in);
+        al.counting = true;
+        r->score = al.align();
+        r->counters[0] = (long long)r->in->bpsA->arcs.size();
+        r->counters[1] = (long long)r->in->bpsB->arcs.size();
+        r->counters[2] = (long long)r->in->am->ams.size();
+        r->counters[3] = al.cnt_G;
+        r->counters[4] = al.cnt_C;
+        r->counters[5] = al.cnt_K;
+        al.counting = false;
+        r->D.resize(r->in->am->ams.size());
+        for (size_t k = 0; k < r->D.size(); ++k) r->D[k] = al.D(r->in->am->ams[k]);
+        if (do_trace) {
+            al.trace();
+            r->edges = al.edges;
+            r->sA = al.structA;
+            r->sB = al.structB;
+        }
+        return r;
+    } catch (std::exception &e) {
+        set_err(err, errlen, e.what());
+        return nullptr;
+    }
+}
+
+extern "C" void oracle_free(oracle_result *r) { delete r; }
+extern "C" long long oracle_score(const oracle_result *r) { return r->score; }
+extern "C" double oracle_partition_function(const oracle_result *r) { return r->pf; }
+extern "C" int oracle_lengths(const oracle_result *r, int *a, int *b) {
+    *a = r->in->A.length();
+    *b = r->in->B.length();
+    return 0;
+}
+extern "C" int oracle_edges(const oracle_result *r, int *a, int *b, int cap) {
+    int n = (int)r->edges.size();
+    for (int k = 0; k < n && k < cap; ++k) {
+        a[k] = r->edges[k].first;
+        b[k] = r->edges[k].second;
+    }
+    return n;
+}
+extern "C" int oracle_structures(const oracle_result *r, char *sa, char *sb, int cap) {
+    if ((int)r->sA.size() >= cap || (int)r->sB.size() >= cap) return -1;
+    std::strcpy(sa, r->sA.c_str());
+    std::strcpy(sb, r->sB.c_str());
+    return 0;
+}
+extern "C" int oracle_band(const oracle_result *r, int *mn, int *mx, int cap) {
+    const auto &b = *r->in->band;
+    int n = (int)b.min_col.size();
+    for (int i = 0; i < n && i < cap; ++i) {
+        mn[i] = (int)b.min_col[i];
+        mx[i] = (int)b.max_col[i];
+    }
+    return n;
+}
+extern "C" int oracle_arcmatches(const oracle_result *r, int *al, int *ar, int *bl, int *br, long long *score,
+                                 long long *D, int cap) {
+    const auto &ams = r->in->am->ams;
+    int n = (int)ams.size();
+    for (int k = 0; k < n && k < cap; ++k) {
+        al[k] = ams[k].a.left;
+        ar[k] = ams[k].a.right;
+        bl[k] = ams[k].b.left;
+        br[k] = ams[k].b.right;
+        score[k] = r->in->sc->arcmatch(ams[k]);
+        D[k] = k < (int)r->D.size() ? r->D[k] : 0;
+    }
+    return n;
+}
+extern "C" int oracle_tables(const oracle_result *r, long long *sigma, long long *gA, long long *gB) {
+    const auto &sc = *r->in->sc;
+    int la = r->in->A.length(), lb = r->in->B.length();
+    for (int i = 0; i <= la; ++i)
+        for (int j = 0; j <= lb; ++j) sigma[i * (lb + 1) + j] = sc.sigma_tab(i, j);
+    for (int i = 0; i <= la; ++i) gA[i] = sc.gapA_tab[i];
+    for (int j = 0; j <= lb; ++j) gB[j] = sc.gapB_tab[j];
+    return 0;
+}
+extern "C" int oracle_counters(const oracle_result *r, long long *out6) {
+    for (int k = 0; k < 6; ++k) out6[k] = r->counters[k];
+    return 0;
+}
+extern "C" int oracle_am_probs(const oracle_result *r, double *prob, int cap) {
+    int n = (int)r->am_prob.size();
+    for (int k = 0; k < n && k < cap; ++k) prob[k] = r->am_prob[k];
+    return n;
+}
+extern "C" int oracle_bm_probs(const oracle_result *r, double *prob, int cap) {
+    int n = (int)r->bm_prob.size();
+    for (int k = 0; k < n && k < cap; ++k) prob[k] = r->bm_prob[k];
+    return n;
+}
+extern "C" int oracle_pf_D(const oracle_result *r, double *D, double *Dp, int cap) {
+    int n = (int)r->pfD.size();
+    for (int k = 0; k < n && k < cap; ++k) {
+        D[k] = r->pfD[k];
+        Dp[k] = k < (int)r->pfDprime.size() ? r->pfDprime[k] : 0.0;
+    }
+    return n;
+}
+
+extern "C" int oracle_align_many(const char **ppA, const char **ppB, int npairs, const oracle_params *p,
+                                 const char *ribosum_path, int nthreads, long long *scores) {
+    std::atomic<int> next(0), failed(0);
+    auto work = [&]() {
+        for (;;) {
+            int k = next.fetch_add(1);
+            if (k >= npairs) break;
+            try {
+                auto in = orc::prepare(ppA[k], ppB[k], *p, ribosum_path);
+                orc::Aligner al(*in);
+                scores[k] = al.align();
+                al.trace();
+            } catch (...) {
+                failed++;
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    if (nthreads < 1) nthreads = 1;
+    for (int t = 0; t < nthreads; ++t) th.emplace_back(work);
+    for (auto &t : th) t.join();
+    return failed ? -1 : npairs;
+}
+
+extern "C" oracle_result *oracle_pf(const char *, const char *, const oracle_params *, const char *, int, char *err,
+                                    int errlen) {
+    set_err(err, errlen, "not implemented");
+    return nullptr;
+}

@@ -1,0 +1,100 @@
+/* CPU oracle: a plain restatement of LocARNA 2.0.1's pairwise alignment
+ * path, used ONLY as the checker by tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py.  Nothing in the product links or calls it.
+ *
+ * Each function in oracle.cc cites the reference file:line it follows
+ * (paths relative to the LocARNA source tree).
+ *
+ * Pinning (how the oracle itself is checked, see tests/test_oracle_*.py):
+ *   - pp reader, RIBOSUM reader: the reference's own data files
+ *     (Data/Examples/example.pp, Data/Matrices/RIBOSUM85_60) with values
+ *     checked by hand-derivable properties;
+ *   - trace band (PFGotoh): brute-force enumeration of all pairwise
+ *     alignments on tiny inputs (trace probabilities) and the closed form
+ *     of the band for identical sequences;
+ *   - integer DP: exhaustive enumeration of all alignments x nested
+ *     arc-match sets on tiny inputs (max score and the traceback's tie rule),
+ *     reduction to textbook Gotoh / Smith-Waterman-Gotoh without arcs,
+ *     re-scoring of every traced alignment;
+ *   - partition function: exhaustive enumeration of Z and of match
+ *     probabilities on tiny inputs, reduction to the Gotoh forward algorithm
+ *     without arcs.
+ */
+#ifndef LOCARNA_ORACLE_H
+#define LOCARNA_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int indel;             /* --indel            (score, e.g. -350)  */
+    int indel_opening;     /* --indel-opening    (score, e.g. -500)  */
+    int struct_weight;     /* --struct-weight    (200)               */
+    int tau;               /* --tau              (0 locarna cfgs, 50 locarna_p) */
+    int match, mismatch;   /* fallback for non-ACGU symbols (50, 0)  */
+    int temperature_alipf; /* --temperature-alipf (300)              */
+    int sequ_local;        /* --sequ-local                           */
+    int max_bp_span;       /* --maxBPspan, -1 = unlimited            */
+    int band_extended;     /* 1: band PF in long double (locarna), 0: double (locarna_p) */
+    int use_ribosum;       /* 1: RIBOSUM85_60, 0: match/mismatch      */
+    double min_prob;       /* --min-prob                             */
+    double min_trace_prob; /* --min-trace-probability (<=0: no band)  */
+    double exp_prob;       /* --exp-prob; <=0: 1/(2 len)              */
+    double pf_scale;       /* --pf-scale                             */
+    double min_am_prob;    /* --min-am-prob                          */
+    double min_bm_prob;    /* --min-bm-prob                          */
+} oracle_params;
+
+typedef struct oracle_result oracle_result;
+
+/* Integer pairwise alignment (locarna).  ribosum_path: RIBOSUM file.  On
+ * error returns NULL and writes a message into err. */
+oracle_result *oracle_align(const char *ppA, const char *ppB,
+                            const oracle_params *p, const char *ribosum_path,
+                            int do_trace, char *err, int errlen);
+
+/* LocARNA-P: inside, outside, arc-match and base-match probabilities. */
+oracle_result *oracle_pf(const char *ppA, const char *ppB,
+                         const oracle_params *p, const char *ribosum_path,
+                         int include_am_in_bm, char *err, int errlen);
+
+void oracle_free(oracle_result *r);
+
+long long oracle_score(const oracle_result *r);
+double oracle_partition_function(const oracle_result *r);
+int oracle_lengths(const oracle_result *r, int *lenA, int *lenB);
+/* alignment edges: a[k], b[k] with -1 for a gap; returns count */
+int oracle_edges(const oracle_result *r, int *a, int *b, int cap);
+/* structure strings of the traced alignment (dot bracket per sequence) */
+int oracle_structures(const oracle_result *r, char *sa, char *sb, int cap);
+/* band: min_col[0..lenA], max_col[0..lenA] */
+int oracle_band(const oracle_result *r, int *min_col, int *max_col, int cap);
+/* arc matches in reference index order: ends, score s(am), D(am) (integer
+ * path) */
+int oracle_arcmatches(const oracle_result *r, int *al, int *ar, int *bl,
+                      int *br, long long *score, long long *D, int cap);
+/* score tables: sigma (lenA+1)*(lenB+1) row major, gapA[lenA+1], gapB[lenB+1] */
+int oracle_tables(const oracle_result *r, long long *sigma, long long *gapA, long long *gapB);
+/* work counters: [N_A, N_B, A, G, C, K] */
+int oracle_counters(const oracle_result *r, long long *out6);
+/* probabilities (pf): arc matches as in oracle_arcmatches order */
+int oracle_am_probs(const oracle_result *r, double *prob, int cap);
+/* base match probabilities as dense (lenA+1)*(lenB+1) row major */
+int oracle_bm_probs(const oracle_result *r, double *prob, int cap);
+/* arc match inside D (pf) and outside D' in arcmatch order */
+int oracle_pf_D(const oracle_result *r, double *D, double *Dprime, int cap);
+
+/* CPU baseline: align many pairs with nthreads worker threads (scores
+ * only, with traceback).  Returns number of pairs done or <0. */
+int oracle_align_many(const char **ppA, const char **ppB, int npairs,
+                      const oracle_params *p, const char *ribosum_path,
+                      int nthreads, long long *scores);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,167 @@
+"""ctypes wrapper of the CPU oracle (test infrastructure only).
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module; the product never does.
+"""
+import ctypes
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "_build", "liboracle.so")
+RIBOSUM = os.path.join(HERE, "data", "RIBOSUM85_60")
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("indel", ctypes.c_int), ("indel_opening", ctypes.c_int),
+        ("struct_weight", ctypes.c_int), ("tau", ctypes.c_int),
+        ("match", ctypes.c_int), ("mismatch", ctypes.c_int),
+        ("temperature_alipf", ctypes.c_int), ("sequ_local", ctypes.c_int),
+        ("max_bp_span", ctypes.c_int), ("band_extended", ctypes.c_int),
+        ("use_ribosum", ctypes.c_int),
+        ("min_prob", ctypes.c_double), ("min_trace_prob", ctypes.c_double),
+        ("exp_prob", ctypes.c_double), ("pf_scale", ctypes.c_double),
+        ("min_am_prob", ctypes.c_double), ("min_bm_prob", ctypes.c_double),
+    ]
+
+
+def locarna_params(**kw):
+    """Defaults of the benchmark configurations (locarna CLI flags
+    --indel=-350 --indel-opening=-500 --struct-weight=200 --tau=0
+    --min-prob=0.0005 --maxBPspan=100, min-trace-probability 1e-4)."""
+    d = dict(indel=-350, indel_opening=-500, struct_weight=200, tau=0, match=50,
+             mismatch=0, temperature_alipf=300, sequ_local=0, max_bp_span=100,
+             band_extended=1, use_ribosum=1, min_prob=0.0005, min_trace_prob=1e-4,
+             exp_prob=-1.0, pf_scale=1.0, min_am_prob=0.001, min_bm_prob=0.001)
+    d.update(kw)
+    return Params(**d)
+
+
+def locarna_p_params(**kw):
+    """locarna_p defaults used by the probabilistic configurations: tau 50,
+    double band PF, min-trace-probability 1e-5."""
+    d = dict(tau=50, band_extended=0, min_trace_prob=1e-5)
+    d.update(kw)
+    return locarna_params(**d)
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = ctypes.CDLL(LIB)
+        L.oracle_align.restype = ctypes.c_void_p
+        L.oracle_align.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Params),
+                                   ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.oracle_pf.restype = ctypes.c_void_p
+        L.oracle_pf.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Params),
+                                ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.oracle_free.argtypes = [ctypes.c_void_p]
+        L.oracle_score.restype = ctypes.c_longlong
+        L.oracle_score.argtypes = [ctypes.c_void_p]
+        L.oracle_partition_function.restype = ctypes.c_double
+        L.oracle_partition_function.argtypes = [ctypes.c_void_p]
+        for f in ("oracle_lengths", "oracle_edges", "oracle_structures", "oracle_band",
+                  "oracle_arcmatches", "oracle_counters", "oracle_am_probs",
+                  "oracle_bm_probs", "oracle_pf_D", "oracle_tables"):
+            getattr(L, f).argtypes = None
+        _lib = L
+    return _lib
+
+
+class Result:
+    """Integer (or PF) result of one pair."""
+
+    def __init__(self, handle):
+        L = load()
+        self._h = ctypes.c_void_p(handle)
+        a, b = ctypes.c_int(), ctypes.c_int()
+        L.oracle_lengths(self._h, ctypes.byref(a), ctypes.byref(b))
+        self.lenA, self.lenB = a.value, b.value
+        self.score = L.oracle_score(self._h)
+        self.pf = L.oracle_partition_function(self._h)
+        cap = self.lenA + self.lenB + 2
+        ea, eb = (ctypes.c_int * cap)(), (ctypes.c_int * cap)()
+        n = L.oracle_edges(self._h, ea, eb, cap)
+        self.edges = [(ea[k], eb[k]) for k in range(n)]
+        sa = ctypes.create_string_buffer(cap + 2)
+        sb = ctypes.create_string_buffer(cap + 2)
+        L.oracle_structures(self._h, sa, sb, cap + 2)
+        self.structA, self.structB = sa.value.decode(), sb.value.decode()
+        mn, mx = (ctypes.c_int * (self.lenA + 1))(), (ctypes.c_int * (self.lenA + 1))()
+        L.oracle_band(self._h, mn, mx, self.lenA + 1)
+        self.min_col, self.max_col = list(mn), list(mx)
+        c6 = (ctypes.c_longlong * 6)()
+        L.oracle_counters(self._h, c6)
+        self.counters = dict(zip(["N_A", "N_B", "A", "G", "C", "K"], list(c6)))
+        na = self.counters["A"]
+        arrs = [(ctypes.c_int * max(na, 1))() for _ in range(4)]
+        sc, D = (ctypes.c_longlong * max(na, 1))(), (ctypes.c_longlong * max(na, 1))()
+        L.oracle_arcmatches(self._h, *arrs, sc, D, na)
+        self.arcmatches = [(arrs[0][k], arrs[1][k], arrs[2][k], arrs[3][k]) for k in range(na)]
+        self.am_scores = list(sc)[:na]
+        self.D = list(D)[:na]
+        nb = (self.lenA + 1) * (self.lenB + 1)
+        sig = (ctypes.c_longlong * nb)()
+        gA = (ctypes.c_longlong * (self.lenA + 1))()
+        gB = (ctypes.c_longlong * (self.lenB + 1))()
+        L.oracle_tables(self._h, sig, gA, gB)
+        self.sigma = {(i, j): sig[i * (self.lenB + 1) + j]
+                      for i in range(1, self.lenA + 1) for j in range(1, self.lenB + 1)}
+        self.gapA, self.gapB = list(gA), list(gB)
+        pr = (ctypes.c_double * max(na, 1))()
+        npr = L.oracle_am_probs(self._h, pr, na)
+        self.am_probs = list(pr)[:npr]
+        nb = (self.lenA + 1) * (self.lenB + 1)
+        bm = (ctypes.c_double * nb)()
+        nbm = L.oracle_bm_probs(self._h, bm, nb)
+        self.bm_probs = list(bm)[:nbm]
+        pd, pdp = (ctypes.c_double * max(na, 1))(), (ctypes.c_double * max(na, 1))()
+        npd = L.oracle_pf_D(self._h, pd, pdp, na)
+        self.pf_D, self.pf_Dprime = list(pd)[:npd], list(pdp)[:npd]
+        L.oracle_free(self._h)
+        self._h = None
+
+
+def align(ppA, ppB, params=None, trace=True):
+    L = load()
+    params = params or locarna_params()
+    err = ctypes.create_string_buffer(512)
+    h = L.oracle_align(ppA.encode(), ppB.encode(), ctypes.byref(params), RIBOSUM.encode(),
+                       1 if trace else 0, err, 512)
+    if not h:
+        raise RuntimeError("oracle_align: " + err.value.decode())
+    return Result(h)
+
+
+def pf(ppA, ppB, params=None, include_am_in_bm=False):
+    L = load()
+    params = params or locarna_p_params()
+    err = ctypes.create_string_buffer(512)
+    h = L.oracle_pf(ppA.encode(), ppB.encode(), ctypes.byref(params), RIBOSUM.encode(),
+                    1 if include_am_in_bm else 0, err, 512)
+    if not h:
+        raise RuntimeError("oracle_pf: " + err.value.decode())
+    return Result(h)
+
+
+def align_many(pp_pairs, params=None, nthreads=1):
+    L = load()
+    params = params or locarna_params()
+    n = len(pp_pairs)
+    A = (ctypes.c_char_p * n)(*[a.encode() for a, _ in pp_pairs])
+    B = (ctypes.c_char_p * n)(*[b.encode() for _, b in pp_pairs])
+    scores = (ctypes.c_longlong * n)()
+    rc = L.oracle_align_many(A, B, n, ctypes.byref(params), RIBOSUM.encode(), nthreads, scores)
+    if rc < 0:
+        raise RuntimeError("oracle_align_many failed")
+    return list(scores)

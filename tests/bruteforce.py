@@ -1,0 +1,202 @@
+"""Exhaustive enumeration of LocARNA structure alignments on tiny inputs.
+
+Independent of any DP: every alignment of A and B (a monotone path of match /
+deletion / insertion edges through valid cells of the trace band) is
+enumerated together with every nested set of matched arc pairs whose end
+edges are matches of the path.  The score of a structure alignment is
+
+    sum over match edges that are not arc ends  sigma(i, j)
+  + sum over maximal gap runs                   alpha + sum gamma
+  + sum over matched arc pairs                  s(am)
+
+(the model of the reference DP, src/LocARNA/aligner.cc:153-233 with the arc
+match decomposition of :574-610; a group's leading run along its first row /
+column is its init, :264-368).  This pins the oracle's optimum, its
+partition function, its arc-match probabilities and its traceback tie rule
+(aligner.cc:1082-1343: at each cell, backwards, the first of match >
+deletion of length 1, 2, ... > insertion of length 1, 2, ... > arc matches in
+(al desc, bl desc) order that continues an optimal alignment).
+"""
+import math
+
+
+class Problem:
+    def __init__(self, lenA, lenB, sigma, gapA, gapB, alpha, arcmatches, am_score,
+                 min_col, max_col, local=False):
+        self.n, self.m = lenA, lenB
+        self.sigma = sigma                  # dict (i, j) -> score
+        self.gapA, self.gapB = gapA, gapB   # lists indexed 1..n / 1..m
+        self.alpha = alpha
+        self.min_col, self.max_col = min_col, max_col
+        self.local = local
+        self.arcmatches = arcmatches        # list of (al, ar, bl, br)
+        self.am_score = am_score
+        # arc matches by right ends, sorted (al desc, bl desc)
+        self.by_right = {}
+        for k, (al, ar, bl, br) in enumerate(arcmatches):
+            self.by_right.setdefault((ar, br), []).append((al, bl, k))
+        for v in self.by_right.values():
+            v.sort(key=lambda t: (-t[0], -t[1]))
+
+    def valid(self, i, j):
+        return 0 <= i <= self.n and self.min_col[i] <= j <= self.max_col[i]
+
+
+# An alignment of a group is a tuple of steps, LAST step first:
+#   ('M', i, j)          match edge into (i, j)
+#   ('D', k, i, j)       k deletions ending at (i, j)
+#   ('I', k, i, j)       k insertions ending at (i, j)
+#   ('A', rank, k, inner) arc match k (rank in its right-end list) whose
+#                        inner alignment is `inner`
+#   ('pad', i, j)        the group's init run from its origin to (i, j)
+
+def group_alignments(P, r0, c0, i, j, top, memo):
+    """All structure alignments of the group with origin (r0, c0) ending at
+    cell (i, j): list of (score, steps, last_edge_type)."""
+    key = (r0, c0, i, j, top)
+    if key in memo:
+        return memo[key]
+    out = []
+    zero_init = top and P.local
+    if (i, j) == (r0, c0):
+        out.append((0, (), None))
+    elif i == r0:
+        if all(P.valid(r0, c) for c in range(c0, j + 1)):
+            sc = 0 if zero_init else P.alpha + sum(P.gapB[c] for c in range(c0 + 1, j + 1))
+            out.append((sc, (('pad', i, j),), 'I'))
+    elif j == c0:
+        if all(P.valid(r, c0) for r in range(r0, i + 1)):
+            sc = 0 if zero_init else P.alpha + sum(P.gapA[r] for r in range(r0 + 1, i + 1))
+            out.append((sc, (('pad', i, j),), 'D'))
+    elif P.valid(i, j):
+        if P.valid(i - 1, j - 1):
+            for sc, st, _ in group_alignments(P, r0, c0, i - 1, j - 1, top, memo):
+                out.append((sc + P.sigma[(i, j)], (('M', i, j),) + st, 'M'))
+        k = 1
+        while i - k >= r0 and P.valid(i - k, j):
+            cost = P.alpha + sum(P.gapA[r] for r in range(i - k + 1, i + 1))
+            for sc, st, last in group_alignments(P, r0, c0, i - k, j, top, memo):
+                if last != 'D':
+                    out.append((sc + cost, (('D', k, i, j),) + st, 'D'))
+            k += 1
+        k = 1
+        while j - k >= c0 and P.valid(i, j - k):
+            cost = P.alpha + sum(P.gapB[c] for c in range(j - k + 1, j + 1))
+            for sc, st, last in group_alignments(P, r0, c0, i, j - k, top, memo):
+                if last != 'I':
+                    out.append((sc + cost, (('I', k, i, j),) + st, 'I'))
+            k += 1
+        for rank, (al, bl, kk) in enumerate(P.by_right.get((i, j), [])):
+            if al <= r0 or bl <= c0:
+                continue
+            inner = group_alignments(P, al, bl, i - 1, j - 1, False, memo)
+            prefix = group_alignments(P, r0, c0, al - 1, bl - 1, top, memo)
+            for isc, ist, _ in inner:
+                for psc, pst, _ in prefix:
+                    out.append((psc + isc + P.am_score[kk], (('A', rank, kk, ist),) + pst, 'A'))
+    memo[key] = out
+    return out
+
+
+def _decision(step):
+    t = step[0]
+    if t == 'M':
+        return (0,)
+    if t == 'D':
+        return (1, step[1])
+    if t == 'I':
+        return (2, step[1])
+    if t == 'A':
+        return (3, step[1], trace_key(step[3]))
+    return (9,)
+
+
+def trace_key(steps):
+    """Backward decision sequence; the traceback picks the minimum."""
+    return tuple(_decision(s) for s in steps)
+
+
+def edges(P, steps, r0, c0, top):
+    """Forward edge list (a, b), -1 for gaps, and the matched arc indices."""
+    out, arcs = [], []
+    for s in reversed(steps):
+        t = s[0]
+        if t == 'pad':
+            if not (top and P.local):
+                if s[1] == r0:
+                    out += [(-1, c) for c in range(c0 + 1, s[2] + 1)]
+                else:
+                    out += [(r, -1) for r in range(r0 + 1, s[1] + 1)]
+        elif t == 'M':
+            out.append((s[1], s[2]))
+        elif t == 'D':
+            out += [(r, -1) for r in range(s[2] - s[1] + 1, s[2] + 1)]
+        elif t == 'I':
+            out += [(-1, c) for c in range(s[3] - s[1] + 1, s[3] + 1)]
+        elif t == 'A':
+            al, ar, bl, br = P.arcmatches[s[2]]
+            arcs.append(s[2])
+            out.append((al, bl))
+            e2, a2 = edges(P, s[3], al, bl, False)
+            out += e2
+            arcs += a2
+            out.append((ar, br))
+    return out, arcs
+
+
+def best_global(P):
+    """(max score, traced edges, traced arc-match indices, #alignments)."""
+    alns = group_alignments(P, 0, 0, P.n, P.m, True, {})
+    best = max(sc for sc, _, _ in alns)
+    chosen = min((st for sc, st, _ in alns if sc == best), key=trace_key)
+    e, a = edges(P, chosen, 0, 0, True)
+    return best, e, a, len(alns)
+
+
+def best_local(P):
+    """Sequence-local top level (aligner.cc:823-880): alignments start at any
+    valid top-level cell with score 0 (the top level's init row/column are 0)
+    and end at any valid cell; the empty alignment scores 0.  Returns the
+    optimum and the set of optimal end cells."""
+    memo = {}
+    best, ends = 0, []
+    for i in range(P.n + 1):
+        for j in range(P.m + 1):
+            if not P.valid(i, j):
+                continue
+            for r0 in range(i + 1):
+                for c0 in range(j + 1):
+                    if not P.valid(r0, c0):
+                        continue
+                    if (r0, c0) != (0, 0) and (r0 == 0 or c0 == 0):
+                        continue  # starts on the zero init row / column are covered by (0,0)
+                    for sc, st, _ in group_alignments(P, r0, c0, i, j, True, memo):
+                        if sc > best:
+                            best, ends = sc, [(i, j)]
+                        elif sc == best and (i, j) not in ends:
+                            ends.append((i, j))
+    return best, ends
+
+
+def partition_function(P, T):
+    """Z = sum exp(score / T) over all structure alignments and the
+    probability of every arc match and of every base-match edge that is not
+    an arc end."""
+    alns = group_alignments(P, 0, 0, P.n, P.m, True, {})
+    Z = 0.0
+    am = [0.0] * len(P.arcmatches)
+    bm = {}
+    for sc, st, _ in alns:
+        w = math.exp(sc / T)
+        Z += w
+        e, arcs = edges(P, st, 0, 0, True)
+        ends = set()
+        for k in arcs:
+            am[k] += w
+            al, ar, bl, br = P.arcmatches[k]
+            ends.add((al, bl))
+            ends.add((ar, br))
+        for (a, b) in e:
+            if a > 0 and b > 0 and (a, b) not in ends:
+                bm[(a, b)] = bm.get((a, b), 0.0) + w
+    return Z, [x / Z for x in am], {k: v / Z for k, v in bm.items()}, len(alns)

@@ -1,0 +1,92 @@
+"""Pins the oracle's integer DP, traceback and band against exhaustive
+enumeration on tiny inputs (tests/bruteforce.py)."""
+import pytest
+
+import bruteforce as bf
+from oracle import oracle
+from tiny import tiny_pair
+
+
+def problem_from(r, alpha, local=False):
+    return bf.Problem(r.lenA, r.lenB, r.sigma, r.gapA, r.gapB, alpha, r.arcmatches,
+                      r.am_scores, r.min_col, r.max_col, local=local)
+
+
+CASES = [
+    # (label, params)
+    ("ribosum_fullband", dict(min_trace_prob=0.0)),
+    ("ribosum_band", dict(min_trace_prob=0.02)),
+    ("matchmismatch_ties", dict(use_ribosum=0, indel=-50, indel_opening=-100,
+                                struct_weight=100, min_trace_prob=0.0)),
+    ("tau_band", dict(tau=100, min_trace_prob=0.05)),
+    ("linear_gaps", dict(indel_opening=0, min_trace_prob=0.0)),
+]
+
+
+@pytest.mark.parametrize("label,kw", CASES, ids=[c[0] for c in CASES])
+@pytest.mark.parametrize("seed", range(12))
+def test_global_optimum_and_traceback(label, kw, seed):
+    ppA, ppB = tiny_pair(seed * 7 + sum(map(ord, label)))
+    params = oracle.locarna_params(min_prob=0.01, max_bp_span=-1, **kw)
+    r = oracle.align(ppA, ppB, params)
+    P = problem_from(r, params.indel_opening)
+    best, edges, arcs, nal = bf.best_global(P)
+    assert nal > 0
+    assert r.score == best
+    if params.indel_opening != 0:
+        # tie rule of the affine traceback (aligner.cc:1082-1229)
+        assert r.edges == edges
+        traced = sorted(r.arcmatches[k] for k in arcs)
+        got = []
+        sA, sB = r.structA, r.structB
+        for (al, ar, bl, br) in r.arcmatches:
+            if sA[al - 1] == '(' and sA[ar - 1] == ')' and sB[bl - 1] == '(' and sB[br - 1] == ')':
+                if (al, bl) in r.edges and (ar, br) in r.edges:
+                    got.append((al, ar, bl, br))
+        assert set(traced) <= set(got)
+
+
+@pytest.mark.parametrize("seed", range(12))
+def test_local_optimum(seed):
+    ppA, ppB = tiny_pair(1000 + seed)
+    params = oracle.locarna_params(min_prob=0.01, max_bp_span=-1, sequ_local=1,
+                                   min_trace_prob=0.0, indel=-150, indel_opening=-200)
+    r = oracle.align(ppA, ppB, params)
+    P = problem_from(r, params.indel_opening, local=True)
+    best, ends = bf.best_local(P)
+    assert r.score == best
+    if best > 0:
+        # the traced alignment ends in the first optimal cell in row-major order
+        last = [e for e in r.edges if e[0] > 0 and e[1] > 0][-1]
+        assert last == min(ends)
+
+
+TIE_CASES = [
+    # homopolymers with equal-probability arcs sharing ends: many tied optima
+    ("AAAAAAA", [(1, 6), (2, 6), (1, 7), (2, 7)], "AAAAAAA", [(1, 6), (2, 6), (1, 7), (2, 7)]),
+    ("GGGGGG", [(1, 5), (2, 5), (2, 6)], "GGGGGGG", [(1, 5), (2, 6), (3, 7), (2, 7)]),
+    ("ACACAC", [(1, 5), (2, 6), (1, 6)], "CACACA", [(1, 5), (2, 6), (1, 6)]),
+    ("UUUUUUU", [(1, 4), (4, 7), (1, 7)], "UUUUUU", [(1, 4), (3, 6), (1, 6)]),
+    # arc matches (1,6)~(1,7) and (1,6)~(2,7) tie: the gap moves inside or
+    # outside the arc; the traceback must take bl = 2 first (bl descending)
+    ("AAAAAA", [(1, 6)], "AAAAAAA", [(1, 7), (2, 7)]),
+    ("AAAAAAA", [(1, 7), (2, 7)], "AAAAAA", [(1, 6)]),
+]
+
+
+@pytest.mark.parametrize("case", range(len(TIE_CASES)))
+@pytest.mark.parametrize("weights", [(100, -50, -100), (200, -100, -300), (50, 0, -50)])
+def test_tie_rich_traceback(case, weights):
+    from locarna_amd.synth import pp_text
+    sA, arcsA, sB, arcsB = TIE_CASES[case]
+    ppA = pp_text("a", sA, [(i, j, 0.5) for i, j in arcsA])
+    ppB = pp_text("b", sB, [(i, j, 0.5) for i, j in arcsB])
+    sw, indel, opening = weights
+    params = oracle.locarna_params(min_prob=0.01, max_bp_span=-1, use_ribosum=0,
+                                   struct_weight=sw, indel=indel, indel_opening=opening,
+                                   min_trace_prob=0.0)
+    r = oracle.align(ppA, ppB, params)
+    P = problem_from(r, params.indel_opening)
+    best, edges, arcs, _ = bf.best_global(P)
+    assert r.score == best
+    assert r.edges == edges
