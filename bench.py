@@ -1,0 +1,136 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X LocARNA path: batched all-pairs pairwise
+alignment (mlocarna's all-pairs stage, reference src/Utils/mlocarna:
+3547-3640, one `locarna` per pair) on one GPU.
+
+Workload "allpairs64": 64 seeded random RNAs, lengths uniform in [60, 200],
+GC uniform in [0.35, 0.65]; base-pair probabilities from the library's
+McCaskill-style folding (cut 0.0005, span <= 100); all 2016 pairs aligned
+globally with locarna's flags --indel=-350 --indel-opening=-500
+--struct-weight=200 --tau=0 --min-prob=0.0005 --maxBPspan=100 (trace band
+at the default min-trace-probability 1e-4).  One step = score and
+traceback of all 2016 pairs, inputs resident in HBM, as one hipGraph-
+replayed persistent launch.
+
+Prints one JSON line (metric pairs/s).  Also reported: GCUPS, the plain
+one-pair-per-launch rate of the same kernel, the integer-op roofline of the
+alignment kernel (HIP events on its stream), and the CPU oracle timed on the
+host cores over the same pairs (scores compared bit for bit).
+"""
+import argparse
+import concurrent.futures as cf
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Peak int32 VALU issue rate: 256 CUs x 4 SIMD x 16 lanes x 2.4 GHz.
+P_INT32_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="allpairs64")
+    ap.add_argument("--mode", type=int, default=2, help="0 persistent launch, 2 hipGraph replay")
+    ap.add_argument("--cpu-baseline", choices=["auto", "skip"], default="auto")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-sample", type=int, default=0, help="pairs timed on the CPU (0: all)")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (the all-pairs stage would shard "
+              "pairs across GPUs by estimated work; see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from locarna_amd import api, synth
+
+    pps, pairs, span = synth.workload(args.workload)
+    params = api.locarna_params(max_bp_span=span)
+    nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    with cf.ThreadPoolExecutor(nthreads) as ex:
+        prepared = list(ex.map(lambda ab: api.Pair(pps[ab[0]], pps[ab[1]], params), pairs))
+    npairs = len(prepared)
+    batch = api.Batch(prepared, device=torch.cuda.current_device())
+
+    # untimed counting run: DP cells and arc candidates the kernel evaluates
+    batch.run(mode=0, trace=True, count=True)
+    cells = cands = 0
+    for k in range(npairs):
+        c, q = batch.counters(k)
+        cells += c
+        cands += q
+    arcm = sum(p.num_arcmatches for p in prepared)
+
+    for _ in range(args.warmup):
+        batch.run(mode=args.mode, trace=True)
+    torch.cuda.synchronize()
+    dev_ms = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        dev_ms.append(batch.run(mode=args.mode, trace=True))
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = npairs / (ms_per_step / 1e3)
+    kern_ms = sum(dev_ms) / len(dev_ms)
+    gpu_scores = [batch.score(k) for k in range(npairs)]
+
+    plain = None
+    if not args.no_plain:
+        plain_ms = batch.run(mode=1, trace=True)
+        plain = npairs / (plain_ms / 1e3)
+
+    # integer-op roofline of lcg_align_pairs (W_int = 11 C + 2 K + A)
+    w_int = 11 * cells + 2 * cands + arcm
+    achieved = w_int / (kern_ms / 1e3) / 1e12
+    roofline = {"bound": "valu", "achieved": round(achieved, 4), "peak": round(P_INT32_TOPS, 2),
+                "unit": "TOPS", "frac": round(achieved / P_INT32_TOPS, 5), "traffic": None,
+                "kernel": "lcg_align_pairs", "kernel_ms": round(kern_ms, 3),
+                "int_ops_per_launch": w_int}
+
+    cpu = None
+    if args.cpu_baseline == "auto":
+        from oracle import oracle
+        n_s = args.cpu_sample or npairs
+        idx = list(range(0, npairs, max(1, npairs // n_s)))[:n_s]
+        sample = [(pps[pairs[k][0]], pps[pairs[k][1]]) for k in idx]
+        t0 = time.perf_counter()
+        cpu_scores = oracle.align_many(sample, oracle.locarna_params(max_bp_span=span), nthreads=nthreads)
+        dt = time.perf_counter() - t0
+        exact = sum(1 for k, s in zip(idx, cpu_scores) if gpu_scores[k] == s)
+        cpu = {"value": round(len(sample) / dt, 3), "unit": "pairs/s", "cores": nthreads, "kind": "port",
+               "sample": "%d of %d pairs (score + traceback per pair, CPU oracle restating locarna)"
+                         % (len(sample), npairs),
+               "scores_bit_exact": "%d/%d" % (exact, len(sample))}
+
+    out = {
+        "metric": "pairs/sec (batched all-pairs LocARNA alignment, score + traceback)",
+        "value": round(value, 2), "unit": "pairs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": args.workload, "pairs": npairs, "max_bp_span": span,
+                   "launch": {0: "persistent", 2: "hipGraph"}.get(args.mode, str(args.mode)),
+                   "waves_per_workgroup": batch.waves, "workgroups": batch.workgroups,
+                   "lds_bytes": batch.lds_bytes},
+        "gcups": round(cells / (ms_per_step / 1e3) / 1e9, 3),
+        "dp_cells_per_step": cells, "arc_candidates_per_step": cands,
+        "plain_pairs_per_sec": round(plain, 2) if plain else None,
+        "batched_vs_plain": round(value / plain, 2) if plain else None,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["vs_cpu_baseline"] = round(value / cpu["value"], 2)
+    batch.destroy()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,121 @@
+/* C ABI of the MI355X-native LocARNA pairwise alignment path.
+ *
+ * Plain C types only; every call returns an int status (0 = ok, negative =
+ * error class, see LCG_E_*).  A caller that replaces LocARNA's in-process
+ * alignment (src/locarna.cc:727-785: Aligner(AlignerParams).align() /
+ * trace() / get_alignment()) or mlocarna's per-pair subprocess loop
+ * (src/Utils/mlocarna:3547-3640 -> :2991-3022) binds these entry points.
+ *
+ * Flow: lcg_pair_prepare() reads two #PP 2.0 texts and builds, on the host,
+ * what the reference builds before Aligner::align(): RnaData
+ * (rna_data.cc:984-1124), the trace band (TraceController +
+ * restrict_trace_by_probabilities, main_helper.icc:408-426), BasePairs /
+ * ArcMatches (basepairs.cc:156-220, arc_matches.cc:130-188) and Scoring
+ * (scoring.cc:28-554).  lcg_batch_create() uploads any number of prepared
+ * pairs into one device arena; lcg_batch_run() runs the gfx950 kernels
+ * (fill of the arc-match table D, top level, traceback) for all pairs;
+ * results are then read per pair.
+ */
+#ifndef LOCARNA_AMD_H
+#define LOCARNA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LCG_OK 0
+#define LCG_E_ARG (-1)      /* invalid argument                              */
+#define LCG_E_INPUT (-2)    /* malformed input (LocARNA::failure equivalent)  */
+#define LCG_E_HIP (-3)      /* HIP runtime error                             */
+#define LCG_E_CAPACITY (-4) /* problem exceeds the kernel's capacity          */
+#define LCG_E_STATE (-5)    /* result requested before it was computed        */
+
+/* Parameters of a run; field meaning and defaults follow the locarna /
+ * locarna_p command lines (src/locarna.cc:83-272, src/locarna_p.cc:95-200). */
+typedef struct {
+    int32_t indel;             /* --indel                                   */
+    int32_t indel_opening;     /* --indel-opening                           */
+    int32_t struct_weight;     /* --struct-weight                           */
+    int32_t tau;               /* --tau                                     */
+    int32_t match, mismatch;   /* --match / --mismatch (non-RIBOSUM symbols) */
+    int32_t temperature_alipf; /* --temperature-alipf                       */
+    int32_t sequ_local;        /* --sequ-local                              */
+    int32_t max_bp_span;       /* --maxBPspan (-1: unlimited)               */
+    int32_t band_extended;     /* trace band PF in long double (locarna) or
+                                  double (locarna_p default)                 */
+    int32_t use_ribosum;       /* --use-ribosum (built-in RIBOSUM85_60)      */
+    double min_prob;           /* --min-prob                                */
+    double min_trace_prob;     /* --min-trace-probability (<= 0: no band)    */
+    double exp_prob;           /* --exp-prob (<= 0: 1/(2 length))            */
+    double pf_scale;           /* --pf-scale                                */
+    double min_am_prob;        /* --min-am-prob                             */
+    double min_bm_prob;        /* --min-bm-prob                             */
+} lcg_params;
+
+typedef struct lcg_pair lcg_pair;
+typedef struct lcg_batch lcg_batch;
+
+/* Host preparation of one pair (replaces the construction of RnaData,
+ * TraceController, ArcMatches and Scoring in src/locarna.cc:455-703). */
+int lcg_pair_prepare(const char *ppA, const char *ppB, const lcg_params *params, lcg_pair **out, char *err,
+                     int errlen);
+void lcg_pair_free(lcg_pair *pair);
+/* lengths, number of arc matches, number of arc-match groups and an
+ * estimate of the DP cells of the pair */
+int lcg_pair_info(const lcg_pair *pair, int32_t *lenA, int32_t *lenB, int32_t *num_arcmatches, int32_t *num_groups,
+                  int64_t *cells);
+/* trace band: min_col[0..lenA], max_col[0..lenA]
+ * (TraceController::min_col / max_col, trace_controller.hh) */
+int lcg_pair_band(const lcg_pair *pair, int32_t *min_col, int32_t *max_col, int32_t cap);
+/* arc matches in the reference's index order (ArcMatches::arcmatch(idx)),
+ * with their score (Scoring::arcmatch, scoring.cc:440-554) */
+int lcg_pair_arcmatches(const lcg_pair *pair, int32_t *al, int32_t *ar, int32_t *bl, int32_t *br, int64_t *score,
+                        int32_t cap);
+
+/* Device batch of prepared pairs (device >= 0: HIP device ordinal).  The
+ * pairs may be freed after the call. */
+int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t device, lcg_batch **out, char *err,
+                     int errlen);
+void lcg_batch_destroy(lcg_batch *batch);
+/* Runs the kernels for all pairs of the batch.
+ *   mode 0: batched, one persistent launch over a work queue (all pairs)
+ *   mode 1: plain, one launch per pair (same kernel), pairs in order
+ *   mode 2: batched, replay of a captured hipGraph of mode 0
+ * trace: bit 0: also run the traceback; bit 1: count the DP cells and arc
+ * candidates the fill evaluates (lcg_batch_counters; not with mode 2).
+ * device_ms (may be NULL): device time of the run from HIP events. */
+int lcg_batch_run(lcg_batch *batch, int32_t mode, int32_t trace, float *device_ms);
+/* launch configuration chosen for the batch: waves per workgroup,
+ * workgroups, LDS bytes per workgroup */
+int lcg_batch_config(const lcg_batch *batch, int32_t *waves, int32_t *workgroups, int32_t *lds_bytes);
+/* Score of pair k (Aligner::align(), aligner.cc:924-962). */
+int lcg_batch_score(const lcg_batch *batch, int32_t k, int64_t *score);
+/* Alignment of pair k (Aligner::trace() + get_alignment(),
+ * aligner.cc:1345-1363): edges a[t], b[t] (1-based positions, -1 = gap) in
+ * alignment order, and the two structure strings (dot bracket).  Returns the
+ * number of edges (>= 0) or an error. */
+int lcg_batch_alignment(const lcg_batch *batch, int32_t k, int32_t *a, int32_t *b, int32_t cap, char *structA,
+                        char *structB, int32_t scap);
+/* D table of pair k in the reference's arc-match order
+ * (AlignerImpl::D(am), aligner.cc:574-610).  Returns the count. */
+int lcg_batch_D(const lcg_batch *batch, int32_t k, int64_t *D, int32_t cap);
+
+/* Work counted by the last run with trace bit 1 set: DP cells and arc-match
+ * candidates evaluated by the fill of D and the top level of pair k. */
+int lcg_batch_counters(const lcg_batch *batch, int32_t k, int64_t *cells, int64_t *candidates);
+
+/* Synthetic inputs (no counterpart in the reference; they replace
+ * RNAfold -p, which the reference calls through ViennaRNA,
+ * rna_data.cc:52-61). */
+int lcg_synth_sequences(uint64_t seed, int count, int len_min, int len_max, double gc_min, double gc_max,
+                        char *out_seqs, int stride, int *out_lens);
+int lcg_synth_bpp(const char *seq, int len, int max_span, double cutoff, int *out_i, int *out_j, double *out_p,
+                  int cap);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,194 @@
+"""Thin Python binding of the C ABI (include/locarna_amd.h), used by the
+tests and bench.py.  All compute happens in liblocarna_amd.so; there is no
+Python or CPU fallback for the alignment kernels."""
+import ctypes
+
+from . import _lib
+
+
+class Params(ctypes.Structure):
+    """lcg_params (include/locarna_amd.h)."""
+    _fields_ = [
+        ("indel", ctypes.c_int32), ("indel_opening", ctypes.c_int32),
+        ("struct_weight", ctypes.c_int32), ("tau", ctypes.c_int32),
+        ("match", ctypes.c_int32), ("mismatch", ctypes.c_int32),
+        ("temperature_alipf", ctypes.c_int32), ("sequ_local", ctypes.c_int32),
+        ("max_bp_span", ctypes.c_int32), ("band_extended", ctypes.c_int32),
+        ("use_ribosum", ctypes.c_int32),
+        ("min_prob", ctypes.c_double), ("min_trace_prob", ctypes.c_double),
+        ("exp_prob", ctypes.c_double), ("pf_scale", ctypes.c_double),
+        ("min_am_prob", ctypes.c_double), ("min_bm_prob", ctypes.c_double),
+    ]
+
+
+def locarna_params(**kw):
+    """locarna flags of the integer workloads: --indel=-350
+    --indel-opening=-500 --struct-weight=200 --tau=0 --min-prob=0.0005
+    --maxBPspan=100; other options at their locarna defaults
+    (min-trace-probability 1e-4, trace band in long double)."""
+    d = dict(indel=-350, indel_opening=-500, struct_weight=200, tau=0, match=50,
+             mismatch=0, temperature_alipf=300, sequ_local=0, max_bp_span=100,
+             band_extended=1, use_ribosum=1, min_prob=0.0005, min_trace_prob=1e-4,
+             exp_prob=-1.0, pf_scale=1.0, min_am_prob=0.001, min_bm_prob=0.001)
+    d.update(kw)
+    return Params(**d)
+
+
+def locarna_p_params(**kw):
+    """locarna_p defaults of the probabilistic workloads: tau 50, trace band
+    in double, min-trace-probability 1e-5."""
+    d = dict(tau=50, band_extended=0, min_trace_prob=1e-5)
+    d.update(kw)
+    return locarna_params(**d)
+
+
+class LcgError(RuntimeError):
+    pass
+
+
+def _sig(L):
+    c = ctypes
+    L.lcg_pair_prepare.argtypes = [c.c_char_p, c.c_char_p, c.POINTER(Params), c.POINTER(c.c_void_p),
+                                   c.c_char_p, c.c_int]
+    L.lcg_pair_free.argtypes = [c.c_void_p]
+    L.lcg_pair_info.argtypes = [c.c_void_p, c.POINTER(c.c_int32), c.POINTER(c.c_int32),
+                                c.POINTER(c.c_int32), c.POINTER(c.c_int32), c.POINTER(c.c_int64)]
+    L.lcg_pair_band.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_int32]
+    L.lcg_pair_arcmatches.argtypes = [c.c_void_p] + [c.c_void_p] * 5 + [c.c_int32]
+    L.lcg_batch_create.argtypes = [c.POINTER(c.c_void_p), c.c_int32, c.c_int32, c.POINTER(c.c_void_p),
+                                   c.c_char_p, c.c_int]
+    L.lcg_batch_destroy.argtypes = [c.c_void_p]
+    L.lcg_batch_run.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.POINTER(c.c_float)]
+    L.lcg_batch_config.argtypes = [c.c_void_p, c.POINTER(c.c_int32), c.POINTER(c.c_int32),
+                                   c.POINTER(c.c_int32)]
+    L.lcg_batch_score.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_int64)]
+    L.lcg_batch_alignment.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_void_p, c.c_int32,
+                                      c.c_char_p, c.c_char_p, c.c_int32]
+    L.lcg_batch_D.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int32]
+    L.lcg_batch_counters.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_int64), c.POINTER(c.c_int64)]
+    return L
+
+
+_L = None
+
+
+def lib():
+    global _L
+    if _L is None:
+        _L = _sig(_lib.load())
+    return _L
+
+
+class Pair:
+    """A prepared pair (lcg_pair_prepare)."""
+
+    def __init__(self, ppA, ppB, params=None):
+        L = lib()
+        self.params = params or locarna_params()
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = L.lcg_pair_prepare(ppA.encode(), ppB.encode(), ctypes.byref(self.params), ctypes.byref(h), err, 512)
+        if rc != 0:
+            raise LcgError("lcg_pair_prepare (%d): %s" % (rc, err.value.decode()))
+        self.h = h
+        a, b, na, ng = (ctypes.c_int32() for _ in range(4))
+        cells = ctypes.c_int64()
+        L.lcg_pair_info(h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(na), ctypes.byref(ng),
+                        ctypes.byref(cells))
+        self.lenA, self.lenB, self.num_arcmatches, self.num_groups = a.value, b.value, na.value, ng.value
+        self.cells = cells.value
+
+    def band(self):
+        n = self.lenA + 1
+        mn, mx = (ctypes.c_int32 * n)(), (ctypes.c_int32 * n)()
+        lib().lcg_pair_band(self.h, mn, mx, n)
+        return list(mn), list(mx)
+
+    def arcmatches(self):
+        n = max(self.num_arcmatches, 1)
+        arrs = [(ctypes.c_int32 * n)() for _ in range(4)]
+        sc = (ctypes.c_int64 * n)()
+        lib().lcg_pair_arcmatches(self.h, *arrs, sc, n)
+        k = self.num_arcmatches
+        return [(arrs[0][t], arrs[1][t], arrs[2][t], arrs[3][t]) for t in range(k)], list(sc)[:k]
+
+    def free(self):
+        if self.h:
+            lib().lcg_pair_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Device batch (lcg_batch_create / lcg_batch_run)."""
+
+    def __init__(self, pairs, device=0):
+        L = lib()
+        self.pairs = pairs
+        arr = (ctypes.c_void_p * len(pairs))(*[p.h.value for p in pairs])
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = L.lcg_batch_create(arr, len(pairs), device, ctypes.byref(h), err, 512)
+        if rc != 0:
+            raise LcgError("lcg_batch_create (%d): %s" % (rc, err.value.decode()))
+        self.h = h
+        w, g, lds = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        L.lcg_batch_config(h, ctypes.byref(w), ctypes.byref(g), ctypes.byref(lds))
+        self.waves, self.workgroups, self.lds_bytes = w.value, g.value, lds.value
+
+    def run(self, mode=0, trace=True, count=False):
+        ms = ctypes.c_float()
+        flags = (1 if trace else 0) | (2 if count else 0)
+        rc = lib().lcg_batch_run(self.h, mode, flags, ctypes.byref(ms))
+        if rc != 0:
+            raise LcgError("lcg_batch_run failed (%d)" % rc)
+        return ms.value
+
+    def score(self, k):
+        s = ctypes.c_int64()
+        rc = lib().lcg_batch_score(self.h, k, ctypes.byref(s))
+        if rc != 0:
+            raise LcgError("lcg_batch_score (%d)" % rc)
+        return s.value
+
+    def alignment(self, k):
+        p = self.pairs[k]
+        cap = p.lenA + p.lenB + 2
+        a, b = (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)()
+        sa = ctypes.create_string_buffer(cap + 2)
+        sb = ctypes.create_string_buffer(cap + 2)
+        n = lib().lcg_batch_alignment(self.h, k, a, b, cap, sa, sb, cap + 2)
+        if n < 0:
+            raise LcgError("lcg_batch_alignment (%d)" % n)
+        return [(a[t], b[t]) for t in range(n)], sa.value.decode(), sb.value.decode()
+
+    def counters(self, k):
+        c, q = ctypes.c_int64(), ctypes.c_int64()
+        rc = lib().lcg_batch_counters(self.h, k, ctypes.byref(c), ctypes.byref(q))
+        if rc != 0:
+            raise LcgError("lcg_batch_counters (%d)" % rc)
+        return c.value, q.value
+
+    def D(self, k):
+        n = max(self.pairs[k].num_arcmatches, 1)
+        out = (ctypes.c_int64 * n)()
+        cnt = lib().lcg_batch_D(self.h, k, out, n)
+        if cnt < 0:
+            raise LcgError("lcg_batch_D (%d)" % cnt)
+        return list(out)[:cnt]
+
+    def destroy(self):
+        if self.h:
+            lib().lcg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass

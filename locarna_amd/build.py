@@ -17,8 +17,15 @@ INCLUDE = os.path.join(ROOT, "include")
 
 HOST_SOURCES = [
     "synth.cpp",
+    "host.cpp",
 ]
-HIP_SOURCES = []
+# host code that calls the HIP runtime API (no kernels): compiled by g++
+HIPAPI_SOURCES = [
+    "batch.cpp",
+]
+HIP_SOURCES = [
+    "align_kernels.hip",
+]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 CXX = os.environ.get("CXX", "g++")
@@ -43,6 +50,11 @@ def build(verbose=False):
     for src in HOST_SOURCES:
         obj = os.path.join(OUT, os.path.splitext(src)[0] + ".o")
         _run([CXX] + HOST_FLAGS + ["-c", "-o", obj, os.path.join(CSRC, src)])
+        objs.append(obj)
+    for src in HIPAPI_SOURCES:
+        obj = os.path.join(OUT, os.path.splitext(src)[0] + ".o")
+        _run([CXX] + HOST_FLAGS + ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", "-o", obj,
+                                   os.path.join(CSRC, src)])
         objs.append(obj)
     for src in HIP_SOURCES:
         obj = os.path.join(OUT, os.path.splitext(src)[0] + ".hip.o")

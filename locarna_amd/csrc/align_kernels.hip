@@ -1,0 +1,522 @@
+// gfx950 kernels of the integer LocARNA alignment (max-plus DP, int32).
+//
+// One workgroup of NW wave64s aligns one pair at a time and pulls pairs
+// from a work queue (persistent grid).  Per pair:
+//   1. fill D: rows of left ends al = n .. 1; the arc-match groups (al, bl)
+//      of one row are independent and are spread over the waves; each wave
+//      computes its group's matrix M row by row with lanes over columns
+//      (reference: AlignerImpl::align_D / align_in_arcmatch / align_noex /
+//      fill_D_entries, src/LocARNA/aligner.cc:153-233, 264-437, 574-732);
+//   2. the top level (global or sequence-local, aligner.cc:736-880);
+//   3. the traceback (aligner.cc:967-1358), on the device: the top level is
+//      traced by one wave, every traced arc match is re-aligned and traced
+//      by any free wave (breadth first).  Each emitted edge is written to the
+//      slot i+j of the cell it reaches, which orders the edges without a sort.
+//
+// Row recurrence of one group (origin (al, bl)), lane l owns column j:
+//   s(i,j)  = max(M(i-1,j-1) + sigma(i,j), E(i,j), max_arcs M(p-1,q-1) + D(a,b))
+//   E(i,j)  = max(E(i-1,j) + gA(i), M(i-1,j) + gA(i) + alpha)
+//   F(i,j)  = max(F(i,j-1) + gB(j) + max(0,alpha), s(i,j-1) + gB(j) + alpha)
+//   M(i,j)  = max(s(i,j), F(i,j))
+// The F chain is a max-plus linear recurrence along the row; it is solved by
+// a prefix sum and a prefix max across the wave (DPP row_shr / row_bcast):
+//   F(j) = P(j) + max_{k<=j} (c(k) - P(k)),  P = inclusive sum of a.
+// Scores are int32 with minus infinity LCG_NEG = -2^29; values below -2^28
+// are normalized to LCG_NEG, which leaves every finite result unchanged.
+#include <hip/hip_runtime.h>
+
+#include "device_layout.h"
+
+namespace {
+
+__device__ __forceinline__ int clampneg(int x) { return x < LCG_NEG_LIMIT ? LCG_NEG : x; }
+
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ int dpp(int old, int v) {
+    return __builtin_amdgcn_update_dpp(old, v, CTRL, ROWMASK, 0xf, false);
+}
+
+// inclusive prefix sum over the 64 lanes
+__device__ __forceinline__ int wave_scan_add(int v) {
+    v += dpp<0x111, 0xf>(0, v);  // row_shr:1
+    v += dpp<0x112, 0xf>(0, v);  // row_shr:2
+    v += dpp<0x114, 0xf>(0, v);  // row_shr:4
+    v += dpp<0x118, 0xf>(0, v);  // row_shr:8
+    v += dpp<0x142, 0xa>(0, v);  // row_bcast:15 -> rows 1, 3
+    v += dpp<0x143, 0xc>(0, v);  // row_bcast:31 -> rows 2, 3
+    return v;
+}
+
+// inclusive prefix max over the 64 lanes
+__device__ __forceinline__ int wave_scan_max(int v) {
+    const int id = -0x7fffffff;
+    v = max(v, dpp<0x111, 0xf>(id, v));
+    v = max(v, dpp<0x112, 0xf>(id, v));
+    v = max(v, dpp<0x114, 0xf>(id, v));
+    v = max(v, dpp<0x118, 0xf>(id, v));
+    v = max(v, dpp<0x142, 0xa>(id, v));
+    v = max(v, dpp<0x143, 0xc>(id, v));
+    return v;
+}
+
+// lane l receives lane l-1's value; lane 0 receives `first`
+__device__ __forceinline__ int wave_shift_up(int v, int first) {
+    return __builtin_amdgcn_update_dpp(first, v, 0x138, 0xf, 0xf, false);  // wave_shr:1
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
+__device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+struct Pair {
+    const int *__restrict__ mn;    // LDS: min_col[0..n]
+    const int *__restrict__ mx;    // LDS: max_col[0..n]
+    const int *__restrict__ coff;  // LDS: cell_off[0..n+1]
+    const int *__restrict__ sig;   // LDS: 8x8 sigma
+    const int *__restrict__ codeA;
+    const int *__restrict__ codeB;
+    const int *__restrict__ gapA;
+    const int *__restrict__ gapB;
+    const int *__restrict__ cptr;
+    const int *__restrict__ cand;
+    const int *__restrict__ amscore;
+    int *D;
+    int *edges;
+    int *structs;
+    int n, m;
+    int alpha, alpha_pos;
+};
+
+// Row i's stored column range in the group with origin (al, bl) and
+// column bound br (exclusive): [lo, hi].
+__device__ __forceinline__ int row_lo(const Pair &P, int i, int bl) { return max(bl, P.mn[i]); }
+__device__ __forceinline__ int row_hi(const Pair &P, int i, int br) { return min(br - 1, P.mx[i]); }
+
+struct TopBest {
+    int score, i, j;
+};
+
+// Fills the matrix of the group with origin (al, bl), rows al..ar-1, columns
+// bl..br-1 (within the band) into Mst (row r = i - al at Mst + r*W, column j
+// at j - row_lo(i)).  Est holds E over columns [bl, br).
+// GLOBAL: Mst/Est are in global memory (fence per row); TOP_LOCAL: the
+// sequence-local top level (values clamped at 0, first maximum tracked).
+template <bool GLOBAL, bool TOP_LOCAL>
+__device__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W, int *Mst, int *Est, bool global_init,
+                           TopBest *best, unsigned long long *counts = nullptr) {
+    unsigned cnt_cells = 0, cnt_cand = 0;
+    const int lane = lane_id();
+    const int alpha = P.alpha;
+    for (int c = lane; c < br - bl; c += 64) Est[c] = LCG_NEG;
+    // row al: M(al, bl) = 0, M(al, j) = alpha + sum gapB (global) or 0
+    const int hi0 = row_hi(P, al, br);
+    {
+        int carry = global_init ? alpha : 0;
+        for (int j0 = bl + 1; j0 <= hi0; j0 += 64) {
+            const int j = j0 + lane;
+            const bool act = j <= hi0;
+            const int g = (act && global_init) ? P.gapB[j] : 0;
+            const int ps = wave_scan_add(g);
+            if (act) Mst[j - bl] = carry + ps;
+            carry += readlane(ps, 63);
+        }
+        if (lane == 0) Mst[0] = 0;
+    }
+    if (GLOBAL) __threadfence_block();
+    int colv = global_init ? alpha : 0;
+    bool colopen = true;
+    int lo_p = bl, hi_p = hi0;
+    for (int i = al + 1; i < ar; ++i) {
+        const int r = i - al;
+        int *row = Mst + r * W;
+        const int *prow = row - W;
+        const int mni = P.mn[i];
+        if (colopen && mni <= bl) {
+            if (global_init) colv += P.gapA[i];
+        } else {
+            colopen = false;
+        }
+        const int lo = max(bl, mni), hi = min(br - 1, P.mx[i]);
+        if (colopen && lane == 0) row[0] = colv;
+        const int jlo = max(bl + 1, mni);
+        int Fcarry = LCG_NEG;
+        int sprev = colopen ? colv : LCG_NEG;  // M(i, jlo - 1)
+        const int gA = P.gapA[i];
+        const int *sigrow = P.sig + P.codeA[i] * LCG_NCODES;
+        const int cbase = P.coff[i] - mni;
+        for (int j0 = jlo; j0 <= hi; j0 += 64) {
+            const int j = j0 + lane;
+            const bool act = j <= hi;
+            int s = LCG_NEG, gB = 0;
+            if (act) {
+                const int up = (j >= lo_p && j <= hi_p) ? prow[j - lo_p] : LCG_NEG;
+                const int dg = (j - 1 >= lo_p && j - 1 <= hi_p) ? prow[j - 1 - lo_p] : LCG_NEG;
+                const int e = clampneg(max(Est[j - bl] + gA, up + gA + alpha));
+                Est[j - bl] = e;
+                s = max(dg + sigrow[P.codeB[j]], e);
+                const int cell = cbase + j;
+                const int t1 = P.cptr[cell + 1];
+                ++cnt_cells;
+                for (int t = P.cptr[cell]; t < t1; ++t) {
+                    const int pq = P.cand[2 * t];
+                    const int p = pq & 0xffff;
+                    if (p <= al) break;
+                    const int q = pq >> 16;
+                    if (q <= bl) continue;
+                    ++cnt_cand;
+                    const int v = Mst[(p - 1 - al) * W + (q - 1 - row_lo(P, p - 1, bl))] + P.D[P.cand[2 * t + 1]];
+                    s = max(s, v);
+                }
+                s = clampneg(s);
+                if (TOP_LOCAL) s = max(s, 0);
+                gB = P.gapB[j];
+            }
+            // F chain
+            const int sl = wave_shift_up(s, sprev);
+            const int a = act ? gB + P.alpha_pos : 0;
+            const int c = act ? sl + gB + alpha : LCG_NEG;
+            const int Pa = wave_scan_add(a);
+            const int px = wave_scan_max(act ? c - Pa : LCG_NEG);
+            const int F = Pa + max(Fcarry, px);
+            const int Mv = clampneg(max(s, F));
+            if (act) row[j - lo] = Mv;
+            if (TOP_LOCAL) {
+                // first strict maximum in row-major order (aligner.cc:870-875)
+                const int cand_v = act ? Mv : -1;
+                int wmax = cand_v;
+                for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
+                if (wmax > best->score) {
+                    const unsigned long long bal = __ballot(act && Mv == wmax);
+                    best->score = wmax;
+                    best->i = i;
+                    best->j = j0 + (int)__ffsll((long long)bal) - 1;
+                }
+            }
+            Fcarry = clampneg(readlane(F, 63));
+            sprev = readlane(s, 63);
+        }
+        lo_p = lo;
+        hi_p = hi;
+        if (GLOBAL) __threadfence_block();
+    }
+    if (counts) {
+        atomicAdd(&counts[0], (unsigned long long)cnt_cells);
+        atomicAdd(&counts[1], (unsigned long long)cnt_cand);
+    }
+}
+
+// stored value of cell (i, j) of the group (origin al, bl; bound br)
+__device__ __forceinline__ int Mget(const Pair &P, const int *Mst, int W, int al, int bl, int br, int i, int j) {
+    const int lo = (i == al) ? bl : row_lo(P, i, bl);
+    const int hi = row_hi(P, i, br);
+    if (j < lo || j > hi) return LCG_NEG;
+    return Mst[(i - al) * W + (j - lo)];
+}
+
+__device__ __forceinline__ bool valid(const Pair &P, int i, int j) { return P.mn[i] <= j && j <= P.mx[i]; }
+
+__device__ __forceinline__ void emit(const Pair &P, int a, int b, int slot) {
+    P.edges[slot] = (a << 16) | b;
+}
+
+// Traces one group back from (i, j) to its origin (al, bl) (aligner.cc:
+// trace_in_arcmatch / trace_noex, state E_NO_NO).  Arc matches taken are
+// appended to the work list (tstack, counter *tcount in LDS).
+template <bool GLOBAL>
+__device__ void group_trace(const Pair &P, const int *Mst, int W, int al, int bl, int br, int i, int j, bool top,
+                            bool local, int *tstack, int *tcount) {
+    const int lane = lane_id();
+    if (GLOBAL) __threadfence_block();
+    for (;;) {
+        const int Mij = Mget(P, Mst, W, al, bl, br, i, j);
+        if (top && local && Mij == 0) break;
+        if (i <= al) {
+            if (!(top && local))
+                for (int k = bl + 1 + lane; k <= j; k += 64) emit(P, 0, k, al + k);
+            break;
+        }
+        if (j <= bl) {
+            if (!(top && local))
+                for (int k = al + 1 + lane; k <= i; k += 64) emit(P, k, 0, k + bl);
+            break;
+        }
+        // match
+        if (valid(P, i - 1, j - 1) &&
+            Mij == Mget(P, Mst, W, al, bl, br, i - 1, j - 1) + P.sig[P.codeA[i] * LCG_NCODES + P.codeB[j]]) {
+            if (lane == 0) emit(P, i, j, i + j);
+            --i;
+            --j;
+            continue;
+        }
+        bool done = false;
+        if (P.alpha != 0) {
+            // deletion of length k = 1, 2, ...: lanes test k = base + lane
+            int carry = P.alpha;
+            for (int base = 1; !done && i - base >= al; base += 64) {
+                const int k = base + lane;
+                const bool inr = (i - k >= al) && valid(P, i - k, j);
+                const int g = inr ? P.gapA[i - k + 1] : 0;
+                const int cost = carry + wave_scan_add(g);
+                const bool hit = inr && Mij == Mget(P, Mst, W, al, bl, br, i - k, j) + cost;
+                const unsigned long long bh = __ballot(hit);
+                const unsigned long long bv = __ballot(inr);
+                // the reference stops at the first invalid k
+                const unsigned long long valid_prefix = ~bv ? ((~bv) & (-(~bv))) - 1 : ~0ull;
+                const unsigned long long h = bh & valid_prefix;
+                if (h) {
+                    const int kk = base + (int)__ffsll((long long)h) - 1;
+                    for (int t = i - kk + 1 + lane; t <= i; t += 64) emit(P, t, 0, t + j);
+                    i -= kk;
+                    done = true;
+                } else if (valid_prefix != ~0ull) {
+                    break;
+                }
+                carry = readlane(cost, 63);
+            }
+            if (done) continue;
+            carry = P.alpha;
+            for (int base = 1; !done && j - base >= bl; base += 64) {
+                const int k = base + lane;
+                const bool inr = (j - k >= bl) && valid(P, i, j - k);
+                const int g = inr ? P.gapB[j - k + 1] : 0;
+                const int cost = carry + wave_scan_add(g);
+                const bool hit = inr && Mij == Mget(P, Mst, W, al, bl, br, i, j - k) + cost;
+                const unsigned long long bh = __ballot(hit);
+                const unsigned long long bv = __ballot(inr);
+                const unsigned long long valid_prefix = ~bv ? ((~bv) & (-(~bv))) - 1 : ~0ull;
+                const unsigned long long h = bh & valid_prefix;
+                if (h) {
+                    const int kk = base + (int)__ffsll((long long)h) - 1;
+                    for (int t = j - kk + 1 + lane; t <= j; t += 64) emit(P, 0, t, i + t);
+                    j -= kk;
+                    done = true;
+                } else if (valid_prefix != ~0ull) {
+                    break;
+                }
+                carry = readlane(cost, 63);
+            }
+            if (done) continue;
+        } else {
+            if (valid(P, i - 1, j) && Mij == Mget(P, Mst, W, al, bl, br, i - 1, j) + P.gapA[i]) {
+                if (lane == 0) emit(P, i, 0, i + j);
+                --i;
+                continue;
+            }
+            if (valid(P, i, j - 1) && Mij == Mget(P, Mst, W, al, bl, br, i, j - 1) + P.gapB[j]) {
+                if (lane == 0) emit(P, 0, j, i + j);
+                --j;
+                continue;
+            }
+        }
+        if (!valid(P, i - 1, j - 1)) break;
+        // arc matches with right ends (i, j) in list order (al desc, bl desc)
+        const int cell = P.coff[i] + (j - P.mn[i]);
+        const int t0 = P.cptr[cell], t1 = P.cptr[cell + 1];
+        int found = -1, fp = 0, fq = 0;
+        for (int base = t0; base < t1 && found < 0; base += 64) {
+            const int t = base + lane;
+            bool hit = false;
+            int p = 0, q = 0, am = 0;
+            if (t < t1) {
+                const int pq = P.cand[2 * t];
+                p = pq & 0xffff;
+                q = pq >> 16;
+                am = P.cand[2 * t + 1];
+                hit = p > al && q > bl && Mij == Mget(P, Mst, W, al, bl, br, p - 1, q - 1) + P.D[am];
+            }
+            const unsigned long long bh = __ballot(hit);
+            if (bh) {
+                const int src = (int)__ffsll((long long)bh) - 1;
+                found = readlane(am, src);
+                fp = readlane(p, src);
+                fq = readlane(q, src);
+            }
+        }
+        if (found < 0) break;
+        if (lane == 0) {
+            emit(P, fp, fq, fp + fq);
+            emit(P, i, j, i + j);
+            P.structs[fp - 1] = 1;
+            P.structs[i - 1] = 2;
+            P.structs[P.n + fq - 1] = 1;
+            P.structs[P.n + j - 1] = 2;
+            const int slot = atomicAdd(tcount, 1);
+            tstack[2 * slot] = fp | (fq << 16);
+            tstack[2 * slot + 1] = i | (j << 16);
+        }
+        i = fp - 1;
+        j = fq - 1;
+    }
+}
+
+// stored width of the rows al..ar-1 of a region (origin al, bl; bound br)
+__device__ int region_width(const Pair &P, int al, int bl, int ar, int br) {
+    int w = 0;
+    for (int i = al; i < ar; ++i) {
+        const int lo = (i == al) ? bl : row_lo(P, i, bl);
+        w = max(w, row_hi(P, i, br) - lo + 1);
+    }
+    return w;
+}
+
+}  // namespace
+
+extern "C" __global__ void __launch_bounds__(512) lcg_align_pairs(const PairDesc *__restrict__ descs,
+                                                                  const int *__restrict__ order, int *queue,
+                                                                  int *arena, BatchParams bp) {
+    extern __shared__ int lds[];
+    __shared__ int s_pair, s_tcount;
+    const int nwaves = blockDim.x >> 6;
+    const int wave = threadIdx.x >> 6;
+    const int lane = lane_id();
+    int *hdr = lds;
+    int *wave_area = lds + bp.lds_hdr_ints;
+    for (;;) {
+        if (threadIdx.x == 0) s_pair = atomicAdd(queue, 1);
+        __syncthreads();
+        const int k = s_pair;
+        if (k >= bp.npairs) break;
+        const PairDesc d = descs[order[k]];
+        Pair P;
+        const int n = d.n, m = d.m;
+        int *mn = hdr, *mx = hdr + (bp.max_len + 1), *coff = hdr + 2 * (bp.max_len + 1);
+        int *sig = hdr + 3 * (bp.max_len + 2);
+        for (int t = threadIdx.x; t <= n; t += blockDim.x) {
+            mn[t] = arena[d.band + t];
+            mx[t] = arena[d.band + n + 1 + t];
+        }
+        for (int t = threadIdx.x; t <= n + 1; t += blockDim.x) coff[t] = arena[d.band + 2 * (n + 1) + t];
+        for (int t = threadIdx.x; t < LCG_NCODES * LCG_NCODES; t += blockDim.x) sig[t] = arena[d.sigma + t];
+        if (bp.trace) {
+            for (int t = threadIdx.x; t <= n + m; t += blockDim.x) arena[d.edges + t] = -1;
+            for (int t = threadIdx.x; t < n + m; t += blockDim.x) arena[d.structs + t] = 0;
+        }
+        if (threadIdx.x == 0) s_tcount = 0;
+        P.mn = mn;
+        P.mx = mx;
+        P.coff = coff;
+        P.sig = sig;
+        P.codeA = arena + d.codes;
+        P.codeB = arena + d.codes + n + 1;
+        P.gapA = arena + d.gaps;
+        P.gapB = arena + d.gaps + n + 1;
+        P.cptr = arena + d.cptr;
+        P.cand = arena + d.cand;
+        P.amscore = arena + d.amscore;
+        P.D = arena + d.D;
+        P.edges = arena + d.edges;
+        P.structs = arena + d.structs;
+        P.n = n;
+        P.m = m;
+        P.alpha = bp.alpha;
+        P.alpha_pos = bp.alpha > 0 ? bp.alpha : 0;
+        __syncthreads();
+
+        unsigned long long *counts = bp.count ? (unsigned long long *)(arena + d.out + 4) : nullptr;
+        if (bp.count && threadIdx.x < 4) arena[d.out + 4 + threadIdx.x] = 0;
+        __syncthreads();
+        int *Est = wave_area + wave * bp.lds_wave_ints;
+        int *Mw = Est + bp.ecap;
+        const int *rows = arena + d.rows;
+        const int *groups = arena + d.groups;
+        const int *gams = arena + d.gams;
+        // ---- 1. fill D, rows al = n .. 1
+        for (int al = n; al >= 1; --al) {
+            const int g0 = rows[al + 1], g1 = rows[al];
+            if (g0 == g1) continue;
+            for (int g = g0 + wave; g < g1; g += nwaves) {
+                const int *G = groups + g * LCG_GROUP_INTS;
+                const int gal = G[0] & 0xffff, gbl = G[0] >> 16;
+                const int gar = G[1] & 0xffff, gbr = G[1] >> 16;
+                const int W = G[2];
+                group_fill<false, false>(P, gal, gbl, gar, gbr, W, Mw, Est, true, nullptr, counts);
+                for (int t = G[3] + lane; t < G[4]; t += 64) {
+                    const int rb = gams[2 * t], am = gams[2 * t + 1];
+                    const int ar = rb & 0xffff, br = rb >> 16;
+                    const int v = Mw[(ar - 1 - gal) * W + (br - 1 - row_lo(P, ar - 1, gbl))] + P.amscore[am];
+                    P.D[am] = v;  // not normalized, like the reference's InftyInt sum
+                }
+            }
+            __threadfence_block();
+            __syncthreads();
+        }
+        // ---- 2. top level (wave 0)
+        const int top_ints = (m + 1) + (n + 1) * d.wtop;
+        const bool top_in_lds = top_ints <= nwaves * bp.lds_wave_ints;
+        int *topE = top_in_lds ? wave_area : arena + d.topM;
+        int *topM = topE + (m + 1);
+        if (wave == 0) {
+            TopBest best{0, 0, 0};
+            if (bp.local) {
+                if (top_in_lds)
+                    group_fill<false, true>(P, 0, 0, n + 1, m + 1, d.wtop, topM, topE, false, &best, counts);
+                else
+                    group_fill<true, true>(P, 0, 0, n + 1, m + 1, d.wtop, topM, topE, false, &best, counts);
+            } else {
+                if (top_in_lds)
+                    group_fill<false, false>(P, 0, 0, n + 1, m + 1, d.wtop, topM, topE, true, nullptr, counts);
+                else
+                    group_fill<true, false>(P, 0, 0, n + 1, m + 1, d.wtop, topM, topE, true, nullptr, counts);
+                best.score = Mget(P, topM, d.wtop, 0, 0, m + 1, n, m);
+                best.i = n;
+                best.j = m;
+            }
+            if (lane == 0) {
+                arena[d.out + 0] = best.score;
+                arena[d.out + 1] = best.i;
+                arena[d.out + 2] = best.j;
+            }
+            // ---- 3a. trace the top level
+            if (bp.trace) {
+                if (top_in_lds)
+                    group_trace<false>(P, topM, d.wtop, 0, 0, m + 1, best.i, best.j, true, bp.local != 0,
+                                       arena + d.tstack, &s_tcount);
+                else
+                    group_trace<true>(P, topM, d.wtop, 0, 0, m + 1, best.i, best.j, true, bp.local != 0,
+                                      arena + d.tstack, &s_tcount);
+            }
+        }
+        __threadfence_block();
+        __syncthreads();
+        // ---- 3b. trace inside the traced arc matches, breadth first
+        if (bp.trace) {
+            int lo = 0, hi = s_tcount;
+            __syncthreads();
+            while (lo < hi) {
+                for (int e = lo + wave; e < hi; e += nwaves) {
+                    const int *ts = arena + d.tstack + 2 * e;
+                    const int al = ts[0] & 0xffff, bl = ts[0] >> 16;
+                    const int ar = ts[1] & 0xffff, br = ts[1] >> 16;
+                    const int W = region_width(P, al, bl, ar, br);
+                    group_fill<false, false>(P, al, bl, ar, br, W, Mw, Est, true, nullptr);
+                    group_trace<false>(P, Mw, W, al, bl, br, ar - 1, br - 1, false, false, arena + d.tstack,
+                                       &s_tcount);
+                }
+                __threadfence_block();
+                __syncthreads();
+                lo = hi;
+                hi = s_tcount;
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+}
+
+extern "C" __global__ void lcg_reset_queue(int *queue) {
+    if (threadIdx.x == 0) *queue = 0;
+}
+
+extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, BatchParams bp,
+                                int grid, int waves, size_t lds_bytes, hipStream_t stream) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)lcg_align_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024 - 256);
+        if (e != hipSuccess) return (int)e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL(lcg_reset_queue, dim3(1), dim3(64), 0, stream, queue);
+    hipLaunchKernelGGL(lcg_align_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue, arena,
+                       bp);
+    return (int)hipGetLastError();
+}

@@ -1,0 +1,558 @@
+// Packing of prepared pairs into the device arena, launch configuration and
+// the C ABI (include/locarna_amd.h).
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "device_layout.h"
+#include "host.hpp"
+#include "locarna_amd.h"
+
+using namespace locarna_amd;
+
+extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, BatchParams bp,
+                                int grid, int waves, size_t lds_bytes, hipStream_t stream);
+
+struct lcg_pair {
+    lcg_params params;
+    std::unique_ptr<RnaData> A, B;
+    std::unique_ptr<BasePairs> bA, bB;
+    std::unique_ptr<TraceController> tc;
+    std::unique_ptr<ArcMatches> am;
+    std::unique_ptr<Scoring> sc;
+    // packed device image, offsets relative to the blob start
+    std::vector<int32_t> blob;
+    PairDesc desc;
+    int64_t cells = 0;  // DP cells (groups + top level) as a work estimate
+    int ecap = 0;       // max columns of a group region (E vector)
+};
+
+namespace {
+
+void set_err(char *err, int errlen, const std::string &msg) {
+    if (err && errlen > 0) {
+        std::strncpy(err, msg.c_str(), errlen - 1);
+        err[errlen - 1] = 0;
+    }
+}
+
+// Appends an array of `count` ints to the blob; returns its offset.
+int32_t reserve(std::vector<int32_t> &blob, size_t count, int32_t fill = 0) {
+    size_t off = blob.size();
+    blob.resize(off + count, fill);
+    return (int32_t)off;
+}
+
+void pack(lcg_pair &P) {
+    const TraceController &tc = *P.tc;
+    const Scoring &sc = *P.sc;
+    const BasePairs &bA = *P.bA, &bB = *P.bB;
+    const ArcMatches &ams = *P.am;
+    const int n = P.A->length(), m = P.B->length();
+    const int nam = ams.num_arc_matches();
+    auto &blob = P.blob;
+    PairDesc d;
+    std::memset(&d, 0, sizeof d);
+    d.n = n;
+    d.m = m;
+    d.nam = nam;
+    // band + band cell offsets
+    d.band = reserve(blob, 3 * (size_t)(n + 1) + 1);
+    int ncells = 0;
+    for (int i = 0; i <= n; ++i) {
+        blob[d.band + i] = tc.min_col(i);
+        blob[d.band + n + 1 + i] = tc.max_col(i);
+        blob[d.band + 2 * (n + 1) + i] = ncells;
+        // rows of a band can be empty (min_col > max_col), e.g. at the ends
+        // of a sequence-local band (trace_controller.cc:565-597)
+        ncells += std::max(0, tc.max_col(i) - tc.min_col(i) + 1);
+    }
+    blob[d.band + 3 * (n + 1)] = ncells;
+    d.ncells = ncells;
+    d.codes = reserve(blob, (size_t)n + m + 2);
+    for (int i = 1; i <= n; ++i) blob[d.codes + i] = sc.codeA(i);
+    for (int j = 1; j <= m; ++j) blob[d.codes + n + 1 + j] = sc.codeB(j);
+    d.gaps = reserve(blob, (size_t)n + m + 2);
+    for (int i = 1; i <= n; ++i) blob[d.gaps + i] = (int32_t)sc.gapA(i);
+    for (int j = 1; j <= m; ++j) blob[d.gaps + n + 1 + j] = (int32_t)sc.gapB(j);
+    d.sigma = reserve(blob, LCG_NCODES * LCG_NCODES);
+    for (int a = 0; a < LCG_NCODES; ++a)
+        for (int b = 0; b < LCG_NCODES; ++b) blob[d.sigma + a * LCG_NCODES + b] = (int32_t)sc.sigma_code(a, b);
+
+    // arc matches by left ends (groups) and by right ends (candidates)
+    std::vector<std::vector<int>> by_left((size_t)(n + 1) * (m + 1));
+    std::vector<std::vector<int>> by_right((size_t)(n + 1) * (m + 1));
+    for (int k = 0; k < nam; ++k) {
+        const ArcMatch &x = ams.arcmatch(k);
+        const Arc &a = bA.arc(x.arcA), &b = bB.arc(x.arcB);
+        by_left[(size_t)a.left * (m + 1) + b.left].push_back(k);
+        by_right[(size_t)a.right * (m + 1) + b.right].push_back(k);
+    }
+    // groups, al descending, bl descending (aligner.cc:659-732)
+    struct G {
+        int al, bl, ar, br, W, b, e;
+    };
+    std::vector<G> groups;
+    std::vector<int32_t> gams;
+    std::vector<int32_t> rows(n + 2, 0);
+    int64_t cells = 0;
+    int ecap = 0, gbuf = 0;
+    auto row_width = [&](int i, int al, int bl, int br) {
+        int lo = (i == al) ? bl : std::max(bl, tc.min_col(i));
+        int hi = std::min(br - 1, tc.max_col(i));
+        return std::max(0, hi - lo + 1);
+    };
+    rows[n + 1] = 0;
+    for (int al = n; al >= 1; --al) {
+        const int max_bl = std::min(m, tc.max_col(al)), min_bl = std::max(1, tc.min_col(al));
+        for (int bl = max_bl; bl >= min_bl; --bl) {
+            if (!tc.is_valid_match(al, bl)) continue;
+            const auto &lst = by_left[(size_t)al * (m + 1) + bl];
+            if (lst.empty()) continue;
+            int ar = al, br = bl;
+            for (int k : lst) {
+                const ArcMatch &x = ams.arcmatch(k);
+                ar = std::max(ar, bA.arc(x.arcA).right);
+                br = std::max(br, bB.arc(x.arcB).right);
+            }
+            int W = 0;
+            for (int i = al; i < ar; ++i) {
+                int w = row_width(i, al, bl, br);
+                W = std::max(W, w);
+                if (i > al) cells += std::max(0, std::min(br - 1, tc.max_col(i)) - std::max(bl + 1, tc.min_col(i)) + 1);
+            }
+            G g{al, bl, ar, br, W, (int)gams.size() / 2, 0};
+            for (int k : lst) {
+                const ArcMatch &x = ams.arcmatch(k);
+                gams.push_back(bA.arc(x.arcA).right | (bB.arc(x.arcB).right << 16));
+                gams.push_back(k);
+            }
+            g.e = (int)gams.size() / 2;
+            groups.push_back(g);
+            ecap = std::max(ecap, br - bl);
+            gbuf = std::max(gbuf, (ar - al) * W);
+        }
+        rows[al] = (int)groups.size();
+    }
+    rows[0] = (int)groups.size();
+    d.ngroups = (int)groups.size();
+    d.rows = reserve(blob, rows.size());
+    std::copy(rows.begin(), rows.end(), blob.begin() + d.rows);
+    d.groups = reserve(blob, (size_t)LCG_GROUP_INTS * groups.size());
+    for (size_t g = 0; g < groups.size(); ++g) {
+        int32_t *G8 = &blob[d.groups + LCG_GROUP_INTS * g];
+        G8[0] = groups[g].al | (groups[g].bl << 16);
+        G8[1] = groups[g].ar | (groups[g].br << 16);
+        G8[2] = groups[g].W;
+        G8[3] = groups[g].b;
+        G8[4] = groups[g].e;
+    }
+    d.gams = reserve(blob, gams.size());
+    std::copy(gams.begin(), gams.end(), blob.begin() + d.gams);
+    d.amscore = reserve(blob, nam);
+    for (int k = 0; k < nam; ++k) blob[d.amscore + k] = (int32_t)sc.arcmatch(k);
+    // candidate CSR over band cells, each list sorted (al desc, bl desc)
+    d.cptr = reserve(blob, (size_t)ncells + 1);
+    d.cand = reserve(blob, 2 * (size_t)nam);
+    int t = 0, next_cell = 0;
+    for (int i = 0; i <= n; ++i) {
+        for (int j = tc.min_col(i); j <= tc.max_col(i); ++j) {
+            const int cell = blob[d.band + 2 * (n + 1) + i] + (j - tc.min_col(i));
+            if (cell != next_cell++) throw failure("internal: band cell numbering");
+            blob[d.cptr + cell] = t;
+            if (i == 0 || j == 0) continue;
+            auto lst = by_right[(size_t)i * (m + 1) + j];
+            std::sort(lst.begin(), lst.end(), [&](int x, int y) {
+                const Arc &ax = bA.arc(ams.arcmatch(x).arcA), &ay = bA.arc(ams.arcmatch(y).arcA);
+                const Arc &bx = bB.arc(ams.arcmatch(x).arcB), &by = bB.arc(ams.arcmatch(y).arcB);
+                return ax.left > ay.left || (ax.left == ay.left && bx.left > by.left);
+            });
+            for (int k : lst) {
+                const ArcMatch &x = ams.arcmatch(k);
+                blob[d.cand + 2 * t] = bA.arc(x.arcA).left | (bB.arc(x.arcB).left << 16);
+                blob[d.cand + 2 * t + 1] = k;
+                ++t;
+            }
+        }
+    }
+    if (next_cell != ncells) throw failure("internal: band cell count");
+    blob[d.cptr + ncells] = t;
+    if (t != nam) throw failure("internal: arc match right-end lists incomplete");
+    d.D = reserve(blob, nam);
+    d.edges = reserve(blob, (size_t)n + m + 1, -1);
+    d.structs = reserve(blob, (size_t)n + m);
+    d.tstack = reserve(blob, 2 * (size_t)(n + 1));
+    int wtop = 0;
+    for (int i = 0; i <= n; ++i) {
+        wtop = std::max(wtop, row_width(i, 0, 0, m + 1));
+        if (i > 0) cells += std::max(0, std::min(m, tc.max_col(i)) - std::max(1, tc.min_col(i)) + 1);
+    }
+    d.wtop = wtop;
+    d.topM = reserve(blob, (size_t)(m + 1) + (size_t)(n + 1) * wtop);
+    // out[4..7] holds two uint64 counters: keep them 8-byte aligned (every
+    // blob is a multiple of 4 ints, so blob offsets survive concatenation)
+    if (blob.size() & 1) blob.push_back(0);
+    d.out = reserve(blob, 8);
+    while (blob.size() & 3) blob.push_back(0);
+    d.gbuf = gbuf;
+    P.desc = d;
+    P.cells = cells;
+    P.ecap = ecap;
+}
+
+}  // namespace
+
+extern "C" int lcg_pair_prepare(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair **out, char *err,
+                                int errlen) {
+    if (!ppA || !ppB || !p || !out) return LCG_E_ARG;
+    *out = nullptr;
+    try {
+        auto P = std::make_unique<lcg_pair>();
+        P->params = *p;
+        P->A.reset(new RnaData(ppA, p->min_prob, p->max_bp_span));
+        P->B.reset(new RnaData(ppB, p->min_prob, p->max_bp_span));
+        if (P->A->length() >= 0xffff || P->B->length() >= 0xffff) throw failure("sequence too long");
+        ScoringParams sp;
+        sp.indel = p->indel;
+        sp.indel_opening = p->indel_opening;
+        sp.struct_weight = p->struct_weight;
+        sp.tau = p->tau;
+        sp.match = p->match;
+        sp.mismatch = p->mismatch;
+        sp.temperature_alipf = p->temperature_alipf;
+        sp.use_ribosum = p->use_ribosum != 0;
+        sp.exp_prob = p->exp_prob;
+        P->tc.reset(new TraceController(P->A->length(), P->B->length()));
+        restrict_trace_by_probabilities(*P->tc, *P->A, *P->B, sp, p->min_trace_prob, p->band_extended != 0,
+                                        p->sequ_local != 0);
+        P->bA.reset(new BasePairs(P->A.get(), p->min_prob));
+        P->bB.reset(new BasePairs(P->B.get(), p->min_prob));
+        P->am.reset(new ArcMatches(*P->bA, *P->bB, *P->tc));
+        P->sc.reset(new Scoring(*P->A, *P->B, *P->am, sp));
+        pack(*P);
+        *out = P.release();
+        return LCG_OK;
+    } catch (failure &f) {
+        set_err(err, errlen, f.what());
+        return LCG_E_INPUT;
+    } catch (std::exception &e) {
+        set_err(err, errlen, e.what());
+        return LCG_E_INPUT;
+    }
+}
+
+extern "C" void lcg_pair_free(lcg_pair *p) { delete p; }
+
+extern "C" int lcg_pair_info(const lcg_pair *p, int32_t *lenA, int32_t *lenB, int32_t *nam, int32_t *ngroups,
+                             int64_t *cells) {
+    if (!p) return LCG_E_ARG;
+    if (lenA) *lenA = p->desc.n;
+    if (lenB) *lenB = p->desc.m;
+    if (nam) *nam = p->desc.nam;
+    if (ngroups) *ngroups = p->desc.ngroups;
+    if (cells) *cells = p->cells;
+    return LCG_OK;
+}
+
+extern "C" int lcg_pair_band(const lcg_pair *p, int32_t *mn, int32_t *mx, int32_t cap) {
+    if (!p || cap < p->desc.n + 1) return LCG_E_ARG;
+    for (int i = 0; i <= p->desc.n; ++i) {
+        mn[i] = p->tc->min_col(i);
+        mx[i] = p->tc->max_col(i);
+    }
+    return p->desc.n + 1;
+}
+
+extern "C" int lcg_pair_arcmatches(const lcg_pair *p, int32_t *al, int32_t *ar, int32_t *bl, int32_t *br,
+                                   int64_t *score, int32_t cap) {
+    if (!p) return LCG_E_ARG;
+    const int nam = p->am->num_arc_matches();
+    if (cap < nam) return LCG_E_ARG;
+    for (int k = 0; k < nam; ++k) {
+        const ArcMatch &x = p->am->arcmatch(k);
+        al[k] = p->bA->arc(x.arcA).left;
+        ar[k] = p->bA->arc(x.arcA).right;
+        bl[k] = p->bB->arc(x.arcB).left;
+        br[k] = p->bB->arc(x.arcB).right;
+        score[k] = p->sc->arcmatch(k);
+    }
+    return nam;
+}
+
+// ------------------------------------------------------------------ batch
+
+struct lcg_batch {
+    int device = 0;
+    int npairs = 0;
+    std::vector<PairDesc> descs;      // absolute offsets
+    std::vector<int32_t> host_arena;  // image uploaded at creation
+    std::vector<int32_t> order;       // work queue order (longest first)
+    int32_t *d_arena = nullptr;
+    PairDesc *d_descs = nullptr;
+    int32_t *d_order = nullptr;
+    int32_t *d_queue = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_trace = -1;
+    BatchParams bp;
+    int waves = 1, grid = 1;
+    size_t lds_bytes = 0;
+    bool have_results = false;
+    bool fetched = false;
+    bool traced = false;
+    std::vector<int32_t> result;  // arena copied back after a run
+};
+
+namespace {
+int hip_fail(hipError_t e, char *err = nullptr, int errlen = 0) {
+    set_err(err, errlen, std::string("HIP: ") + hipGetErrorString(e));
+    return LCG_E_HIP;
+}
+#define HIPCHECK(x)                          \
+    do {                                     \
+        hipError_t e_ = (x);                 \
+        if (e_ != hipSuccess) return hip_fail(e_); \
+    } while (0)
+}  // namespace
+
+extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t device, lcg_batch **out, char *err,
+                                int errlen) {
+    if (!pairs || npairs <= 0 || !out) return LCG_E_ARG;
+    *out = nullptr;
+    auto B = std::make_unique<lcg_batch>();
+    B->device = device;
+    B->npairs = npairs;
+    const lcg_params &p0 = pairs[0]->params;
+    int max_len = 0, ecap = 1, gbuf = 1;
+    size_t total = 0;
+    for (int k = 0; k < npairs; ++k) {
+        const lcg_pair *P = pairs[k];
+        if (P->params.indel_opening != p0.indel_opening || P->params.sequ_local != p0.sequ_local) {
+            set_err(err, errlen, "all pairs of a batch must share indel_opening and sequ_local");
+            return LCG_E_ARG;
+        }
+        max_len = std::max(max_len, P->desc.n);
+        ecap = std::max(ecap, P->ecap);
+        gbuf = std::max(gbuf, P->desc.gbuf);
+        total += P->blob.size();
+    }
+    if (total >= (size_t)0x7fffffff) {
+        set_err(err, errlen, "batch too large for one arena");
+        return LCG_E_CAPACITY;
+    }
+    B->host_arena.reserve(total);
+    for (int k = 0; k < npairs; ++k) {
+        const lcg_pair *P = pairs[k];
+        const int32_t base = (int32_t)B->host_arena.size();
+        B->host_arena.insert(B->host_arena.end(), P->blob.begin(), P->blob.end());
+        PairDesc d = P->desc;
+        int32_t *offs[] = {&d.band, &d.codes, &d.gaps, &d.sigma, &d.rows, &d.groups, &d.gams, &d.amscore,
+                           &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.topM, &d.out};
+        for (int32_t *o : offs) *o += base;
+        B->descs.push_back(d);
+    }
+    // longest pairs first (greedy LPT over the persistent workgroups)
+    B->order.resize(npairs);
+    std::iota(B->order.begin(), B->order.end(), 0);
+    std::vector<int64_t> work(npairs);
+    for (int k = 0; k < npairs; ++k) work[k] = pairs[k]->cells;
+    std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) { return work[a] > work[b]; });
+    // LDS: header (band arrays of the longest sequence) + per-wave E and M
+    const int hdr = 3 * (max_len + 2) + LCG_NCODES * LCG_NCODES + 8;
+    const int ecap4 = (ecap + 3) & ~3;
+    const int wave_ints = ecap4 + ((gbuf + 3) & ~3);
+    const int lds_limit = (160 * 1024 - 256) / 4;  // 256 bytes for the kernel's static LDS
+    int waves = std::min(8, (lds_limit - hdr) / wave_ints);
+    if (const char *w = std::getenv("LCG_WAVES")) waves = std::min(waves, std::max(1, std::atoi(w)));
+    if (waves < 1) {
+        set_err(err, errlen, "arc-match group matrix exceeds the LDS of one workgroup");
+        return LCG_E_CAPACITY;
+    }
+    B->waves = waves;
+    B->lds_bytes = (size_t)(hdr + waves * wave_ints) * 4;
+    B->bp.alpha = p0.indel_opening;
+    B->bp.local = p0.sequ_local;
+    B->bp.trace = 1;
+    B->bp.npairs = npairs;
+    B->bp.lds_wave_ints = wave_ints;
+    B->bp.lds_hdr_ints = hdr;
+    B->bp.max_len = max_len;
+    B->bp.ecap = ecap4;
+
+    hipError_t e;
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, err, errlen);
+    int cus = 0;
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
+        return hip_fail(e, err, errlen);
+    const int per_cu = std::max(1, (int)((160 * 1024) / B->lds_bytes));
+    B->grid = std::min(npairs, cus * per_cu);
+    if ((e = hipStreamCreate(&B->stream)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipEventCreate(&B->ev0)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipEventCreate(&B->ev1)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMalloc(&B->d_arena, B->host_arena.size() * 4)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMalloc(&B->d_descs, sizeof(PairDesc) * npairs)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMalloc(&B->d_order, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMalloc(&B->d_queue, 64)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMemcpy(B->d_arena, B->host_arena.data(), B->host_arena.size() * 4, hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return hip_fail(e, err, errlen);
+    if ((e = hipMemcpy(B->d_descs, B->descs.data(), sizeof(PairDesc) * npairs, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, err, errlen);
+    if ((e = hipMemcpy(B->d_order, B->order.data(), 4 * (size_t)npairs, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, err, errlen);
+    *out = B.release();
+    return LCG_OK;
+}
+
+extern "C" void lcg_batch_destroy(lcg_batch *B) {
+    if (!B) return;
+    hipSetDevice(B->device);
+    if (B->graph_exec) hipGraphExecDestroy(B->graph_exec);
+    if (B->graph) hipGraphDestroy(B->graph);
+    if (B->d_arena) hipFree(B->d_arena);
+    if (B->d_descs) hipFree(B->d_descs);
+    if (B->d_order) hipFree(B->d_order);
+    if (B->d_queue) hipFree(B->d_queue);
+    if (B->ev0) hipEventDestroy(B->ev0);
+    if (B->ev1) hipEventDestroy(B->ev1);
+    if (B->stream) hipStreamDestroy(B->stream);
+    delete B;
+}
+
+extern "C" int lcg_batch_config(const lcg_batch *B, int32_t *waves, int32_t *workgroups, int32_t *lds_bytes) {
+    if (!B) return LCG_E_ARG;
+    if (waves) *waves = B->waves;
+    if (workgroups) *workgroups = B->grid;
+    if (lds_bytes) *lds_bytes = (int32_t)B->lds_bytes;
+    return LCG_OK;
+}
+
+extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *device_ms) {
+    if (!B || mode < 0 || mode > 2) return LCG_E_ARG;
+    HIPCHECK(hipSetDevice(B->device));
+    BatchParams bp = B->bp;
+    bp.trace = (trace & 1) ? 1 : 0;
+    bp.count = (trace & 2) ? 1 : 0;
+    if (mode == 2 && bp.count) return LCG_E_ARG;
+    HIPCHECK(hipEventRecord(B->ev0, B->stream));
+    if (mode == 0) {
+        int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, bp, B->grid, B->waves,
+                                  B->lds_bytes, B->stream);
+        if (rc) return hip_fail((hipError_t)rc);
+    } else if (mode == 1) {
+        BatchParams one = bp;
+        one.npairs = 1;
+        for (int k = 0; k < B->npairs; ++k) {
+            int rc = lcg_launch_align(B->d_descs, B->d_order + k, B->d_queue, B->d_arena, one, 1, B->waves,
+                                      B->lds_bytes, B->stream);
+            if (rc) return hip_fail((hipError_t)rc);
+        }
+    } else {
+        if (!B->graph_exec || B->graph_trace != bp.trace) {
+            if (B->graph_exec) hipGraphExecDestroy(B->graph_exec);
+            if (B->graph) hipGraphDestroy(B->graph);
+            B->graph_exec = nullptr;
+            B->graph = nullptr;
+            hipStream_t cs;
+            HIPCHECK(hipStreamCreate(&cs));
+            HIPCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+            int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, bp, B->grid, B->waves,
+                                      B->lds_bytes, cs);
+            hipGraph_t g = nullptr;
+            hipError_t ec = hipStreamEndCapture(cs, &g);
+            hipStreamDestroy(cs);
+            if (rc) return hip_fail((hipError_t)rc);
+            if (ec != hipSuccess) return hip_fail(ec);
+            B->graph = g;
+            HIPCHECK(hipGraphInstantiate(&B->graph_exec, g, nullptr, nullptr, 0));
+            B->graph_trace = bp.trace;
+        }
+        HIPCHECK(hipGraphLaunch(B->graph_exec, B->stream));
+    }
+    HIPCHECK(hipEventRecord(B->ev1, B->stream));
+    HIPCHECK(hipEventSynchronize(B->ev1));
+    if (device_ms) HIPCHECK(hipEventElapsedTime(device_ms, B->ev0, B->ev1));
+    B->have_results = true;
+    B->fetched = false;
+    B->traced = bp.trace != 0;
+    return LCG_OK;
+}
+
+namespace {
+// Copies the arena back once per run, on the first result request.
+int fetch(const lcg_batch *Bc) {
+    lcg_batch *B = const_cast<lcg_batch *>(Bc);
+    if (!B->have_results) return LCG_E_STATE;
+    if (B->fetched) return LCG_OK;
+    HIPCHECK(hipSetDevice(B->device));
+    B->result.resize(B->host_arena.size());
+    HIPCHECK(hipMemcpy(B->result.data(), B->d_arena, B->result.size() * 4, hipMemcpyDeviceToHost));
+    B->fetched = true;
+    return LCG_OK;
+}
+}  // namespace
+
+extern "C" int lcg_batch_score(const lcg_batch *B, int32_t k, int64_t *score) {
+    if (!B || k < 0 || k >= B->npairs || !score) return LCG_E_ARG;
+    if (int rc = fetch(B)) return rc;
+    *score = B->result[B->descs[k].out];
+    return LCG_OK;
+}
+
+extern "C" int lcg_batch_alignment(const lcg_batch *B, int32_t k, int32_t *a, int32_t *b, int32_t cap,
+                                   char *structA, char *structB, int32_t scap) {
+    if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    if (!B->traced) return LCG_E_STATE;
+    if (int rc = fetch(B)) return rc;
+    const PairDesc &d = B->descs[k];
+    int cnt = 0;
+    for (int s = 0; s <= d.n + d.m; ++s) {
+        const int32_t v = B->result[d.edges + s];
+        if (v == -1) continue;
+        if (cnt >= cap) return LCG_E_ARG;
+        const int x = v >> 16, y = v & 0xffff;
+        a[cnt] = x ? x : -1;
+        b[cnt] = y ? y : -1;
+        ++cnt;
+    }
+    if (structA && structB) {
+        if (scap <= d.n || scap <= d.m) return LCG_E_ARG;
+        static const char sym[3] = {'.', '(', ')'};
+        for (int i = 0; i < d.n; ++i) structA[i] = sym[B->result[d.structs + i]];
+        for (int j = 0; j < d.m; ++j) structB[j] = sym[B->result[d.structs + d.n + j]];
+        structA[d.n] = 0;
+        structB[d.m] = 0;
+    }
+    return cnt;
+}
+
+extern "C" int lcg_batch_D(const lcg_batch *B, int32_t k, int64_t *D, int32_t cap) {
+    if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    if (int rc = fetch(B)) return rc;
+    const PairDesc &d = B->descs[k];
+    if (cap < d.nam) return LCG_E_ARG;
+    for (int t = 0; t < d.nam; ++t) {
+        D[t] = B->result[d.D + t];
+    }
+    return d.nam;
+}
+
+extern "C" int lcg_batch_counters(const lcg_batch *B, int32_t k, int64_t *cells, int64_t *candidates) {
+    if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    if (int rc = fetch(B)) return rc;
+    const PairDesc &d = B->descs[k];
+    uint64_t c0, c1;
+    std::memcpy(&c0, &B->result[d.out + 4], 8);
+    std::memcpy(&c1, &B->result[d.out + 6], 8);
+    if (cells) *cells = (int64_t)c0;
+    if (candidates) *candidates = (int64_t)c1;
+    return LCG_OK;
+}

@@ -1,0 +1,403 @@
+// Host preprocessing for the MI355X aligner (see host.hpp).
+#include "host.hpp"
+
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <sstream>
+
+#include "ribosum85_60.h"
+
+namespace locarna_amd {
+
+// ------------------------------------------------------------------ RnaData
+
+namespace {
+// A line that is neither empty nor starts with white space (white-space
+// lines are comments in pp 2.0), trailing blanks removed.
+bool next_content_line(std::istream &in, std::string &line) {
+    while (std::getline(in, line)) {
+        if (!line.empty() && !std::isspace((unsigned char)line[0])) {
+            size_t e = line.find_last_not_of(" \t\r");
+            line.resize(e + 1);
+            return true;
+        }
+    }
+    return false;
+}
+bool starts_with(const std::string &s, const char *p) { return s.compare(0, std::strlen(p), p) == 0; }
+}  // namespace
+
+RnaData::RnaData(const std::string &pp_text, double p_bpcut, long max_bp_span) : p_bpcut_(p_bpcut) {
+    std::istringstream in(pp_text);
+    std::string line;
+    std::getline(in, line);
+    if (!starts_with(line, "#PP 2")) throw failure("RnaData: input is not in #PP 2.0 format");
+    std::vector<std::string> names;
+    std::map<std::string, std::string> rows;
+    while (next_content_line(in, line)) {
+        if (starts_with(line, "#END")) break;
+        if (line[0] == '#') continue;  // #A anchors, #S structure: not used by this path
+        std::istringstream ls(line);
+        std::string nm, s;
+        ls >> nm >> s;
+        if (!rows.count(nm)) names.push_back(nm);
+        rows[nm] += s;
+    }
+    if (names.size() != 1) throw failure("RnaData: expected exactly one sequence in the pp alignment block");
+    name_ = names[0];
+    seq_ = rows[name_];
+    for (auto &c : seq_) {  // aux.cc:65-70
+        c = (char)std::toupper((unsigned char)c);
+        if (c == 'T') c = 'U';
+    }
+    if (!next_content_line(in, line) || line != "#SECTION BASEPAIRS")
+        throw failure("Syntax error: Expected base pair section header.");
+    const size_t span_max = max_bp_span < 0 ? std::numeric_limits<size_t>::max() : (size_t)max_bp_span;
+    while (next_content_line(in, line)) {
+        if (line[0] == '#') {
+            if (starts_with(line, "#END")) break;
+            if (starts_with(line, "#BPCUT")) {
+                std::istringstream ls(line);
+                std::string kw;
+                double p;
+                ls >> kw >> p;
+                if (ls.fail()) throw failure("Syntax error: Cannot parse line \"" + line + "\" in base pairs section.");
+                p_bpcut_ = std::max(p, p_bpcut_);
+            }
+            continue;
+        }
+        std::istringstream ls(line);
+        size_t i, j;
+        double p;
+        ls >> i >> j >> p;
+        if (ls.fail()) throw failure("Syntax error: Cannot parse line \"" + line + "\" in base pairs section.");
+        if (!(1 <= i && i < j && j <= seq_.size()))
+            throw failure("Syntax error: Invalid indices in PP input line \"" + line + "\".");
+        if (p <= p_bpcut_ || j - i + 1 > span_max) continue;
+        probs_[{(int)i, (int)j}] = p;
+    }
+    for (const auto &x : probs_) pairs_.push_back(x);
+}
+
+double RnaData::arc_prob(int i, int j) const {
+    auto it = probs_.find({i, j});
+    return it == probs_.end() ? 0.0 : it->second;
+}
+
+double RnaData::prob_paired_upstream(int i) const {
+    double p = 0.0;
+    for (auto it = probs_.lower_bound({i, 0}); it != probs_.end() && it->first.first == i; ++it) p += it->second;
+    return p;
+}
+
+double RnaData::prob_paired_downstream(int j) const {
+    double p = 0.0;  // summed over i ascending, as the reference does
+    for (int i = 1; i < j; ++i) {
+        auto it = probs_.find({i, j});
+        if (it != probs_.end()) p += it->second;
+    }
+    return p;
+}
+
+// ---------------------------------------------------------------- BasePairs
+
+BasePairs::BasePairs(const RnaData *rna, double min_prob) : len_(rna->length()) {
+    left_.resize(len_ + 2);
+    right_.resize(len_ + 2);
+    // arc index order: left end from len-3 down to 1, right end ascending
+    // (basepairs.cc:162-177)
+    std::vector<std::vector<std::pair<int, double>>> by_left(len_ + 1);
+    for (const auto &x : rna->pairs()) {
+        int i = x.first.first, j = x.first.second;
+        if (j >= i + 3) by_left[i].push_back({j, x.second});
+    }
+    for (int i = len_ - 3; i >= 1; --i) {
+        for (const auto &jp : by_left[i]) {  // ascending j
+            if (jp.second >= min_prob) {
+                int idx = (int)arcs_.size();
+                arcs_.push_back(Arc{idx, i, jp.first});
+                left_[i].push_back(idx);
+                right_[jp.first].push_back(idx);
+            }
+        }
+    }
+    for (auto &l : left_)
+        std::sort(l.begin(), l.end(), [&](int a, int b) { return arcs_[a].right < arcs_[b].right; });
+    for (auto &l : right_)
+        std::sort(l.begin(), l.end(), [&](int a, int b) { return arcs_[a].left > arcs_[b].left; });
+}
+
+// ---------------------------------------------------------- TraceController
+
+TraceController::TraceController(int lenA, int lenB)
+    : lenA_(lenA), lenB_(lenB), min_col_(lenA + 1, 0), max_col_(lenA + 1, lenB) {}
+
+void TraceController::restrict_by_trace_probabilities(const std::vector<double> &probs, double min_prob) {
+    const int W = lenB_ + 1;
+    for (int i = 0; i <= lenA_; ++i) {
+        int lo = max_col_[i], hi = min_col_[i];
+        for (int j = min_col_[i]; j <= max_col_[i]; ++j) {
+            if (probs[(size_t)i * W + j] >= min_prob) {
+                lo = std::min(lo, j);
+                hi = std::max(hi, j);
+            }
+        }
+        min_col_[i] = std::max(min_col_[i], lo);
+        max_col_[i] = std::min(max_col_[i], hi);
+    }
+    int run = 0;
+    for (int i = 0; i <= lenA_; ++i) run = max_col_[i] = std::max(max_col_[i], run);
+    run = max_col_[lenA_];
+    for (int i = lenA_; i >= 0; --i) run = min_col_[i] = std::min(min_col_[i], run);
+}
+
+// ----------------------------------------------------------- band (PFGotoh)
+
+namespace {
+
+int nt_code(char c) {
+    switch (c) {
+        case 'A': return 0;
+        case 'C': return 1;
+        case 'G': return 2;
+        case 'U': return 3;
+    }
+    return -1;
+}
+
+// Gotoh partition function over cells (i, j) of the band, in the given
+// direction.  Forward: recursion from (0,0) with similarity sim(i, j).
+// Backward: the same recursion on the reversed sequences; written here on
+// original coordinates, Xr(i, j) holds the reference's X_rev(lenA-i,
+// lenB-j), whose similarity at reversed cell (lenA-i, lenB-j) equals
+// sim(i+1, j+1) bit for bit (sqrt terms added in swapped order, which is
+// exact).  Operation order follows edge_probs.icc:76-162.
+template <class T>
+struct GotohPF {
+    int n, m;
+    bool local;
+    double g_open, g_ext;
+    std::vector<T> M, A, B;  // (n+1)*(m+1)
+    T &at(std::vector<T> &v, int i, int j) { return v[(size_t)i * (m + 1) + j]; }
+};
+
+template <class T>
+void gotoh_forward(GotohPF<T> &g, const std::vector<double> &match_w, const std::vector<int> &mn,
+                   const std::vector<int> &mx) {
+    const int n = g.n, m = g.m;
+    const double go = g.g_open, ge = g.g_ext;
+    g.M.assign((size_t)(n + 1) * (m + 1), (T)0);
+    g.A = g.M;
+    g.B = g.M;
+    auto valid = [&](int i, int j) { return mn[i] <= j && j <= mx[i]; };
+    if (valid(0, 0)) g.at(g.M, 0, 0) = g.local ? 0 : 1;
+    if (n > 0 && valid(1, 0)) g.at(g.A, 1, 0) = go * ge;
+    if (m > 0 && valid(0, 1)) g.at(g.B, 0, 1) = go * ge;
+    for (int i = 2; i <= n; ++i) {
+        if (mn[i] > 0) break;
+        g.at(g.A, i, 0) = g.at(g.A, i - 1, 0) * ge;
+    }
+    for (int j = std::max(mn[0], 2); j <= std::min(mx[0], m); ++j) g.at(g.B, 0, j) = g.at(g.B, 0, j - 1) * ge;
+    const double lw = 0;
+    for (int i = 1; i <= n; ++i) {
+        for (int j = std::max(mn[i], 1); j <= std::min(mx[i], m); ++j) {
+            double w = match_w[(size_t)i * (m + 1) + j];
+            g.at(g.M, i, j) = g.at(g.M, i - 1, j - 1) * w + g.at(g.A, i - 1, j - 1) * w +
+                              g.at(g.B, i - 1, j - 1) * w + (g.local ? w : lw);
+            g.at(g.A, i, j) = g.at(g.A, i - 1, j) * ge + g.at(g.M, i - 1, j) * go * ge + g.at(g.B, i - 1, j) * go * ge;
+            g.at(g.B, i, j) = g.at(g.B, i, j - 1) * ge + g.at(g.M, i, j - 1) * go * ge + g.at(g.A, i, j - 1) * go * ge;
+        }
+    }
+}
+
+template <class T>
+std::vector<double> trace_probs(const RnaData &RA, const RnaData &RB, const TraceController &tc,
+                                const ScoringParams &sp, bool local) {
+    const int n = RA.length(), m = RB.length();
+    const double temp = sp.temperature_alipf / 100.0;
+    const double gap_open = sp.indel_opening / 100.0, gap_ext = sp.indel / 100.0;
+    const double sw = sp.struct_weight / 100.0;
+    // base similarity: RIBOSUM bm scores or match / mismatch (main_helper.icc:284-308)
+    double sim[4][4];
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b)
+            sim[a][b] = sp.use_ribosum ? ribosum85_60::bm[a * 4 + b] : (a == b ? sp.match : sp.mismatch);
+    std::vector<double> upA(n + 1, 0.0), dnA(n + 1, 0.0), upB(m + 1, 0.0), dnB(m + 1, 0.0);
+    for (int i = 1; i <= n; ++i) {
+        upA[i] = RA.prob_paired_upstream(i);
+        dnA[i] = RA.prob_paired_downstream(i);
+    }
+    for (int j = 1; j <= m; ++j) {
+        upB[j] = RB.prob_paired_upstream(j);
+        dnB[j] = RB.prob_paired_downstream(j);
+    }
+    // Boltzmann weights of the similarity, forward orientation
+    std::vector<double> fw((size_t)(n + 1) * (m + 1), 0.0);
+    for (int i = 1; i <= n; ++i) {
+        int a = nt_code(RA.nt(i));
+        for (int j = 1; j <= m; ++j) {
+            int b = nt_code(RB.nt(j));
+            double seq = 0;
+            int cnt = 0;
+            if (a >= 0 && b >= 0) {
+                seq += sim[a][b];
+                cnt++;
+            }
+            if (cnt != 0) seq /= cnt;
+            double s = sw * (std::sqrt(dnA[i] * dnB[j]) + std::sqrt(upA[i] * upB[j])) + seq;
+            fw[(size_t)i * (m + 1) + j] = std::exp(s / temp);
+        }
+    }
+    GotohPF<T> F{n, m, local, std::exp(gap_open / temp), std::exp(gap_ext / temp)};
+    gotoh_forward(F, fw, tc.min_cols(), tc.max_cols());
+    // reversed problem: cell (i', j') of the reversed matrices <-> (n-i', m-j')
+    std::vector<double> rw((size_t)(n + 1) * (m + 1), 0.0);
+    for (int ip = 1; ip <= n; ++ip)
+        for (int jp = 1; jp <= m; ++jp) rw[(size_t)ip * (m + 1) + jp] = fw[(size_t)(n + 1 - ip) * (m + 1) + (m + 1 - jp)];
+    std::vector<int> rmn(n + 1), rmx(n + 1);
+    for (int i = 0; i <= n; ++i) {
+        rmx[n - i] = m - tc.min_col(i);
+        rmn[n - i] = m - tc.max_col(i);
+    }
+    GotohPF<T> R{n, m, local, F.g_open, F.g_ext};
+    gotoh_forward(R, rw, rmn, rmx);
+    T z;
+    if (local) {
+        z = 1;
+        for (int i = 0; i <= n; ++i)
+            for (int j = 0; j <= m; ++j) z += F.at(F.M, i, j);
+    } else {
+        z = F.at(F.M, n, m) + F.at(F.A, n, m) + F.at(F.B, n, m);
+    }
+    std::vector<double> probs((size_t)(n + 1) * (m + 1), 0.0);
+    const double locality_add = local ? 1 : 0;
+    const double g_open = std::exp(gap_open / temp);
+    for (int i = 0; i <= n; ++i) {
+        for (int j = std::max(tc.min_col(i), 0); j <= std::min(tc.max_col(i), m); ++j) {
+            int ri = n - i, rj = m - j;
+            T rM = R.at(R.M, ri, rj), rA = R.at(R.A, ri, rj), rB = R.at(R.B, ri, rj);
+            T zij = F.at(F.M, i, j) * (rM + rA + rB + locality_add) + F.at(F.A, i, j) * (rM + rA / g_open + rB) +
+                    F.at(F.B, i, j) * (rM + rA + rB / g_open);
+            probs[(size_t)i * (m + 1) + j] = zij / z;
+        }
+    }
+    return probs;
+}
+
+}  // namespace
+
+void restrict_trace_by_probabilities(TraceController &tc, const RnaData &A, const RnaData &B,
+                                     const ScoringParams &sp, double min_trace_prob, bool extended_precision,
+                                     bool local) {
+    if (!(min_trace_prob > 0.0)) return;
+    std::vector<double> probs = extended_precision ? trace_probs<long double>(A, B, tc, sp, local)
+                                                   : trace_probs<double>(A, B, tc, sp, local);
+    tc.restrict_by_trace_probabilities(probs, min_trace_prob);
+}
+
+// --------------------------------------------------------------- ArcMatches
+
+ArcMatches::ArcMatches(const BasePairs &A, const BasePairs &B, const TraceController &tc, int max_diff_am,
+                       int max_diff_at_am)
+    : A_(A), B_(B) {
+    const int lmax = std::max(A.seqlen(), B.seqlen());
+    const int dlen = max_diff_am < 0 ? lmax : max_diff_am;
+    const int dend = max_diff_at_am < 0 ? lmax : max_diff_at_am;
+    for (int a = 0; a < A.num_bps(); ++a) {
+        const Arc &x = A.arc(a);
+        for (int b = 0; b < B.num_bps(); ++b) {
+            const Arc &y = B.arc(b);
+            bool ok = tc.is_valid_match(x.left, y.left) && tc.is_valid_match(x.right, y.right) &&
+                      std::abs((x.right - x.left) - (y.right - y.left)) <= dlen &&
+                      std::abs(x.left - y.left) <= dend && std::abs(x.right - y.right) <= dend;
+            if (ok) ams_.push_back(ArcMatch{(int)ams_.size(), a, b});
+        }
+    }
+}
+
+// ------------------------------------------------------------------ Scoring
+
+Scoring::Scoring(const RnaData &A, const RnaData &B, const ArcMatches &am, const ScoringParams &p)
+    : p_(p), am_(am) {
+    // symbol codes: ACGU first, other symbols in order of appearance
+    symbols_ = {'A', 'C', 'G', 'U'};
+    auto code_of = [&](char c) {
+        for (size_t k = 0; k < symbols_.size(); ++k)
+            if (symbols_[k] == c) return (int)k;
+        if ((int)symbols_.size() >= NCODES) throw failure("Scoring: too many distinct sequence symbols");
+        symbols_.push_back(c);
+        return (int)symbols_.size() - 1;
+    };
+    const int n = A.length(), m = B.length();
+    codeA_.assign(n + 1, 0);
+    codeB_.assign(m + 1, 0);
+    for (int i = 1; i <= n; ++i) codeA_[i] = code_of(A.nt(i));
+    for (int j = 1; j <= m; ++j) codeB_[j] = code_of(B.nt(j));
+    // sigma per symbol pair (scoring.cc:140-198, single-row alignments)
+    sigma_codes_.assign(NCODES * NCODES, 0);
+    for (size_t a = 0; a < symbols_.size(); ++a) {
+        for (size_t b = 0; b < symbols_.size(); ++b) {
+            char ca = symbols_[a], cb = symbols_[b];
+            score_t s = 0;
+            int xa = nt_code(ca), xb = nt_code(cb);
+            if (p.use_ribosum && xa >= 0 && xb >= 0) {
+                double bg = ribosum85_60::base_nonstruct[xa] * ribosum85_60::base_nonstruct[xb];
+                double corrected = std::log(ribosum85_60::basematch[xa * 4 + xb] / bg) / std::log(2);
+                s += round2score(100.0 * corrected);
+            } else if (ca != 'N' && cb != 'N') {
+                s += (ca == cb) ? p.match : p.mismatch;
+            }
+            sigma_codes_[a * NCODES + b] = round2score(s / (int)1);
+        }
+    }
+    // position specific gap cost of single sequences (scoring.cc:271-311)
+    gapA_.assign(n + 1, 0);
+    gapB_.assign(m + 1, 0);
+    for (int i = 1; i <= n; ++i) gapA_[i] = round2score((1 - 0.0f) * (score_t)p.indel);
+    for (int j = 1; j <= m; ++j) gapB_[j] = round2score((1 - 0.0f) * (score_t)p.indel);
+    // arc weights (scoring.cc:200-265)
+    const double peA = p.exp_prob > 0 ? p.exp_prob : 1.0 / (2.0 * n);
+    const double peB = p.exp_prob > 0 ? p.exp_prob : 1.0 / (2.0 * m);
+    auto weight = [&](double prob, double pe) {
+        return round2score(std::round((score_t)p.struct_weight * (1 - std::log(prob) / std::log(pe))));
+    };
+    const BasePairs &bA = am.bpsA(), &bB = am.bpsB();
+    wA_.resize(bA.num_bps());
+    wB_.resize(bB.num_bps());
+    for (int k = 0; k < bA.num_bps(); ++k) wA_[k] = weight(A.arc_prob(bA.arc(k).left, bA.arc(k).right), peA);
+    for (int k = 0; k < bB.num_bps(); ++k) wB_[k] = weight(B.arc_prob(bB.arc(k).left, bB.arc(k).right), peB);
+    // arc-match scores (scoring.cc:368-554)
+    const int na = am.num_arc_matches();
+    am_score_.resize(na);
+    am_seq_.assign(na, 0);
+    for (int k = 0; k < na; ++k) {
+        const ArcMatch &x = am.arcmatch(k);
+        const Arc &a = bA.arc(x.arcA), &b = bB.arc(x.arcB);
+        score_t seq = 0;
+        if (p.tau != 0) {
+            if (p.use_ribosum) {
+                int al = nt_code(A.nt(a.left)), ar = nt_code(A.nt(a.right));
+                int bl = nt_code(B.nt(b.left)), br = nt_code(B.nt(b.right));
+                if (al >= 0 && ar >= 0 && bl >= 0 && br >= 0) {
+                    double pam = ribosum85_60::arcmatch[(al * 4 + ar) * 16 + (bl * 4 + br)];
+                    double pa = ribosum85_60::basepair[al * 4 + ar], pb = ribosum85_60::basepair[bl * 4 + br];
+                    double sc = 0;
+                    sc += std::log(pam / (pa * pb)) / std::log(2);
+                    seq = round2score(100.0 * sc / 1);
+                }
+            } else {
+                seq = basematch(a.left, b.left) + basematch(a.right, b.right);
+            }
+        }
+        am_seq_[k] = seq;
+        am_score_[k] = ((score_t)p.tau * seq) / 100 + (wA_[x.arcA] + wB_[x.arcB]);
+    }
+}
+
+score_t Scoring::basematch(int i, int j) const { return sigma_codes_[codeA_[i] * NCODES + codeB_[j]]; }
+
+}  // namespace locarna_amd

@@ -1,0 +1,164 @@
+// Host side of the MI355X LocARNA aligner: input data, score tables, trace
+// band and arc-match lists, in the shape the gfx950 kernels consume.
+//
+// The classes keep the names and meaning of the reference interface
+// (LocARNA 2.0.1, src/LocARNA/*.hh) so that callers of RnaData, BasePairs,
+// TraceController, ArcMatches, Scoring and Aligner find the same objects;
+// the data layout is our own (flat arrays, CSR lists, int32 scores).
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace locarna_amd {
+
+typedef long score_t;  // reference: scoring_fwd.hh:13-17
+
+// reference: aux.hh:229-278 (LocARNA::failure)
+struct failure : std::runtime_error {
+    explicit failure(const std::string &s) : std::runtime_error(s) {}
+};
+
+// Sequence plus filtered base-pair probabilities read from #PP 2.0 text.
+// reference: RnaData, rna_data.hh:37; reader rna_data.cc:984-1124
+class RnaData {
+public:
+    RnaData(const std::string &pp_text, double p_bpcut, long max_bp_span);
+    int length() const { return (int)seq_.size(); }
+    const std::string &sequence() const { return seq_; }  // 0-based chars
+    char nt(int i) const { return seq_[i - 1]; }           // 1-based
+    const std::string &name() const { return name_; }
+    double arc_prob(int i, int j) const;
+    double prob_paired_upstream(int i) const;    // rna_data.cc:713-722
+    double prob_paired_downstream(int i) const;  // rna_data.cc:724-732
+    // base pairs kept after filtering, ordered i ascending, j ascending
+    const std::vector<std::pair<std::pair<int, int>, double>> &pairs() const { return pairs_; }
+
+private:
+    std::string name_, seq_;
+    std::map<std::pair<int, int>, double> probs_;
+    std::vector<std::pair<std::pair<int, int>, double>> pairs_;
+    double p_bpcut_;
+};
+
+struct Arc {
+    int idx, left, right;
+};
+
+// reference: BasePairs, basepairs.hh; basepairs.cc:156-220
+class BasePairs {
+public:
+    BasePairs(const RnaData *rna, double min_prob);
+    int num_bps() const { return (int)arcs_.size(); }
+    const Arc &arc(int idx) const { return arcs_[idx]; }
+    int seqlen() const { return len_; }
+    // arcs with right end j sorted by left end descending (right_adjlist)
+    const std::vector<int> &right_adj(int j) const { return right_[j]; }
+    // arcs with left end i sorted by right end ascending (left_adjlist)
+    const std::vector<int> &left_adj(int i) const { return left_[i]; }
+
+private:
+    int len_;
+    std::vector<Arc> arcs_;
+    std::vector<std::vector<int>> left_, right_;
+};
+
+// Per-row valid column range of the DP matrices.
+// reference: TraceController, trace_controller.hh:328-336
+class TraceController {
+public:
+    TraceController(int lenA, int lenB);
+    int rows() const { return lenA_; }
+    int min_col(int i) const { return min_col_[i]; }
+    int max_col(int i) const { return max_col_[i]; }
+    bool is_valid(int i, int j) const { return min_col_[i] <= j && j <= max_col_[i]; }
+    bool is_valid_match(int i, int j) const { return is_valid(i, j) && is_valid(i - 1, j - 1); }
+    // trace_controller.cc:565-597
+    void restrict_by_trace_probabilities(const std::vector<double> &probs, double min_prob);
+    const std::vector<int> &min_cols() const { return min_col_; }
+    const std::vector<int> &max_cols() const { return max_col_; }
+
+private:
+    int lenA_, lenB_;
+    std::vector<int> min_col_, max_col_;
+};
+
+// Numeric parameters; names follow the locarna / locarna_p command lines
+// (src/locarna.cc:83-272, src/locarna_p.cc:95-200) and ScoringParams
+// (scoring.hh:51-225).
+struct ScoringParams {
+    int indel = -150;
+    int indel_opening = -750;
+    int struct_weight = 200;
+    int tau = 50;
+    int match = 50;
+    int mismatch = 0;
+    int temperature_alipf = 300;
+    bool use_ribosum = true;
+    double exp_prob = -1.0;  // <= 0: 1/(2 len)  (aux.hh:216-220)
+};
+
+// Trace-probability band: Gotoh partition function over the StralScore
+// similarity, forward and backward, then the probability cut and the
+// monotone fix-ups.  reference: edge_probs.icc:7-162, :212-268;
+// stral_score.cc:29-60; main_helper.icc:371-426
+void restrict_trace_by_probabilities(TraceController &tc, const RnaData &A, const RnaData &B,
+                                     const ScoringParams &sp, double min_trace_prob,
+                                     bool extended_precision, bool local);
+
+struct ArcMatch {
+    int idx;
+    int arcA, arcB;  // arc indices
+};
+
+// Valid arc matches in reference order (arc A index outer, arc B inner).
+// reference: ArcMatches, arc_matches.cc:19-48, 130-188
+class ArcMatches {
+public:
+    ArcMatches(const BasePairs &A, const BasePairs &B, const TraceController &tc, int max_diff_am = -1,
+               int max_diff_at_am = -1);
+    int num_arc_matches() const { return (int)ams_.size(); }
+    const ArcMatch &arcmatch(int k) const { return ams_[k]; }
+    const BasePairs &bpsA() const { return A_; }
+    const BasePairs &bpsB() const { return B_; }
+
+private:
+    const BasePairs &A_, &B_;
+    std::vector<ArcMatch> ams_;
+};
+
+// Integer score tables.  reference: Scoring, scoring.cc:28-554
+class Scoring {
+public:
+    Scoring(const RnaData &A, const RnaData &B, const ArcMatches &am, const ScoringParams &p);
+    static score_t round2score(double d) { return (score_t)((d < 0) ? (d - 0.5) : (d + 0.5)); }
+    score_t basematch(int i, int j) const;  // sigma(i, j)
+    score_t gapA(int i) const { return gapA_[i]; }
+    score_t gapB(int j) const { return gapB_[j]; }
+    score_t indel_opening() const { return p_.indel_opening; }
+    score_t arcmatch(int am) const { return am_score_[am]; }
+    score_t weightA(int arc) const { return wA_[arc]; }
+    score_t weightB(int arc) const { return wB_[arc]; }
+    const ScoringParams &params() const { return p_; }
+    // single-sequence symbol code (0..3 = ACGU, 4.. other symbols)
+    int codeA(int i) const { return codeA_[i]; }
+    int codeB(int j) const { return codeB_[j]; }
+    static const int NCODES = 8;
+    score_t sigma_code(int a, int b) const { return sigma_codes_[a * NCODES + b]; }
+    // ribosum arc-match term riboX (scoring.cc:368-438) before the tau factor
+    score_t arc_sequence_term(int am) const { return am_seq_[am]; }
+
+private:
+    ScoringParams p_;
+    const ArcMatches &am_;
+    std::vector<int> codeA_, codeB_;
+    std::vector<char> symbols_;
+    std::vector<score_t> sigma_codes_;
+    std::vector<score_t> gapA_, gapB_, wA_, wB_, am_score_, am_seq_;
+};
+
+}  // namespace locarna_amd
