@@ -1,25 +1,31 @@
 // gfx950 kernels of the integer LocARNA alignment (max-plus DP, int32).
 //
-// One workgroup of NW wave64s aligns one pair at a time and pulls pairs
-// from a work queue (persistent grid).  Per pair:
-//   1. fill D: rows of left ends al = n .. 1; the arc-match groups (al, bl)
-//      of one row are independent and are spread over the waves; each wave
-//      computes its group's matrix M row by row with lanes over columns
-//      (reference: AlignerImpl::align_D / align_in_arcmatch / align_noex /
+// A workgroup of NW wave64s aligns one pair at a time, pulling pairs from a
+// work queue (persistent grid, longest pairs first).  Per pair:
+//   1. fill D: rows of left ends al = n .. 1.  The arc-match groups (al, bl)
+//      of one row are independent; the host packs them into the workgroup's
+//      LDS pool (rounds of at most NW groups) and each wave computes its
+//      group's matrix M row by row, lanes over columns (reference:
+//      AlignerImpl::align_D / align_in_arcmatch / align_noex /
 //      fill_D_entries, src/LocARNA/aligner.cc:153-233, 264-437, 574-732);
 //   2. the top level (global or sequence-local, aligner.cc:736-880);
-//   3. the traceback (aligner.cc:967-1358), on the device: the top level is
-//      traced by one wave, every traced arc match is re-aligned and traced
-//      by any free wave (breadth first).  Each emitted edge is written to the
-//      slot i+j of the cell it reaches, which orders the edges without a sort.
+//   3. the traceback (aligner.cc:967-1358), on the device: one wave traces
+//      the top level, every traced arc match is re-aligned and traced by a
+//      free wave (breadth first).  Each edge is written to slot i+j of the
+//      cell it reaches, which orders the edges without a sort.
 //
-// Row recurrence of one group (origin (al, bl)), lane l owns column j:
-//   s(i,j)  = max(M(i-1,j-1) + sigma(i,j), E(i,j), max_arcs M(p-1,q-1) + D(a,b))
-//   E(i,j)  = max(E(i-1,j) + gA(i), M(i-1,j) + gA(i) + alpha)
-//   F(i,j)  = max(F(i,j-1) + gB(j) + max(0,alpha), s(i,j-1) + gB(j) + alpha)
-//   M(i,j)  = max(s(i,j), F(i,j))
-// The F chain is a max-plus linear recurrence along the row; it is solved by
-// a prefix sum and a prefix max across the wave (DPP row_shr / row_bcast):
+// Row recurrence of a group (origin (al, bl)), lane l owns column j:
+//   s(i,j) = max(M(i-1,j-1) + sigma(i,j), E(i,j), S(i,j))
+//   S(i,j) = max over arc matches (p..i, q..j), p > al, q > bl, of
+//            M(p-1,q-1) + D(p..i, q..j)
+//   E(i,j) = max(E(i-1,j) + gA(i), M(i-1,j) + gA(i) + alpha)
+//   F(i,j) = max(F(i,j-1) + gB(j) + max(0,alpha), s(i,j-1) + gB(j) + alpha)
+//   M(i,j) = max(s(i,j), F(i,j))
+// S is evaluated candidate-parallel: the arc matches ending in row i within
+// the group's columns are one contiguous run of the candidate list; lanes
+// stride over it and combine with LDS atomic max.  The F chain is a max-plus
+// linear recurrence along the row, solved by a prefix sum and a prefix max
+// across the wave (DPP row_shr / row_bcast):
 //   F(j) = P(j) + max_{k<=j} (c(k) - P(k)),  P = inclusive sum of a.
 // Scores are int32 with minus infinity LCG_NEG = -2^29; values below -2^28
 // are normalized to LCG_NEG, which leaves every finite result unchanged.
@@ -67,17 +73,29 @@ __device__ __forceinline__ int wave_shift_up(int v, int first) {
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 __device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
 
+// Orders the memory accesses of one wave to a region it owns: LDS
+// instructions of a wave complete in order, so only the compiler needs a
+// barrier; global scratch additionally needs the L1 written back and
+// invalidated (agent-scope fence).
+template <bool GLOBAL>
+__device__ __forceinline__ void region_sync() {
+    if (GLOBAL)
+        __threadfence();
+    else
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+}
+
 struct Pair {
-    const int *__restrict__ mn;    // LDS: min_col[0..n]
-    const int *__restrict__ mx;    // LDS: max_col[0..n]
-    const int *__restrict__ coff;  // LDS: cell_off[0..n+1]
-    const int *__restrict__ sig;   // LDS: 8x8 sigma
-    const int *__restrict__ codeA;
-    const int *__restrict__ codeB;
-    const int *__restrict__ gapA;
-    const int *__restrict__ gapB;
-    const int *__restrict__ cptr;
-    const int *__restrict__ cand;
+    const int *__restrict__ mn;     // LDS: min_col[0..n]
+    const int *__restrict__ mx;     // LDS: max_col[0..n]
+    const int *__restrict__ coff;   // LDS: cell_off[0..n+1]
+    const int *__restrict__ codeA;  // LDS
+    const int *__restrict__ codeB;  // LDS
+    const int *__restrict__ gapA;   // LDS
+    const int *__restrict__ gapB;   // LDS
+    const int *__restrict__ sig;    // LDS: 8x8 sigma
+    const int *__restrict__ cptr;   // global
+    int4 *cand;                     // global: (al|bl<<16, j, D, am)
     const int *__restrict__ amscore;
     int *D;
     int *edges;
@@ -96,16 +114,19 @@ struct TopBest {
 };
 
 // Fills the matrix of the group with origin (al, bl), rows al..ar-1, columns
-// bl..br-1 (within the band) into Mst (row r = i - al at Mst + r*W, column j
-// at j - row_lo(i)).  Est holds E over columns [bl, br).
-// GLOBAL: Mst/Est are in global memory (fence per row); TOP_LOCAL: the
-// sequence-local top level (values clamped at 0, first maximum tracked).
+// bl..br-1 (within the band), into the region at `base` (layout E | S | M,
+// M row r = i - al at M + r*W, column j at j - row_lo(i)).
+// GLOBAL: the region is global scratch; TOP_LOCAL: the sequence-local top
+// level (values clamped at 0, first maximum tracked).
 template <bool GLOBAL, bool TOP_LOCAL>
-__device__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W, int *Mst, int *Est, bool global_init,
-                           TopBest *best, unsigned long long *counts = nullptr) {
-    unsigned cnt_cells = 0, cnt_cand = 0;
+__device__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W, int *base, bool global_init,
+                           TopBest *best, unsigned long long *counts) {
     const int lane = lane_id();
     const int alpha = P.alpha;
+    int *Est = base;
+    int *Srow = base + (br - bl);
+    int *Mst = Srow + W;
+    unsigned cnt_cells = 0, cnt_cand = 0;
     for (int c = lane; c < br - bl; c += 64) Est[c] = LCG_NEG;
     // row al: M(al, bl) = 0, M(al, j) = alpha + sum gapB (global) or 0
     const int hi0 = row_hi(P, al, br);
@@ -121,7 +142,7 @@ __device__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W,
         }
         if (lane == 0) Mst[0] = 0;
     }
-    if (GLOBAL) __threadfence_block();
+    region_sync<GLOBAL>();
     int colv = global_init ? alpha : 0;
     bool colopen = true;
     int lo_p = bl, hi_p = hi0;
@@ -138,65 +159,67 @@ __device__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W,
         const int lo = max(bl, mni), hi = min(br - 1, P.mx[i]);
         if (colopen && lane == 0) row[0] = colv;
         const int jlo = max(bl + 1, mni);
-        int Fcarry = LCG_NEG;
-        int sprev = colopen ? colv : LCG_NEG;  // M(i, jlo - 1)
-        const int gA = P.gapA[i];
-        const int *sigrow = P.sig + P.codeA[i] * LCG_NCODES;
-        const int cbase = P.coff[i] - mni;
-        for (int j0 = jlo; j0 <= hi; j0 += 64) {
-            const int j = j0 + lane;
-            const bool act = j <= hi;
-            int s = LCG_NEG, gB = 0;
-            if (act) {
-                const int up = (j >= lo_p && j <= hi_p) ? prow[j - lo_p] : LCG_NEG;
-                const int dg = (j - 1 >= lo_p && j - 1 <= hi_p) ? prow[j - 1 - lo_p] : LCG_NEG;
-                const int e = clampneg(max(Est[j - bl] + gA, up + gA + alpha));
-                Est[j - bl] = e;
-                s = max(dg + sigrow[P.codeB[j]], e);
-                const int cell = cbase + j;
-                const int t1 = P.cptr[cell + 1];
-                ++cnt_cells;
-                for (int t = P.cptr[cell]; t < t1; ++t) {
-                    const int pq = P.cand[2 * t];
-                    const int p = pq & 0xffff;
-                    if (p <= al) break;
-                    const int q = pq >> 16;
-                    if (q <= bl) continue;
+        if (jlo <= hi) {
+            // arc candidates of the row, candidate-parallel
+            for (int c = lane; c <= hi - jlo; c += 64) Srow[c] = LCG_NEG;
+            region_sync<GLOBAL>();
+            const int cb = P.coff[i] - mni;
+            const int t0 = P.cptr[cb + jlo], t1 = P.cptr[cb + hi + 1];
+            for (int t = t0 + lane; t < t1; t += 64) {
+                const int4 e = P.cand[t];
+                const int p = e.x & 0xffff, q = e.x >> 16;
+                if (p > al && q > bl) {
+                    const int v = Mst[(p - 1 - al) * W + (q - 1 - row_lo(P, p - 1, bl))] + e.z;
+                    atomicMax(&Srow[e.y - jlo], v);
                     ++cnt_cand;
-                    const int v = Mst[(p - 1 - al) * W + (q - 1 - row_lo(P, p - 1, bl))] + P.D[P.cand[2 * t + 1]];
-                    s = max(s, v);
-                }
-                s = clampneg(s);
-                if (TOP_LOCAL) s = max(s, 0);
-                gB = P.gapB[j];
-            }
-            // F chain
-            const int sl = wave_shift_up(s, sprev);
-            const int a = act ? gB + P.alpha_pos : 0;
-            const int c = act ? sl + gB + alpha : LCG_NEG;
-            const int Pa = wave_scan_add(a);
-            const int px = wave_scan_max(act ? c - Pa : LCG_NEG);
-            const int F = Pa + max(Fcarry, px);
-            const int Mv = clampneg(max(s, F));
-            if (act) row[j - lo] = Mv;
-            if (TOP_LOCAL) {
-                // first strict maximum in row-major order (aligner.cc:870-875)
-                const int cand_v = act ? Mv : -1;
-                int wmax = cand_v;
-                for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
-                if (wmax > best->score) {
-                    const unsigned long long bal = __ballot(act && Mv == wmax);
-                    best->score = wmax;
-                    best->i = i;
-                    best->j = j0 + (int)__ffsll((long long)bal) - 1;
                 }
             }
-            Fcarry = clampneg(readlane(F, 63));
-            sprev = readlane(s, 63);
+            region_sync<GLOBAL>();
+            int Fcarry = LCG_NEG;
+            int sprev = colopen ? colv : LCG_NEG;  // M(i, jlo - 1)
+            const int gA = P.gapA[i];
+            const int *sigrow = P.sig + P.codeA[i] * LCG_NCODES;
+            for (int j0 = jlo; j0 <= hi; j0 += 64) {
+                const int j = j0 + lane;
+                const bool act = j <= hi;
+                int s = LCG_NEG, gB = 0;
+                if (act) {
+                    const int up = (j >= lo_p && j <= hi_p) ? prow[j - lo_p] : LCG_NEG;
+                    const int dg = (j - 1 >= lo_p && j - 1 <= hi_p) ? prow[j - 1 - lo_p] : LCG_NEG;
+                    const int e = clampneg(max(Est[j - bl] + gA, up + gA + alpha));
+                    Est[j - bl] = e;
+                    s = clampneg(max(max(dg + sigrow[P.codeB[j]], e), Srow[j - jlo]));
+                    if (TOP_LOCAL) s = max(s, 0);
+                    gB = P.gapB[j];
+                    ++cnt_cells;
+                }
+                // F chain
+                const int sl = wave_shift_up(s, sprev);
+                const int a = act ? gB + P.alpha_pos : 0;
+                const int c = act ? sl + gB + alpha : LCG_NEG;
+                const int Pa = wave_scan_add(a);
+                const int px = wave_scan_max(act ? c - Pa : LCG_NEG);
+                const int F = Pa + max(Fcarry, px);
+                const int Mv = clampneg(max(s, F));
+                if (act) row[j - lo] = Mv;
+                if (TOP_LOCAL) {
+                    // first strict maximum in row-major order (aligner.cc:870-875)
+                    int wmax = act ? Mv : -1;
+                    for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
+                    if (wmax > best->score) {
+                        const unsigned long long bal = __ballot(act && Mv == wmax);
+                        best->score = wmax;
+                        best->i = i;
+                        best->j = j0 + (int)__ffsll((long long)bal) - 1;
+                    }
+                }
+                Fcarry = clampneg(readlane(F, 63));
+                sprev = readlane(s, 63);
+            }
         }
         lo_p = lo;
         hi_p = hi;
-        if (GLOBAL) __threadfence_block();
+        region_sync<GLOBAL>();
     }
     if (counts) {
         atomicAdd(&counts[0], (unsigned long long)cnt_cells);
@@ -214,18 +237,17 @@ __device__ __forceinline__ int Mget(const Pair &P, const int *Mst, int W, int al
 
 __device__ __forceinline__ bool valid(const Pair &P, int i, int j) { return P.mn[i] <= j && j <= P.mx[i]; }
 
-__device__ __forceinline__ void emit(const Pair &P, int a, int b, int slot) {
-    P.edges[slot] = (a << 16) | b;
-}
+__device__ __forceinline__ void emit(const Pair &P, int a, int b, int slot) { P.edges[slot] = (a << 16) | b; }
 
 // Traces one group back from (i, j) to its origin (al, bl) (aligner.cc:
-// trace_in_arcmatch / trace_noex, state E_NO_NO).  Arc matches taken are
-// appended to the work list (tstack, counter *tcount in LDS).
+// trace_in_arcmatch / trace_noex, state E_NO_NO).  Mst is the group's
+// matrix.  Arc matches taken are appended to the work list (tstack, counter
+// *tcount in LDS).
 template <bool GLOBAL>
 __device__ void group_trace(const Pair &P, const int *Mst, int W, int al, int bl, int br, int i, int j, bool top,
                             bool local, int *tstack, int *tcount) {
     const int lane = lane_id();
-    if (GLOBAL) __threadfence_block();
+    region_sync<GLOBAL>();
     for (;;) {
         const int Mij = Mget(P, Mst, W, al, bl, br, i, j);
         if (top && local && Mij == 0) break;
@@ -249,7 +271,8 @@ __device__ void group_trace(const Pair &P, const int *Mst, int W, int al, int bl
         }
         bool done = false;
         if (P.alpha != 0) {
-            // deletion of length k = 1, 2, ...: lanes test k = base + lane
+            // deletion of length k = 1, 2, ...: lanes test k = base + lane;
+            // the reference stops at the first k outside the band
             int carry = P.alpha;
             for (int base = 1; !done && i - base >= al; base += 64) {
                 const int k = base + lane;
@@ -258,9 +281,8 @@ __device__ void group_trace(const Pair &P, const int *Mst, int W, int al, int bl
                 const int cost = carry + wave_scan_add(g);
                 const bool hit = inr && Mij == Mget(P, Mst, W, al, bl, br, i - k, j) + cost;
                 const unsigned long long bh = __ballot(hit);
-                const unsigned long long bv = __ballot(inr);
-                // the reference stops at the first invalid k
-                const unsigned long long valid_prefix = ~bv ? ((~bv) & (-(~bv))) - 1 : ~0ull;
+                const unsigned long long nv = ~__ballot(inr);
+                const unsigned long long valid_prefix = nv ? (nv & (0ull - nv)) - 1 : ~0ull;
                 const unsigned long long h = bh & valid_prefix;
                 if (h) {
                     const int kk = base + (int)__ffsll((long long)h) - 1;
@@ -281,8 +303,8 @@ __device__ void group_trace(const Pair &P, const int *Mst, int W, int al, int bl
                 const int cost = carry + wave_scan_add(g);
                 const bool hit = inr && Mij == Mget(P, Mst, W, al, bl, br, i, j - k) + cost;
                 const unsigned long long bh = __ballot(hit);
-                const unsigned long long bv = __ballot(inr);
-                const unsigned long long valid_prefix = ~bv ? ((~bv) & (-(~bv))) - 1 : ~0ull;
+                const unsigned long long nv = ~__ballot(inr);
+                const unsigned long long valid_prefix = nv ? (nv & (0ull - nv)) - 1 : ~0ull;
                 const unsigned long long h = bh & valid_prefix;
                 if (h) {
                     const int kk = base + (int)__ffsll((long long)h) - 1;
@@ -317,11 +339,11 @@ __device__ void group_trace(const Pair &P, const int *Mst, int W, int al, int bl
             bool hit = false;
             int p = 0, q = 0, am = 0;
             if (t < t1) {
-                const int pq = P.cand[2 * t];
-                p = pq & 0xffff;
-                q = pq >> 16;
-                am = P.cand[2 * t + 1];
-                hit = p > al && q > bl && Mij == Mget(P, Mst, W, al, bl, br, p - 1, q - 1) + P.D[am];
+                const int4 e = P.cand[t];
+                p = e.x & 0xffff;
+                q = e.x >> 16;
+                am = e.w;
+                hit = p > al && q > bl && Mij == Mget(P, Mst, W, al, bl, br, p - 1, q - 1) + e.z;
             }
             const unsigned long long bh = __ballot(hit);
             if (bh) {
@@ -358,31 +380,46 @@ __device__ int region_width(const Pair &P, int al, int bl, int ar, int br) {
     return w;
 }
 
+template <bool GLOBAL>
+__device__ void fill_and_trace(const Pair &P, int al, int bl, int ar, int br, int W, int *base, int *tstack,
+                               int *tcount) {
+    group_fill<GLOBAL, false>(P, al, bl, ar, br, W, base, true, nullptr, nullptr);
+    group_trace<GLOBAL>(P, base + (br - bl) + W, W, al, bl, br, ar - 1, br - 1, false, false, tstack, tcount);
+}
+
 }  // namespace
 
-extern "C" __global__ void __launch_bounds__(512) lcg_align_pairs(const PairDesc *__restrict__ descs,
-                                                                  const int *__restrict__ order, int *queue,
-                                                                  int *arena, BatchParams bp) {
+extern "C" __global__ void __launch_bounds__(1024) lcg_align_pairs(const PairDesc *__restrict__ descs,
+                                                                   const int *__restrict__ order, int *queue,
+                                                                   int *arena, int *scratch, BatchParams bp) {
     extern __shared__ int lds[];
     __shared__ int s_pair, s_tcount;
     const int nwaves = blockDim.x >> 6;
     const int wave = threadIdx.x >> 6;
     const int lane = lane_id();
-    int *hdr = lds;
-    int *wave_area = lds + bp.lds_hdr_ints;
+    const int L = bp.max_len;
+    int *mn = lds, *mx = lds + (L + 1), *coff = lds + 2 * (L + 1);
+    int *codeA = lds + 3 * L + 4, *codeB = lds + 4 * L + 5;
+    int *gapA = lds + 5 * L + 6, *gapB = lds + 6 * L + 7;
+    int *sig = lds + 7 * L + 8;
+    int *pool = lds + bp.hdr_ints;
+    int *wave_scratch = scratch + (size_t)(blockIdx.x * nwaves + wave) * bp.scratch_stride;
     for (;;) {
         if (threadIdx.x == 0) s_pair = atomicAdd(queue, 1);
         __syncthreads();
         const int k = s_pair;
         if (k >= bp.npairs) break;
         const PairDesc d = descs[order[k]];
-        Pair P;
         const int n = d.n, m = d.m;
-        int *mn = hdr, *mx = hdr + (bp.max_len + 1), *coff = hdr + 2 * (bp.max_len + 1);
-        int *sig = hdr + 3 * (bp.max_len + 2);
         for (int t = threadIdx.x; t <= n; t += blockDim.x) {
             mn[t] = arena[d.band + t];
             mx[t] = arena[d.band + n + 1 + t];
+            codeA[t] = arena[d.codes + t];
+            gapA[t] = arena[d.gaps + t];
+        }
+        for (int t = threadIdx.x; t <= m; t += blockDim.x) {
+            codeB[t] = arena[d.codes + n + 1 + t];
+            gapB[t] = arena[d.gaps + n + 1 + t];
         }
         for (int t = threadIdx.x; t <= n + 1; t += blockDim.x) coff[t] = arena[d.band + 2 * (n + 1) + t];
         for (int t = threadIdx.x; t < LCG_NCODES * LCG_NCODES; t += blockDim.x) sig[t] = arena[d.sigma + t];
@@ -390,17 +427,19 @@ extern "C" __global__ void __launch_bounds__(512) lcg_align_pairs(const PairDesc
             for (int t = threadIdx.x; t <= n + m; t += blockDim.x) arena[d.edges + t] = -1;
             for (int t = threadIdx.x; t < n + m; t += blockDim.x) arena[d.structs + t] = 0;
         }
+        if (bp.count && threadIdx.x < 4) arena[d.out + 4 + threadIdx.x] = 0;
         if (threadIdx.x == 0) s_tcount = 0;
+        Pair P;
         P.mn = mn;
         P.mx = mx;
         P.coff = coff;
+        P.codeA = codeA;
+        P.codeB = codeB;
+        P.gapA = gapA;
+        P.gapB = gapB;
         P.sig = sig;
-        P.codeA = arena + d.codes;
-        P.codeB = arena + d.codes + n + 1;
-        P.gapA = arena + d.gaps;
-        P.gapB = arena + d.gaps + n + 1;
         P.cptr = arena + d.cptr;
-        P.cand = arena + d.cand;
+        P.cand = reinterpret_cast<int4 *>(arena + d.cand);
         P.amscore = arena + d.amscore;
         P.D = arena + d.D;
         P.edges = arena + d.edges;
@@ -409,53 +448,57 @@ extern "C" __global__ void __launch_bounds__(512) lcg_align_pairs(const PairDesc
         P.m = m;
         P.alpha = bp.alpha;
         P.alpha_pos = bp.alpha > 0 ? bp.alpha : 0;
+        unsigned long long *counts = bp.count ? (unsigned long long *)(arena + d.out + 4) : nullptr;
+        __threadfence_block();
         __syncthreads();
 
-        unsigned long long *counts = bp.count ? (unsigned long long *)(arena + d.out + 4) : nullptr;
-        if (bp.count && threadIdx.x < 4) arena[d.out + 4 + threadIdx.x] = 0;
-        __syncthreads();
-        int *Est = wave_area + wave * bp.lds_wave_ints;
-        int *Mw = Est + bp.ecap;
         const int *rows = arena + d.rows;
+        const int *rounds = arena + d.rounds;
         const int *groups = arena + d.groups;
-        const int *gams = arena + d.gams;
-        // ---- 1. fill D, rows al = n .. 1
+        const int4 *gams = reinterpret_cast<const int4 *>(arena + d.gams);
+        // ---- 1. fill D, rows al = n .. 1, groups packed into LDS rounds
         for (int al = n; al >= 1; --al) {
             const int g0 = rows[al + 1], g1 = rows[al];
             if (g0 == g1) continue;
-            for (int g = g0 + wave; g < g1; g += nwaves) {
-                const int *G = groups + g * LCG_GROUP_INTS;
-                const int gal = G[0] & 0xffff, gbl = G[0] >> 16;
-                const int gar = G[1] & 0xffff, gbr = G[1] >> 16;
-                const int W = G[2];
-                group_fill<false, false>(P, gal, gbl, gar, gbr, W, Mw, Est, true, nullptr, counts);
-                for (int t = G[3] + lane; t < G[4]; t += 64) {
-                    const int rb = gams[2 * t], am = gams[2 * t + 1];
-                    const int ar = rb & 0xffff, br = rb >> 16;
-                    const int v = Mw[(ar - 1 - gal) * W + (br - 1 - row_lo(P, ar - 1, gbl))] + P.amscore[am];
-                    P.D[am] = v;  // not normalized, like the reference's InftyInt sum
+            const int nr = rounds[al];
+            for (int rnd = 0; rnd < nr; ++rnd) {
+                for (int g = g0; g < g1; ++g) {
+                    const int *G = groups + g * LCG_GROUP_INTS;
+                    if (G[5] != (rnd | (wave << 16))) continue;
+                    const int gal = G[0] & 0xffff, gbl = G[0] >> 16;
+                    const int gar = G[1] & 0xffff, gbr = G[1] >> 16;
+                    const int W = G[2];
+                    int *base = pool + G[6];
+                    group_fill<false, false>(P, gal, gbl, gar, gbr, W, base, true, nullptr, counts);
+                    const int *Mst = base + (gbr - gbl) + W;
+                    for (int t = G[3] + lane; t < G[4]; t += 64) {
+                        const int4 ga = gams[t];
+                        const int ar = ga.x & 0xffff, br = ga.x >> 16;
+                        const int v = Mst[(ar - 1 - gal) * W + (br - 1 - row_lo(P, ar - 1, gbl))] + P.amscore[ga.z];
+                        P.cand[ga.y].z = v;  // D, not normalized (the reference's InftyInt sum)
+                        P.D[ga.z] = v;
+                    }
                 }
+                __threadfence_block();
+                __syncthreads();
             }
-            __threadfence_block();
-            __syncthreads();
         }
         // ---- 2. top level (wave 0)
-        const int top_ints = (m + 1) + (n + 1) * d.wtop;
-        const bool top_in_lds = top_ints <= nwaves * bp.lds_wave_ints;
-        int *topE = top_in_lds ? wave_area : arena + d.topM;
-        int *topM = topE + (m + 1);
+        const bool top_in_lds = d.top_ints <= bp.pool_ints;
+        int *topbase = top_in_lds ? pool : wave_scratch;
+        int *topM = topbase + (m + 1) + d.wtop;
         if (wave == 0) {
             TopBest best{0, 0, 0};
             if (bp.local) {
                 if (top_in_lds)
-                    group_fill<false, true>(P, 0, 0, n + 1, m + 1, d.wtop, topM, topE, false, &best, counts);
+                    group_fill<false, true>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, false, &best, counts);
                 else
-                    group_fill<true, true>(P, 0, 0, n + 1, m + 1, d.wtop, topM, topE, false, &best, counts);
+                    group_fill<true, true>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, false, &best, counts);
             } else {
                 if (top_in_lds)
-                    group_fill<false, false>(P, 0, 0, n + 1, m + 1, d.wtop, topM, topE, true, nullptr, counts);
+                    group_fill<false, false>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, true, nullptr, counts);
                 else
-                    group_fill<true, false>(P, 0, 0, n + 1, m + 1, d.wtop, topM, topE, true, nullptr, counts);
+                    group_fill<true, false>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, true, nullptr, counts);
                 best.score = Mget(P, topM, d.wtop, 0, 0, m + 1, n, m);
                 best.i = n;
                 best.j = m;
@@ -479,6 +522,8 @@ extern "C" __global__ void __launch_bounds__(512) lcg_align_pairs(const PairDesc
         __syncthreads();
         // ---- 3b. trace inside the traced arc matches, breadth first
         if (bp.trace) {
+            const int slot = (bp.pool_ints / nwaves) & ~3;
+            int *wave_pool = pool + wave * slot;
             int lo = 0, hi = s_tcount;
             __syncthreads();
             while (lo < hi) {
@@ -487,9 +532,11 @@ extern "C" __global__ void __launch_bounds__(512) lcg_align_pairs(const PairDesc
                     const int al = ts[0] & 0xffff, bl = ts[0] >> 16;
                     const int ar = ts[1] & 0xffff, br = ts[1] >> 16;
                     const int W = region_width(P, al, bl, ar, br);
-                    group_fill<false, false>(P, al, bl, ar, br, W, Mw, Est, true, nullptr);
-                    group_trace<false>(P, Mw, W, al, bl, br, ar - 1, br - 1, false, false, arena + d.tstack,
-                                       &s_tcount);
+                    const int need = (br - bl) + W + (ar - al) * W;
+                    if (need <= slot)
+                        fill_and_trace<false>(P, al, bl, ar, br, W, wave_pool, arena + d.tstack, &s_tcount);
+                    else
+                        fill_and_trace<true>(P, al, bl, ar, br, W, wave_scratch, arena + d.tstack, &s_tcount);
                 }
                 __threadfence_block();
                 __syncthreads();
@@ -506,8 +553,8 @@ extern "C" __global__ void lcg_reset_queue(int *queue) {
     if (threadIdx.x == 0) *queue = 0;
 }
 
-extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, BatchParams bp,
-                                int grid, int waves, size_t lds_bytes, hipStream_t stream) {
+extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, int *scratch,
+                                BatchParams bp, int grid, int waves, size_t lds_bytes, hipStream_t stream) {
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void *)lcg_align_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -517,6 +564,6 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
     }
     hipLaunchKernelGGL(lcg_reset_queue, dim3(1), dim3(64), 0, stream, queue);
     hipLaunchKernelGGL(lcg_align_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue, arena,
-                       bp);
+                       scratch, bp);
     return (int)hipGetLastError();
 }

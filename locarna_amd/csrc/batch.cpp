@@ -16,8 +16,8 @@
 
 using namespace locarna_amd;
 
-extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, BatchParams bp,
-                                int grid, int waves, size_t lds_bytes, hipStream_t stream);
+extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, int *scratch,
+                                BatchParams bp, int grid, int waves, size_t lds_bytes, hipStream_t stream);
 
 struct lcg_pair {
     lcg_params params;
@@ -96,7 +96,7 @@ void pack(lcg_pair &P) {
     }
     // groups, al descending, bl descending (aligner.cc:659-732)
     struct G {
-        int al, bl, ar, br, W, b, e;
+        int al, bl, ar, br, W, b, e, lds;
     };
     std::vector<G> groups;
     std::vector<int32_t> gams;
@@ -127,16 +127,20 @@ void pack(lcg_pair &P) {
                 W = std::max(W, w);
                 if (i > al) cells += std::max(0, std::min(br - 1, tc.max_col(i)) - std::max(bl + 1, tc.min_col(i)) + 1);
             }
-            G g{al, bl, ar, br, W, (int)gams.size() / 2, 0};
+            // LDS region E | S | M, rounded to 4 ints
+            const int lds = ((br - bl) + W + (ar - al) * W + 3) & ~3;
+            G g{al, bl, ar, br, W, (int)gams.size() / 4, 0, lds};
             for (int k : lst) {
                 const ArcMatch &x = ams.arcmatch(k);
                 gams.push_back(bA.arc(x.arcA).right | (bB.arc(x.arcB).right << 16));
+                gams.push_back(0);  // candidate entry, set below
                 gams.push_back(k);
+                gams.push_back(0);
             }
-            g.e = (int)gams.size() / 2;
+            g.e = (int)gams.size() / 4;
             groups.push_back(g);
             ecap = std::max(ecap, br - bl);
-            gbuf = std::max(gbuf, (ar - al) * W);
+            gbuf = std::max(gbuf, lds);
         }
         rows[al] = (int)groups.size();
     }
@@ -144,6 +148,7 @@ void pack(lcg_pair &P) {
     d.ngroups = (int)groups.size();
     d.rows = reserve(blob, rows.size());
     std::copy(rows.begin(), rows.end(), blob.begin() + d.rows);
+    d.rounds = reserve(blob, rows.size());  // filled by the batch's LDS schedule
     d.groups = reserve(blob, (size_t)LCG_GROUP_INTS * groups.size());
     for (size_t g = 0; g < groups.size(); ++g) {
         int32_t *G8 = &blob[d.groups + LCG_GROUP_INTS * g];
@@ -152,14 +157,18 @@ void pack(lcg_pair &P) {
         G8[2] = groups[g].W;
         G8[3] = groups[g].b;
         G8[4] = groups[g].e;
+        G8[7] = groups[g].lds;
     }
+    while (blob.size() & 3) blob.push_back(0);
     d.gams = reserve(blob, gams.size());
-    std::copy(gams.begin(), gams.end(), blob.begin() + d.gams);
     d.amscore = reserve(blob, nam);
     for (int k = 0; k < nam; ++k) blob[d.amscore + k] = (int32_t)sc.arcmatch(k);
-    // candidate CSR over band cells, each list sorted (al desc, bl desc)
+    // candidate CSR over band cells, each list sorted (al desc, bl desc);
+    // entries (al | bl<<16, j, D, am) are 16-byte aligned
     d.cptr = reserve(blob, (size_t)ncells + 1);
-    d.cand = reserve(blob, 2 * (size_t)nam);
+    while (blob.size() & 3) blob.push_back(0);
+    d.cand = reserve(blob, 4 * (size_t)nam);
+    std::vector<int> entry_of(nam, -1);
     int t = 0, next_cell = 0;
     for (int i = 0; i <= n; ++i) {
         for (int j = tc.min_col(i); j <= tc.max_col(i); ++j) {
@@ -175,8 +184,11 @@ void pack(lcg_pair &P) {
             });
             for (int k : lst) {
                 const ArcMatch &x = ams.arcmatch(k);
-                blob[d.cand + 2 * t] = bA.arc(x.arcA).left | (bB.arc(x.arcB).left << 16);
-                blob[d.cand + 2 * t + 1] = k;
+                blob[d.cand + 4 * t] = bA.arc(x.arcA).left | (bB.arc(x.arcB).left << 16);
+                blob[d.cand + 4 * t + 1] = j;
+                blob[d.cand + 4 * t + 2] = LCG_NEG;
+                blob[d.cand + 4 * t + 3] = k;
+                entry_of[k] = t;
                 ++t;
             }
         }
@@ -184,6 +196,8 @@ void pack(lcg_pair &P) {
     if (next_cell != ncells) throw failure("internal: band cell count");
     blob[d.cptr + ncells] = t;
     if (t != nam) throw failure("internal: arc match right-end lists incomplete");
+    for (size_t q = 0; q < gams.size(); q += 4) gams[q + 1] = entry_of[gams[q + 2]];
+    std::copy(gams.begin(), gams.end(), blob.begin() + d.gams);
     d.D = reserve(blob, nam);
     d.edges = reserve(blob, (size_t)n + m + 1, -1);
     d.structs = reserve(blob, (size_t)n + m);
@@ -194,13 +208,13 @@ void pack(lcg_pair &P) {
         if (i > 0) cells += std::max(0, std::min(m, tc.max_col(i)) - std::max(1, tc.min_col(i)) + 1);
     }
     d.wtop = wtop;
-    d.topM = reserve(blob, (size_t)(m + 1) + (size_t)(n + 1) * wtop);
+    d.top_ints = ((m + 1) + wtop + (n + 1) * wtop + 3) & ~3;
     // out[4..7] holds two uint64 counters: keep them 8-byte aligned (every
     // blob is a multiple of 4 ints, so blob offsets survive concatenation)
     if (blob.size() & 1) blob.push_back(0);
     d.out = reserve(blob, 8);
     while (blob.size() & 3) blob.push_back(0);
-    d.gbuf = gbuf;
+    d.gmax = gbuf;
     P.desc = d;
     P.cells = cells;
     P.ecap = ecap;
@@ -297,6 +311,7 @@ struct lcg_batch {
     PairDesc *d_descs = nullptr;
     int32_t *d_order = nullptr;
     int32_t *d_queue = nullptr;
+    int32_t *d_scratch = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     hipGraph_t graph = nullptr;
@@ -331,7 +346,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     B->device = device;
     B->npairs = npairs;
     const lcg_params &p0 = pairs[0]->params;
-    int max_len = 0, ecap = 1, gbuf = 1;
+    int max_len = 1, gmax = 4, top_max = 4;
     size_t total = 0;
     for (int k = 0; k < npairs; ++k) {
         const lcg_pair *P = pairs[k];
@@ -339,9 +354,9 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
             set_err(err, errlen, "all pairs of a batch must share indel_opening and sequ_local");
             return LCG_E_ARG;
         }
-        max_len = std::max(max_len, P->desc.n);
-        ecap = std::max(ecap, P->ecap);
-        gbuf = std::max(gbuf, P->desc.gbuf);
+        max_len = std::max(max_len, std::max(P->desc.n, P->desc.m));
+        gmax = std::max(gmax, P->desc.gmax);
+        top_max = std::max(top_max, P->desc.top_ints);
         total += P->blob.size();
     }
     if (total >= (size_t)0x7fffffff) {
@@ -355,7 +370,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
         B->host_arena.insert(B->host_arena.end(), P->blob.begin(), P->blob.end());
         PairDesc d = P->desc;
         int32_t *offs[] = {&d.band, &d.codes, &d.gaps, &d.sigma, &d.rows, &d.groups, &d.gams, &d.amscore,
-                           &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.topM, &d.out};
+                           &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.out, &d.rounds};
         for (int32_t *o : offs) *o += base;
         B->descs.push_back(d);
     }
@@ -365,35 +380,73 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     std::vector<int64_t> work(npairs);
     for (int k = 0; k < npairs; ++k) work[k] = pairs[k]->cells;
     std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) { return work[a] > work[b]; });
-    // LDS: header (band arrays of the longest sequence) + per-wave E and M
-    const int hdr = 3 * (max_len + 2) + LCG_NCODES * LCG_NCODES + 8;
-    const int ecap4 = (ecap + 3) & ~3;
-    const int wave_ints = ecap4 + ((gbuf + 3) & ~3);
-    const int lds_limit = (160 * 1024 - 256) / 4;  // 256 bytes for the kernel's static LDS
-    int waves = std::min(8, (lds_limit - hdr) / wave_ints);
-    if (const char *w = std::getenv("LCG_WAVES")) waves = std::min(waves, std::max(1, std::atoi(w)));
-    if (waves < 1) {
-        set_err(err, errlen, "arc-match group matrix exceeds the LDS of one workgroup");
+
+    // Launch shape: LDS = header (per-pair band / codes / gaps / sigma of the
+    // longest sequence) + pool for the group regions of one row.  Prefer two
+    // workgroups of 8 waves per CU; fall back to one workgroup when the
+    // largest group region does not fit half the LDS.
+    const int hdr = (7 * max_len + 72 + 3) & ~3;
+    const int lds_ints_cu = (160 * 1024 - 512) / 4;  // static LDS of the kernel included
+    int waves = 8, wgs_per_cu = 2;
+    if (const char *w = std::getenv("LCG_WAVES")) waves = std::min(16, std::max(1, std::atoi(w)));
+    if (const char *w = std::getenv("LCG_WGS_PER_CU")) wgs_per_cu = std::min(4, std::max(1, std::atoi(w)));
+    int pool = lds_ints_cu / wgs_per_cu - hdr;
+    while (pool < gmax && wgs_per_cu > 1) {
+        --wgs_per_cu;
+        pool = lds_ints_cu / wgs_per_cu - hdr;
+    }
+    pool &= ~3;
+    if (pool < gmax) {
+        set_err(err, errlen, "an arc-match group region exceeds the LDS of one workgroup");
         return LCG_E_CAPACITY;
     }
+    // per-row LDS schedule: groups of a row, largest first, into rounds of at
+    // most `waves` regions that fit the pool together
+    for (int k = 0; k < npairs; ++k) {
+        const PairDesc &d = B->descs[k];
+        int32_t *A = B->host_arena.data();
+        for (int al = d.n; al >= 1; --al) {
+            const int g0 = A[d.rows + al + 1], g1 = A[d.rows + al];
+            std::vector<int> idx;
+            for (int g = g0; g < g1; ++g) idx.push_back(g);
+            std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+                return A[d.groups + x * LCG_GROUP_INTS + 7] > A[d.groups + y * LCG_GROUP_INTS + 7];
+            });
+            std::vector<int> used, cnt;
+            for (int g : idx) {
+                int32_t *G = A + d.groups + g * LCG_GROUP_INTS;
+                size_t r = 0;
+                while (r < used.size() && (cnt[r] >= waves || used[r] + G[7] > pool)) ++r;
+                if (r == used.size()) {
+                    used.push_back(0);
+                    cnt.push_back(0);
+                }
+                G[5] = (int)r | (cnt[r] << 16);
+                G[6] = used[r];
+                used[r] += G[7];
+                cnt[r] += 1;
+            }
+            A[d.rounds + al] = (int)used.size();
+        }
+    }
     B->waves = waves;
-    B->lds_bytes = (size_t)(hdr + waves * wave_ints) * 4;
+    B->lds_bytes = (size_t)(hdr + pool) * 4;
     B->bp.alpha = p0.indel_opening;
     B->bp.local = p0.sequ_local;
     B->bp.trace = 1;
     B->bp.npairs = npairs;
-    B->bp.lds_wave_ints = wave_ints;
-    B->bp.lds_hdr_ints = hdr;
+    B->bp.hdr_ints = hdr;
+    B->bp.pool_ints = pool;
     B->bp.max_len = max_len;
-    B->bp.ecap = ecap4;
+    B->bp.count = 0;
+    B->bp.scratch_stride = top_max;
 
     hipError_t e;
     if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, err, errlen);
     int cus = 0;
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
         return hip_fail(e, err, errlen);
-    const int per_cu = std::max(1, (int)((160 * 1024) / B->lds_bytes));
-    B->grid = std::min(npairs, cus * per_cu);
+    B->grid = std::min(npairs, cus * wgs_per_cu);
     if ((e = hipStreamCreate(&B->stream)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipEventCreate(&B->ev0)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipEventCreate(&B->ev1)) != hipSuccess) return hip_fail(e, err, errlen);
@@ -401,6 +454,8 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     if ((e = hipMalloc(&B->d_descs, sizeof(PairDesc) * npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_order, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_queue, 64)) != hipSuccess) return hip_fail(e, err, errlen);
+    const size_t scratch_ints = (size_t)B->grid * waves * top_max;
+    if ((e = hipMalloc(&B->d_scratch, scratch_ints * 4)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMemcpy(B->d_arena, B->host_arena.data(), B->host_arena.size() * 4, hipMemcpyHostToDevice)) !=
         hipSuccess)
         return hip_fail(e, err, errlen);
@@ -421,6 +476,7 @@ extern "C" void lcg_batch_destroy(lcg_batch *B) {
     if (B->d_descs) hipFree(B->d_descs);
     if (B->d_order) hipFree(B->d_order);
     if (B->d_queue) hipFree(B->d_queue);
+    if (B->d_scratch) hipFree(B->d_scratch);
     if (B->ev0) hipEventDestroy(B->ev0);
     if (B->ev1) hipEventDestroy(B->ev1);
     if (B->stream) hipStreamDestroy(B->stream);
@@ -444,14 +500,14 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
     if (mode == 2 && bp.count) return LCG_E_ARG;
     HIPCHECK(hipEventRecord(B->ev0, B->stream));
     if (mode == 0) {
-        int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, bp, B->grid, B->waves,
+        int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, B->d_scratch, bp, B->grid, B->waves,
                                   B->lds_bytes, B->stream);
         if (rc) return hip_fail((hipError_t)rc);
     } else if (mode == 1) {
         BatchParams one = bp;
         one.npairs = 1;
         for (int k = 0; k < B->npairs; ++k) {
-            int rc = lcg_launch_align(B->d_descs, B->d_order + k, B->d_queue, B->d_arena, one, 1, B->waves,
+            int rc = lcg_launch_align(B->d_descs, B->d_order + k, B->d_queue, B->d_arena, B->d_scratch, one, 1, B->waves,
                                       B->lds_bytes, B->stream);
             if (rc) return hip_fail((hipError_t)rc);
         }
@@ -464,7 +520,7 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
             hipStream_t cs;
             HIPCHECK(hipStreamCreate(&cs));
             HIPCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, bp, B->grid, B->waves,
+            int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, B->d_scratch, bp, B->grid, B->waves,
                                       B->lds_bytes, cs);
             hipGraph_t g = nullptr;
             hipError_t ec = hipStreamEndCapture(cs, &g);
