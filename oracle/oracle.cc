@@ -809,6 +809,263 @@ struct Aligner {
     }
 };
 
+
+// ---------------------------------------------------------------- AlignerP
+// Inside: aligner_p.icc:148-438 (init_M, init_E, comp_E/F/M_entry,
+// align_inside_arcmatch, fill_D, align_D, align_inside), pf_score_t = double.
+// Outside quantities: reverse-mode adjoint of the inside recursion, group by
+// group in reverse order; Dbar(am) = dZ/dD(am) gives P(am) = D Dbar / Z, the
+// quantity aligner_p.icc:1150-1195 computes as D D' pf_scale / (Z e^{s/T}).
+// Base-match probabilities follow aligner_p.icc:1201-1399: match edges
+// directly inside arc matches with P(am) > sqrt(min_am_prob), plus the top
+// level, optionally plus P(am) at both arc ends.
+struct AlignerP {
+    Inputs &in;
+    const Band &tc;
+    const ArcMatches &ams;
+    const Scoring &sc;
+    const BasePairs &bpsA, &bpsB;
+    int lenA, lenB;
+    double T, ps;
+    Mat<double> esig;
+    std::vector<double> egA, egB, eam;
+    double eo;
+    Mat<double> M;
+    std::vector<double> E;
+    double F = 0;
+    Mat<double> Dmat;  // N_A x N_B, 0 where invalid (aligner_p.icc:25)
+    double Z = 0;
+
+    explicit AlignerP(Inputs &in_)
+        : in(in_), tc(*in_.band), ams(*in_.am), sc(*in_.sc), bpsA(*in_.bpsA), bpsB(*in_.bpsB),
+          lenA(in_.A.length()), lenB(in_.B.length()), T((double)in_.p.temperature_alipf), ps(in_.p.pf_scale) {
+        // scoring.hh:882-931: boltzmann_weight(s) = exp(s / (pf_score_t)T)
+        esig.resize(lenA + 1, lenB + 1, 0.0);
+        for (int i = 1; i <= lenA; ++i)
+            for (int j = 1; j <= lenB; ++j) esig(i, j) = exp(sc.sigma_tab(i, j) / T);
+        egA.assign(lenA + 1, 0.0);
+        egB.assign(lenB + 1, 0.0);
+        for (int i = 1; i <= lenA; ++i) egA[i] = exp(sc.gapA_tab[i] / T);
+        for (int j = 1; j <= lenB; ++j) egB[j] = exp(sc.gapB_tab[j] / T);
+        eo = exp(in.p.indel_opening / T);
+        eam.resize(ams.ams.size());
+        for (size_t k = 0; k < ams.ams.size(); ++k) eam[k] = exp(sc.arcmatch(ams.ams[k]) / T);
+        M.resize(lenA + 1, lenB + 1, 0.0);
+        E.assign(lenB + 1, 0.0);
+        Dmat.resize(bpsA.arcs.size(), bpsB.arcs.size(), 0.0);
+    }
+    double D(const Arc &a, const Arc &b) const { return Dmat(a.idx, b.idx); }
+
+    void init_M(size_t al, size_t ar, size_t bl, size_t br) {  // aligner_p.icc:150-189
+        M(al, bl) = 1.0 / ps;
+        double indel_score = eo / ps;
+        size_t i;
+        for (i = al + 1; i < ar; i++) {
+            if (tc.min_col[i] > bl) break;
+            indel_score *= egA[i];
+            M(i, bl) = indel_score;
+        }
+        for (; i < ar; i++) M(i, tc.min_col[i] - 1) = 0;
+        size_t max_col = std::min(br - 1, tc.max_col[al]);
+        indel_score = eo / ps;
+        size_t j;
+        for (j = bl + 1; j <= max_col; j++) {
+            indel_score *= egB[j];
+            M(al, j) = indel_score;
+        }
+        for (size_t i2 = al + 1; i2 < ar; i2++)
+            for (; j < std::min(br, tc.max_col[i2] + 1); ++j) M(i2 - 1, j) = 0;
+    }
+    double comp_M_entry(size_t al, size_t bl, size_t i, size_t j) {  // aligner_p.icc:234-278
+        double pf = M(i - 1, j - 1) * esig(i, j) + E[j] + F;
+        const auto &adjlA = bpsA.right_adj[i];
+        const auto &adjlB = bpsB.right_adj[j];
+        for (auto a = adjlA.begin(); (size_t)a->left > al; ++a)
+            for (auto b = adjlB.begin(); (size_t)b->left > bl; ++b)
+                pf += M(a->left - 1, b->left - 1) * D(*a, *b) * ps;
+        return pf;
+    }
+    void align_inside_arcmatch(size_t al, size_t ar, size_t bl, size_t br) {  // aligner_p.icc:285-312
+        init_M(al, ar, bl, br);
+        for (size_t j = bl; j < br; j++) E[j] = 0;
+        for (size_t i = al + 1; i < ar; i++) {
+            F = 0;
+            size_t min_col = std::max(bl + 1, tc.min_col[i]);
+            size_t max_col = std::min(br - 1, tc.max_col[i]);
+            for (size_t j = min_col; j <= max_col; j++) {
+                E[j] = E[j] * egA[i] + (M(i - 1, j) - E[j]) * egA[i] * eo;
+                F = F * egB[j] + (M(i, j - 1) - F) * egB[j] * eo;
+                M(i, j) = comp_M_entry(al, bl, i, j);
+            }
+        }
+    }
+    void max_right_ends(int al, int bl, int *ar, int *br) const { ams.max_right_ends(al, bl, ar, br); }
+    double inside() {  // aligner_p.icc:358-438
+        for (size_t al = lenA; al >= 1; al--) {
+            size_t min_bl = std::max((size_t)1, tc.min_col[al]);
+            size_t max_bl = std::min((size_t)lenB, tc.max_col[al]);
+            for (size_t bl = max_bl; bl >= min_bl; bl--) {
+                int max_ar = (int)al, max_br = (int)bl;
+                max_right_ends((int)al, (int)bl, &max_ar, &max_br);
+                align_inside_arcmatch(al, max_ar, bl, max_br);
+                for (int x : ams.left_lists(al, bl)) {
+                    const ArcMatch &am = ams.ams[x];
+                    Dmat(am.a.idx, am.b.idx) = M(am.a.right - 1, am.b.right - 1) * eam[x];
+                }
+            }
+        }
+        align_inside_arcmatch(0, lenA + 1, 0, lenB + 1);
+        Z = M(lenA, lenB);
+        return Z;
+    }
+
+    // ---- adjoint of align_inside_arcmatch for one group.  Recomputes the
+    // group's forward values, then sweeps cells in reverse order.  Seeds:
+    // Mbar at cells.  Accumulates into Dbar (if given), match-edge weights
+    // (edge(i,j) += M(i-1,j-1) esig Mbar(i,j)) and gap-opening weight.
+    struct Fwd {
+        size_t al, ar, bl, br;
+        Mat<double> Mv, Ev, Fv;  // values after the cell update
+    };
+    void forward_group(size_t al, size_t ar, size_t bl, size_t br, Fwd &f) {
+        f.al = al; f.ar = ar; f.bl = bl; f.br = br;
+        align_inside_arcmatch(al, ar, bl, br);
+        // re-run capturing E, F per cell (values identical to the fill above)
+        f.Mv.resize(lenA + 1, lenB + 1, 0.0);
+        f.Ev.resize(lenA + 1, lenB + 1, 0.0);
+        f.Fv.resize(lenA + 1, lenB + 1, 0.0);
+        for (size_t i = al; i < ar; ++i)
+            for (size_t j = bl; j < br; ++j) f.Mv(i, j) = M(i, j);
+        std::vector<double> e(lenB + 1, 0.0);
+        for (size_t i = al + 1; i < ar; i++) {
+            double fv = 0;
+            size_t min_col = std::max(bl + 1, tc.min_col[i]);
+            size_t max_col = std::min(br - 1, tc.max_col[i]);
+            for (size_t j = min_col; j <= max_col; j++) {
+                e[j] = e[j] * egA[i] + (M(i - 1, j) - e[j]) * egA[i] * eo;
+                fv = fv * egB[j] + (M(i, j - 1) - fv) * egB[j] * eo;
+                f.Ev(i, j) = e[j];
+                f.Fv(i, j) = fv;
+            }
+        }
+    }
+    // Mbar: seeds on entry (indexed like M), consumed
+    void reverse_group(const Fwd &f, Mat<double> &Mbar, std::vector<double> *Dbar, Mat<double> *edge,
+                       double *openings) {
+        const size_t al = f.al, ar = f.ar, bl = f.bl, br = f.br;
+        std::vector<double> Ebar(lenB + 1, 0.0);
+        for (size_t i = ar - 1; i > al; --i) {
+            double Fbar = 0;
+            size_t min_col = std::max(bl + 1, tc.min_col[i]);
+            size_t max_col = std::min(br - 1, tc.max_col[i]);
+            if (min_col > max_col) {
+                for (size_t j = bl; j < br; ++j) Ebar[j] = 0;  // E of empty row: chain broken? keep semantics
+                continue;
+            }
+            for (size_t j = max_col; j >= min_col; --j) {
+                const double g = Mbar(i, j);
+                // M(i,j) = M(i-1,j-1) esig + E + F + arcs
+                Mbar(i - 1, j - 1) += g * esig(i, j);
+                if (edge) (*edge)(i, j) += g * f.Mv(i - 1, j - 1) * esig(i, j);
+                const double Eb = Ebar[j] + g;
+                const double Fb = Fbar + g;
+                const auto &adjlA = bpsA.right_adj[i];
+                const auto &adjlB = bpsB.right_adj[j];
+                for (auto a = adjlA.begin(); (size_t)a->left > al; ++a)
+                    for (auto b = adjlB.begin(); (size_t)b->left > bl; ++b) {
+                        const double dv = D(*a, *b);
+                        if (dv == 0) continue;
+                        Mbar(a->left - 1, b->left - 1) += g * dv * ps;
+                        if (Dbar) {
+                            int k = am_index(a->idx, b->idx);
+                            if (k >= 0) (*Dbar)[k] += g * f.Mv(a->left - 1, b->left - 1) * ps;
+                        }
+                    }
+                // F(i,j) = Fprev gB (1-o) + M(i,j-1) gB o
+                const double Mleft = f.Mv(i, j - 1);
+                const double Fprev = (j > min_col) ? f.Fv(i, j - 1) : 0.0;
+                Mbar(i, j - 1) += Fb * egB[j] * eo;
+                if (openings) *openings += Fb * (Mleft - Fprev) * egB[j] * eo;
+                Fbar = Fb * egB[j] * (1 - eo);
+                // E(i,j) = Eprev gA (1-o) + M(i-1,j) gA o
+                const double Eprev = (i - 1 > al && f.Ev(i - 1, j) != 0) ? f.Ev(i - 1, j) : eprev_at(f, i, j);
+                const double Mup = f.Mv(i - 1, j);
+                Mbar(i - 1, j) += Eb * egA[i] * eo;
+                if (openings) *openings += Eb * (Mup - Eprev) * egA[i] * eo;
+                Ebar[j] = Eb * egA[i] * (1 - eo);
+                if (j == min_col) break;
+            }
+            // columns of row i not computed: their E chain ends here
+            for (size_t j = bl; j < br; ++j)
+                if (j < min_col || j > max_col) Ebar[j] = 0;
+        }
+        // init row al and column bl carry one opening each (init_M)
+        if (openings) {
+            for (size_t j = bl + 1; j < br && j <= tc.max_col[al]; ++j) *openings += Mbar(al, j) * f.Mv(al, j);
+            for (size_t i = al + 1; i < ar && tc.min_col[i] <= bl; ++i) *openings += Mbar(i, bl) * f.Mv(i, bl);
+        }
+    }
+    // E value entering row i at column j (E of the last row above that
+    // computed column j, 0 if none)
+    double eprev_at(const Fwd &f, size_t i, size_t j) const {
+        return (i - 1 > f.al) ? f.Ev(i - 1, j) : 0.0;
+    }
+    std::vector<int> am_of;  // arcA idx * N_B + arcB idx -> am index
+    int am_index(int a, int b) {
+        if (am_of.empty()) {
+            am_of.assign(bpsA.arcs.size() * bpsB.arcs.size(), -1);
+            for (size_t k = 0; k < ams.ams.size(); ++k) am_of[ams.ams[k].a.idx * bpsB.arcs.size() + ams.ams[k].b.idx] = (int)k;
+        }
+        return am_of[(size_t)a * bpsB.arcs.size() + b];
+    }
+
+    // Outside pass; returns Dbar (per am), full match-edge weights, restricted
+    // match-edge weights (reference base-match semantics), openings weight.
+    void outside(std::vector<double> &Dbar, Mat<double> &edge_full, Mat<double> &edge_ref, double &openings) {
+        const size_t nA = ams.ams.size();
+        Dbar.assign(nA, 0.0);
+        edge_full.resize(lenA + 1, lenB + 1, 0.0);
+        edge_ref.resize(lenA + 1, lenB + 1, 0.0);
+        openings = 0;
+        Mat<double> Mbar;
+        Fwd f;
+        // top level: Z = M(n, m)
+        forward_group(0, lenA + 1, 0, lenB + 1, f);
+        Mbar.resize(lenA + 1, lenB + 1, 0.0);
+        Mbar(lenA, lenB) = 1.0;
+        Mat<double> Mbar2 = Mbar;
+        reverse_group(f, Mbar, &Dbar, &edge_full, &openings);
+        reverse_group(f, Mbar2, nullptr, &edge_ref, nullptr);
+        const double thr = sqrt(in.p.min_am_prob);
+        for (size_t al = 1; al <= (size_t)lenA; ++al) {
+            size_t min_bl = std::max((size_t)1, tc.min_col[al]);
+            size_t max_bl = std::min((size_t)lenB, tc.max_col[al]);
+            for (size_t bl = min_bl; bl <= max_bl; ++bl) {
+                const auto &lst = ams.left_lists(al, bl);
+                if (lst.empty()) continue;
+                int max_ar = (int)al, max_br = (int)bl;
+                max_right_ends((int)al, (int)bl, &max_ar, &max_br);
+                forward_group(al, max_ar, bl, max_br, f);
+                Mbar.resize(lenA + 1, lenB + 1, 0.0);
+                Mbar2.resize(lenA + 1, lenB + 1, 0.0);
+                bool any_sig = false;
+                for (int x : lst) {
+                    const ArcMatch &am = ams.ams[x];
+                    const double seed = Dbar[x] * eam[x];
+                    Mbar(am.a.right - 1, am.b.right - 1) += seed;
+                    const double pam = Dmat(am.a.idx, am.b.idx) * Dbar[x] / Z;
+                    if (pam > thr) {
+                        Mbar2(am.a.right - 1, am.b.right - 1) += seed;
+                        any_sig = true;
+                    }
+                }
+                reverse_group(f, Mbar, &Dbar, &edge_full, &openings);
+                if (any_sig) reverse_group(f, Mbar2, nullptr, &edge_ref, nullptr);
+            }
+        }
+    }
+};
+
 }  // namespace orc
 
 // ================================================================ C API
@@ -819,7 +1076,8 @@ struct oracle_result {
     std::vector<std::pair<int, int>> edges;
     std::string sA, sB;
     std::vector<long long> D;
-    std::vector<double> am_prob, bm_prob, pfD, pfDprime;
+    std::vector<double> am_prob, bm_prob, bm_full, pfD, pfDprime;
+    double openings = 0;
     long long counters[6] = {0, 0, 0, 0, 0, 0};
 };
 
@@ -961,8 +1219,57 @@ extern "C" int oracle_align_many(const char **ppA, const char **ppB, int npairs,
     return failed ? -1 : npairs;
 }
 
-extern "C" oracle_result *oracle_pf(const char *, const char *, const oracle_params *, const char *, int, char *err,
-                                    int errlen) {
-    set_err(err, errlen, "not implemented");
-    return nullptr;
+extern "C" oracle_result *oracle_pf(const char *ppA, const char *ppB, const oracle_params *p,
+                                    const char *ribosum_path, int include_am_in_bm, char *err, int errlen) {
+    try {
+        auto *r = new oracle_result();
+        r->in = orc::prepare(ppA, ppB, *p, ribosum_path);
+        orc::AlignerP ap(*r->in);
+        r->pf = ap.inside();
+        r->counters[0] = (long long)r->in->bpsA->arcs.size();
+        r->counters[1] = (long long)r->in->bpsB->arcs.size();
+        r->counters[2] = (long long)r->in->am->ams.size();
+        std::vector<double> Dbar;
+        orc::Mat<double> edge_full, edge_ref;
+        double openings = 0;
+        ap.outside(Dbar, edge_full, edge_ref, openings);
+        const auto &ams = r->in->am->ams;
+        const int la = r->in->A.length(), lb = r->in->B.length();
+        r->am_prob.resize(ams.size());
+        r->pfD.resize(ams.size());
+        r->pfDprime.resize(ams.size());
+        for (size_t k = 0; k < ams.size(); ++k) {
+            const double D = ap.Dmat(ams[k].a.idx, ams[k].b.idx);
+            r->pfD[k] = D;
+            r->pfDprime[k] = Dbar[k] * ap.eam[k] / ap.ps;  // the reference's D'
+            r->am_prob[k] = D * Dbar[k] / ap.Z;
+        }
+        r->bm_prob.assign((size_t)(la + 1) * (lb + 1), 0.0);
+        r->bm_full.assign((size_t)(la + 1) * (lb + 1), 0.0);
+        const auto &tc = *r->in->band;
+        for (int i = 1; i <= la; ++i)
+            for (int j = 1; j <= lb; ++j) {
+                r->bm_full[(size_t)i * (lb + 1) + j] = edge_full(i, j) / ap.Z;
+                if (tc.is_valid(i, j) && tc.is_valid_match(i, j))
+                    r->bm_prob[(size_t)i * (lb + 1) + j] = edge_ref(i, j) / ap.Z;
+            }
+        if (include_am_in_bm) {
+            for (size_t k = 0; k < ams.size(); ++k) {
+                r->bm_prob[(size_t)ams[k].a.left * (lb + 1) + ams[k].b.left] += r->am_prob[k];
+                r->bm_prob[(size_t)ams[k].a.right * (lb + 1) + ams[k].b.right] += r->am_prob[k];
+            }
+        }
+        r->openings = openings / ap.Z;
+        return r;
+    } catch (std::exception &e) {
+        set_err(err, errlen, e.what());
+        return nullptr;
+    }
+}
+
+extern "C" int oracle_pf_extra(const oracle_result *r, double *bm_full, int cap, double *openings) {
+    int n = (int)r->bm_full.size();
+    for (int k = 0; k < n && k < cap; ++k) bm_full[k] = r->bm_full[k];
+    if (openings) *openings = r->openings;
+    return n;
 }

@@ -87,6 +87,10 @@ int oracle_bm_probs(const oracle_result *r, double *prob, int cap);
 /* arc match inside D (pf) and outside D' in arcmatch order */
 int oracle_pf_D(const oracle_result *r, double *D, double *Dprime, int cap);
 
+/* pf extras: unrestricted base-match (match edge, not arc end) marginals,
+ * dense (lenA+1)*(lenB+1), and the expected number of gap openings */
+int oracle_pf_extra(const oracle_result *r, double *bm_full, int cap, double *openings);
+
 /* CPU baseline: align many pairs with nthreads worker threads (scores
  * only, with traceback).  Returns number of pairs done or <0. */
 int oracle_align_many(const char **ppA, const char **ppB, int npairs,

@@ -72,7 +72,7 @@ def load():
         L.oracle_partition_function.argtypes = [ctypes.c_void_p]
         for f in ("oracle_lengths", "oracle_edges", "oracle_structures", "oracle_band",
                   "oracle_arcmatches", "oracle_counters", "oracle_am_probs",
-                  "oracle_bm_probs", "oracle_pf_D", "oracle_tables"):
+                  "oracle_bm_probs", "oracle_pf_D", "oracle_tables", "oracle_pf_extra"):
             getattr(L, f).argtypes = None
         _lib = L
     return _lib
@@ -128,6 +128,11 @@ class Result:
         pd, pdp = (ctypes.c_double * max(na, 1))(), (ctypes.c_double * max(na, 1))()
         npd = L.oracle_pf_D(self._h, pd, pdp, na)
         self.pf_D, self.pf_Dprime = list(pd)[:npd], list(pdp)[:npd]
+        bf = (ctypes.c_double * nb)()
+        op = ctypes.c_double()
+        nbf = L.oracle_pf_extra(self._h, bf, nb, ctypes.byref(op))
+        self.bm_full = list(bf)[:nbf]
+        self.openings = op.value
         L.oracle_free(self._h)
         self._h = None
 

@@ -200,3 +200,56 @@ def partition_function(P, T):
             if a > 0 and b > 0 and (a, b) not in ends:
                 bm[(a, b)] = bm.get((a, b), 0.0) + w
     return Z, [x / Z for x in am], {k: v / Z for k, v in bm.items()}, len(alns)
+
+
+def _walk(P, steps, r0, c0, top, owner, out):
+    """Collects (kind, payload, owner) for the structure alignment: match
+    edges ('M', (i, j), owner arc match or None), arc matches ('A', k,
+    owner) and gap runs ('G', length, owner)."""
+    for s in reversed(steps):
+        t = s[0]
+        if t == 'pad':
+            length = (s[2] - c0) if s[1] == r0 else (s[1] - r0)
+            if length > 0:
+                out.append(('G', length, owner))
+        elif t == 'M':
+            out.append(('M', (s[1], s[2]), owner))
+        elif t in ('D', 'I'):
+            out.append(('G', s[1], owner))
+        elif t == 'A':
+            al, ar, bl, br = P.arcmatches[s[2]]
+            out.append(('A', s[2], owner))
+            _walk(P, s[3], al, bl, False, s[2], out)
+
+
+def pf_details(P, T, min_am_prob):
+    """Z, P(am), unrestricted match-edge marginals, the reference's
+    base-match probabilities (match edges directly inside arc matches with
+    P(am) > sqrt(min_am_prob) or at the top level), and the expected number
+    of gap openings."""
+    alns = group_alignments(P, 0, 0, P.n, P.m, True, {})
+    Z = 0.0
+    am = [0.0] * len(P.arcmatches)
+    full, items = {}, []
+    open_w = 0.0
+    for sc, st, _ in alns:
+        w = math.exp(sc / T)
+        Z += w
+        out = []
+        _walk(P, st, 0, 0, True, None, out)
+        items.append((w, out))
+        for kind, x, owner in out:
+            if kind == 'A':
+                am[x] += w
+            elif kind == 'M':
+                full[x] = full.get(x, 0.0) + w
+            elif kind == 'G':
+                open_w += w
+    am = [x / Z for x in am]
+    thr = math.sqrt(min_am_prob)
+    ref = {}
+    for w, out in items:
+        for kind, x, owner in out:
+            if kind == 'M' and (owner is None or am[owner] > thr):
+                ref[x] = ref.get(x, 0.0) + w
+    return (Z, am, {k: v / Z for k, v in full.items()}, {k: v / Z for k, v in ref.items()}, open_w / Z)
