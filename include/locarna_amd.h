@@ -106,6 +106,29 @@ int lcg_batch_D(const lcg_batch *batch, int32_t k, int64_t *D, int32_t cap);
  * candidates evaluated by the fill of D and the top level of pair k. */
 int lcg_batch_counters(const lcg_batch *batch, int32_t k, int64_t *cells, int64_t *candidates);
 
+/* ---- LocARNA-P (FP64): inside partition function and its adjoint.
+ * Score parameters theta (LCG_NTHETA = 275 doubles): base-match scores of
+ * the 4x4 ACGU pairs (16, row-major A,C,G,U), the RIBOSUM arc-match term of
+ * the 16x16 base-pair pairs (256, index (uA*16 + uB), pair index = left*4 +
+ * right), indel, indel opening, struct weight.  Every score of a pair is
+ * affine in theta around the reference scores (theta offsets `dtheta` = 0
+ * give the reference's rounded integer scores).
+ * lcg_batch_pf_run: runs inside + adjoint for all pairs (AlignerP::
+ * align_inside / align_outside, aligner_p.icc:413-438, 1113-1139). */
+#define LCG_NTHETA 275
+int lcg_batch_pf_run(lcg_batch *batch, const double *dtheta, float *device_ms);
+/* Z (AlignerP::align_inside) and the expected number of gap openings */
+int lcg_batch_pf_result(const lcg_batch *batch, int32_t k, double *Z, double *openings);
+/* arc-match probabilities in the reference's arc-match order
+ * (compute_arcmatch_probabilities, aligner_p.icc:1150-1195) */
+int lcg_batch_am_probs(const lcg_batch *batch, int32_t k, double *prob, int32_t cap);
+/* base-match probabilities, dense (lenA+1)*(lenB+1): mode 0 as
+ * compute_basematch_probabilities(false) (aligner_p.icc:1201-1399), mode 1
+ * with --include-am-in-bm, mode 2 marginals of all match edges */
+int lcg_batch_bm_probs(const lcg_batch *batch, int32_t k, int32_t mode, double *dense, int32_t cap);
+/* d log Z / d theta of pair k (no counterpart in the reference) */
+int lcg_batch_pf_grad(const lcg_batch *batch, int32_t k, double *grad, int32_t cap);
+
 /* Synthetic inputs (no counterpart in the reference; they replace
  * RNAfold -p, which the reference calls through ViennaRNA,
  * rna_data.cc:52-61). */

@@ -66,7 +66,15 @@ def _sig(L):
                                       c.c_char_p, c.c_char_p, c.c_int32]
     L.lcg_batch_D.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int32]
     L.lcg_batch_counters.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_int64), c.POINTER(c.c_int64)]
+    L.lcg_batch_pf_run.argtypes = [c.c_void_p, c.c_void_p, c.POINTER(c.c_float)]
+    L.lcg_batch_pf_result.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_double), c.POINTER(c.c_double)]
+    L.lcg_batch_am_probs.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int32]
+    L.lcg_batch_bm_probs.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_void_p, c.c_int32]
+    L.lcg_batch_pf_grad.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int32]
     return L
+
+
+NTHETA = 16 + 256 + 3  # sigma 4x4, arc-match term 16x16, indel, indel opening, struct weight
 
 
 _L = None
@@ -181,6 +189,52 @@ class Batch:
         if cnt < 0:
             raise LcgError("lcg_batch_D (%d)" % cnt)
         return list(out)[:cnt]
+
+    # ---- LocARNA-P
+    def pf_run(self, dtheta=None):
+        """Inside + adjoint for all pairs; dtheta: offsets of the NTHETA score
+        parameters from the reference scores (None: reference)."""
+        ms = ctypes.c_float()
+        arr = None
+        if dtheta is not None:
+            arr = (ctypes.c_double * NTHETA)(*[float(x) for x in dtheta])
+        rc = lib().lcg_batch_pf_run(self.h, arr, ctypes.byref(ms))
+        if rc != 0:
+            raise LcgError("lcg_batch_pf_run failed (%d)" % rc)
+        return ms.value
+
+    def pf_result(self, k):
+        z, op = ctypes.c_double(), ctypes.c_double()
+        rc = lib().lcg_batch_pf_result(self.h, k, ctypes.byref(z), ctypes.byref(op))
+        if rc != 0:
+            raise LcgError("lcg_batch_pf_result (%d)" % rc)
+        return z.value, op.value
+
+    def am_probs(self, k):
+        n = max(self.pairs[k].num_arcmatches, 1)
+        out = (ctypes.c_double * n)()
+        cnt = lib().lcg_batch_am_probs(self.h, k, out, n)
+        if cnt < 0:
+            raise LcgError("lcg_batch_am_probs (%d)" % cnt)
+        return list(out)[:cnt]
+
+    def bm_probs(self, k, mode=0):
+        """dense (lenA+1)*(lenB+1): mode 0 reference base-match probabilities,
+        1 with arc-match ends (--include-am-in-bm), 2 all match edges"""
+        p = self.pairs[k]
+        n = (p.lenA + 1) * (p.lenB + 1)
+        out = (ctypes.c_double * n)()
+        cnt = lib().lcg_batch_bm_probs(self.h, k, mode, out, n)
+        if cnt < 0:
+            raise LcgError("lcg_batch_bm_probs (%d)" % cnt)
+        return list(out)
+
+    def pf_grad(self, k):
+        out = (ctypes.c_double * NTHETA)()
+        rc = lib().lcg_batch_pf_grad(self.h, k, out, NTHETA)
+        if rc < 0:
+            raise LcgError("lcg_batch_pf_grad (%d)" % rc)
+        return list(out)
 
     def destroy(self):
         if self.h:

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -11,6 +12,7 @@
 #include <vector>
 
 #include "device_layout.h"
+#include "pf_layout.h"
 #include "host.hpp"
 #include "locarna_amd.h"
 
@@ -31,6 +33,7 @@ struct lcg_pair {
     PairDesc desc;
     int64_t cells = 0;  // DP cells (groups + top level) as a work estimate
     int ecap = 0;       // max columns of a group region (E vector)
+    std::vector<int> entry_of;  // arc match -> candidate entry
 };
 
 namespace {
@@ -197,6 +200,7 @@ void pack(lcg_pair &P) {
     blob[d.cptr + ncells] = t;
     if (t != nam) throw failure("internal: arc match right-end lists incomplete");
     for (size_t q = 0; q < gams.size(); q += 4) gams[q + 1] = entry_of[gams[q + 2]];
+    P.entry_of = entry_of;
     std::copy(gams.begin(), gams.end(), blob.begin() + d.gams);
     d.D = reserve(blob, nam);
     d.edges = reserve(blob, (size_t)n + m + 1, -1);
@@ -301,6 +305,26 @@ extern "C" int lcg_pair_arcmatches(const lcg_pair *p, int32_t *al, int32_t *ar, 
 
 // ------------------------------------------------------------------ batch
 
+// What the LocARNA-P tables of a pair depend on, captured from the prepared
+// pair: reference scores and the derivative of every score with respect to
+// the parameters theta (scores are affine in theta around the reference
+// parameters; the reference's rounding is kept at theta = 0).
+struct PfHost {
+    int n = 0, m = 0, ncells = 0, nam = 0;
+    double T = 300, ps = 1, thr = 0;
+    int tau = 0;
+    long alpha = 0;
+    std::vector<int> codeA, codeB;
+    std::vector<long> gapA, gapB;
+    long sigma[LCG_NCODES][LCG_NCODES];
+    std::vector<int> entry_of;     // am -> entry
+    std::vector<long> am_score;    // by am
+    std::vector<int> am_type;      // by am: (uA * 16 + uB), -1 if no RIBOSUM term
+    std::vector<double> am_f;      // by am: f(arcA) + f(arcB), f = 1 - ln p / ln pe
+    std::vector<int> am_ends;      // by am: al, ar, bl, br
+    std::vector<int> min_col, max_col;
+};
+
 struct lcg_batch {
     int device = 0;
     int npairs = 0;
@@ -324,6 +348,18 @@ struct lcg_batch {
     bool fetched = false;
     bool traced = false;
     std::vector<int32_t> result;  // arena copied back after a run
+    // LocARNA-P
+    std::vector<PfHost> pfh;
+    std::vector<PfDesc> pdescs;
+    std::vector<double> pf_arena;
+    double *d_pfa = nullptr;
+    PfDesc *d_pdescs = nullptr;
+    double *d_pfscratch = nullptr;
+    int pf_grid = 0, pf_waves = 8;
+    size_t pf_lds = 0;
+    PfBatchParams pbp;
+    bool pf_have = false;
+    std::vector<double> pf_result;
 };
 
 namespace {
@@ -337,6 +373,73 @@ int hip_fail(hipError_t e, char *err = nullptr, int errlen = 0) {
         if (e_ != hipSuccess) return hip_fail(e_); \
     } while (0)
 }  // namespace
+
+static int nt4(char c) {
+    switch (c) {
+        case 'A': return 0;
+        case 'C': return 1;
+        case 'G': return 2;
+        case 'U': return 3;
+    }
+    return -1;
+}
+
+static void capture_pf_host(lcg_batch &B, lcg_pair *const *pairs, int npairs) {
+    B.pfh.resize(npairs);
+    for (int k = 0; k < npairs; ++k) {
+        const lcg_pair &P = *pairs[k];
+        PfHost &h = B.pfh[k];
+        h.n = P.desc.n;
+        h.m = P.desc.m;
+        h.ncells = P.desc.ncells;
+        h.nam = P.desc.nam;
+        h.T = (double)P.params.temperature_alipf;
+        h.ps = P.params.pf_scale;
+        h.thr = std::sqrt(P.params.min_am_prob);
+        h.tau = P.params.tau;
+        h.alpha = P.params.indel_opening;
+        h.codeA.assign(h.n + 1, 0);
+        h.codeB.assign(h.m + 1, 0);
+        h.gapA.assign(h.n + 1, 0);
+        h.gapB.assign(h.m + 1, 0);
+        for (int i = 1; i <= h.n; ++i) {
+            h.codeA[i] = P.sc->codeA(i);
+            h.gapA[i] = P.sc->gapA(i);
+        }
+        for (int j = 1; j <= h.m; ++j) {
+            h.codeB[j] = P.sc->codeB(j);
+            h.gapB[j] = P.sc->gapB(j);
+        }
+        for (int a = 0; a < LCG_NCODES; ++a)
+            for (int b = 0; b < LCG_NCODES; ++b) h.sigma[a][b] = P.sc->sigma_code(a, b);
+        h.entry_of = P.entry_of;
+        const BasePairs &bA = *P.bA, &bB = *P.bB;
+        const double peA = P.params.exp_prob > 0 ? P.params.exp_prob : 1.0 / (2.0 * h.n);
+        const double peB = P.params.exp_prob > 0 ? P.params.exp_prob : 1.0 / (2.0 * h.m);
+        h.am_score.resize(h.nam);
+        h.am_type.resize(h.nam);
+        h.am_f.resize(h.nam);
+        h.am_ends.resize(4 * (size_t)h.nam);
+        for (int t = 0; t < h.nam; ++t) {
+            const ArcMatch &x = P.am->arcmatch(t);
+            const Arc &a = bA.arc(x.arcA), &b = bB.arc(x.arcB);
+            h.am_score[t] = P.sc->arcmatch(t);
+            const int u0 = nt4(P.A->nt(a.left)), u1 = nt4(P.A->nt(a.right));
+            const int v0 = nt4(P.B->nt(b.left)), v1 = nt4(P.B->nt(b.right));
+            h.am_type[t] = (P.params.use_ribosum && u0 >= 0 && u1 >= 0 && v0 >= 0 && v1 >= 0)
+                               ? (u0 * 4 + u1) * 16 + (v0 * 4 + v1)
+                               : -1;
+            const double pa = P.A->arc_prob(a.left, a.right), pb = P.B->arc_prob(b.left, b.right);
+            h.am_f[t] = (1 - std::log(pa) / std::log(peA)) + (1 - std::log(pb) / std::log(peB));
+            h.am_ends[4 * t] = a.left;
+            h.am_ends[4 * t + 1] = a.right;
+            h.am_ends[4 * t + 2] = b.left;
+            h.am_ends[4 * t + 3] = b.right;
+        }
+        h.min_col = P.tc->min_cols();
+        h.max_col = P.tc->max_cols();
+    }
+}
 
 extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t device, lcg_batch **out, char *err,
                                 int errlen) {
@@ -463,6 +566,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
         return hip_fail(e, err, errlen);
     if ((e = hipMemcpy(B->d_order, B->order.data(), 4 * (size_t)npairs, hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, err, errlen);
+    capture_pf_host(*B, pairs, npairs);
     *out = B.release();
     return LCG_OK;
 }
@@ -477,6 +581,9 @@ extern "C" void lcg_batch_destroy(lcg_batch *B) {
     if (B->d_order) hipFree(B->d_order);
     if (B->d_queue) hipFree(B->d_queue);
     if (B->d_scratch) hipFree(B->d_scratch);
+    if (B->d_pfa) hipFree(B->d_pfa);
+    if (B->d_pdescs) hipFree(B->d_pdescs);
+    if (B->d_pfscratch) hipFree(B->d_pfscratch);
     if (B->ev0) hipEventDestroy(B->ev0);
     if (B->ev1) hipEventDestroy(B->ev1);
     if (B->stream) hipStreamDestroy(B->stream);
@@ -611,4 +718,192 @@ extern "C" int lcg_batch_counters(const lcg_batch *B, int32_t k, int64_t *cells,
     if (cells) *cells = (int64_t)c0;
     if (candidates) *candidates = (int64_t)c1;
     return LCG_OK;
+}
+
+// ------------------------------------------------------------ LocARNA-P
+
+extern "C" int lcg_launch_pf(const PairDesc *descs, const PfDesc *pdescs, const int *order, int *queue, int *arena,
+                             double *pfa, double *scratch, PfBatchParams bp, int grid, int waves, size_t lds_bytes,
+                             hipStream_t stream);
+
+namespace {
+int pf_setup(lcg_batch &B) {
+    if (B.d_pfa) return LCG_OK;
+    const int np = B.npairs;
+    B.pdescs.resize(np);
+    size_t off = 0;
+    int max_len = 1;
+    size_t stride = 0;
+    for (int k = 0; k < np; ++k) {
+        const PfHost &h = B.pfh[k];
+        PfDesc &f = B.pdescs[k];
+        std::memset(&f, 0, sizeof f);
+        auto take = [&](size_t cnt) {
+            int32_t o = (int32_t)off;
+            off += (cnt + 1) & ~(size_t)1;
+            return o;
+        };
+        f.egA = take(h.n + 1);
+        f.egB = take(h.m + 1);
+        f.es = take(LCG_NCODES * LCG_NCODES);
+        f.eam = take(h.nam);
+        f.Dc = take(h.nam);
+        f.Dbar = take(h.nam);
+        f.edge_full = take(h.ncells);
+        f.edge_ref = take(h.ncells);
+        f.scal = take(4);
+        f.out = take(4);
+        max_len = std::max(max_len, std::max(h.n, h.m));
+        const PairDesc &d = B.descs[k];
+        const size_t top_cells = (size_t)(d.n + 1) * d.wtop;
+        stride = std::max(stride, 4 * top_cells + 2 * (size_t)(d.m + 1) + 8);
+        // groups are no larger than the top level region
+    }
+    B.pf_arena.assign(off, 0.0);
+    B.pf_waves = 8;
+    B.pbp.npairs = np;
+    B.pbp.max_len = max_len;
+    B.pbp.scratch_stride = (int32_t)((stride + 31) & ~(size_t)31);
+    const int hdr_ints = ((5 * max_len + 6 + 1) & ~1) + 2 * (2 * (max_len + 1) + LCG_NCODES * LCG_NCODES);
+    B.pf_lds = (size_t)hdr_ints * 4;
+    hipError_t e;
+    HIPCHECK(hipSetDevice(B.device));
+    int cus = 0;
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B.device));
+    B.pf_grid = std::min(np, 2 * cus);
+    HIPCHECK(hipMalloc(&B.d_pfa, std::max<size_t>(1, off) * 8));
+    HIPCHECK(hipMalloc(&B.d_pdescs, sizeof(PfDesc) * np));
+    HIPCHECK(hipMalloc(&B.d_pfscratch, (size_t)B.pf_grid * B.pf_waves * B.pbp.scratch_stride * 8));
+    HIPCHECK(hipMemcpy(B.d_pdescs, B.pdescs.data(), sizeof(PfDesc) * np, hipMemcpyHostToDevice));
+    (void)e;
+    return LCG_OK;
+}
+
+// Boltzmann tables of pair k for parameters theta0 + dtheta.
+void pf_tables(const lcg_batch &B, int k, const double *dth, double *A) {
+    const PfHost &h = B.pfh[k];
+    const PfDesc &f = B.pdescs[k];
+    auto dt = [&](int idx) { return dth ? dth[idx] : 0.0; };
+    const double T = h.T;
+    const double dindel = dt(272), dopen = dt(273), dsw = dt(274);
+    for (int i = 1; i <= h.n; ++i) A[f.egA + i] = std::exp((h.gapA[i] + dindel) / T);
+    for (int j = 1; j <= h.m; ++j) A[f.egB + j] = std::exp((h.gapB[j] + dindel) / T);
+    for (int a = 0; a < LCG_NCODES; ++a)
+        for (int b = 0; b < LCG_NCODES; ++b)
+            A[f.es + a * LCG_NCODES + b] = std::exp((h.sigma[a][b] + ((a < 4 && b < 4) ? dt(a * 4 + b) : 0.0)) / T);
+    for (int am = 0; am < h.nam; ++am) {
+        double s = (double)h.am_score[am];
+        if (h.am_type[am] >= 0) s += (h.tau / 100.0) * dt(16 + h.am_type[am]);
+        s += dsw * h.am_f[am];
+        A[f.eam + h.entry_of[am]] = std::exp(s / T);
+    }
+    A[f.scal + 0] = std::exp((h.alpha + dopen) / T);
+    A[f.scal + 1] = h.ps;
+    A[f.scal + 2] = h.thr;
+}
+}  // namespace
+
+extern "C" int lcg_batch_pf_run(lcg_batch *B, const double *dtheta, float *device_ms) {
+    if (!B) return LCG_E_ARG;
+    if (int rc = pf_setup(*B)) return rc;
+    for (int k = 0; k < B->npairs; ++k) pf_tables(*B, k, dtheta, B->pf_arena.data());
+    HIPCHECK(hipSetDevice(B->device));
+    HIPCHECK(hipMemcpy(B->d_pfa, B->pf_arena.data(), B->pf_arena.size() * 8, hipMemcpyHostToDevice));
+    HIPCHECK(hipEventRecord(B->ev0, B->stream));
+    int rc = lcg_launch_pf(B->d_descs, B->d_pdescs, B->d_order, B->d_queue, B->d_arena, B->d_pfa, B->d_pfscratch,
+                           B->pbp, B->pf_grid, B->pf_waves, B->pf_lds, B->stream);
+    if (rc) return hip_fail((hipError_t)rc);
+    HIPCHECK(hipEventRecord(B->ev1, B->stream));
+    HIPCHECK(hipEventSynchronize(B->ev1));
+    if (device_ms) HIPCHECK(hipEventElapsedTime(device_ms, B->ev0, B->ev1));
+    B->pf_result.resize(B->pf_arena.size());
+    HIPCHECK(hipMemcpy(B->pf_result.data(), B->d_pfa, B->pf_result.size() * 8, hipMemcpyDeviceToHost));
+    B->pf_have = true;
+    return LCG_OK;
+}
+
+extern "C" int lcg_batch_pf_result(const lcg_batch *B, int32_t k, double *Z, double *openings) {
+    if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    if (!B->pf_have) return LCG_E_STATE;
+    const PfDesc &f = B->pdescs[k];
+    const double z = B->pf_result[f.out];
+    if (Z) *Z = z;
+    if (openings) *openings = B->pf_result[f.out + 1] / z;
+    return LCG_OK;
+}
+
+extern "C" int lcg_batch_am_probs(const lcg_batch *B, int32_t k, double *prob, int32_t cap) {
+    if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    if (!B->pf_have) return LCG_E_STATE;
+    const PfHost &h = B->pfh[k];
+    const PfDesc &f = B->pdescs[k];
+    if (cap < h.nam) return LCG_E_ARG;
+    const double Z = B->pf_result[f.out];
+    for (int am = 0; am < h.nam; ++am) {
+        const int t = h.entry_of[am];
+        prob[am] = B->pf_result[f.Dc + t] * B->pf_result[f.Dbar + t] / Z;
+    }
+    return h.nam;
+}
+
+extern "C" int lcg_batch_bm_probs(const lcg_batch *B, int32_t k, int32_t mode, double *dense, int32_t cap) {
+    if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    if (!B->pf_have) return LCG_E_STATE;
+    const PfHost &h = B->pfh[k];
+    const PfDesc &f = B->pdescs[k];
+    const int W = h.m + 1;
+    if (cap < (h.n + 1) * W) return LCG_E_ARG;
+    const double Z = B->pf_result[f.out];
+    std::fill(dense, dense + (size_t)(h.n + 1) * W, 0.0);
+    const int32_t edge = (mode == 2) ? f.edge_full : f.edge_ref;
+    int cell = 0;
+    for (int i = 0; i <= h.n; ++i) {
+        for (int j = h.min_col[i]; j <= h.max_col[i]; ++j, ++cell) {
+            if (i == 0 || j == 0) continue;
+            const bool vm = h.min_col[i - 1] <= j - 1 && j - 1 <= h.max_col[i - 1];
+            if (mode == 2 || vm) dense[(size_t)i * W + j] = B->pf_result[edge + cell] / Z;
+        }
+    }
+    if (mode == 1) {  // --include-am-in-bm (aligner_p.icc:1382-1397)
+        for (int am = 0; am < h.nam; ++am) {
+            const int t = h.entry_of[am];
+            const double p = B->pf_result[f.Dc + t] * B->pf_result[f.Dbar + t] / Z;
+            dense[(size_t)h.am_ends[4 * am] * W + h.am_ends[4 * am + 2]] += p;
+            dense[(size_t)h.am_ends[4 * am + 1] * W + h.am_ends[4 * am + 3]] += p;
+        }
+    }
+    return (h.n + 1) * W;
+}
+
+extern "C" int lcg_batch_pf_grad(const lcg_batch *B, int32_t k, double *grad, int32_t cap) {
+    if (!B || k < 0 || k >= B->npairs || cap < LCG_NTHETA) return LCG_E_ARG;
+    if (!B->pf_have) return LCG_E_STATE;
+    const PfHost &h = B->pfh[k];
+    const PfDesc &f = B->pdescs[k];
+    const double Z = B->pf_result[f.out];
+    const double T = h.T;
+    std::fill(grad, grad + LCG_NTHETA, 0.0);
+    double matches = 0.0;
+    int cell = 0;
+    for (int i = 0; i <= h.n; ++i) {
+        for (int j = h.min_col[i]; j <= h.max_col[i]; ++j, ++cell) {
+            if (i == 0 || j == 0) continue;
+            const double p = B->pf_result[f.edge_full + cell] / Z;
+            matches += p;
+            const int a = h.codeA[i], b = h.codeB[j];
+            if (a < 4 && b < 4) grad[a * 4 + b] += p / T;
+        }
+    }
+    double dsw = 0.0;
+    for (int am = 0; am < h.nam; ++am) {
+        const int t = h.entry_of[am];
+        const double p = B->pf_result[f.Dc + t] * B->pf_result[f.Dbar + t] / Z;
+        matches += 2 * p;
+        if (h.am_type[am] >= 0) grad[16 + h.am_type[am]] += (h.tau / 100.0) * p / T;
+        dsw += p * h.am_f[am];
+    }
+    grad[272] = (h.n + h.m - 2 * matches) / T;
+    grad[273] = (B->pf_result[f.out + 1] / Z) / T;
+    grad[274] = dsw / T;
+    return LCG_NTHETA;
 }

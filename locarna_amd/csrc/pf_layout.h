@@ -1,0 +1,35 @@
+// Device layout of the LocARNA-P (FP64) data of a batch: a double arena
+// next to the int32 arena of device_layout.h (whose band, groups and
+// candidate lists the PF kernels reuse).
+//
+// Per pair (offsets in doubles):
+//   egA[n+1], egB[m+1]  exp(gap score / T)
+//   es[64]              exp(sigma / T) by symbol codes
+//   eam[nam]            exp(s(am) / T) per candidate entry
+//   Dc[nam], Dbar[nam]  inside D and its adjoint dZ/dD per candidate entry
+//   edge_full[ncells]   weight of match edges into each band cell (all groups)
+//   edge_ref[ncells]    the same restricted to the reference's base-match
+//                       probability cases (top level and arc matches with
+//                       P(am) > sqrt(min_am_prob))
+//   scal[4]             exp(indel_opening / T), pf_scale, sqrt(min_am_prob)
+//   out[4]              Z, expected-openings weight
+#pragma once
+#include <stdint.h>
+
+typedef struct {
+    int32_t egA, egB, es, eam;
+    int32_t Dc, Dbar, edge_full, edge_ref;
+    int32_t scal, out;
+    int32_t pad[6];
+} PfDesc;  // 16 ints
+
+typedef struct {
+    int32_t npairs;
+    int32_t max_len;
+    int32_t scratch_stride;  // doubles of global scratch per (workgroup, wave)
+    int32_t pad[5];
+} PfBatchParams;
+
+// score parameters theta: sigma 4x4, arc-match term 16x16, indel,
+// indel opening, struct weight
+#define LCG_NTHETA (16 + 256 + 3)

@@ -1,0 +1,87 @@
+"""GPU parity of the LocARNA-P path against the oracle: log Z within 1e-9
+relative, arc-match and base-match probabilities within 1e-6 absolute; the
+gradient of log Z against central finite differences of the GPU's own
+log Z (h = 1e-4) and against the oracle's marginals."""
+import math
+
+import pytest
+
+from locarna_amd import api, synth
+from oracle import oracle
+from tiny import tiny_pair
+
+pytestmark = pytest.mark.gpu
+
+TOL_LOGZ = 1e-9
+TOL_P = 1e-6
+
+
+def _compare(B, k, o, label, include_am=False):
+    Z, openings = B.pf_result(k)
+    assert abs(math.log(Z) - math.log(o.pf)) <= TOL_LOGZ * max(1.0, abs(math.log(o.pf))), label
+    am = B.am_probs(k)
+    assert len(am) == len(o.am_probs)
+    for t, (x, y) in enumerate(zip(am, o.am_probs)):
+        assert abs(x - y) <= TOL_P, (label, t, x, y)
+    bm = B.bm_probs(k, 1 if include_am else 0)
+    for t, (x, y) in enumerate(zip(bm, o.bm_probs)):
+        assert abs(x - y) <= TOL_P, (label, t, x, y)
+    full = B.bm_probs(k, 2)
+    for t, (x, y) in enumerate(zip(full, o.bm_full)):
+        assert abs(x - y) <= TOL_P, (label, "full", t, x, y)
+    assert abs(openings - o.openings) <= 1e-9 * max(1.0, o.openings), label
+
+
+def _run(pp_pairs, kw, include_am=False):
+    po = oracle.locarna_p_params(**kw)
+    pa = api.locarna_p_params(**kw)
+    ores = [oracle.pf(a, b, po, include_am_in_bm=include_am) for a, b in pp_pairs]
+    P = [api.Pair(a, b, pa) for a, b in pp_pairs]
+    B = api.Batch(P)
+    print("pf run", len(P), kw, flush=True)
+    B.pf_run()
+    for k, o in enumerate(ores):
+        _compare(B, k, o, k, include_am)
+    return B, P, ores
+
+
+@pytest.mark.parametrize("variant", ["default", "band", "nothreshold", "include_am"])
+def test_pf_tiny_batch(variant):
+    kw = dict(min_prob=0.01, max_bp_span=-1, min_trace_prob=0.0)
+    inc = False
+    if variant == "band":
+        kw.update(min_trace_prob=0.01)
+    elif variant == "nothreshold":
+        kw.update(min_am_prob=0.0)
+    elif variant == "include_am":
+        inc = True
+    _run([tiny_pair(7000 + s) for s in range(30)], kw, include_am=inc)
+
+
+def test_pf_workload_subset():
+    pps, pairs, span = synth.workload("pf16x150", limit_pairs=6)
+    B, P, ores = _run([(pps[a], pps[b]) for a, b in pairs], dict(max_bp_span=span))
+    B.destroy()
+
+
+def test_gradient_finite_differences():
+    pps, pairs, span = synth.workload("pf16x150", limit_pairs=2)
+    pa = api.locarna_p_params(max_bp_span=span)
+    P = [api.Pair(pps[a], pps[b], pa) for a, b in pairs]
+    B = api.Batch(P)
+    B.pf_run()
+    g = [B.pf_grad(k) for k in range(len(P))]
+    h = 1e-4
+    # components: sigma GC, sigma AU, an arc-match term, indel, opening, struct weight
+    for comp in [2 * 4 + 1, 0 * 4 + 3, 16 + (2 * 4 + 1) * 16 + (1 * 4 + 2), 272, 273, 274]:
+        d = [0.0] * api.NTHETA
+        d[comp] = h
+        B.pf_run(d)
+        up = [math.log(B.pf_result(k)[0]) for k in range(len(P))]
+        d[comp] = -h
+        B.pf_run(d)
+        dn = [math.log(B.pf_result(k)[0]) for k in range(len(P))]
+        for k in range(len(P)):
+            fd = (up[k] - dn[k]) / (2 * h)
+            assert abs(fd - g[k][comp]) <= 1e-6 * max(1.0, abs(g[k][comp])), (comp, k, fd, g[k][comp])
+    B.destroy()
