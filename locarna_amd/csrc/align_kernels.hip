@@ -146,6 +146,24 @@ __device__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W,
     int colv = global_init ? alpha : 0;
     bool colopen = true;
     int lo_p = bl, hi_p = hi0;
+    // Candidate entries of row i are fetched one row ahead: the loads only
+    // need the band, the M values they index are read after row i-1 is done.
+    auto row_range = [&](int r, int &t0, int &t1) {
+        const int jl = max(bl + 1, P.mn[r]), jh = min(br - 1, P.mx[r]);
+        if (jl > jh) {
+            t0 = t1 = 0;
+            return;
+        }
+        const int cb = P.coff[r] - P.mn[r];
+        t0 = P.cptr[cb + jl];
+        t1 = P.cptr[cb + jh + 1];
+    };
+    int nt0 = 0, nt1 = 0;
+    int4 ne = make_int4(0, 0, 0, 0);
+    if (al + 1 < ar) {
+        row_range(al + 1, nt0, nt1);
+        if (nt0 + lane < nt1) ne = P.cand[nt0 + lane];
+    }
     for (int i = al + 1; i < ar; ++i) {
         const int r = i - al;
         int *row = Mst + r * W;
@@ -159,14 +177,18 @@ __device__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W,
         const int lo = max(bl, mni), hi = min(br - 1, P.mx[i]);
         if (colopen && lane == 0) row[0] = colv;
         const int jlo = max(bl + 1, mni);
+        const int t0 = nt0, t1 = nt1;
+        const int4 e0 = ne;
+        if (i + 1 < ar) {
+            row_range(i + 1, nt0, nt1);
+            ne = (nt0 + lane < nt1) ? P.cand[nt0 + lane] : make_int4(0, 0, 0, 0);
+        }
         if (jlo <= hi) {
             // arc candidates of the row, candidate-parallel
             for (int c = lane; c <= hi - jlo; c += 64) Srow[c] = LCG_NEG;
             region_sync<GLOBAL>();
-            const int cb = P.coff[i] - mni;
-            const int t0 = P.cptr[cb + jlo], t1 = P.cptr[cb + hi + 1];
             for (int t = t0 + lane; t < t1; t += 64) {
-                const int4 e = P.cand[t];
+                const int4 e = (t - t0 < 64) ? e0 : P.cand[t];
                 const int p = e.x & 0xffff, q = e.x >> 16;
                 if (p > al && q > bl) {
                     const int v = Mst[(p - 1 - al) * W + (q - 1 - row_lo(P, p - 1, bl))] + e.z;

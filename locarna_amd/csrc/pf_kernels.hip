@@ -21,7 +21,9 @@
 // Cell recurrence (inside), lane l owns column j of row i:
 //   E(i,j) = E(i-1,j) gA (1 - o) + M(i-1,j) gA o
 //   s(i,j) = M(i-1,j-1) es(i,j) + E(i,j) + ps * sum_arcs M(p-1,q-1) D
-//   F(i,j) = F(i,j-1) gB (1 - o) + s(i,j-1) gB o      (s(i,jlo-1) = M(i,jlo-1))
+//   F(i,j) = F(i,j-1) gB + s(i,j-1) gB o              (s(i,jlo-1) = M(i,jlo-1))
+// (the reference's F gB + (M(i,j-1) - F) gB o with M = s + F; the E chain
+// keeps its (1 - o) since M(i-1,j) contains E(i-1,j))
 //   M(i,j) = s(i,j) + F(i,j)
 // The F chain is a linear recurrence along the row, solved by a wave scan
 // of (multiplier, offset) pairs; its adjoint is the mirrored suffix scan.
@@ -177,10 +179,10 @@ __device__ void pf_fill(const PfPair &P, const Region &R) {
                 s += sa * ps;
                 gB = P.egB[j];
             }
-            // F chain: F_j = F_{j-1} gB (1-o) + s_{j-1} gB o
+            // F chain: F_j = F_{j-1} gB + s_{j-1} gB o
             double sl = __shfl_up(s, 1);
             if (lane == 0) sl = sprev;
-            double a = act ? gB * (1.0 - eo) : 1.0, b = act ? sl * gB * eo : 0.0;
+            double a = act ? gB : 1.0, b = act ? sl * gB * eo : 0.0;
             wave_scan_affine(a, b);
             const double F = a * Fcarry + b;
             if (act) {
@@ -224,7 +226,7 @@ __device__ void pf_reverse(const PfPair &P, const Region &R, bool want_dbar, dou
         // chunks from the right: F adjoint flows right to left
         const int nch = (hi - jlo) / 64;
         double Fb_in = 0.0;        // Fbar of the column right of the chunk
-        double a_in = 0.0;         // gB(1-o) of that column
+        double a_in = 0.0;         // gB of that column
         for (int ch = nch; ch >= 0; --ch) {
             const int j0 = jlo + ch * 64;
             const int j = j0 + lane;
@@ -232,8 +234,8 @@ __device__ void pf_reverse(const PfPair &P, const Region &R, bool want_dbar, dou
             const size_t idx = act ? (size_t)(i - al) * W + (j - lo) : 0;
             const double g = act ? R.Mb[idx] : 0.0;
             const double gB = act ? P.egB[j] : 1.0;
-            // Fbar_j = g_j + a_{j+1} Fbar_{j+1}, a_{j+1} = gB(j+1)(1-o)
-            double a_next = __shfl_down(act ? gB * (1.0 - eo) : 0.0, 1);
+            // Fbar_j = g_j + gB(j+1) Fbar_{j+1}
+            double a_next = __shfl_down(act ? gB : 0.0, 1);
             if (lane == 63 || !(j + 1 <= hi)) a_next = (lane == 63) ? a_in : 0.0;
             double c = act ? a_next : 0.0, gg = act ? g : 0.0;
             wave_suffix_affine(c, gg);
@@ -243,7 +245,7 @@ __device__ void pf_reverse(const PfPair &P, const Region &R, bool want_dbar, dou
             double gB_next = __shfl_down(gB, 1);
             if (lane == 63) {
                 Fb_next = Fb_in;
-                gB_next = (a_in != 0.0) ? a_in / (1.0 - eo) : 0.0;
+                gB_next = a_in;
             }
             if (!(j + 1 <= hi) && lane != 63) {
                 Fb_next = 0.0;
@@ -290,7 +292,7 @@ __device__ void pf_reverse(const PfPair &P, const Region &R, bool want_dbar, dou
                 }
             }
             Fb_in = readlane_d(Fb, 0);
-            a_in = readlane_d(act ? gB * (1.0 - eo) : 0.0, 0);
+            a_in = readlane_d(act ? gB : 0.0, 0);
         }
         scratch_sync();
     }
