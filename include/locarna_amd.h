@@ -122,6 +122,11 @@ int lcg_batch_pf_result(const lcg_batch *batch, int32_t k, double *Z, double *op
 /* arc-match probabilities in the reference's arc-match order
  * (compute_arcmatch_probabilities, aligner_p.icc:1150-1195) */
 int lcg_batch_am_probs(const lcg_batch *batch, int32_t k, double *prob, int32_t cap);
+/* inside D(am) and outside D'(am) of every arc match in the reference's
+ * arc-match order (AlignerP::D / Dprime over Dmat and Dmatprime,
+ * aligner_p.icc:117-128, 450-1139; D'(am) = dZ/dD(am) e^{s(am)/T} / pf_scale).
+ * Either pointer may be NULL.  Returns the count. */
+int lcg_batch_pf_D(const lcg_batch *batch, int32_t k, double *D, double *Dprime, int32_t cap);
 /* base-match probabilities, dense (lenA+1)*(lenB+1): mode 0 as
  * compute_basematch_probabilities(false) (aligner_p.icc:1201-1399), mode 1
  * with --include-am-in-bm, mode 2 marginals of all match edges */

@@ -69,6 +69,7 @@ def _sig(L):
     L.lcg_batch_pf_run.argtypes = [c.c_void_p, c.c_void_p, c.POINTER(c.c_float)]
     L.lcg_batch_pf_result.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_double), c.POINTER(c.c_double)]
     L.lcg_batch_am_probs.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int32]
+    L.lcg_batch_pf_D.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_void_p, c.c_int32]
     L.lcg_batch_bm_probs.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_void_p, c.c_int32]
     L.lcg_batch_pf_grad.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int32]
     return L
@@ -217,6 +218,15 @@ class Batch:
         if cnt < 0:
             raise LcgError("lcg_batch_am_probs (%d)" % cnt)
         return list(out)[:cnt]
+
+    def pf_D(self, k):
+        """(inside D, outside D') per arc match, reference arc-match order"""
+        n = max(self.pairs[k].num_arcmatches, 1)
+        D, Dp = (ctypes.c_double * n)(), (ctypes.c_double * n)()
+        cnt = lib().lcg_batch_pf_D(self.h, k, D, Dp, n)
+        if cnt < 0:
+            raise LcgError("lcg_batch_pf_D (%d)" % cnt)
+        return list(D)[:cnt], list(Dp)[:cnt]
 
     def bm_probs(self, k, mode=0):
         """dense (lenA+1)*(lenB+1): mode 0 reference base-match probabilities,

@@ -761,6 +761,7 @@ int pf_setup(lcg_batch &B) {
     }
     B.pf_arena.assign(off, 0.0);
     B.pf_waves = 8;
+    if (const char *w = std::getenv("LCG_PF_WAVES")) B.pf_waves = std::min(8, std::max(1, std::atoi(w)));
     B.pbp.npairs = np;
     B.pbp.max_len = max_len;
     B.pbp.scratch_stride = (int32_t)((stride + 31) & ~(size_t)31);
@@ -842,6 +843,20 @@ extern "C" int lcg_batch_am_probs(const lcg_batch *B, int32_t k, double *prob, i
     for (int am = 0; am < h.nam; ++am) {
         const int t = h.entry_of[am];
         prob[am] = B->pf_result[f.Dc + t] * B->pf_result[f.Dbar + t] / Z;
+    }
+    return h.nam;
+}
+
+extern "C" int lcg_batch_pf_D(const lcg_batch *B, int32_t k, double *D, double *Dprime, int32_t cap) {
+    if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    if (!B->pf_have) return LCG_E_STATE;
+    const PfHost &h = B->pfh[k];
+    const PfDesc &f = B->pdescs[k];
+    if (cap < h.nam) return LCG_E_ARG;
+    for (int am = 0; am < h.nam; ++am) {
+        const int t = h.entry_of[am];
+        if (D) D[am] = B->pf_result[f.Dc + t];
+        if (Dprime) Dprime[am] = B->pf_result[f.Dbar + t] * B->pf_result[f.eam + t] / h.ps;
     }
     return h.nam;
 }

@@ -287,6 +287,9 @@ __device__ void pf_reverse(const PfPair &P, const Region &R, bool want_dbar, dou
                     if (q <= bl) continue;
                     const double Dv = P.Dc[t];
                     if (Dv == 0.0) continue;
+                    // M(p-1, q-1) outside the band reads as 0 (aligner_p.icc:150-189 zeroes
+                    // every such cell the recursion can reach)
+                    if (!R.stored(P, p - 1, q - 1)) continue;
                     atomicAdd(R.at(R.Mb, P, p - 1, q - 1), sb * Dv * ps);
                     if (want_dbar) atomicAdd(&P.Dbar[t], sb * (*R.at(R.M, P, p - 1, q - 1)) * ps);
                 }
@@ -308,7 +311,7 @@ __device__ void pf_reverse(const PfPair &P, const Region &R, bool want_dbar, dou
     }
     if (openings) {
         const double tot = wave_sum(open_acc);
-        if (lane == 0) *openings += tot;
+        if (lane == 0) atomicAdd(openings, tot);  // waves of the workgroup share it
     }
 }
 
@@ -419,7 +422,7 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
                 for (int t = G[3] + lane; t < G[4]; t += 64) {
                     const int4 ga = gams[t];
                     const int ar = ga.x & 0xffff, br = ga.x >> 16;
-                    P.Dc[ga.y] = *R.at(R.M, P, ar - 1, br - 1) * P.eam[ga.y];
+                    P.Dc[ga.y] = R.get(R.M, P, ar - 1, br - 1) * P.eam[ga.y];
                 }
             }
             __threadfence();
@@ -456,7 +459,8 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
                 for (int t = G[3] + lane; t < G[4]; t += 64) {
                     const int4 ga = gams[t];
                     const int ar = ga.x & 0xffff, br = ga.x >> 16;
-                    atomicAdd(R.at(R.Mb, P, ar - 1, br - 1), P.Dbar[ga.y] * P.eam[ga.y]);
+                    if (R.stored(P, ar - 1, br - 1))
+                        atomicAdd(R.at(R.Mb, P, ar - 1, br - 1), P.Dbar[ga.y] * P.eam[ga.y]);
                 }
                 scratch_sync();
                 pf_reverse(P, R, true, P.edge_full, &s_open);
@@ -471,7 +475,7 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
                     for (int t = G[3] + lane; t < G[4]; t += 64) {
                         const int4 ga = gams[t];
                         const int ar = ga.x & 0xffff, br = ga.x >> 16;
-                        if (P.Dc[ga.y] * P.Dbar[ga.y] / Z > thr)
+                        if (P.Dc[ga.y] * P.Dbar[ga.y] / Z > thr && R.stored(P, ar - 1, br - 1))
                             atomicAdd(R.at(R.Mb, P, ar - 1, br - 1), P.Dbar[ga.y] * P.eam[ga.y]);
                     }
                     scratch_sync();

@@ -1,5 +1,5 @@
 """GPU parity of the LocARNA-P path against the oracle: log Z within 1e-9
-relative, arc-match and base-match probabilities within 1e-6 absolute; the
+relative, D and D' per arc match within 1e-9 relative, arc-match and base-match probabilities within 1e-6 absolute; the
 gradient of log Z against central finite differences of the GPU's own
 log Z (h = 1e-4) and against the oracle's marginals."""
 import math
@@ -14,11 +14,17 @@ pytestmark = pytest.mark.gpu
 
 TOL_LOGZ = 1e-9
 TOL_P = 1e-6
+TOL_REL_D = 1e-9  # inside D and outside D' per arc match, relative
 
 
 def _compare(B, k, o, label, include_am=False):
     Z, openings = B.pf_result(k)
     assert abs(math.log(Z) - math.log(o.pf)) <= TOL_LOGZ * max(1.0, abs(math.log(o.pf))), label
+    D, Dp = B.pf_D(k)
+    for t, (x, y) in enumerate(zip(D, o.pf_D)):
+        assert abs(x - y) <= TOL_REL_D * abs(y), (label, "D", t, x, y)
+    for t, (x, y) in enumerate(zip(Dp, o.pf_Dprime)):
+        assert abs(x - y) <= TOL_REL_D * abs(y), (label, "D'", t, x, y)
     am = B.am_probs(k)
     assert len(am) == len(o.am_probs)
     for t, (x, y) in enumerate(zip(am, o.am_probs)):
