@@ -95,7 +95,9 @@ struct Pair {
     const int *__restrict__ gapB;   // LDS
     const int *__restrict__ sig;    // LDS: 8x8 sigma
     const int *__restrict__ cptr;   // global
-    int4 *cand;                     // global: (al|bl<<16, j, D, am)
+    int4 *cand;                     // global: (al|bl<<16, j, D, min_col(al-1))
+    const int *fbp, *fbk;           // LDS: fill blocks of row i, (p, first entry)
+    int *fcand;                     // global: fill entries, 3 ints each
     const int *__restrict__ amscore;
     int *D;
     int *edges;
@@ -108,6 +110,11 @@ struct Pair {
 // column bound br (exclusive): [lo, hi].
 __device__ __forceinline__ int row_lo(const Pair &P, int i, int bl) { return max(bl, P.mn[i]); }
 __device__ __forceinline__ int row_hi(const Pair &P, int i, int br) { return min(br - 1, P.mx[i]); }
+
+// fill-list entry: (p | q<<16, j | min_col(p-1)<<16, D)
+struct FillEntry {
+    int pq, jm, D;
+};
 
 struct TopBest {
     int score, i, j;
@@ -190,8 +197,9 @@ __device__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W,
             for (int t = t0 + lane; t < t1; t += 64) {
                 const int4 e = (t - t0 < 64) ? e0 : P.cand[t];
                 const int p = e.x & 0xffff, q = e.x >> 16;
-                if (p > al && q > bl) {
-                    const int v = Mst[(p - 1 - al) * W + (q - 1 - row_lo(P, p - 1, bl))] + e.z;
+                const int rl = max(bl, e.w);  // first stored column of row p-1
+                if (p > al && q > bl && q - 1 >= rl) {
+                    const int v = Mst[(p - 1 - al) * W + (q - 1 - rl)] + e.z;
                     atomicMax(&Srow[e.y - jlo], v);
                     ++cnt_cand;
                 }
@@ -247,6 +255,185 @@ __device__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W,
         atomicAdd(&counts[0], (unsigned long long)cnt_cells);
         atomicAdd(&counts[1], (unsigned long long)cnt_cand);
     }
+}
+
+// Fill of an arc-match group whose columns bl+1 .. br-1 fit NC chunks of 64
+// lanes and whose rows al+1 .. ar-1 fit 64*RT entries of a lane-indexed row
+// table.  Same recurrence and region layout as group_fill (M rows at
+// base + (br-bl) + W), but lane l of chunk c owns column bl+1+64c+l for all
+// rows: the previous row's M, the column's E, gap score and symbol code stay
+// in registers, so a row costs one LDS read (the arc term S) besides the
+// stores.  S is kept at base + (j - bl - 1) and reset to minus infinity as it
+// is read.  The arc candidates come from the fill list (arc matches by right
+// end in A, left end p descending): a row's candidates with p > al are one
+// prefix, located from the LDS block table once per group; the first 256
+// entries of the next row are fetched while the current row is computed.
+template <bool GLOBAL, int NC, int RT>
+__device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, int W, int *base,
+                                 unsigned long long *counts) {
+    const int lane = lane_id();
+    const int alpha = P.alpha;
+    int *Srow = base;
+    int *Mst = base + (br - bl) + W;
+    const int nrows = ar - al - 1;
+    unsigned cnt_cells = 0, cnt_cand = 0;
+    // fill-list range [rt0, rt1) of row al+1+r (the blocks with left end
+    // p > al), lane r % 64 of table r / 64
+    int rt0[RT], rt1[RT];
+#pragma unroll
+    for (int h = 0; h < RT; ++h) {
+        const int r = lane + 64 * h;
+        rt0[h] = rt1[h] = 0;
+        if (r < nrows) {
+            const int i = al + 1 + r;
+            if (max(bl + 1, P.mn[i]) <= min(br - 1, P.mx[i])) {
+                int b = P.fbp[i];
+                const int be = P.fbp[i + 1];
+                rt0[h] = P.fbk[2 * b + 1];
+                while (b < be && P.fbk[2 * b] > al) ++b;
+                rt1[h] = P.fbk[2 * b + 1];
+            }
+        }
+    }
+    auto row_cands = [&](int r, int &t0, int &t1) {
+        t0 = t1 = 0;
+#pragma unroll
+        for (int h = 0; h < RT; ++h)
+            if ((r >> 6) == h) {
+                t0 = readlane(rt0[h], r & 63);
+                t1 = readlane(rt1[h], r & 63);
+            }
+    };
+    const FillEntry none{0, 0, 0};
+    auto fetch = [&](int t, int t1) { return t < t1 ? *reinterpret_cast<const FillEntry *>(P.fcand + 3 * t) : none; };
+    for (int c = lane; c < br - bl; c += 64) Srow[c] = LCG_NEG;
+    // row al (init row): M(al, bl) = 0, M(al, j) = alpha + sum gapB
+    int Mp[NC], Ec[NC], gBc[NC], cBc[NC];
+    const int hi0 = row_hi(P, al, br);
+    {
+        int carry = alpha;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int j = bl + 1 + 64 * c + lane;
+            const bool in = j <= hi0;
+            const bool col = j <= br - 1;
+            const int ps = wave_scan_add(in ? P.gapB[j] : 0);
+            Mp[c] = in ? carry + ps : LCG_NEG;
+            if (in) Mst[j - bl] = carry + ps;
+            carry += readlane(ps, 63);
+            Ec[c] = LCG_NEG;
+            gBc[c] = col ? P.gapB[j] : 0;
+            cBc[c] = col ? P.codeB[j] : 0;
+        }
+        if (lane == 0) Mst[0] = 0;
+    }
+    region_sync<GLOBAL>();
+    int colv = alpha;
+    bool colopen = true;
+    int Mcol_prev = 0;  // M(i-1, bl)
+    // entries of the current row, 4 chunks of 64 (fetched during the
+    // previous row); entries beyond 256 are loaded when used
+    FillEntry cur[4];
+    {
+        int u0 = 0, u1 = 0;
+        if (nrows > 0) row_cands(0, u0, u1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cur[k] = fetch(u0 + 64 * k + lane, u1);
+    }
+    for (int i = al + 1; i < ar; ++i) {
+        const int r = i - al - 1;
+        const int mni = P.mn[i];
+        int Mcol = LCG_NEG;  // M(i, bl)
+        if (colopen && mni <= bl) {
+            colv += P.gapA[i];
+            Mcol = colv;
+        } else {
+            colopen = false;
+        }
+        const int lo = max(bl, mni), hi = min(br - 1, P.mx[i]);
+        int *row = Mst + (r + 1) * W;
+        if (colopen && lane == 0) row[0] = colv;
+        const int jlo = max(bl + 1, mni);
+        int t0, t1, u0 = 0, u1 = 0;
+        row_cands(r, t0, t1);
+        if (r + 1 < nrows) row_cands(r + 1, u0, u1);
+        const FillEntry nx0 = fetch(u0 + lane, u1), nx1 = fetch(u0 + 64 + lane, u1);
+        // arc terms, candidate-parallel over the row's prefix p > al
+        auto arc = [&](const FillEntry &e) {
+            const int p = e.pq & 0xffff, q = e.pq >> 16;
+            const int j = e.jm & 0xffff;
+            const int rl = max(bl, e.jm >> 16);  // first stored column of row p-1
+            if (q > bl && j >= jlo && j <= hi && q - 1 >= rl) {
+                const int v = Mst[(p - 1 - al) * W + (q - 1 - rl)] + e.D;
+                atomicMax(&Srow[j - bl - 1], v);
+                ++cnt_cand;
+            }
+        };
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (t0 + 64 * k + lane < t1) arc(cur[k]);
+        for (int t = t0 + 256 + lane; t < t1; t += 64) arc(fetch(t, t1));
+        cur[2] = fetch(u0 + 128 + lane, u1);
+        cur[3] = fetch(u0 + 192 + lane, u1);
+        region_sync<GLOBAL>();
+        int Fcarry = LCG_NEG;
+        int sprev = Mcol;      // s(i, bl) entering the F chain: M(i, bl)
+        int dgin = Mcol_prev;  // M(i-1, bl)
+        const int gA = P.gapA[i];
+        const int *sigrow = P.sig + P.codeA[i] * LCG_NCODES;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int j = bl + 1 + 64 * c + lane;
+            const bool act = j >= jlo && j <= hi;
+            const int up = Mp[c];
+            const int dg = wave_shift_up(Mp[c], dgin);
+            dgin = readlane(Mp[c], 63);
+            int s = LCG_NEG;
+            if (act) {
+                const int e = clampneg(max(Ec[c] + gA, up + gA + alpha));
+                Ec[c] = e;
+                const int sv = Srow[j - bl - 1];
+                Srow[j - bl - 1] = LCG_NEG;
+                s = clampneg(max(max(dg + sigrow[cBc[c]], e), sv));
+                ++cnt_cells;
+            }
+            const int sl = wave_shift_up(s, sprev);
+            const int a = act ? gBc[c] + P.alpha_pos : 0;
+            const int cc = act ? sl + gBc[c] + alpha : LCG_NEG;
+            const int Pa = wave_scan_add(a);
+            const int px = wave_scan_max(act ? cc - Pa : LCG_NEG);
+            const int F = Pa + max(Fcarry, px);
+            const int Mv = clampneg(max(s, F));
+            if (act) row[j - lo] = Mv;
+            Mp[c] = act ? Mv : LCG_NEG;
+            Fcarry = clampneg(readlane(F, 63));
+            sprev = readlane(s, 63);
+        }
+        cur[0] = nx0;
+        cur[1] = nx1;
+        Mcol_prev = Mcol;
+        region_sync<GLOBAL>();
+    }
+    if (counts) {
+        atomicAdd(&counts[0], (unsigned long long)cnt_cells);
+        atomicAdd(&counts[1], (unsigned long long)cnt_cand);
+    }
+}
+
+// Fill of an arc-match group (fill-D and trace re-fills), by the fixed-column
+// kernel when the group fits it, else by the general one.
+template <bool GLOBAL>
+__device__ void fill_group(const Pair &P, int al, int bl, int ar, int br, int W, int *base,
+                           unsigned long long *counts, bool fixed_cols) {
+    const int cols = br - bl - 1, rows = ar - al - 1;
+    if (!fixed_cols)
+        group_fill<GLOBAL, false>(P, al, bl, ar, br, W, base, true, nullptr, counts);
+    else if (cols <= 64 && rows <= 128)
+        group_fill_fixed<GLOBAL, 1, 2>(P, al, bl, ar, br, W, base, counts);
+    else if (cols <= 128 && rows <= 128)
+        group_fill_fixed<GLOBAL, 2, 2>(P, al, bl, ar, br, W, base, counts);
+    else
+        group_fill<GLOBAL, false>(P, al, bl, ar, br, W, base, true, nullptr, counts);
 }
 
 // stored value of cell (i, j) of the group (origin al, bl; bound br)
@@ -355,27 +542,27 @@ __device__ void group_trace(const Pair &P, const int *Mst, int W, int al, int bl
         // arc matches with right ends (i, j) in list order (al desc, bl desc)
         const int cell = P.coff[i] + (j - P.mn[i]);
         const int t0 = P.cptr[cell], t1 = P.cptr[cell + 1];
-        int found = -1, fp = 0, fq = 0;
-        for (int base = t0; base < t1 && found < 0; base += 64) {
+        bool found = false;
+        int fp = 0, fq = 0;
+        for (int base = t0; base < t1 && !found; base += 64) {
             const int t = base + lane;
             bool hit = false;
-            int p = 0, q = 0, am = 0;
+            int p = 0, q = 0;
             if (t < t1) {
                 const int4 e = P.cand[t];
                 p = e.x & 0xffff;
                 q = e.x >> 16;
-                am = e.w;
                 hit = p > al && q > bl && Mij == Mget(P, Mst, W, al, bl, br, p - 1, q - 1) + e.z;
             }
             const unsigned long long bh = __ballot(hit);
             if (bh) {
                 const int src = (int)__ffsll((long long)bh) - 1;
-                found = readlane(am, src);
+                found = true;
                 fp = readlane(p, src);
                 fq = readlane(q, src);
             }
         }
-        if (found < 0) break;
+        if (!found) break;
         if (lane == 0) {
             emit(P, fp, fq, fp + fq);
             emit(P, i, j, i + j);
@@ -445,6 +632,9 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_align_pairs(const PairDes
         }
         for (int t = threadIdx.x; t <= n + 1; t += blockDim.x) coff[t] = arena[d.band + 2 * (n + 1) + t];
         for (int t = threadIdx.x; t < LCG_NCODES * LCG_NCODES; t += blockDim.x) sig[t] = arena[d.sigma + t];
+        int *fbp = lds + bp.fptr_lds, *fbk = lds + bp.fblk_lds;
+        for (int t = threadIdx.x; t <= n + 1; t += blockDim.x) fbp[t] = arena[d.fptr + t];
+        for (int t = threadIdx.x; t < 2 * d.nfblk; t += blockDim.x) fbk[t] = arena[d.fblk + t];
         if (bp.trace) {
             for (int t = threadIdx.x; t <= n + m; t += blockDim.x) arena[d.edges + t] = -1;
             for (int t = threadIdx.x; t < n + m; t += blockDim.x) arena[d.structs + t] = 0;
@@ -462,6 +652,9 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_align_pairs(const PairDes
         P.sig = sig;
         P.cptr = arena + d.cptr;
         P.cand = reinterpret_cast<int4 *>(arena + d.cand);
+        P.fbp = fbp;
+        P.fbk = fbk;
+        P.fcand = arena + d.fcand;
         P.amscore = arena + d.amscore;
         P.D = arena + d.D;
         P.edges = arena + d.edges;
@@ -491,13 +684,14 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_align_pairs(const PairDes
                     const int gar = G[1] & 0xffff, gbr = G[1] >> 16;
                     const int W = G[2];
                     int *base = pool + G[6];
-                    group_fill<false, false>(P, gal, gbl, gar, gbr, W, base, true, nullptr, counts);
+                    fill_group<false>(P, gal, gbl, gar, gbr, W, base, counts, bp.fixed_cols != 0);
                     const int *Mst = base + (gbr - gbl) + W;
                     for (int t = G[3] + lane; t < G[4]; t += 64) {
                         const int4 ga = gams[t];
                         const int ar = ga.x & 0xffff, br = ga.x >> 16;
                         const int v = Mst[(ar - 1 - gal) * W + (br - 1 - row_lo(P, ar - 1, gbl))] + P.amscore[ga.z];
                         P.cand[ga.y].z = v;  // D, not normalized (the reference's InftyInt sum)
+                        P.fcand[3 * ga.w + 2] = v;
                         P.D[ga.z] = v;
                     }
                 }

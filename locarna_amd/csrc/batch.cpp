@@ -167,7 +167,8 @@ void pack(lcg_pair &P) {
     d.amscore = reserve(blob, nam);
     for (int k = 0; k < nam; ++k) blob[d.amscore + k] = (int32_t)sc.arcmatch(k);
     // candidate CSR over band cells, each list sorted (al desc, bl desc);
-    // entries (al | bl<<16, j, D, am) are 16-byte aligned
+    // entries (al | bl<<16, j, D, min_col(al - 1)) are 16-byte aligned; the
+    // last field locates row al - 1 of a group region without a lookup
     d.cptr = reserve(blob, (size_t)ncells + 1);
     while (blob.size() & 3) blob.push_back(0);
     d.cand = reserve(blob, 4 * (size_t)nam);
@@ -190,7 +191,7 @@ void pack(lcg_pair &P) {
                 blob[d.cand + 4 * t] = bA.arc(x.arcA).left | (bB.arc(x.arcB).left << 16);
                 blob[d.cand + 4 * t + 1] = j;
                 blob[d.cand + 4 * t + 2] = LCG_NEG;
-                blob[d.cand + 4 * t + 3] = k;
+                blob[d.cand + 4 * t + 3] = tc.min_col(bA.arc(x.arcA).left - 1);
                 entry_of[k] = t;
                 ++t;
             }
@@ -199,7 +200,51 @@ void pack(lcg_pair &P) {
     if (next_cell != ncells) throw failure("internal: band cell count");
     blob[d.cptr + ncells] = t;
     if (t != nam) throw failure("internal: arc match right-end lists incomplete");
-    for (size_t q = 0; q < gams.size(); q += 4) gams[q + 1] = entry_of[gams[q + 2]];
+    // fill-D candidate list: the arc matches of row i (right end in A),
+    // blocks of one arc of A by left end p descending, j ascending in a block
+    std::vector<std::vector<int>> by_rightA(n + 1);
+    for (int k = 0; k < nam; ++k) by_rightA[bA.arc(ams.arcmatch(k).arcA).right].push_back(k);
+    d.fptr = reserve(blob, (size_t)n + 2);
+    d.fcand = reserve(blob, 3 * (size_t)nam);
+    std::vector<int32_t> blk;
+    std::vector<int> fentry_of(nam, -1);
+    int tf = 0;
+    for (int i = 0; i <= n; ++i) {
+        blob[d.fptr + i] = (int32_t)(blk.size() / 2);
+        auto lst = by_rightA[i];
+        std::sort(lst.begin(), lst.end(), [&](int x, int y) {
+            const ArcMatch &ax = ams.arcmatch(x), &ay = ams.arcmatch(y);
+            const int px = bA.arc(ax.arcA).left, py = bA.arc(ay.arcA).left;
+            if (px != py) return px > py;
+            const int jx = bB.arc(ax.arcB).right, jy = bB.arc(ay.arcB).right;
+            if (jx != jy) return jx < jy;
+            return bB.arc(ax.arcB).left > bB.arc(ay.arcB).left;
+        });
+        int last_p = -1;
+        for (int k : lst) {
+            const ArcMatch &x = ams.arcmatch(k);
+            const int p = bA.arc(x.arcA).left, q = bB.arc(x.arcB).left, j = bB.arc(x.arcB).right;
+            if (p != last_p) {
+                blk.push_back(p);
+                blk.push_back(tf);
+                last_p = p;
+            }
+            blob[d.fcand + 3 * tf] = p | (q << 16);
+            blob[d.fcand + 3 * tf + 1] = j | (tc.min_col(p - 1) << 16);
+            blob[d.fcand + 3 * tf + 2] = LCG_NEG;
+            fentry_of[k] = tf++;
+        }
+    }
+    blob[d.fptr + n + 1] = (int32_t)(blk.size() / 2);
+    blk.push_back(0);
+    blk.push_back(tf);
+    d.nfblk = (int32_t)(blk.size() / 2);
+    d.fblk = reserve(blob, blk.size());
+    std::copy(blk.begin(), blk.end(), blob.begin() + d.fblk);
+    for (size_t q = 0; q < gams.size(); q += 4) {
+        gams[q + 1] = entry_of[gams[q + 2]];
+        gams[q + 3] = fentry_of[gams[q + 2]];
+    }
     P.entry_of = entry_of;
     std::copy(gams.begin(), gams.end(), blob.begin() + d.gams);
     d.D = reserve(blob, nam);
@@ -449,7 +494,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     B->device = device;
     B->npairs = npairs;
     const lcg_params &p0 = pairs[0]->params;
-    int max_len = 1, gmax = 4, top_max = 4;
+    int max_len = 1, gmax = 4, top_max = 4, nfblk_max = 1;
     size_t total = 0;
     for (int k = 0; k < npairs; ++k) {
         const lcg_pair *P = pairs[k];
@@ -459,6 +504,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
         }
         max_len = std::max(max_len, std::max(P->desc.n, P->desc.m));
         gmax = std::max(gmax, P->desc.gmax);
+        nfblk_max = std::max(nfblk_max, P->desc.nfblk);
         top_max = std::max(top_max, P->desc.top_ints);
         total += P->blob.size();
     }
@@ -473,7 +519,8 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
         B->host_arena.insert(B->host_arena.end(), P->blob.begin(), P->blob.end());
         PairDesc d = P->desc;
         int32_t *offs[] = {&d.band, &d.codes, &d.gaps, &d.sigma, &d.rows, &d.groups, &d.gams, &d.amscore,
-                           &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.out, &d.rounds};
+                           &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.out, &d.rounds,
+                           &d.fptr, &d.fblk, &d.fcand};
         for (int32_t *o : offs) *o += base;
         B->descs.push_back(d);
     }
@@ -485,12 +532,18 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) { return work[a] > work[b]; });
 
     // Launch shape: LDS = header (per-pair band / codes / gaps / sigma of the
-    // longest sequence) + pool for the group regions of one row.  Prefer two
-    // workgroups of 8 waves per CU; fall back to one workgroup when the
-    // largest group region does not fit half the LDS.
-    const int hdr = (7 * max_len + 72 + 3) & ~3;
+    // longest sequence) + pool for the group regions of one row.  One
+    // workgroup of 16 waves per CU with the whole LDS as its pool: a larger
+    // pool packs more groups of a row into one round, which measured faster
+    // than two workgroups of 8 waves with half the pool each (allpairs64:
+    // 108 ms against 173 ms per step).
+    // header: band, codes, gaps, sigma (7 L + 72 ints), fill-list row
+    // pointers (L + 2) and blocks (2 per block)
+    const int fptr_lds = 7 * max_len + 72;
+    const int fblk_lds = fptr_lds + max_len + 2;
+    const int hdr = (fblk_lds + 2 * nfblk_max + 3) & ~3;
     const int lds_ints_cu = (160 * 1024 - 512) / 4;  // static LDS of the kernel included
-    int waves = 8, wgs_per_cu = 2;
+    int waves = 16, wgs_per_cu = 1;
     if (const char *w = std::getenv("LCG_WAVES")) waves = std::min(16, std::max(1, std::atoi(w)));
     if (const char *w = std::getenv("LCG_WGS_PER_CU")) wgs_per_cu = std::min(4, std::max(1, std::atoi(w)));
     int pool = lds_ints_cu / wgs_per_cu - hdr;
@@ -539,10 +592,14 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     B->bp.trace = 1;
     B->bp.npairs = npairs;
     B->bp.hdr_ints = hdr;
+    B->bp.fptr_lds = fptr_lds;
+    B->bp.fblk_lds = fblk_lds;
     B->bp.pool_ints = pool;
     B->bp.max_len = max_len;
     B->bp.count = 0;
     B->bp.scratch_stride = top_max;
+    B->bp.fixed_cols = 1;
+    if (const char *f = std::getenv("LCG_FIXED_COLS")) B->bp.fixed_cols = std::atoi(f) != 0;
 
     hipError_t e;
     if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, err, errlen);

@@ -19,12 +19,24 @@
 //                                        lds offset, lds ints) where ar, br are
 //                                       the maximal right ends and W the
 //                                       stored row width of the group region
-//   gams:     2 per arc match, group order: (ar | br<<16, candidate entry)
+//   gams:     4 per arc match, group order: (ar | br<<16, candidate entry,
+//             arc match, fill entry)
 //   amscore:  s(am) per arc match (reference index order)
 //   cptr:     ncells + 1                CSR over band cells of the arc matches
 //   cand:     4 per entry               with those right ends, sorted by cell
 //                                       then (al desc, bl desc):
-//                                       (al | bl<<16, j, D(am), am)
+//                                       (al | bl<<16, j, D(am),
+//                                        min_col(al - 1))
+//   fptr:     n + 2                     first fill block of row i
+//   fblk:     2 per block + sentinel    one block per arc of A (left end p,
+//                                       right end i), rows i ascending, p
+//                                       descending within a row: (p, first
+//                                       fill entry); sentinel (0, nam)
+//   fcand:    3 per entry               the arc matches again, by block,
+//                                       j ascending: (p | q<<16,
+//                                       j | min_col(p-1)<<16, D(am)); a group
+//                                       with left end al reads the prefix
+//                                       p > al of each row
 //   D:        per arc match (output of the fill, reference index order)
 //   edges:    n + m + 1 slots (output): slot i+j holds the edge that reaches
 //             cell (i, j): (a << 16) | b with 0 for a gap side, -1 if empty
@@ -55,7 +67,9 @@ typedef struct {
     int32_t gmax;      // max LDS ints of a group region
     int32_t ncells;
     int32_t top_ints;  // LDS ints of the top-level region
-    int32_t pad[8];
+    int32_t fptr, fblk, fcand;
+    int32_t nfblk;     // fill blocks including the sentinel
+    int32_t pad[4];
 } PairDesc;  // 32 ints
 
 typedef struct {
@@ -68,5 +82,8 @@ typedef struct {
     int32_t max_len;         // max sequence length over the batch
     int32_t count;           // count DP cells and arc candidates (out[4..7])
     int32_t scratch_stride;  // ints of global scratch per (workgroup, wave)
-    int32_t pad[7];
+    int32_t fixed_cols;      // fill narrow groups with the fixed-column kernel
+    int32_t fptr_lds;        // LDS offsets (ints) of the pair's fptr and fblk
+    int32_t fblk_lds;
+    int32_t pad[4];
 } BatchParams;
