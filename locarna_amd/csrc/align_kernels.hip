@@ -126,7 +126,7 @@ struct TopBest {
 // GLOBAL: the region is global scratch; TOP_LOCAL: the sequence-local top
 // level (values clamped at 0, first maximum tracked).
 template <bool GLOBAL, bool TOP_LOCAL>
-__device__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W, int *base, bool global_init,
+__device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W, int *base, bool global_init,
                            TopBest *best, unsigned long long *counts) {
     const int lane = lane_id();
     const int alpha = P.alpha;
@@ -358,21 +358,42 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
         row_cands(r, t0, t1);
         if (r + 1 < nrows) row_cands(r + 1, u0, u1);
         const FillEntry nx0 = fetch(u0 + lane, u1), nx1 = fetch(u0 + 64 + lane, u1);
-        // arc terms, candidate-parallel over the row's prefix p > al
-        auto arc = [&](const FillEntry &e) {
+        // arc terms, candidate-parallel over the row's prefix p > al: the
+        // M gathers of all chunks are issued before the LDS atomics (which
+        // the compiler may not reorder them across)
+        auto arc_slot = [&](const FillEntry &e, bool in, int &v) {
             const int p = e.pq & 0xffff, q = e.pq >> 16;
             const int j = e.jm & 0xffff;
             const int rl = max(bl, e.jm >> 16);  // first stored column of row p-1
-            if (q > bl && j >= jlo && j <= hi && q - 1 >= rl) {
-                const int v = Mst[(p - 1 - al) * W + (q - 1 - rl)] + e.D;
-                atomicMax(&Srow[j - bl - 1], v);
-                ++cnt_cand;
-            }
+            const bool ok = in && q > bl && j >= jlo && j <= hi && q - 1 >= rl;
+            // branch-free gather (cell 0 of the region when unused), so that
+            // the gathers of several chunks are in flight together
+            v = Mst[ok ? (p - 1 - al) * W + (q - 1 - rl) : 0] + e.D;
+            cnt_cand += ok;
+            return ok ? j - bl - 1 : -1;
         };
+        {
+            int v[4], slot[4];
+            // rows of at most 128 candidates (most) skip chunks 2 and 3
+            if (t1 - t0 <= 128) {
 #pragma unroll
-        for (int k = 0; k < 4; ++k)
-            if (t0 + 64 * k + lane < t1) arc(cur[k]);
-        for (int t = t0 + 256 + lane; t < t1; t += 64) arc(fetch(t, t1));
+                for (int k = 0; k < 2; ++k) slot[k] = arc_slot(cur[k], t0 + 64 * k + lane < t1, v[k]);
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+                    if (slot[k] >= 0) atomicMax(&Srow[slot[k]], v[k]);
+            } else {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) slot[k] = arc_slot(cur[k], t0 + 64 * k + lane < t1, v[k]);
+#pragma unroll
+                for (int k = 0; k < 4; ++k)
+                    if (slot[k] >= 0) atomicMax(&Srow[slot[k]], v[k]);
+            }
+        }
+        for (int t = t0 + 256 + lane; t < t1; t += 64) {
+            int v;
+            const int sl = arc_slot(fetch(t, t1), true, v);
+            if (sl >= 0) atomicMax(&Srow[sl], v);
+        }
         cur[2] = fetch(u0 + 128 + lane, u1);
         cur[3] = fetch(u0 + 192 + lane, u1);
         region_sync<GLOBAL>();
@@ -388,13 +409,15 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
             const int up = Mp[c];
             const int dg = wave_shift_up(Mp[c], dgin);
             dgin = readlane(Mp[c], 63);
+            // both LDS reads unconditional (slot 0 when inactive), issued together
+            const int sv = Srow[act ? j - bl - 1 : 0];
+            const int sg = sigrow[cBc[c]];
             int s = LCG_NEG;
             if (act) {
                 const int e = clampneg(max(Ec[c] + gA, up + gA + alpha));
                 Ec[c] = e;
-                const int sv = Srow[j - bl - 1];
                 Srow[j - bl - 1] = LCG_NEG;
-                s = clampneg(max(max(dg + sigrow[cBc[c]], e), sv));
+                s = clampneg(max(max(dg + sg, e), sv));
                 ++cnt_cells;
             }
             const int sl = wave_shift_up(s, sprev);
@@ -453,7 +476,7 @@ __device__ __forceinline__ void emit(const Pair &P, int a, int b, int slot) { P.
 // matrix.  Arc matches taken are appended to the work list (tstack, counter
 // *tcount in LDS).
 template <bool GLOBAL>
-__device__ void group_trace(const Pair &P, const int *Mst, int W, int al, int bl, int br, int i, int j, bool top,
+__device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W, int al, int bl, int br, int i, int j, bool top,
                             bool local, int *tstack, int *tcount) {
     const int lane = lane_id();
     region_sync<GLOBAL>();
@@ -580,7 +603,7 @@ __device__ void group_trace(const Pair &P, const int *Mst, int W, int al, int bl
 }
 
 // stored width of the rows al..ar-1 of a region (origin al, bl; bound br)
-__device__ int region_width(const Pair &P, int al, int bl, int ar, int br) {
+__device__ __forceinline__ int region_width(const Pair &P, int al, int bl, int ar, int br) {
     int w = 0;
     for (int i = al; i < ar; ++i) {
         const int lo = (i == al) ? bl : row_lo(P, i, bl);
@@ -590,89 +613,91 @@ __device__ int region_width(const Pair &P, int al, int bl, int ar, int br) {
 }
 
 template <bool GLOBAL>
-__device__ void fill_and_trace(const Pair &P, int al, int bl, int ar, int br, int W, int *base, int *tstack,
+__device__ __forceinline__ void fill_and_trace(const Pair &P, int al, int bl, int ar, int br, int W, int *base, int *tstack,
                                int *tcount) {
-    group_fill<GLOBAL, false>(P, al, bl, ar, br, W, base, true, nullptr, nullptr);
+    fill_group<GLOBAL>(P, al, bl, ar, br, W, base, nullptr, true);
     group_trace<GLOBAL>(P, base + (br - bl) + W, W, al, bl, br, ar - 1, br - 1, false, false, tstack, tcount);
 }
 
 }  // namespace
 
-extern "C" __global__ void __launch_bounds__(1024) lcg_align_pairs(const PairDesc *__restrict__ descs,
-                                                                   const int *__restrict__ order, int *queue,
-                                                                   int *arena, int *scratch, BatchParams bp) {
-    extern __shared__ int lds[];
-    __shared__ int s_pair, s_tcount;
-    const int nwaves = blockDim.x >> 6;
-    const int wave = threadIdx.x >> 6;
-    const int lane = lane_id();
+// Loads the per-pair header into LDS (band, cell offsets, codes, gaps, sigma,
+// fill-list blocks) and sets up the pair's view.  Ends with a barrier.
+__device__ __forceinline__ Pair load_pair(const PairDesc &d, int *arena, int *lds, const BatchParams &bp) {
     const int L = bp.max_len;
+    const int n = d.n, m = d.m;
     int *mn = lds, *mx = lds + (L + 1), *coff = lds + 2 * (L + 1);
     int *codeA = lds + 3 * L + 4, *codeB = lds + 4 * L + 5;
     int *gapA = lds + 5 * L + 6, *gapB = lds + 6 * L + 7;
     int *sig = lds + 7 * L + 8;
+    int *fbp = lds + bp.fptr_lds, *fbk = lds + bp.fblk_lds;
+    for (int t = threadIdx.x; t <= n; t += blockDim.x) {
+        mn[t] = arena[d.band + t];
+        mx[t] = arena[d.band + n + 1 + t];
+        codeA[t] = arena[d.codes + t];
+        gapA[t] = arena[d.gaps + t];
+    }
+    for (int t = threadIdx.x; t <= m; t += blockDim.x) {
+        codeB[t] = arena[d.codes + n + 1 + t];
+        gapB[t] = arena[d.gaps + n + 1 + t];
+    }
+    for (int t = threadIdx.x; t <= n + 1; t += blockDim.x) {
+        coff[t] = arena[d.band + 2 * (n + 1) + t];
+        fbp[t] = arena[d.fptr + t];
+    }
+    for (int t = threadIdx.x; t < LCG_NCODES * LCG_NCODES; t += blockDim.x) sig[t] = arena[d.sigma + t];
+    for (int t = threadIdx.x; t < 2 * d.nfblk; t += blockDim.x) fbk[t] = arena[d.fblk + t];
+    Pair P;
+    P.mn = mn;
+    P.mx = mx;
+    P.coff = coff;
+    P.codeA = codeA;
+    P.codeB = codeB;
+    P.gapA = gapA;
+    P.gapB = gapB;
+    P.sig = sig;
+    P.cptr = arena + d.cptr;
+    P.cand = reinterpret_cast<int4 *>(arena + d.cand);
+    P.fbp = fbp;
+    P.fbk = fbk;
+    P.fcand = arena + d.fcand;
+    P.amscore = arena + d.amscore;
+    P.D = arena + d.D;
+    P.edges = arena + d.edges;
+    P.structs = arena + d.structs;
+    P.n = n;
+    P.m = m;
+    P.alpha = bp.alpha;
+    P.alpha_pos = bp.alpha > 0 ? bp.alpha : 0;
+    return P;
+}
+
+// Kernel 1: fill D of every pair (rows al = n .. 1, the groups of a row
+// packed into LDS rounds by the host).
+extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc *__restrict__ descs,
+                                                                  const int *__restrict__ order, int *queue,
+                                                                  int *arena, BatchParams bp) {
+    extern __shared__ int lds[];
+    __shared__ int s_pair;
+    const int wave = threadIdx.x >> 6;
+    const int lane = lane_id();
     int *pool = lds + bp.hdr_ints;
-    int *wave_scratch = scratch + (size_t)(blockIdx.x * nwaves + wave) * bp.scratch_stride;
     for (;;) {
         if (threadIdx.x == 0) s_pair = atomicAdd(queue, 1);
         __syncthreads();
         const int k = s_pair;
         if (k >= bp.npairs) break;
         const PairDesc d = descs[order[k]];
-        const int n = d.n, m = d.m;
-        for (int t = threadIdx.x; t <= n; t += blockDim.x) {
-            mn[t] = arena[d.band + t];
-            mx[t] = arena[d.band + n + 1 + t];
-            codeA[t] = arena[d.codes + t];
-            gapA[t] = arena[d.gaps + t];
-        }
-        for (int t = threadIdx.x; t <= m; t += blockDim.x) {
-            codeB[t] = arena[d.codes + n + 1 + t];
-            gapB[t] = arena[d.gaps + n + 1 + t];
-        }
-        for (int t = threadIdx.x; t <= n + 1; t += blockDim.x) coff[t] = arena[d.band + 2 * (n + 1) + t];
-        for (int t = threadIdx.x; t < LCG_NCODES * LCG_NCODES; t += blockDim.x) sig[t] = arena[d.sigma + t];
-        int *fbp = lds + bp.fptr_lds, *fbk = lds + bp.fblk_lds;
-        for (int t = threadIdx.x; t <= n + 1; t += blockDim.x) fbp[t] = arena[d.fptr + t];
-        for (int t = threadIdx.x; t < 2 * d.nfblk; t += blockDim.x) fbk[t] = arena[d.fblk + t];
-        if (bp.trace) {
-            for (int t = threadIdx.x; t <= n + m; t += blockDim.x) arena[d.edges + t] = -1;
-            for (int t = threadIdx.x; t < n + m; t += blockDim.x) arena[d.structs + t] = 0;
-        }
         if (bp.count && threadIdx.x < 4) arena[d.out + 4 + threadIdx.x] = 0;
-        if (threadIdx.x == 0) s_tcount = 0;
-        Pair P;
-        P.mn = mn;
-        P.mx = mx;
-        P.coff = coff;
-        P.codeA = codeA;
-        P.codeB = codeB;
-        P.gapA = gapA;
-        P.gapB = gapB;
-        P.sig = sig;
-        P.cptr = arena + d.cptr;
-        P.cand = reinterpret_cast<int4 *>(arena + d.cand);
-        P.fbp = fbp;
-        P.fbk = fbk;
-        P.fcand = arena + d.fcand;
-        P.amscore = arena + d.amscore;
-        P.D = arena + d.D;
-        P.edges = arena + d.edges;
-        P.structs = arena + d.structs;
-        P.n = n;
-        P.m = m;
-        P.alpha = bp.alpha;
-        P.alpha_pos = bp.alpha > 0 ? bp.alpha : 0;
+        const Pair P = load_pair(d, arena, lds, bp);
         unsigned long long *counts = bp.count ? (unsigned long long *)(arena + d.out + 4) : nullptr;
         __threadfence_block();
         __syncthreads();
-
         const int *rows = arena + d.rows;
         const int *rounds = arena + d.rounds;
         const int *groups = arena + d.groups;
         const int4 *gams = reinterpret_cast<const int4 *>(arena + d.gams);
-        // ---- 1. fill D, rows al = n .. 1, groups packed into LDS rounds
-        for (int al = n; al >= 1; --al) {
+        for (int al = d.n; al >= 1; --al) {
             const int g0 = rows[al + 1], g1 = rows[al];
             if (g0 == g1) continue;
             const int nr = rounds[al];
@@ -699,7 +724,39 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_align_pairs(const PairDes
                 __syncthreads();
             }
         }
-        // ---- 2. top level (wave 0)
+        __syncthreads();
+    }
+}
+
+// Kernel 2: top level (global or sequence-local) and traceback of every
+// pair, after kernel 1 has filled D.
+extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc *__restrict__ descs,
+                                                                 const int *__restrict__ order, int *queue,
+                                                                 int *arena, int *scratch, BatchParams bp) {
+    extern __shared__ int lds[];
+    __shared__ int s_pair, s_tcount;
+    const int nwaves = blockDim.x >> 6;
+    const int wave = threadIdx.x >> 6;
+    const int lane = lane_id();
+    int *pool = lds + bp.hdr_ints;
+    int *wave_scratch = scratch + (size_t)(blockIdx.x * nwaves + wave) * bp.scratch_stride;
+    for (;;) {
+        if (threadIdx.x == 0) s_pair = atomicAdd(queue, 1);
+        __syncthreads();
+        const int k = s_pair;
+        if (k >= bp.npairs) break;
+        const PairDesc d = descs[order[k]];
+        const int n = d.n, m = d.m;
+        if (bp.trace) {
+            for (int t = threadIdx.x; t <= n + m; t += blockDim.x) arena[d.edges + t] = -1;
+            for (int t = threadIdx.x; t < n + m; t += blockDim.x) arena[d.structs + t] = 0;
+        }
+        if (threadIdx.x == 0) s_tcount = 0;
+        const Pair P = load_pair(d, arena, lds, bp);
+        unsigned long long *counts = bp.count ? (unsigned long long *)(arena + d.out + 4) : nullptr;
+        __threadfence();
+        __syncthreads();
+        // ---- top level (wave 0)
         const bool top_in_lds = d.top_ints <= bp.pool_ints;
         int *topbase = top_in_lds ? pool : wave_scratch;
         int *topM = topbase + (m + 1) + d.wtop;
@@ -724,7 +781,7 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_align_pairs(const PairDes
                 arena[d.out + 1] = best.i;
                 arena[d.out + 2] = best.j;
             }
-            // ---- 3a. trace the top level
+            // ---- trace the top level
             if (bp.trace) {
                 if (top_in_lds)
                     group_trace<false>(P, topM, d.wtop, 0, 0, m + 1, best.i, best.j, true, bp.local != 0,
@@ -736,23 +793,36 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_align_pairs(const PairDes
         }
         __threadfence_block();
         __syncthreads();
-        // ---- 3b. trace inside the traced arc matches, breadth first
+        // ---- trace inside the traced arc matches, breadth first.  A level's
+        // tasks share the LDS pool in equal slots; a task whose region does
+        // not fit its slot waits for a pass in which wave 0 has the whole
+        // pool (every group region fits the pool, checked at batch creation).
         if (bp.trace) {
-            const int slot = (bp.pool_ints / nwaves) & ~3;
-            int *wave_pool = pool + wave * slot;
             int lo = 0, hi = s_tcount;
             __syncthreads();
             while (lo < hi) {
-                for (int e = lo + wave; e < hi; e += nwaves) {
+                const int nt = hi - lo;
+                const int nw = nt < nwaves ? nt : nwaves;
+                const int slot = (bp.pool_ints / nw) & ~3;
+                for (int e = lo + wave; wave < nw && e < hi; e += nw) {
                     const int *ts = arena + d.tstack + 2 * e;
                     const int al = ts[0] & 0xffff, bl = ts[0] >> 16;
                     const int ar = ts[1] & 0xffff, br = ts[1] >> 16;
                     const int W = region_width(P, al, bl, ar, br);
-                    const int need = (br - bl) + W + (ar - al) * W;
-                    if (need <= slot)
-                        fill_and_trace<false>(P, al, bl, ar, br, W, wave_pool, arena + d.tstack, &s_tcount);
-                    else
-                        fill_and_trace<true>(P, al, bl, ar, br, W, wave_scratch, arena + d.tstack, &s_tcount);
+                    if ((br - bl) + W + (ar - al) * W <= slot)
+                        fill_and_trace<false>(P, al, bl, ar, br, W, pool + wave * slot, arena + d.tstack, &s_tcount);
+                }
+                __threadfence_block();
+                __syncthreads();
+                if (wave == 0) {
+                    for (int e = lo; e < hi; ++e) {
+                        const int *ts = arena + d.tstack + 2 * e;
+                        const int al = ts[0] & 0xffff, bl = ts[0] >> 16;
+                        const int ar = ts[1] & 0xffff, br = ts[1] >> 16;
+                        const int W = region_width(P, al, bl, ar, br);
+                        if ((br - bl) + W + (ar - al) * W > slot)
+                            fill_and_trace<false>(P, al, bl, ar, br, W, pool, arena + d.tstack, &s_tcount);
+                    }
                 }
                 __threadfence_block();
                 __syncthreads();
@@ -773,13 +843,20 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
                                 BatchParams bp, int grid, int waves, size_t lds_bytes, hipStream_t stream) {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)lcg_align_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
+        hipError_t e = hipFuncSetAttribute((const void *)lcg_fill_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            160 * 1024 - 256);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void *)lcg_top_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    160 * 1024 - 256);
         if (e != hipSuccess) return (int)e;
         attr_set = true;
     }
+    // two work counters, 64 bytes apart
     hipLaunchKernelGGL(lcg_reset_queue, dim3(1), dim3(64), 0, stream, queue);
-    hipLaunchKernelGGL(lcg_align_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue, arena,
-                       scratch, bp);
+    hipLaunchKernelGGL(lcg_reset_queue, dim3(1), dim3(64), 0, stream, queue + 16);
+    hipLaunchKernelGGL(lcg_fill_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue, arena,
+                       bp);
+    hipLaunchKernelGGL(lcg_top_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue + 16,
+                       arena, scratch, bp);
     return (int)hipGetLastError();
 }

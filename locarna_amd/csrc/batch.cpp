@@ -613,7 +613,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     if ((e = hipMalloc(&B->d_arena, B->host_arena.size() * 4)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_descs, sizeof(PairDesc) * npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_order, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
-    if ((e = hipMalloc(&B->d_queue, 64)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMalloc(&B->d_queue, 256)) != hipSuccess) return hip_fail(e, err, errlen);
     const size_t scratch_ints = (size_t)B->grid * waves * top_max;
     if ((e = hipMalloc(&B->d_scratch, scratch_ints * 4)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMemcpy(B->d_arena, B->host_arena.data(), B->host_arena.size() * 4, hipMemcpyHostToDevice)) !=
