@@ -840,7 +840,8 @@ extern "C" __global__ void lcg_reset_queue(int *queue) {
 }
 
 extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, int *scratch,
-                                BatchParams bp, int grid, int waves, size_t lds_bytes, hipStream_t stream) {
+                                BatchParams bp, int grid, int waves, size_t lds_bytes, BatchParams bp_top, int top_grid,
+                                int top_waves, size_t top_lds_bytes, hipStream_t stream) {
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void *)lcg_fill_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -856,7 +857,7 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
     hipLaunchKernelGGL(lcg_reset_queue, dim3(1), dim3(64), 0, stream, queue + 16);
     hipLaunchKernelGGL(lcg_fill_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue, arena,
                        bp);
-    hipLaunchKernelGGL(lcg_top_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue + 16,
-                       arena, scratch, bp);
+    hipLaunchKernelGGL(lcg_top_pairs, dim3(top_grid), dim3(top_waves * 64), top_lds_bytes, stream, descs, order,
+                       queue + 16, arena, scratch, bp_top);
     return (int)hipGetLastError();
 }

@@ -19,7 +19,8 @@
 using namespace locarna_amd;
 
 extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, int *scratch,
-                                BatchParams bp, int grid, int waves, size_t lds_bytes, hipStream_t stream);
+                                BatchParams bp, int grid, int waves, size_t lds_bytes, BatchParams bp_top, int top_grid,
+                                int top_waves, size_t top_lds_bytes, hipStream_t stream);
 
 struct lcg_pair {
     lcg_params params;
@@ -386,9 +387,12 @@ struct lcg_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_trace = -1;
-    BatchParams bp;
+    BatchParams bp;      // fill-D kernel
     int waves = 1, grid = 1;
     size_t lds_bytes = 0;
+    BatchParams bp_top;  // top-level / traceback kernel
+    int top_waves = 1, top_grid = 1;
+    size_t top_lds_bytes = 0;
     bool have_results = false;
     bool fetched = false;
     bool traced = false;
@@ -600,6 +604,21 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     B->bp.scratch_stride = top_max;
     B->bp.fixed_cols = 1;
     if (const char *f = std::getenv("LCG_FIXED_COLS")) B->bp.fixed_cols = std::atoi(f) != 0;
+    // The top-level / traceback kernel runs the top level on one wave: two
+    // workgroups of 8 waves per CU (half the LDS each) keep more pairs in
+    // flight, as long as every group region still fits the smaller pool.
+    int top_waves = 8, top_wgs = 2;
+    if (const char *w = std::getenv("LCG_TOP_WAVES")) top_waves = std::min(16, std::max(1, std::atoi(w)));
+    if (const char *w = std::getenv("LCG_TOP_WGS_PER_CU")) top_wgs = std::min(4, std::max(1, std::atoi(w)));
+    int top_pool = (lds_ints_cu / top_wgs - hdr) & ~3;
+    while (top_pool < gmax && top_wgs > 1) {
+        --top_wgs;
+        top_pool = (lds_ints_cu / top_wgs - hdr) & ~3;
+    }
+    B->bp_top = B->bp;
+    B->bp_top.pool_ints = top_pool;
+    B->top_waves = top_waves;
+    B->top_lds_bytes = (size_t)(hdr + top_pool) * 4;
 
     hipError_t e;
     if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, err, errlen);
@@ -607,6 +626,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
         return hip_fail(e, err, errlen);
     B->grid = std::min(npairs, cus * wgs_per_cu);
+    B->top_grid = std::min(npairs, cus * top_wgs);
     if ((e = hipStreamCreate(&B->stream)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipEventCreate(&B->ev0)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipEventCreate(&B->ev1)) != hipSuccess) return hip_fail(e, err, errlen);
@@ -614,7 +634,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     if ((e = hipMalloc(&B->d_descs, sizeof(PairDesc) * npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_order, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_queue, 256)) != hipSuccess) return hip_fail(e, err, errlen);
-    const size_t scratch_ints = (size_t)B->grid * waves * top_max;
+    const size_t scratch_ints = (size_t)B->top_grid * B->top_waves * top_max;
     if ((e = hipMalloc(&B->d_scratch, scratch_ints * 4)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMemcpy(B->d_arena, B->host_arena.data(), B->host_arena.size() * 4, hipMemcpyHostToDevice)) !=
         hipSuccess)
@@ -662,17 +682,21 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
     bp.trace = (trace & 1) ? 1 : 0;
     bp.count = (trace & 2) ? 1 : 0;
     if (mode == 2 && bp.count) return LCG_E_ARG;
+    BatchParams top = B->bp_top;
+    top.trace = bp.trace;
+    top.count = bp.count;
     HIPCHECK(hipEventRecord(B->ev0, B->stream));
     if (mode == 0) {
         int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, B->d_scratch, bp, B->grid, B->waves,
-                                  B->lds_bytes, B->stream);
+                                  B->lds_bytes, top, B->top_grid, B->top_waves, B->top_lds_bytes, B->stream);
         if (rc) return hip_fail((hipError_t)rc);
     } else if (mode == 1) {
-        BatchParams one = bp;
+        BatchParams one = bp, one_top = top;
         one.npairs = 1;
+        one_top.npairs = 1;
         for (int k = 0; k < B->npairs; ++k) {
             int rc = lcg_launch_align(B->d_descs, B->d_order + k, B->d_queue, B->d_arena, B->d_scratch, one, 1, B->waves,
-                                      B->lds_bytes, B->stream);
+                                      B->lds_bytes, one_top, 1, B->top_waves, B->top_lds_bytes, B->stream);
             if (rc) return hip_fail((hipError_t)rc);
         }
     } else {
@@ -685,7 +709,7 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
             HIPCHECK(hipStreamCreate(&cs));
             HIPCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
             int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, B->d_scratch, bp, B->grid, B->waves,
-                                      B->lds_bytes, cs);
+                                      B->lds_bytes, top, B->top_grid, B->top_waves, B->top_lds_bytes, cs);
             hipGraph_t g = nullptr;
             hipError_t ec = hipStreamEndCapture(cs, &g);
             hipStreamDestroy(cs);
