@@ -42,6 +42,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=0, help="pairs timed on the CPU (0: all)")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--sequ-local", action="store_true", help="locarna --sequ-local (sequence-local top level)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the all-pairs stage would shard "
@@ -52,20 +53,23 @@ def main():
     from locarna_amd import api, synth
 
     pps, pairs, span = synth.workload(args.workload)
-    params = api.locarna_params(max_bp_span=span)
+    params = api.locarna_params(max_bp_span=span, sequ_local=1 if args.sequ_local else 0)
     nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
     with cf.ThreadPoolExecutor(nthreads) as ex:
         prepared = list(ex.map(lambda ab: api.Pair(pps[ab[0]], pps[ab[1]], params), pairs))
     npairs = len(prepared)
     batch = api.Batch(prepared, device=torch.cuda.current_device())
 
-    # untimed counting run: DP cells and arc candidates the kernel evaluates
+    # untimed counting run: DP cells and arc candidates each kernel evaluates
     batch.run(mode=0, trace=True, count=True)
-    cells = cands = 0
+    fill_cells = fill_cands = top_cells = top_cands = 0
     for k in range(npairs):
-        c, q = batch.counters(k)
-        cells += c
-        cands += q
+        fc, fq, tc, tq = batch.counters_split(k)
+        fill_cells += fc
+        fill_cands += fq
+        top_cells += tc
+        top_cands += tq
+    cells, cands = fill_cells + top_cells, fill_cands + top_cands
     arcm = sum(p.num_arcmatches for p in prepared)
 
     for _ in range(args.warmup):
@@ -77,6 +81,15 @@ def main():
         dev_ms.append(batch.run(mode=args.mode, trace=True))
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # per-kernel device times: the same step as persistent launches (mode 0),
+    # HIP events recorded on the launch stream before, between and after the
+    # fill-D and top-level kernels
+    fill_ms, top_ms = [], []
+    for _ in range(max(1, min(args.steps, 3))):
+        batch.run(mode=0, trace=True)
+        f, t = batch.kernel_times()
+        fill_ms.append(f)
+        top_ms.append(t)
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = npairs / (ms_per_step / 1e3)
     kern_ms = sum(dev_ms) / len(dev_ms)
@@ -87,13 +100,26 @@ def main():
         plain_ms = batch.run(mode=1, trace=True)
         plain = npairs / (plain_ms / 1e3)
 
-    # integer-op roofline of lcg_align_pairs (W_int = 11 C + 2 K + A)
-    w_int = 11 * cells + 2 * cands + arcm
-    achieved = w_int / (kern_ms / 1e3) / 1e12
-    roofline = {"bound": "valu", "achieved": round(achieved, 4), "peak": round(P_INT32_TOPS, 2),
-                "unit": "TOPS", "frac": round(achieved / P_INT32_TOPS, 5), "traffic": None,
-                "kernel": "lcg_align_pairs", "kernel_ms": round(kern_ms, 3),
-                "int_ops_per_launch": w_int}
+    # integer-op roofline of the dominant kernel, lcg_fill_pairs (fill of D
+    # over all arc-match groups): W_int = 11 C + 2 K + A over its own cells
+    # C and candidates K, divided by its device time (HIP events around it
+    # on its stream)
+    w_fill = 11 * fill_cells + 2 * fill_cands + arcm
+    fill_avg = sum(fill_ms) / len(fill_ms) if fill_ms else None
+    achieved = w_fill / (fill_avg / 1e3) / 1e12 if fill_avg else None
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "lcg_fill_pairs_%s_traffic.json" % args.workload)
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "valu", "achieved": round(achieved, 4) if achieved else None,
+                "peak": round(P_INT32_TOPS, 2), "unit": "TOPS",
+                "frac": round(achieved / P_INT32_TOPS, 5) if achieved else None, "traffic": traffic,
+                "kernel": "lcg_fill_pairs", "kernel_ms": round(fill_avg, 3) if fill_avg else None,
+                "int_ops_per_launch": w_fill,
+                "other_kernel": {"name": "lcg_top_pairs",
+                                 "ms": round(sum(top_ms) / len(top_ms), 3) if top_ms else None,
+                                 "cells": top_cells, "candidates": top_cands}}
 
     cpu = None
     if args.cpu_baseline == "auto":
@@ -102,7 +128,9 @@ def main():
         idx = list(range(0, npairs, max(1, npairs // n_s)))[:n_s]
         sample = [(pps[pairs[k][0]], pps[pairs[k][1]]) for k in idx]
         t0 = time.perf_counter()
-        cpu_scores = oracle.align_many(sample, oracle.locarna_params(max_bp_span=span), nthreads=nthreads)
+        cpu_scores = oracle.align_many(sample, oracle.locarna_params(max_bp_span=span,
+                                                                     sequ_local=1 if args.sequ_local else 0),
+                                       nthreads=nthreads)
         dt = time.perf_counter() - t0
         exact = sum(1 for k, s in zip(idx, cpu_scores) if gpu_scores[k] == s)
         cpu = {"value": round(len(sample) / dt, 3), "unit": "pairs/s", "cores": nthreads, "kind": "port",
@@ -116,6 +144,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
         "config": {"workload": args.workload, "pairs": npairs, "max_bp_span": span,
+                   "mode": "sequence-local" if args.sequ_local else "global",
                    "launch": {0: "persistent", 2: "hipGraph"}.get(args.mode, str(args.mode)),
                    "waves_per_workgroup": batch.waves, "workgroups": batch.workgroups,
                    "lds_bytes": batch.lds_bytes},

@@ -105,6 +105,14 @@ int lcg_batch_D(const lcg_batch *batch, int32_t k, int64_t *D, int32_t cap);
 /* Work counted by the last run with trace bit 1 set: DP cells and arc-match
  * candidates evaluated by the fill of D and the top level of pair k. */
 int lcg_batch_counters(const lcg_batch *batch, int32_t k, int64_t *cells, int64_t *candidates);
+/* The same split by kernel: the fill of D (all arc-match groups) and the top
+ * level (no counterpart in the reference; for the work figures of a
+ * roofline). */
+int lcg_batch_counters_split(const lcg_batch *batch, int32_t k, int64_t *fill_cells, int64_t *fill_candidates,
+                             int64_t *top_cells, int64_t *top_candidates);
+/* Device times of the fill-D kernel and of the top-level/traceback kernel in
+ * the last mode-0 run (no counterpart in the reference). */
+int lcg_batch_kernel_times(const lcg_batch *batch, float *fill_ms, float *top_ms);
 
 /* ---- LocARNA-P (FP64): inside partition function and its adjoint.
  * Score parameters theta (LCG_NTHETA = 275 doubles): base-match scores of

@@ -66,6 +66,8 @@ def _sig(L):
                                       c.c_char_p, c.c_char_p, c.c_int32]
     L.lcg_batch_D.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int32]
     L.lcg_batch_counters.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_int64), c.POINTER(c.c_int64)]
+    L.lcg_batch_counters_split.argtypes = [c.c_void_p, c.c_int32] + [c.POINTER(c.c_int64)] * 4
+    L.lcg_batch_kernel_times.argtypes = [c.c_void_p, c.POINTER(c.c_float), c.POINTER(c.c_float)]
     L.lcg_batch_pf_run.argtypes = [c.c_void_p, c.c_void_p, c.POINTER(c.c_float)]
     L.lcg_batch_pf_result.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_double), c.POINTER(c.c_double)]
     L.lcg_batch_am_probs.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int32]
@@ -182,6 +184,22 @@ class Batch:
         if rc != 0:
             raise LcgError("lcg_batch_counters (%d)" % rc)
         return c.value, q.value
+
+    def counters_split(self, k):
+        """(fill cells, fill candidates, top cells, top candidates)"""
+        v = [ctypes.c_int64() for _ in range(4)]
+        rc = lib().lcg_batch_counters_split(self.h, k, *[ctypes.byref(x) for x in v])
+        if rc != 0:
+            raise LcgError("lcg_batch_counters_split (%d)" % rc)
+        return tuple(x.value for x in v)
+
+    def kernel_times(self):
+        """(fill-D ms, top-level/traceback ms) of the last mode-0 run"""
+        f, t = ctypes.c_float(), ctypes.c_float()
+        rc = lib().lcg_batch_kernel_times(self.h, ctypes.byref(f), ctypes.byref(t))
+        if rc != 0:
+            raise LcgError("lcg_batch_kernel_times (%d)" % rc)
+        return f.value, t.value
 
     def D(self, k):
         n = max(self.pairs[k].num_arcmatches, 1)

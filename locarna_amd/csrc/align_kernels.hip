@@ -752,8 +752,9 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
             for (int t = threadIdx.x; t < n + m; t += blockDim.x) arena[d.structs + t] = 0;
         }
         if (threadIdx.x == 0) s_tcount = 0;
+        if (bp.count && threadIdx.x < 4) arena[d.out + 8 + threadIdx.x] = 0;
         const Pair P = load_pair(d, arena, lds, bp);
-        unsigned long long *counts = bp.count ? (unsigned long long *)(arena + d.out + 4) : nullptr;
+        unsigned long long *counts = bp.count ? (unsigned long long *)(arena + d.out + 8) : nullptr;  // top level
         __threadfence();
         __syncthreads();
         // ---- top level (wave 0)
@@ -841,7 +842,7 @@ extern "C" __global__ void lcg_reset_queue(int *queue) {
 
 extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, int *scratch,
                                 BatchParams bp, int grid, int waves, size_t lds_bytes, BatchParams bp_top, int top_grid,
-                                int top_waves, size_t top_lds_bytes, hipStream_t stream) {
+                                int top_waves, size_t top_lds_bytes, hipEvent_t mid, hipStream_t stream) {
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void *)lcg_fill_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -857,6 +858,7 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
     hipLaunchKernelGGL(lcg_reset_queue, dim3(1), dim3(64), 0, stream, queue + 16);
     hipLaunchKernelGGL(lcg_fill_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue, arena,
                        bp);
+    if (mid) hipEventRecord(mid, stream);
     hipLaunchKernelGGL(lcg_top_pairs, dim3(top_grid), dim3(top_waves * 64), top_lds_bytes, stream, descs, order,
                        queue + 16, arena, scratch, bp_top);
     return (int)hipGetLastError();

@@ -20,7 +20,7 @@ using namespace locarna_amd;
 
 extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, int *scratch,
                                 BatchParams bp, int grid, int waves, size_t lds_bytes, BatchParams bp_top, int top_grid,
-                                int top_waves, size_t top_lds_bytes, hipStream_t stream);
+                                int top_waves, size_t top_lds_bytes, hipEvent_t mid, hipStream_t stream);
 
 struct lcg_pair {
     lcg_params params;
@@ -259,10 +259,11 @@ void pack(lcg_pair &P) {
     }
     d.wtop = wtop;
     d.top_ints = ((m + 1) + wtop + (n + 1) * wtop + 3) & ~3;
-    // out[4..7] holds two uint64 counters: keep them 8-byte aligned (every
+    // out[4..11] holds four uint64 counters (cells and candidates of the
+    // fill-D kernel, then of the top-level kernel): keep them 8-byte aligned (every
     // blob is a multiple of 4 ints, so blob offsets survive concatenation)
     if (blob.size() & 1) blob.push_back(0);
-    d.out = reserve(blob, 8);
+    d.out = reserve(blob, 12);
     while (blob.size() & 3) blob.push_back(0);
     d.gmax = gbuf;
     P.desc = d;
@@ -383,7 +384,8 @@ struct lcg_batch {
     int32_t *d_queue = nullptr;
     int32_t *d_scratch = nullptr;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_mid = nullptr;
+    float fill_ms = -1, top_ms = -1;  // per-kernel times of the last mode-0 run
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_trace = -1;
@@ -630,6 +632,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     if ((e = hipStreamCreate(&B->stream)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipEventCreate(&B->ev0)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipEventCreate(&B->ev1)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipEventCreate(&B->ev_mid)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_arena, B->host_arena.size() * 4)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_descs, sizeof(PairDesc) * npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_order, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
@@ -663,6 +666,7 @@ extern "C" void lcg_batch_destroy(lcg_batch *B) {
     if (B->d_pfscratch) hipFree(B->d_pfscratch);
     if (B->ev0) hipEventDestroy(B->ev0);
     if (B->ev1) hipEventDestroy(B->ev1);
+    if (B->ev_mid) hipEventDestroy(B->ev_mid);
     if (B->stream) hipStreamDestroy(B->stream);
     delete B;
 }
@@ -688,7 +692,7 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
     HIPCHECK(hipEventRecord(B->ev0, B->stream));
     if (mode == 0) {
         int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, B->d_scratch, bp, B->grid, B->waves,
-                                  B->lds_bytes, top, B->top_grid, B->top_waves, B->top_lds_bytes, B->stream);
+                                  B->lds_bytes, top, B->top_grid, B->top_waves, B->top_lds_bytes, B->ev_mid, B->stream);
         if (rc) return hip_fail((hipError_t)rc);
     } else if (mode == 1) {
         BatchParams one = bp, one_top = top;
@@ -696,7 +700,7 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
         one_top.npairs = 1;
         for (int k = 0; k < B->npairs; ++k) {
             int rc = lcg_launch_align(B->d_descs, B->d_order + k, B->d_queue, B->d_arena, B->d_scratch, one, 1, B->waves,
-                                      B->lds_bytes, one_top, 1, B->top_waves, B->top_lds_bytes, B->stream);
+                                      B->lds_bytes, one_top, 1, B->top_waves, B->top_lds_bytes, nullptr, B->stream);
             if (rc) return hip_fail((hipError_t)rc);
         }
     } else {
@@ -709,7 +713,7 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
             HIPCHECK(hipStreamCreate(&cs));
             HIPCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
             int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, B->d_scratch, bp, B->grid, B->waves,
-                                      B->lds_bytes, top, B->top_grid, B->top_waves, B->top_lds_bytes, cs);
+                                      B->lds_bytes, top, B->top_grid, B->top_waves, B->top_lds_bytes, nullptr, cs);
             hipGraph_t g = nullptr;
             hipError_t ec = hipStreamEndCapture(cs, &g);
             hipStreamDestroy(cs);
@@ -724,6 +728,12 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
     HIPCHECK(hipEventRecord(B->ev1, B->stream));
     HIPCHECK(hipEventSynchronize(B->ev1));
     if (device_ms) HIPCHECK(hipEventElapsedTime(device_ms, B->ev0, B->ev1));
+    if (mode == 0) {
+        HIPCHECK(hipEventElapsedTime(&B->fill_ms, B->ev0, B->ev_mid));
+        HIPCHECK(hipEventElapsedTime(&B->top_ms, B->ev_mid, B->ev1));
+    } else {
+        B->fill_ms = B->top_ms = -1;
+    }
     B->have_results = true;
     B->fetched = false;
     B->traced = bp.trace != 0;
@@ -794,10 +804,35 @@ extern "C" int lcg_batch_counters(const lcg_batch *B, int32_t k, int64_t *cells,
     if (int rc = fetch(B)) return rc;
     const PairDesc &d = B->descs[k];
     uint64_t c0, c1;
+    uint64_t c2, c3;
     std::memcpy(&c0, &B->result[d.out + 4], 8);
     std::memcpy(&c1, &B->result[d.out + 6], 8);
-    if (cells) *cells = (int64_t)c0;
-    if (candidates) *candidates = (int64_t)c1;
+    std::memcpy(&c2, &B->result[d.out + 8], 8);
+    std::memcpy(&c3, &B->result[d.out + 10], 8);
+    if (cells) *cells = (int64_t)(c0 + c2);
+    if (candidates) *candidates = (int64_t)(c1 + c3);
+    return LCG_OK;
+}
+
+extern "C" int lcg_batch_counters_split(const lcg_batch *B, int32_t k, int64_t *fill_cells, int64_t *fill_candidates,
+                                        int64_t *top_cells, int64_t *top_candidates) {
+    if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    if (int rc = fetch(B)) return rc;
+    const PairDesc &d = B->descs[k];
+    uint64_t c[4];
+    for (int t = 0; t < 4; ++t) std::memcpy(&c[t], &B->result[d.out + 4 + 2 * t], 8);
+    if (fill_cells) *fill_cells = (int64_t)c[0];
+    if (fill_candidates) *fill_candidates = (int64_t)c[1];
+    if (top_cells) *top_cells = (int64_t)c[2];
+    if (top_candidates) *top_candidates = (int64_t)c[3];
+    return LCG_OK;
+}
+
+extern "C" int lcg_batch_kernel_times(const lcg_batch *B, float *fill_ms, float *top_ms) {
+    if (!B) return LCG_E_ARG;
+    if (B->fill_ms < 0) return LCG_E_STATE;
+    if (fill_ms) *fill_ms = B->fill_ms;
+    if (top_ms) *top_ms = B->top_ms;
     return LCG_OK;
 }
 

@@ -42,7 +42,8 @@
 //             cell (i, j): (a << 16) | b with 0 for a gap side, -1 if empty
 //   structA/B: n + m ints (output): 0 '.', 1 '(', 2 ')'
 //   tstack:   2 * (n + 1) ints scratch (traceback work list)
-//   out:      [score, max_i, max_j, status, cells (u64), candidates (u64)]
+//   out:      [score, max_i, max_j, status, fill-D cells, candidates (u64),
+//             top-level cells, candidates (u64)]
 //
 // A group region in LDS (or scratch) is [E: br-bl][S: W][M: (ar-al) * W]:
 // E the deletion vector over columns bl..br-1, S the arc-candidate maxima of
@@ -80,7 +81,7 @@ typedef struct {
     int32_t hdr_ints;        // per-pair LDS header (band, codes, gaps, sigma)
     int32_t pool_ints;       // LDS pool for group regions
     int32_t max_len;         // max sequence length over the batch
-    int32_t count;           // count DP cells and arc candidates (out[4..7])
+    int32_t count;           // count DP cells and arc candidates (out[4..11])
     int32_t scratch_stride;  // ints of global scratch per (workgroup, wave)
     int32_t fixed_cols;      // fill narrow groups with the fixed-column kernel
     int32_t fptr_lds;        // LDS offsets (ints) of the pair's fptr and fblk

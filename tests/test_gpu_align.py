@@ -69,6 +69,13 @@ def test_allpairs_subset_local():
     _run_set([(pps[a], pps[b]) for a, b in pairs], dict(sequ_local=1))
 
 
+@pytest.mark.parametrize("local", [0, 1])
+def test_long500(local):
+    """Four pairs of 500-nt RNAs, max-bp-span 200, global and sequence-local."""
+    pps, pairs, span = synth.workload("long500")
+    _run_set([(pps[a], pps[b]) for a, b in pairs], dict(max_bp_span=span, sequ_local=local), modes=(0, 2))
+
+
 def test_counters_match_oracle():
     """The kernel's own count of DP cells and arc candidates equals the
     oracle's (C and K of the work model), and the host's cell estimate."""
