@@ -444,8 +444,10 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
 }
 
 // Fill of an arc-match group (fill-D and trace re-fills), by the fixed-column
-// kernel when the group fits it, else by the general one.
-template <bool GLOBAL>
+// kernel when the group fits it, else by the general one.  WIDE enables the
+// 4-chunk variant (columns up to 256), left out of the traceback kernel for
+// its register budget.
+template <bool GLOBAL, bool WIDE = true>
 __device__ void fill_group(const Pair &P, int al, int bl, int ar, int br, int W, int *base,
                            unsigned long long *counts, bool fixed_cols) {
     const int cols = br - bl - 1, rows = ar - al - 1;
@@ -455,6 +457,8 @@ __device__ void fill_group(const Pair &P, int al, int bl, int ar, int br, int W,
         group_fill_fixed<GLOBAL, 1, 2>(P, al, bl, ar, br, W, base, counts);
     else if (cols <= 128 && rows <= 128)
         group_fill_fixed<GLOBAL, 2, 2>(P, al, bl, ar, br, W, base, counts);
+    else if (WIDE && cols <= 256 && rows <= 256)
+        group_fill_fixed<GLOBAL, 4, 4>(P, al, bl, ar, br, W, base, counts);
     else
         group_fill<GLOBAL, false>(P, al, bl, ar, br, W, base, true, nullptr, counts);
 }
@@ -615,7 +619,7 @@ __device__ __forceinline__ int region_width(const Pair &P, int al, int bl, int a
 template <bool GLOBAL>
 __device__ __forceinline__ void fill_and_trace(const Pair &P, int al, int bl, int ar, int br, int W, int *base, int *tstack,
                                int *tcount) {
-    fill_group<GLOBAL>(P, al, bl, ar, br, W, base, nullptr, true);
+    fill_group<GLOBAL, false>(P, al, bl, ar, br, W, base, nullptr, true);
     group_trace<GLOBAL>(P, base + (br - bl) + W, W, al, bl, br, ar - 1, br - 1, false, false, tstack, tcount);
 }
 
@@ -673,30 +677,63 @@ __device__ __forceinline__ Pair load_pair(const PairDesc &d, int *arena, int *ld
 }
 
 // Kernel 1: fill D of every pair (rows al = n .. 1, the groups of a row
-// packed into LDS rounds by the host).
+// packed into LDS rounds by the host).  With K = bp.wg_per_pair > 1 (batches
+// of fewer pairs than CUs), K workgroups share each pair: workgroup
+// blockIdx.x serves pair blockIdx.x / K as member blockIdx.x % K, takes the
+// groups the host assigned to that member, and the K members meet at a
+// bounded barrier (a global arrival counter) after every row with groups.
+// Group schedule word G[5] = round | wave << 12 | member << 16.
+#define LCG_SPIN_LIMIT (1 << 22)
+
 extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc *__restrict__ descs,
                                                                   const int *__restrict__ order, int *queue,
-                                                                  int *arena, BatchParams bp) {
+                                                                  int *arena, int *barriers, BatchParams bp) {
     extern __shared__ int lds[];
     __shared__ int s_pair;
     const int wave = threadIdx.x >> 6;
     const int lane = lane_id();
     int *pool = lds + bp.hdr_ints;
-    for (;;) {
-        if (threadIdx.x == 0) s_pair = atomicAdd(queue, 1);
+    const int K = bp.wg_per_pair;
+    const int member = K > 1 ? (int)blockIdx.x % K : 0;
+    for (int iter = 0;; ++iter) {
+        if (K > 1) {
+            if (iter > 0) break;
+            if (threadIdx.x == 0) s_pair = (int)blockIdx.x / K;
+        } else if (threadIdx.x == 0) {
+            s_pair = atomicAdd(queue, 1);
+        }
         __syncthreads();
         const int k = s_pair;
         if (k >= bp.npairs) break;
         const PairDesc d = descs[order[k]];
-        if (bp.count && threadIdx.x < 4) arena[d.out + 4 + threadIdx.x] = 0;
+        if (member == 0) {
+            if (bp.count && threadIdx.x < 4) arena[d.out + 4 + threadIdx.x] = 0;
+            if (threadIdx.x == 0) arena[d.out + 3] = 0;  // status
+        }
         const Pair P = load_pair(d, arena, lds, bp);
         unsigned long long *counts = bp.count ? (unsigned long long *)(arena + d.out + 4) : nullptr;
+        if (K > 1 && counts) {
+            // members count into the pair's words after member 0 cleared them
+            if (threadIdx.x == 0) {
+                __threadfence();
+                atomicAdd(&barriers[k], 1);
+                for (int spin = 0; atomicAdd(&barriers[k], 0) < K; ++spin) {
+                    if (spin > LCG_SPIN_LIMIT) {
+                        atomicExch(&arena[d.out + 3], 2);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+        }
+        int passed = (K > 1 && counts) ? 1 : 0;  // barrier generations passed
         __threadfence_block();
         __syncthreads();
         const int *rows = arena + d.rows;
         const int *rounds = arena + d.rounds;
         const int *groups = arena + d.groups;
         const int4 *gams = reinterpret_cast<const int4 *>(arena + d.gams);
+        const int mine = member << 16;
         for (int al = d.n; al >= 1; --al) {
             const int g0 = rows[al + 1], g1 = rows[al];
             if (g0 == g1) continue;
@@ -704,7 +741,7 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc
             for (int rnd = 0; rnd < nr; ++rnd) {
                 for (int g = g0; g < g1; ++g) {
                     const int *G = groups + g * LCG_GROUP_INTS;
-                    if (G[5] != (rnd | (wave << 16))) continue;
+                    if (G[5] != (rnd | (wave << 12) | mine)) continue;
                     const int gal = G[0] & 0xffff, gbl = G[0] >> 16;
                     const int gar = G[1] & 0xffff, gbr = G[1] >> 16;
                     const int W = G[2];
@@ -723,9 +760,33 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc
                 __threadfence_block();
                 __syncthreads();
             }
+            if (K > 1) {
+                // every member's D of row al is written before any member
+                // reads it in a later row: each wave completes its stores at
+                // device scope, then thread 0 arrives and waits for all
+                __threadfence();
+                __syncthreads();
+                ++passed;
+                if (threadIdx.x == 0) {
+                    atomicAdd(&barriers[k], 1);
+                    for (int spin = 0; atomicAdd(&barriers[k], 0) < K * passed; ++spin) {
+                        if (spin > LCG_SPIN_LIMIT) {
+                            atomicExch(&arena[d.out + 3], 2);
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(2);
+                    }
+                }
+                __syncthreads();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            }
         }
         __syncthreads();
     }
+}
+
+extern "C" __global__ void lcg_reset_barriers(int *barriers, int n) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) barriers[t] = 0;
 }
 
 // Kernel 2: top level (global or sequence-local) and traceback of every
@@ -841,8 +902,8 @@ extern "C" __global__ void lcg_reset_queue(int *queue) {
 }
 
 extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, int *scratch,
-                                BatchParams bp, int grid, int waves, size_t lds_bytes, BatchParams bp_top, int top_grid,
-                                int top_waves, size_t top_lds_bytes, hipEvent_t mid, hipStream_t stream) {
+                                int *barriers, BatchParams bp, int grid, int waves, size_t lds_bytes, BatchParams bp_top,
+                                int top_grid, int top_waves, size_t top_lds_bytes, hipEvent_t mid, hipStream_t stream) {
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void *)lcg_fill_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -853,11 +914,14 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
         if (e != hipSuccess) return (int)e;
         attr_set = true;
     }
-    // two work counters, 64 bytes apart
+    // two work counters, 64 bytes apart; the per-pair barrier counters of
+    // the multi-workgroup fill
     hipLaunchKernelGGL(lcg_reset_queue, dim3(1), dim3(64), 0, stream, queue);
     hipLaunchKernelGGL(lcg_reset_queue, dim3(1), dim3(64), 0, stream, queue + 16);
+    if (bp.wg_per_pair > 1)
+        hipLaunchKernelGGL(lcg_reset_barriers, dim3(1), dim3(256), 0, stream, barriers, bp.npairs);
     hipLaunchKernelGGL(lcg_fill_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue, arena,
-                       bp);
+                       barriers, bp);
     if (mid) hipEventRecord(mid, stream);
     hipLaunchKernelGGL(lcg_top_pairs, dim3(top_grid), dim3(top_waves * 64), top_lds_bytes, stream, descs, order,
                        queue + 16, arena, scratch, bp_top);

@@ -19,7 +19,8 @@
 using namespace locarna_amd;
 
 extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, int *scratch,
-                                BatchParams bp, int grid, int waves, size_t lds_bytes, BatchParams bp_top, int top_grid,
+                                int *barriers, BatchParams bp, int grid, int waves, size_t lds_bytes, BatchParams bp_top,
+                                int top_grid,
                                 int top_waves, size_t top_lds_bytes, hipEvent_t mid, hipStream_t stream);
 
 struct lcg_pair {
@@ -382,6 +383,8 @@ struct lcg_batch {
     PairDesc *d_descs = nullptr;
     int32_t *d_order = nullptr;
     int32_t *d_queue = nullptr;
+    int32_t *d_barriers = nullptr;  // per-pair arrival counters (multi-workgroup fill)
+    int wg_per_pair = 1;
     int32_t *d_scratch = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_mid = nullptr;
@@ -563,7 +566,19 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
         return LCG_E_CAPACITY;
     }
     // per-row LDS schedule: groups of a row, largest first, into rounds of at
-    // most `waves` regions that fit the pool together
+    // most `waves` regions that fit the pool together.  With K workgroups
+    // per pair (fewer pairs than CUs), the groups of a row are first dealt to
+    // the K members, longest first to the least loaded member.
+    int cus = 0;
+    {
+        hipError_t e0 = hipSetDevice(device);
+        if (e0 == hipSuccess) e0 = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
+        if (e0 != hipSuccess) return hip_fail(e0, err, errlen);
+    }
+    int K = 1;
+    if (npairs < cus) K = std::min(64, cus / npairs);
+    if (const char *w = std::getenv("LCG_WG_PER_PAIR")) K = std::min(64, std::max(1, std::atoi(w)));
+    K = std::min(K, std::max(1, cus / npairs));  // all workgroups of the fill stay co-resident
     for (int k = 0; k < npairs; ++k) {
         const PairDesc &d = B->descs[k];
         int32_t *A = B->host_arena.data();
@@ -571,26 +586,51 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
             const int g0 = A[d.rows + al + 1], g1 = A[d.rows + al];
             std::vector<int> idx;
             for (int g = g0; g < g1; ++g) idx.push_back(g);
-            std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
-                return A[d.groups + x * LCG_GROUP_INTS + 7] > A[d.groups + y * LCG_GROUP_INTS + 7];
-            });
-            std::vector<int> used, cnt;
-            for (int g : idx) {
-                int32_t *G = A + d.groups + g * LCG_GROUP_INTS;
-                size_t r = 0;
-                while (r < used.size() && (cnt[r] >= waves || used[r] + G[7] > pool)) ++r;
-                if (r == used.size()) {
-                    used.push_back(0);
-                    cnt.push_back(0);
+            auto rows_of = [&](int g) {
+                const int32_t *G = A + d.groups + g * LCG_GROUP_INTS;
+                return (G[1] & 0xffff) - (G[0] & 0xffff);
+            };
+            std::vector<std::vector<int>> mem(K);
+            {
+                std::vector<int> by_len = idx;
+                std::stable_sort(by_len.begin(), by_len.end(), [&](int x, int y) { return rows_of(x) > rows_of(y); });
+                std::vector<long> load(K, 0);
+                for (int g : by_len) {
+                    const int m = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                    mem[m].push_back(g);
+                    load[m] += rows_of(g);
                 }
-                G[5] = (int)r | (cnt[r] << 16);
-                G[6] = used[r];
-                used[r] += G[7];
-                cnt[r] += 1;
             }
-            A[d.rounds + al] = (int)used.size();
+            int nr = 0;
+            for (int m = 0; m < K; ++m) {
+                auto &lst = mem[m];
+                std::stable_sort(lst.begin(), lst.end(), [&](int x, int y) {
+                    return A[d.groups + x * LCG_GROUP_INTS + 7] > A[d.groups + y * LCG_GROUP_INTS + 7];
+                });
+                std::vector<int> used, cnt;
+                for (int g : lst) {
+                    int32_t *G = A + d.groups + g * LCG_GROUP_INTS;
+                    size_t r = 0;
+                    while (r < used.size() && (cnt[r] >= waves || used[r] + G[7] > pool)) ++r;
+                    if (r == used.size()) {
+                        used.push_back(0);
+                        cnt.push_back(0);
+                    }
+                    G[5] = (int)r | (cnt[r] << 12) | (m << 16);
+                    G[6] = used[r];
+                    used[r] += G[7];
+                    cnt[r] += 1;
+                }
+                nr = std::max(nr, (int)used.size());
+            }
+            if (nr >= 4096) {
+                set_err(err, errlen, "too many LDS rounds in one row");
+                return LCG_E_CAPACITY;
+            }
+            A[d.rounds + al] = nr;
         }
     }
+    B->wg_per_pair = K;
     B->waves = waves;
     B->lds_bytes = (size_t)(hdr + pool) * 4;
     B->bp.alpha = p0.indel_opening;
@@ -604,6 +644,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     B->bp.max_len = max_len;
     B->bp.count = 0;
     B->bp.scratch_stride = top_max;
+    B->bp.wg_per_pair = K;
     B->bp.fixed_cols = 1;
     if (const char *f = std::getenv("LCG_FIXED_COLS")) B->bp.fixed_cols = std::atoi(f) != 0;
     // The top-level / traceback kernel runs the top level on one wave: two
@@ -624,10 +665,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
 
     hipError_t e;
     if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, err, errlen);
-    int cus = 0;
-    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
-        return hip_fail(e, err, errlen);
-    B->grid = std::min(npairs, cus * wgs_per_cu);
+    B->grid = K > 1 ? npairs * K : std::min(npairs, cus * wgs_per_cu);
     B->top_grid = std::min(npairs, cus * top_wgs);
     if ((e = hipStreamCreate(&B->stream)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipEventCreate(&B->ev0)) != hipSuccess) return hip_fail(e, err, errlen);
@@ -637,6 +675,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     if ((e = hipMalloc(&B->d_descs, sizeof(PairDesc) * npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_order, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_queue, 256)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMalloc(&B->d_barriers, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     const size_t scratch_ints = (size_t)B->top_grid * B->top_waves * top_max;
     if ((e = hipMalloc(&B->d_scratch, scratch_ints * 4)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMemcpy(B->d_arena, B->host_arena.data(), B->host_arena.size() * 4, hipMemcpyHostToDevice)) !=
@@ -660,6 +699,7 @@ extern "C" void lcg_batch_destroy(lcg_batch *B) {
     if (B->d_descs) hipFree(B->d_descs);
     if (B->d_order) hipFree(B->d_order);
     if (B->d_queue) hipFree(B->d_queue);
+    if (B->d_barriers) hipFree(B->d_barriers);
     if (B->d_scratch) hipFree(B->d_scratch);
     if (B->d_pfa) hipFree(B->d_pfa);
     if (B->d_pdescs) hipFree(B->d_pdescs);
@@ -691,7 +731,7 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
     top.count = bp.count;
     HIPCHECK(hipEventRecord(B->ev0, B->stream));
     if (mode == 0) {
-        int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, B->d_scratch, bp, B->grid, B->waves,
+        int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, B->d_scratch, B->d_barriers, bp, B->grid, B->waves,
                                   B->lds_bytes, top, B->top_grid, B->top_waves, B->top_lds_bytes, B->ev_mid, B->stream);
         if (rc) return hip_fail((hipError_t)rc);
     } else if (mode == 1) {
@@ -699,7 +739,7 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
         one.npairs = 1;
         one_top.npairs = 1;
         for (int k = 0; k < B->npairs; ++k) {
-            int rc = lcg_launch_align(B->d_descs, B->d_order + k, B->d_queue, B->d_arena, B->d_scratch, one, 1, B->waves,
+            int rc = lcg_launch_align(B->d_descs, B->d_order + k, B->d_queue, B->d_arena, B->d_scratch, B->d_barriers, one, B->wg_per_pair, B->waves,
                                       B->lds_bytes, one_top, 1, B->top_waves, B->top_lds_bytes, nullptr, B->stream);
             if (rc) return hip_fail((hipError_t)rc);
         }
@@ -712,7 +752,7 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
             hipStream_t cs;
             HIPCHECK(hipStreamCreate(&cs));
             HIPCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
-            int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, B->d_scratch, bp, B->grid, B->waves,
+            int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, B->d_scratch, B->d_barriers, bp, B->grid, B->waves,
                                       B->lds_bytes, top, B->top_grid, B->top_waves, B->top_lds_bytes, nullptr, cs);
             hipGraph_t g = nullptr;
             hipError_t ec = hipStreamEndCapture(cs, &g);
@@ -750,6 +790,9 @@ int fetch(const lcg_batch *Bc) {
     B->result.resize(B->host_arena.size());
     HIPCHECK(hipMemcpy(B->result.data(), B->d_arena, B->result.size() * 4, hipMemcpyDeviceToHost));
     B->fetched = true;
+    // a bounded wait of the multi-workgroup fill that expired invalidates the run
+    for (const PairDesc &d : B->descs)
+        if (B->result[d.out + 3] != 0) return LCG_E_STATE;
     return LCG_OK;
 }
 }  // namespace

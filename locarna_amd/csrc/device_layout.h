@@ -15,7 +15,8 @@
 //                                       (groups are stored al descending)
 //   rounds:   n+2                       LDS packing rounds per row
 //   groups:   LCG_GROUP_INTS per group  (al | bl<<16, ar | br<<16, W,
-//                                        am_begin, am_end, round | wave<<16,
+//                                        am_begin, am_end,
+//                                        round | wave<<12 | member<<16,
 //                                        lds offset, lds ints) where ar, br are
 //                                       the maximal right ends and W the
 //                                       stored row width of the group region
@@ -42,7 +43,8 @@
 //             cell (i, j): (a << 16) | b with 0 for a gap side, -1 if empty
 //   structA/B: n + m ints (output): 0 '.', 1 '(', 2 ')'
 //   tstack:   2 * (n + 1) ints scratch (traceback work list)
-//   out:      [score, max_i, max_j, status, fill-D cells, candidates (u64),
+//   out:      [score, max_i, max_j, status (0 ok, 2 a bounded wait of the
+//             multi-workgroup fill expired), fill-D cells, candidates (u64),
 //             top-level cells, candidates (u64)]
 //
 // A group region in LDS (or scratch) is [E: br-bl][S: W][M: (ar-al) * W]:
@@ -86,5 +88,6 @@ typedef struct {
     int32_t fixed_cols;      // fill narrow groups with the fixed-column kernel
     int32_t fptr_lds;        // LDS offsets (ints) of the pair's fptr and fblk
     int32_t fblk_lds;
-    int32_t pad[4];
+    int32_t wg_per_pair;     // workgroups sharing one pair in the fill of D
+    int32_t pad[3];
 } BatchParams;
