@@ -247,6 +247,8 @@ void pack(lcg_pair &P) {
         gams[q + 1] = entry_of[gams[q + 2]];
         gams[q + 3] = fentry_of[gams[q + 2]];
     }
+    d.f2c = reserve(blob, nam);
+    for (int k = 0; k < nam; ++k) blob[d.f2c + fentry_of[k]] = entry_of[k];
     P.entry_of = entry_of;
     std::copy(gams.begin(), gams.end(), blob.begin() + d.gams);
     d.D = reserve(blob, nam);
@@ -407,6 +409,7 @@ struct lcg_batch {
     std::vector<PfDesc> pdescs;
     std::vector<double> pf_arena;
     double *d_pfa = nullptr;
+    int32_t *d_pfsched = nullptr;  // LocARNA-P group schedule (rounds, LDS offsets)
     PfDesc *d_pdescs = nullptr;
     double *d_pfscratch = nullptr;
     int pf_grid = 0, pf_waves = 8;
@@ -529,7 +532,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
         PairDesc d = P->desc;
         int32_t *offs[] = {&d.band, &d.codes, &d.gaps, &d.sigma, &d.rows, &d.groups, &d.gams, &d.amscore,
                            &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.out, &d.rounds,
-                           &d.fptr, &d.fblk, &d.fcand};
+                           &d.fptr, &d.fblk, &d.fcand, &d.f2c};
         for (int32_t *o : offs) *o += base;
         B->descs.push_back(d);
     }
@@ -704,6 +707,7 @@ extern "C" void lcg_batch_destroy(lcg_batch *B) {
     if (B->d_pfa) hipFree(B->d_pfa);
     if (B->d_pdescs) hipFree(B->d_pdescs);
     if (B->d_pfscratch) hipFree(B->d_pfscratch);
+    if (B->d_pfsched) hipFree(B->d_pfsched);
     if (B->ev0) hipEventDestroy(B->ev0);
     if (B->ev1) hipEventDestroy(B->ev1);
     if (B->ev_mid) hipEventDestroy(B->ev_mid);
@@ -882,8 +886,8 @@ extern "C" int lcg_batch_kernel_times(const lcg_batch *B, float *fill_ms, float 
 // ------------------------------------------------------------ LocARNA-P
 
 extern "C" int lcg_launch_pf(const PairDesc *descs, const PfDesc *pdescs, const int *order, int *queue, int *arena,
-                             double *pfa, double *scratch, PfBatchParams bp, int grid, int waves, size_t lds_bytes,
-                             hipStream_t stream);
+                             double *pfa, double *scratch, const int *sched, PfBatchParams bp, int grid, int waves,
+                             size_t lds_bytes, hipStream_t stream);
 
 namespace {
 int pf_setup(lcg_batch &B) {
@@ -912,6 +916,7 @@ int pf_setup(lcg_batch &B) {
         f.edge_ref = take(h.ncells);
         f.scal = take(4);
         f.out = take(4);
+        f.fD = take(h.nam);
         max_len = std::max(max_len, std::max(h.n, h.m));
         const PairDesc &d = B.descs[k];
         const size_t top_cells = (size_t)(d.n + 1) * d.wtop;
@@ -919,21 +924,90 @@ int pf_setup(lcg_batch &B) {
         // groups are no larger than the top level region
     }
     B.pf_arena.assign(off, 0.0);
-    B.pf_waves = 8;
-    if (const char *w = std::getenv("LCG_PF_WAVES")) B.pf_waves = std::min(8, std::max(1, std::atoi(w)));
+    B.pf_waves = 16;
+    if (const char *w = std::getenv("LCG_PF_WAVES")) B.pf_waves = std::min(16, std::max(1, std::atoi(w)));
+    int nfblk_max = 1;
+    for (int k = 0; k < np; ++k) nfblk_max = std::max(nfblk_max, B.descs[k].nfblk);
     B.pbp.npairs = np;
     B.pbp.max_len = max_len;
+    B.pbp.nfblk_max = nfblk_max;
+    // LDS: int header (band, cell offsets, codes, fill blocks), double
+    // header (exp gap scores, exp sigma), then the pool of group regions
+    const int hdr_ints = (6 * max_len + 8 + 2 * nfblk_max + 1) & ~1;
+    const int hdr_doubles = 2 * (max_len + 1) + LCG_NCODES * LCG_NCODES;
+    const int lds_total = 160 * 1024 - 512;
+    const int pool = (lds_total - 4 * hdr_ints - 8 * hdr_doubles) / 8;
+    if (pool < 64) return LCG_E_CAPACITY;
+    B.pbp.pool = pool;
+    B.pf_lds = (size_t)4 * hdr_ints + (size_t)8 * (hdr_doubles + pool);
+    // schedule: per row, the groups in rounds of at most pf_waves; narrow
+    // groups (columns and rows at most 128) get LDS regions that fit the pool
+    // together, the others run in per-wave global scratch (offset -1)
+    std::vector<int32_t> sched;
+    size_t max_fixed = 0;
+    for (int k = 0; k < np; ++k) {
+        const PairDesc &d = B.descs[k];
+        PfDesc &f = B.pdescs[k];
+        const int32_t *A = B.host_arena.data();
+        f.sched = (int32_t)sched.size();
+        sched.resize(sched.size() + 4 * (size_t)d.ngroups);
+        f.rin = (int32_t)sched.size();
+        sched.resize(sched.size() + d.n + 2, 0);
+        f.rout = (int32_t)sched.size();
+        sched.resize(sched.size() + d.n + 2, 0);
+        const int gbase = d.groups;
+        for (int al = d.n; al >= 1; --al) {
+            const int g0 = A[d.rows + al + 1], g1 = A[d.rows + al];
+            for (int pass = 0; pass < 2; ++pass) {
+                std::vector<int> idx;
+                for (int g = g0; g < g1; ++g) idx.push_back(g);
+                auto size_of = [&](int g) -> long {
+                    const int32_t *G = A + gbase + g * LCG_GROUP_INTS;
+                    const int gal = G[0] & 0xffff, gbl = G[0] >> 16, gar = G[1] & 0xffff, gbr = G[1] >> 16;
+                    if (gbr - gbl - 1 > 128 || gar - gal - 1 > 128) return -1;
+                    const long cells = (long)(gar - gal) * G[2];
+                    const long sz = (gbr - gbl) + (pass == 0 ? 1 : 2) * cells;
+                    return sz <= pool ? ((sz + 1) & ~1L) : -1;
+                };
+                std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return size_of(x) > size_of(y); });
+                std::vector<long> used;
+                std::vector<int> cnt;
+                for (int g : idx) {
+                    const long sz = size_of(g);
+                    size_t r = 0;
+                    while (r < used.size() && (cnt[r] >= B.pf_waves || (sz > 0 && used[r] + sz > pool))) ++r;
+                    if (r == used.size()) {
+                        used.push_back(0);
+                        cnt.push_back(0);
+                    }
+                    int32_t *S = &sched[f.sched + 4 * (size_t)g + 2 * pass];
+                    S[0] = (int32_t)r | (cnt[r] << 12);
+                    S[1] = sz > 0 ? (int32_t)used[r] : -1;
+                    if (sz > 0) {
+                        used[r] += sz;
+                        const int32_t *G = A + gbase + g * LCG_GROUP_INTS;
+                        max_fixed = std::max(max_fixed, (size_t)2 * ((G[1] & 0xffff) - (G[0] & 0xffff)) * G[2]);
+                    }
+                    cnt[r] += 1;
+                }
+                sched[(pass == 0 ? f.rin : f.rout) + al] = (int32_t)used.size();
+            }
+        }
+        // group indices in sched are relative to the pair's group array
+        (void)gbase;
+    }
+    stride = std::max(stride, max_fixed);
     B.pbp.scratch_stride = (int32_t)((stride + 31) & ~(size_t)31);
-    const int hdr_ints = ((5 * max_len + 6 + 1) & ~1) + 2 * (2 * (max_len + 1) + LCG_NCODES * LCG_NCODES);
-    B.pf_lds = (size_t)hdr_ints * 4;
     hipError_t e;
     HIPCHECK(hipSetDevice(B.device));
     int cus = 0;
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B.device));
-    B.pf_grid = std::min(np, 2 * cus);
+    B.pf_grid = std::min(np, cus);
     HIPCHECK(hipMalloc(&B.d_pfa, std::max<size_t>(1, off) * 8));
     HIPCHECK(hipMalloc(&B.d_pdescs, sizeof(PfDesc) * np));
     HIPCHECK(hipMalloc(&B.d_pfscratch, (size_t)B.pf_grid * B.pf_waves * B.pbp.scratch_stride * 8));
+    HIPCHECK(hipMalloc(&B.d_pfsched, std::max<size_t>(1, sched.size()) * 4));
+    HIPCHECK(hipMemcpy(B.d_pfsched, sched.data(), sched.size() * 4, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(B.d_pdescs, B.pdescs.data(), sizeof(PfDesc) * np, hipMemcpyHostToDevice));
     (void)e;
     return LCG_OK;
@@ -971,7 +1045,7 @@ extern "C" int lcg_batch_pf_run(lcg_batch *B, const double *dtheta, float *devic
     HIPCHECK(hipMemcpy(B->d_pfa, B->pf_arena.data(), B->pf_arena.size() * 8, hipMemcpyHostToDevice));
     HIPCHECK(hipEventRecord(B->ev0, B->stream));
     int rc = lcg_launch_pf(B->d_descs, B->d_pdescs, B->d_order, B->d_queue, B->d_arena, B->d_pfa, B->d_pfscratch,
-                           B->pbp, B->pf_grid, B->pf_waves, B->pf_lds, B->stream);
+                           B->d_pfsched, B->pbp, B->pf_grid, B->pf_waves, B->pf_lds, B->stream);
     if (rc) return hip_fail((hipError_t)rc);
     HIPCHECK(hipEventRecord(B->ev1, B->stream));
     HIPCHECK(hipEventSynchronize(B->ev1));

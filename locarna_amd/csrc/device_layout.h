@@ -38,6 +38,7 @@
 //                                       j | min_col(p-1)<<16, D(am)); a group
 //                                       with left end al reads the prefix
 //                                       p > al of each row
+//   f2c:      per fill entry            its index in cand (cell order)
 //   D:        per arc match (output of the fill, reference index order)
 //   edges:    n + m + 1 slots (output): slot i+j holds the edge that reaches
 //             cell (i, j): (a << 16) | b with 0 for a gap side, -1 if empty
@@ -72,7 +73,8 @@ typedef struct {
     int32_t top_ints;  // LDS ints of the top-level region
     int32_t fptr, fblk, fcand;
     int32_t nfblk;     // fill blocks including the sentinel
-    int32_t pad[4];
+    int32_t f2c;       // fill entry -> candidate entry (cell order)
+    int32_t pad[3];
 } PairDesc;  // 32 ints
 
 typedef struct {

@@ -82,6 +82,10 @@ struct PfPair {
     double *edge_full, *edge_ref;  // per band cell
     double eo, ps;
     int n, m;
+    const int *fbp, *fbk;  // LDS: fill blocks of row i (left end p, first entry)
+    const int *fcand;      // int arena: fill entries (p | q<<16, j | min_col(p-1)<<16, -)
+    const int *f2c;        // fill entry -> candidate entry
+    double *fD;            // inside D per fill entry
 };
 
 __device__ __forceinline__ int row_lo(const PfPair &P, int i, int bl) { return max(bl, P.mn[i]); }
@@ -91,16 +95,16 @@ struct Region {
     int al, bl, ar, br, W;
     double *M, *E, *F, *Mb;  // matrices (R x W)
     double *Ev, *Eb;         // vectors over columns bl..br-1
-    __device__ double *at(double *X, const PfPair &P, int i, int j) const {
+    __device__ __forceinline__ double *at(double *X, const PfPair &P, int i, int j) const {
         const int lo = (i == al) ? bl : row_lo(P, i, bl);
         return X + (size_t)(i - al) * W + (j - lo);
     }
-    __device__ bool stored(const PfPair &P, int i, int j) const {
+    __device__ __forceinline__ bool stored(const PfPair &P, int i, int j) const {
         if (i < al || i >= ar) return false;
         const int lo = (i == al) ? bl : row_lo(P, i, bl);
         return j >= lo && j <= row_hi(P, i, br);
     }
-    __device__ double get(double *X, const PfPair &P, int i, int j) const {
+    __device__ __forceinline__ double get(double *X, const PfPair &P, int i, int j) const {
         return stored(P, i, j) ? *at(X, P, i, j) : 0.0;
     }
 };
@@ -315,7 +319,303 @@ __device__ void pf_reverse(const PfPair &P, const Region &R, bool want_dbar, dou
     }
 }
 
-__device__ void zero_region_bar(const Region &R, const PfPair &P) {
+// ---- fixed-column group kernels (columns bl+1 .. br-1 in NC chunks of 64
+// lanes, lane l of chunk c owns column bl+1+64c+l in every row; rows
+// al+1 .. ar-1 at most 128).  Region in LDS (doubles): [S: br-bl] [M:
+// (ar-al) W] [Mbar: (ar-al) W], rows stored from max(bl, min_col(i)) as in
+// the general layout.  The arc candidates of row i are the fill-list prefix
+// p > al (left ends descending), located from the LDS block table.
+
+struct PfEnt {
+    int pq, jm;
+    double D;
+};
+
+__device__ __forceinline__ double shift_up_d(double v, double first) {
+    const double r = __shfl_up(v, 1);
+    return lane_id() == 0 ? first : r;
+}
+__device__ __forceinline__ double shift_down_d(double v, double last) {
+    const double r = __shfl_down(v, 1);
+    return lane_id() == 63 ? last : r;
+}
+
+// candidate range [t0, t1) of rows al+1+r, r = lane + 64 h
+__device__ __forceinline__ void pf_row_table(const PfPair &P, int al, int bl, int br, int nrows, int rt0[2],
+                                             int rt1[2]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int r = lane_id() + 64 * h;
+        rt0[h] = rt1[h] = 0;
+        if (r < nrows) {
+            const int i = al + 1 + r;
+            if (max(bl + 1, P.mn[i]) <= min(br - 1, P.mx[i])) {
+                int b = P.fbp[i];
+                const int be = P.fbp[i + 1];
+                rt0[h] = P.fbk[2 * b + 1];
+                while (b < be && P.fbk[2 * b] > al) ++b;
+                rt1[h] = P.fbk[2 * b + 1];
+            }
+        }
+    }
+}
+__device__ __forceinline__ void pf_row_cands(const int rt0[2], const int rt1[2], int r, int &t0, int &t1) {
+    t0 = (r < 64) ? __builtin_amdgcn_readlane(rt0[0], r & 63) : __builtin_amdgcn_readlane(rt0[1], r & 63);
+    t1 = (r < 64) ? __builtin_amdgcn_readlane(rt1[0], r & 63) : __builtin_amdgcn_readlane(rt1[1], r & 63);
+}
+__device__ __forceinline__ PfEnt pf_fetch(const PfPair &P, int t, int t1) {
+    PfEnt e{0, 0, 0.0};
+    if (t < t1) {
+        e.pq = P.fcand[3 * t];
+        e.jm = P.fcand[3 * t + 1];
+        e.D = P.fD[t];
+    }
+    return e;
+}
+__device__ __forceinline__ void lds_fence() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); }
+
+// Inside fill of a group (aligner_p.icc:150-312), values as pf_fill.  When
+// Es/Ss are given (global scratch, layout of M), E and s = M - F of every
+// computed cell are stored for the adjoint.
+template <int NC>
+__device__ __forceinline__ void pf_fwd_fixed(const PfPair &P, int al, int bl, int ar, int br, int W, double *base, double *Es,
+                             double *Ss) {
+    const int lane = lane_id();
+    const double eo = P.eo, ps = P.ps;
+    double *Srow = base;
+    double *Mst = base + (br - bl);
+    const int nrows = ar - al - 1;
+    int rt0[2], rt1[2];
+    pf_row_table(P, al, bl, br, nrows, rt0, rt1);
+    for (int c = lane; c < br - bl; c += 64) Srow[c] = 0.0;
+    double Mp[NC], Ep[NC], gBc[NC];
+    int cBc[NC];
+    const int hi0 = min(br - 1, P.mx[al]);
+    {
+        double carry = eo / ps;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int j = bl + 1 + 64 * c + lane;
+            const bool in = j <= hi0, col = j <= br - 1;
+            double a = in ? P.egB[j] : 1.0, b = 0.0;
+            wave_scan_affine(a, b);  // prefix products
+            Mp[c] = in ? carry * a : 0.0;
+            if (in) Mst[j - bl] = carry * a;
+            carry *= readlane_d(a, 63);
+            Ep[c] = 0.0;
+            gBc[c] = col ? P.egB[j] : 1.0;
+            cBc[c] = col ? P.codeB[j] : 0;
+        }
+        if (lane == 0) Mst[0] = 1.0 / ps;
+    }
+    lds_fence();
+    double colv = eo / ps;
+    bool colopen = true;
+    double Mcol_prev = 1.0 / ps;  // M(i-1, bl)
+    PfEnt cur[4];
+    {
+        int u0 = 0, u1 = 0;
+        if (nrows > 0) pf_row_cands(rt0, rt1, 0, u0, u1);
+#pragma unroll
+        for (int k = 0; k < 4; ++k) cur[k] = pf_fetch(P, u0 + 64 * k + lane, u1);
+    }
+    for (int i = al + 1; i < ar; ++i) {
+        const int r = i - al - 1;
+        const int mni = P.mn[i];
+        double Mcol = 0.0;  // M(i, bl)
+        if (colopen && mni <= bl) {
+            colv *= P.egA[i];
+            Mcol = colv;
+        } else {
+            colopen = false;
+        }
+        const int lo = max(bl, mni), hi = min(br - 1, P.mx[i]);
+        double *row = Mst + (size_t)(r + 1) * W;
+        if (colopen && lane == 0) row[0] = colv;
+        const int jlo = max(bl + 1, mni);
+        int t0, t1;
+        pf_row_cands(rt0, rt1, r, t0, t1);
+        // arc terms ps * M(p-1, q-1) * D, candidate-parallel, LDS atomic adds
+        auto arc = [&](const PfEnt &e, bool in) {
+            const int p = e.pq & 0xffff, q = e.pq >> 16, j = e.jm & 0xffff;
+            const int rl = max(bl, e.jm >> 16);
+            const bool ok = in && q > bl && j >= jlo && j <= hi && q - 1 >= rl;
+            const double v = Mst[ok ? (p - 1 - al) * W + (q - 1 - rl) : 0] * e.D * ps;
+            if (ok) atomicAdd(&Srow[j - bl - 1], v);
+        };
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            if (t0 + 64 * k < t1) arc(cur[k], t0 + 64 * k + lane < t1);
+        for (int t = t0 + 256 + lane; t < t1; t += 64) arc(pf_fetch(P, t, t1), true);
+        if (r + 1 < nrows) {
+            int u0, u1;
+            pf_row_cands(rt0, rt1, r + 1, u0, u1);
+#pragma unroll
+            for (int k = 0; k < 4; ++k) cur[k] = pf_fetch(P, u0 + 64 * k + lane, u1);
+        }
+        lds_fence();
+        double Fcarry = 0.0, sprev = Mcol, dgin = Mcol_prev;
+        const double gA = P.egA[i];
+        const double *esrow = P.es + P.codeA[i] * LCG_NCODES;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int j = bl + 1 + 64 * c + lane;
+            const bool act = j >= jlo && j <= hi;
+            const double up = Mp[c];
+            const double dg = shift_up_d(Mp[c], dgin);
+            dgin = readlane_d(Mp[c], 63);
+            double s = 0.0, e = 0.0;
+            if (act) {
+                const double ep = Ep[c];
+                e = ep * gA + (up - ep) * gA * eo;
+                Ep[c] = e;
+                const double sv = Srow[j - bl - 1];
+                Srow[j - bl - 1] = 0.0;
+                s = dg * esrow[cBc[c]] + e + sv;
+            }
+            const double sl = shift_up_d(s, sprev);
+            double a = act ? gBc[c] : 1.0, b = act ? sl * gBc[c] * eo : 0.0;
+            wave_scan_affine(a, b);
+            const double F = a * Fcarry + b;
+            const double Mv = s + F;
+            if (act) {
+                const size_t idx = (size_t)(r + 1) * W + (j - lo);
+                row[j - lo] = Mv;
+                if (Es) {
+                    Es[idx] = e;
+                    Ss[idx] = s;
+                }
+            }
+            Mp[c] = act ? Mv : 0.0;
+            Fcarry = readlane_d(F, 63);
+            sprev = readlane_d(s, 63);
+        }
+        Mcol_prev = Mcol;
+        lds_fence();
+    }
+}
+
+// Adjoint of pf_fwd_fixed for a group whose Mbar rows (LDS, after M) hold the
+// seeds; Es/Ss from the forward recompute.  FULL: also Dbar (dZ/dD per
+// candidate entry, global atomics) and the gap-opening weight (returned);
+// edge receives the match-edge weights.  Same terms as pf_reverse.
+template <int NC, bool FULL>
+__device__ __forceinline__ double pf_rev_fixed(const PfPair &P, int al, int bl, int ar, int br, int W, double *base, const double *Es,
+                               const double *Ss, double *edge) {
+    const int lane = lane_id();
+    const double eo = P.eo, ps = P.ps;
+    double *Sb = base;
+    const double *Mst = base + (br - bl);
+    double *Mb = base + (br - bl) + (size_t)(ar - al) * W;
+    const int nrows = ar - al - 1;
+    int rt0[2], rt1[2];
+    pf_row_table(P, al, bl, br, nrows, rt0, rt1);
+    double carry[NC], Eb[NC], gBc[NC];
+    int cBc[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const int j = bl + 1 + 64 * c + lane;
+        carry[c] = 0.0;
+        Eb[c] = 0.0;
+        gBc[c] = (j <= br - 1) ? P.egB[j] : 1.0;
+        cBc[c] = (j <= br - 1) ? P.codeB[j] : 0;
+    }
+    double colcarry = 0.0;  // contribution of row i+1 to Mbar(i, bl)
+    double open_acc = 0.0;
+    lds_fence();
+    for (int i = ar - 1; i > al; --i) {
+        const int r = i - al - 1;
+        const int mni = P.mn[i];
+        const int lo = max(bl, mni), hi = min(br - 1, P.mx[i]), jlo = max(bl + 1, mni);
+        const bool colopen_i = mni <= bl;
+        const int lo_u = (i - 1 == al) ? bl : max(bl, P.mn[i - 1]), hi_u = min(br - 1, P.mx[i - 1]);
+        const int jlo_u = max(bl + 1, P.mn[i - 1]);
+        const bool upc_row = i - 1 > al;  // row i-1 computed (has E)
+        const double *rowM = Mst + (size_t)(r + 1) * W;
+        const double *rowU = Mst + (size_t)r * W;
+        const double *rowB = Mb + (size_t)(r + 1) * W;
+        const double gA = P.egA[i];
+        const double *esrow = P.es + P.codeA[i] * LCG_NCODES;
+        const int ecell = P.coff[i] - mni;
+        double Fb_in = 0.0, gB_in = 0.0, d_in = 0.0;
+        double newcarry[NC];
+        double colcarry_next = 0.0, colb = 0.0;
+#pragma unroll
+        for (int cc = NC - 1; cc >= 0; --cc) {
+            const int j = bl + 1 + 64 * cc + lane;
+            const bool act = j >= jlo && j <= hi;
+            const double g = act ? rowB[j - lo] + carry[cc] : 0.0;
+            const double gB = act ? gBc[cc] : 0.0;
+            const double gB_next = shift_down_d(gB, gB_in);
+            double cm = act ? gB_next : 0.0, gg = g;
+            wave_suffix_affine(cm, gg);
+            const double Fb = cm * Fb_in + gg;  // Fbar_j
+            const double Fb_next = shift_down_d(Fb, Fb_in);
+            const double sb = act ? g + Fb_next * gB_next * eo : 0.0;
+            if (act) Sb[j - bl - 1] = sb;
+            const bool bcol = act && j == jlo && jlo == bl + 1 && colopen_i;  // s_{j-1} = M(i, bl)
+            if (FULL && act) {
+                const double sl = (j == jlo) ? (bcol ? rowM[0] : 0.0) : Ss[(size_t)(r + 1) * W + (j - 1 - lo)];
+                open_acc += Fb * sl * gB * eo;
+            }
+            if (cc == 0) colb = readlane_d(bcol ? Fb * gB * eo : 0.0, 0);
+            // E chain: E(i,j) = E(i-1,j) gA (1-o) + M(i-1,j) gA o
+            const double eb = act ? sb + Eb[cc] : 0.0;
+            const bool upst = j >= lo_u && j <= hi_u;
+            if (FULL && act) {
+                const double up = upst ? rowU[j - lo_u] : 0.0;
+                const bool upc = upc_row && j >= jlo_u && j <= hi_u;
+                const double Ep = upc ? Es[(size_t)r * W + (j - lo_u)] : 0.0;
+                open_acc += eb * (up - Ep) * gA * eo;
+            }
+            Eb[cc] = act ? eb * gA * (1.0 - eo) : 0.0;
+            // match: s_j uses M(i-1, j-1) es
+            const bool dgst = act && j - 1 >= lo_u && j - 1 <= hi_u;
+            const double es = esrow[cBc[cc]];
+            if (dgst) atomicAdd(&edge[ecell + j], sb * rowU[j - 1 - lo_u] * es);
+            const double d = dgst ? sb * es : 0.0;  // to Mbar(i-1, j-1)
+            const double d_right = shift_down_d(d, d_in);
+            newcarry[cc] = upst ? (act ? eb * gA * eo : 0.0) + d_right : 0.0;
+            if (cc == 0) colcarry_next = readlane_d(d, 0);
+            d_in = readlane_d(d, 0);
+            Fb_in = readlane_d(Fb, 0);
+            gB_in = readlane_d(gB, 0);
+        }
+        // the init column cell (i, bl): one opening per value
+        if (FULL && colopen_i && lane == 0) open_acc += (rowB[0] + colcarry + colb) * rowM[0];
+        lds_fence();
+        // arc terms: Mbar(p-1, q-1) += sbar D ps; Dbar += sbar M(p-1, q-1) ps
+        int t0, t1;
+        pf_row_cands(rt0, rt1, r, t0, t1);
+        for (int t = t0 + lane; t < t1; t += 64) {
+            const PfEnt e = pf_fetch(P, t, t1);
+            const int p = e.pq & 0xffff, q = e.pq >> 16, j = e.jm & 0xffff;
+            const int rl = max(bl, e.jm >> 16);
+            if (q > bl && j >= jlo && j <= hi && q - 1 >= rl && e.D != 0.0) {
+                const double sb = Sb[j - bl - 1];
+                const int idx = (p - 1 - al) * W + (q - 1 - rl);
+                atomicAdd(&Mb[idx], sb * e.D * ps);
+                if (FULL) atomicAdd(&P.Dbar[P.f2c[t]], sb * Mst[idx] * ps);
+            }
+        }
+        lds_fence();
+#pragma unroll
+        for (int c = 0; c < NC; ++c) carry[c] = newcarry[c];
+        colcarry = (lo_u == bl) ? colcarry_next : 0.0;
+    }
+    // init row al: one opening per value M(al, j), j > bl
+    if (FULL) {
+        const int hi0 = min(br - 1, P.mx[al]);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int j = bl + 1 + 64 * c + lane;
+            if (j <= hi0) open_acc += (Mb[j - bl] + carry[c]) * Mst[j - bl];
+        }
+    }
+    return wave_sum(open_acc);
+}
+
+__device__ __forceinline__ void zero_region_bar(const Region &R, const PfPair &P) {
     const int lane = lane_id();
     const size_t total = (size_t)(R.ar - R.al) * R.W;
     for (size_t c = lane; c < total; c += 64) R.Mb[c] = 0.0;
@@ -324,11 +624,19 @@ __device__ void zero_region_bar(const Region &R, const PfPair &P) {
 
 }  // namespace
 
-extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *__restrict__ descs,
-                                                               const PfDesc *__restrict__ pdescs,
-                                                               const int *__restrict__ order, int *queue,
-                                                               int *arena, double *pfa, double *scratch,
-                                                               PfBatchParams bp) {
+// One workgroup per pair (persistent, pairs from a queue).  Inside: rows
+// al = n .. 1, the groups of a row in the host's LDS rounds; narrow groups
+// (columns and rows at most 128) run the fixed-column kernels on LDS
+// regions, others the general kernels on per-wave global scratch.  Top
+// level (wave 0, global scratch): inside, full and restricted adjoint.
+// Outside: rows al = 1 .. n, each group's forward values recomputed, then
+// the full adjoint (Dbar, edge weights, openings) and, if any of its arc
+// matches has P(am) > sqrt(min_am_prob), the restricted one.
+extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *__restrict__ descs,
+                                                                const PfDesc *__restrict__ pdescs,
+                                                                const int *__restrict__ order, int *queue,
+                                                                int *arena, double *pfa, double *scratch,
+                                                                const int *__restrict__ sched, PfBatchParams bp) {
     extern __shared__ int lds[];
     __shared__ int s_pair;
     __shared__ double s_open;
@@ -338,8 +646,10 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
     const int L = bp.max_len;
     int *mn = lds, *mx = lds + (L + 1), *coff = lds + 2 * (L + 1);
     int *codeA = lds + 3 * L + 4, *codeB = lds + 4 * L + 5;
-    double *dl = reinterpret_cast<double *>(lds + ((5 * L + 6 + 1) & ~1));
+    int *fbp = lds + 5 * L + 6, *fbk = lds + 6 * L + 8;
+    double *dl = reinterpret_cast<double *>(lds + ((6 * L + 8 + 2 * bp.nfblk_max + 1) & ~1));
     double *egA = dl, *egB = dl + (L + 1), *es = dl + 2 * (L + 1);
+    double *pool = es + LCG_NCODES * LCG_NCODES;
     double *wscr = scratch + (size_t)(blockIdx.x * nwaves + wave) * bp.scratch_stride;
     for (;;) {
         if (threadIdx.x == 0) s_pair = atomicAdd(queue, 1);
@@ -360,11 +670,16 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
             codeB[t] = arena[d.codes + n + 1 + t];
             egB[t] = pfa[f.egB + t];
         }
-        for (int t = threadIdx.x; t <= n + 1; t += blockDim.x) coff[t] = arena[d.band + 2 * (n + 1) + t];
+        for (int t = threadIdx.x; t <= n + 1; t += blockDim.x) {
+            coff[t] = arena[d.band + 2 * (n + 1) + t];
+            fbp[t] = arena[d.fptr + t];
+        }
+        for (int t = threadIdx.x; t < 2 * d.nfblk; t += blockDim.x) fbk[t] = arena[d.fblk + t];
         for (int t = threadIdx.x; t < LCG_NCODES * LCG_NCODES; t += blockDim.x) es[t] = pfa[f.es + t];
         for (int t = threadIdx.x; t < d.nam; t += blockDim.x) {
             pfa[f.Dc + t] = 0.0;
             pfa[f.Dbar + t] = 0.0;
+            pfa[f.fD + t] = 0.0;
         }
         for (int t = threadIdx.x; t < d.ncells; t += blockDim.x) {
             pfa[f.edge_full + t] = 0.0;
@@ -391,11 +706,17 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
         P.ps = pfa[f.scal + 1];
         P.n = n;
         P.m = m;
+        P.fbp = fbp;
+        P.fbk = fbk;
+        P.fcand = arena + d.fcand;
+        P.f2c = arena + d.f2c;
+        P.fD = pfa + f.fD;
         __threadfence();
         __syncthreads();
         const int *rows = arena + d.rows;
         const int *groups = arena + d.groups;
         const int4 *gams = reinterpret_cast<const int4 *>(arena + d.gams);
+        const int *S = sched + f.sched;
         auto region = [&](int al, int bl, int ar, int br, int W) {
             Region R;
             R.al = al;
@@ -412,21 +733,54 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
             R.Eb = R.Ev + (br - bl);
             return R;
         };
+        // stored value of (i, j) in a fixed region (row al from bl)
+        auto fixed_at = [&](int al, int bl, int br, int W, int i, int j) {
+            const int lo = (i == al) ? bl : max(bl, P.mn[i]);
+            const int hi = min(br - 1, P.mx[i]);
+            return (j >= lo && j <= hi) ? (i - al) * W + (j - lo) : -1;
+        };
         // ---- inside: rows al = n .. 1
         for (int al = n; al >= 1; --al) {
             const int g0 = rows[al + 1], g1 = rows[al];
-            for (int g = g0 + wave; g < g1; g += nwaves) {
-                const int *G = groups + g * LCG_GROUP_INTS;
-                const Region R = region(G[0] & 0xffff, G[0] >> 16, G[1] & 0xffff, G[1] >> 16, G[2]);
-                pf_fill(P, R);
-                for (int t = G[3] + lane; t < G[4]; t += 64) {
-                    const int4 ga = gams[t];
-                    const int ar = ga.x & 0xffff, br = ga.x >> 16;
-                    P.Dc[ga.y] = R.get(R.M, P, ar - 1, br - 1) * P.eam[ga.y];
+            if (g0 == g1) continue;
+            const int nr = sched[f.rin + al];
+            for (int rnd = 0; rnd < nr; ++rnd) {
+                for (int g = g0; g < g1; ++g) {
+                    if (S[4 * g] != (rnd | (wave << 12))) continue;
+                    const int *G = groups + g * LCG_GROUP_INTS;
+                    const int gal = G[0] & 0xffff, gbl = G[0] >> 16, gar = G[1] & 0xffff, gbr = G[1] >> 16;
+                    const int W = G[2];
+                    const int off = S[4 * g + 1];
+                    if (off >= 0) {
+                        double *base = pool + off;
+                        if (gbr - gbl - 1 <= 64)
+                            pf_fwd_fixed<1>(P, gal, gbl, gar, gbr, W, base, nullptr, nullptr);
+                        else
+                            pf_fwd_fixed<2>(P, gal, gbl, gar, gbr, W, base, nullptr, nullptr);
+                        const double *Mst = base + (gbr - gbl);
+                        for (int t = G[3] + lane; t < G[4]; t += 64) {
+                            const int4 ga = gams[t];
+                            const int ar = ga.x & 0xffff, br = ga.x >> 16;
+                            const int x = fixed_at(gal, gbl, gbr, W, ar - 1, br - 1);
+                            const double v = (x >= 0 ? Mst[x] : 0.0) * P.eam[ga.y];
+                            P.Dc[ga.y] = v;
+                            P.fD[ga.w] = v;
+                        }
+                    } else {
+                        const Region R = region(gal, gbl, gar, gbr, W);
+                        pf_fill(P, R);
+                        for (int t = G[3] + lane; t < G[4]; t += 64) {
+                            const int4 ga = gams[t];
+                            const int ar = ga.x & 0xffff, br = ga.x >> 16;
+                            const double v = R.get(R.M, P, ar - 1, br - 1) * P.eam[ga.y];
+                            P.Dc[ga.y] = v;
+                            P.fD[ga.w] = v;
+                        }
+                    }
                 }
+                __threadfence_block();
+                __syncthreads();
             }
-            __threadfence();
-            __syncthreads();
         }
         // ---- top level and its adjoint (wave 0)
         if (wave == 0) {
@@ -434,7 +788,6 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
             pf_fill(P, R);
             const double Z = *R.at(R.M, P, n, m);
             if (lane == 0) pfa[f.out + 0] = Z;
-            // full adjoint, seed dZ/dM(n,m) = 1
             zero_region_bar(R, P);
             if (lane == 0) *R.at(R.Mb, P, n, m) = 1.0;
             scratch_sync();
@@ -451,39 +804,83 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
         // ---- outside over the groups, rows al = 1 .. n
         for (int al = 1; al <= n; ++al) {
             const int g0 = rows[al + 1], g1 = rows[al];
-            for (int g = g0 + wave; g < g1; g += nwaves) {
-                const int *G = groups + g * LCG_GROUP_INTS;
-                const Region R = region(G[0] & 0xffff, G[0] >> 16, G[1] & 0xffff, G[1] >> 16, G[2]);
-                pf_fill(P, R);
-                zero_region_bar(R, P);
-                for (int t = G[3] + lane; t < G[4]; t += 64) {
-                    const int4 ga = gams[t];
-                    const int ar = ga.x & 0xffff, br = ga.x >> 16;
-                    if (R.stored(P, ar - 1, br - 1))
-                        atomicAdd(R.at(R.Mb, P, ar - 1, br - 1), P.Dbar[ga.y] * P.eam[ga.y]);
-                }
-                scratch_sync();
-                pf_reverse(P, R, true, P.edge_full, &s_open);
-                // restricted pass for the base-match probabilities
-                bool any = false;
-                for (int t = G[3] + lane; t < G[4]; t += 64) {
-                    const int4 ga = gams[t];
-                    if (P.Dc[ga.y] * P.Dbar[ga.y] / Z > thr) any = true;
-                }
-                if (__ballot(any)) {
-                    zero_region_bar(R, P);
+            if (g0 == g1) continue;
+            const int nr = sched[f.rout + al];
+            for (int rnd = 0; rnd < nr; ++rnd) {
+                for (int g = g0; g < g1; ++g) {
+                    if (S[4 * g + 2] != (rnd | (wave << 12))) continue;
+                    const int *G = groups + g * LCG_GROUP_INTS;
+                    const int gal = G[0] & 0xffff, gbl = G[0] >> 16, gar = G[1] & 0xffff, gbr = G[1] >> 16;
+                    const int W = G[2];
+                    const int off = S[4 * g + 3];
+                    bool any = false;
                     for (int t = G[3] + lane; t < G[4]; t += 64) {
                         const int4 ga = gams[t];
-                        const int ar = ga.x & 0xffff, br = ga.x >> 16;
-                        if (P.Dc[ga.y] * P.Dbar[ga.y] / Z > thr && R.stored(P, ar - 1, br - 1))
-                            atomicAdd(R.at(R.Mb, P, ar - 1, br - 1), P.Dbar[ga.y] * P.eam[ga.y]);
+                        if (P.Dc[ga.y] * P.Dbar[ga.y] / Z > thr) any = true;
                     }
-                    scratch_sync();
-                    pf_reverse(P, R, false, P.edge_ref, nullptr);
+                    const bool sig = __ballot(any) != 0;
+                    if (off >= 0) {
+                        double *base = pool + off;
+                        const size_t cells = (size_t)(gar - gal) * W;
+                        double *Mb = base + (gbr - gbl) + cells;
+                        double *Es = wscr, *Ss = wscr + cells;
+                        const bool nc1 = gbr - gbl - 1 <= 64;
+                        if (nc1)
+                            pf_fwd_fixed<1>(P, gal, gbl, gar, gbr, W, base, Es, Ss);
+                        else
+                            pf_fwd_fixed<2>(P, gal, gbl, gar, gbr, W, base, Es, Ss);
+                        for (int pass = 0; pass < (sig ? 2 : 1); ++pass) {
+                            for (size_t c = lane; c < cells; c += 64) Mb[c] = 0.0;
+                            lds_fence();
+                            for (int t = G[3] + lane; t < G[4]; t += 64) {
+                                const int4 ga = gams[t];
+                                if (pass == 1 && !(P.Dc[ga.y] * P.Dbar[ga.y] / Z > thr)) continue;
+                                const int ar = ga.x & 0xffff, br = ga.x >> 16;
+                                const int x = fixed_at(gal, gbl, gbr, W, ar - 1, br - 1);
+                                if (x >= 0) atomicAdd(&Mb[x], P.Dbar[ga.y] * P.eam[ga.y]);
+                            }
+                            __threadfence();  // Es/Ss (global) and the seeds (LDS) before the sweep
+                            if (pass == 0) {
+                                const double o = nc1 ? pf_rev_fixed<1, true>(P, gal, gbl, gar, gbr, W, base, Es, Ss,
+                                                                                P.edge_full)
+                                                     : pf_rev_fixed<2, true>(P, gal, gbl, gar, gbr, W, base, Es, Ss,
+                                                                                P.edge_full);
+                                if (lane == 0) atomicAdd(&s_open, o);
+                            } else {
+                                if (nc1)
+                                    pf_rev_fixed<1, false>(P, gal, gbl, gar, gbr, W, base, Es, Ss, P.edge_ref);
+                                else
+                                    pf_rev_fixed<2, false>(P, gal, gbl, gar, gbr, W, base, Es, Ss, P.edge_ref);
+                            }
+                        }
+                    } else {
+                        const Region R = region(gal, gbl, gar, gbr, W);
+                        pf_fill(P, R);
+                        zero_region_bar(R, P);
+                        for (int t = G[3] + lane; t < G[4]; t += 64) {
+                            const int4 ga = gams[t];
+                            const int ar = ga.x & 0xffff, br = ga.x >> 16;
+                            if (R.stored(P, ar - 1, br - 1))
+                                atomicAdd(R.at(R.Mb, P, ar - 1, br - 1), P.Dbar[ga.y] * P.eam[ga.y]);
+                        }
+                        scratch_sync();
+                        pf_reverse(P, R, true, P.edge_full, &s_open);
+                        if (sig) {
+                            zero_region_bar(R, P);
+                            for (int t = G[3] + lane; t < G[4]; t += 64) {
+                                const int4 ga = gams[t];
+                                const int ar = ga.x & 0xffff, br = ga.x >> 16;
+                                if (P.Dc[ga.y] * P.Dbar[ga.y] / Z > thr && R.stored(P, ar - 1, br - 1))
+                                    atomicAdd(R.at(R.Mb, P, ar - 1, br - 1), P.Dbar[ga.y] * P.eam[ga.y]);
+                            }
+                            scratch_sync();
+                            pf_reverse(P, R, false, P.edge_ref, nullptr);
+                        }
+                    }
                 }
+                __threadfence();
+                __syncthreads();
             }
-            __threadfence();
-            __syncthreads();
         }
         if (threadIdx.x == 0) pfa[f.out + 1] = s_open;
         __syncthreads();
@@ -495,10 +892,17 @@ extern "C" __global__ void lcg_pf_reset_queue(int *queue) {
 }
 
 extern "C" int lcg_launch_pf(const PairDesc *descs, const PfDesc *pdescs, const int *order, int *queue, int *arena,
-                             double *pfa, double *scratch, PfBatchParams bp, int grid, int waves, size_t lds_bytes,
-                             hipStream_t stream) {
+                             double *pfa, double *scratch, const int *sched, PfBatchParams bp, int grid, int waves,
+                             size_t lds_bytes, hipStream_t stream) {
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute((const void *)lcg_pf_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           160 * 1024 - 256);
+        if (e != hipSuccess) return (int)e;
+        attr_set = true;
+    }
     hipLaunchKernelGGL(lcg_pf_reset_queue, dim3(1), dim3(64), 0, stream, queue);
     hipLaunchKernelGGL(lcg_pf_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, pdescs, order, queue,
-                       arena, pfa, scratch, bp);
+                       arena, pfa, scratch, sched, bp);
     return (int)hipGetLastError();
 }

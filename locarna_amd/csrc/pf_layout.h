@@ -13,6 +13,13 @@
 //                       P(am) > sqrt(min_am_prob))
 //   scal[4]             exp(indel_opening / T), pf_scale, sqrt(min_am_prob)
 //   out[4]              Z, expected-openings weight
+//   fD[nam]             inside D per fill entry (the fill list of the int
+//                       arena, for the candidate gathers of the group kernels)
+//
+// Per pair in the int schedule buffer (offsets in ints): per group 4 ints
+// (inside round | wave<<12, inside LDS offset in doubles, outside round |
+// wave<<12, outside LDS offset; offset -1: the group runs in global
+// scratch), and the number of rounds of every row for both passes.
 #pragma once
 #include <stdint.h>
 
@@ -20,14 +27,18 @@ typedef struct {
     int32_t egA, egB, es, eam;
     int32_t Dc, Dbar, edge_full, edge_ref;
     int32_t scal, out;
-    int32_t pad[6];
+    int32_t fD;
+    int32_t sched, rin, rout;  // int schedule buffer
+    int32_t pad[2];
 } PfDesc;  // 16 ints
 
 typedef struct {
     int32_t npairs;
     int32_t max_len;
     int32_t scratch_stride;  // doubles of global scratch per (workgroup, wave)
-    int32_t pad[5];
+    int32_t pool;            // doubles of LDS pool for group regions
+    int32_t nfblk_max;       // fill blocks of the largest pair (LDS header)
+    int32_t pad[3];
 } PfBatchParams;
 
 // score parameters theta: sigma 4x4, arc-match term 16x16, indel,
