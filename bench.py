@@ -43,6 +43,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=0, help="pairs timed on the CPU (0: all)")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--sequ-local", action="store_true", help="locarna --sequ-local (sequence-local top level)")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the secondary configurations (long500, LocARNA-P)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the all-pairs stage would shard "
@@ -158,7 +160,87 @@ def main():
     if cpu:
         out["vs_cpu_baseline"] = round(value / cpu["value"], 2)
     batch.destroy()
+    if not args.no_extra:
+        out["other_configs"] = extra_configs(args, api, synth, torch, nthreads)
     print(json.dumps(out))
+
+
+# FP64 peak of MI355X (vector and MFMA alike), TFLOP/s
+P_FP64_TFLOPS = 78.6
+
+
+def extra_configs(args, api, synth, torch, nthreads):
+    """Secondary figures: the long-pair integer workload (global and
+    sequence-local) and the LocARNA-P workloads (inside + outside +
+    gradient), each with its CPU oracle timed on a sample of its pairs."""
+    from oracle import oracle
+    res = {}
+    for local in (0, 1):
+        pps, pairs, span = synth.workload("long500")
+        pa = api.locarna_params(max_bp_span=span, sequ_local=local)
+        P = [api.Pair(pps[a], pps[b], pa) for a, b in pairs]
+        B = api.Batch(P)
+        B.run(mode=0, trace=True, count=True)
+        cells = sum(B.counters(k)[0] for k in range(len(P)))
+        B.run(mode=2, trace=True)
+        ms = [B.run(mode=2, trace=True) for _ in range(3)]
+        t = sum(ms) / len(ms)
+        scores = [B.score(k) for k in range(len(P))]
+        B.destroy()
+        t0 = time.perf_counter()
+        cs = oracle.align_many([(pps[a], pps[b]) for a, b in pairs],
+                               oracle.locarna_params(max_bp_span=span, sequ_local=local), nthreads=nthreads)
+        dt = time.perf_counter() - t0
+        res["long500_" + ("local" if local else "global")] = {
+            "pairs": len(P), "ms_per_step": round(t, 3), "pairs_per_sec": round(len(P) / (t / 1e3), 2),
+            "gcups": round(cells / (t / 1e3) / 1e9, 3), "workgroups_per_pair": B.workgroups // len(P) if B.workgroups else None,
+            "cpu_baseline_pairs_per_sec": round(len(P) / dt, 3), "cpu_cores": nthreads,
+            "scores_bit_exact": "%d/%d" % (sum(1 for x, y in zip(scores, cs) if x == y), len(P))}
+    for wl, nsample in (("pf16x150", 32), ("grad256", 32)):
+        pps, pairs, span = synth.workload(wl)
+        pa = api.locarna_p_params(max_bp_span=span)
+        P = [api.Pair(pps[a], pps[b], pa) for a, b in pairs]
+        B = api.Batch(P)
+        B.run(mode=0, trace=False, count=True)
+        C = K = 0
+        for k in range(len(P)):
+            c, q = B.counters(k)
+            C += c
+            K += q
+        A = sum(p.num_arcmatches for p in P)
+        B.pf_run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        dev = []
+        steps = 3
+        for _ in range(steps):
+            dev.append(B.pf_run())
+            grads = [B.pf_grad(k) for k in range(len(P))]
+        wall = (time.perf_counter() - t0) / steps
+        dev_ms = sum(dev) / len(dev)
+        w_pf = 3 * (13 * C + 2 * K + A)
+        ach = w_pf / (dev_ms / 1e3) / 1e12
+        idx = list(range(0, len(P), max(1, len(P) // nsample)))[:nsample]
+        po = oracle.locarna_p_params(max_bp_span=span)
+        t0 = time.perf_counter()
+        import concurrent.futures as cf2
+        with cf2.ThreadPoolExecutor(nthreads) as ex:
+            ores = list(ex.map(lambda k: oracle.pf(pps[pairs[k][0]], pps[pairs[k][1]], po), idx))
+        dt = time.perf_counter() - t0
+        import math
+        ok = sum(1 for k, o in zip(idx, ores)
+                 if abs(math.log(B.pf_result(k)[0]) - math.log(o.pf)) <= 1e-9 * max(1.0, abs(math.log(o.pf))))
+        B.destroy()
+        res[wl] = {"pairs": len(P), "pf_ms_per_step": round(dev_ms, 3),
+                   "grad_pairs_per_sec": round(len(P) / wall, 2),
+                   "pf_pairs_per_sec": round(len(P) / (dev_ms / 1e3), 2), "dtype": "f64",
+                   "roofline": {"bound": "valu", "achieved": round(ach, 5), "peak": P_FP64_TFLOPS, "unit": "TFLOP/s",
+                                "frac": round(ach / P_FP64_TFLOPS, 6), "kernel": "lcg_pf_pairs",
+                                "flops_per_launch": w_pf},
+                   "cpu_baseline_pairs_per_sec": round(len(idx) / dt, 3), "cpu_cores": nthreads,
+                   "cpu_sample": "%d of %d pairs (inside + outside, CPU oracle)" % (len(idx), len(P)),
+                   "logZ_within_1e-9": "%d/%d" % (ok, len(idx))}
+    return res
 
 
 if __name__ == "__main__":
