@@ -102,6 +102,15 @@ int lcg_batch_alignment(const lcg_batch *batch, int32_t k, int32_t *a, int32_t *
  * (AlignerImpl::D(am), aligner.cc:574-610).  Returns the count. */
 int lcg_batch_D(const lcg_batch *batch, int32_t k, int64_t *D, int32_t cap);
 
+/* LDS schedule of the fill of D for a prepared pair, as lcg_batch_create
+ * computes it (no counterpart in the reference; exported for inspection):
+ * per group 4 ints (round | wave<<12 | member<<16, LDS offset, LDS ints,
+ * al | bl<<16),
+ * per row al the number of rounds (rounds[0..n+1]).  `members` workgroups
+ * share the pair.  Returns the number of groups. */
+int lcg_pair_schedule(const lcg_pair *pair, int32_t waves, int32_t pool_ints, int32_t members,
+                      int32_t *group_words, int32_t *rounds, int32_t cap);
+
 /* Work counted by the last run with trace bit 1 set: DP cells and arc-match
  * candidates evaluated by the fill of D and the top level of pair k. */
 int lcg_batch_counters(const lcg_batch *batch, int32_t k, int64_t *cells, int64_t *candidates);

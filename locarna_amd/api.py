@@ -55,6 +55,7 @@ def _sig(L):
                                 c.POINTER(c.c_int32), c.POINTER(c.c_int32), c.POINTER(c.c_int64)]
     L.lcg_pair_band.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_int32]
     L.lcg_pair_arcmatches.argtypes = [c.c_void_p] + [c.c_void_p] * 5 + [c.c_int32]
+    L.lcg_pair_schedule.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_int32, c.c_void_p, c.c_void_p, c.c_int32]
     L.lcg_batch_create.argtypes = [c.POINTER(c.c_void_p), c.c_int32, c.c_int32, c.POINTER(c.c_void_p),
                                    c.c_char_p, c.c_int]
     L.lcg_batch_destroy.argtypes = [c.c_void_p]
@@ -122,6 +123,20 @@ class Pair:
         lib().lcg_pair_arcmatches(self.h, *arrs, sc, n)
         k = self.num_arcmatches
         return [(arrs[0][t], arrs[1][t], arrs[2][t], arrs[3][t]) for t in range(k)], list(sc)[:k]
+
+    def schedule(self, waves, pool_ints, members=1):
+        """LDS schedule of the fill of D: list of (round, wave, member, lds
+        offset, lds ints, al, bl) per group, and the rounds per row."""
+        cap = max(4 * self.num_groups, self.lenA + 2, 1)
+        words, rounds = (ctypes.c_int32 * cap)(), (ctypes.c_int32 * cap)()
+        n = lib().lcg_pair_schedule(self.h, waves, pool_ints, members, words, rounds, cap)
+        if n < 0:
+            raise LcgError("lcg_pair_schedule (%d)" % n)
+        out = []
+        for g in range(n):
+            w, off, size, lr = words[4 * g], words[4 * g + 1], words[4 * g + 2], words[4 * g + 3]
+            out.append((w & 0xfff, (w >> 12) & 0xf, w >> 16, off, size, lr & 0xffff, lr >> 16))
+        return out, list(rounds)[:self.lenA + 2]
 
     def free(self):
         if self.h:

@@ -498,6 +498,82 @@ static void capture_pf_host(lcg_batch &B, lcg_pair *const *pairs, int npairs) {
     }
 }
 
+namespace {
+// LDS schedule of the fill of D for one pair (A: arena holding the pair at
+// the offsets of d).  The groups of each row are dealt to K workgroup
+// members (longest first to the least loaded member); each member packs its
+// groups into rounds of at most `waves` regions that fit `pool` ints
+// together, largest first.  Writes G[5] = round | wave<<12 | member<<16,
+// G[6] = LDS offset, and the number of rounds of each row.  False if a row
+// needs 4096 rounds or more.
+bool schedule_groups(int32_t *A, const PairDesc &d, int waves, int pool, int K) {
+    for (int al = d.n; al >= 1; --al) {
+        const int g0 = A[d.rows + al + 1], g1 = A[d.rows + al];
+        auto rows_of = [&](int g) {
+            const int32_t *G = A + d.groups + g * LCG_GROUP_INTS;
+            return (G[1] & 0xffff) - (G[0] & 0xffff);
+        };
+        std::vector<std::vector<int>> mem(K);
+        {
+            std::vector<int> by_len;
+            for (int g = g0; g < g1; ++g) by_len.push_back(g);
+            std::stable_sort(by_len.begin(), by_len.end(), [&](int x, int y) { return rows_of(x) > rows_of(y); });
+            std::vector<long> load(K, 0);
+            for (int g : by_len) {
+                const int m = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                mem[m].push_back(g);
+                load[m] += rows_of(g);
+            }
+        }
+        int nr = 0;
+        for (int m = 0; m < K; ++m) {
+            auto &lst = mem[m];
+            std::stable_sort(lst.begin(), lst.end(), [&](int x, int y) {
+                return A[d.groups + x * LCG_GROUP_INTS + 7] > A[d.groups + y * LCG_GROUP_INTS + 7];
+            });
+            std::vector<int> used, cnt;
+            for (int g : lst) {
+                int32_t *G = A + d.groups + g * LCG_GROUP_INTS;
+                size_t r = 0;
+                while (r < used.size() && (cnt[r] >= waves || used[r] + G[7] > pool)) ++r;
+                if (r == used.size()) {
+                    used.push_back(0);
+                    cnt.push_back(0);
+                }
+                G[5] = (int)r | (cnt[r] << 12) | (m << 16);
+                G[6] = used[r];
+                used[r] += G[7];
+                cnt[r] += 1;
+            }
+            nr = std::max(nr, (int)used.size());
+        }
+        if (nr >= 4096) return false;
+        A[d.rounds + al] = nr;
+    }
+    return true;
+}
+}  // namespace
+
+extern "C" int lcg_pair_schedule(const lcg_pair *p, int32_t waves, int32_t pool_ints, int32_t members,
+                                 int32_t *group_words, int32_t *rounds, int32_t cap) {
+    if (!p || waves < 1 || waves > 16 || members < 1 || members > 64 || pool_ints < 4) return LCG_E_ARG;
+    const PairDesc &d = p->desc;
+    if (cap < 4 * d.ngroups || cap < d.n + 2) return LCG_E_ARG;
+    for (int g = 0; g < d.ngroups; ++g)
+        if (p->blob[d.groups + g * LCG_GROUP_INTS + 7] > pool_ints) return LCG_E_CAPACITY;
+    std::vector<int32_t> A = p->blob;
+    if (!schedule_groups(A.data(), d, waves, pool_ints, members)) return LCG_E_CAPACITY;
+    for (int g = 0; g < d.ngroups; ++g) {
+        const int32_t *G = A.data() + d.groups + g * LCG_GROUP_INTS;
+        group_words[4 * g] = G[5];
+        group_words[4 * g + 1] = G[6];
+        group_words[4 * g + 2] = G[7];
+        group_words[4 * g + 3] = G[0];
+    }
+    for (int i = 0; i <= d.n + 1; ++i) rounds[i] = A[d.rounds + i];
+    return d.ngroups;
+}
+
 extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t device, lcg_batch **out, char *err,
                                 int errlen) {
     if (!pairs || npairs <= 0 || !out) return LCG_E_ARG;
@@ -583,54 +659,9 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     if (const char *w = std::getenv("LCG_WG_PER_PAIR")) K = std::min(64, std::max(1, std::atoi(w)));
     K = std::min(K, std::max(1, cus / npairs));  // all workgroups of the fill stay co-resident
     for (int k = 0; k < npairs; ++k) {
-        const PairDesc &d = B->descs[k];
-        int32_t *A = B->host_arena.data();
-        for (int al = d.n; al >= 1; --al) {
-            const int g0 = A[d.rows + al + 1], g1 = A[d.rows + al];
-            std::vector<int> idx;
-            for (int g = g0; g < g1; ++g) idx.push_back(g);
-            auto rows_of = [&](int g) {
-                const int32_t *G = A + d.groups + g * LCG_GROUP_INTS;
-                return (G[1] & 0xffff) - (G[0] & 0xffff);
-            };
-            std::vector<std::vector<int>> mem(K);
-            {
-                std::vector<int> by_len = idx;
-                std::stable_sort(by_len.begin(), by_len.end(), [&](int x, int y) { return rows_of(x) > rows_of(y); });
-                std::vector<long> load(K, 0);
-                for (int g : by_len) {
-                    const int m = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-                    mem[m].push_back(g);
-                    load[m] += rows_of(g);
-                }
-            }
-            int nr = 0;
-            for (int m = 0; m < K; ++m) {
-                auto &lst = mem[m];
-                std::stable_sort(lst.begin(), lst.end(), [&](int x, int y) {
-                    return A[d.groups + x * LCG_GROUP_INTS + 7] > A[d.groups + y * LCG_GROUP_INTS + 7];
-                });
-                std::vector<int> used, cnt;
-                for (int g : lst) {
-                    int32_t *G = A + d.groups + g * LCG_GROUP_INTS;
-                    size_t r = 0;
-                    while (r < used.size() && (cnt[r] >= waves || used[r] + G[7] > pool)) ++r;
-                    if (r == used.size()) {
-                        used.push_back(0);
-                        cnt.push_back(0);
-                    }
-                    G[5] = (int)r | (cnt[r] << 12) | (m << 16);
-                    G[6] = used[r];
-                    used[r] += G[7];
-                    cnt[r] += 1;
-                }
-                nr = std::max(nr, (int)used.size());
-            }
-            if (nr >= 4096) {
-                set_err(err, errlen, "too many LDS rounds in one row");
-                return LCG_E_CAPACITY;
-            }
-            A[d.rounds + al] = nr;
+        if (!schedule_groups(B->host_arena.data(), B->descs[k], waves, pool, K)) {
+            set_err(err, errlen, "too many LDS rounds in one row");
+            return LCG_E_CAPACITY;
         }
     }
     B->wg_per_pair = K;
