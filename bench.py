@@ -247,6 +247,16 @@ def extra_configs(args, api, synth, torch, nthreads):
         import math
         ok = sum(1 for k, o in zip(idx, ores)
                  if abs(math.log(B.pf_result(k)[0]) - math.log(o.pf)) <= 1e-9 * max(1.0, abs(math.log(o.pf))))
+        fit = None
+        if wl == "grad256":
+            # 20 gradient steps through the torch.autograd binding, fitting theta
+            # to the expected alignment statistics of a seeded generating theta
+            from locarna_amd.torch_grad import fit_expected_statistics, generating_theta
+            t0 = time.perf_counter()
+            losses, dist, _ = fit_expected_statistics(B, generating_theta(seed=6, scale=30.0), steps=20)
+            fit = {"steps": 20, "s_per_step": round((time.perf_counter() - t0) / 21, 3),
+                   "loss_first": round(losses[0], 4), "loss_last": round(losses[-1], 4),
+                   "theta_dist_first": round(dist[0], 3), "theta_dist_last": round(dist[-1], 3)}
         B.destroy()
         res[wl] = {"pairs": len(P), "pf_ms_per_step": round(dev_ms, 3),
                    "grad_pairs_per_sec": round(len(P) / wall, 2),
@@ -257,6 +267,8 @@ def extra_configs(args, api, synth, torch, nthreads):
                    "cpu_baseline_pairs_per_sec": round(len(idx) / dt, 3), "cpu_cores": nthreads,
                    "cpu_sample": "%d of %d pairs (inside + outside, CPU oracle)" % (len(idx), len(P)),
                    "logZ_within_1e-9": "%d/%d" % (ok, len(idx))}
+        if fit:
+            res[wl]["fit"] = fit
     return res
 
 

@@ -1,0 +1,47 @@
+"""The torch.autograd binding of d log Z / d theta: backward against central
+finite differences of the GPU's own log Z along random directions, and the
+fitting loop (expected statistics of a generating theta) decreasing its loss
+and its distance to the generating theta."""
+import pytest
+import torch
+
+from locarna_amd import api, synth
+from locarna_amd.torch_grad import LogZ, fit_expected_statistics, generating_theta
+
+pytestmark = pytest.mark.gpu
+
+
+def _batch(npairs):
+    pps, pairs, span = synth.workload("pf16x150", limit_pairs=npairs)
+    pa = api.locarna_p_params(max_bp_span=span)
+    P = [api.Pair(pps[a], pps[b], pa) for a, b in pairs]
+    return api.Batch(P)
+
+
+def test_backward_matches_finite_differences():
+    B = _batch(2)
+    gen = torch.Generator().manual_seed(11)
+    theta = (torch.randn(api.NTHETA, generator=gen, dtype=torch.float64) * 5).requires_grad_(True)
+    w = torch.tensor([0.7, -1.3], dtype=torch.float64)
+    (LogZ.apply(theta, B) * w).sum().backward()
+    g = theta.grad.clone()
+    h = 1e-3
+    for trial in range(3):
+        v = torch.randn(api.NTHETA, generator=gen, dtype=torch.float64)
+        v /= v.norm()
+        with torch.no_grad():
+            up = (LogZ.apply(theta + h * v, B) * w).sum().item()
+            dn = (LogZ.apply(theta - h * v, B) * w).sum().item()
+        fd = (up - dn) / (2 * h)
+        assert abs(fd - (g @ v).item()) <= 1e-6 * max(1.0, abs(fd)), (trial, fd, (g @ v).item())
+    B.destroy()
+
+
+def test_fitting_moves_towards_generating_theta():
+    B = _batch(4)
+    theta_gen = generating_theta(seed=6, scale=30.0)
+    losses, dist, _ = fit_expected_statistics(B, theta_gen, steps=6)
+    print("losses", losses, "dist", dist, flush=True)
+    assert losses[-1] < losses[0]
+    assert dist[-1] < dist[0]
+    B.destroy()
