@@ -74,10 +74,37 @@ int lcg_pair_band(const lcg_pair *pair, int32_t *min_col, int32_t *max_col, int3
 int lcg_pair_arcmatches(const lcg_pair *pair, int32_t *al, int32_t *ar, int32_t *bl, int32_t *br, int64_t *score,
                         int32_t cap);
 
+/* Host preparation of one pair without the arc-match enumeration: RnaData,
+ * trace band, BasePairs and the score tables as lcg_pair_prepare, while the
+ * arc matches (ArcMatches constructor, arc_matches.cc:130-188: predicate
+ * :19-48 over all arc pairs, index order, lists by common left and right
+ * ends) are compacted on the device by lcg_batch_create, which also builds
+ * every arc-match structure of the pair's device image there.  Such pairs
+ * report -1 arc matches and groups from lcg_pair_info (see
+ * lcg_batch_pair_info), and lcg_pair_arcmatches / lcg_pair_schedule return
+ * LCG_E_STATE.  A batch takes either kind of pair, not both; LocARNA-P
+ * (lcg_batch_pf_run) needs pairs from lcg_pair_prepare. */
+int lcg_pair_prepare_device(const char *ppA, const char *ppB, const lcg_params *params, lcg_pair **out, char *err,
+                            int errlen);
+
 /* Device batch of prepared pairs (device >= 0: HIP device ordinal).  The
  * pairs may be freed after the call. */
 int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t device, lcg_batch **out, char *err,
                      int errlen);
+/* Arc matches and groups of pair k of a batch (ArcMatches::num_arc_matches;
+ * the groups of AlignerImpl::align_D, aligner.cc:659-732) and the size of
+ * its device image in ints. */
+int lcg_batch_pair_info(const lcg_batch *batch, int32_t k, int32_t *num_arcmatches, int32_t *num_groups,
+                        int64_t *image_ints);
+/* Copies the device image of pair k (device_layout.h) to dst; returns its
+ * size in ints (no counterpart in the reference; for the comparison of the
+ * device compaction with the host packing). */
+int lcg_batch_segment(const lcg_batch *batch, int32_t k, int32_t *dst, int64_t cap);
+/* Reruns the device compaction of a batch of lcg_pair_prepare_device pairs
+ * (both passes, same inputs, same result) and reports the device time of
+ * each pass (no counterpart in the reference; for timing).  Results of an
+ * earlier lcg_batch_run are invalidated. */
+int lcg_batch_compact_run(lcg_batch *batch, float *count_ms, float *build_ms);
 void lcg_batch_destroy(lcg_batch *batch);
 /* Runs the kernels for all pairs of the batch.
  *   mode 0: batched, one persistent launch over a work queue (all pairs)

@@ -50,6 +50,7 @@ def _sig(L):
     c = ctypes
     L.lcg_pair_prepare.argtypes = [c.c_char_p, c.c_char_p, c.POINTER(Params), c.POINTER(c.c_void_p),
                                    c.c_char_p, c.c_int]
+    L.lcg_pair_prepare_device.argtypes = L.lcg_pair_prepare.argtypes
     L.lcg_pair_free.argtypes = [c.c_void_p]
     L.lcg_pair_info.argtypes = [c.c_void_p, c.POINTER(c.c_int32), c.POINTER(c.c_int32),
                                 c.POINTER(c.c_int32), c.POINTER(c.c_int32), c.POINTER(c.c_int64)]
@@ -75,6 +76,10 @@ def _sig(L):
     L.lcg_batch_pf_D.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_void_p, c.c_int32]
     L.lcg_batch_bm_probs.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_void_p, c.c_int32]
     L.lcg_batch_pf_grad.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int32]
+    L.lcg_batch_pair_info.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_int32), c.POINTER(c.c_int32),
+                                      c.POINTER(c.c_int64)]
+    L.lcg_batch_segment.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int64]
+    L.lcg_batch_compact_run.argtypes = [c.c_void_p, c.POINTER(c.c_float), c.POINTER(c.c_float)]
     return L
 
 
@@ -92,14 +97,18 @@ def lib():
 
 
 class Pair:
-    """A prepared pair (lcg_pair_prepare)."""
+    """A prepared pair (lcg_pair_prepare).  With device_compaction=True the
+    arc matches are not enumerated on the host (lcg_pair_prepare_device):
+    lcg_batch_create compacts them on the device."""
 
-    def __init__(self, ppA, ppB, params=None):
+    def __init__(self, ppA, ppB, params=None, device_compaction=False):
         L = lib()
         self.params = params or locarna_params()
+        self.device_compaction = device_compaction
         h = ctypes.c_void_p()
         err = ctypes.create_string_buffer(512)
-        rc = L.lcg_pair_prepare(ppA.encode(), ppB.encode(), ctypes.byref(self.params), ctypes.byref(h), err, 512)
+        fn = L.lcg_pair_prepare_device if device_compaction else L.lcg_pair_prepare
+        rc = fn(ppA.encode(), ppB.encode(), ctypes.byref(self.params), ctypes.byref(h), err, 512)
         if rc != 0:
             raise LcgError("lcg_pair_prepare (%d): %s" % (rc, err.value.decode()))
         self.h = h
@@ -167,6 +176,31 @@ class Batch:
         L.lcg_batch_config(h, ctypes.byref(w), ctypes.byref(g), ctypes.byref(lds))
         self.waves, self.workgroups, self.lds_bytes = w.value, g.value, lds.value
 
+    def pair_info(self, k):
+        """(arc matches, groups, image ints) of pair k."""
+        nam, ng, sz = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int64()
+        rc = lib().lcg_batch_pair_info(self.h, k, ctypes.byref(nam), ctypes.byref(ng), ctypes.byref(sz))
+        if rc != 0:
+            raise LcgError("lcg_batch_pair_info (%d)" % rc)
+        return nam.value, ng.value, sz.value
+
+    def segment(self, k):
+        """Device image of pair k (int32 list)."""
+        n = self.pair_info(k)[2]
+        buf = (ctypes.c_int32 * max(n, 1))()
+        rc = lib().lcg_batch_segment(self.h, k, buf, n)
+        if rc < 0:
+            raise LcgError("lcg_batch_segment (%d)" % rc)
+        return buf[:rc]
+
+    def compact_run(self):
+        """Reruns the device compaction; returns (count_ms, build_ms)."""
+        a, b = ctypes.c_float(), ctypes.c_float()
+        rc = lib().lcg_batch_compact_run(self.h, ctypes.byref(a), ctypes.byref(b))
+        if rc != 0:
+            raise LcgError("lcg_batch_compact_run failed (%d)" % rc)
+        return a.value, b.value
+
     def run(self, mode=0, trace=True, count=False):
         ms = ctypes.c_float()
         flags = (1 if trace else 0) | (2 if count else 0)
@@ -217,7 +251,7 @@ class Batch:
         return f.value, t.value
 
     def D(self, k):
-        n = max(self.pairs[k].num_arcmatches, 1)
+        n = max(self.pair_info(k)[0], 1)
         out = (ctypes.c_int64 * n)()
         cnt = lib().lcg_batch_D(self.h, k, out, n)
         if cnt < 0:

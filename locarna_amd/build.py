@@ -26,6 +26,7 @@ HIPAPI_SOURCES = [
 HIP_SOURCES = [
     "align_kernels.hip",
     "pf_kernels.hip",
+    "compact_kernels.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

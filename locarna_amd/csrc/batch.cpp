@@ -11,12 +11,19 @@
 #include <string>
 #include <vector>
 
+#include "compact_layout.h"
 #include "device_layout.h"
 #include "pf_layout.h"
 #include "host.hpp"
 #include "locarna_amd.h"
 
 using namespace locarna_amd;
+
+extern "C" int lcg_launch_compact_count(const CompactDesc *cds, const int *cin, int *ws, int *stats, int npairs,
+                                        int lds_bytes, hipStream_t stream);
+extern "C" int lcg_launch_compact_build(const CompactDesc *cds, const PairDesc *descs, const int *cin, int *arena,
+                                        int *ws, int *ws2, int *stats, CompactParams cp, int lds_bytes,
+                                        hipStream_t stream);
 
 extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, int *scratch,
                                 int *barriers, BatchParams bp, int grid, int waves, size_t lds_bytes, BatchParams bp_top,
@@ -36,6 +43,11 @@ struct lcg_pair {
     int64_t cells = 0;  // DP cells (groups + top level) as a work estimate
     int ecap = 0;       // max columns of a group region (E vector)
     std::vector<int> entry_of;  // arc match -> candidate entry
+    // prepared for the device compaction (lcg_pair_prepare_device): no arc
+    // matches on the host, no blob; the compaction inputs instead
+    bool deferred = false;
+    std::vector<int32_t> cimg;  // compaction inputs, offsets of cdesc relative to it
+    CompactDesc cdesc;
 };
 
 namespace {
@@ -52,6 +64,58 @@ int32_t reserve(std::vector<int32_t> &blob, size_t count, int32_t fill = 0) {
     size_t off = blob.size();
     blob.resize(off + count, fill);
     return (int32_t)off;
+}
+
+// Offsets of a pair's arena image (device_layout.h) from its sizes, in the
+// order the host packing reserves them; the device compaction lays out its
+// pairs with it, so both produce the same image.  Returns the image size.
+size_t layout_pair(PairDesc &d, int n, int m, int ncells, int ngroups, int nam, int nfblk) {
+    size_t off = 0;
+    auto take = [&](size_t count) {
+        const int32_t o = (int32_t)off;
+        off += count;
+        return o;
+    };
+    auto align = [&](size_t a) { off = (off + a - 1) / a * a; };
+    d.band = take(3 * (size_t)(n + 1) + 1);
+    d.codes = take((size_t)n + m + 2);
+    d.gaps = take((size_t)n + m + 2);
+    d.sigma = take(LCG_NCODES * LCG_NCODES);
+    d.rows = take((size_t)n + 2);
+    d.rounds = take((size_t)n + 2);
+    d.groups = take((size_t)LCG_GROUP_INTS * ngroups);
+    align(4);
+    d.gams = take(4 * (size_t)nam);
+    d.amscore = take(nam);
+    d.cptr = take((size_t)ncells + 1);
+    align(4);
+    d.cand = take(4 * (size_t)nam);
+    d.fptr = take((size_t)n + 2);
+    d.fcand = take(3 * (size_t)nam);
+    d.fblk = take(2 * (size_t)nfblk);
+    d.f2c = take(nam);
+    d.D = take(nam);
+    d.edges = take((size_t)n + m + 1);
+    d.structs = take((size_t)n + m);
+    d.tstack = take(2 * (size_t)(n + 1));
+    align(2);
+    d.out = take(12);
+    align(4);
+    return off;
+}
+
+// Top-level region of a pair: stored row width and LDS ints; adds the
+// top-level DP cells to `cells`.
+void top_region(const std::vector<int> &mn, const std::vector<int> &mx, int n, int m, int &wtop, int &top_ints,
+                int64_t &cells) {
+    wtop = 0;
+    for (int i = 0; i <= n; ++i) {
+        const int lo = (i == 0) ? 0 : std::max(0, mn[i]);
+        const int hi = std::min(m, mx[i]);
+        wtop = std::max(wtop, std::max(0, hi - lo + 1));
+        if (i > 0) cells += std::max(0, std::min(m, mx[i]) - std::max(1, mn[i]) + 1);
+    }
+    top_ints = ((m + 1) + wtop + (n + 1) * wtop + 3) & ~3;
 }
 
 void pack(lcg_pair &P) {
@@ -269,15 +333,116 @@ void pack(lcg_pair &P) {
     d.out = reserve(blob, 12);
     while (blob.size() & 3) blob.push_back(0);
     d.gmax = gbuf;
+    {
+        PairDesc L = d;
+        const size_t size = layout_pair(L, n, m, ncells, d.ngroups, nam, d.nfblk);
+        if (size != blob.size() || std::memcmp(&L, &d, sizeof d) != 0) throw failure("internal: arena layout");
+    }
     P.desc = d;
     P.cells = cells;
     P.ecap = ecap;
 }
 
+// Inputs of the device compaction of one pair: arcs in index order with
+// their weights and base-pair types, the arcs of each left end (a range of
+// the index order) and of each right end (right adjacency order), the band
+// and the riboX table.
+void prepare_compaction_inputs(lcg_pair &P) {
+    const TraceController &tc = *P.tc;
+    const Scoring &sc = *P.sc;
+    const BasePairs &bA = *P.bA, &bB = *P.bB;
+    const int n = P.A->length(), m = P.B->length();
+    const int NA = bA.num_bps(), NB = bB.num_bps();
+    auto &img = P.cimg;
+    img.clear();
+    CompactDesc c;
+    std::memset(&c, 0, sizeof c);
+    c.n = n;
+    c.m = m;
+    c.NA = NA;
+    c.NB = NB;
+    c.W64 = (NB + 63) / 64;
+    auto side = [&](const BasePairs &bp, const RnaData &rna, int len, bool isA) {
+        const int N = bp.num_bps();
+        const int32_t arcs = reserve(img, N), w = reserve(img, N), pt = reserve(img, N);
+        const int32_t la = reserve(img, 2 * (size_t)(len + 2)), rp = reserve(img, (size_t)len + 2);
+        const int32_t radj = reserve(img, N);
+        for (int a = 0; a < N; ++a) {
+            const Arc &x = bp.arc(a);
+            img[arcs + a] = x.left | (x.right << 16);
+            img[w + a] = (int32_t)(isA ? sc.weightA(a) : sc.weightB(a));
+            const int u = nt_code(rna.nt(x.left)), v = nt_code(rna.nt(x.right));
+            img[pt + a] = (u >= 0 && v >= 0) ? u * 4 + v : -1;
+        }
+        for (int i = 0; i <= len + 1; ++i) {
+            const auto &lst = bp.left_adj(i);
+            const int b0 = lst.empty() ? 0 : lst.front();
+            for (size_t t = 0; t < lst.size(); ++t)
+                if (lst[t] != b0 + (int)t) throw failure("internal: arcs of a left end not contiguous");
+            img[la + 2 * i] = b0;
+            img[la + 2 * i + 1] = b0 + (int)lst.size();
+        }
+        int u = 0;
+        for (int i = 0; i <= len + 1; ++i) {
+            img[rp + i] = u;
+            if (i <= len)
+                for (int a : bp.right_adj(i)) img[radj + u++] = a;
+        }
+        if (u != N) throw failure("internal: right adjacency incomplete");
+        if (isA) {
+            c.arcsA = arcs, c.wA = w, c.ptA = pt, c.lA = la, c.rpA = rp, c.radjA = radj;
+        } else {
+            c.arcsB = arcs, c.wB = w, c.ptB = pt, c.lB = la, c.rpB = rp, c.radjB = radj;
+        }
+    };
+    side(bA, *P.A, n, true);
+    side(bB, *P.B, m, false);
+    c.band = reserve(img, 2 * (size_t)(n + 1));
+    for (int i = 0; i <= n; ++i) {
+        img[c.band + i] = tc.min_col(i);
+        img[c.band + n + 1 + i] = tc.max_col(i);
+    }
+    c.tab = reserve(img, 256);
+    c.tau = P.params.tau;
+    c.ribosum = P.params.use_ribosum != 0;
+    if (c.tau != 0 && c.ribosum)
+        for (int x = 0; x < 256; ++x)
+            img[c.tab + x] = (int32_t)Scoring::ribosum_arc_term((x >> 6) & 3, (x >> 4) & 3, (x >> 2) & 3, x & 3);
+    // ArcMatches without max-diff-am / max-diff-at-am (arc_matches.cc:19-34)
+    c.dlen = c.dend = std::max(n, m);
+    while (img.size() & 3) img.push_back(0);
+    P.cdesc = c;
+}
+
+// Host-computed parts of the arena image of a device-compacted pair (band,
+// codes, gap scores, sigma; edges cleared to -1).  The device writes the
+// arc-match structures.
+void write_host_parts(const lcg_pair &P, const PairDesc &d, int32_t *img) {
+    const TraceController &tc = *P.tc;
+    const Scoring &sc = *P.sc;
+    const int n = d.n, m = d.m;
+    int ncells = 0;
+    for (int i = 0; i <= n; ++i) {
+        img[d.band + i] = tc.min_col(i);
+        img[d.band + n + 1 + i] = tc.max_col(i);
+        img[d.band + 2 * (n + 1) + i] = ncells;
+        ncells += std::max(0, tc.max_col(i) - tc.min_col(i) + 1);
+    }
+    img[d.band + 3 * (n + 1)] = ncells;
+    for (int i = 1; i <= n; ++i) img[d.codes + i] = sc.codeA(i);
+    for (int j = 1; j <= m; ++j) img[d.codes + n + 1 + j] = sc.codeB(j);
+    for (int i = 1; i <= n; ++i) img[d.gaps + i] = (int32_t)sc.gapA(i);
+    for (int j = 1; j <= m; ++j) img[d.gaps + n + 1 + j] = (int32_t)sc.gapB(j);
+    for (int a = 0; a < LCG_NCODES; ++a)
+        for (int b = 0; b < LCG_NCODES; ++b) img[d.sigma + a * LCG_NCODES + b] = (int32_t)sc.sigma_code(a, b);
+    for (int t = 0; t < n + m + 1; ++t) img[d.edges + t] = -1;
+}
+
 }  // namespace
 
-extern "C" int lcg_pair_prepare(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair **out, char *err,
-                                int errlen) {
+namespace {
+int prepare(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair **out, char *err, int errlen,
+            bool deferred) {
     if (!ppA || !ppB || !p || !out) return LCG_E_ARG;
     *out = nullptr;
     try {
@@ -301,9 +466,20 @@ extern "C" int lcg_pair_prepare(const char *ppA, const char *ppB, const lcg_para
                                         p->sequ_local != 0);
         P->bA.reset(new BasePairs(P->A.get(), p->min_prob));
         P->bB.reset(new BasePairs(P->B.get(), p->min_prob));
-        P->am.reset(new ArcMatches(*P->bA, *P->bB, *P->tc));
-        P->sc.reset(new Scoring(*P->A, *P->B, *P->am, sp));
-        pack(*P);
+        if (deferred) {
+            P->am.reset(new ArcMatches(*P->bA, *P->bB));
+            P->sc.reset(new Scoring(*P->A, *P->B, *P->am, sp));
+            P->deferred = true;
+            std::memset(&P->desc, 0, sizeof P->desc);
+            P->desc.n = P->A->length();
+            P->desc.m = P->B->length();
+            P->desc.nam = P->desc.ngroups = -1;
+            prepare_compaction_inputs(*P);
+        } else {
+            P->am.reset(new ArcMatches(*P->bA, *P->bB, *P->tc));
+            P->sc.reset(new Scoring(*P->A, *P->B, *P->am, sp));
+            pack(*P);
+        }
         *out = P.release();
         return LCG_OK;
     } catch (failure &f) {
@@ -313,6 +489,17 @@ extern "C" int lcg_pair_prepare(const char *ppA, const char *ppB, const lcg_para
         set_err(err, errlen, e.what());
         return LCG_E_INPUT;
     }
+}
+}  // namespace
+
+extern "C" int lcg_pair_prepare(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair **out, char *err,
+                                int errlen) {
+    return prepare(ppA, ppB, p, out, err, errlen, false);
+}
+
+extern "C" int lcg_pair_prepare_device(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair **out,
+                                       char *err, int errlen) {
+    return prepare(ppA, ppB, p, out, err, errlen, true);
 }
 
 extern "C" void lcg_pair_free(lcg_pair *p) { delete p; }
@@ -340,6 +527,7 @@ extern "C" int lcg_pair_band(const lcg_pair *p, int32_t *mn, int32_t *mx, int32_
 extern "C" int lcg_pair_arcmatches(const lcg_pair *p, int32_t *al, int32_t *ar, int32_t *bl, int32_t *br,
                                    int64_t *score, int32_t cap) {
     if (!p) return LCG_E_ARG;
+    if (p->deferred) return LCG_E_STATE;
     const int nam = p->am->num_arc_matches();
     if (cap < nam) return LCG_E_ARG;
     for (int k = 0; k < nam; ++k) {
@@ -417,6 +605,15 @@ struct lcg_batch {
     PfBatchParams pbp;
     bool pf_have = false;
     std::vector<double> pf_result;
+    // device compaction (pairs from lcg_pair_prepare_device)
+    bool device_built = false;
+    std::vector<int32_t> cstats;      // per pair LCG_CS_INTS (counts, status)
+    std::vector<CompactDesc> cdescs;  // absolute offsets
+    int32_t *d_cin = nullptr, *d_ws = nullptr, *d_ws2 = nullptr, *d_cstats = nullptr;
+    CompactDesc *d_cdescs = nullptr;
+    CompactParams cp;
+    int comp_lds = 0;
+    float count_ms = -1, build_ms = -1;
 };
 
 namespace {
@@ -557,6 +754,7 @@ bool schedule_groups(int32_t *A, const PairDesc &d, int waves, int pool, int K) 
 extern "C" int lcg_pair_schedule(const lcg_pair *p, int32_t waves, int32_t pool_ints, int32_t members,
                                  int32_t *group_words, int32_t *rounds, int32_t cap) {
     if (!p || waves < 1 || waves > 16 || members < 1 || members > 64 || pool_ints < 4) return LCG_E_ARG;
+    if (p->deferred) return LCG_E_STATE;
     const PairDesc &d = p->desc;
     if (cap < 4 * d.ngroups || cap < d.n + 2) return LCG_E_ARG;
     for (int g = 0; g < d.ngroups; ++g)
@@ -574,6 +772,168 @@ extern "C" int lcg_pair_schedule(const lcg_pair *p, int32_t waves, int32_t pool_
     return d.ngroups;
 }
 
+namespace {
+// Device compaction, pass 1: uploads the compaction inputs of the pairs,
+// counts arc matches, groups and fill blocks on the device, and lays out the
+// arena image of every pair from the counts (host parts written, arc-match
+// structures left to pass 2).
+int device_count_and_layout(lcg_batch &B, lcg_pair *const *pairs, int npairs, std::vector<int64_t> &work,
+                            char *err, int errlen) {
+    std::vector<int32_t> cin;
+    size_t ws_ints = 0;
+    int max_len = 1;
+    B.cdescs.resize(npairs);
+    for (int k = 0; k < npairs; ++k) {
+        const lcg_pair *P = pairs[k];
+        const int32_t base = (int32_t)cin.size();
+        cin.insert(cin.end(), P->cimg.begin(), P->cimg.end());
+        CompactDesc c = P->cdesc;
+        int32_t *offs[] = {&c.arcsA, &c.arcsB, &c.wA, &c.wB, &c.ptA, &c.ptB, &c.lA, &c.lB,
+                           &c.rpA, &c.rpB, &c.radjA, &c.radjB, &c.band, &c.tab};
+        for (int32_t *o : offs) *o += base;
+        const size_t words = (size_t)c.NA * c.W64;
+        ws_ints = (ws_ints + 1) & ~(size_t)1;
+        c.ws_bits = (int32_t)ws_ints;
+        ws_ints += 2 * words;
+        c.ws_wpre = (int32_t)ws_ints;
+        ws_ints += words;
+        c.ws_rowcnt = (int32_t)ws_ints;
+        ws_ints += c.NA;
+        c.ws_sblk = (int32_t)ws_ints;
+        ws_ints += c.NA + 1;
+        c.ws_blka = (int32_t)ws_ints;
+        ws_ints += c.NA;
+        c.ws_blks = (int32_t)ws_ints;
+        ws_ints += c.NA;
+        if (ws_ints >= (size_t)0x7fffffff || cin.size() >= (size_t)0x7fffffff) {
+            set_err(err, errlen, "batch too large for the compaction workspace");
+            return LCG_E_CAPACITY;
+        }
+        B.cdescs[k] = c;
+        max_len = std::max(max_len, std::max(c.n, c.m));
+    }
+    B.comp_lds = (3 * (max_len + 1) + 1) * 4;
+    if (B.comp_lds > 160 * 1024 - 1024) {
+        set_err(err, errlen, "sequence too long for the compaction kernels");
+        return LCG_E_CAPACITY;
+    }
+    hipError_t e;
+    if ((e = hipMalloc(&B.d_cin, std::max<size_t>(cin.size(), 4) * 4)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMalloc(&B.d_ws, std::max<size_t>(ws_ints, 4) * 4)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMalloc(&B.d_cdescs, sizeof(CompactDesc) * npairs)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMalloc(&B.d_cstats, 4 * (size_t)LCG_CS_INTS * npairs)) != hipSuccess)
+        return hip_fail(e, err, errlen);
+    if ((e = hipMemcpy(B.d_cin, cin.data(), cin.size() * 4, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, err, errlen);
+    if ((e = hipMemcpy(B.d_cdescs, B.cdescs.data(), sizeof(CompactDesc) * npairs, hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return hip_fail(e, err, errlen);
+    if ((e = hipEventRecord(B.ev0, B.stream)) != hipSuccess) return hip_fail(e, err, errlen);
+    if (int rc = lcg_launch_compact_count(B.d_cdescs, B.d_cin, B.d_ws, B.d_cstats, npairs, B.comp_lds, B.stream))
+        return hip_fail((hipError_t)rc, err, errlen);
+    if ((e = hipEventRecord(B.ev1, B.stream)) != hipSuccess) return hip_fail(e, err, errlen);
+    B.cstats.resize((size_t)LCG_CS_INTS * npairs);
+    if ((e = hipMemcpyAsync(B.cstats.data(), B.d_cstats, B.cstats.size() * 4, hipMemcpyDeviceToHost, B.stream)) !=
+        hipSuccess)
+        return hip_fail(e, err, errlen);
+    if ((e = hipStreamSynchronize(B.stream)) != hipSuccess) return hip_fail(e, err, errlen);
+    hipEventElapsedTime(&B.count_ms, B.ev0, B.ev1);
+    // arena layout from the counts
+    B.descs.resize(npairs);
+    size_t total = 0;
+    std::vector<size_t> size(npairs);
+    for (int k = 0; k < npairs; ++k) {
+        const lcg_pair &P = *pairs[k];
+        const int32_t *S = &B.cstats[(size_t)LCG_CS_INTS * k];
+        PairDesc d;
+        std::memset(&d, 0, sizeof d);
+        d.n = P.desc.n;
+        d.m = P.desc.m;
+        d.nam = S[LCG_CS_NAM];
+        d.ngroups = S[LCG_CS_NGROUPS];
+        d.nfblk = S[LCG_CS_NBLK] + 1;
+        int ncells = 0;
+        for (int i = 0; i <= d.n; ++i) ncells += std::max(0, P.tc->max_col(i) - P.tc->min_col(i) + 1);
+        d.ncells = ncells;
+        size[k] = layout_pair(d, d.n, d.m, ncells, d.ngroups, d.nam, d.nfblk);
+        d.gmax = S[LCG_CS_GMAX];
+        int64_t cells = (int64_t)(((uint64_t)(uint32_t)S[LCG_CS_CELLS + 1] << 32) | (uint32_t)S[LCG_CS_CELLS]);
+        top_region(P.tc->min_cols(), P.tc->max_cols(), d.n, d.m, d.wtop, d.top_ints, cells);
+        work[k] = cells;
+        B.descs[k] = d;
+        total += size[k];
+    }
+    if (total >= (size_t)0x7fffffff) {
+        set_err(err, errlen, "batch too large for one arena");
+        return LCG_E_CAPACITY;
+    }
+    B.host_arena.assign(total, 0);
+    size_t base = 0;
+    for (int k = 0; k < npairs; ++k) {
+        PairDesc &d = B.descs[k];
+        write_host_parts(*pairs[k], d, B.host_arena.data() + base);
+        int32_t *offs[] = {&d.band, &d.codes, &d.gaps, &d.sigma, &d.rows, &d.groups, &d.gams, &d.amscore,
+                           &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.out, &d.rounds,
+                           &d.fptr, &d.fblk, &d.fcand, &d.f2c};
+        for (int32_t *o : offs) *o += (int32_t)base;
+        base += size[k];
+    }
+    return LCG_OK;
+}
+
+// Device compaction, pass 2 (after the arena image is on the device): the
+// arc-match structures and the LDS schedule of every pair.
+int device_build(lcg_batch &B, int npairs, int max_len, int waves, int pool, int K, char *err, int errlen) {
+    size_t ws2 = 0;
+    for (int k = 0; k < npairs; ++k) {
+        CompactDesc &c = B.cdescs[k];
+        const PairDesc &d = B.descs[k];
+        c.ws_eof = (int32_t)ws2;
+        ws2 += d.nam;
+        c.ws_fof = (int32_t)ws2;
+        ws2 += d.nam;
+        c.ws_sched = (int32_t)ws2;
+        ws2 += 3 * (size_t)d.ngroups + (size_t)K * (d.n + 2);
+        if (ws2 >= (size_t)0x7fffffff) {
+            set_err(err, errlen, "batch too large for the compaction workspace");
+            return LCG_E_CAPACITY;
+        }
+    }
+    hipError_t e;
+    if ((e = hipMalloc(&B.d_ws2, std::max<size_t>(ws2, 4) * 4)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMemcpy(B.d_cdescs, B.cdescs.data(), sizeof(CompactDesc) * npairs, hipMemcpyHostToDevice)) !=
+        hipSuccess)
+        return hip_fail(e, err, errlen);
+    B.cp.npairs = npairs;
+    B.cp.max_len = max_len;
+    B.cp.waves = waves;
+    B.cp.pool = pool;
+    B.cp.members = K;
+    if ((e = hipEventRecord(B.ev0, B.stream)) != hipSuccess) return hip_fail(e, err, errlen);
+    if (int rc = lcg_launch_compact_build(B.d_cdescs, B.d_descs, B.d_cin, B.d_arena, B.d_ws, B.d_ws2, B.d_cstats,
+                                          B.cp, B.comp_lds, B.stream))
+        return hip_fail((hipError_t)rc, err, errlen);
+    if ((e = hipEventRecord(B.ev1, B.stream)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMemcpyAsync(B.cstats.data(), B.d_cstats, B.cstats.size() * 4, hipMemcpyDeviceToHost, B.stream)) !=
+        hipSuccess)
+        return hip_fail(e, err, errlen);
+    if ((e = hipStreamSynchronize(B.stream)) != hipSuccess) return hip_fail(e, err, errlen);
+    hipEventElapsedTime(&B.build_ms, B.ev0, B.ev1);
+    for (int k = 0; k < npairs; ++k) {
+        const int st = B.cstats[(size_t)LCG_CS_INTS * k + LCG_CS_STATUS];
+        if (st == 1) {
+            set_err(err, errlen, "too many LDS rounds in one row");
+            return LCG_E_CAPACITY;
+        }
+        if (st != 0) {
+            set_err(err, errlen, "internal: device arc-match compaction inconsistent with its counts");
+            return LCG_E_STATE;
+        }
+    }
+    return LCG_OK;
+}
+}  // namespace
+
 extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t device, lcg_batch **out, char *err,
                                 int errlen) {
     if (!pairs || npairs <= 0 || !out) return LCG_E_ARG;
@@ -582,41 +942,63 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     B->device = device;
     B->npairs = npairs;
     const lcg_params &p0 = pairs[0]->params;
-    int max_len = 1, gmax = 4, top_max = 4, nfblk_max = 1;
-    size_t total = 0;
+    const bool deferred = pairs[0]->deferred;
     for (int k = 0; k < npairs; ++k) {
         const lcg_pair *P = pairs[k];
         if (P->params.indel_opening != p0.indel_opening || P->params.sequ_local != p0.sequ_local) {
             set_err(err, errlen, "all pairs of a batch must share indel_opening and sequ_local");
             return LCG_E_ARG;
         }
-        max_len = std::max(max_len, std::max(P->desc.n, P->desc.m));
-        gmax = std::max(gmax, P->desc.gmax);
-        nfblk_max = std::max(nfblk_max, P->desc.nfblk);
-        top_max = std::max(top_max, P->desc.top_ints);
-        total += P->blob.size();
+        if (P->deferred != deferred) {
+            set_err(err, errlen, "a batch takes either host-packed or device-compacted pairs, not both");
+            return LCG_E_ARG;
+        }
     }
-    if (total >= (size_t)0x7fffffff) {
-        set_err(err, errlen, "batch too large for one arena");
-        return LCG_E_CAPACITY;
+    hipError_t e;
+    int cus = 0;
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
+        return hip_fail(e, err, errlen);
+    if ((e = hipStreamCreate(&B->stream)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipEventCreate(&B->ev0)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipEventCreate(&B->ev1)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipEventCreate(&B->ev_mid)) != hipSuccess) return hip_fail(e, err, errlen);
+
+    std::vector<int64_t> work(npairs);
+    if (deferred) {
+        B->device_built = true;
+        if (int rc = device_count_and_layout(*B, pairs, npairs, work, err, errlen)) return rc;
+    } else {
+        size_t total = 0;
+        for (int k = 0; k < npairs; ++k) total += pairs[k]->blob.size();
+        if (total >= (size_t)0x7fffffff) {
+            set_err(err, errlen, "batch too large for one arena");
+            return LCG_E_CAPACITY;
+        }
+        B->host_arena.reserve(total);
+        for (int k = 0; k < npairs; ++k) {
+            const lcg_pair *P = pairs[k];
+            const int32_t base = (int32_t)B->host_arena.size();
+            B->host_arena.insert(B->host_arena.end(), P->blob.begin(), P->blob.end());
+            PairDesc d = P->desc;
+            int32_t *offs[] = {&d.band, &d.codes, &d.gaps, &d.sigma, &d.rows, &d.groups, &d.gams, &d.amscore,
+                               &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.out, &d.rounds,
+                               &d.fptr, &d.fblk, &d.fcand, &d.f2c};
+            for (int32_t *o : offs) *o += base;
+            B->descs.push_back(d);
+            work[k] = P->cells;
+        }
     }
-    B->host_arena.reserve(total);
-    for (int k = 0; k < npairs; ++k) {
-        const lcg_pair *P = pairs[k];
-        const int32_t base = (int32_t)B->host_arena.size();
-        B->host_arena.insert(B->host_arena.end(), P->blob.begin(), P->blob.end());
-        PairDesc d = P->desc;
-        int32_t *offs[] = {&d.band, &d.codes, &d.gaps, &d.sigma, &d.rows, &d.groups, &d.gams, &d.amscore,
-                           &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.out, &d.rounds,
-                           &d.fptr, &d.fblk, &d.fcand, &d.f2c};
-        for (int32_t *o : offs) *o += base;
-        B->descs.push_back(d);
+    int max_len = 1, gmax = 4, top_max = 4, nfblk_max = 1;
+    for (const PairDesc &d : B->descs) {
+        max_len = std::max(max_len, std::max(d.n, d.m));
+        gmax = std::max(gmax, d.gmax);
+        nfblk_max = std::max(nfblk_max, d.nfblk);
+        top_max = std::max(top_max, d.top_ints);
     }
     // longest pairs first (greedy LPT over the persistent workgroups)
     B->order.resize(npairs);
     std::iota(B->order.begin(), B->order.end(), 0);
-    std::vector<int64_t> work(npairs);
-    for (int k = 0; k < npairs; ++k) work[k] = pairs[k]->cells;
     std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) { return work[a] > work[b]; });
 
     // Launch shape: LDS = header (per-pair band / codes / gaps / sigma of the
@@ -648,20 +1030,16 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     // most `waves` regions that fit the pool together.  With K workgroups
     // per pair (fewer pairs than CUs), the groups of a row are first dealt to
     // the K members, longest first to the least loaded member.
-    int cus = 0;
-    {
-        hipError_t e0 = hipSetDevice(device);
-        if (e0 == hipSuccess) e0 = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device);
-        if (e0 != hipSuccess) return hip_fail(e0, err, errlen);
-    }
     int K = 1;
     if (npairs < cus) K = std::min(64, cus / npairs);
     if (const char *w = std::getenv("LCG_WG_PER_PAIR")) K = std::min(64, std::max(1, std::atoi(w)));
     K = std::min(K, std::max(1, cus / npairs));  // all workgroups of the fill stay co-resident
-    for (int k = 0; k < npairs; ++k) {
-        if (!schedule_groups(B->host_arena.data(), B->descs[k], waves, pool, K)) {
-            set_err(err, errlen, "too many LDS rounds in one row");
-            return LCG_E_CAPACITY;
+    if (!deferred) {
+        for (int k = 0; k < npairs; ++k) {
+            if (!schedule_groups(B->host_arena.data(), B->descs[k], waves, pool, K)) {
+                set_err(err, errlen, "too many LDS rounds in one row");
+                return LCG_E_CAPACITY;
+            }
         }
     }
     B->wg_per_pair = K;
@@ -697,14 +1075,8 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     B->top_waves = top_waves;
     B->top_lds_bytes = (size_t)(hdr + top_pool) * 4;
 
-    hipError_t e;
-    if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, err, errlen);
     B->grid = K > 1 ? npairs * K : std::min(npairs, cus * wgs_per_cu);
     B->top_grid = std::min(npairs, cus * top_wgs);
-    if ((e = hipStreamCreate(&B->stream)) != hipSuccess) return hip_fail(e, err, errlen);
-    if ((e = hipEventCreate(&B->ev0)) != hipSuccess) return hip_fail(e, err, errlen);
-    if ((e = hipEventCreate(&B->ev1)) != hipSuccess) return hip_fail(e, err, errlen);
-    if ((e = hipEventCreate(&B->ev_mid)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_arena, B->host_arena.size() * 4)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_descs, sizeof(PairDesc) * npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_order, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
@@ -719,7 +1091,11 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
         return hip_fail(e, err, errlen);
     if ((e = hipMemcpy(B->d_order, B->order.data(), 4 * (size_t)npairs, hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, err, errlen);
-    capture_pf_host(*B, pairs, npairs);
+    if (deferred) {
+        if (int rc = device_build(*B, npairs, max_len, waves, pool, K, err, errlen)) return rc;
+    } else {
+        capture_pf_host(*B, pairs, npairs);
+    }
     *out = B.release();
     return LCG_OK;
 }
@@ -742,8 +1118,63 @@ extern "C" void lcg_batch_destroy(lcg_batch *B) {
     if (B->ev0) hipEventDestroy(B->ev0);
     if (B->ev1) hipEventDestroy(B->ev1);
     if (B->ev_mid) hipEventDestroy(B->ev_mid);
+    if (B->d_cin) hipFree(B->d_cin);
+    if (B->d_ws) hipFree(B->d_ws);
+    if (B->d_ws2) hipFree(B->d_ws2);
+    if (B->d_cstats) hipFree(B->d_cstats);
+    if (B->d_cdescs) hipFree(B->d_cdescs);
     if (B->stream) hipStreamDestroy(B->stream);
     delete B;
+}
+
+extern "C" int lcg_batch_pair_info(const lcg_batch *B, int32_t k, int32_t *nam, int32_t *ngroups,
+                                   int64_t *arena_ints) {
+    if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    const PairDesc &d = B->descs[k];
+    if (nam) *nam = d.nam;
+    if (ngroups) *ngroups = d.ngroups;
+    if (arena_ints) {
+        const size_t end = k + 1 < B->npairs ? (size_t)B->descs[k + 1].band : B->host_arena.size();
+        *arena_ints = (int64_t)(end - (size_t)d.band);
+    }
+    return LCG_OK;
+}
+
+extern "C" int lcg_batch_segment(const lcg_batch *B, int32_t k, int32_t *dst, int64_t cap) {
+    if (!B || !dst || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    const size_t begin = (size_t)B->descs[k].band;
+    const size_t end = k + 1 < B->npairs ? (size_t)B->descs[k + 1].band : B->host_arena.size();
+    if ((size_t)cap < end - begin) return LCG_E_ARG;
+    HIPCHECK(hipSetDevice(B->device));
+    HIPCHECK(hipStreamSynchronize(B->stream));
+    HIPCHECK(hipMemcpy(dst, B->d_arena + begin, (end - begin) * 4, hipMemcpyDeviceToHost));
+    return (int)(end - begin);
+}
+
+extern "C" int lcg_batch_compact_run(lcg_batch *B, float *count_ms, float *build_ms) {
+    if (!B) return LCG_E_ARG;
+    if (!B->device_built) return LCG_E_STATE;
+    HIPCHECK(hipSetDevice(B->device));
+    HIPCHECK(hipEventRecord(B->ev0, B->stream));
+    if (int rc = lcg_launch_compact_count(B->d_cdescs, B->d_cin, B->d_ws, B->d_cstats, B->npairs, B->comp_lds,
+                                          B->stream))
+        return hip_fail((hipError_t)rc);
+    HIPCHECK(hipEventRecord(B->ev_mid, B->stream));
+    if (int rc = lcg_launch_compact_build(B->d_cdescs, B->d_descs, B->d_cin, B->d_arena, B->d_ws, B->d_ws2,
+                                          B->d_cstats, B->cp, B->comp_lds, B->stream))
+        return hip_fail((hipError_t)rc);
+    HIPCHECK(hipEventRecord(B->ev1, B->stream));
+    std::vector<int32_t> st(B->cstats.size());
+    HIPCHECK(hipMemcpyAsync(st.data(), B->d_cstats, st.size() * 4, hipMemcpyDeviceToHost, B->stream));
+    HIPCHECK(hipStreamSynchronize(B->stream));
+    HIPCHECK(hipEventElapsedTime(&B->count_ms, B->ev0, B->ev_mid));
+    HIPCHECK(hipEventElapsedTime(&B->build_ms, B->ev_mid, B->ev1));
+    if (count_ms) *count_ms = B->count_ms;
+    if (build_ms) *build_ms = B->build_ms;
+    for (size_t t = 0; t < st.size(); ++t)
+        if (st[t] != B->cstats[t]) return LCG_E_STATE;  // a rerun must reproduce the counts and status
+    B->have_results = false;
+    return LCG_OK;
 }
 
 extern "C" int lcg_batch_config(const lcg_batch *B, int32_t *waves, int32_t *workgroups, int32_t *lds_bytes) {
@@ -1070,6 +1501,7 @@ void pf_tables(const lcg_batch &B, int k, const double *dth, double *A) {
 
 extern "C" int lcg_batch_pf_run(lcg_batch *B, const double *dtheta, float *device_ms) {
     if (!B) return LCG_E_ARG;
+    if (B->device_built) return LCG_E_STATE;  // LocARNA-P reads the host arc-match lists
     if (int rc = pf_setup(*B)) return rc;
     for (int k = 0; k < B->npairs; ++k) pf_tables(*B, k, dtheta, B->pf_arena.data());
     HIPCHECK(hipSetDevice(B->device));

@@ -157,8 +157,6 @@ void TraceController::restrict_by_trace_probabilities(const std::vector<double> 
 
 // ----------------------------------------------------------- band (PFGotoh)
 
-namespace {
-
 int nt_code(char c) {
     switch (c) {
         case 'A': return 0;
@@ -168,6 +166,8 @@ int nt_code(char c) {
     }
     return -1;
 }
+
+namespace {
 
 // Gotoh partition function over cells (i, j) of the band, in the given
 // direction.  Forward: recursion from (0,0) with similarity sim(i, j).
@@ -382,13 +382,7 @@ Scoring::Scoring(const RnaData &A, const RnaData &B, const ArcMatches &am, const
             if (p.use_ribosum) {
                 int al = nt_code(A.nt(a.left)), ar = nt_code(A.nt(a.right));
                 int bl = nt_code(B.nt(b.left)), br = nt_code(B.nt(b.right));
-                if (al >= 0 && ar >= 0 && bl >= 0 && br >= 0) {
-                    double pam = ribosum85_60::arcmatch[(al * 4 + ar) * 16 + (bl * 4 + br)];
-                    double pa = ribosum85_60::basepair[al * 4 + ar], pb = ribosum85_60::basepair[bl * 4 + br];
-                    double sc = 0;
-                    sc += std::log(pam / (pa * pb)) / std::log(2);
-                    seq = round2score(100.0 * sc / 1);
-                }
+                if (al >= 0 && ar >= 0 && bl >= 0 && br >= 0) seq = ribosum_arc_term(al, ar, bl, br);
             } else {
                 seq = basematch(a.left, b.left) + basematch(a.right, b.right);
             }
@@ -396,6 +390,14 @@ Scoring::Scoring(const RnaData &A, const RnaData &B, const ArcMatches &am, const
         am_seq_[k] = seq;
         am_score_[k] = ((score_t)p.tau * seq) / 100 + (wA_[x.arcA] + wB_[x.arcB]);
     }
+}
+
+score_t Scoring::ribosum_arc_term(int al, int ar, int bl, int br) {
+    double pam = ribosum85_60::arcmatch[(al * 4 + ar) * 16 + (bl * 4 + br)];
+    double pa = ribosum85_60::basepair[al * 4 + ar], pb = ribosum85_60::basepair[bl * 4 + br];
+    double sc = 0;
+    sc += std::log(pam / (pa * pb)) / std::log(2);
+    return round2score(100.0 * sc / 1);
 }
 
 score_t Scoring::basematch(int i, int j) const { return sigma_codes_[codeA_[i] * NCODES + codeB_[j]]; }

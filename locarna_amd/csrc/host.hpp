@@ -102,6 +102,9 @@ struct ScoringParams {
     double exp_prob = -1.0;  // <= 0: 1/(2 len)  (aux.hh:216-220)
 };
 
+// RIBOSUM nucleotide code: 0..3 for A, C, G, U; -1 for any other symbol
+int nt_code(char c);
+
 // Trace-probability band: Gotoh partition function over the StralScore
 // similarity, forward and backward, then the probability cut and the
 // monotone fix-ups.  reference: edge_probs.icc:7-162, :212-268;
@@ -121,6 +124,9 @@ class ArcMatches {
 public:
     ArcMatches(const BasePairs &A, const BasePairs &B, const TraceController &tc, int max_diff_am = -1,
                int max_diff_at_am = -1);
+    // no arc matches enumerated on the host: the device compaction builds
+    // the lists (the object still carries the two base-pair sets)
+    ArcMatches(const BasePairs &A, const BasePairs &B) : A_(A), B_(B) {}
     int num_arc_matches() const { return (int)ams_.size(); }
     const ArcMatch &arcmatch(int k) const { return ams_[k]; }
     const BasePairs &bpsA() const { return A_; }
@@ -151,6 +157,9 @@ public:
     score_t sigma_code(int a, int b) const { return sigma_codes_[a * NCODES + b]; }
     // ribosum arc-match term riboX (scoring.cc:368-438) before the tau factor
     score_t arc_sequence_term(int am) const { return am_seq_[am]; }
+    // riboX term of one arc match from the nucleotide codes (0..3 = ACGU) of
+    // its four ends, RIBOSUM85_60 (scoring.cc:368-438, single sequences)
+    static score_t ribosum_arc_term(int al, int ar, int bl, int br);
 
 private:
     ScoringParams p_;
