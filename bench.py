@@ -19,6 +19,7 @@ host cores over the same pairs (scores compared bit for bit).
 """
 import argparse
 import concurrent.futures as cf
+import hashlib
 import json
 import os
 import sys
@@ -58,25 +59,46 @@ def main():
     params = api.locarna_params(max_bp_span=span, sequ_local=1 if args.sequ_local else 0)
     nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
     # end to end: pp texts in host memory -> prepared pairs (host threads) ->
-    # device arena -> one launch -> scores and alignments back on the host
-    torch.cuda.synchronize()
-    e0 = time.perf_counter()
-    with cf.ThreadPoolExecutor(nthreads) as ex:
-        prepared = list(ex.map(lambda ab: api.Pair(pps[ab[0]], pps[ab[1]], params), pairs))
-    e1 = time.perf_counter()
+    # device arena -> one launch -> scores and alignments back on the host.
+    # Twice: arc matches compacted and packed on the device (the default
+    # path), and enumerated and packed on the host.
+    def end_to_end(device_compaction):
+        torch.cuda.synchronize()
+        e0 = time.perf_counter()
+        with cf.ThreadPoolExecutor(nthreads) as ex:
+            prep = list(ex.map(lambda ab: api.Pair(pps[ab[0]], pps[ab[1]], params,
+                                                   device_compaction=device_compaction), pairs))
+        e1 = time.perf_counter()
+        b = api.Batch(prep, device=torch.cuda.current_device())
+        e2 = time.perf_counter()
+        # digest of every pair's device image before any run (untimed)
+        image = [hashlib.blake2b(b.segment(k).tobytes(), digest_size=16).digest() for k in range(len(prep))]
+        e2x = time.perf_counter()
+        e0 += e2x - e2
+        e1 += e2x - e2
+        e2 = e2x
+        b.run(mode=0, trace=True)
+        e3 = time.perf_counter()
+        for k in range(len(prep)):
+            b.score(k)
+            b.alignment(k)
+        e4 = time.perf_counter()
+        rec = {"pairs_per_sec": round(len(prep) / (e4 - e0), 2), "prepare_s": round(e1 - e0, 3),
+               "upload_s": round(e2 - e1, 3), "run_s": round(e3 - e2, 3), "fetch_s": round(e4 - e3, 3)}
+        return prep, b, rec, image
+
+    prepared_host, batch_host, e2e_host, image_host = end_to_end(False)
+    batch_host.destroy()
+    prepared, batch, e2e_dev, image_dev = end_to_end(True)
     npairs = len(prepared)
-    batch = api.Batch(prepared, device=torch.cuda.current_device())
-    e2 = time.perf_counter()
-    batch.run(mode=0, trace=True)
-    e3 = time.perf_counter()
-    for k in range(npairs):
-        batch.score(k)
-        batch.alignment(k)
-    e4 = time.perf_counter()
-    end_to_end = {"pairs_per_sec": round(npairs / (e4 - e0), 2), "prepare_s": round(e1 - e0, 3),
-                  "upload_s": round(e2 - e1, 3), "run_s": round(e3 - e2, 3), "fetch_s": round(e4 - e3, 3),
-                  "host_threads": nthreads,
-                  "note": "first launch includes one-time kernel loading"}
+    arcm = sum(batch.pair_info(k)[0] for k in range(npairs))
+    groups = sum(batch.pair_info(k)[1] for k in range(npairs))
+    same_image = image_host == image_dev
+    del image_host, image_dev
+    end_to_end = {"device_compaction": e2e_dev, "host_compaction": e2e_host,
+                  "host_threads": nthreads, "images_identical": same_image,
+                  "note": "first launches include one-time kernel loading; upload_s of the device "
+                          "compaction path includes its two compaction kernels"}
 
     # untimed counting run: DP cells and arc candidates each kernel evaluates
     batch.run(mode=0, trace=True, count=True)
@@ -88,7 +110,6 @@ def main():
         top_cells += tc
         top_cands += tq
     cells, cands = fill_cells + top_cells, fill_cands + top_cands
-    arcm = sum(p.num_arcmatches for p in prepared)
 
     for _ in range(args.warmup):
         batch.run(mode=args.mode, trace=True)
@@ -139,6 +160,36 @@ def main():
                                  "ms": round(sum(top_ms) / len(top_ms), 3) if top_ms else None,
                                  "cells": top_cells, "candidates": top_cands}}
 
+    # device arc-match compaction (count + build kernels) rerun on the same
+    # inputs; algorithmic bytes: the image structures it writes (per arc
+    # match 52 B: score 4, candidate 16, fill entry 12, group entry 16, fill
+    # -> candidate 4; per group 32 B; per band cell 4 B), the validity bitmap
+    # (written, read back: 16 B per 64-bit word) and the arc inputs
+    comp = None
+    if args.steps > 0:
+        cms = [batch.compact_run() for _ in range(max(3, min(args.steps, 10)))]
+        c_ms = sum(a for a, _ in cms) / len(cms)
+        b_ms = sum(b for _, b in cms) / len(cms)
+        words = sum(p.compaction_words for p in prepared)
+        band_cells = sum(p.band_cells for p in prepared)
+        arcs = sum(p.num_arcs for p in prepared)
+        alg = 52 * arcm + 32 * groups + 4 * band_cells + 16 * words + 20 * arcs
+        t_s = (c_ms + b_ms) / 1e3
+        prep_diff = e2e_host["prepare_s"] - e2e_dev["prepare_s"]
+        comp = {"kernels": ["lcg_compact_count", "lcg_compact_build"],
+                "count_ms": round(c_ms, 3), "build_ms": round(b_ms, 3),
+                "pairs_per_sec": round(npairs / t_s, 1), "arc_matches": arcm, "groups": groups,
+                "roofline": {"bound": "hbm", "achieved": round(alg / t_s / 1e9, 2), "peak": 8000.0,
+                             "unit": "GB/s", "frac": round(alg / t_s / 1e9 / 8000.0, 5),
+                             "algorithmic_bytes": alg},
+                "cpu_baseline": {"value": round(npairs / prep_diff, 1) if prep_diff > 0 else None,
+                                 "unit": "pairs/s", "cores": nthreads, "kind": "port",
+                                 "sample": "all %d pairs: host preparation with the host arc-match "
+                                           "enumeration and packing minus host preparation without "
+                                           "them, same threads" % npairs},
+                "images_identical_to_host": same_image}
+        batch.run(mode=args.mode, trace=True)
+
     cpu = None
     if args.cpu_baseline == "auto":
         from oracle import oracle
@@ -172,6 +223,7 @@ def main():
         "batched_vs_plain": round(value / plain, 2) if plain else None,
         "roofline": roofline,
         "end_to_end": end_to_end,
+        "arc_match_compaction": comp,
         "cpu_baseline": cpu,
     }
     if cpu:

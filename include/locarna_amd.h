@@ -66,6 +66,9 @@ void lcg_pair_free(lcg_pair *pair);
  * estimate of the DP cells of the pair */
 int lcg_pair_info(const lcg_pair *pair, int32_t *lenA, int32_t *lenB, int32_t *num_arcmatches, int32_t *num_groups,
                   int64_t *cells);
+/* arcs of the two sequences after the probability cut
+ * (BasePairs::num_bps, basepairs.hh) */
+int lcg_pair_num_arcs(const lcg_pair *pair, int32_t *num_arcsA, int32_t *num_arcsB);
 /* trace band: min_col[0..lenA], max_col[0..lenA]
  * (TraceController::min_col / max_col, trace_controller.hh) */
 int lcg_pair_band(const lcg_pair *pair, int32_t *min_col, int32_t *max_col, int32_t cap);

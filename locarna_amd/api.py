@@ -54,6 +54,7 @@ def _sig(L):
     L.lcg_pair_free.argtypes = [c.c_void_p]
     L.lcg_pair_info.argtypes = [c.c_void_p, c.POINTER(c.c_int32), c.POINTER(c.c_int32),
                                 c.POINTER(c.c_int32), c.POINTER(c.c_int32), c.POINTER(c.c_int64)]
+    L.lcg_pair_num_arcs.argtypes = [c.c_void_p, c.POINTER(c.c_int32), c.POINTER(c.c_int32)]
     L.lcg_pair_band.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_int32]
     L.lcg_pair_arcmatches.argtypes = [c.c_void_p] + [c.c_void_p] * 5 + [c.c_int32]
     L.lcg_pair_schedule.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_int32, c.c_void_p, c.c_void_p, c.c_int32]
@@ -118,6 +119,25 @@ class Pair:
                         ctypes.byref(cells))
         self.lenA, self.lenB, self.num_arcmatches, self.num_groups = a.value, b.value, na.value, ng.value
         self.cells = cells.value
+
+    @property
+    def num_arcs(self):
+        """arcs of A plus arcs of B (BasePairs::num_bps)"""
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        lib().lcg_pair_num_arcs(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value + b.value
+
+    @property
+    def compaction_words(self):
+        """64-bit words of the arc-pair validity bitmap: N_A * ceil(N_B / 64)"""
+        a, b = ctypes.c_int32(), ctypes.c_int32()
+        lib().lcg_pair_num_arcs(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value * ((b.value + 63) // 64)
+
+    @property
+    def band_cells(self):
+        mn, mx = self.band()
+        return sum(max(0, y - x + 1) for x, y in zip(mn, mx))
 
     def band(self):
         n = self.lenA + 1
@@ -185,10 +205,11 @@ class Batch:
         return nam.value, ng.value, sz.value
 
     def segment(self, k):
-        """Device image of pair k (int32 list)."""
+        """Device image of pair k (numpy int32 array)."""
+        import numpy as np
         n = self.pair_info(k)[2]
-        buf = (ctypes.c_int32 * max(n, 1))()
-        rc = lib().lcg_batch_segment(self.h, k, buf, n)
+        buf = np.zeros(max(n, 1), dtype=np.int32)
+        rc = lib().lcg_batch_segment(self.h, k, buf.ctypes.data_as(ctypes.c_void_p), n)
         if rc < 0:
             raise LcgError("lcg_batch_segment (%d)" % rc)
         return buf[:rc]

@@ -897,6 +897,22 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
     }
 }
 
+// Copies the outputs of every pair (edges, structure strings, traceback
+// stack, out words: one contiguous range of its image) into a packed
+// buffer, so that the host reads the results of a run with one copy.
+extern "C" __global__ void lcg_gather_outputs(const PairDesc *descs, const int *res_off, const int *arena, int *dst) {
+    const PairDesc d = descs[blockIdx.x];
+    const int len = d.out + 12 - d.edges;
+    int *o = dst + res_off[blockIdx.x];
+    for (int t = threadIdx.x; t < len; t += blockDim.x) o[t] = arena[d.edges + t];
+}
+
+extern "C" int lcg_launch_gather(const PairDesc *descs, const int *res_off, const int *arena, int *dst, int npairs,
+                                 hipStream_t stream) {
+    hipLaunchKernelGGL(lcg_gather_outputs, dim3(npairs), dim3(256), 0, stream, descs, res_off, arena, dst);
+    return (int)hipGetLastError();
+}
+
 extern "C" __global__ void lcg_reset_queue(int *queue) {
     if (threadIdx.x == 0) *queue = 0;
 }

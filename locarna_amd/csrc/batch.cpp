@@ -19,6 +19,8 @@
 
 using namespace locarna_amd;
 
+extern "C" int lcg_launch_gather(const PairDesc *descs, const int *res_off, const int *arena, int *dst, int npairs,
+                                 hipStream_t stream);
 extern "C" int lcg_launch_compact_count(const CompactDesc *cds, const int *cin, int *ws, int *stats, int npairs,
                                         int lds_bytes, hipStream_t stream);
 extern "C" int lcg_launch_compact_build(const CompactDesc *cds, const PairDesc *descs, const int *cin, int *arena,
@@ -343,6 +345,28 @@ void pack(lcg_pair &P) {
     P.ecap = ecap;
 }
 
+// Head of the arena image of a pair (offsets of d relative to the image):
+// band (min_col, max_col, band cell offsets), codes, gap scores, sigma.
+void write_header(const lcg_pair &P, const PairDesc &d, int32_t *img) {
+    const TraceController &tc = *P.tc;
+    const Scoring &sc = *P.sc;
+    const int n = d.n, m = d.m;
+    int ncells = 0;
+    for (int i = 0; i <= n; ++i) {
+        img[d.band + i] = tc.min_col(i);
+        img[d.band + n + 1 + i] = tc.max_col(i);
+        img[d.band + 2 * (n + 1) + i] = ncells;
+        ncells += std::max(0, tc.max_col(i) - tc.min_col(i) + 1);
+    }
+    img[d.band + 3 * (n + 1)] = ncells;
+    for (int i = 1; i <= n; ++i) img[d.codes + i] = sc.codeA(i);
+    for (int j = 1; j <= m; ++j) img[d.codes + n + 1 + j] = sc.codeB(j);
+    for (int i = 1; i <= n; ++i) img[d.gaps + i] = (int32_t)sc.gapA(i);
+    for (int j = 1; j <= m; ++j) img[d.gaps + n + 1 + j] = (int32_t)sc.gapB(j);
+    for (int a = 0; a < LCG_NCODES; ++a)
+        for (int b = 0; b < LCG_NCODES; ++b) img[d.sigma + a * LCG_NCODES + b] = (int32_t)sc.sigma_code(a, b);
+}
+
 // Inputs of the device compaction of one pair: arcs in index order with
 // their weights and base-pair types, the arcs of each left end (a range of
 // the index order) and of each right end (right adjacency order), the band
@@ -402,6 +426,16 @@ void prepare_compaction_inputs(lcg_pair &P) {
         img[c.band + i] = tc.min_col(i);
         img[c.band + n + 1 + i] = tc.max_col(i);
     }
+    {
+        PairDesc L;
+        std::memset(&L, 0, sizeof L);
+        layout_pair(L, n, m, 0, 0, 0, 1);
+        c.hdr_len = L.rows;  // band, codes, gaps, sigma are the head of the image
+        c.hdr = reserve(img, c.hdr_len);
+        L.n = n;
+        L.m = m;
+        write_header(P, L, img.data() + c.hdr);
+    }
     c.tab = reserve(img, 256);
     c.tau = P.params.tau;
     c.ribosum = P.params.use_ribosum != 0;
@@ -412,30 +446,6 @@ void prepare_compaction_inputs(lcg_pair &P) {
     c.dlen = c.dend = std::max(n, m);
     while (img.size() & 3) img.push_back(0);
     P.cdesc = c;
-}
-
-// Host-computed parts of the arena image of a device-compacted pair (band,
-// codes, gap scores, sigma; edges cleared to -1).  The device writes the
-// arc-match structures.
-void write_host_parts(const lcg_pair &P, const PairDesc &d, int32_t *img) {
-    const TraceController &tc = *P.tc;
-    const Scoring &sc = *P.sc;
-    const int n = d.n, m = d.m;
-    int ncells = 0;
-    for (int i = 0; i <= n; ++i) {
-        img[d.band + i] = tc.min_col(i);
-        img[d.band + n + 1 + i] = tc.max_col(i);
-        img[d.band + 2 * (n + 1) + i] = ncells;
-        ncells += std::max(0, tc.max_col(i) - tc.min_col(i) + 1);
-    }
-    img[d.band + 3 * (n + 1)] = ncells;
-    for (int i = 1; i <= n; ++i) img[d.codes + i] = sc.codeA(i);
-    for (int j = 1; j <= m; ++j) img[d.codes + n + 1 + j] = sc.codeB(j);
-    for (int i = 1; i <= n; ++i) img[d.gaps + i] = (int32_t)sc.gapA(i);
-    for (int j = 1; j <= m; ++j) img[d.gaps + n + 1 + j] = (int32_t)sc.gapB(j);
-    for (int a = 0; a < LCG_NCODES; ++a)
-        for (int b = 0; b < LCG_NCODES; ++b) img[d.sigma + a * LCG_NCODES + b] = (int32_t)sc.sigma_code(a, b);
-    for (int t = 0; t < n + m + 1; ++t) img[d.edges + t] = -1;
 }
 
 }  // namespace
@@ -515,6 +525,13 @@ extern "C" int lcg_pair_info(const lcg_pair *p, int32_t *lenA, int32_t *lenB, in
     return LCG_OK;
 }
 
+extern "C" int lcg_pair_num_arcs(const lcg_pair *p, int32_t *num_arcsA, int32_t *num_arcsB) {
+    if (!p) return LCG_E_ARG;
+    if (num_arcsA) *num_arcsA = p->bA->num_bps();
+    if (num_arcsB) *num_arcsB = p->bB->num_bps();
+    return LCG_OK;
+}
+
 extern "C" int lcg_pair_band(const lcg_pair *p, int32_t *mn, int32_t *mx, int32_t cap) {
     if (!p || cap < p->desc.n + 1) return LCG_E_ARG;
     for (int i = 0; i <= p->desc.n; ++i) {
@@ -567,7 +584,8 @@ struct lcg_batch {
     int device = 0;
     int npairs = 0;
     std::vector<PairDesc> descs;      // absolute offsets
-    std::vector<int32_t> host_arena;  // image uploaded at creation
+    std::vector<int32_t> host_arena;  // image uploaded at creation (host packing)
+    size_t arena_ints = 0;
     std::vector<int32_t> order;       // work queue order (longest first)
     int32_t *d_arena = nullptr;
     PairDesc *d_descs = nullptr;
@@ -591,7 +609,11 @@ struct lcg_batch {
     bool have_results = false;
     bool fetched = false;
     bool traced = false;
-    std::vector<int32_t> result;  // arena copied back after a run
+    // outputs of the last run: per pair the range [edges, out + 12) of its
+    // image, packed at res_off[k] (gathered on the device, one copy back)
+    std::vector<int32_t> result;
+    std::vector<int32_t> res_off;
+    int32_t *d_res = nullptr, *d_resoff = nullptr;
     // LocARNA-P
     std::vector<PfHost> pfh;
     std::vector<PfDesc> pdescs;
@@ -789,7 +811,7 @@ int device_count_and_layout(lcg_batch &B, lcg_pair *const *pairs, int npairs, st
         cin.insert(cin.end(), P->cimg.begin(), P->cimg.end());
         CompactDesc c = P->cdesc;
         int32_t *offs[] = {&c.arcsA, &c.arcsB, &c.wA, &c.wB, &c.ptA, &c.ptB, &c.lA, &c.lB,
-                           &c.rpA, &c.rpB, &c.radjA, &c.radjB, &c.band, &c.tab};
+                           &c.rpA, &c.rpB, &c.radjA, &c.radjB, &c.band, &c.tab, &c.hdr};
         for (int32_t *o : offs) *o += base;
         const size_t words = (size_t)c.NA * c.W64;
         ws_ints = (ws_ints + 1) & ~(size_t)1;
@@ -867,11 +889,10 @@ int device_count_and_layout(lcg_batch &B, lcg_pair *const *pairs, int npairs, st
         set_err(err, errlen, "batch too large for one arena");
         return LCG_E_CAPACITY;
     }
-    B.host_arena.assign(total, 0);
+    B.arena_ints = total;
     size_t base = 0;
     for (int k = 0; k < npairs; ++k) {
         PairDesc &d = B.descs[k];
-        write_host_parts(*pairs[k], d, B.host_arena.data() + base);
         int32_t *offs[] = {&d.band, &d.codes, &d.gaps, &d.sigma, &d.rows, &d.groups, &d.gams, &d.amscore,
                            &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.out, &d.rounds,
                            &d.fptr, &d.fblk, &d.fcand, &d.f2c};
@@ -976,6 +997,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
             return LCG_E_CAPACITY;
         }
         B->host_arena.reserve(total);
+        B->arena_ints = total;
         for (int k = 0; k < npairs; ++k) {
             const lcg_pair *P = pairs[k];
             const int32_t base = (int32_t)B->host_arena.size();
@@ -1077,19 +1099,32 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
 
     B->grid = K > 1 ? npairs * K : std::min(npairs, cus * wgs_per_cu);
     B->top_grid = std::min(npairs, cus * top_wgs);
-    if ((e = hipMalloc(&B->d_arena, B->host_arena.size() * 4)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMalloc(&B->d_arena, B->arena_ints * 4)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_descs, sizeof(PairDesc) * npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_order, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_queue, 256)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_barriers, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     const size_t scratch_ints = (size_t)B->top_grid * B->top_waves * top_max;
     if ((e = hipMalloc(&B->d_scratch, scratch_ints * 4)) != hipSuccess) return hip_fail(e, err, errlen);
-    if ((e = hipMemcpy(B->d_arena, B->host_arena.data(), B->host_arena.size() * 4, hipMemcpyHostToDevice)) !=
-        hipSuccess)
-        return hip_fail(e, err, errlen);
+    if (deferred)  // the compaction writes the image over a cleared arena
+        e = hipMemsetAsync(B->d_arena, 0, B->arena_ints * 4, B->stream);
+    else
+        e = hipMemcpy(B->d_arena, B->host_arena.data(), B->arena_ints * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMemcpy(B->d_descs, B->descs.data(), sizeof(PairDesc) * npairs, hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, err, errlen);
     if ((e = hipMemcpy(B->d_order, B->order.data(), 4 * (size_t)npairs, hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, err, errlen);
+    B->res_off.resize(npairs);
+    size_t res_ints = 0;
+    for (int k = 0; k < npairs; ++k) {
+        B->res_off[k] = (int32_t)res_ints;
+        res_ints += (size_t)(B->descs[k].out + 12 - B->descs[k].edges);
+    }
+    B->result.resize(res_ints);
+    if ((e = hipMalloc(&B->d_res, res_ints * 4)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMalloc(&B->d_resoff, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMemcpy(B->d_resoff, B->res_off.data(), 4 * (size_t)npairs, hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, err, errlen);
     if (deferred) {
         if (int rc = device_build(*B, npairs, max_len, waves, pool, K, err, errlen)) return rc;
@@ -1118,6 +1153,8 @@ extern "C" void lcg_batch_destroy(lcg_batch *B) {
     if (B->ev0) hipEventDestroy(B->ev0);
     if (B->ev1) hipEventDestroy(B->ev1);
     if (B->ev_mid) hipEventDestroy(B->ev_mid);
+    if (B->d_res) hipFree(B->d_res);
+    if (B->d_resoff) hipFree(B->d_resoff);
     if (B->d_cin) hipFree(B->d_cin);
     if (B->d_ws) hipFree(B->d_ws);
     if (B->d_ws2) hipFree(B->d_ws2);
@@ -1134,7 +1171,7 @@ extern "C" int lcg_batch_pair_info(const lcg_batch *B, int32_t k, int32_t *nam, 
     if (nam) *nam = d.nam;
     if (ngroups) *ngroups = d.ngroups;
     if (arena_ints) {
-        const size_t end = k + 1 < B->npairs ? (size_t)B->descs[k + 1].band : B->host_arena.size();
+        const size_t end = k + 1 < B->npairs ? (size_t)B->descs[k + 1].band : B->arena_ints;
         *arena_ints = (int64_t)(end - (size_t)d.band);
     }
     return LCG_OK;
@@ -1143,7 +1180,7 @@ extern "C" int lcg_batch_pair_info(const lcg_batch *B, int32_t k, int32_t *nam, 
 extern "C" int lcg_batch_segment(const lcg_batch *B, int32_t k, int32_t *dst, int64_t cap) {
     if (!B || !dst || k < 0 || k >= B->npairs) return LCG_E_ARG;
     const size_t begin = (size_t)B->descs[k].band;
-    const size_t end = k + 1 < B->npairs ? (size_t)B->descs[k + 1].band : B->host_arena.size();
+    const size_t end = k + 1 < B->npairs ? (size_t)B->descs[k + 1].band : B->arena_ints;
     if ((size_t)cap < end - begin) return LCG_E_ARG;
     HIPCHECK(hipSetDevice(B->device));
     HIPCHECK(hipStreamSynchronize(B->stream));
@@ -1247,26 +1284,33 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
 }
 
 namespace {
-// Copies the arena back once per run, on the first result request.
+// Copies the outputs of a run back once, on the first result request.
 int fetch(const lcg_batch *Bc) {
     lcg_batch *B = const_cast<lcg_batch *>(Bc);
     if (!B->have_results) return LCG_E_STATE;
     if (B->fetched) return LCG_OK;
     HIPCHECK(hipSetDevice(B->device));
-    B->result.resize(B->host_arena.size());
-    HIPCHECK(hipMemcpy(B->result.data(), B->d_arena, B->result.size() * 4, hipMemcpyDeviceToHost));
+    if (int rc = lcg_launch_gather(B->d_descs, B->d_resoff, B->d_arena, B->d_res, B->npairs, B->stream))
+        return hip_fail((hipError_t)rc);
+    HIPCHECK(hipMemcpyAsync(B->result.data(), B->d_res, B->result.size() * 4, hipMemcpyDeviceToHost, B->stream));
+    HIPCHECK(hipStreamSynchronize(B->stream));
     B->fetched = true;
     // a bounded wait of the multi-workgroup fill that expired invalidates the run
-    for (const PairDesc &d : B->descs)
-        if (B->result[d.out + 3] != 0) return LCG_E_STATE;
+    for (int k = 0; k < B->npairs; ++k)
+        if (B->result[B->res_off[k] + (B->descs[k].out + 3 - B->descs[k].edges)] != 0) return LCG_E_STATE;
     return LCG_OK;
+}
+
+// output word at arena offset `off` (in [edges, out + 12)) of pair k
+inline int32_t res(const lcg_batch *B, int k, int32_t off) {
+    return B->result[B->res_off[k] + (off - B->descs[k].edges)];
 }
 }  // namespace
 
 extern "C" int lcg_batch_score(const lcg_batch *B, int32_t k, int64_t *score) {
     if (!B || k < 0 || k >= B->npairs || !score) return LCG_E_ARG;
     if (int rc = fetch(B)) return rc;
-    *score = B->result[B->descs[k].out];
+    *score = res(B, k, B->descs[k].out);
     return LCG_OK;
 }
 
@@ -1278,7 +1322,7 @@ extern "C" int lcg_batch_alignment(const lcg_batch *B, int32_t k, int32_t *a, in
     const PairDesc &d = B->descs[k];
     int cnt = 0;
     for (int s = 0; s <= d.n + d.m; ++s) {
-        const int32_t v = B->result[d.edges + s];
+        const int32_t v = res(B, k, d.edges + s);
         if (v == -1) continue;
         if (cnt >= cap) return LCG_E_ARG;
         const int x = v >> 16, y = v & 0xffff;
@@ -1289,8 +1333,8 @@ extern "C" int lcg_batch_alignment(const lcg_batch *B, int32_t k, int32_t *a, in
     if (structA && structB) {
         if (scap <= d.n || scap <= d.m) return LCG_E_ARG;
         static const char sym[3] = {'.', '(', ')'};
-        for (int i = 0; i < d.n; ++i) structA[i] = sym[B->result[d.structs + i]];
-        for (int j = 0; j < d.m; ++j) structB[j] = sym[B->result[d.structs + d.n + j]];
+        for (int i = 0; i < d.n; ++i) structA[i] = sym[res(B, k, d.structs + i)];
+        for (int j = 0; j < d.m; ++j) structB[j] = sym[res(B, k, d.structs + d.n + j)];
         structA[d.n] = 0;
         structB[d.m] = 0;
     }
@@ -1302,9 +1346,9 @@ extern "C" int lcg_batch_D(const lcg_batch *B, int32_t k, int64_t *D, int32_t ca
     if (int rc = fetch(B)) return rc;
     const PairDesc &d = B->descs[k];
     if (cap < d.nam) return LCG_E_ARG;
-    for (int t = 0; t < d.nam; ++t) {
-        D[t] = B->result[d.D + t];
-    }
+    std::vector<int32_t> v(std::max(d.nam, 1));
+    HIPCHECK(hipMemcpy(v.data(), B->d_arena + d.D, 4 * (size_t)d.nam, hipMemcpyDeviceToHost));
+    for (int t = 0; t < d.nam; ++t) D[t] = v[t];
     return d.nam;
 }
 
@@ -1312,14 +1356,13 @@ extern "C" int lcg_batch_counters(const lcg_batch *B, int32_t k, int64_t *cells,
     if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
     if (int rc = fetch(B)) return rc;
     const PairDesc &d = B->descs[k];
-    uint64_t c0, c1;
-    uint64_t c2, c3;
-    std::memcpy(&c0, &B->result[d.out + 4], 8);
-    std::memcpy(&c1, &B->result[d.out + 6], 8);
-    std::memcpy(&c2, &B->result[d.out + 8], 8);
-    std::memcpy(&c3, &B->result[d.out + 10], 8);
-    if (cells) *cells = (int64_t)(c0 + c2);
-    if (candidates) *candidates = (int64_t)(c1 + c3);
+    uint64_t c[4];
+    for (int t = 0; t < 4; ++t) {
+        const uint32_t lo = (uint32_t)res(B, k, d.out + 4 + 2 * t), hi = (uint32_t)res(B, k, d.out + 5 + 2 * t);
+        c[t] = ((uint64_t)hi << 32) | lo;
+    }
+    if (cells) *cells = (int64_t)(c[0] + c[2]);
+    if (candidates) *candidates = (int64_t)(c[1] + c[3]);
     return LCG_OK;
 }
 
@@ -1329,7 +1372,10 @@ extern "C" int lcg_batch_counters_split(const lcg_batch *B, int32_t k, int64_t *
     if (int rc = fetch(B)) return rc;
     const PairDesc &d = B->descs[k];
     uint64_t c[4];
-    for (int t = 0; t < 4; ++t) std::memcpy(&c[t], &B->result[d.out + 4 + 2 * t], 8);
+    for (int t = 0; t < 4; ++t) {
+        const uint32_t lo = (uint32_t)res(B, k, d.out + 4 + 2 * t), hi = (uint32_t)res(B, k, d.out + 5 + 2 * t);
+        c[t] = ((uint64_t)hi << 32) | lo;
+    }
     if (fill_cells) *fill_cells = (int64_t)c[0];
     if (fill_candidates) *fill_candidates = (int64_t)c[1];
     if (top_cells) *top_cells = (int64_t)c[2];

@@ -319,6 +319,11 @@ __global__ __launch_bounds__(CT) void lcg_compact_build(const CompactDesc *__res
     int *S = stats + (size_t)k * LCG_CS_INTS;
     const int nam = S[LCG_CS_NAM], ngroups = S[LCG_CS_NGROUPS], nblk = S[LCG_CS_NBLK];
     if (threadIdx.x == 0) s_status = 0;
+    // head of the image from the host (band, codes, gaps, sigma); the rest
+    // of the arena is zero, edges start empty (-1)
+    for (int t = threadIdx.x; t < cd.hdr_len; t += CT) arena[d.band + t] = cin[cd.hdr + t];
+    for (int t = threadIdx.x; t <= d.n + d.m; t += CT) arena[d.edges + t] = -1;
+    __syncthreads();
     load_band(lds, arena + d.band, arena + d.band + d.n + 1, d.n, tmp);
     P.mn = lds;
     P.mx = lds + (P.n + 1);

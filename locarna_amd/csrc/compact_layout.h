@@ -41,7 +41,9 @@ typedef struct {
     int32_t ws_eof;        // nam: candidate entry of arc match k
     int32_t ws_fof;        // nam: fill entry of arc match k
     int32_t ws_sched;      // 3 * ngroups + K * (n + 2): schedule scratch
-    int32_t pad[4];
+    int32_t hdr, hdr_len;  // head of the pair's image (band, codes, gap
+                           // scores, sigma), copied to the arena by pass 2
+    int32_t pad[2];
 } CompactDesc;  // 36 ints
 
 // Per-pair results of the counting pass (and status of the build pass).

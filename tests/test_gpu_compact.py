@@ -6,6 +6,7 @@ host packing of the same pairs int for int, and both batches align to the
 same scores, alignments and D tables.  The host packing itself is checked
 against the oracle in test_gpu_align.py; the tiny pairs are also checked
 against the oracle here.  Calls the kernels through the C ABI."""
+import numpy as np
 import pytest
 
 from locarna_amd import api, synth
@@ -19,8 +20,9 @@ def _same_images(hb, db, label):
     for k in range(len(hb.pairs)):
         assert hb.pair_info(k) == db.pair_info(k), (label, k)
         h, d = hb.segment(k), db.segment(k)
-        if h != d:
-            bad = [t for t in range(len(h)) if h[t] != d[t]]
+        assert len(h) == len(d), (label, k)
+        bad = np.nonzero(h != d)[0]
+        if len(bad):
             raise AssertionError("%s pair %d: %d ints differ, first at %d (%d != %d)"
                                  % (label, k, len(bad), bad[0], h[bad[0]], d[bad[0]]))
 
