@@ -357,7 +357,9 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
         int t0, t1, u0 = 0, u1 = 0;
         row_cands(r, t0, t1);
         if (r + 1 < nrows) row_cands(r + 1, u0, u1);
-        const FillEntry nx0 = fetch(u0 + lane, u1), nx1 = fetch(u0 + 64 + lane, u1);
+        // the next row's first chunks (the second only if it has more than 64)
+        const FillEntry nx0 = fetch(u0 + lane, u1);
+        const FillEntry nx1 = (u1 - u0 > 64) ? fetch(u0 + 64 + lane, u1) : none;
         // arc terms, candidate-parallel over the row's prefix p > al: the
         // M gathers of all chunks are issued before the LDS atomics (which
         // the compiler may not reorder them across)
@@ -374,8 +376,12 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
         };
         {
             int v[4], slot[4];
-            // rows of at most 128 candidates (most) skip chunks 2 and 3
-            if (t1 - t0 <= 128) {
+            // rows of at most 64 candidates (most) use chunk 0 only, rows of
+            // at most 128 skip chunks 2 and 3
+            if (t1 - t0 <= 64) {
+                slot[0] = arc_slot(cur[0], t0 + lane < t1, v[0]);
+                if (slot[0] >= 0) atomicMax(&Srow[slot[0]], v[0]);
+            } else if (t1 - t0 <= 128) {
 #pragma unroll
                 for (int k = 0; k < 2; ++k) slot[k] = arc_slot(cur[k], t0 + 64 * k + lane < t1, v[k]);
 #pragma unroll
@@ -394,8 +400,10 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
             const int sl = arc_slot(fetch(t, t1), true, v);
             if (sl >= 0) atomicMax(&Srow[sl], v);
         }
-        cur[2] = fetch(u0 + 128 + lane, u1);
-        cur[3] = fetch(u0 + 192 + lane, u1);
+        if (u1 - u0 > 128) {
+            cur[2] = fetch(u0 + 128 + lane, u1);
+            cur[3] = fetch(u0 + 192 + lane, u1);
+        }
         region_sync<GLOBAL>();
         int Fcarry = LCG_NEG;
         int sprev = Mcol;      // s(i, bl) entering the F chain: M(i, bl)
