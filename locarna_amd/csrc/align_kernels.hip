@@ -305,7 +305,13 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
             }
     };
     const FillEntry none{0, 0, 0};
-    auto fetch = [&](int t, int t1) { return t < t1 ? *reinterpret_cast<const FillEntry *>(P.fcand + 3 * t) : none; };
+    // one unpredicated 12-byte load per lane: lanes past the row's range
+    // read its last entry (or entry 0 of the pair; a group implies arc
+    // matches), and arc_slot ignores them through its lane flag
+    auto fetch = [&](int t, int t1) {
+        const int tt = t < t1 ? t : max(t1 - 1, 0);
+        return *reinterpret_cast<const FillEntry *>(P.fcand + 3 * tt);
+    };
     for (int c = lane; c < br - bl; c += 64) Srow[c] = LCG_NEG;
     // row al (init row): M(al, bl) = 0, M(al, j) = alpha + sum gapB
     int Mp[NC], Ec[NC], gBc[NC], cBc[NC];
