@@ -314,7 +314,10 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
     };
     for (int c = lane; c < br - bl; c += 64) Srow[c] = LCG_NEG;
     // row al (init row): M(al, bl) = 0, M(al, j) = alpha + sum gapB
-    int Mp[NC], Ec[NC], gBc[NC], cBc[NC];
+    // Pre: prefix over the chunk's columns of gB(j) + max(0, alpha), the
+    // slope of the F chain (fixed per column; a row's active columns are one
+    // range, whose prefix is Pre minus Pre before the range)
+    int Mp[NC], Ec[NC], gBc[NC], cBc[NC], Pre[NC];
     const int hi0 = row_hi(P, al, br);
     {
         int carry = alpha;
@@ -330,6 +333,7 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
             Ec[c] = LCG_NEG;
             gBc[c] = col ? P.gapB[j] : 0;
             cBc[c] = col ? P.codeB[j] : 0;
+            Pre[c] = wave_scan_add(col ? gBc[c] + P.alpha_pos : 0);
         }
         if (lane == 0) Mst[0] = 0;
     }
@@ -376,7 +380,7 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
             const bool ok = in && q > bl && j >= jlo && j <= hi && q - 1 >= rl;
             // branch-free gather (cell 0 of the region when unused), so that
             // the gathers of several chunks are in flight together
-            v = Mst[ok ? (p - 1 - al) * W + (q - 1 - rl) : 0] + e.D;
+            v = Mst[ok ? __mul24(p - 1 - al, W) + (q - 1 - rl) : 0] + e.D;
             cnt_cand += ok;
             return ok ? j - bl - 1 : -1;
         };
@@ -435,11 +439,13 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
                 ++cnt_cells;
             }
             const int sl = wave_shift_up(s, sprev);
-            const int a = act ? gBc[c] + P.alpha_pos : 0;
             const int cc = act ? sl + gBc[c] + alpha : LCG_NEG;
-            const int Pa = wave_scan_add(a);
-            const int px = wave_scan_max(act ? cc - Pa : LCG_NEG);
-            const int F = Pa + max(Fcarry, px);
+            // F(j) = Pa(j) + max(Fcarry, max_k (cc(k) - Pa(k))) with Pa the
+            // prefix from the first active column js; Pa = Pre - Pre(js - 1)
+            const int x = max(jlo, bl + 1 + 64 * c) - (bl + 1 + 64 * c) - 1;  // lane before js
+            const int C = x < 0 ? 0 : readlane(Pre[c], min(x, 63));
+            const int px = wave_scan_max(act ? cc - Pre[c] : LCG_NEG);
+            const int F = Pre[c] + max(Fcarry - C, px);
             const int Mv = clampneg(max(s, F));
             if (act) row[j - lo] = Mv;
             Mp[c] = act ? Mv : LCG_NEG;
