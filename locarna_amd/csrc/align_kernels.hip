@@ -75,12 +75,13 @@ __device__ __forceinline__ int readlane(int v, int l) { return __builtin_amdgcn_
 
 // Orders the memory accesses of one wave to a region it owns: LDS
 // instructions of a wave complete in order, so only the compiler needs a
-// barrier; global scratch additionally needs the L1 written back and
-// invalidated (agent-scope fence).
+// barrier; a region in global scratch belongs to one wave of the workgroup,
+// whose vector L1 (write-through, one per CU) serves all its accesses, so a
+// workgroup-scope fence (wait for the outstanding accesses) suffices.
 template <bool GLOBAL>
 __device__ __forceinline__ void region_sync() {
     if (GLOBAL)
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     else
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
 }
@@ -836,7 +837,7 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
         if (bp.count && threadIdx.x < 4) arena[d.out + 8 + threadIdx.x] = 0;
         const Pair P = load_pair(d, arena, lds, bp);
         unsigned long long *counts = bp.count ? (unsigned long long *)(arena + d.out + 8) : nullptr;  // top level
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // the pair is this workgroup's alone
         __syncthreads();
         // ---- top level (wave 0)
         const bool top_in_lds = d.top_ints <= bp.pool_ints;

@@ -109,7 +109,12 @@ struct Region {
     }
 };
 
-__device__ __forceinline__ void scratch_sync() { __threadfence(); }
+// Orders a workgroup's accesses to its global scratch and to the pair's
+// arrays.  One workgroup owns a pair, its waves run on one CU and share that
+// CU's vector L1 (write-through), so a workgroup-scope fence (wait for the
+// outstanding accesses) suffices; an agent-scope fence would write back and
+// invalidate the L2 on every call.
+__device__ __forceinline__ void scratch_sync() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
 
 // Inside fill of a region (aligner_p.icc:150-312).  Stores M, E, F.
 __device__ void pf_fill(const PfPair &P, const Region &R) {
@@ -711,7 +716,7 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
         P.fcand = arena + d.fcand;
         P.f2c = arena + d.f2c;
         P.fD = pfa + f.fD;
-        __threadfence();
+        scratch_sync();
         __syncthreads();
         const int *rows = arena + d.rows;
         const int *groups = arena + d.groups;
@@ -797,7 +802,7 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
             scratch_sync();
             pf_reverse(P, R, false, P.edge_ref, nullptr);
         }
-        __threadfence();
+        scratch_sync();
         __syncthreads();
         const double Z = pfa[f.out + 0];
         const double thr = pfa[f.scal + 2];
@@ -839,7 +844,7 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
                                 const int x = fixed_at(gal, gbl, gbr, W, ar - 1, br - 1);
                                 if (x >= 0) atomicAdd(&Mb[x], P.Dbar[ga.y] * P.eam[ga.y]);
                             }
-                            __threadfence();  // Es/Ss (global) and the seeds (LDS) before the sweep
+                            scratch_sync();  // Es/Ss (global) and the seeds (LDS) before the sweep
                             if (pass == 0) {
                                 const double o = nc1 ? pf_rev_fixed<1, true>(P, gal, gbl, gar, gbr, W, base, Es, Ss,
                                                                                 P.edge_full)
@@ -878,7 +883,7 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
                         }
                     }
                 }
-                __threadfence();
+                scratch_sync();
                 __syncthreads();
             }
         }
