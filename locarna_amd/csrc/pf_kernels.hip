@@ -36,39 +36,70 @@ namespace {
 
 __device__ __forceinline__ int lane_id() { return threadIdx.x & 63; }
 
+// DPP move of a double (both halves): lane l receives the lane CTRL names;
+// lanes without a source, and rows outside ROWMASK, receive `old`
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_d(double old, double v) {
+    const unsigned long long o = (unsigned long long)__double_as_longlong(old);
+    const unsigned long long x = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp((int)(unsigned)o, (int)(unsigned)x, CTRL, ROWMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp((int)(unsigned)(o >> 32), (int)(unsigned)(x >> 32), CTRL, ROWMASK,
+                                               0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+    const unsigned long long x = (unsigned long long)__double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(unsigned)x, l);
+    const int hi = __builtin_amdgcn_readlane((int)(unsigned)(x >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// one step of the prefix composition: (a, b) after the map from the source
+// lane (identity where there is none)
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ void affine_step(double &a, double &b) {
+    const double pa = dpp_d<CTRL, ROWMASK>(1.0, a), pb = dpp_d<CTRL, ROWMASK>(0.0, b);
+    b = a * pb + b;
+    a = a * pa;
+}
+
 // inclusive scan of the affine maps x -> a x + b over the lanes (lane 0
-// first): result (A_l, B_l) with F_l = A_l F_in + B_l
+// first): result (A_l, B_l) with F_l = A_l F_in + B_l.  DPP row shifts
+// within rows of 16, then row broadcasts of lanes 15 and 31.
 __device__ __forceinline__ void wave_scan_affine(double &a, double &b) {
-    const int lane = lane_id();
-    for (int d = 1; d < 64; d <<= 1) {
-        const double pa = __shfl_up(a, d), pb = __shfl_up(b, d);
-        if (lane >= d) {
-            b = a * pb + b;
-            a = a * pa;
-        }
-    }
+    affine_step<0x111, 0xf>(a, b);  // row_shr:1
+    affine_step<0x112, 0xf>(a, b);  // row_shr:2
+    affine_step<0x114, 0xf>(a, b);  // row_shr:4
+    affine_step<0x118, 0xf>(a, b);  // row_shr:8
+    affine_step<0x142, 0xa>(a, b);  // row_bcast:15 -> rows 1, 3
+    affine_step<0x143, 0xc>(a, b);  // row_bcast:31 -> rows 2, 3
 }
 
 // suffix version: x_l = g_l + c_l x_{l+1} (last lane first); returns the
-// pair (C_l, G_l) with x_l = C_l x_in + G_l where x_in enters at lane 63
+// pair (C_l, G_l) with x_l = C_l x_in + G_l where x_in enters at lane 63.
+// DPP row shifts to the left within rows of 16; then each row is composed
+// with the rows after it (their totals read from lanes 16, 32, 48).
 __device__ __forceinline__ void wave_suffix_affine(double &c, double &g) {
-    const int lane = lane_id();
-    for (int d = 1; d < 64; d <<= 1) {
-        const double nc = __shfl_down(c, d), ng = __shfl_down(g, d);
-        if (lane + d < 64) {
-            g = c * ng + g;
-            c = c * nc;
-        }
-    }
+    affine_step<0x101, 0xf>(c, g);  // row_shl:1
+    affine_step<0x102, 0xf>(c, g);  // row_shl:2
+    affine_step<0x104, 0xf>(c, g);  // row_shl:4
+    affine_step<0x108, 0xf>(c, g);  // row_shl:8
+    const double c1 = readlane_d(c, 16), g1 = readlane_d(g, 16);
+    const double c2 = readlane_d(c, 32), g2 = readlane_d(g, 32);
+    const double c3 = readlane_d(c, 48), g3 = readlane_d(g, 48);
+    const double s1c = c2 * c3, s1g = c2 * g3 + g2;      // rows 2..3
+    const double s0c = c1 * s1c, s0g = c1 * s1g + g1;    // rows 1..3
+    const int row = lane_id() >> 4;
+    const double sc = row == 0 ? s0c : row == 1 ? s1c : row == 2 ? c3 : 1.0;
+    const double sg = row == 0 ? s0g : row == 1 ? s1g : row == 2 ? g3 : 0.0;
+    g = c * sg + g;
+    c = c * sc;
 }
 
 __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o);
     return v;
-}
-
-__device__ __forceinline__ double readlane_d(double v, int l) {
-    return __shfl(v, l);
 }
 
 struct PfPair {
@@ -189,7 +220,7 @@ __device__ void pf_fill(const PfPair &P, const Region &R) {
                 gB = P.egB[j];
             }
             // F chain: F_j = F_{j-1} gB + s_{j-1} gB o
-            double sl = __shfl_up(s, 1);
+            double sl = dpp_d<0x138, 0xf>(s, s);  // lane l-1 (lane 0: its own)
             if (lane == 0) sl = sprev;
             double a = act ? gB : 1.0, b = act ? sl * gB * eo : 0.0;
             wave_scan_affine(a, b);
@@ -244,14 +275,14 @@ __device__ void pf_reverse(const PfPair &P, const Region &R, bool want_dbar, dou
             const double g = act ? R.Mb[idx] : 0.0;
             const double gB = act ? P.egB[j] : 1.0;
             // Fbar_j = g_j + gB(j+1) Fbar_{j+1}
-            double a_next = __shfl_down(act ? gB : 0.0, 1);
+            double a_next = dpp_d<0x130, 0xf>(act ? gB : 0.0, act ? gB : 0.0);  // lane l+1
             if (lane == 63 || !(j + 1 <= hi)) a_next = (lane == 63) ? a_in : 0.0;
             double c = act ? a_next : 0.0, gg = act ? g : 0.0;
             wave_suffix_affine(c, gg);
             const double Fb = c * Fb_in + gg;  // Fbar_j
             // sbar_j = g_j + Fbar_{j+1} gB(j+1) o
-            double Fb_next = __shfl_down(Fb, 1);
-            double gB_next = __shfl_down(gB, 1);
+            double Fb_next = dpp_d<0x130, 0xf>(Fb, Fb);
+            double gB_next = dpp_d<0x130, 0xf>(gB, gB);
             if (lane == 63) {
                 Fb_next = Fb_in;
                 gB_next = a_in;
@@ -336,14 +367,10 @@ struct PfEnt {
     double D;
 };
 
-__device__ __forceinline__ double shift_up_d(double v, double first) {
-    const double r = __shfl_up(v, 1);
-    return lane_id() == 0 ? first : r;
-}
-__device__ __forceinline__ double shift_down_d(double v, double last) {
-    const double r = __shfl_down(v, 1);
-    return lane_id() == 63 ? last : r;
-}
+// lane l receives lane l-1's value, lane 0 `first` (DPP wave_shr:1)
+__device__ __forceinline__ double shift_up_d(double v, double first) { return dpp_d<0x138, 0xf>(first, v); }
+// lane l receives lane l+1's value, lane 63 `last` (DPP wave_shl:1)
+__device__ __forceinline__ double shift_down_d(double v, double last) { return dpp_d<0x130, 0xf>(last, v); }
 
 // candidate range [t0, t1) of rows al+1+r, r = lane + 64 h
 __device__ __forceinline__ void pf_row_table(const PfPair &P, int al, int bl, int br, int nrows, int rt0[2],
