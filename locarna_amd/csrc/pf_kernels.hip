@@ -395,13 +395,19 @@ __device__ __forceinline__ void pf_row_cands(const int rt0[2], const int rt1[2],
     t0 = (r < 64) ? __builtin_amdgcn_readlane(rt0[0], r & 63) : __builtin_amdgcn_readlane(rt0[1], r & 63);
     t1 = (r < 64) ? __builtin_amdgcn_readlane(rt1[0], r & 63) : __builtin_amdgcn_readlane(rt1[1], r & 63);
 }
+// Unpredicated loads: lanes past the range read its last entry (or entry 0
+// of the pair; a group implies arc matches) and are ignored by the caller's
+// lane flag.
 __device__ __forceinline__ PfEnt pf_fetch(const PfPair &P, int t, int t1) {
-    PfEnt e{0, 0, 0.0};
-    if (t < t1) {
-        e.pq = P.fcand[3 * t];
-        e.jm = P.fcand[3 * t + 1];
-        e.D = P.fD[t];
-    }
+    const int tt = t < t1 ? t : max(t1 - 1, 0);
+    PfEnt e;
+    struct PJ {
+        int x, y;
+    };  // 4-byte aligned pair (one 8-byte load)
+    const PJ pj = *reinterpret_cast<const PJ *>(P.fcand + 3 * tt);
+    e.pq = pj.x;
+    e.jm = pj.y;
+    e.D = P.fD[tt];
     return e;
 }
 __device__ __forceinline__ void lds_fence() { __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront"); }
@@ -479,11 +485,12 @@ __device__ __forceinline__ void pf_fwd_fixed(const PfPair &P, int al, int bl, in
         for (int k = 0; k < 4; ++k)
             if (t0 + 64 * k < t1) arc(cur[k], t0 + 64 * k + lane < t1);
         for (int t = t0 + 256 + lane; t < t1; t += 64) arc(pf_fetch(P, t, t1), true);
-        if (r + 1 < nrows) {
+        if (r + 1 < nrows) {  // the next row's chunks
             int u0, u1;
             pf_row_cands(rt0, rt1, r + 1, u0, u1);
 #pragma unroll
-            for (int k = 0; k < 4; ++k) cur[k] = pf_fetch(P, u0 + 64 * k + lane, u1);
+            for (int k = 0; k < 4; ++k)
+                if (u0 + 64 * k < u1) cur[k] = pf_fetch(P, u0 + 64 * k + lane, u1);
         }
         lds_fence();
         double Fcarry = 0.0, sprev = Mcol, dgin = Mcol_prev;
@@ -554,9 +561,29 @@ __device__ __forceinline__ double pf_rev_fixed(const PfPair &P, int al, int bl, 
     }
     double colcarry = 0.0;  // contribution of row i+1 to Mbar(i, bl)
     double open_acc = 0.0;
+    // first 64 fill entries of the row (and their candidate entries),
+    // fetched one row ahead; rows are visited downwards
+    PfEnt ce;
+    int cx;
+    {
+        int u0 = 0, u1 = 0;
+        if (nrows > 0) pf_row_cands(rt0, rt1, nrows - 1, u0, u1);
+        ce = pf_fetch(P, u0 + lane, u1);
+        cx = FULL ? P.f2c[u0 + lane < u1 ? u0 + lane : max(u1 - 1, 0)] : 0;
+    }
     lds_fence();
     for (int i = ar - 1; i > al; --i) {
         const int r = i - al - 1;
+        PfEnt ne = ce;
+        int nx = cx;
+        if (r > 0) {
+            int u0, u1;
+            pf_row_cands(rt0, rt1, r - 1, u0, u1);
+            if (u1 > u0) {
+                ne = pf_fetch(P, u0 + lane, u1);
+                if (FULL) nx = P.f2c[u0 + lane < u1 ? u0 + lane : u1 - 1];
+            }
+        }
         const int mni = P.mn[i];
         const int lo = max(bl, mni), hi = min(br - 1, P.mx[i]), jlo = max(bl + 1, mni);
         const bool colopen_i = mni <= bl;
@@ -619,17 +646,20 @@ __device__ __forceinline__ double pf_rev_fixed(const PfPair &P, int al, int bl, 
         // arc terms: Mbar(p-1, q-1) += sbar D ps; Dbar += sbar M(p-1, q-1) ps
         int t0, t1;
         pf_row_cands(rt0, rt1, r, t0, t1);
-        for (int t = t0 + lane; t < t1; t += 64) {
-            const PfEnt e = pf_fetch(P, t, t1);
+        auto arc = [&](const PfEnt &e, int x, bool in) {
             const int p = e.pq & 0xffff, q = e.pq >> 16, j = e.jm & 0xffff;
             const int rl = max(bl, e.jm >> 16);
-            if (q > bl && j >= jlo && j <= hi && q - 1 >= rl && e.D != 0.0) {
+            if (in && q > bl && j >= jlo && j <= hi && q - 1 >= rl && e.D != 0.0) {
                 const double sb = Sb[j - bl - 1];
                 const int idx = (p - 1 - al) * W + (q - 1 - rl);
                 atomicAdd(&Mb[idx], sb * e.D * ps);
-                if (FULL) atomicAdd(&P.Dbar[P.f2c[t]], sb * Mst[idx] * ps);
+                if (FULL) atomicAdd(&P.Dbar[x], sb * Mst[idx] * ps);
             }
-        }
+        };
+        if (t1 > t0) arc(ce, cx, t0 + lane < t1);
+        for (int t = t0 + 64 + lane; t < t1; t += 64) arc(pf_fetch(P, t, t1), FULL ? P.f2c[t] : 0, true);
+        ce = ne;
+        cx = nx;
         lds_fence();
 #pragma unroll
         for (int c = 0; c < NC; ++c) carry[c] = newcarry[c];
