@@ -1081,10 +1081,12 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     B->bp.wg_per_pair = K;
     B->bp.fixed_cols = 1;
     if (const char *f = std::getenv("LCG_FIXED_COLS")) B->bp.fixed_cols = std::atoi(f) != 0;
-    // The top-level / traceback kernel runs the top level on one wave: two
-    // workgroups of 8 waves per CU (half the LDS each) keep more pairs in
-    // flight, as long as every group region still fits the smaller pool.
-    int top_waves = 8, top_wgs = 2;
+    // The top-level / traceback kernel runs the top level on one wave: four
+    // workgroups of 4 waves per CU (a quarter of the LDS each) keep more
+    // pairs in flight, as long as every group region still fits the smaller
+    // pool (allpairs64: 3.36 ms against 4.57 ms with 2 x 8 and 7.42 ms with
+    // 1 x 16).
+    int top_waves = 4, top_wgs = 4;
     if (const char *w = std::getenv("LCG_TOP_WAVES")) top_waves = std::min(16, std::max(1, std::atoi(w)));
     if (const char *w = std::getenv("LCG_TOP_WGS_PER_CU")) top_wgs = std::min(4, std::max(1, std::atoi(w)));
     int top_pool = (lds_ints_cu / top_wgs - hdr) & ~3;
