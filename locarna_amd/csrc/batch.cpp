@@ -1396,8 +1396,8 @@ extern "C" int lcg_batch_kernel_times(const lcg_batch *B, float *fill_ms, float 
 // ------------------------------------------------------------ LocARNA-P
 
 extern "C" int lcg_launch_pf(const PairDesc *descs, const PfDesc *pdescs, const int *order, int *queue, int *arena,
-                             double *pfa, double *scratch, const int *sched, PfBatchParams bp, int grid, int waves,
-                             size_t lds_bytes, hipStream_t stream);
+                             double *pfa, double *scratch, const int *sched, int *barriers, PfBatchParams bp, int grid,
+                             int waves, size_t lds_bytes, hipStream_t stream);
 
 namespace {
 int pf_setup(lcg_batch &B) {
@@ -1450,9 +1450,20 @@ int pf_setup(lcg_batch &B) {
     if (pool < 64) return LCG_E_CAPACITY;
     B.pbp.pool = pool;
     B.pf_lds = (size_t)4 * hdr_ints + (size_t)8 * (hdr_doubles + pool);
-    // schedule: per row, the groups in rounds of at most pf_waves; narrow
-    // groups (columns and rows at most 128) get LDS regions that fit the pool
-    // together, the others run in per-wave global scratch (offset -1)
+    // workgroups per pair: with fewer pairs than CUs, K workgroups share a
+    // pair (all resident, one per CU)
+    int cus = 0;
+    HIPCHECK(hipSetDevice(B.device));
+    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B.device));
+    int K = np < cus ? std::min(64, cus / np) : 1;
+    if (const char *w = std::getenv("LCG_PF_WG_PER_PAIR")) K = std::min(64, std::max(1, std::atoi(w)));
+    K = std::min(K, std::max(1, cus / np));
+    B.pbp.wg_per_pair = K;
+    // schedule: per row, the groups dealt to the K members (longest first to
+    // the least loaded), then per member in rounds of at most pf_waves;
+    // narrow groups (columns and rows at most 128) get LDS regions that fit
+    // the pool together, the others run in per-wave global scratch (offset
+    // -1).  Word: round | wave<<12 | member<<16.
     std::vector<int32_t> sched;
     size_t max_fixed = 0;
     for (int k = 0; k < np; ++k) {
@@ -1469,8 +1480,6 @@ int pf_setup(lcg_batch &B) {
         for (int al = d.n; al >= 1; --al) {
             const int g0 = A[d.rows + al + 1], g1 = A[d.rows + al];
             for (int pass = 0; pass < 2; ++pass) {
-                std::vector<int> idx;
-                for (int g = g0; g < g1; ++g) idx.push_back(g);
                 auto size_of = [&](int g) -> long {
                     const int32_t *G = A + gbase + g * LCG_GROUP_INTS;
                     const int gal = G[0] & 0xffff, gbl = G[0] >> 16, gar = G[1] & 0xffff, gbr = G[1] >> 16;
@@ -1479,28 +1488,49 @@ int pf_setup(lcg_batch &B) {
                     const long sz = (gbr - gbl) + (pass == 0 ? 1 : 2) * cells;
                     return sz <= pool ? ((sz + 1) & ~1L) : -1;
                 };
-                std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return size_of(x) > size_of(y); });
-                std::vector<long> used;
-                std::vector<int> cnt;
-                for (int g : idx) {
-                    const long sz = size_of(g);
-                    size_t r = 0;
-                    while (r < used.size() && (cnt[r] >= B.pf_waves || (sz > 0 && used[r] + sz > pool))) ++r;
-                    if (r == used.size()) {
-                        used.push_back(0);
-                        cnt.push_back(0);
+                auto rows_of = [&](int g) {
+                    const int32_t *G = A + gbase + g * LCG_GROUP_INTS;
+                    return (G[1] & 0xffff) - (G[0] & 0xffff);
+                };
+                std::vector<std::vector<int>> mem(K);
+                {
+                    std::vector<int> by_len;
+                    for (int g = g0; g < g1; ++g) by_len.push_back(g);
+                    std::stable_sort(by_len.begin(), by_len.end(), [&](int x, int y) { return rows_of(x) > rows_of(y); });
+                    std::vector<long> load(K, 0);
+                    for (int g : by_len) {
+                        const int mm = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                        mem[mm].push_back(g);
+                        load[mm] += rows_of(g);
                     }
-                    int32_t *S = &sched[f.sched + 4 * (size_t)g + 2 * pass];
-                    S[0] = (int32_t)r | (cnt[r] << 12);
-                    S[1] = sz > 0 ? (int32_t)used[r] : -1;
-                    if (sz > 0) {
-                        used[r] += sz;
-                        const int32_t *G = A + gbase + g * LCG_GROUP_INTS;
-                        max_fixed = std::max(max_fixed, (size_t)2 * ((G[1] & 0xffff) - (G[0] & 0xffff)) * G[2]);
-                    }
-                    cnt[r] += 1;
                 }
-                sched[(pass == 0 ? f.rin : f.rout) + al] = (int32_t)used.size();
+                size_t nr = 0;
+                for (int mm = 0; mm < K; ++mm) {
+                    std::vector<int> &idx = mem[mm];
+                    std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return size_of(x) > size_of(y); });
+                    std::vector<long> used;
+                    std::vector<int> cnt;
+                    for (int g : idx) {
+                        const long sz = size_of(g);
+                        size_t r = 0;
+                        while (r < used.size() && (cnt[r] >= B.pf_waves || (sz > 0 && used[r] + sz > pool))) ++r;
+                        if (r == used.size()) {
+                            used.push_back(0);
+                            cnt.push_back(0);
+                        }
+                        int32_t *S = &sched[f.sched + 4 * (size_t)g + 2 * pass];
+                        S[0] = (int32_t)r | (cnt[r] << 12) | (mm << 16);
+                        S[1] = sz > 0 ? (int32_t)used[r] : -1;
+                        if (sz > 0) {
+                            used[r] += sz;
+                            const int32_t *G = A + gbase + g * LCG_GROUP_INTS;
+                            max_fixed = std::max(max_fixed, (size_t)2 * ((G[1] & 0xffff) - (G[0] & 0xffff)) * G[2]);
+                        }
+                        cnt[r] += 1;
+                    }
+                    nr = std::max(nr, used.size());
+                }
+                sched[(pass == 0 ? f.rin : f.rout) + al] = (int32_t)nr;
             }
         }
         // group indices in sched are relative to the pair's group array
@@ -1509,10 +1539,7 @@ int pf_setup(lcg_batch &B) {
     stride = std::max(stride, max_fixed);
     B.pbp.scratch_stride = (int32_t)((stride + 31) & ~(size_t)31);
     hipError_t e;
-    HIPCHECK(hipSetDevice(B.device));
-    int cus = 0;
-    HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B.device));
-    B.pf_grid = std::min(np, cus);
+    B.pf_grid = K > 1 ? np * K : std::min(np, cus);
     HIPCHECK(hipMalloc(&B.d_pfa, std::max<size_t>(1, off) * 8));
     HIPCHECK(hipMalloc(&B.d_pdescs, sizeof(PfDesc) * np));
     HIPCHECK(hipMalloc(&B.d_pfscratch, (size_t)B.pf_grid * B.pf_waves * B.pbp.scratch_stride * 8));
@@ -1556,13 +1583,16 @@ extern "C" int lcg_batch_pf_run(lcg_batch *B, const double *dtheta, float *devic
     HIPCHECK(hipMemcpy(B->d_pfa, B->pf_arena.data(), B->pf_arena.size() * 8, hipMemcpyHostToDevice));
     HIPCHECK(hipEventRecord(B->ev0, B->stream));
     int rc = lcg_launch_pf(B->d_descs, B->d_pdescs, B->d_order, B->d_queue, B->d_arena, B->d_pfa, B->d_pfscratch,
-                           B->d_pfsched, B->pbp, B->pf_grid, B->pf_waves, B->pf_lds, B->stream);
+                           B->d_pfsched, B->d_barriers, B->pbp, B->pf_grid, B->pf_waves, B->pf_lds, B->stream);
     if (rc) return hip_fail((hipError_t)rc);
     HIPCHECK(hipEventRecord(B->ev1, B->stream));
     HIPCHECK(hipEventSynchronize(B->ev1));
     if (device_ms) HIPCHECK(hipEventElapsedTime(device_ms, B->ev0, B->ev1));
     B->pf_result.resize(B->pf_arena.size());
     HIPCHECK(hipMemcpy(B->pf_result.data(), B->d_pfa, B->pf_result.size() * 8, hipMemcpyDeviceToHost));
+    // a bounded wait of the multi-workgroup path that expired invalidates the run
+    for (const PfDesc &f : B->pdescs)
+        if (B->pf_result[f.out + 2] != 0.0) return LCG_E_STATE;
     B->pf_have = true;
     return LCG_OK;
 }

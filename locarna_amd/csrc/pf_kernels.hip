@@ -694,11 +694,14 @@ __device__ __forceinline__ void zero_region_bar(const Region &R, const PfPair &P
 // Outside: rows al = 1 .. n, each group's forward values recomputed, then
 // the full adjoint (Dbar, edge weights, openings) and, if any of its arc
 // matches has P(am) > sqrt(min_am_prob), the restricted one.
+#define LCG_PF_SPIN_LIMIT (1 << 22)
+
 extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *__restrict__ descs,
                                                                 const PfDesc *__restrict__ pdescs,
                                                                 const int *__restrict__ order, int *queue,
                                                                 int *arena, double *pfa, double *scratch,
-                                                                const int *__restrict__ sched, PfBatchParams bp) {
+                                                                const int *__restrict__ sched, int *barriers,
+                                                                PfBatchParams bp) {
     extern __shared__ int lds[];
     __shared__ int s_pair;
     __shared__ double s_open;
@@ -713,8 +716,18 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
     double *egA = dl, *egB = dl + (L + 1), *es = dl + 2 * (L + 1);
     double *pool = es + LCG_NCODES * LCG_NCODES;
     double *wscr = scratch + (size_t)(blockIdx.x * nwaves + wave) * bp.scratch_stride;
-    for (;;) {
-        if (threadIdx.x == 0) s_pair = atomicAdd(queue, 1);
+    // K > 1: K workgroups (members) share pair blockIdx / K, all resident;
+    // the groups of a row are dealt to the members, which meet at a barrier
+    // after every row (arrival counter per pair, bounded wait)
+    const int K = bp.wg_per_pair;
+    const int member = K > 1 ? (int)blockIdx.x % K : 0;
+    for (int iter = 0;; ++iter) {
+        if (K > 1) {
+            if (iter > 0) break;
+            if (threadIdx.x == 0) s_pair = (int)blockIdx.x / K;
+        } else if (threadIdx.x == 0) {
+            s_pair = atomicAdd(queue, 1);
+        }
         __syncthreads();
         const int k = s_pair;
         if (k >= bp.npairs) break;
@@ -738,16 +751,41 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
         }
         for (int t = threadIdx.x; t < 2 * d.nfblk; t += blockDim.x) fbk[t] = arena[d.fblk + t];
         for (int t = threadIdx.x; t < LCG_NCODES * LCG_NCODES; t += blockDim.x) es[t] = pfa[f.es + t];
-        for (int t = threadIdx.x; t < d.nam; t += blockDim.x) {
-            pfa[f.Dc + t] = 0.0;
-            pfa[f.Dbar + t] = 0.0;
-            pfa[f.fD + t] = 0.0;
-        }
-        for (int t = threadIdx.x; t < d.ncells; t += blockDim.x) {
-            pfa[f.edge_full + t] = 0.0;
-            pfa[f.edge_ref + t] = 0.0;
+        if (member == 0) {  // the pair's accumulators (then a barrier when K > 1)
+            for (int t = threadIdx.x; t < d.nam; t += blockDim.x) {
+                pfa[f.Dc + t] = 0.0;
+                pfa[f.Dbar + t] = 0.0;
+                pfa[f.fD + t] = 0.0;
+            }
+            for (int t = threadIdx.x; t < d.ncells; t += blockDim.x) {
+                pfa[f.edge_full + t] = 0.0;
+                pfa[f.edge_ref + t] = 0.0;
+            }
+            if (threadIdx.x == 0) {
+                pfa[f.out + 1] = 0.0;  // gap openings (members add theirs)
+                pfa[f.out + 2] = 0.0;  // status: 1 if a bounded wait expired
+            }
         }
         if (threadIdx.x == 0) s_open = 0.0;
+        int passed = 0;  // barrier generations passed
+        // all members' global writes before any member goes on
+        auto pair_barrier = [&]() {
+            __threadfence();
+            __syncthreads();
+            ++passed;
+            if (threadIdx.x == 0) {
+                atomicAdd(&barriers[k], 1);
+                for (int spin = 0; atomicAdd(&barriers[k], 0) < K * passed; ++spin) {
+                    if (spin > LCG_PF_SPIN_LIMIT) {
+                        pfa[f.out + 2] = 1.0;
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            __syncthreads();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        };
         PfPair P;
         P.mn = mn;
         P.mx = mx;
@@ -773,8 +811,13 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
         P.fcand = arena + d.fcand;
         P.f2c = arena + d.f2c;
         P.fD = pfa + f.fD;
-        scratch_sync();
-        __syncthreads();
+        if (K > 1) {
+            pair_barrier();
+        } else {
+            scratch_sync();
+            __syncthreads();
+        }
+        const int mine = member << 16;
         const int *rows = arena + d.rows;
         const int *groups = arena + d.groups;
         const int4 *gams = reinterpret_cast<const int4 *>(arena + d.gams);
@@ -808,7 +851,7 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
             const int nr = sched[f.rin + al];
             for (int rnd = 0; rnd < nr; ++rnd) {
                 for (int g = g0; g < g1; ++g) {
-                    if (S[4 * g] != (rnd | (wave << 12))) continue;
+                    if (S[4 * g] != (rnd | (wave << 12) | mine)) continue;
                     const int *G = groups + g * LCG_GROUP_INTS;
                     const int gal = G[0] & 0xffff, gbl = G[0] >> 16, gar = G[1] & 0xffff, gbr = G[1] >> 16;
                     const int W = G[2];
@@ -843,9 +886,10 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
                 __threadfence_block();
                 __syncthreads();
             }
+            if (K > 1) pair_barrier();
         }
-        // ---- top level and its adjoint (wave 0)
-        if (wave == 0) {
+        // ---- top level and its adjoint (wave 0 of member 0)
+        if (wave == 0 && member == 0) {
             const Region R = region(0, 0, n + 1, m + 1, d.wtop);
             pf_fill(P, R);
             const double Z = *R.at(R.M, P, n, m);
@@ -859,8 +903,12 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
             scratch_sync();
             pf_reverse(P, R, false, P.edge_ref, nullptr);
         }
-        scratch_sync();
-        __syncthreads();
+        if (K > 1) {
+            pair_barrier();
+        } else {
+            scratch_sync();
+            __syncthreads();
+        }
         const double Z = pfa[f.out + 0];
         const double thr = pfa[f.scal + 2];
         // ---- outside over the groups, rows al = 1 .. n
@@ -870,7 +918,7 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
             const int nr = sched[f.rout + al];
             for (int rnd = 0; rnd < nr; ++rnd) {
                 for (int g = g0; g < g1; ++g) {
-                    if (S[4 * g + 2] != (rnd | (wave << 12))) continue;
+                    if (S[4 * g + 2] != (rnd | (wave << 12) | mine)) continue;
                     const int *G = groups + g * LCG_GROUP_INTS;
                     const int gal = G[0] & 0xffff, gbl = G[0] >> 16, gar = G[1] & 0xffff, gbr = G[1] >> 16;
                     const int W = G[2];
@@ -943,10 +991,20 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
                 scratch_sync();
                 __syncthreads();
             }
+            if (K > 1) pair_barrier();
         }
-        if (threadIdx.x == 0) pfa[f.out + 1] = s_open;
+        if (threadIdx.x == 0) {
+            if (K > 1)
+                atomicAdd(&pfa[f.out + 1], s_open);
+            else
+                pfa[f.out + 1] = s_open;
+        }
         __syncthreads();
     }
+}
+
+extern "C" __global__ void lcg_pf_reset_barriers(int *barriers, int n) {
+    for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) barriers[t] = 0;
 }
 
 extern "C" __global__ void lcg_pf_reset_queue(int *queue) {
@@ -954,8 +1012,8 @@ extern "C" __global__ void lcg_pf_reset_queue(int *queue) {
 }
 
 extern "C" int lcg_launch_pf(const PairDesc *descs, const PfDesc *pdescs, const int *order, int *queue, int *arena,
-                             double *pfa, double *scratch, const int *sched, PfBatchParams bp, int grid, int waves,
-                             size_t lds_bytes, hipStream_t stream) {
+                             double *pfa, double *scratch, const int *sched, int *barriers, PfBatchParams bp, int grid,
+                             int waves, size_t lds_bytes, hipStream_t stream) {
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void *)lcg_pf_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -964,7 +1022,9 @@ extern "C" int lcg_launch_pf(const PairDesc *descs, const PfDesc *pdescs, const 
         attr_set = true;
     }
     hipLaunchKernelGGL(lcg_pf_reset_queue, dim3(1), dim3(64), 0, stream, queue);
+    if (bp.wg_per_pair > 1)
+        hipLaunchKernelGGL(lcg_pf_reset_barriers, dim3(1), dim3(256), 0, stream, barriers, bp.npairs);
     hipLaunchKernelGGL(lcg_pf_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, pdescs, order, queue,
-                       arena, pfa, scratch, sched, bp);
+                       arena, pfa, scratch, sched, barriers, bp);
     return (int)hipGetLastError();
 }

@@ -38,7 +38,8 @@ typedef struct {
     int32_t scratch_stride;  // doubles of global scratch per (workgroup, wave)
     int32_t pool;            // doubles of LDS pool for group regions
     int32_t nfblk_max;       // fill blocks of the largest pair (LDS header)
-    int32_t pad[3];
+    int32_t wg_per_pair;     // workgroups sharing one pair (fewer pairs than CUs)
+    int32_t pad[2];
 } PfBatchParams;
 
 // score parameters theta: sigma 4x4, arc-match term 16x16, indel,
