@@ -164,6 +164,11 @@ int lcg_batch_kernel_times(const lcg_batch *batch, float *fill_ms, float *top_ms
  * align_inside / align_outside, aligner_p.icc:413-438, 1113-1139). */
 #define LCG_NTHETA 275
 int lcg_batch_pf_run(lcg_batch *batch, const double *dtheta, float *device_ms);
+/* LocARNA-P launch configuration of the batch (no counterpart in the
+ * reference): the first `nstatic` pairs of the work order (largest first) run
+ * on `wg_per_pair` workgroups each, the others one workgroup each from a
+ * queue; `workgroups` in all. */
+int lcg_batch_pf_config(lcg_batch *batch, int32_t *wg_per_pair, int32_t *nstatic, int32_t *workgroups);
 /* Z (AlignerP::align_inside) and the expected number of gap openings */
 int lcg_batch_pf_result(const lcg_batch *batch, int32_t k, double *Z, double *openings);
 /* arc-match probabilities in the reference's arc-match order

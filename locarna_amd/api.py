@@ -77,6 +77,7 @@ def _sig(L):
     L.lcg_batch_pf_D.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_void_p, c.c_int32]
     L.lcg_batch_bm_probs.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_void_p, c.c_int32]
     L.lcg_batch_pf_grad.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int32]
+    L.lcg_batch_pf_config.argtypes = [c.c_void_p] + [c.POINTER(c.c_int32)] * 3
     L.lcg_batch_pair_info.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_int32), c.POINTER(c.c_int32),
                                       c.POINTER(c.c_int64)]
     L.lcg_batch_segment.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int64]
@@ -280,6 +281,14 @@ class Batch:
         return list(out)[:cnt]
 
     # ---- LocARNA-P
+    def pf_config(self):
+        """(workgroups per split pair, split pairs, workgroups) of LocARNA-P"""
+        a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        rc = lib().lcg_batch_pf_config(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        if rc != 0:
+            raise LcgError("lcg_batch_pf_config (%d)" % rc)
+        return a.value, b.value, c.value
+
     def pf_run(self, dtheta=None):
         """Inside + adjoint for all pairs; dtheta: offsets of the NTHETA score
         parameters from the reference scores (None: reference)."""

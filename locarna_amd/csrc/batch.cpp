@@ -587,6 +587,7 @@ struct lcg_batch {
     std::vector<int32_t> host_arena;  // image uploaded at creation (host packing)
     size_t arena_ints = 0;
     std::vector<int32_t> order;       // work queue order (longest first)
+    std::vector<int64_t> work;        // DP cells per pair (work estimate)
     int32_t *d_arena = nullptr;
     PairDesc *d_descs = nullptr;
     int32_t *d_order = nullptr;
@@ -1019,6 +1020,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
         top_max = std::max(top_max, d.top_ints);
     }
     // longest pairs first (greedy LPT over the persistent workgroups)
+    B->work = work;
     B->order.resize(npairs);
     std::iota(B->order.begin(), B->order.end(), 0);
     std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) { return work[a] > work[b]; });
@@ -1458,7 +1460,39 @@ int pf_setup(lcg_batch &B) {
     int K = np < cus ? std::min(64, cus / np) : 1;
     if (const char *w = std::getenv("LCG_PF_WG_PER_PAIR")) K = std::min(64, std::max(1, std::atoi(w)));
     K = std::min(K, std::max(1, cus / np));
+    // the first nst pairs of the work order get K workgroups each, statically;
+    // the others are pulled from the queue by the remaining workgroups.  With
+    // at most half as many pairs as CUs all pairs are static.  Otherwise the largest pairs
+    // get two workgroups where that shortens the estimated makespan (a pair
+    // on two workgroups measured about 1.75x faster; the rest by longest
+    // processing time first over the other workgroups).
+    int nst = K > 1 ? np : 0;
+    if (K == 1 && cus >= 4 && !std::getenv("LCG_PF_WG_PER_PAIR")) {
+        auto makespan = [&](int ns) {
+            double t = 0;
+            for (int q = 0; q < ns; ++q) t = std::max(t, (double)B.work[B.order[q]] / 1.75);
+            const int rest = cus - 2 * ns;
+            if (ns < np && rest < 1) return 1e300;
+            std::vector<double> load(std::max(rest, 1), 0.0);
+            for (int q = ns; q < np; ++q) {
+                auto it = std::min_element(load.begin(), load.end());
+                *it += (double)B.work[B.order[q]];
+            }
+            for (double x : load) t = std::max(t, x);
+            return t;
+        };
+        double best = makespan(0);
+        for (int ns = 1; 2 * ns <= cus && ns <= np; ++ns) {
+            const double t = makespan(ns);
+            if (t < best) {
+                best = t;
+                nst = ns;
+                K = 2;
+            }
+        }
+    }
     B.pbp.wg_per_pair = K;
+    B.pbp.nstatic = nst;
     // schedule: per row, the groups dealt to the K members (longest first to
     // the least loaded), then per member in rounds of at most pf_waves;
     // narrow groups (columns and rows at most 128) get LDS regions that fit
@@ -1466,10 +1500,13 @@ int pf_setup(lcg_batch &B) {
     // -1).  Word: round | wave<<12 | member<<16.
     std::vector<int32_t> sched;
     size_t max_fixed = 0;
+    std::vector<int> Kp(np, 1);  // workgroups of each pair
+    for (int q = 0; q < nst; ++q) Kp[B.order[q]] = K;
     for (int k = 0; k < np; ++k) {
         const PairDesc &d = B.descs[k];
         PfDesc &f = B.pdescs[k];
         const int32_t *A = B.host_arena.data();
+        const int KK = Kp[k];
         f.sched = (int32_t)sched.size();
         sched.resize(sched.size() + 4 * (size_t)d.ngroups);
         f.rin = (int32_t)sched.size();
@@ -1492,12 +1529,12 @@ int pf_setup(lcg_batch &B) {
                     const int32_t *G = A + gbase + g * LCG_GROUP_INTS;
                     return (G[1] & 0xffff) - (G[0] & 0xffff);
                 };
-                std::vector<std::vector<int>> mem(K);
+                std::vector<std::vector<int>> mem(KK);
                 {
                     std::vector<int> by_len;
                     for (int g = g0; g < g1; ++g) by_len.push_back(g);
                     std::stable_sort(by_len.begin(), by_len.end(), [&](int x, int y) { return rows_of(x) > rows_of(y); });
-                    std::vector<long> load(K, 0);
+                    std::vector<long> load(KK, 0);
                     for (int g : by_len) {
                         const int mm = (int)(std::min_element(load.begin(), load.end()) - load.begin());
                         mem[mm].push_back(g);
@@ -1505,7 +1542,7 @@ int pf_setup(lcg_batch &B) {
                     }
                 }
                 size_t nr = 0;
-                for (int mm = 0; mm < K; ++mm) {
+                for (int mm = 0; mm < KK; ++mm) {
                     std::vector<int> &idx = mem[mm];
                     std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) { return size_of(x) > size_of(y); });
                     std::vector<long> used;
@@ -1539,7 +1576,8 @@ int pf_setup(lcg_batch &B) {
     stride = std::max(stride, max_fixed);
     B.pbp.scratch_stride = (int32_t)((stride + 31) & ~(size_t)31);
     hipError_t e;
-    B.pf_grid = K > 1 ? np * K : std::min(np, cus);
+    // all workgroups resident at once (one per CU): static members + the rest
+    B.pf_grid = nst == np ? np * K : (nst > 0 ? cus : std::min(np, cus));
     HIPCHECK(hipMalloc(&B.d_pfa, std::max<size_t>(1, off) * 8));
     HIPCHECK(hipMalloc(&B.d_pdescs, sizeof(PfDesc) * np));
     HIPCHECK(hipMalloc(&B.d_pfscratch, (size_t)B.pf_grid * B.pf_waves * B.pbp.scratch_stride * 8));
@@ -1594,6 +1632,16 @@ extern "C" int lcg_batch_pf_run(lcg_batch *B, const double *dtheta, float *devic
     for (const PfDesc &f : B->pdescs)
         if (B->pf_result[f.out + 2] != 0.0) return LCG_E_STATE;
     B->pf_have = true;
+    return LCG_OK;
+}
+
+extern "C" int lcg_batch_pf_config(lcg_batch *B, int32_t *wg_per_pair, int32_t *nstatic, int32_t *workgroups) {
+    if (!B) return LCG_E_ARG;
+    if (B->device_built) return LCG_E_STATE;
+    if (int rc = pf_setup(*B)) return rc;
+    if (wg_per_pair) *wg_per_pair = B->pbp.wg_per_pair;
+    if (nstatic) *nstatic = B->pbp.nstatic;
+    if (workgroups) *workgroups = B->pf_grid;
     return LCG_OK;
 }
 

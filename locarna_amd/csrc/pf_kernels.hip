@@ -716,17 +716,20 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
     double *egA = dl, *egB = dl + (L + 1), *es = dl + 2 * (L + 1);
     double *pool = es + LCG_NCODES * LCG_NCODES;
     double *wscr = scratch + (size_t)(blockIdx.x * nwaves + wave) * bp.scratch_stride;
-    // K > 1: K workgroups (members) share pair blockIdx / K, all resident;
-    // the groups of a row are dealt to the members, which meet at a barrier
-    // after every row (arrival counter per pair, bounded wait)
-    const int K = bp.wg_per_pair;
-    const int member = K > 1 ? (int)blockIdx.x % K : 0;
+    // The first nstatic pairs of the queue order have wg_per_pair workgroups
+    // (members) each, blocks K k .. K k + K - 1, all resident; the groups of a
+    // row are dealt to the members, which meet at a barrier after every row
+    // (arrival counter per pair, bounded wait).  The other workgroups take
+    // the remaining pairs from the queue, one workgroup per pair.
+    const bool stat = (int)blockIdx.x < bp.wg_per_pair * bp.nstatic;
+    const int K = stat ? bp.wg_per_pair : 1;
+    const int member = stat ? (int)blockIdx.x % K : 0;
     for (int iter = 0;; ++iter) {
-        if (K > 1) {
+        if (stat) {
             if (iter > 0) break;
             if (threadIdx.x == 0) s_pair = (int)blockIdx.x / K;
         } else if (threadIdx.x == 0) {
-            s_pair = atomicAdd(queue, 1);
+            s_pair = bp.nstatic + atomicAdd(queue, 1);
         }
         __syncthreads();
         const int k = s_pair;

@@ -38,8 +38,11 @@ typedef struct {
     int32_t scratch_stride;  // doubles of global scratch per (workgroup, wave)
     int32_t pool;            // doubles of LDS pool for group regions
     int32_t nfblk_max;       // fill blocks of the largest pair (LDS header)
-    int32_t wg_per_pair;     // workgroups sharing one pair (fewer pairs than CUs)
-    int32_t pad[2];
+    int32_t wg_per_pair;     // workgroups sharing one of the first nstatic pairs
+    int32_t nstatic;         // pairs (in queue order) with wg_per_pair
+                             // workgroups each, assigned by block index; the
+                             // other workgroups take the rest from the queue
+    int32_t pad[1];
 } PfBatchParams;
 
 // score parameters theta: sigma 4x4, arc-match term 16x16, indel,

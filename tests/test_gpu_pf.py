@@ -91,3 +91,38 @@ def test_gradient_finite_differences():
             fd = (up[k] - dn[k]) / (2 * h)
             assert abs(fd - g[k][comp]) <= 1e-6 * max(1.0, abs(g[k][comp])), (comp, k, fd, g[k][comp])
     B.destroy()
+
+
+def test_pf_split_pairs_match_single(monkeypatch):
+    # 160 pairs of 80-250 nt: more pairs than half the CUs, so the largest
+    # pairs run on two workgroups each and the rest from the queue; every
+    # result matches the same batch with one workgroup per pair, and a
+    # sample matches the oracle
+    pps, pairs, span = synth.workload("grad256")
+    sel = pairs[:160]
+    pa = api.locarna_p_params(max_bp_span=span)
+    P = [api.Pair(pps[a], pps[b], pa) for a, b in sel]
+    mixed = api.Batch(P)
+    K, nst, grid = mixed.pf_config()
+    print("pf config", K, nst, grid, flush=True)
+    assert K == 2 and nst > 0
+    monkeypatch.setenv("LCG_PF_WG_PER_PAIR", "1")
+    single = api.Batch(P)
+    assert single.pf_config()[1] == 0
+    mixed.pf_run()
+    single.pf_run()
+    for k in range(len(P)):
+        Zm, om = mixed.pf_result(k)
+        Zs, os_ = single.pf_result(k)
+        assert abs(Zm - Zs) <= 1e-12 * abs(Zs), k
+        assert abs(om - os_) <= 1e-9 * max(1.0, abs(os_)), k
+        for x, y in zip(mixed.am_probs(k), single.am_probs(k)):
+            assert abs(x - y) <= 1e-10, k
+        for x, y in zip(mixed.pf_grad(k), single.pf_grad(k)):
+            assert abs(x - y) <= 1e-9 * max(1.0, abs(y)), k
+    po = oracle.locarna_p_params(max_bp_span=span)
+    for k in range(0, len(P), 40):
+        o = oracle.pf(pps[sel[k][0]], pps[sel[k][1]], po)
+        _compare(mixed, k, o, ("split", k))
+    mixed.destroy()
+    single.destroy()
