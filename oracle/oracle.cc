@@ -1118,6 +1118,47 @@ extern "C" oracle_result *oracle_align(const char *ppA, const char *ppB, const o
     }
 }
 
+// The band step's trace probabilities (PFTraceProbs, edge_probs.icc:212-268)
+// over the full matrix, with the similarity sigma(i, j) (StralScore) of
+// every cell and the Boltzmann gap weights, for the check of the Gotoh
+// partition function against an enumeration of all alignments.
+// probs, sim: (lenA+1) x (lenB+1) row major; w[0] = exp(gap opening / T),
+// w[1] = exp(gap extension / T), w[2] = T, w[3] = lenA, w[4] = lenB.
+// Returns the cell count.
+extern "C" int oracle_trace_probs(const char *ppA, const char *ppB, const oracle_params *p,
+                                  const char *ribosum_path, double *probs, double *sim, double *w, int cap) {
+    try {
+        using namespace orc;
+        std::unique_ptr<Ribosum> rib;
+        if (p->use_ribosum) rib.reset(new Ribosum(read_ribosum(ribosum_path)));
+        RnaData A = read_pp(ppA, p->min_prob, p->max_bp_span), B = read_pp(ppB, p->min_prob, p->max_bp_span);
+        const int n = A.length(), m = B.length();
+        if (cap < (n + 1) * (m + 1)) return -1;
+        double simm[4][4];
+        for (int i = 0; i < 4; ++i)
+            for (int j = 0; j < 4; ++j) simm[i][j] = rib ? rib->bm[i][j] : (i == j ? p->match : p->mismatch);
+        const Band band(n, m);
+        const double go = p->indel_opening / 100.0, ge = p->indel / 100.0, sw = p->struct_weight / 100.0;
+        const double T = p->temperature_alipf / 100.0;
+        Mat<double> pr = p->band_extended ? trace_probs<long double>(A, B, band, simm, go, ge, sw, T, p->sequ_local)
+                                          : trace_probs<double>(A, B, band, simm, go, ge, sw, T, p->sequ_local);
+        StralScore sc(A, B, simm, sw, go, ge);
+        for (int i = 0; i <= n; ++i)
+            for (int j = 0; j <= m; ++j) {
+                probs[i * (m + 1) + j] = pr(i, j);
+                sim[i * (m + 1) + j] = (i > 0 && j > 0) ? sc.sigma(i, j) : 0.0;
+            }
+        w[0] = exp(go / T);
+        w[1] = exp(ge / T);
+        w[2] = T;
+        w[3] = n;
+        w[4] = m;
+        return (n + 1) * (m + 1);
+    } catch (std::exception &) {
+        return -2;
+    }
+}
+
 extern "C" void oracle_free(oracle_result *r) { delete r; }
 extern "C" long long oracle_score(const oracle_result *r) { return r->score; }
 extern "C" double oracle_partition_function(const oracle_result *r) { return r->pf; }

@@ -9,9 +9,10 @@
  *   - pp reader, RIBOSUM reader: the reference's own data files
  *     (Data/Examples/example.pp, Data/Matrices/RIBOSUM85_60) with values
  *     checked by hand-derivable properties;
- *   - trace band (PFGotoh): brute-force enumeration of all pairwise
- *     alignments on tiny inputs (trace probabilities) and the closed form
- *     of the band for identical sequences;
+ *   - trace band (PFGotoh, PFTraceProbs): enumeration of all alignments
+ *     (global) or of all prefixes and suffixes (sequence-local) on tiny
+ *     inputs, trace probabilities to 1e-12 in long double and double
+ *     (tests/test_oracle_band.py, through oracle_trace_probs);
  *   - integer DP: exhaustive enumeration of all alignments x nested
  *     arc-match sets on tiny inputs (max score and the traceback's tie rule),
  *     reduction to textbook Gotoh / Smith-Waterman-Gotoh without arcs,
@@ -62,6 +63,10 @@ oracle_result *oracle_pf(const char *ppA, const char *ppB,
                          int include_am_in_bm, char *err, int errlen);
 
 void oracle_free(oracle_result *r);
+/* band step: trace probabilities over the full matrix, the similarity of
+ * every cell and w = (exp(opening/T), exp(extension/T), T, lenA, lenB) */
+int oracle_trace_probs(const char *ppA, const char *ppB, const oracle_params *p, const char *ribosum_path,
+                       double *probs, double *sim, double *w, int cap);
 
 long long oracle_score(const oracle_result *r);
 double oracle_partition_function(const oracle_result *r);

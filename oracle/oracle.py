@@ -170,3 +170,26 @@ def align_many(pp_pairs, params=None, nthreads=1):
     if rc < 0:
         raise RuntimeError("oracle_align_many failed")
     return list(scores)
+
+
+def trace_probs(ppA, ppB, params=None):
+    """Band step of the oracle over the full matrix: (probs, sim, (w_open,
+    w_ext, T)) with probs[i][j] the trace probability of cell (i, j) and
+    sim[i][j] the similarity sigma(i, j) it used (i, j >= 1)."""
+    L = load()
+    params = params or locarna_params()
+    cap = (len(ppA) + 2) * (len(ppB) + 2)  # the lengths are below the text sizes
+    probs = (ctypes.c_double * cap)()
+    sim = (ctypes.c_double * cap)()
+    w = (ctypes.c_double * 5)()
+    L.oracle_trace_probs.restype = ctypes.c_int
+    L.oracle_trace_probs.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Params), ctypes.c_char_p,
+                                     ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int]
+    cnt = L.oracle_trace_probs(ppA.encode(), ppB.encode(), ctypes.byref(params), RIBOSUM.encode(), probs, sim, w,
+                               cap)
+    if cnt < 0:
+        raise RuntimeError("oracle_trace_probs failed (%d)" % cnt)
+    n, m = int(w[3]), int(w[4])
+    P = [[probs[i * (m + 1) + j] for j in range(m + 1)] for i in range(n + 1)]
+    S = [[sim[i * (m + 1) + j] for j in range(m + 1)] for i in range(n + 1)]
+    return P, S, (w[0], w[1], w[2])
