@@ -164,6 +164,17 @@ int lcg_batch_kernel_times(const lcg_batch *batch, float *fill_ms, float *top_ms
  * align_inside / align_outside, aligner_p.icc:413-438, 1113-1139). */
 #define LCG_NTHETA 275
 int lcg_batch_pf_run(lcg_batch *batch, const double *dtheta, float *device_ms);
+/* Stochastic traceback (no counterpart in the reference's AlignerP; for
+ * fitting score parameters to sampled alignments): draws `nsamples`
+ * alignments of pair k from the Boltzmann distribution of the last
+ * lcg_batch_pf_run (global top level), seeded, and returns the mean and
+ * variance of their feature vectors over the LCG_NTHETA parameters (score of
+ * an alignment = reference score + theta . features: sequence match edges by
+ * ACGU pair, tau/100 per arc match by base-pair types, gap positions, gap
+ * openings, struct-weight terms).  The mean estimates
+ * T d log Z / d theta (lcg_batch_pf_grad). */
+int lcg_batch_pf_sample(const lcg_batch *batch, int32_t k, uint64_t seed, int32_t nsamples, double *mean,
+                        double *var, int32_t cap);
 /* LocARNA-P launch configuration of the batch (no counterpart in the
  * reference): the first `nstatic` pairs of the work order (largest first) run
  * on `wg_per_pair` workgroups each, the others one workgroup each from a

@@ -77,6 +77,8 @@ def _sig(L):
     L.lcg_batch_pf_D.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_void_p, c.c_int32]
     L.lcg_batch_bm_probs.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_void_p, c.c_int32]
     L.lcg_batch_pf_grad.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int32]
+    L.lcg_batch_pf_sample.argtypes = [c.c_void_p, c.c_int32, c.c_uint64, c.c_int32, c.c_void_p, c.c_void_p,
+                                      c.c_int32]
     L.lcg_batch_pf_config.argtypes = [c.c_void_p] + [c.POINTER(c.c_int32)] * 3
     L.lcg_batch_pair_info.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_int32), c.POINTER(c.c_int32),
                                       c.POINTER(c.c_int64)]
@@ -281,6 +283,15 @@ class Batch:
         return list(out)[:cnt]
 
     # ---- LocARNA-P
+    def pf_sample(self, k, seed, nsamples):
+        """(mean, variance) of the feature vectors of `nsamples` alignments of
+        pair k drawn from the Boltzmann distribution of the last pf_run."""
+        mean, var = (ctypes.c_double * NTHETA)(), (ctypes.c_double * NTHETA)()
+        rc = lib().lcg_batch_pf_sample(self.h, k, seed, nsamples, mean, var, NTHETA)
+        if rc < 0:
+            raise LcgError("lcg_batch_pf_sample (%d)" % rc)
+        return list(mean), list(var)
+
     def pf_config(self):
         """(workgroups per split pair, split pairs, workgroups) of LocARNA-P"""
         a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <numeric>
 #include <string>
@@ -1710,6 +1711,295 @@ extern "C" int lcg_batch_bm_probs(const lcg_batch *B, int32_t k, int32_t mode, d
         }
     }
     return (h.n + 1) * W;
+}
+
+namespace {
+// ------------------------------------------------------------ sampling
+// Stochastic traceback of LocARNA-P: alignments drawn from the Boltzmann
+// distribution of the last lcg_batch_pf_run.  Host code: the inside
+// matrices of the top level and of every group a sample enters are
+// recomputed from the device's D values with the same tables and the same
+// recurrence as the kernels, then sampled backwards with the affine states
+// (total, s = not ending with an insertion, E = deletion, F = insertion):
+//   E(i,j) = E(i-1,j) gA + (M(i-1,j) - E(i-1,j)) gA o
+//   s(i,j) = M(i-1,j-1) es(i,j) + E(i,j) + ps sum_arcs M(p-1,q-1) D(am)
+//   F(i,j) = F(i,j-1) gB + s(i,j-1) gB o,   M = s + F
+// Row al and column bl of a region hold the initial gap runs.  Global top
+// level only.
+struct SampleRegion {
+    int al = 0, bl = 0, ar = 0, br = 0, w = 0;
+    std::vector<double> M, E, F, S;  // dense (ar - al) x (br - bl + 1), 0 outside
+    std::vector<uint8_t> st;         // stored cell
+    size_t at(int i, int j) const { return (size_t)(i - al) * w + (j - bl); }
+    bool in(int i, int j) const { return i >= al && i < ar && j >= bl && j <= br - 1 && st[at(i, j)]; }
+    double get(const std::vector<double> &X, int i, int j) const { return in(i, j) ? X[at(i, j)] : 0.0; }
+};
+
+struct Sampler {
+    const lcg_batch &B;
+    int k;
+    const PfHost &h;
+    const PfDesc &f;
+    const PairDesc &d;
+    const int32_t *A;        // host arena
+    const double *tab;       // Boltzmann tables of the last run
+    const double *res;       // device results of the last run (Dc)
+    double eo, ps;
+    std::vector<int> c2am;   // candidate entry -> arc match
+    std::map<std::pair<int, int>, int> group_of;
+    std::map<int, SampleRegion> cache;  // region by group (-1: top level)
+    uint64_t rng;
+
+    Sampler(const lcg_batch &B_, int k_, uint64_t seed)
+        : B(B_), k(k_), h(B_.pfh[k_]), f(B_.pdescs[k_]), d(B_.descs[k_]), A(B_.host_arena.data()),
+          tab(B_.pf_arena.data()), res(B_.pf_result.data()), rng(seed) {
+        eo = tab[f.scal + 0];
+        ps = tab[f.scal + 1];
+        c2am.assign(h.nam, -1);
+        for (int am = 0; am < h.nam; ++am) c2am[h.entry_of[am]] = am;
+        for (int g = 0; g < d.ngroups; ++g) {
+            const int32_t *G = A + d.groups + g * LCG_GROUP_INTS;
+            group_of[{G[0] & 0xffff, G[0] >> 16}] = g;
+        }
+    }
+    double uniform() {  // splitmix64, u = (x >> 11) 2^-53
+        uint64_t z = (rng += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    }
+    double gA(int i) const { return tab[f.egA + i]; }
+    double gB(int j) const { return tab[f.egB + j]; }
+    double es(int i, int j) const { return tab[f.es + h.codeA[i] * LCG_NCODES + h.codeB[j]]; }
+    int mn(int i) const { return h.min_col[i]; }
+    int mx(int i) const { return h.max_col[i]; }
+    int cell(int i, int j) const { return A[d.band + 2 * (h.n + 1) + i] + (j - mn(i)); }
+
+    const SampleRegion &region(int g) {
+        auto it = cache.find(g);
+        if (it != cache.end()) return it->second;
+        SampleRegion R;
+        if (g < 0) {
+            R.al = 0, R.bl = 0, R.ar = h.n + 1, R.br = h.m + 1;
+        } else {
+            const int32_t *G = A + d.groups + g * LCG_GROUP_INTS;
+            R.al = G[0] & 0xffff, R.bl = G[0] >> 16, R.ar = G[1] & 0xffff, R.br = G[1] >> 16;
+        }
+        R.w = R.br - R.bl + 1;
+        const size_t cells = (size_t)(R.ar - R.al) * R.w;
+        R.M.assign(cells, 0.0);
+        R.E.assign(cells, 0.0);
+        R.F.assign(cells, 0.0);
+        R.S.assign(cells, 0.0);
+        R.st.assign(cells, 0);
+        const int al = R.al, bl = R.bl, ar = R.ar, br = R.br;
+        // row al: M(al, bl) = 1/ps, M(al, j) = o/ps prod gB
+        const int hi0 = std::min(br - 1, mx(al));
+        R.st[R.at(al, bl)] = 1;
+        R.M[R.at(al, bl)] = R.S[R.at(al, bl)] = 1.0 / ps;
+        double run = eo / ps;
+        for (int j = bl + 1; j <= hi0; ++j) {
+            run *= gB(j);
+            R.st[R.at(al, j)] = 1;
+            R.M[R.at(al, j)] = R.S[R.at(al, j)] = run;
+        }
+        std::vector<double> Ev(br - bl + 1, 0.0);
+        double colv = eo / ps;
+        bool colopen = true;
+        for (int i = al + 1; i < ar; ++i) {
+            if (colopen && mn(i) <= bl) {
+                colv *= gA(i);
+                R.st[R.at(i, bl)] = 1;
+                R.M[R.at(i, bl)] = R.S[R.at(i, bl)] = colv;
+            } else {
+                colopen = false;
+            }
+            const int jlo = std::max(bl + 1, mn(i)), hi = std::min(br - 1, mx(i));
+            double Fprev = 0.0, sprev = colopen ? colv : 0.0;
+            for (int j = jlo; j <= hi; ++j) {
+                const double up = R.get(R.M, i - 1, j), dg = R.get(R.M, i - 1, j - 1);
+                const double ep = Ev[j - bl];
+                const double e = ep * gA(i) + (up - ep) * gA(i) * eo;
+                Ev[j - bl] = e;
+                double sv = dg * es(i, j) + e;
+                const int c = cell(i, j);
+                double sa = 0.0;
+                for (int t = A[d.cptr + c]; t < A[d.cptr + c + 1]; ++t) {
+                    const int p = A[d.cand + 4 * t] & 0xffff, q = A[d.cand + 4 * t] >> 16;
+                    if (p <= al) break;
+                    if (q <= bl) continue;
+                    sa += R.get(R.M, p - 1, q - 1) * res[f.Dc + t];
+                }
+                sv += sa * ps;
+                const double Fv = Fprev * gB(j) + sprev * gB(j) * eo;
+                const size_t x = R.at(i, j);
+                R.st[x] = 1;
+                R.E[x] = e;
+                R.S[x] = sv;
+                R.F[x] = Fv;
+                R.M[x] = sv + Fv;
+                Fprev = Fv;
+                sprev = sv;
+            }
+        }
+        return cache.emplace(g, std::move(R)).first->second;
+    }
+
+    // features of one alignment (score = reference score + theta . f)
+    std::vector<double> feat;
+    int nmatch = 0;
+    void open_run() { feat[273] += 1.0; }
+    void match_edge(int i, int j) {
+        ++nmatch;
+        const int a = h.codeA[i], b = h.codeB[j];
+        if (a < 4 && b < 4) feat[a * 4 + b] += 1.0;
+    }
+
+    // samples region R backwards from (i, j) in the total state
+    void sample(int g, int i, int j) {
+        const SampleRegion *Rp = &region(g);
+        enum { TOT, SS, SNE, EE, FF } state = TOT;  // SNE: s without the deletion term
+        for (;;) {
+            const SampleRegion &R = *Rp;
+            if (state == TOT || state == SS || state == SNE) {
+                if (i == R.al) {  // initial insertion run of row al
+                    if (j > R.bl) open_run();
+                    return;
+                }
+                if (j == R.bl) {  // initial deletion run of column bl
+                    open_run();
+                    return;
+                }
+            }
+            const size_t x = R.at(i, j);
+            if (state == TOT) {
+                state = uniform() * R.M[x] < R.F[x] ? FF : SS;
+                continue;
+            }
+            if (state == FF) {  // insertion of j
+                const double ext = R.get(R.F, i, j - 1) * gB(j);
+                const double tot = R.F[x];
+                j -= 1;
+                if (uniform() * tot < ext) {
+                    state = FF;
+                } else {
+                    open_run();
+                    state = SS;
+                }
+                continue;
+            }
+            if (state == EE) {  // deletion of i
+                const double Ep = R.get(R.E, i - 1, j), Mp = R.get(R.M, i - 1, j);
+                const double ext = Ep * gA(i), opn = std::max(0.0, Mp - Ep) * gA(i) * eo;
+                i -= 1;
+                if (uniform() * (ext + opn) < ext) {
+                    state = EE;
+                } else {
+                    open_run();
+                    // total without the deletion states: F or s without E
+                    const SampleRegion &Q = *Rp;
+                    if (i == Q.al || j == Q.bl) {
+                        state = TOT;
+                    } else {
+                        const size_t y = Q.at(i, j);
+                        const double sne = std::max(0.0, Q.S[y] - Q.E[y]);
+                        state = uniform() * (sne + Q.F[y]) < Q.F[y] ? FF : SNE;
+                    }
+                }
+                continue;
+            }
+            // SS / SNE: match, (deletion), arc matches
+            const double mt = R.get(R.M, i - 1, j - 1) * es(i, j);
+            const double de = state == SS ? R.E[x] : 0.0;
+            double u = uniform() * (state == SS ? R.S[x] : std::max(0.0, R.S[x] - R.E[x]));
+            if (u < mt) {
+                match_edge(i, j);
+                i -= 1;
+                j -= 1;
+                state = TOT;
+                continue;
+            }
+            u -= mt;
+            if (u < de) {
+                state = EE;
+                continue;
+            }
+            u -= de;
+            const int c = cell(i, j);
+            int pick = -1, last = -1;
+            for (int t = A[d.cptr + c]; t < A[d.cptr + c + 1]; ++t) {
+                const int p = A[d.cand + 4 * t] & 0xffff, q = A[d.cand + 4 * t] >> 16;
+                if (p <= R.al) break;
+                if (q <= R.bl) continue;
+                const double v = R.get(R.M, p - 1, q - 1) * res[f.Dc + t] * ps;
+                if (v <= 0.0) continue;
+                last = t;
+                if (u < v) {
+                    pick = t;
+                    break;
+                }
+                u -= v;
+            }
+            if (pick < 0) pick = last;  // rounding at the end of the list
+            if (pick < 0) {  // no arc term: rounding in the choice between match and deletion
+                if (mt > 0.0) {
+                    match_edge(i, j);
+                    i -= 1;
+                    j -= 1;
+                    state = TOT;
+                } else {
+                    state = EE;
+                }
+                continue;
+            }
+            const int am = c2am[pick];
+            const int p = A[d.cand + 4 * pick] & 0xffff, q = A[d.cand + 4 * pick] >> 16;
+            nmatch += 2;
+            if (h.am_type[am] >= 0) feat[16 + h.am_type[am]] += h.tau / 100.0;
+            feat[274] += h.am_f[am];
+            // inside of the arc match: its group from (i-1, j-1) back to (p, q)
+            sample(group_of.at({p, q}), i - 1, j - 1);
+            Rp = &region(g);
+            i = p - 1;
+            j = q - 1;
+            state = TOT;
+        }
+    }
+
+    std::vector<double> one() {
+        feat.assign(LCG_NTHETA, 0.0);
+        nmatch = 0;
+        sample(-1, h.n, h.m);
+        feat[272] = h.n + h.m - 2.0 * nmatch;
+        return feat;
+    }
+};
+}  // namespace
+
+extern "C" int lcg_batch_pf_sample(const lcg_batch *B, int32_t k, uint64_t seed, int32_t nsamples, double *mean,
+                                   double *var, int32_t cap) {
+    if (!B || k < 0 || k >= B->npairs || nsamples < 1 || !mean || cap < LCG_NTHETA) return LCG_E_ARG;
+    if (!B->pf_have) return LCG_E_STATE;
+    if (B->bp.local) return LCG_E_ARG;  // the sampler follows the global top level
+    try {
+        Sampler S(*B, k, seed);
+        std::vector<double> sum(LCG_NTHETA, 0.0), sum2(LCG_NTHETA, 0.0);
+        for (int r = 0; r < nsamples; ++r) {
+            const std::vector<double> x = S.one();
+            for (int t = 0; t < LCG_NTHETA; ++t) {
+                sum[t] += x[t];
+                sum2[t] += x[t] * x[t];
+            }
+        }
+        for (int t = 0; t < LCG_NTHETA; ++t) {
+            mean[t] = sum[t] / nsamples;
+            if (var) var[t] = std::max(0.0, sum2[t] / nsamples - mean[t] * mean[t]);
+        }
+        return LCG_NTHETA;
+    } catch (std::exception &) {
+        return LCG_E_STATE;
+    }
 }
 
 extern "C" int lcg_batch_pf_grad(const lcg_batch *B, int32_t k, double *grad, int32_t cap) {

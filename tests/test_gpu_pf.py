@@ -126,3 +126,49 @@ def test_pf_split_pairs_match_single(monkeypatch):
         _compare(mixed, k, o, ("split", k))
     mixed.destroy()
     single.destroy()
+
+
+def _check_samples(B, k, T, S, label):
+    mean, var = B.pf_sample(k, 1234 + k, S)
+    g = B.pf_grad(k)
+    worst = 0.0
+    for t in range(api.NTHETA):
+        expect = T * g[t]
+        # sample variance, floored by the Poisson variance of a count with
+        # this mean (rare features may not occur in S samples)
+        sd = math.sqrt(max(var[t], abs(expect)) / S)
+        dev = abs(mean[t] - expect)
+        # 5 standard errors (275 components) plus rounding
+        assert dev <= 5 * sd + 1e-7 * max(1.0, abs(expect)), (label, k, t, mean[t], expect, sd)
+        if sd > 0:
+            worst = max(worst, dev / sd)
+    return worst
+
+
+@pytest.mark.parametrize("variant", ["default", "tau100"])
+def test_sampled_alignments_match_gradient(variant):
+    # the mean features of sampled alignments estimate T dlogZ/dtheta (the
+    # adjoint's gradient, itself checked against finite differences)
+    kw = dict(min_prob=0.01, max_bp_span=-1, min_trace_prob=0.0)
+    if variant == "tau100":
+        kw.update(tau=100)
+    pairs = [tiny_pair(7300 + s) for s in range(8)]
+    pa = api.locarna_p_params(**kw)
+    P = [api.Pair(a, b, pa) for a, b in pairs]
+    B = api.Batch(P)
+    B.pf_run()
+    for k in range(len(P)):
+        _check_samples(B, k, pa.temperature_alipf, 20000, variant)
+    B.destroy()
+
+
+def test_sampled_alignments_workload():
+    pps, pairs, span = synth.workload("pf16x150", limit_pairs=3)
+    pa = api.locarna_p_params(max_bp_span=span)
+    P = [api.Pair(pps[a], pps[b], pa) for a, b in pairs]
+    B = api.Batch(P)
+    B.pf_run()
+    for k in range(len(P)):
+        w = _check_samples(B, k, pa.temperature_alipf, 3000, "pf16x150")
+        print("pair", k, "largest deviation / standard error", round(w, 2), flush=True)
+    B.destroy()
