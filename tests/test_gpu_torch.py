@@ -1,12 +1,14 @@
 """The torch.autograd binding of d log Z / d theta: backward against central
 finite differences of the GPU's own log Z along random directions, and the
-fitting loop (expected statistics of a generating theta) decreasing its loss
-and its distance to the generating theta."""
+fitting loops (expected statistics of a generating theta, and alignments
+sampled at it) decreasing their loss and their distance to the generating
+theta."""
 import pytest
 import torch
 
 from locarna_amd import api, synth
-from locarna_amd.torch_grad import LogZ, fit_expected_statistics, generating_theta
+from locarna_amd.torch_grad import (LogZ, fit_expected_statistics, fit_sampled_alignments, generating_theta,
+                                    sampled_statistics)
 
 pytestmark = pytest.mark.gpu
 
@@ -41,6 +43,21 @@ def test_fitting_moves_towards_generating_theta():
     B = _batch(4)
     theta_gen = generating_theta(seed=6, scale=30.0)
     losses, dist, _ = fit_expected_statistics(B, theta_gen, steps=6)
+    print("losses", losses, "dist", dist, flush=True)
+    assert losses[-1] < losses[0]
+    assert dist[-1] < dist[0]
+    B.destroy()
+
+
+def test_fit_to_sampled_alignments():
+    B = _batch(4)
+    theta_gen = generating_theta(seed=6, scale=30.0)
+    # seeded: the same seed gives the same target, another seed another one
+    t1 = sampled_statistics(B, theta_gen, samples=32, seed=3)
+    t2 = sampled_statistics(B, theta_gen, samples=32, seed=3)
+    t3 = sampled_statistics(B, theta_gen, samples=32, seed=4)
+    assert torch.equal(t1, t2) and not torch.equal(t1, t3)
+    losses, dist, _ = fit_sampled_alignments(B, theta_gen, samples=256, seed=6, steps=6)
     print("losses", losses, "dist", dist, flush=True)
     assert losses[-1] < losses[0]
     assert dist[-1] < dist[0]
