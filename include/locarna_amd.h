@@ -94,6 +94,48 @@ int lcg_pair_prepare_device(const char *ppA, const char *ppB, const lcg_params *
  * pairs may be freed after the call. */
 int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t device, lcg_batch **out, char *err,
                      int errlen);
+
+/* Launch shape of a batch's kernels (no counterpart in the reference, which
+ * runs on the CPU).  A field of 0 takes the library's default; the
+ * environment variables LCG_WAVES, LCG_WGS_PER_CU, LCG_WG_PER_PAIR,
+ * LCG_TOP_WAVES, LCG_TOP_WGS_PER_CU, LCG_PF_WAVES and LCG_PF_WG_PER_PAIR
+ * override the defaults, and nonzero fields override both.  Results do not
+ * depend on the shape (integer results bit for bit). */
+typedef struct {
+    int32_t waves;          /* fill of D: waves per workgroup, 1..16 (default 16) */
+    int32_t wgs_per_cu;     /* fill: workgroups per CU, 1..4, with at least as many pairs as CUs (1) */
+    int32_t wg_per_pair;    /* fill: workgroups per pair, 1..64, with fewer pairs than CUs (CUs / pairs) */
+    int32_t top_waves;      /* top level + traceback: waves per workgroup, 1..16 (4) */
+    int32_t top_wgs_per_cu; /* top level + traceback: workgroups per CU, 1..4 (4) */
+    int32_t pf_waves;       /* LocARNA-P: waves per workgroup, 1..16 (16) */
+    int32_t pf_wg_per_pair; /* LocARNA-P: workgroups per pair, 1..64; 0: two for the largest pairs where
+                               that shortens the estimated makespan, else CUs / pairs */
+    int32_t reserved;
+} lcg_shape;
+/* lcg_batch_create with a requested launch shape (NULL: defaults). */
+int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, int32_t device, const lcg_shape *shape,
+                            lcg_batch **out, char *err, int errlen);
+/* Effective launch shape (shapes whose regions do not fit the LDS are
+ * reduced to fewer workgroups per CU). */
+int lcg_batch_get_shape(const lcg_batch *batch, lcg_shape *shape);
+/* Applies another launch shape to a batch: recomputes the LDS schedule of
+ * the fill when its shape changes (on the host, or by the device
+ * compaction) and invalidates the results of earlier runs.  On an error the
+ * batch keeps its previous shape. */
+int lcg_batch_reshape(lcg_batch *batch, const lcg_shape *shape, char *err, int errlen);
+/* Autotuner: times candidate launch shapes on this batch (minimum of `reps`
+ * runs after one warm-up run each) and leaves the batch at the fastest.
+ * path 0: integer path (score + traceback); fill shapes (waves, workgroups
+ * per CU or per pair) timed by the fill kernel, then top-level shapes with
+ * the best fill, the winner kept only if its graph replay beats the
+ * starting shape's; every candidate's scores, alignments and structures
+ * must equal the starting shape's, else LCG_E_STATE.  path 1: LocARNA-P
+ * (waves, workgroups per pair), every candidate's partition functions within
+ * 1e-12 relative of the starting shape's.  best: the chosen shape;
+ * best_ms / default_ms: device time of one run at the chosen and at the
+ * starting shape. */
+int lcg_batch_autotune(lcg_batch *batch, int32_t path, int32_t reps, lcg_shape *best, float *best_ms,
+                       float *default_ms, char *err, int errlen);
 /* Arc matches and groups of pair k of a batch (ArcMatches::num_arc_matches;
  * the groups of AlignerImpl::align_D, aligner.cc:659-732) and the size of
  * its device image in ints. */

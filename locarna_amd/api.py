@@ -42,6 +42,15 @@ def locarna_p_params(**kw):
     return locarna_params(**d)
 
 
+class Shape(ctypes.Structure):
+    """lcg_shape (include/locarna_amd.h): launch shape, 0 = default."""
+    _fields_ = [(n, ctypes.c_int32) for n in ("waves", "wgs_per_cu", "wg_per_pair", "top_waves",
+                                              "top_wgs_per_cu", "pf_waves", "pf_wg_per_pair", "reserved")]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
+
+
 class LcgError(RuntimeError):
     pass
 
@@ -84,6 +93,12 @@ def _sig(L):
                                       c.POINTER(c.c_int64)]
     L.lcg_batch_segment.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int64]
     L.lcg_batch_compact_run.argtypes = [c.c_void_p, c.POINTER(c.c_float), c.POINTER(c.c_float)]
+    L.lcg_batch_create_shaped.argtypes = [c.POINTER(c.c_void_p), c.c_int32, c.c_int32, c.POINTER(Shape),
+                                          c.POINTER(c.c_void_p), c.c_char_p, c.c_int]
+    L.lcg_batch_get_shape.argtypes = [c.c_void_p, c.POINTER(Shape)]
+    L.lcg_batch_reshape.argtypes = [c.c_void_p, c.POINTER(Shape), c.c_char_p, c.c_int]
+    L.lcg_batch_autotune.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.POINTER(Shape), c.POINTER(c.c_float),
+                                     c.POINTER(c.c_float), c.c_char_p, c.c_int]
     return L
 
 
@@ -185,19 +200,53 @@ class Pair:
 class Batch:
     """Device batch (lcg_batch_create / lcg_batch_run)."""
 
-    def __init__(self, pairs, device=0):
+    def __init__(self, pairs, device=0, shape=None):
+        """shape: dict of lcg_shape fields (missing or 0: default)."""
         L = lib()
         self.pairs = pairs
         arr = (ctypes.c_void_p * len(pairs))(*[p.h.value for p in pairs])
         h = ctypes.c_void_p()
         err = ctypes.create_string_buffer(512)
-        rc = L.lcg_batch_create(arr, len(pairs), device, ctypes.byref(h), err, 512)
+        if shape is None:
+            rc = L.lcg_batch_create(arr, len(pairs), device, ctypes.byref(h), err, 512)
+        else:
+            rc = L.lcg_batch_create_shaped(arr, len(pairs), device, ctypes.byref(Shape(**shape)), ctypes.byref(h),
+                                           err, 512)
         if rc != 0:
             raise LcgError("lcg_batch_create (%d): %s" % (rc, err.value.decode()))
         self.h = h
+        self._config()
+
+    def _config(self):
         w, g, lds = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        L.lcg_batch_config(h, ctypes.byref(w), ctypes.byref(g), ctypes.byref(lds))
+        lib().lcg_batch_config(self.h, ctypes.byref(w), ctypes.byref(g), ctypes.byref(lds))
         self.waves, self.workgroups, self.lds_bytes = w.value, g.value, lds.value
+
+    def shape(self):
+        """Effective launch shape (dict)."""
+        s = Shape()
+        rc = lib().lcg_batch_get_shape(self.h, ctypes.byref(s))
+        if rc != 0:
+            raise LcgError("lcg_batch_get_shape (%d)" % rc)
+        return s.as_dict()
+
+    def reshape(self, **shape):
+        err = ctypes.create_string_buffer(512)
+        rc = lib().lcg_batch_reshape(self.h, ctypes.byref(Shape(**shape)), err, 512)
+        if rc != 0:
+            raise LcgError("lcg_batch_reshape (%d): %s" % (rc, err.value.decode()))
+        self._config()
+
+    def autotune(self, path=0, reps=3):
+        """Times candidate launch shapes (path 0: integer, 1: LocARNA-P) and
+        keeps the fastest; returns (shape, best_ms, default_ms)."""
+        s, b, d = Shape(), ctypes.c_float(), ctypes.c_float()
+        err = ctypes.create_string_buffer(512)
+        rc = lib().lcg_batch_autotune(self.h, path, reps, ctypes.byref(s), ctypes.byref(b), ctypes.byref(d), err, 512)
+        if rc != 0:
+            raise LcgError("lcg_batch_autotune (%d): %s" % (rc, err.value.decode()))
+        self._config()
+        return s.as_dict(), b.value, d.value
 
     def pair_info(self, k):
         """(arc matches, groups, image ints) of pair k."""

@@ -4,8 +4,10 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <memory>
 #include <numeric>
@@ -602,6 +604,11 @@ struct lcg_batch {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     int graph_trace = -1;
+    // launch shape: requested fields (0 = default), the effective shape and
+    // what choosing one needs (CU count, LDS header ints, largest region)
+    lcg_shape req{};
+    lcg_shape eff{};
+    int cus = 1, hdr = 0, gmax = 4, top_max = 4, max_len = 1;
     BatchParams bp;      // fill-D kernel
     int waves = 1, grid = 1;
     size_t lds_bytes = 0;
@@ -923,6 +930,10 @@ int device_build(lcg_batch &B, int npairs, int max_len, int waves, int pool, int
         }
     }
     hipError_t e;
+    if (B.d_ws2) {  // a reshape with another number of workgroups per pair
+        hipFree(B.d_ws2);
+        B.d_ws2 = nullptr;
+    }
     if ((e = hipMalloc(&B.d_ws2, std::max<size_t>(ws2, 4) * 4)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMemcpy(B.d_cdescs, B.cdescs.data(), sizeof(CompactDesc) * npairs, hipMemcpyHostToDevice)) !=
         hipSuccess)
@@ -957,13 +968,107 @@ int device_build(lcg_batch &B, int npairs, int max_len, int waves, int pool, int
 }
 }  // namespace
 
+namespace {
+// A requested field (> 0) wins over the environment variable, which wins
+// over the default.
+int pick(int req, const char *env, int dflt, int lo, int hi) {
+    if (req > 0) return std::min(hi, std::max(lo, req));
+    if (const char *w = std::getenv(env)) return std::min(hi, std::max(lo, std::atoi(w)));
+    return dflt;
+}
+
+constexpr int kLdsIntsCu = (160 * 1024 - 512) / 4;  // static LDS of the kernels included
+
+struct FillShape {
+    int waves, wgs_per_cu, K, pool;
+};
+struct TopShape {
+    int waves, wgs_per_cu, pool;
+};
+
+// Fill of D: one workgroup of 16 waves per CU with the whole LDS as its
+// pool by default: a larger pool packs more groups of a row into one round,
+// which measured faster than two workgroups of 8 waves with half the pool
+// each (allpairs64: 108 ms against 173 ms per step).  With fewer pairs than
+// CUs, K workgroups share a pair (all co-resident).
+int choose_fill(const lcg_batch &B, FillShape &s, char *err, int errlen) {
+    s.waves = pick(B.req.waves, "LCG_WAVES", 16, 1, 16);
+    s.wgs_per_cu = pick(B.req.wgs_per_cu, "LCG_WGS_PER_CU", 1, 1, 4);
+    s.pool = kLdsIntsCu / s.wgs_per_cu - B.hdr;
+    while (s.pool < B.gmax && s.wgs_per_cu > 1) {
+        --s.wgs_per_cu;
+        s.pool = kLdsIntsCu / s.wgs_per_cu - B.hdr;
+    }
+    s.pool &= ~3;
+    if (s.pool < B.gmax) {
+        set_err(err, errlen, "an arc-match group region exceeds the LDS of one workgroup");
+        return LCG_E_CAPACITY;
+    }
+    const int np = B.npairs;
+    s.K = pick(B.req.wg_per_pair, "LCG_WG_PER_PAIR", np < B.cus ? std::min(64, B.cus / np) : 1, 1, 64);
+    s.K = std::min(s.K, std::max(1, B.cus / np));  // all workgroups of the fill stay co-resident
+    return LCG_OK;
+}
+
+// The top-level / traceback kernel runs the top level on one wave: four
+// workgroups of 4 waves per CU (a quarter of the LDS each) by default keep
+// more pairs in flight, as long as every group region still fits the
+// smaller pool (allpairs64: 3.36 ms against 4.57 ms with 2 x 8 and 7.42 ms
+// with 1 x 16).
+TopShape choose_top(const lcg_batch &B) {
+    TopShape t;
+    t.waves = pick(B.req.top_waves, "LCG_TOP_WAVES", 4, 1, 16);
+    t.wgs_per_cu = pick(B.req.top_wgs_per_cu, "LCG_TOP_WGS_PER_CU", 4, 1, 4);
+    t.pool = (kLdsIntsCu / t.wgs_per_cu - B.hdr) & ~3;
+    while (t.pool < B.gmax && t.wgs_per_cu > 1) {
+        --t.wgs_per_cu;
+        t.pool = (kLdsIntsCu / t.wgs_per_cu - B.hdr) & ~3;
+    }
+    return t;
+}
+
+// LDS schedule of every pair of a host-packed batch into its host image.
+bool schedule_all(lcg_batch &B, const FillShape &s) {
+    for (int k = 0; k < B.npairs; ++k)
+        if (!schedule_groups(B.host_arena.data(), B.descs[k], s.waves, s.pool, s.K)) return false;
+    return true;
+}
+
+// Launch parameters of both kernels for the chosen shapes.
+void apply_shape(lcg_batch &B, const FillShape &s, const TopShape &t) {
+    const int np = B.npairs;
+    B.wg_per_pair = s.K;
+    B.waves = s.waves;
+    B.lds_bytes = (size_t)(B.hdr + s.pool) * 4;
+    B.bp.pool_ints = s.pool;
+    B.bp.wg_per_pair = s.K;
+    B.grid = s.K > 1 ? np * s.K : std::min(np, B.cus * s.wgs_per_cu);
+    B.bp_top = B.bp;
+    B.bp_top.pool_ints = t.pool;
+    B.top_waves = t.waves;
+    B.top_lds_bytes = (size_t)(B.hdr + t.pool) * 4;
+    B.top_grid = std::min(np, B.cus * t.wgs_per_cu);
+    B.eff.waves = s.waves;
+    B.eff.wgs_per_cu = s.wgs_per_cu;
+    B.eff.wg_per_pair = s.K;
+    B.eff.top_waves = t.waves;
+    B.eff.top_wgs_per_cu = t.wgs_per_cu;
+}
+}  // namespace
+
 extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t device, lcg_batch **out, char *err,
                                 int errlen) {
+    return lcg_batch_create_shaped(pairs, npairs, device, nullptr, out, err, errlen);
+}
+
+extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, int32_t device, const lcg_shape *shape,
+                                       lcg_batch **out, char *err, int errlen) {
     if (!pairs || npairs <= 0 || !out) return LCG_E_ARG;
     *out = nullptr;
     auto B = std::make_unique<lcg_batch>();
     B->device = device;
     B->npairs = npairs;
+    if (shape) B->req = *shape;
     const lcg_params &p0 = pairs[0]->params;
     const bool deferred = pairs[0]->deferred;
     for (int k = 0; k < npairs; ++k) {
@@ -1026,84 +1131,40 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     std::iota(B->order.begin(), B->order.end(), 0);
     std::stable_sort(B->order.begin(), B->order.end(), [&](int a, int b) { return work[a] > work[b]; });
 
-    // Launch shape: LDS = header (per-pair band / codes / gaps / sigma of the
-    // longest sequence) + pool for the group regions of one row.  One
-    // workgroup of 16 waves per CU with the whole LDS as its pool: a larger
-    // pool packs more groups of a row into one round, which measured faster
-    // than two workgroups of 8 waves with half the pool each (allpairs64:
-    // 108 ms against 173 ms per step).
-    // header: band, codes, gaps, sigma (7 L + 72 ints), fill-list row
-    // pointers (L + 2) and blocks (2 per block)
+    // LDS header: band, codes, gaps, sigma (7 L + 72 ints) of the longest
+    // sequence, fill-list row pointers (L + 2) and blocks (2 per block); the
+    // rest of a workgroup's LDS is the pool for the group regions of a row.
+    // Per-row LDS schedule: groups of a row, largest first, into rounds of at
+    // most `waves` regions that fit the pool together; with K workgroups per
+    // pair the groups of a row are first dealt to the K members, longest
+    // first to the least loaded member.
     const int fptr_lds = 7 * max_len + 72;
     const int fblk_lds = fptr_lds + max_len + 2;
-    const int hdr = (fblk_lds + 2 * nfblk_max + 3) & ~3;
-    const int lds_ints_cu = (160 * 1024 - 512) / 4;  // static LDS of the kernel included
-    int waves = 16, wgs_per_cu = 1;
-    if (const char *w = std::getenv("LCG_WAVES")) waves = std::min(16, std::max(1, std::atoi(w)));
-    if (const char *w = std::getenv("LCG_WGS_PER_CU")) wgs_per_cu = std::min(4, std::max(1, std::atoi(w)));
-    int pool = lds_ints_cu / wgs_per_cu - hdr;
-    while (pool < gmax && wgs_per_cu > 1) {
-        --wgs_per_cu;
-        pool = lds_ints_cu / wgs_per_cu - hdr;
-    }
-    pool &= ~3;
-    if (pool < gmax) {
-        set_err(err, errlen, "an arc-match group region exceeds the LDS of one workgroup");
+    B->hdr = (fblk_lds + 2 * nfblk_max + 3) & ~3;
+    B->cus = cus;
+    B->gmax = gmax;
+    B->top_max = top_max;
+    B->max_len = max_len;
+    FillShape fs;
+    if (int rc = choose_fill(*B, fs, err, errlen)) return rc;
+    const TopShape ts = choose_top(*B);
+    if (!deferred && !schedule_all(*B, fs)) {
+        set_err(err, errlen, "too many LDS rounds in one row");
         return LCG_E_CAPACITY;
     }
-    // per-row LDS schedule: groups of a row, largest first, into rounds of at
-    // most `waves` regions that fit the pool together.  With K workgroups
-    // per pair (fewer pairs than CUs), the groups of a row are first dealt to
-    // the K members, longest first to the least loaded member.
-    int K = 1;
-    if (npairs < cus) K = std::min(64, cus / npairs);
-    if (const char *w = std::getenv("LCG_WG_PER_PAIR")) K = std::min(64, std::max(1, std::atoi(w)));
-    K = std::min(K, std::max(1, cus / npairs));  // all workgroups of the fill stay co-resident
-    if (!deferred) {
-        for (int k = 0; k < npairs; ++k) {
-            if (!schedule_groups(B->host_arena.data(), B->descs[k], waves, pool, K)) {
-                set_err(err, errlen, "too many LDS rounds in one row");
-                return LCG_E_CAPACITY;
-            }
-        }
-    }
-    B->wg_per_pair = K;
-    B->waves = waves;
-    B->lds_bytes = (size_t)(hdr + pool) * 4;
     B->bp.alpha = p0.indel_opening;
     B->bp.local = p0.sequ_local;
     B->bp.trace = 1;
     B->bp.npairs = npairs;
-    B->bp.hdr_ints = hdr;
+    B->bp.hdr_ints = B->hdr;
     B->bp.fptr_lds = fptr_lds;
     B->bp.fblk_lds = fblk_lds;
-    B->bp.pool_ints = pool;
     B->bp.max_len = max_len;
     B->bp.count = 0;
     B->bp.scratch_stride = top_max;
-    B->bp.wg_per_pair = K;
     B->bp.fixed_cols = 1;
     if (const char *f = std::getenv("LCG_FIXED_COLS")) B->bp.fixed_cols = std::atoi(f) != 0;
-    // The top-level / traceback kernel runs the top level on one wave: four
-    // workgroups of 4 waves per CU (a quarter of the LDS each) keep more
-    // pairs in flight, as long as every group region still fits the smaller
-    // pool (allpairs64: 3.36 ms against 4.57 ms with 2 x 8 and 7.42 ms with
-    // 1 x 16).
-    int top_waves = 4, top_wgs = 4;
-    if (const char *w = std::getenv("LCG_TOP_WAVES")) top_waves = std::min(16, std::max(1, std::atoi(w)));
-    if (const char *w = std::getenv("LCG_TOP_WGS_PER_CU")) top_wgs = std::min(4, std::max(1, std::atoi(w)));
-    int top_pool = (lds_ints_cu / top_wgs - hdr) & ~3;
-    while (top_pool < gmax && top_wgs > 1) {
-        --top_wgs;
-        top_pool = (lds_ints_cu / top_wgs - hdr) & ~3;
-    }
-    B->bp_top = B->bp;
-    B->bp_top.pool_ints = top_pool;
-    B->top_waves = top_waves;
-    B->top_lds_bytes = (size_t)(hdr + top_pool) * 4;
-
-    B->grid = K > 1 ? npairs * K : std::min(npairs, cus * wgs_per_cu);
-    B->top_grid = std::min(npairs, cus * top_wgs);
+    apply_shape(*B, fs, ts);
     if ((e = hipMalloc(&B->d_arena, B->arena_ints * 4)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_descs, sizeof(PairDesc) * npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_order, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
@@ -1132,7 +1193,7 @@ extern "C" int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t 
     if ((e = hipMemcpy(B->d_resoff, B->res_off.data(), 4 * (size_t)npairs, hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, err, errlen);
     if (deferred) {
-        if (int rc = device_build(*B, npairs, max_len, waves, pool, K, err, errlen)) return rc;
+        if (int rc = device_build(*B, npairs, max_len, fs.waves, fs.pool, fs.K, err, errlen)) return rc;
     } else {
         capture_pf_host(*B, pairs, npairs);
     }
@@ -1437,8 +1498,7 @@ int pf_setup(lcg_batch &B) {
         // groups are no larger than the top level region
     }
     B.pf_arena.assign(off, 0.0);
-    B.pf_waves = 16;
-    if (const char *w = std::getenv("LCG_PF_WAVES")) B.pf_waves = std::min(16, std::max(1, std::atoi(w)));
+    B.pf_waves = pick(B.req.pf_waves, "LCG_PF_WAVES", 16, 1, 16);
     int nfblk_max = 1;
     for (int k = 0; k < np; ++k) nfblk_max = std::max(nfblk_max, B.descs[k].nfblk);
     B.pbp.npairs = np;
@@ -1458,8 +1518,8 @@ int pf_setup(lcg_batch &B) {
     int cus = 0;
     HIPCHECK(hipSetDevice(B.device));
     HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, B.device));
-    int K = np < cus ? std::min(64, cus / np) : 1;
-    if (const char *w = std::getenv("LCG_PF_WG_PER_PAIR")) K = std::min(64, std::max(1, std::atoi(w)));
+    const bool forced = B.req.pf_wg_per_pair > 0 || std::getenv("LCG_PF_WG_PER_PAIR");
+    int K = pick(B.req.pf_wg_per_pair, "LCG_PF_WG_PER_PAIR", np < cus ? std::min(64, cus / np) : 1, 1, 64);
     K = std::min(K, std::max(1, cus / np));
     // the first nst pairs of the work order get K workgroups each, statically;
     // the others are pulled from the queue by the remaining workgroups.  With
@@ -1468,7 +1528,7 @@ int pf_setup(lcg_batch &B) {
     // on two workgroups measured about 1.75x faster; the rest by longest
     // processing time first over the other workgroups).
     int nst = K > 1 ? np : 0;
-    if (K == 1 && cus >= 4 && !std::getenv("LCG_PF_WG_PER_PAIR")) {
+    if (K == 1 && cus >= 4 && !forced) {
         auto makespan = [&](int ns) {
             double t = 0;
             for (int q = 0; q < ns; ++q) t = std::max(t, (double)B.work[B.order[q]] / 1.75);
@@ -1579,6 +1639,8 @@ int pf_setup(lcg_batch &B) {
     hipError_t e;
     // all workgroups resident at once (one per CU): static members + the rest
     B.pf_grid = nst == np ? np * K : (nst > 0 ? cus : std::min(np, cus));
+    B.eff.pf_waves = B.pf_waves;
+    B.eff.pf_wg_per_pair = forced ? K : 0;  // 0: chosen per pair by the makespan rule above
     HIPCHECK(hipMalloc(&B.d_pfa, std::max<size_t>(1, off) * 8));
     HIPCHECK(hipMalloc(&B.d_pdescs, sizeof(PfDesc) * np));
     HIPCHECK(hipMalloc(&B.d_pfscratch, (size_t)B.pf_grid * B.pf_waves * B.pbp.scratch_stride * 8));
@@ -2033,4 +2095,299 @@ extern "C" int lcg_batch_pf_grad(const lcg_batch *B, int32_t k, double *grad, in
     grad[273] = (B->pf_result[f.out + 1] / Z) / T;
     grad[274] = dsw / T;
     return LCG_NTHETA;
+}
+
+namespace {
+// Fingerprint of the integer results of the last run: score, alignment
+// edges and structure strings of every pair (FNV-1a over the words).
+int result_digest(const lcg_batch *B, uint64_t &h) {
+    h = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) {
+        h ^= v;
+        h *= 1099511628211ull;
+    };
+    std::vector<int32_t> a, b;
+    std::vector<char> sa, sb;
+    for (int k = 0; k < B->npairs; ++k) {
+        int64_t sc = 0;
+        if (int rc = lcg_batch_score(B, k, &sc)) return rc;
+        mix((uint64_t)sc);
+        const PairDesc &d = B->descs[k];
+        const int cap = d.n + d.m + 2;
+        a.assign(cap, 0);
+        b.assign(cap, 0);
+        sa.assign(cap, 0);
+        sb.assign(cap, 0);
+        const int cnt = lcg_batch_alignment(B, k, a.data(), b.data(), cap, sa.data(), sb.data(), cap);
+        if (cnt < 0) return cnt;
+        for (int t = 0; t < cnt; ++t) mix(((uint64_t)(uint32_t)a[t] << 32) | (uint32_t)b[t]);
+        for (int t = 0; t < d.n; ++t) mix((uint64_t)sa[t]);
+        for (int t = 0; t < d.m; ++t) mix((uint64_t)sb[t]);
+    }
+    return LCG_OK;
+}
+
+void drop_pf_state(lcg_batch &B) {
+    if (B.d_pfa) hipFree(B.d_pfa);
+    if (B.d_pdescs) hipFree(B.d_pdescs);
+    if (B.d_pfscratch) hipFree(B.d_pfscratch);
+    if (B.d_pfsched) hipFree(B.d_pfsched);
+    B.d_pfa = nullptr;
+    B.d_pdescs = nullptr;
+    B.d_pfscratch = nullptr;
+    B.d_pfsched = nullptr;
+    B.pf_have = false;
+}
+
+std::string shape_text(const lcg_shape &s) {
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "waves %d, wgs/CU %d, wgs/pair %d, top waves %d, top wgs/CU %d, pf waves %d, pf wgs/pair %d",
+                  s.waves, s.wgs_per_cu, s.wg_per_pair, s.top_waves, s.top_wgs_per_cu, s.pf_waves, s.pf_wg_per_pair);
+    return buf;
+}
+}  // namespace
+
+extern "C" int lcg_batch_get_shape(const lcg_batch *B, lcg_shape *shape) {
+    if (!B || !shape) return LCG_E_ARG;
+    *shape = B->eff;
+    if (!B->d_pfa) {  // LocARNA-P not set up yet: what it would take
+        shape->pf_waves = pick(B->req.pf_waves, "LCG_PF_WAVES", 16, 1, 16);
+        shape->pf_wg_per_pair = B->req.pf_wg_per_pair;
+    }
+    return LCG_OK;
+}
+
+extern "C" int lcg_batch_reshape(lcg_batch *B, const lcg_shape *shape, char *err, int errlen) {
+    if (!B || !shape) return LCG_E_ARG;
+    HIPCHECK(hipSetDevice(B->device));
+    HIPCHECK(hipStreamSynchronize(B->stream));
+    const lcg_shape old = B->req;
+    FillShape fs0;
+    if (int rc = choose_fill(*B, fs0, err, errlen)) return rc;
+    B->req = *shape;
+    FillShape fs;
+    if (int rc = choose_fill(*B, fs, err, errlen)) {
+        B->req = old;
+        return rc;
+    }
+    const TopShape ts = choose_top(*B);
+    if (fs.waves != fs0.waves || fs.pool != fs0.pool || fs.K != fs0.K) {
+        if (B->device_built) {
+            // as at creation: both compaction passes (the build checks the
+            // counts of the count pass), the image over a cleared arena
+            auto rebuild = [&](const FillShape &f, char *e, int elen) -> int {
+                HIPCHECK(hipMemsetAsync(B->d_arena, 0, B->arena_ints * 4, B->stream));
+                if (int rc = lcg_launch_compact_count(B->d_cdescs, B->d_cin, B->d_ws, B->d_cstats, B->npairs,
+                                                      B->comp_lds, B->stream))
+                    return hip_fail((hipError_t)rc, e, elen);
+                return device_build(*B, B->npairs, B->max_len, f.waves, f.pool, f.K, e, elen);
+            };
+            if (int rc = rebuild(fs, err, errlen)) {
+                // back to the shape that was built before
+                B->req = old;
+                char e2[64];
+                if (rebuild(fs0, e2, sizeof e2)) return LCG_E_STATE;
+                return rc;
+            }
+        } else {
+            if (!schedule_all(*B, fs)) {
+                B->req = old;
+                schedule_all(*B, fs0);  // succeeded before
+                set_err(err, errlen, "too many LDS rounds in one row");
+                return LCG_E_CAPACITY;
+            }
+            HIPCHECK(hipMemcpy(B->d_arena, B->host_arena.data(), B->arena_ints * 4, hipMemcpyHostToDevice));
+        }
+    }
+    const size_t old_scratch = (size_t)B->top_grid * B->top_waves * B->top_max;
+    apply_shape(*B, fs, ts);
+    const size_t scratch_ints = (size_t)B->top_grid * B->top_waves * B->top_max;
+    if (scratch_ints > old_scratch) {
+        HIPCHECK(hipFree(B->d_scratch));
+        B->d_scratch = nullptr;
+        HIPCHECK(hipMalloc(&B->d_scratch, scratch_ints * 4));
+    }
+    if (B->graph_exec) hipGraphExecDestroy(B->graph_exec);
+    if (B->graph) hipGraphDestroy(B->graph);
+    B->graph_exec = nullptr;
+    B->graph = nullptr;
+    B->graph_trace = -1;
+    B->have_results = false;
+    B->fetched = false;
+    if (B->d_pfa && (shape->pf_waves != old.pf_waves || shape->pf_wg_per_pair != old.pf_wg_per_pair))
+        drop_pf_state(*B);  // set up again with the new shape by the next LocARNA-P run
+    return LCG_OK;
+}
+
+extern "C" int lcg_batch_autotune(lcg_batch *B, int32_t path, int32_t reps, lcg_shape *best, float *best_ms,
+                                  float *default_ms, char *err, int errlen) {
+    if (!B || (path != 0 && path != 1) || reps < 1 || reps > 100) return LCG_E_ARG;
+    const lcg_shape start = B->req;
+    lcg_shape cur;
+    lcg_batch_get_shape(B, &cur);
+    const int np = B->npairs, cus = B->cus;
+    auto fail = [&](int rc, const std::string &msg) {
+        set_err(err, errlen, msg.c_str());
+        char e2[64];
+        lcg_batch_reshape(B, &start, e2, sizeof e2);
+        return rc;
+    };
+    if (path == 0) {
+        // integer path: fill shapes first (timed by the fill kernel alone),
+        // then top-level shapes with the best fill; every candidate's scores
+        // and alignments must equal those of the starting shape
+        float g0 = 1e30f;
+        for (int r = 0; r <= reps; ++r) {
+            float ms = 0;
+            if (int rc = lcg_batch_run(B, 2, 1, &ms)) return fail(rc, "run failed at the starting shape");
+            if (r > 0) g0 = std::min(g0, ms);
+        }
+        uint64_t ref = 0;
+        if (int rc = result_digest(B, ref)) return fail(rc, "results of the starting shape unavailable");
+        auto time_shape = [&](const lcg_shape &s, float &fill, float &top, bool &skip) -> int {
+            char e2[128];
+            skip = false;
+            const int rc = lcg_batch_reshape(B, &s, e2, sizeof e2);
+            if (rc == LCG_E_CAPACITY) {  // shape does not fit these pairs
+                skip = true;
+                return LCG_OK;
+            }
+            if (rc) return rc;
+            fill = top = 1e30f;
+            for (int r = 0; r <= reps; ++r) {
+                if (int rc2 = lcg_batch_run(B, 0, 1, nullptr)) return rc2;
+                if (r > 0) {
+                    fill = std::min(fill, B->fill_ms);
+                    top = std::min(top, B->top_ms);
+                }
+            }
+            uint64_t h = 0;
+            if (int rc2 = result_digest(B, h)) return rc2;
+            if (h != ref) return LCG_E_STATE;
+            return LCG_OK;
+        };
+        std::vector<lcg_shape> fills;
+        auto add_fill = [&](int w, int g, int k) {
+            lcg_shape s = cur;
+            s.waves = w;
+            s.wgs_per_cu = g;
+            s.wg_per_pair = k;
+            fills.push_back(s);
+        };
+        if (np >= cus) {
+            const int wg[][2] = {{16, 1}, {12, 1}, {8, 1}, {8, 2}, {4, 2}, {4, 4}};
+            for (auto &x : wg) add_fill(x[0], x[1], 1);
+        } else {
+            const int kmax = std::min(64, cus / np);
+            std::vector<int> ks = {kmax, kmax / 2, kmax / 4, 1};
+            ks.erase(std::remove_if(ks.begin(), ks.end(), [](int k) { return k < 1; }), ks.end());
+            std::sort(ks.begin(), ks.end(), std::greater<int>());
+            ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+            for (int k : ks) add_fill(16, 1, k);
+            add_fill(8, 1, kmax);
+        }
+        lcg_shape bestf = cur;
+        float bf = 1e30f;
+        for (const lcg_shape &s : fills) {
+            float fill = 0, top = 0;
+            bool skip = false;
+            const int rc = time_shape(s, fill, top, skip);
+            if (rc == LCG_E_STATE) return fail(rc, "results differ at shape " + shape_text(s));
+            if (rc) return fail(rc, "run failed at shape " + shape_text(s));
+            if (!skip && fill < bf) {
+                bf = fill;
+                lcg_batch_get_shape(B, &bestf);
+            }
+        }
+        const int tw[][2] = {{4, 4}, {8, 2}, {2, 4}, {16, 1}, {8, 1}, {4, 2}};
+        lcg_shape bestt = bestf;
+        float bt = 1e30f;
+        for (auto &x : tw) {
+            lcg_shape s = bestf;
+            s.top_waves = x[0];
+            s.top_wgs_per_cu = x[1];
+            float fill = 0, top = 0;
+            bool skip = false;
+            const int rc = time_shape(s, fill, top, skip);
+            if (rc == LCG_E_STATE) return fail(rc, "results differ at shape " + shape_text(s));
+            if (rc) return fail(rc, "run failed at shape " + shape_text(s));
+            if (!skip && top < bt) {
+                bt = top;
+                lcg_batch_get_shape(B, &bestt);
+            }
+        }
+        // the winner, timed as the starting shape was (graph replay); kept
+        // only if it beats the starting shape
+        char e2[128];
+        if (int rc = lcg_batch_reshape(B, &bestt, e2, sizeof e2)) return fail(rc, e2);
+        float g1 = 1e30f;
+        for (int r = 0; r <= reps; ++r) {
+            float ms = 0;
+            if (int rc = lcg_batch_run(B, 2, 1, &ms)) return fail(rc, "run failed at the chosen shape");
+            if (r > 0) g1 = std::min(g1, ms);
+        }
+        if (g1 > g0) {
+            if (int rc = lcg_batch_reshape(B, &start, e2, sizeof e2)) return rc;
+            g1 = g0;
+        }
+        if (default_ms) *default_ms = g0;
+        if (best_ms) *best_ms = g1;
+        if (best) lcg_batch_get_shape(B, best);
+        return LCG_OK;
+    }
+    // LocARNA-P: waves and workgroups per pair of the inside/outside kernel;
+    // every candidate's log Z must agree with the starting shape's to 1e-12
+    if (B->device_built) return LCG_E_STATE;
+    auto time_pf = [&](float &t) -> int {
+        t = 1e30f;
+        for (int r = 0; r <= reps; ++r) {
+            float ms = 0;
+            if (int rc = lcg_batch_pf_run(B, nullptr, &ms)) return rc;
+            if (r > 0) t = std::min(t, ms);
+        }
+        return LCG_OK;
+    };
+    float p0 = 0;
+    if (int rc = time_pf(p0)) return fail(rc, "LocARNA-P run failed at the starting shape");
+    std::vector<double> z0(np);
+    for (int k = 0; k < np; ++k) z0[k] = B->pf_result[B->pdescs[k].out];
+    std::vector<lcg_shape> cands;
+    std::vector<int> ks = {0, 1};
+    if (np < cus) {
+        const int kmax = std::min(64, cus / np);
+        for (int k : {kmax, kmax / 2, 2})
+            if (k > 1) ks.push_back(k);
+    }
+    std::sort(ks.begin(), ks.end());
+    ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
+    for (int w : {16, 8})
+        for (int k : ks) {
+            lcg_shape s = B->req;
+            s.pf_waves = w;
+            s.pf_wg_per_pair = k;
+            cands.push_back(s);
+        }
+    lcg_shape bests = B->req;
+    float bp = p0;
+    for (const lcg_shape &s : cands) {
+        char e2[128];
+        if (int rc = lcg_batch_reshape(B, &s, e2, sizeof e2)) return fail(rc, e2);
+        float t = 0;
+        if (int rc = time_pf(t)) return fail(rc, "LocARNA-P run failed at shape " + shape_text(s));
+        for (int k = 0; k < np; ++k) {
+            const double z = B->pf_result[B->pdescs[k].out];
+            if (!(std::fabs(z - z0[k]) <= 1e-12 * std::fabs(z0[k])))
+                return fail(LCG_E_STATE, "partition functions differ at shape " + shape_text(s));
+        }
+        if (t < bp) {
+            bp = t;
+            bests = s;
+        }
+    }
+    char e2[128];
+    if (int rc = lcg_batch_reshape(B, &bests, e2, sizeof e2)) return fail(rc, e2);
+    if (default_ms) *default_ms = p0;
+    if (best_ms) *best_ms = bp;
+    if (best) lcg_batch_get_shape(B, best);
+    return LCG_OK;
 }
