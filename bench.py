@@ -46,6 +46,8 @@ def main():
     ap.add_argument("--sequ-local", action="store_true", help="locarna --sequ-local (sequence-local top level)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the secondary configurations (long500, LocARNA-P)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes (no lcg_batch_autotune before timing)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (the all-pairs stage would shard "
@@ -110,6 +112,16 @@ def main():
         top_cells += tc
         top_cands += tq
     cells, cands = fill_cells + top_cells, fill_cands + top_cands
+
+    # launch shape: the autotuner times candidate shapes on this batch (each
+    # must reproduce the default shape's scores and alignments) and keeps the
+    # fastest; untimed
+    tune = None
+    if not args.no_autotune:
+        a0 = time.perf_counter()
+        shape, best_ms, default_ms = batch.autotune(path=0, reps=2)
+        tune = {"shape": shape, "tuned_ms": round(best_ms, 3), "default_ms": round(default_ms, 3),
+                "tuning_s": round(time.perf_counter() - a0, 2)}
 
     for _ in range(args.warmup):
         batch.run(mode=args.mode, trace=True)
@@ -216,7 +228,7 @@ def main():
                    "mode": "sequence-local" if args.sequ_local else "global",
                    "launch": {0: "persistent", 2: "hipGraph"}.get(args.mode, str(args.mode)),
                    "waves_per_workgroup": batch.waves, "workgroups": batch.workgroups,
-                   "lds_bytes": batch.lds_bytes},
+                   "lds_bytes": batch.lds_bytes, "autotune": tune},
         "gcups": round(cells / (ms_per_step / 1e3) / 1e9, 3),
         "dp_cells_per_step": cells, "arc_candidates_per_step": cands,
         "plain_pairs_per_sec": round(plain, 2) if plain else None,
@@ -251,6 +263,10 @@ def extra_configs(args, api, synth, torch, nthreads):
         B = api.Batch(P)
         B.run(mode=0, trace=True, count=True)
         cells = sum(B.counters(k)[0] for k in range(len(P)))
+        tune = None
+        if not args.no_autotune:
+            shape, best_ms, default_ms = B.autotune(path=0, reps=1)
+            tune = {"shape": shape, "tuned_ms": round(best_ms, 3), "default_ms": round(default_ms, 3)}
         B.run(mode=2, trace=True)
         ms = [B.run(mode=2, trace=True) for _ in range(3)]
         t = sum(ms) / len(ms)
@@ -264,7 +280,8 @@ def extra_configs(args, api, synth, torch, nthreads):
             "pairs": len(P), "ms_per_step": round(t, 3), "pairs_per_sec": round(len(P) / (t / 1e3), 2),
             "gcups": round(cells / (t / 1e3) / 1e9, 3), "workgroups_per_pair": B.workgroups // len(P) if B.workgroups else None,
             "cpu_baseline_pairs_per_sec": round(len(P) / dt, 3), "cpu_cores": nthreads,
-            "scores_bit_exact": "%d/%d" % (sum(1 for x, y in zip(scores, cs) if x == y), len(P))}
+            "scores_bit_exact": "%d/%d" % (sum(1 for x, y in zip(scores, cs) if x == y), len(P)),
+            "autotune": tune}
     for wl, nsample in (("pf16x150", 32), ("grad256", 32)):
         pps, pairs, span = synth.workload(wl)
         pa = api.locarna_p_params(max_bp_span=span)
@@ -277,6 +294,11 @@ def extra_configs(args, api, synth, torch, nthreads):
             C += c
             K += q
         A = sum(p.num_arcmatches for p in P)
+        tune = None
+        if not args.no_autotune:
+            shape, best_ms, default_ms = B.autotune(path=1, reps=1)
+            tune = {"shape": {k: shape[k] for k in ("pf_waves", "pf_wg_per_pair")},
+                    "tuned_ms": round(best_ms, 3), "default_ms": round(default_ms, 3)}
         B.pf_run()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -325,7 +347,7 @@ def extra_configs(args, api, synth, torch, nthreads):
                                 "flops_per_launch": w_pf},
                    "cpu_baseline_pairs_per_sec": round(len(idx) / dt, 3), "cpu_cores": nthreads,
                    "cpu_sample": "%d of %d pairs (inside + outside, CPU oracle)" % (len(idx), len(P)),
-                   "logZ_within_1e-9": "%d/%d" % (ok, len(idx))}
+                   "logZ_within_1e-9": "%d/%d" % (ok, len(idx)), "autotune": tune}
         if fit:
             res[wl]["fit"] = fit
     return res
