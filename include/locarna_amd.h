@@ -90,6 +90,39 @@ int lcg_pair_arcmatches(const lcg_pair *pair, int32_t *al, int32_t *ar, int32_t 
 int lcg_pair_prepare_device(const char *ppA, const char *ppB, const lcg_params *params, lcg_pair **out, char *err,
                             int errlen);
 
+/* Preparation of many pairs with the same parameters (mlocarna's all-pairs
+ * stage, src/Utils/mlocarna:3547-3640, which runs the per-pair preparation
+ * of src/locarna.cc:455-703 once per pair): pp parsing, score tables and arc
+ * matches on host threads, and with LCG_PREPARE_DEVICE_BAND the trace band
+ * (PFGotoh + PFTraceProbs, edge_probs.icc:76-268, and the cut of
+ * trace_controller.cc:565-597) of all pairs in one kernel launch.  The
+ * kernel computes in double with a power-of-two scale per anti-diagonal; a
+ * pair with a trace probability within band_eps (relative) of the cut, or a
+ * value the host precision (long double for locarna, double for
+ * locarna_p) could not represent or that the kernel did not keep exactly,
+ * is banded on the host instead, so every band equals lcg_pair_prepare's.
+ * Pairs longer than 1024 are banded on the host.  out[k] is the pair of
+ * ppA[k], ppB[k], as lcg_pair_prepare (or lcg_pair_prepare_device with
+ * LCG_PREPARE_DEVICE_COMPACTION) returns it. */
+#define LCG_PREPARE_DEVICE_COMPACTION 1
+#define LCG_PREPARE_DEVICE_BAND 2
+typedef struct {
+    int32_t flags;   /* LCG_PREPARE_* */
+    int32_t device;  /* HIP device of the band kernel */
+    int32_t threads; /* host threads; <= 0: the hardware's, at most 16 */
+    int32_t reserved;
+    double band_eps; /* <= 0: 1e-9 */
+} lcg_prepare_opts;
+typedef struct {
+    float band_ms;         /* device time of the band kernel(s) */
+    int32_t device_banded; /* pairs whose band the kernel computed */
+    int32_t host_banded;   /* pairs banded on the host (too long, flagged, or no device band asked) */
+    int32_t reserved;
+} lcg_prepare_stats;
+int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB, int32_t npairs, const lcg_params *params,
+                      const lcg_prepare_opts *opts, lcg_pair **out, lcg_prepare_stats *stats, char *err,
+                      int errlen);
+
 /* Device batch of prepared pairs (device >= 0: HIP device ordinal).  The
  * pairs may be freed after the call. */
 int lcg_batch_create(lcg_pair *const *pairs, int32_t npairs, int32_t device, lcg_batch **out, char *err,

@@ -42,6 +42,18 @@ def locarna_p_params(**kw):
     return locarna_params(**d)
 
 
+class PrepareOpts(ctypes.Structure):
+    """lcg_prepare_opts (include/locarna_amd.h)."""
+    _fields_ = [("flags", ctypes.c_int32), ("device", ctypes.c_int32), ("threads", ctypes.c_int32),
+                ("reserved", ctypes.c_int32), ("band_eps", ctypes.c_double)]
+
+
+class PrepareStats(ctypes.Structure):
+    """lcg_prepare_stats (include/locarna_amd.h)."""
+    _fields_ = [("band_ms", ctypes.c_float), ("device_banded", ctypes.c_int32),
+                ("host_banded", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class Shape(ctypes.Structure):
     """lcg_shape (include/locarna_amd.h): launch shape, 0 = default."""
     _fields_ = [(n, ctypes.c_int32) for n in ("waves", "wgs_per_cu", "wg_per_pair", "top_waves",
@@ -93,6 +105,9 @@ def _sig(L):
                                       c.POINTER(c.c_int64)]
     L.lcg_batch_segment.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int64]
     L.lcg_batch_compact_run.argtypes = [c.c_void_p, c.POINTER(c.c_float), c.POINTER(c.c_float)]
+    L.lcg_pairs_prepare.argtypes = [c.POINTER(c.c_char_p), c.POINTER(c.c_char_p), c.c_int32, c.POINTER(Params),
+                                    c.POINTER(PrepareOpts), c.POINTER(c.c_void_p), c.POINTER(PrepareStats),
+                                    c.c_char_p, c.c_int]
     L.lcg_batch_create_shaped.argtypes = [c.POINTER(c.c_void_p), c.c_int32, c.c_int32, c.POINTER(Shape),
                                           c.POINTER(c.c_void_p), c.c_char_p, c.c_int]
     L.lcg_batch_get_shape.argtypes = [c.c_void_p, c.POINTER(Shape)]
@@ -120,16 +135,19 @@ class Pair:
     arc matches are not enumerated on the host (lcg_pair_prepare_device):
     lcg_batch_create compacts them on the device."""
 
-    def __init__(self, ppA, ppB, params=None, device_compaction=False):
+    def __init__(self, ppA, ppB, params=None, device_compaction=False, _handle=None):
         L = lib()
         self.params = params or locarna_params()
         self.device_compaction = device_compaction
-        h = ctypes.c_void_p()
-        err = ctypes.create_string_buffer(512)
-        fn = L.lcg_pair_prepare_device if device_compaction else L.lcg_pair_prepare
-        rc = fn(ppA.encode(), ppB.encode(), ctypes.byref(self.params), ctypes.byref(h), err, 512)
-        if rc != 0:
-            raise LcgError("lcg_pair_prepare (%d): %s" % (rc, err.value.decode()))
+        if _handle is None:
+            h = ctypes.c_void_p()
+            err = ctypes.create_string_buffer(512)
+            fn = L.lcg_pair_prepare_device if device_compaction else L.lcg_pair_prepare
+            rc = fn(ppA.encode(), ppB.encode(), ctypes.byref(self.params), ctypes.byref(h), err, 512)
+            if rc != 0:
+                raise LcgError("lcg_pair_prepare (%d): %s" % (rc, err.value.decode()))
+        else:
+            h = _handle
         self.h = h
         a, b, na, ng = (ctypes.c_int32() for _ in range(4))
         cells = ctypes.c_int64()
@@ -195,6 +213,29 @@ class Pair:
             self.free()
         except Exception:
             pass
+
+
+def prepare_pairs(pp_pairs, params=None, device_compaction=False, device_band=False, threads=0, device=0,
+                  band_eps=0.0):
+    """Prepares many pairs with the same parameters (lcg_pairs_prepare):
+    host threads, and with device_band=True the trace bands of all pairs in
+    one kernel launch (flagged pairs rechecked on the host).  Returns the
+    list of Pair and a dict of the preparation statistics."""
+    L = lib()
+    params = params or locarna_params()
+    n = len(pp_pairs)
+    A = (ctypes.c_char_p * n)(*[a.encode() for a, _ in pp_pairs])
+    B = (ctypes.c_char_p * n)(*[b.encode() for _, b in pp_pairs])
+    out = (ctypes.c_void_p * max(n, 1))()
+    flags = (1 if device_compaction else 0) | (2 if device_band else 0)
+    opts = PrepareOpts(flags=flags, device=device, threads=threads, reserved=0, band_eps=band_eps)
+    st = PrepareStats()
+    err = ctypes.create_string_buffer(512)
+    rc = L.lcg_pairs_prepare(A, B, n, ctypes.byref(params), ctypes.byref(opts), out, ctypes.byref(st), err, 512)
+    if rc != 0:
+        raise LcgError("lcg_pairs_prepare (%d): %s" % (rc, err.value.decode()))
+    pairs = [Pair(None, None, params, device_compaction, _handle=ctypes.c_void_p(out[k])) for k in range(n)]
+    return pairs, {"band_ms": st.band_ms, "device_banded": st.device_banded, "host_banded": st.host_banded}
 
 
 class Batch:

@@ -27,6 +27,7 @@ HIP_SOURCES = [
     "align_kernels.hip",
     "pf_kernels.hip",
     "compact_kernels.hip",
+    "band_kernels.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -3,6 +3,8 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <atomic>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -10,10 +12,13 @@
 #include <functional>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
+#include <thread>
 #include <vector>
 
+#include "band_layout.h"
 #include "compact_layout.h"
 #include "device_layout.h"
 #include "pf_layout.h"
@@ -37,6 +42,7 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
 
 struct lcg_pair {
     lcg_params params;
+    ScoringParams sp;
     std::unique_ptr<RnaData> A, B;
     std::unique_ptr<BasePairs> bA, bB;
     std::unique_ptr<TraceController> tc;
@@ -454,45 +460,65 @@ void prepare_compaction_inputs(lcg_pair &P) {
 }  // namespace
 
 namespace {
+// Stage 1 of the preparation of a pair: both RnaData (pp reader), the
+// score parameters and the unrestricted band.
+void prepare_parse(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair &P) {
+    P.params = *p;
+    P.A.reset(new RnaData(ppA, p->min_prob, p->max_bp_span));
+    P.B.reset(new RnaData(ppB, p->min_prob, p->max_bp_span));
+    if (P.A->length() >= 0xffff || P.B->length() >= 0xffff) throw failure("sequence too long");
+    ScoringParams &sp = P.sp;
+    sp.indel = p->indel;
+    sp.indel_opening = p->indel_opening;
+    sp.struct_weight = p->struct_weight;
+    sp.tau = p->tau;
+    sp.match = p->match;
+    sp.mismatch = p->mismatch;
+    sp.temperature_alipf = p->temperature_alipf;
+    sp.use_ribosum = p->use_ribosum != 0;
+    sp.exp_prob = p->exp_prob;
+    P.tc.reset(new TraceController(P.A->length(), P.B->length()));
+}
+
+// Stage 2 on the host: the trace-probability band in the precision of the
+// reference (long double for locarna, double for locarna_p).
+void prepare_band_host(lcg_pair &P) {
+    const lcg_params *p = &P.params;
+    restrict_trace_by_probabilities(*P.tc, *P.A, *P.B, P.sp, p->min_trace_prob, p->band_extended != 0,
+                                    p->sequ_local != 0);
+}
+
+// Stage 3: base pairs, arc matches (host) or compaction inputs (device),
+// score tables and the device image.
+void prepare_finish(lcg_pair &P, bool deferred) {
+    const lcg_params *p = &P.params;
+    P.bA.reset(new BasePairs(P.A.get(), p->min_prob));
+    P.bB.reset(new BasePairs(P.B.get(), p->min_prob));
+    if (deferred) {
+        P.am.reset(new ArcMatches(*P.bA, *P.bB));
+        P.sc.reset(new Scoring(*P.A, *P.B, *P.am, P.sp));
+        P.deferred = true;
+        std::memset(&P.desc, 0, sizeof P.desc);
+        P.desc.n = P.A->length();
+        P.desc.m = P.B->length();
+        P.desc.nam = P.desc.ngroups = -1;
+        prepare_compaction_inputs(P);
+    } else {
+        P.am.reset(new ArcMatches(*P.bA, *P.bB, *P.tc));
+        P.sc.reset(new Scoring(*P.A, *P.B, *P.am, P.sp));
+        pack(P);
+    }
+}
+
 int prepare(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair **out, char *err, int errlen,
             bool deferred) {
     if (!ppA || !ppB || !p || !out) return LCG_E_ARG;
     *out = nullptr;
     try {
         auto P = std::make_unique<lcg_pair>();
-        P->params = *p;
-        P->A.reset(new RnaData(ppA, p->min_prob, p->max_bp_span));
-        P->B.reset(new RnaData(ppB, p->min_prob, p->max_bp_span));
-        if (P->A->length() >= 0xffff || P->B->length() >= 0xffff) throw failure("sequence too long");
-        ScoringParams sp;
-        sp.indel = p->indel;
-        sp.indel_opening = p->indel_opening;
-        sp.struct_weight = p->struct_weight;
-        sp.tau = p->tau;
-        sp.match = p->match;
-        sp.mismatch = p->mismatch;
-        sp.temperature_alipf = p->temperature_alipf;
-        sp.use_ribosum = p->use_ribosum != 0;
-        sp.exp_prob = p->exp_prob;
-        P->tc.reset(new TraceController(P->A->length(), P->B->length()));
-        restrict_trace_by_probabilities(*P->tc, *P->A, *P->B, sp, p->min_trace_prob, p->band_extended != 0,
-                                        p->sequ_local != 0);
-        P->bA.reset(new BasePairs(P->A.get(), p->min_prob));
-        P->bB.reset(new BasePairs(P->B.get(), p->min_prob));
-        if (deferred) {
-            P->am.reset(new ArcMatches(*P->bA, *P->bB));
-            P->sc.reset(new Scoring(*P->A, *P->B, *P->am, sp));
-            P->deferred = true;
-            std::memset(&P->desc, 0, sizeof P->desc);
-            P->desc.n = P->A->length();
-            P->desc.m = P->B->length();
-            P->desc.nam = P->desc.ngroups = -1;
-            prepare_compaction_inputs(*P);
-        } else {
-            P->am.reset(new ArcMatches(*P->bA, *P->bB, *P->tc));
-            P->sc.reset(new Scoring(*P->A, *P->B, *P->am, sp));
-            pack(*P);
-        }
+        prepare_parse(ppA, ppB, p, *P);
+        prepare_band_host(*P);
+        prepare_finish(*P, deferred);
         *out = P.release();
         return LCG_OK;
     } catch (failure &f) {
@@ -2389,5 +2415,254 @@ extern "C" int lcg_batch_autotune(lcg_batch *B, int32_t path, int32_t reps, lcg_
     if (default_ms) *default_ms = p0;
     if (best_ms) *best_ms = bp;
     if (best) lcg_batch_get_shape(B, best);
+    return LCG_OK;
+}
+
+extern "C" size_t lcg_band_lds_bytes(int lmax);
+extern "C" int lcg_launch_band(const BandDesc *descs, const int *order, int *queue, int *iarena,
+                               const double *darena, double *fwd, BandParams bp, int grid, hipStream_t stream);
+
+namespace {
+// fn(k) for k in [0, n) on `threads` host threads.  Returns the first k
+// whose call threw (its message in msg), or -1.
+template <class Fn>
+int parallel_for(int n, int threads, Fn fn, std::string &msg) {
+    std::atomic<int> next{0}, bad{-1};
+    std::mutex mu;
+    auto work = [&]() {
+        for (;;) {
+            const int k = next.fetch_add(1);
+            if (k >= n || bad.load() >= 0) return;
+            try {
+                fn(k);
+            } catch (std::exception &e) {
+                std::lock_guard<std::mutex> g(mu);
+                if (bad.load() < 0) {
+                    msg = e.what();
+                    bad.store(k);
+                }
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < std::min(threads, n); ++t) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+    return bad.load();
+}
+
+// Device memory of one band launch, freed on every path.
+struct BandBuffers {
+    BandDesc *descs = nullptr;
+    int32_t *order = nullptr, *queue = nullptr, *iarena = nullptr;
+    double *darena = nullptr, *fwd = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    ~BandBuffers() {
+        if (descs) hipFree(descs);
+        if (order) hipFree(order);
+        if (queue) hipFree(queue);
+        if (iarena) hipFree(iarena);
+        if (darena) hipFree(darena);
+        if (fwd) hipFree(fwd);
+        if (ev0) hipEventDestroy(ev0);
+        if (ev1) hipEventDestroy(ev1);
+        if (stream) hipStreamDestroy(stream);
+    }
+};
+
+// Trace bands of the pairs `todo` (all with the same parameters) on the
+// device, in chunks of at most 4 GiB of forward values.  A pair the kernel
+// flags keeps its unrestricted band and is appended to `recheck` (the host
+// bands it); the others get the kernel's rows.
+int device_bands(std::vector<std::unique_ptr<lcg_pair>> &P, std::vector<int> todo, const lcg_params &p,
+                 int device, double eps, std::vector<int> &recheck, float &ms_total, char *err, int errlen) {
+    if (todo.empty()) return LCG_OK;
+    auto cells = [&](int k) { return (int64_t)(P[k]->A->length() + 1) * (P[k]->B->length() + 1); };
+    std::stable_sort(todo.begin(), todo.end(), [&](int a, int b) { return cells(a) > cells(b); });
+    hipError_t e;
+    int cus = 0;
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
+        return hip_fail(e, err, errlen);
+    BandParams bp;
+    std::memset(&bp, 0, sizeof bp);
+    const ScoringParams &sp = P[todo[0]]->sp;
+    // as the host band (edge_probs.icc:76-162 with main_helper.icc:386-396)
+    bp.temp = sp.temperature_alipf / 100.0;
+    bp.g_open = std::exp((sp.indel_opening / 100.0) / bp.temp);
+    bp.g_ext = std::exp((sp.indel / 100.0) / bp.temp);
+    bp.sw = sp.struct_weight / 100.0;
+    bp.thr = p.min_trace_prob;
+    bp.eps = eps;
+    band_similarity(sp, bp.sim);
+    bp.local = p.sequ_local != 0;
+    bp.extended = p.band_extended != 0;
+    const int64_t budget = (int64_t)1 << 29;  // doubles (4 GiB)
+    size_t c0 = 0;
+    while (c0 < todo.size()) {
+        size_t c1 = c0;
+        int64_t fw = 0;
+        while (c1 < todo.size() && (c1 == c0 || fw + 3 * cells(todo[c1]) <= budget)) fw += 3 * cells(todo[c1++]);
+        const int np = (int)(c1 - c0);
+        std::vector<BandDesc> descs(np);
+        std::vector<int32_t> ia;
+        std::vector<double> da;
+        int lmax = 1;
+        int64_t foff = 0;
+        for (int q = 0; q < np; ++q) {
+            const lcg_pair &X = *P[todo[c0 + q]];
+            const int n = X.A->length(), m = X.B->length();
+            BandDesc &d = descs[q];
+            d.n = n;
+            d.m = m;
+            lmax = std::max(lmax, std::max(n, m));
+            d.codes = (int32_t)ia.size();
+            ia.push_back(-1);
+            for (int i = 1; i <= n; ++i) ia.push_back(nt_code(X.A->nt(i)));
+            ia.push_back(-1);
+            for (int j = 1; j <= m; ++j) ia.push_back(nt_code(X.B->nt(j)));
+            d.band = (int32_t)ia.size();
+            for (int i = 0; i <= n; ++i) ia.push_back(X.tc->min_col(i));
+            for (int i = 0; i <= n; ++i) ia.push_back(X.tc->max_col(i));
+            d.rows = (int32_t)ia.size();
+            ia.resize(ia.size() + 2 * (n + 1) + 1, 0);
+            d.prof = (int32_t)da.size();
+            da.push_back(0.0);
+            for (int i = 1; i <= n; ++i) da.push_back(X.A->prob_paired_upstream(i));
+            da.push_back(0.0);
+            for (int i = 1; i <= n; ++i) da.push_back(X.A->prob_paired_downstream(i));
+            da.push_back(0.0);
+            for (int j = 1; j <= m; ++j) da.push_back(X.B->prob_paired_upstream(j));
+            da.push_back(0.0);
+            for (int j = 1; j <= m; ++j) da.push_back(X.B->prob_paired_downstream(j));
+            d.fwd = foff;
+            foff += 3 * (int64_t)(n + 1) * (m + 1);
+        }
+        if (ia.size() >= (size_t)INT_MAX || da.size() >= (size_t)INT_MAX) {
+            set_err(err, errlen, "band batch too large");
+            return LCG_E_CAPACITY;
+        }
+        bp.npairs = np;
+        bp.lmax = lmax;
+        std::vector<int32_t> order(np);
+        std::iota(order.begin(), order.end(), 0);
+        const size_t lds = lcg_band_lds_bytes(lmax);
+        const int per_cu = std::max(1, std::min(8, (int)((160 * 1024 - 256) / lds)));
+        const int grid = std::min(np, cus * per_cu);
+        BandBuffers bb;
+        if ((e = hipStreamCreate(&bb.stream)) != hipSuccess) return hip_fail(e, err, errlen);
+        if ((e = hipEventCreate(&bb.ev0)) != hipSuccess) return hip_fail(e, err, errlen);
+        if ((e = hipEventCreate(&bb.ev1)) != hipSuccess) return hip_fail(e, err, errlen);
+        if ((e = hipMalloc(&bb.descs, sizeof(BandDesc) * np)) != hipSuccess) return hip_fail(e, err, errlen);
+        if ((e = hipMalloc(&bb.order, 4 * (size_t)np)) != hipSuccess) return hip_fail(e, err, errlen);
+        if ((e = hipMalloc(&bb.queue, 256)) != hipSuccess) return hip_fail(e, err, errlen);
+        if ((e = hipMalloc(&bb.iarena, 4 * ia.size())) != hipSuccess) return hip_fail(e, err, errlen);
+        if ((e = hipMalloc(&bb.darena, 8 * std::max<size_t>(1, da.size()))) != hipSuccess)
+            return hip_fail(e, err, errlen);
+        if ((e = hipMalloc(&bb.fwd, 8 * (size_t)foff)) != hipSuccess) return hip_fail(e, err, errlen);
+        if ((e = hipMemcpy(bb.descs, descs.data(), sizeof(BandDesc) * np, hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(e, err, errlen);
+        if ((e = hipMemcpy(bb.order, order.data(), 4 * (size_t)np, hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(e, err, errlen);
+        if ((e = hipMemcpy(bb.iarena, ia.data(), 4 * ia.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(e, err, errlen);
+        if ((e = hipMemcpy(bb.darena, da.data(), 8 * da.size(), hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(e, err, errlen);
+        if ((e = hipEventRecord(bb.ev0, bb.stream)) != hipSuccess) return hip_fail(e, err, errlen);
+        if (int rc = lcg_launch_band(bb.descs, bb.order, bb.queue, bb.iarena, bb.darena, bb.fwd, bp, grid,
+                                     bb.stream))
+            return hip_fail((hipError_t)rc, err, errlen);
+        if ((e = hipEventRecord(bb.ev1, bb.stream)) != hipSuccess) return hip_fail(e, err, errlen);
+        if ((e = hipMemcpyAsync(ia.data(), bb.iarena, 4 * ia.size(), hipMemcpyDeviceToHost, bb.stream)) !=
+            hipSuccess)
+            return hip_fail(e, err, errlen);
+        if ((e = hipStreamSynchronize(bb.stream)) != hipSuccess) return hip_fail(e, err, errlen);
+        float ms = 0;
+        hipEventElapsedTime(&ms, bb.ev0, bb.ev1);
+        ms_total += ms;
+        for (int q = 0; q < np; ++q) {
+            lcg_pair &X = *P[todo[c0 + q]];
+            const BandDesc &d = descs[q];
+            const int n = d.n;
+            if (ia[d.rows + 2 * (n + 1)] != 0) {
+                recheck.push_back(todo[c0 + q]);
+                continue;
+            }
+            std::vector<int> first(n + 1), last(n + 1);
+            for (int i = 0; i <= n; ++i) {
+                const int lo = ia[d.rows + i], hi = ia[d.rows + n + 1 + i];
+                first[i] = lo == INT_MAX ? -1 : lo;
+                last[i] = hi;
+            }
+            X.tc->restrict_by_rows(first, last);
+        }
+        c0 = c1;
+    }
+    return LCG_OK;
+}
+}  // namespace
+
+extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB, int32_t npairs,
+                                 const lcg_params *p, const lcg_prepare_opts *opts, lcg_pair **out,
+                                 lcg_prepare_stats *stats, char *err, int errlen) {
+    if (!ppA || !ppB || !p || !out || npairs < 0) return LCG_E_ARG;
+    for (int k = 0; k < npairs; ++k) {
+        out[k] = nullptr;
+        if (!ppA[k] || !ppB[k]) return LCG_E_ARG;
+    }
+    lcg_prepare_opts o;
+    std::memset(&o, 0, sizeof o);
+    if (opts) o = *opts;
+    const int threads = o.threads > 0 ? o.threads : std::max(1, std::min(16, (int)std::thread::hardware_concurrency()));
+    const double eps = o.band_eps > 0 ? o.band_eps : 1e-9;
+    const bool deferred = (o.flags & LCG_PREPARE_DEVICE_COMPACTION) != 0;
+    const bool dev_band = (o.flags & LCG_PREPARE_DEVICE_BAND) != 0;
+    std::vector<std::unique_ptr<lcg_pair>> P(npairs);
+    std::string msg;
+    auto input_error = [&](int k) {
+        set_err(err, errlen, "pair " + std::to_string(k) + ": " + msg);
+        return LCG_E_INPUT;
+    };
+    int bad = parallel_for(
+        npairs, threads,
+        [&](int k) {
+            P[k].reset(new lcg_pair);
+            prepare_parse(ppA[k], ppB[k], p, *P[k]);
+        },
+        msg);
+    if (bad >= 0) return input_error(bad);
+    std::vector<int> host_band;
+    float band_ms = 0;
+    int on_device = 0;
+    if (p->min_trace_prob > 0.0) {
+        if (dev_band) {
+            std::vector<int> todo, recheck;
+            for (int k = 0; k < npairs; ++k) {
+                if (P[k]->A->length() <= LCG_BAND_MAX_LEN && P[k]->B->length() <= LCG_BAND_MAX_LEN)
+                    todo.push_back(k);
+                else
+                    host_band.push_back(k);
+            }
+            if (int rc = device_bands(P, todo, *p, o.device, eps, recheck, band_ms, err, errlen)) return rc;
+            on_device = (int)(todo.size() - recheck.size());
+            host_band.insert(host_band.end(), recheck.begin(), recheck.end());
+        } else {
+            host_band.resize(npairs);
+            std::iota(host_band.begin(), host_band.end(), 0);
+        }
+        bad = parallel_for((int)host_band.size(), threads, [&](int t) { prepare_band_host(*P[host_band[t]]); },
+                           msg);
+        if (bad >= 0) return input_error(host_band[bad]);
+    }
+    bad = parallel_for(npairs, threads, [&](int k) { prepare_finish(*P[k], deferred); }, msg);
+    if (bad >= 0) return input_error(bad);
+    for (int k = 0; k < npairs; ++k) out[k] = P[k].release();
+    if (stats) {
+        std::memset(stats, 0, sizeof *stats);
+        stats->band_ms = band_ms;
+        stats->device_banded = on_device;
+        stats->host_banded = (int32_t)host_band.size();
+    }
     return LCG_OK;
 }

@@ -138,14 +138,23 @@ TraceController::TraceController(int lenA, int lenB)
 
 void TraceController::restrict_by_trace_probabilities(const std::vector<double> &probs, double min_prob) {
     const int W = lenB_ + 1;
+    std::vector<int> first(lenA_ + 1, -1), last(lenA_ + 1, -1);
     for (int i = 0; i <= lenA_; ++i) {
-        int lo = max_col_[i], hi = min_col_[i];
         for (int j = min_col_[i]; j <= max_col_[i]; ++j) {
             if (probs[(size_t)i * W + j] >= min_prob) {
-                lo = std::min(lo, j);
-                hi = std::max(hi, j);
+                if (first[i] < 0) first[i] = j;
+                last[i] = j;
             }
         }
+    }
+    restrict_by_rows(first, last);
+}
+
+void TraceController::restrict_by_rows(const std::vector<int> &first, const std::vector<int> &last) {
+    for (int i = 0; i <= lenA_; ++i) {
+        // no passing column: the row's range becomes empty (max < min)
+        const int lo = first[i] >= 0 ? std::min(first[i], max_col_[i]) : max_col_[i];
+        const int hi = last[i] >= 0 ? std::max(last[i], min_col_[i]) : min_col_[i];
         min_col_[i] = std::max(min_col_[i], lo);
         max_col_[i] = std::min(max_col_[i], hi);
     }
@@ -165,6 +174,12 @@ int nt_code(char c) {
         case 'U': return 3;
     }
     return -1;
+}
+
+void band_similarity(const ScoringParams &sp, double sim[16]) {
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b < 4; ++b)
+            sim[a * 4 + b] = sp.use_ribosum ? ribosum85_60::bm[a * 4 + b] : (a == b ? sp.match : sp.mismatch);
 }
 
 namespace {
@@ -221,11 +236,9 @@ std::vector<double> trace_probs(const RnaData &RA, const RnaData &RB, const Trac
     const double temp = sp.temperature_alipf / 100.0;
     const double gap_open = sp.indel_opening / 100.0, gap_ext = sp.indel / 100.0;
     const double sw = sp.struct_weight / 100.0;
-    // base similarity: RIBOSUM bm scores or match / mismatch (main_helper.icc:284-308)
-    double sim[4][4];
-    for (int a = 0; a < 4; ++a)
-        for (int b = 0; b < 4; ++b)
-            sim[a][b] = sp.use_ribosum ? ribosum85_60::bm[a * 4 + b] : (a == b ? sp.match : sp.mismatch);
+    double simv[16];
+    band_similarity(sp, simv);
+    auto sim = [&](int a, int b) { return simv[a * 4 + b]; };
     std::vector<double> upA(n + 1, 0.0), dnA(n + 1, 0.0), upB(m + 1, 0.0), dnB(m + 1, 0.0);
     for (int i = 1; i <= n; ++i) {
         upA[i] = RA.prob_paired_upstream(i);
@@ -244,7 +257,7 @@ std::vector<double> trace_probs(const RnaData &RA, const RnaData &RB, const Trac
             double seq = 0;
             int cnt = 0;
             if (a >= 0 && b >= 0) {
-                seq += sim[a][b];
+                seq += sim(a, b);
                 cnt++;
             }
             if (cnt != 0) seq /= cnt;

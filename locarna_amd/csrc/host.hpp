@@ -79,6 +79,9 @@ public:
     bool is_valid_match(int i, int j) const { return is_valid(i, j) && is_valid(i - 1, j - 1); }
     // trace_controller.cc:565-597
     void restrict_by_trace_probabilities(const std::vector<double> &probs, double min_prob);
+    // the same from the first and last column of each row whose trace
+    // probability reaches the cut (-1: no column of the row does)
+    void restrict_by_rows(const std::vector<int> &first, const std::vector<int> &last);
     const std::vector<int> &min_cols() const { return min_col_; }
     const std::vector<int> &max_cols() const { return max_col_; }
 
@@ -104,6 +107,10 @@ struct ScoringParams {
 
 // RIBOSUM nucleotide code: 0..3 for A, C, G, U; -1 for any other symbol
 int nt_code(char c);
+
+// Base similarity of the trace band's Gotoh partition function: RIBOSUM
+// base-match scores, or match / mismatch (main_helper.icc:284-308).
+void band_similarity(const ScoringParams &sp, double sim[16]);
 
 // Trace-probability band: Gotoh partition function over the StralScore
 // similarity, forward and backward, then the probability cut and the
