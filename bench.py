@@ -60,16 +60,19 @@ def main():
     pps, pairs, span = synth.workload(args.workload)
     params = api.locarna_params(max_bp_span=span, sequ_local=1 if args.sequ_local else 0)
     nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
-    # end to end: pp texts in host memory -> prepared pairs (host threads) ->
-    # device arena -> one launch -> scores and alignments back on the host.
-    # Twice: arc matches compacted and packed on the device (the default
-    # path), and enumerated and packed on the host.
-    def end_to_end(device_compaction):
+    # end to end: pp texts in host memory -> prepared pairs (lcg_pairs_prepare
+    # on host threads) -> device arena -> one launch -> scores and alignments
+    # back on the host.  Three times: trace band on the device and arc
+    # matches compacted on the device (the default path), band on the host
+    # with device compaction, and band, arc matches and packing on the host.
+    pp_list = [(pps[a], pps[b]) for a, b in pairs]
+
+    def end_to_end(device_compaction, device_band=False):
         torch.cuda.synchronize()
         e0 = time.perf_counter()
-        with cf.ThreadPoolExecutor(nthreads) as ex:
-            prep = list(ex.map(lambda ab: api.Pair(pps[ab[0]], pps[ab[1]], params,
-                                                   device_compaction=device_compaction), pairs))
+        prep, pst = api.prepare_pairs(pp_list, params, device_compaction=device_compaction,
+                                      device_band=device_band, threads=nthreads,
+                                      device=torch.cuda.current_device())
         e1 = time.perf_counter()
         b = api.Batch(prep, device=torch.cuda.current_device())
         e2 = time.perf_counter()
@@ -87,20 +90,63 @@ def main():
         e4 = time.perf_counter()
         rec = {"pairs_per_sec": round(len(prep) / (e4 - e0), 2), "prepare_s": round(e1 - e0, 3),
                "upload_s": round(e2 - e1, 3), "run_s": round(e3 - e2, 3), "fetch_s": round(e4 - e3, 3)}
+        if device_band:
+            rec["band"] = pst
         return prep, b, rec, image
 
     prepared_host, batch_host, e2e_host, image_host = end_to_end(False)
     batch_host.destroy()
-    prepared, batch, e2e_dev, image_dev = end_to_end(True)
+    prepared_hb, batch_hb, e2e_dev, image_hb = end_to_end(True)
+    batch_hb.destroy()
+    # the band kernel's one-time load and first allocation outside the timing
+    api.prepare_pairs(pp_list[:64], params, device_compaction=True, device_band=True, threads=nthreads,
+                      device=torch.cuda.current_device())
+    prepared, batch, e2e_band, image_dev = end_to_end(True, device_band=True)
     npairs = len(prepared)
     arcm = sum(batch.pair_info(k)[0] for k in range(npairs))
     groups = sum(batch.pair_info(k)[1] for k in range(npairs))
-    same_image = image_host == image_dev
-    del image_host, image_dev
-    end_to_end = {"device_compaction": e2e_dev, "host_compaction": e2e_host,
-                  "host_threads": nthreads, "images_identical": same_image,
-                  "note": "first launches include one-time kernel loading; upload_s of the device "
-                          "compaction path includes its two compaction kernels"}
+    same_image = image_host == image_dev == image_hb
+    same_band = all(x.band() == y.band() for x, y in zip(prepared, prepared_host))
+    del image_host, image_dev, image_hb
+    end_to_end = {"device_band_and_compaction": e2e_band, "device_compaction": e2e_dev,
+                  "host_compaction": e2e_host, "host_threads": nthreads, "images_identical": same_image,
+                  "bands_identical": same_band,
+                  "note": "the host and device compaction paths include their kernels' one-time "
+                          "loading (the band kernel is loaded before its path); upload_s of the device "
+                          "compaction paths includes their two compaction kernels; prepare_s of the "
+                          "device band path includes the band kernel"}
+
+    # trace band kernel (lcg_band_pairs) over all pairs, timed by HIP events
+    # around it; algorithmic bytes: the forward values written once and read
+    # once (48 B per matrix cell); CPU baseline: the host band (long double,
+    # as the reference) of the same pairs on the same threads
+    band = None
+    if args.steps > 0:
+        runs = [api.prepare_pairs(pp_list, params, device_compaction=True, device_band=True, threads=nthreads,
+                                  device=torch.cuda.current_device())[1] for _ in range(3)]
+        b_ms = sum(r["band_ms"] for r in runs) / len(runs)
+        mcells = sum((p.lenA + 1) * (p.lenB + 1) for p in prepared)
+        t0 = time.perf_counter()
+        api.prepare_pairs(pp_list, params, device_compaction=True, device_band=False, threads=nthreads)
+        t_host = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        api.prepare_pairs(pp_list, api.locarna_params(max_bp_span=span, min_trace_prob=0.0,
+                                                      sequ_local=1 if args.sequ_local else 0),
+                          device_compaction=True, device_band=False, threads=nthreads)
+        t_noband = time.perf_counter() - t0
+        host_band_s = t_host - t_noband
+        band = {"kernel": "lcg_band_pairs", "kernel_ms": round(b_ms, 3),
+                "pairs_per_sec": round(npairs / (b_ms / 1e3), 1), "matrix_cells": mcells,
+                "device_banded": runs[-1]["device_banded"], "host_rechecked": runs[-1]["host_banded"],
+                "roofline": {"bound": "hbm", "achieved": round(48 * mcells / (b_ms / 1e3) / 1e9, 2),
+                             "peak": 8000.0, "unit": "GB/s",
+                             "frac": round(48 * mcells / (b_ms / 1e3) / 1e9 / 8000.0, 5),
+                             "algorithmic_bytes": 48 * mcells},
+                "cpu_baseline": {"value": round(npairs / host_band_s, 1) if host_band_s > 0 else None,
+                                 "unit": "pairs/s", "cores": nthreads, "kind": "port",
+                                 "sample": "all %d pairs: host preparation with the long-double band minus "
+                                           "host preparation without a band, same threads" % npairs},
+                "bands_identical_to_host": same_band}
 
     # untimed counting run: DP cells and arc candidates each kernel evaluates
     batch.run(mode=0, trace=True, count=True)
@@ -236,6 +282,7 @@ def main():
         "roofline": roofline,
         "end_to_end": end_to_end,
         "arc_match_compaction": comp,
+        "trace_band": band,
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -117,7 +117,9 @@ typedef struct {
     float band_ms;         /* device time of the band kernel(s) */
     int32_t device_banded; /* pairs whose band the kernel computed */
     int32_t host_banded;   /* pairs banded on the host (too long, flagged, or no device band asked) */
-    int32_t reserved;
+    float parse_ms;        /* wall time: pp parsing and score inputs (host threads) */
+    float band_wall_ms;    /* wall time: band stage (device: inputs, copies, kernel; host bands) */
+    float finish_ms;       /* wall time: base pairs, arc matches or compaction inputs, images */
 } lcg_prepare_stats;
 int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB, int32_t npairs, const lcg_params *params,
                       const lcg_prepare_opts *opts, lcg_pair **out, lcg_prepare_stats *stats, char *err,

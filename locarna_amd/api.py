@@ -51,7 +51,8 @@ class PrepareOpts(ctypes.Structure):
 class PrepareStats(ctypes.Structure):
     """lcg_prepare_stats (include/locarna_amd.h)."""
     _fields_ = [("band_ms", ctypes.c_float), ("device_banded", ctypes.c_int32),
-                ("host_banded", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+                ("host_banded", ctypes.c_int32), ("parse_ms", ctypes.c_float), ("band_wall_ms", ctypes.c_float),
+                ("finish_ms", ctypes.c_float)]
 
 
 class Shape(ctypes.Structure):
@@ -235,7 +236,8 @@ def prepare_pairs(pp_pairs, params=None, device_compaction=False, device_band=Fa
     if rc != 0:
         raise LcgError("lcg_pairs_prepare (%d): %s" % (rc, err.value.decode()))
     pairs = [Pair(None, None, params, device_compaction, _handle=ctypes.c_void_p(out[k])) for k in range(n)]
-    return pairs, {"band_ms": st.band_ms, "device_banded": st.device_banded, "host_banded": st.host_banded}
+    return pairs, {"band_ms": st.band_ms, "device_banded": st.device_banded, "host_banded": st.host_banded,
+                   "parse_ms": st.parse_ms, "band_wall_ms": st.band_wall_ms, "finish_ms": st.finish_ms}
 
 
 class Batch:

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <climits>
 #include <cmath>
@@ -2476,7 +2477,8 @@ struct BandBuffers {
 // flags keeps its unrestricted band and is appended to `recheck` (the host
 // bands it); the others get the kernel's rows.
 int device_bands(std::vector<std::unique_ptr<lcg_pair>> &P, std::vector<int> todo, const lcg_params &p,
-                 int device, double eps, std::vector<int> &recheck, float &ms_total, char *err, int errlen) {
+                 int device, double eps, int threads, std::vector<int> &recheck, float &ms_total, char *err,
+                 int errlen) {
     if (todo.empty()) return LCG_OK;
     auto cells = [&](int k) { return (int64_t)(P[k]->A->length() + 1) * (P[k]->B->length() + 1); };
     std::stable_sort(todo.begin(), todo.end(), [&](int a, int b) { return cells(a) > cells(b); });
@@ -2505,9 +2507,11 @@ int device_bands(std::vector<std::unique_ptr<lcg_pair>> &P, std::vector<int> tod
         int64_t fw = 0;
         while (c1 < todo.size() && (c1 == c0 || fw + 3 * cells(todo[c1]) <= budget)) fw += 3 * cells(todo[c1++]);
         const int np = (int)(c1 - c0);
+        // layout first (offsets from the lengths), then every pair's inputs
+        // on the host threads (the structure profiles sum over the pairs of
+        // each position)
         std::vector<BandDesc> descs(np);
-        std::vector<int32_t> ia;
-        std::vector<double> da;
+        size_t ni = 0, nd = 0;
         int lmax = 1;
         int64_t foff = 0;
         for (int q = 0; q < np; ++q) {
@@ -2517,32 +2521,51 @@ int device_bands(std::vector<std::unique_ptr<lcg_pair>> &P, std::vector<int> tod
             d.n = n;
             d.m = m;
             lmax = std::max(lmax, std::max(n, m));
-            d.codes = (int32_t)ia.size();
-            ia.push_back(-1);
-            for (int i = 1; i <= n; ++i) ia.push_back(nt_code(X.A->nt(i)));
-            ia.push_back(-1);
-            for (int j = 1; j <= m; ++j) ia.push_back(nt_code(X.B->nt(j)));
-            d.band = (int32_t)ia.size();
-            for (int i = 0; i <= n; ++i) ia.push_back(X.tc->min_col(i));
-            for (int i = 0; i <= n; ++i) ia.push_back(X.tc->max_col(i));
-            d.rows = (int32_t)ia.size();
-            ia.resize(ia.size() + 2 * (n + 1) + 1, 0);
-            d.prof = (int32_t)da.size();
-            da.push_back(0.0);
-            for (int i = 1; i <= n; ++i) da.push_back(X.A->prob_paired_upstream(i));
-            da.push_back(0.0);
-            for (int i = 1; i <= n; ++i) da.push_back(X.A->prob_paired_downstream(i));
-            da.push_back(0.0);
-            for (int j = 1; j <= m; ++j) da.push_back(X.B->prob_paired_upstream(j));
-            da.push_back(0.0);
-            for (int j = 1; j <= m; ++j) da.push_back(X.B->prob_paired_downstream(j));
+            d.codes = (int32_t)ni;
+            d.band = (int32_t)(ni + n + m + 2);
+            d.rows = (int32_t)(ni + n + m + 2 + 2 * (n + 1));
+            ni += (size_t)(n + m + 2) + 2 * (n + 1) + 2 * (n + 1) + 1;
+            d.prof = (int32_t)nd;
+            nd += 2 * (size_t)(n + 1) + 2 * (size_t)(m + 1);
             d.fwd = foff;
             foff += 3 * (int64_t)(n + 1) * (m + 1);
+            if (ni >= (size_t)INT_MAX || nd >= (size_t)INT_MAX) break;
         }
-        if (ia.size() >= (size_t)INT_MAX || da.size() >= (size_t)INT_MAX) {
+        if (ni >= (size_t)INT_MAX || nd >= (size_t)INT_MAX) {
             set_err(err, errlen, "band batch too large");
             return LCG_E_CAPACITY;
         }
+        std::vector<int32_t> ia(ni, 0);
+        std::vector<double> da(nd, 0.0);
+        std::string msg;
+        parallel_for(
+            np, threads,
+            [&](int q) {
+                const lcg_pair &X = *P[todo[c0 + q]];
+                const BandDesc &d = descs[q];
+                const int n = d.n, m = d.m;
+                int32_t *c = ia.data() + d.codes;
+                c[0] = -1;
+                for (int i = 1; i <= n; ++i) c[i] = nt_code(X.A->nt(i));
+                c[n + 1] = -1;
+                for (int j = 1; j <= m; ++j) c[n + 1 + j] = nt_code(X.B->nt(j));
+                int32_t *b = ia.data() + d.band;
+                for (int i = 0; i <= n; ++i) {
+                    b[i] = X.tc->min_col(i);
+                    b[n + 1 + i] = X.tc->max_col(i);
+                }
+                double *f = da.data() + d.prof;
+                for (int i = 1; i <= n; ++i) {
+                    f[i] = X.A->prob_paired_upstream(i);
+                    f[n + 1 + i] = X.A->prob_paired_downstream(i);
+                }
+                double *g = f + 2 * (n + 1);
+                for (int j = 1; j <= m; ++j) {
+                    g[j] = X.B->prob_paired_upstream(j);
+                    g[m + 1 + j] = X.B->prob_paired_downstream(j);
+                }
+            },
+            msg);
         bp.npairs = np;
         bp.lmax = lmax;
         std::vector<int32_t> order(np);
@@ -2620,6 +2643,10 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
     const bool dev_band = (o.flags & LCG_PREPARE_DEVICE_BAND) != 0;
     std::vector<std::unique_ptr<lcg_pair>> P(npairs);
     std::string msg;
+    using clk = std::chrono::steady_clock;
+    auto since = [](clk::time_point t) { return std::chrono::duration<float, std::milli>(clk::now() - t).count(); };
+    auto t0 = clk::now();
+    float parse_ms = 0, band_wall_ms = 0;
     auto input_error = [&](int k) {
         set_err(err, errlen, "pair " + std::to_string(k) + ": " + msg);
         return LCG_E_INPUT;
@@ -2632,6 +2659,8 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
         },
         msg);
     if (bad >= 0) return input_error(bad);
+    parse_ms = since(t0);
+    t0 = clk::now();
     std::vector<int> host_band;
     float band_ms = 0;
     int on_device = 0;
@@ -2644,7 +2673,8 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
                 else
                     host_band.push_back(k);
             }
-            if (int rc = device_bands(P, todo, *p, o.device, eps, recheck, band_ms, err, errlen)) return rc;
+            if (int rc = device_bands(P, todo, *p, o.device, eps, threads, recheck, band_ms, err, errlen))
+                return rc;
             on_device = (int)(todo.size() - recheck.size());
             host_band.insert(host_band.end(), recheck.begin(), recheck.end());
         } else {
@@ -2655,6 +2685,8 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
                            msg);
         if (bad >= 0) return input_error(host_band[bad]);
     }
+    band_wall_ms = since(t0);
+    t0 = clk::now();
     bad = parallel_for(npairs, threads, [&](int k) { prepare_finish(*P[k], deferred); }, msg);
     if (bad >= 0) return input_error(bad);
     for (int k = 0; k < npairs; ++k) out[k] = P[k].release();
@@ -2663,6 +2695,9 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
         stats->band_ms = band_ms;
         stats->device_banded = on_device;
         stats->host_banded = (int32_t)host_band.size();
+        stats->parse_ms = parse_ms;
+        stats->band_wall_ms = band_wall_ms;
+        stats->finish_ms = since(t0);
     }
     return LCG_OK;
 }

@@ -13,7 +13,7 @@ def test_batch_prepare_equals_single(device_compaction):
     pp = [(pps[a], pps[b]) for a, b in pairs]
     pa = api.locarna_params(max_bp_span=span)
     many, st = api.prepare_pairs(pp, pa, device_compaction=device_compaction, threads=4)
-    assert st == {"band_ms": 0.0, "device_banded": 0, "host_banded": len(pp)}
+    assert (st["band_ms"], st["device_banded"], st["host_banded"]) == (0.0, 0, len(pp))
     for k, (a, b) in enumerate(pp):
         one = api.Pair(a, b, pa, device_compaction=device_compaction)
         x = many[k]
