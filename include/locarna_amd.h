@@ -77,6 +77,29 @@ int lcg_pair_band(const lcg_pair *pair, int32_t *min_col, int32_t *max_col, int3
 int lcg_pair_arcmatches(const lcg_pair *pair, int32_t *al, int32_t *ar, int32_t *bl, int32_t *br, int64_t *score,
                         int32_t cap);
 
+/* Host preparation of one pair whose arc matches and their scores come from
+ * an arc-match scores text (locarna --read-arcmatch-scores /
+ * --read-arcmatch-probs, src/locarna.cc:599-612;
+ * ArcMatches::read_arcmatch_scores, arc_matches.cc:190-273): one line
+ * "i j k l score" per arc match of arc (i, j) of A with arc (k, l) of B; with
+ * probability_scale >= 0 the fifth field is a probability p and the score
+ * (long)(p * probability_scale).  The arcs of each sequence are the arcs of
+ * the text (in sorted order), the arc matches its lines in order with the
+ * invalid ones (trace band) left out, and the arc-match score is the given
+ * one (Scoring::arcmatch, scoring.cc:541-546).  The pp texts give the
+ * sequences and the trace band.  Malformed lines and positions fail with
+ * LCG_E_INPUT as in the reference, and so do arcs with j - i < 3, which the
+ * reference's BasePairs never registers.  LocARNA-P runs of such pairs treat
+ * the arc-match scores as constants in the gradient. */
+int lcg_pair_prepare_scored(const char *ppA, const char *ppB, const lcg_params *params, const char *arcmatch_scores,
+                            int32_t probability_scale, lcg_pair **out, char *err, int errlen);
+/* Arc-match scores text of a pair (ArcMatches::write_arcmatch_scores,
+ * arc_matches.cc:275-302, locarna --write-arcmatch-scores): "al ar bl br
+ * score" per arc match in index order.  Returns the text length; writes
+ * the text and its terminating 0 into buf when cap exceeds the length.
+ * LCG_E_STATE for pairs of lcg_pair_prepare_device. */
+int64_t lcg_pair_write_arcmatch_scores(const lcg_pair *pair, char *buf, int64_t cap);
+
 /* Host preparation of one pair without the arc-match enumeration: RnaData,
  * trace band, BasePairs and the score tables as lcg_pair_prepare, while the
  * arc matches (ArcMatches constructor, arc_matches.cc:130-188: predicate

@@ -106,6 +106,10 @@ def _sig(L):
                                       c.POINTER(c.c_int64)]
     L.lcg_batch_segment.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int64]
     L.lcg_batch_compact_run.argtypes = [c.c_void_p, c.POINTER(c.c_float), c.POINTER(c.c_float)]
+    L.lcg_pair_prepare_scored.argtypes = [c.c_char_p, c.c_char_p, c.POINTER(Params), c.c_char_p, c.c_int32,
+                                          c.POINTER(c.c_void_p), c.c_char_p, c.c_int]
+    L.lcg_pair_write_arcmatch_scores.argtypes = [c.c_void_p, c.c_char_p, c.c_int64]
+    L.lcg_pair_write_arcmatch_scores.restype = c.c_int64
     L.lcg_pairs_prepare.argtypes = [c.POINTER(c.c_char_p), c.POINTER(c.c_char_p), c.c_int32, c.POINTER(Params),
                                     c.POINTER(PrepareOpts), c.POINTER(c.c_void_p), c.POINTER(PrepareStats),
                                     c.c_char_p, c.c_int]
@@ -156,6 +160,28 @@ class Pair:
                         ctypes.byref(cells))
         self.lenA, self.lenB, self.num_arcmatches, self.num_groups = a.value, b.value, na.value, ng.value
         self.cells = cells.value
+
+    @classmethod
+    def from_arcmatch_scores(cls, ppA, ppB, scores_text, params=None, probability_scale=-1):
+        """Pair whose arc matches and scores come from an arc-match scores
+        text (lcg_pair_prepare_scored)."""
+        params = params or locarna_params()
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(512)
+        rc = lib().lcg_pair_prepare_scored(ppA.encode(), ppB.encode(), ctypes.byref(params), scores_text.encode(),
+                                           probability_scale, ctypes.byref(h), err, 512)
+        if rc != 0:
+            raise LcgError("lcg_pair_prepare_scored (%d): %s" % (rc, err.value.decode()))
+        return cls(None, None, params, False, _handle=h)
+
+    def arcmatch_scores_text(self):
+        """Arc-match scores text (lcg_pair_write_arcmatch_scores)."""
+        n = lib().lcg_pair_write_arcmatch_scores(self.h, None, 0)
+        if n < 0:
+            raise LcgError("lcg_pair_write_arcmatch_scores (%d)" % n)
+        buf = ctypes.create_string_buffer(n + 1)
+        lib().lcg_pair_write_arcmatch_scores(self.h, buf, n + 1)
+        return buf.value.decode()
 
     @property
     def num_arcs(self):

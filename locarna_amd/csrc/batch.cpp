@@ -542,6 +542,60 @@ extern "C" int lcg_pair_prepare_device(const char *ppA, const char *ppB, const l
     return prepare(ppA, ppB, p, out, err, errlen, true);
 }
 
+extern "C" int lcg_pair_prepare_scored(const char *ppA, const char *ppB, const lcg_params *p,
+                                       const char *arcmatch_scores, int32_t probability_scale, lcg_pair **out,
+                                       char *err, int errlen) {
+    if (!ppA || !ppB || !p || !arcmatch_scores || !out) return LCG_E_ARG;
+    *out = nullptr;
+    try {
+        auto P = std::make_unique<lcg_pair>();
+        prepare_parse(ppA, ppB, p, *P);
+        prepare_band_host(*P);
+        const int n = P->A->length(), m = P->B->length();
+        const std::vector<ExplicitArcMatch> lines = parse_arcmatch_scores(arcmatch_scores, n, m, probability_scale);
+        // arcs of the file, as sets (sorted, unique)
+        std::vector<std::pair<int, int>> arcsA, arcsB;
+        for (const ExplicitArcMatch &x : lines) {
+            arcsA.push_back({x.i, x.j});
+            arcsB.push_back({x.k, x.l});
+        }
+        for (auto *v : {&arcsA, &arcsB}) {
+            std::sort(v->begin(), v->end());
+            v->erase(std::unique(v->begin(), v->end()), v->end());
+        }
+        P->bA.reset(new BasePairs(n, arcsA));
+        P->bB.reset(new BasePairs(m, arcsB));
+        P->am.reset(new ArcMatches(*P->bA, *P->bB, *P->tc, lines));
+        P->sc.reset(new Scoring(*P->A, *P->B, *P->am, P->sp));
+        pack(*P);
+        *out = P.release();
+        return LCG_OK;
+    } catch (failure &f) {
+        set_err(err, errlen, f.what());
+        return LCG_E_INPUT;
+    } catch (std::exception &e) {
+        set_err(err, errlen, e.what());
+        return LCG_E_INPUT;
+    }
+}
+
+extern "C" int64_t lcg_pair_write_arcmatch_scores(const lcg_pair *p, char *buf, int64_t cap) {
+    if (!p) return LCG_E_ARG;
+    if (p->deferred) return LCG_E_STATE;
+    std::string text;
+    char line[96];
+    const BasePairs &bA = *p->bA, &bB = *p->bB;
+    for (int k = 0; k < p->am->num_arc_matches(); ++k) {
+        const ArcMatch &x = p->am->arcmatch(k);
+        const Arc &a = bA.arc(x.arcA), &b = bB.arc(x.arcB);
+        std::snprintf(line, sizeof line, "%d %d %d %d %ld\n", a.left, a.right, b.left, b.right,
+                      (long)p->sc->arcmatch(k));
+        text += line;
+    }
+    if (buf && cap > (int64_t)text.size()) std::memcpy(buf, text.c_str(), text.size() + 1);
+    return (int64_t)text.size();
+}
+
 extern "C" void lcg_pair_free(lcg_pair *p) { delete p; }
 
 extern "C" int lcg_pair_info(const lcg_pair *p, int32_t *lenA, int32_t *lenB, int32_t *nam, int32_t *ngroups,
@@ -738,11 +792,12 @@ static void capture_pf_host(lcg_batch &B, lcg_pair *const *pairs, int npairs) {
             h.am_score[t] = P.sc->arcmatch(t);
             const int u0 = nt4(P.A->nt(a.left)), u1 = nt4(P.A->nt(a.right));
             const int v0 = nt4(P.B->nt(b.left)), v1 = nt4(P.B->nt(b.right));
-            h.am_type[t] = (P.params.use_ribosum && u0 >= 0 && u1 >= 0 && v0 >= 0 && v1 >= 0)
+            const bool expl = P.am->explicit_scores();  // explicit scores: constants in theta
+            h.am_type[t] = (!expl && P.params.use_ribosum && u0 >= 0 && u1 >= 0 && v0 >= 0 && v1 >= 0)
                                ? (u0 * 4 + u1) * 16 + (v0 * 4 + v1)
                                : -1;
             const double pa = P.A->arc_prob(a.left, a.right), pb = P.B->arc_prob(b.left, b.right);
-            h.am_f[t] = (1 - std::log(pa) / std::log(peA)) + (1 - std::log(pb) / std::log(peB));
+            h.am_f[t] = expl ? 0.0 : (1 - std::log(pa) / std::log(peA)) + (1 - std::log(pb) / std::log(peB));
             h.am_ends[4 * t] = a.left;
             h.am_ends[4 * t + 1] = a.right;
             h.am_ends[4 * t + 2] = b.left;

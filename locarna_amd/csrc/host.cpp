@@ -131,6 +131,65 @@ BasePairs::BasePairs(const RnaData *rna, double min_prob) : len_(rna->length()) 
         std::sort(l.begin(), l.end(), [&](int a, int b) { return arcs_[a].left > arcs_[b].left; });
 }
 
+BasePairs::BasePairs(int len, const std::vector<std::pair<int, int>> &sorted_arcs) : len_(len) {
+    left_.resize(len_ + 2);
+    right_.resize(len_ + 2);
+    for (const auto &x : sorted_arcs) {
+        const int idx = (int)arcs_.size();
+        arcs_.push_back(Arc{idx, x.first, x.second});
+        left_[x.first].push_back(idx);
+        right_[x.second].push_back(idx);
+    }
+    for (auto &l : left_)
+        std::sort(l.begin(), l.end(), [&](int a, int b) { return arcs_[a].right < arcs_[b].right; });
+    for (auto &l : right_)
+        std::sort(l.begin(), l.end(), [&](int a, int b) { return arcs_[a].left > arcs_[b].left; });
+}
+
+int BasePairs::find(int i, int j) const {
+    if (i < 0 || i > len_) return -1;
+    for (int a : left_[i])
+        if (arcs_[a].right == j) return a;
+    return -1;
+}
+
+std::vector<ExplicitArcMatch> parse_arcmatch_scores(const std::string &text, int lenA, int lenB,
+                                                    int probability_scale) {
+    std::vector<ExplicitArcMatch> out;
+    std::istringstream in(text);
+    std::string line;
+    int lineno = 0;
+    while (std::getline(in, line)) {
+        ++lineno;
+        std::istringstream ls(line);
+        long i = 0, j = 0, k = 0, l = 0;
+        ExplicitArcMatch x;
+        bool ok = static_cast<bool>(ls >> i >> j >> k >> l);
+        if (ok) {
+            if (probability_scale < 0) {
+                long sc = 0;
+                ok = static_cast<bool>(ls >> sc);
+                x.score = sc;
+            } else {
+                double prob = 0;
+                ok = static_cast<bool>(ls >> prob);
+                x.score = (score_t)(prob * (double)probability_scale);
+            }
+        }
+        if (!ok || i <= 0 || j <= 0 || k <= 0 || l <= 0 || i > j || j > lenA || k > l || l > lenB)
+            throw failure("Cannot read arc match scores. Invalid line " + std::to_string(lineno) + ": " + line);
+        if (j - i < 3 || l - k < 3)
+            throw failure("Cannot read arc match scores. Arc shorter than a base pair of BasePairs (j - i < 3) "
+                          "in line " + std::to_string(lineno) + ": " + line);
+        x.i = (int)i;
+        x.j = (int)j;
+        x.k = (int)k;
+        x.l = (int)l;
+        out.push_back(x);
+    }
+    return out;
+}
+
 // ---------------------------------------------------------- TraceController
 
 TraceController::TraceController(int lenA, int lenB)
@@ -332,6 +391,24 @@ ArcMatches::ArcMatches(const BasePairs &A, const BasePairs &B, const TraceContro
     }
 }
 
+ArcMatches::ArcMatches(const BasePairs &A, const BasePairs &B, const TraceController &tc,
+                       const std::vector<ExplicitArcMatch> &lines)
+    : A_(A), B_(B), explicit_(true) {
+    const int lmax = std::max(A.seqlen(), B.seqlen());
+    for (const ExplicitArcMatch &x : lines) {
+        const int a = A.find(x.i, x.j), b = B.find(x.k, x.l);
+        const Arc &p = A.arc(a), &q = B.arc(b);
+        // is_valid_arcmatch (arc_matches.cc:19-48) with the default
+        // max-diff-am and max-diff-at-am (the longer length)
+        const bool ok = tc.is_valid_match(p.left, q.left) && tc.is_valid_match(p.right, q.right) &&
+                        std::abs((p.right - p.left) - (q.right - q.left)) <= lmax &&
+                        std::abs(p.left - q.left) <= lmax && std::abs(p.right - q.right) <= lmax;
+        if (!ok) continue;
+        ams_.push_back(ArcMatch{(int)ams_.size(), a, b});
+        scores_.push_back(x.score);
+    }
+}
+
 // ------------------------------------------------------------------ Scoring
 
 Scoring::Scoring(const RnaData &A, const RnaData &B, const ArcMatches &am, const ScoringParams &p)
@@ -381,12 +458,20 @@ Scoring::Scoring(const RnaData &A, const RnaData &B, const ArcMatches &am, const
     const BasePairs &bA = am.bpsA(), &bB = am.bpsB();
     wA_.resize(bA.num_bps());
     wB_.resize(bB.num_bps());
-    for (int k = 0; k < bA.num_bps(); ++k) wA_[k] = weight(A.arc_prob(bA.arc(k).left, bA.arc(k).right), peA);
-    for (int k = 0; k < bB.num_bps(); ++k) wB_[k] = weight(B.arc_prob(bB.arc(k).left, bB.arc(k).right), peB);
-    // arc-match scores (scoring.cc:368-554)
     const int na = am.num_arc_matches();
     am_score_.resize(na);
     am_seq_.assign(na, 0);
+    if (am.explicit_scores()) {
+        // explicit arc-match scores (Scoring::arcmatch, scoring.cc:541-546);
+        // the arc weights take no part
+        wA_.assign(bA.num_bps(), 0);
+        wB_.assign(bB.num_bps(), 0);
+        for (int k = 0; k < na; ++k) am_score_[k] = am.explicit_score(k);
+        return;
+    }
+    for (int k = 0; k < bA.num_bps(); ++k) wA_[k] = weight(A.arc_prob(bA.arc(k).left, bA.arc(k).right), peA);
+    for (int k = 0; k < bB.num_bps(); ++k) wB_[k] = weight(B.arc_prob(bB.arc(k).left, bB.arc(k).right), peB);
+    // arc-match scores (scoring.cc:368-554)
     for (int k = 0; k < na; ++k) {
         const ArcMatch &x = am.arcmatch(k);
         const Arc &a = bA.arc(x.arcA), &b = bB.arc(x.arcB);

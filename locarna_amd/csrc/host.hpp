@@ -53,6 +53,11 @@ struct Arc {
 class BasePairs {
 public:
     BasePairs(const RnaData *rna, double min_prob);
+    // arcs given as a set, registered in (left, right) ascending order
+    // (BasePairs(len, bpair_set), basepairs.cc:79-94)
+    BasePairs(int len, const std::vector<std::pair<int, int>> &sorted_arcs);
+    // index of arc (i, j), -1 if absent
+    int find(int i, int j) const;
     int num_bps() const { return (int)arcs_.size(); }
     const Arc &arc(int idx) const { return arcs_[idx]; }
     int seqlen() const { return len_; }
@@ -125,6 +130,23 @@ struct ArcMatch {
     int arcA, arcB;  // arc indices
 };
 
+// One line of an arc-match scores file: arcs (i, j) of A and (k, l) of B
+// and the score of matching them.
+struct ExplicitArcMatch {
+    int i, j, k, l;
+    score_t score;
+};
+// Parses an arc-match scores text ("i j k l score" per line; with
+// probability_scale >= 0 the fifth field is a probability p and the score
+// (long)(p * probability_scale)).  reference:
+// ArcMatches::read_arcmatch_scores, arc_matches.cc:190-246.  Throws failure
+// on a malformed line, on i > j, k > l, a zero or too large position (as
+// the reference), and on arcs with j - i < 3 or l - k < 3, which the
+// reference accepts but its own BasePairs never registers
+// (basepairs.cc:162-177) and whose D would read outside their group.
+std::vector<ExplicitArcMatch> parse_arcmatch_scores(const std::string &text, int lenA, int lenB,
+                                                    int probability_scale);
+
 // Valid arc matches in reference order (arc A index outer, arc B inner).
 // reference: ArcMatches, arc_matches.cc:19-48, 130-188
 class ArcMatches {
@@ -134,6 +156,12 @@ public:
     // no arc matches enumerated on the host: the device compaction builds
     // the lists (the object still carries the two base-pair sets)
     ArcMatches(const BasePairs &A, const BasePairs &B) : A_(A), B_(B) {}
+    // arc matches of an arc-match scores file, in line order, the invalid
+    // ones left out, with their explicit scores (arc_matches.cc:248-273)
+    ArcMatches(const BasePairs &A, const BasePairs &B, const TraceController &tc,
+               const std::vector<ExplicitArcMatch> &lines);
+    bool explicit_scores() const { return explicit_; }
+    score_t explicit_score(int k) const { return scores_[k]; }
     int num_arc_matches() const { return (int)ams_.size(); }
     const ArcMatch &arcmatch(int k) const { return ams_[k]; }
     const BasePairs &bpsA() const { return A_; }
@@ -142,6 +170,8 @@ public:
 private:
     const BasePairs &A_, &B_;
     std::vector<ArcMatch> ams_;
+    bool explicit_ = false;
+    std::vector<score_t> scores_;
 };
 
 // Integer score tables.  reference: Scoring, scoring.cc:28-554
