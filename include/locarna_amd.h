@@ -46,6 +46,9 @@ typedef struct {
     int32_t band_extended;     /* trace band PF in long double (locarna) or
                                   double (locarna_p default)                 */
     int32_t use_ribosum;       /* --use-ribosum (built-in RIBOSUM85_60)      */
+    int32_t free_endgaps;      /* --free-endgaps "++--" as bits LCG_FREE_*;
+                                  not with sequ_local (src/locarna.cc:375),
+                                  integer path only                          */
     double min_prob;           /* --min-prob                                */
     double min_trace_prob;     /* --min-trace-probability (<= 0: no band)    */
     double exp_prob;           /* --exp-prob (<= 0: 1/(2 length))            */
@@ -53,6 +56,13 @@ typedef struct {
     double min_am_prob;        /* --min-am-prob                             */
     double min_bm_prob;        /* --min-bm-prob                             */
 } lcg_params;
+
+/* free end gaps (FreeEndgaps, free_endgaps.hh: the characters of the
+ * --free-endgaps string in order) */
+#define LCG_FREE_LEFT_1 1  /* left end of the first sequence  */
+#define LCG_FREE_RIGHT_1 2 /* right end of the first sequence */
+#define LCG_FREE_LEFT_2 4  /* left end of the second sequence */
+#define LCG_FREE_RIGHT_2 8 /* right end of the second sequence */
 
 typedef struct lcg_pair lcg_pair;
 typedef struct lcg_batch lcg_batch;
@@ -220,7 +230,11 @@ int lcg_batch_run(lcg_batch *batch, int32_t mode, int32_t trace, float *device_m
 /* launch configuration chosen for the batch: waves per workgroup,
  * workgroups, LDS bytes per workgroup */
 int lcg_batch_config(const lcg_batch *batch, int32_t *waves, int32_t *workgroups, int32_t *lds_bytes);
-/* Score of pair k (Aligner::align(), aligner.cc:924-962). */
+/* Score of pair k (Aligner::align(), aligner.cc:924-962).  Minus infinity
+ * (the reference's normalized InftyInt -inf, infty_int.cc:7-23: no
+ * alignment reaches the end cell within the band, possible with free end
+ * gaps) is reported as -2^29; every value below -2^28 means minus infinity,
+ * here and in lcg_batch_D. */
 int lcg_batch_score(const lcg_batch *batch, int32_t k, int64_t *score);
 /* Alignment of pair k (Aligner::trace() + get_alignment(),
  * aligner.cc:1345-1363): edges a[t], b[t] (1-based positions, -1 = gap) in

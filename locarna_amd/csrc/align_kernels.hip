@@ -31,6 +31,8 @@
 // are normalized to LCG_NEG, which leaves every finite result unchanged.
 #include <hip/hip_runtime.h>
 
+#include <climits>
+
 #include "device_layout.h"
 
 namespace {
@@ -127,8 +129,8 @@ struct TopBest {
 // GLOBAL: the region is global scratch; TOP_LOCAL: the sequence-local top
 // level (values clamped at 0, first maximum tracked).
 template <bool GLOBAL, bool TOP_LOCAL>
-__device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W, int *base, bool global_init,
-                           TopBest *best, unsigned long long *counts) {
+__device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W, int *base, bool global_col,
+                           bool global_row, TopBest *best, unsigned long long *counts) {
     const int lane = lane_id();
     const int alpha = P.alpha;
     int *Est = base;
@@ -136,14 +138,15 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
     int *Mst = Srow + W;
     unsigned cnt_cells = 0, cnt_cand = 0;
     for (int c = lane; c < br - bl; c += 64) Est[c] = LCG_NEG;
-    // row al: M(al, bl) = 0, M(al, j) = alpha + sum gapB (global) or 0
+    // row al: M(al, bl) = 0, M(al, j) = alpha + sum gapB (global_row) or 0;
+    // column bl likewise with global_col (init_state, aligner.cc:264-368)
     const int hi0 = row_hi(P, al, br);
     {
-        int carry = global_init ? alpha : 0;
+        int carry = global_row ? alpha : 0;
         for (int j0 = bl + 1; j0 <= hi0; j0 += 64) {
             const int j = j0 + lane;
             const bool act = j <= hi0;
-            const int g = (act && global_init) ? P.gapB[j] : 0;
+            const int g = (act && global_row) ? P.gapB[j] : 0;
             const int ps = wave_scan_add(g);
             if (act) Mst[j - bl] = carry + ps;
             carry += readlane(ps, 63);
@@ -151,7 +154,7 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
         if (lane == 0) Mst[0] = 0;
     }
     region_sync<GLOBAL>();
-    int colv = global_init ? alpha : 0;
+    int colv = global_col ? alpha : 0;
     bool colopen = true;
     int lo_p = bl, hi_p = hi0;
     // Candidate entries of row i are fetched one row ahead: the loads only
@@ -178,7 +181,7 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
         const int *prow = row - W;
         const int mni = P.mn[i];
         if (colopen && mni <= bl) {
-            if (global_init) colv += P.gapA[i];
+            if (global_col) colv += P.gapA[i];
         } else {
             colopen = false;
         }
@@ -473,7 +476,7 @@ __device__ void fill_group(const Pair &P, int al, int bl, int ar, int br, int W,
                            unsigned long long *counts, bool fixed_cols) {
     const int cols = br - bl - 1, rows = ar - al - 1;
     if (!fixed_cols)
-        group_fill<GLOBAL, false>(P, al, bl, ar, br, W, base, true, nullptr, counts);
+        group_fill<GLOBAL, false>(P, al, bl, ar, br, W, base, true, true, nullptr, counts);
     else if (cols <= 64 && rows <= 128)
         group_fill_fixed<GLOBAL, 1, 2>(P, al, bl, ar, br, W, base, counts);
     else if (cols <= 128 && rows <= 128)
@@ -481,7 +484,7 @@ __device__ void fill_group(const Pair &P, int al, int bl, int ar, int br, int W,
     else if (WIDE && cols <= 256 && rows <= 256)
         group_fill_fixed<GLOBAL, 4, 4>(P, al, bl, ar, br, W, base, counts);
     else
-        group_fill<GLOBAL, false>(P, al, bl, ar, br, W, base, true, nullptr, counts);
+        group_fill<GLOBAL, false>(P, al, bl, ar, br, W, base, true, true, nullptr, counts);
 }
 
 // stored value of cell (i, j) of the group (origin al, bl; bound br)
@@ -502,19 +505,23 @@ __device__ __forceinline__ void emit(const Pair &P, int a, int b, int slot) { P.
 // *tcount in LDS).
 template <bool GLOBAL>
 __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W, int al, int bl, int br, int i, int j, bool top,
-                            bool local, int *tstack, int *tcount) {
+                            bool local, int free_endgaps, int *tstack, int *tcount) {
     const int lane = lane_id();
     region_sync<GLOBAL>();
     for (;;) {
-        const int Mij = Mget(P, Mst, W, al, bl, br, i, j);
+        // a top-level cell outside the band was never written and holds the
+        // matrix's initial 0 (InftyInt(), infty_int.hh:80, 348): the trace
+        // can start there when free end gaps left (n, m) out of the band
+        const int Mij = (top && i > al && j > bl && !valid(P, i, j)) ? 0 : Mget(P, Mst, W, al, bl, br, i, j);
         if (top && local && Mij == 0) break;
+        // no padding of free leading gaps at the top level (aligner.cc:1260-1289)
         if (i <= al) {
-            if (!(top && local))
+            if (!(top && (local || (free_endgaps & 1))))
                 for (int k = bl + 1 + lane; k <= j; k += 64) emit(P, 0, k, al + k);
             break;
         }
         if (j <= bl) {
-            if (!(top && local))
+            if (!(top && (local || (free_endgaps & 4))))
                 for (int k = al + 1 + lane; k <= i; k += 64) emit(P, k, 0, k + bl);
             break;
         }
@@ -586,7 +593,9 @@ __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W
                 continue;
             }
         }
-        if (!valid(P, i - 1, j - 1)) break;
+        // no arc match ends outside the band (arc_matches.cc:19-48), where
+        // only a top-level trace can start (free end gaps)
+        if (!valid(P, i - 1, j - 1) || !valid(P, i, j)) break;
         // arc matches with right ends (i, j) in list order (al desc, bl desc)
         const int cell = P.coff[i] + (j - P.mn[i]);
         const int t0 = P.cptr[cell], t1 = P.cptr[cell + 1];
@@ -641,7 +650,7 @@ template <bool GLOBAL>
 __device__ __forceinline__ void fill_and_trace(const Pair &P, int al, int bl, int ar, int br, int W, int *base, int *tstack,
                                int *tcount) {
     fill_group<GLOBAL, false>(P, al, bl, ar, br, W, base, nullptr, true);
-    group_trace<GLOBAL>(P, base + (br - bl) + W, W, al, bl, br, ar - 1, br - 1, false, false, tstack, tcount);
+    group_trace<GLOBAL>(P, base + (br - bl) + W, W, al, bl, br, ar - 1, br - 1, false, false, 0, tstack, tcount);
 }
 
 }  // namespace
@@ -845,19 +854,59 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
         int *topM = topbase + (m + 1) + d.wtop;
         if (wave == 0) {
             TopBest best{0, 0, 0};
+            const int fe = bp.free_endgaps;
             if (bp.local) {
                 if (top_in_lds)
-                    group_fill<false, true>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, false, &best, counts);
+                    group_fill<false, true>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, false, false, &best, counts);
                 else
-                    group_fill<true, true>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, false, &best, counts);
+                    group_fill<true, true>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, false, false, &best, counts);
             } else {
+                // free end gaps: column 0 free with the second sequence's
+                // left end, row 0 with the first's (aligner.cc:736-745)
+                const bool gcol = !(fe & 4), grow = !(fe & 1);
                 if (top_in_lds)
-                    group_fill<false, false>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, true, nullptr, counts);
+                    group_fill<false, false>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, gcol, grow, nullptr, counts);
                 else
-                    group_fill<true, false>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, true, nullptr, counts);
-                best.score = Mget(P, topM, d.wtop, 0, 0, m + 1, n, m);
+                    group_fill<true, false>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, gcol, grow, nullptr, counts);
+                if (top_in_lds)
+                    region_sync<false>();
+                else
+                    region_sync<true>();
+                // M(n, m) of the reference's matrix; outside the band (free
+                // end gaps can cut it out) its never-written initial 0
+                best.score = (P.mn[n] <= m && m <= P.mx[n]) ? Mget(P, topM, d.wtop, 0, 0, m + 1, n, m) : 0;
                 best.i = n;
                 best.j = m;
+                if (fe & (2 | 8)) {
+                    // first strict maximum of the last column (second
+                    // sequence's right end; rows with max_col >= m,
+                    // ascending), then of the last row (first sequence's
+                    // right end, band columns ascending) (aligner.cc:775-816)
+                    best.score = LCG_NEG;
+                    best.i = best.j = 0;
+                    auto scan = [&](int cnt, bool col, int base) {
+                        for (int t0 = 0; t0 < cnt; t0 += 64) {
+                            const int t = t0 + lane;
+                            const int ci = col ? base + t : n, cj = col ? m : base + t;
+                            const bool ok = t < cnt && (!col || P.mx[ci] >= m);
+                            const int v = ok ? Mget(P, topM, d.wtop, 0, 0, m + 1, ci, cj) : INT_MIN;
+                            int wmax = v;
+                            for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
+                            if (wmax > best.score) {
+                                const unsigned long long bal = __ballot(ok && v == wmax);
+                                const int l = (int)__ffsll((long long)bal) - 1;
+                                best.score = wmax;
+                                best.i = col ? base + t0 + l : n;
+                                best.j = col ? m : base + t0 + l;
+                            }
+                        }
+                    };
+                    if (fe & 8) scan(n, true, 1);
+                    if (fe & 2) {
+                        const int j0 = max(1, P.mn[n]), j1 = min(m, P.mx[n]);
+                        if (j1 >= j0) scan(j1 - j0 + 1, false, j0);
+                    }
+                }
             }
             if (lane == 0) {
                 arena[d.out + 0] = best.score;
@@ -868,10 +917,10 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
             if (bp.trace) {
                 if (top_in_lds)
                     group_trace<false>(P, topM, d.wtop, 0, 0, m + 1, best.i, best.j, true, bp.local != 0,
-                                       arena + d.tstack, &s_tcount);
+                                       bp.free_endgaps, arena + d.tstack, &s_tcount);
                 else
                     group_trace<true>(P, topM, d.wtop, 0, 0, m + 1, best.i, best.j, true, bp.local != 0,
-                                      arena + d.tstack, &s_tcount);
+                                      bp.free_endgaps, arena + d.tstack, &s_tcount);
             }
         }
         __threadfence_block();

@@ -465,6 +465,9 @@ namespace {
 // score parameters and the unrestricted band.
 void prepare_parse(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair &P) {
     P.params = *p;
+    if (p->sequ_local && p->free_endgaps)  // src/locarna.cc:375-378
+        throw failure("Free endgaps cannot be combined with local alignment.");
+    if (p->free_endgaps & ~15) throw failure("free end gaps: bits 1, 2, 4, 8 only");
     P.A.reset(new RnaData(ppA, p->min_prob, p->max_bp_span));
     P.B.reset(new RnaData(ppB, p->min_prob, p->max_bp_span));
     if (P.A->length() >= 0xffff || P.B->length() >= 0xffff) throw failure("sequence too long");
@@ -486,7 +489,7 @@ void prepare_parse(const char *ppA, const char *ppB, const lcg_params *p, lcg_pa
 void prepare_band_host(lcg_pair &P) {
     const lcg_params *p = &P.params;
     restrict_trace_by_probabilities(*P.tc, *P.A, *P.B, P.sp, p->min_trace_prob, p->band_extended != 0,
-                                    p->sequ_local != 0);
+                                    p->sequ_local != 0, p->free_endgaps);
 }
 
 // Stage 3: base pairs, arc matches (host) or compaction inputs (device),
@@ -1155,8 +1158,9 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
     const bool deferred = pairs[0]->deferred;
     for (int k = 0; k < npairs; ++k) {
         const lcg_pair *P = pairs[k];
-        if (P->params.indel_opening != p0.indel_opening || P->params.sequ_local != p0.sequ_local) {
-            set_err(err, errlen, "all pairs of a batch must share indel_opening and sequ_local");
+        if (P->params.indel_opening != p0.indel_opening || P->params.sequ_local != p0.sequ_local ||
+            P->params.free_endgaps != p0.free_endgaps) {
+            set_err(err, errlen, "all pairs of a batch must share indel_opening, sequ_local and free_endgaps");
             return LCG_E_ARG;
         }
         if (P->deferred != deferred) {
@@ -1236,6 +1240,7 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
     }
     B->bp.alpha = p0.indel_opening;
     B->bp.local = p0.sequ_local;
+    B->bp.free_endgaps = p0.free_endgaps;
     B->bp.trace = 1;
     B->bp.npairs = npairs;
     B->bp.hdr_ints = B->hdr;
@@ -1759,6 +1764,7 @@ void pf_tables(const lcg_batch &B, int k, const double *dth, double *A) {
 
 extern "C" int lcg_batch_pf_run(lcg_batch *B, const double *dtheta, float *device_ms) {
     if (!B) return LCG_E_ARG;
+    if (B->bp.free_endgaps) return LCG_E_ARG;  // free end gaps: integer path only
     if (B->device_built) return LCG_E_STATE;  // LocARNA-P reads the host arc-match lists
     if (int rc = pf_setup(*B)) return rc;
     for (int k = 0; k < B->npairs; ++k) pf_tables(*B, k, dtheta, B->pf_arena.data());
@@ -2723,7 +2729,9 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
         if (dev_band) {
             std::vector<int> todo, recheck;
             for (int k = 0; k < npairs; ++k) {
-                if (P[k]->A->length() <= LCG_BAND_MAX_LEN && P[k]->B->length() <= LCG_BAND_MAX_LEN)
+                // the kernel has no free end gaps: such pairs are banded on the host
+                if (!p->free_endgaps && P[k]->A->length() <= LCG_BAND_MAX_LEN &&
+                    P[k]->B->length() <= LCG_BAND_MAX_LEN)
                     todo.push_back(k);
                 else
                     host_band.push_back(k);

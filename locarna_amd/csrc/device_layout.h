@@ -91,5 +91,6 @@ typedef struct {
     int32_t fptr_lds;        // LDS offsets (ints) of the pair's fptr and fblk
     int32_t fblk_lds;
     int32_t wg_per_pair;     // workgroups sharing one pair in the fill of D
-    int32_t pad[3];
+    int32_t free_endgaps;    // top level: LCG_FREE_* bits (left_1 1, right_1 2, left_2 4, right_2 8)
+    int32_t pad[2];
 } BatchParams;

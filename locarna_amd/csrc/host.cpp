@@ -255,6 +255,7 @@ struct GotohPF {
     int n, m;
     bool local;
     double g_open, g_ext;
+    int free_endgaps;  // LCG_FREE_* bits of this orientation
     std::vector<T> M, A, B;  // (n+1)*(m+1)
     T &at(std::vector<T> &v, int i, int j) { return v[(size_t)i * (m + 1) + j]; }
 };
@@ -276,6 +277,12 @@ void gotoh_forward(GotohPF<T> &g, const std::vector<double> &match_w, const std:
         g.at(g.A, i, 0) = g.at(g.A, i - 1, 0) * ge;
     }
     for (int j = std::max(mn[0], 2); j <= std::min(mx[0], m); ++j) g.at(g.B, 0, j) = g.at(g.B, 0, j - 1) * ge;
+    // free end gaps (edge_probs.icc:120-131): left end of the second
+    // sequence (bit 4) on column 0, of the first (bit 1) on row 0
+    if (g.free_endgaps & 4)
+        for (int i = 1; i <= n; ++i) g.at(g.A, i, 0) += 1;
+    if (g.free_endgaps & 1)
+        for (int j = 1; j <= m; ++j) g.at(g.B, 0, j) += 1;
     const double lw = 0;
     for (int i = 1; i <= n; ++i) {
         for (int j = std::max(mn[i], 1); j <= std::min(mx[i], m); ++j) {
@@ -290,7 +297,7 @@ void gotoh_forward(GotohPF<T> &g, const std::vector<double> &match_w, const std:
 
 template <class T>
 std::vector<double> trace_probs(const RnaData &RA, const RnaData &RB, const TraceController &tc,
-                                const ScoringParams &sp, bool local) {
+                                const ScoringParams &sp, bool local, int fe) {
     const int n = RA.length(), m = RB.length();
     const double temp = sp.temperature_alipf / 100.0;
     const double gap_open = sp.indel_opening / 100.0, gap_ext = sp.indel / 100.0;
@@ -324,7 +331,7 @@ std::vector<double> trace_probs(const RnaData &RA, const RnaData &RB, const Trac
             fw[(size_t)i * (m + 1) + j] = std::exp(s / temp);
         }
     }
-    GotohPF<T> F{n, m, local, std::exp(gap_open / temp), std::exp(gap_ext / temp)};
+    GotohPF<T> F{n, m, local, std::exp(gap_open / temp), std::exp(gap_ext / temp), fe};
     gotoh_forward(F, fw, tc.min_cols(), tc.max_cols());
     // reversed problem: cell (i', j') of the reversed matrices <-> (n-i', m-j')
     std::vector<double> rw((size_t)(n + 1) * (m + 1), 0.0);
@@ -335,7 +342,9 @@ std::vector<double> trace_probs(const RnaData &RA, const RnaData &RB, const Trac
         rmx[n - i] = m - tc.min_col(i);
         rmn[n - i] = m - tc.max_col(i);
     }
-    GotohPF<T> R{n, m, local, F.g_open, F.g_ext};
+    // the reversed sequences' ends: left and right swapped (FreeEndgaps::reverse)
+    const int rfe = ((fe & 2) >> 1) | ((fe & 1) << 1) | ((fe & 8) >> 1) | ((fe & 4) << 1);
+    GotohPF<T> R{n, m, local, F.g_open, F.g_ext, rfe};
     gotoh_forward(R, rw, rmn, rmx);
     T z;
     if (local) {
@@ -344,6 +353,11 @@ std::vector<double> trace_probs(const RnaData &RA, const RnaData &RB, const Trac
             for (int j = 0; j <= m; ++j) z += F.at(F.M, i, j);
     } else {
         z = F.at(F.M, n, m) + F.at(F.A, n, m) + F.at(F.B, n, m);
+        // edge_probs.icc:39-53: under the forward left flags
+        if (fe & 4)
+            for (int i = 0; i <= n; ++i) z += F.at(F.A, i, m);
+        if (fe & 1)
+            for (int j = 0; j <= m; ++j) z += F.at(F.B, n, j);
     }
     std::vector<double> probs((size_t)(n + 1) * (m + 1), 0.0);
     const double locality_add = local ? 1 : 0;
@@ -364,10 +378,10 @@ std::vector<double> trace_probs(const RnaData &RA, const RnaData &RB, const Trac
 
 void restrict_trace_by_probabilities(TraceController &tc, const RnaData &A, const RnaData &B,
                                      const ScoringParams &sp, double min_trace_prob, bool extended_precision,
-                                     bool local) {
+                                     bool local, int free_endgaps) {
     if (!(min_trace_prob > 0.0)) return;
-    std::vector<double> probs = extended_precision ? trace_probs<long double>(A, B, tc, sp, local)
-                                                   : trace_probs<double>(A, B, tc, sp, local);
+    std::vector<double> probs = extended_precision ? trace_probs<long double>(A, B, tc, sp, local, free_endgaps)
+                                                   : trace_probs<double>(A, B, tc, sp, local, free_endgaps);
     tc.restrict_by_trace_probabilities(probs, min_trace_prob);
 }
 

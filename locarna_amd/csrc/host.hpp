@@ -118,12 +118,13 @@ int nt_code(char c);
 void band_similarity(const ScoringParams &sp, double sim[16]);
 
 // Trace-probability band: Gotoh partition function over the StralScore
-// similarity, forward and backward, then the probability cut and the
-// monotone fix-ups.  reference: edge_probs.icc:7-162, :212-268;
+// similarity, forward and backward (free end gaps: LCG_FREE_* bits), then
+// the probability cut and the monotone fix-ups.  reference:
+// edge_probs.icc:7-162, :212-268;
 // stral_score.cc:29-60; main_helper.icc:371-426
 void restrict_trace_by_probabilities(TraceController &tc, const RnaData &A, const RnaData &B,
                                      const ScoringParams &sp, double min_trace_prob,
-                                     bool extended_precision, bool local);
+                                     bool extended_precision, bool local, int free_endgaps = 0);
 
 struct ArcMatch {
     int idx;

@@ -297,17 +297,32 @@ struct StralScore {
     }
 };
 
-// edge_probs.icc:7-162 (PFGotoh), :212-268 (PFTraceProbs); no free end gaps,
-// global (locarna's band never uses the local variant for global alignment;
-// flag_local = sequ_local).
+// edge_probs.icc:7-162 (PFGotoh), :212-268 (PFTraceProbs); flag_local =
+// sequ_local (locarna's band never uses the local variant for global
+// alignment).  Free end gaps (free_endgaps.hh, bits as oracle_params): the
+// forward pass adds 1 to A(i, 0) with the second sequence's left flag and to
+// B(0, j) with the first's (edge_probs.icc:120-131), the reverse pass does
+// the same with the flags of the reversed sequences (left and right
+// swapped, FreeEndgaps::reverse), and the global Z adds the A entries of the
+// last column and the B entries of the last row under the same left flags
+// of the forward pass (:39-53).
+inline bool fe_left_1(int f) { return f & 1; }
+inline bool fe_right_1(int f) { return f & 2; }
+inline bool fe_left_2(int f) { return f & 4; }
+inline bool fe_right_2(int f) { return f & 8; }
+inline int fe_reverse(int f) {  // {right_1, left_1, right_2, left_2}
+    return (fe_right_1(f) ? 1 : 0) | (fe_left_1(f) ? 2 : 0) | (fe_right_2(f) ? 4 : 0) | (fe_left_2(f) ? 8 : 0);
+}
+
 template <class pf_t>
 struct PFGotoh {
     int lenA, lenB;
     double temp;
     bool local;
+    int fe;
     Mat<pf_t> zM, zA, zB, zMr, zAr, zBr;
     pf_t z;
-    void pf_gotoh(Mat<pf_t> &zMl, Mat<pf_t> &zAl, Mat<pf_t> &zBl, const Band &tc, const StralScore &score) {
+    void pf_gotoh(Mat<pf_t> &zMl, Mat<pf_t> &zAl, Mat<pf_t> &zBl, const Band &tc, const StralScore &score, int f) {
         double g_open = exp(score.indel_opening / temp);
         double g_ext = exp(score.indel / temp);
         zMl.resize(lenA + 1, lenB + 1, 0);
@@ -322,6 +337,10 @@ struct PFGotoh {
         }
         for (size_t j = std::max(tc.min_col[0], (size_t)2); j <= std::min(tc.max_col[0], (size_t)lenB); j++)
             zBl(0, j) = zBl(0, j - 1) * g_ext;
+        if (fe_left_2(f))
+            for (int i = 1; i <= lenA; i++) zAl(i, 0) += 1;
+        if (fe_left_1(f))
+            for (int j = 1; j <= lenB; j++) zBl(0, j) += 1;
         for (int i = 1; i <= lenA; i++) {
             for (size_t j = std::max(tc.min_col[i], (size_t)1); j <= std::min(tc.max_col[i], (size_t)lenB); j++) {
                 double match_score_ij = score.sigma(i, j);
@@ -334,18 +353,22 @@ struct PFGotoh {
         }
     }
     PFGotoh(const RnaData &a, const RnaData &b, const Band &tc, const double (*sim)[4], double gap_opening,
-            double gap_extension, double sw, double temp_, bool local_)
-        : lenA(a.length()), lenB(b.length()), temp(temp_), local(local_) {
+            double gap_extension, double sw, double temp_, bool local_, int fe_)
+        : lenA(a.length()), lenB(b.length()), temp(temp_), local(local_), fe(fe_) {
         StralScore score(a, b, sim, sw, gap_opening, gap_extension);
-        pf_gotoh(zM, zA, zB, tc, score);
+        pf_gotoh(zM, zA, zB, tc, score, fe);
         score.reverse();
-        pf_gotoh(zMr, zAr, zBr, tc.reverse(), score);
+        pf_gotoh(zMr, zAr, zBr, tc.reverse(), score, fe_reverse(fe));
         if (local) {
             z = 1;
             for (int i = 0; i <= lenA; i++)
                 for (int j = 0; j <= lenB; j++) z += zM(i, j);
         } else {
             z = zM(lenA, lenB) + zA(lenA, lenB) + zB(lenA, lenB);
+            if (fe_left_2(fe))
+                for (int i = 0; i <= lenA; i++) z += zA(i, lenB);
+            if (fe_left_1(fe))
+                for (int j = 0; j <= lenB; j++) z += zB(lenA, j);
         }
     }
     bool fail() const { return z == (pf_t)0 || std::isnan((double)z) || std::isinf((double)z); }
@@ -353,8 +376,8 @@ struct PFGotoh {
 
 template <class pf_t>
 Mat<double> trace_probs(const RnaData &a, const RnaData &b, const Band &tc, const double (*sim)[4],
-                        double gap_opening, double gap_extension, double sw, double temp, bool local) {
-    PFGotoh<pf_t> g(a, b, tc, sim, gap_opening, gap_extension, sw, temp, local);
+                        double gap_opening, double gap_extension, double sw, double temp, bool local, int fe) {
+    PFGotoh<pf_t> g(a, b, tc, sim, gap_opening, gap_extension, sw, temp, local, fe);
     Mat<double> probs;
     int lenA = a.length(), lenB = b.length();
     probs.resize(lenA + 1, lenB + 1, 0.0);
@@ -526,6 +549,8 @@ struct Inputs {
 
 // src/locarna.cc:398-703 and src/locarna_p.cc:300-477 (parameter plumbing)
 std::unique_ptr<Inputs> prepare(const char *ppA, const char *ppB, const oracle_params &p, const char *rib_path) {
+    if (p.sequ_local && p.free_endgaps)  // locarna.cc:375-378
+        throw std::runtime_error("Free endgaps cannot be combined with local alignment.");
     auto in = std::make_unique<Inputs>();
     in->p = p;
     if (p.use_ribosum) in->rib.reset(new Ribosum(read_ribosum(rib_path)));
@@ -547,10 +572,10 @@ std::unique_ptr<Inputs> prepare(const char *ppA, const char *ppB, const oracle_p
         Mat<double> probs;
         if (p.band_extended)
             probs = trace_probs<long double>(in->A, in->B, *in->band, simm, p.indel_opening / 100.0, p.indel / 100.0,
-                                             p.struct_weight / 100.0, p.temperature_alipf / 100.0, p.sequ_local);
+                                             p.struct_weight / 100.0, p.temperature_alipf / 100.0, p.sequ_local, p.free_endgaps);
         else
             probs = trace_probs<double>(in->A, in->B, *in->band, simm, p.indel_opening / 100.0, p.indel / 100.0,
-                                        p.struct_weight / 100.0, p.temperature_alipf / 100.0, p.sequ_local);
+                                        p.struct_weight / 100.0, p.temperature_alipf / 100.0, p.sequ_local, p.free_endgaps);
         restrict_by_trace_probabilities(*in->band, probs, p.min_trace_prob);
     }
     in->bpsA.reset(new BasePairs(in->A, p.min_prob));
@@ -562,7 +587,7 @@ std::unique_ptr<Inputs> prepare(const char *ppA, const char *ppB, const oracle_p
 
 // ---------------------------------------------------------------- Aligner
 // aligner.cc (state E_NO_NO only: no structure locality, no anchors, no
-// lonely-pair or stacking variants, no free end gaps)
+// lonely-pair or stacking variants; free end gaps at the top level)
 struct Aligner {
     Inputs &in;
     const Band &tc;
@@ -680,18 +705,49 @@ struct Aligner {
         }
     }
 
-    // aligner.cc:736-819 (free end gaps "----")
+    // aligner.cc:736-819: column init free with the second sequence's left
+    // flag, row init with the first's; with a right flag the best of the
+    // last column (second sequence's right end, rows ascending) and then of
+    // the last row (first sequence's right end, columns ascending), first
+    // strict maximum (no anchors: the rightmost anchor is (0, 0))
     sc_t align_top_level_global() {
-        init_state(0, lenA + 1, 0, lenB + 1, true, true);
+        const int fe = in.p.free_endgaps;
+        init_state(0, lenA + 1, 0, lenB + 1, !fe_left_2(fe), !fe_left_1(fe));
         for (size_t i = 1; i <= (size_t)lenA; i++) {
             F = NEG_INF;
             size_t min_col = std::max((size_t)1, tc.min_col[i]);
             size_t max_col = std::min((size_t)lenB, tc.max_col[i]);
             for (size_t j = min_col; j <= max_col; j++) M(i, j) = align_noex(0, 0, i, j);
         }
+        sc_t max_score = M(lenA, lenB);
         max_i = lenA;
         max_j = lenB;
-        return M(lenA, lenB);
+        if (fe_right_1(fe) || fe_right_2(fe)) {
+            max_score = NEG_INF;
+            max_i = 0;
+            max_j = 0;
+        }
+        if (fe_right_2(fe)) {
+            for (size_t i = 1; i <= (size_t)lenA; i++) {
+                if (tc.max_col[i] >= (size_t)lenB && M(i, lenB) > max_score) {
+                    max_score = M(i, lenB);
+                    max_i = (int)i;
+                    max_j = lenB;
+                }
+            }
+        }
+        if (fe_right_1(fe)) {
+            size_t min_col = std::max((size_t)1, tc.min_col[lenA]);
+            size_t max_col = std::min((size_t)lenB, tc.max_col[lenA]);
+            for (size_t j = min_col; j <= max_col; j++) {
+                if (M(lenA, j) > max_score) {
+                    max_score = M(lenA, j);
+                    max_i = lenA;
+                    max_j = (int)j;
+                }
+            }
+        }
+        return max_score;
     }
 
     // aligner.cc:823-880 (no anchors: left anchor (lenA+1,lenB+1), right (0,0))
@@ -790,13 +846,14 @@ struct Aligner {
 
     void trace_in_arcmatch(size_t al, size_t i, size_t bl, size_t j, bool tl) {
         if (tl && in.p.sequ_local && M(i, j) == 0) return;
+        // aligner.cc:1260-1289: no padding of free leading end gaps
         if (i <= al) {
-            if (!(tl && in.p.sequ_local))
+            if (!(tl && (in.p.sequ_local || fe_left_1(in.p.free_endgaps))))
                 for (size_t k = bl + 1; k <= j; k++) append(-1, (int)k);
             return;
         }
         if (j <= bl) {
-            if (!(tl && in.p.sequ_local))
+            if (!(tl && (in.p.sequ_local || fe_left_2(in.p.free_endgaps))))
                 for (size_t k = al + 1; k <= i; k++) append((int)k, -1);
             return;
         }
@@ -1140,8 +1197,9 @@ extern "C" int oracle_trace_probs(const char *ppA, const char *ppB, const oracle
         const Band band(n, m);
         const double go = p->indel_opening / 100.0, ge = p->indel / 100.0, sw = p->struct_weight / 100.0;
         const double T = p->temperature_alipf / 100.0;
-        Mat<double> pr = p->band_extended ? trace_probs<long double>(A, B, band, simm, go, ge, sw, T, p->sequ_local)
-                                          : trace_probs<double>(A, B, band, simm, go, ge, sw, T, p->sequ_local);
+        Mat<double> pr = p->band_extended
+                             ? trace_probs<long double>(A, B, band, simm, go, ge, sw, T, p->sequ_local, p->free_endgaps)
+                             : trace_probs<double>(A, B, band, simm, go, ge, sw, T, p->sequ_local, p->free_endgaps);
         StralScore sc(A, B, simm, sw, go, ge);
         for (int i = 0; i <= n; ++i)
             for (int j = 0; j <= m; ++j) {
@@ -1263,6 +1321,7 @@ extern "C" int oracle_align_many(const char **ppA, const char **ppB, int npairs,
 extern "C" oracle_result *oracle_pf(const char *ppA, const char *ppB, const oracle_params *p,
                                     const char *ribosum_path, int include_am_in_bm, char *err, int errlen) {
     try {
+        if (p->free_endgaps) throw std::runtime_error("free end gaps: integer path only");
         auto *r = new oracle_result();
         r->in = orc::prepare(ppA, ppB, *p, ribosum_path);
         orc::AlignerP ap(*r->in);

@@ -12,7 +12,12 @@
  *   - trace band (PFGotoh, PFTraceProbs): enumeration of all alignments
  *     (global) or of all prefixes and suffixes (sequence-local) on tiny
  *     inputs, trace probabilities to 1e-12 in long double and double
- *     (tests/test_oracle_band.py, through oracle_trace_probs);
+ *     (tests/test_oracle_band.py, through oracle_trace_probs); with free
+ *     end gaps, enumeration of the prefix paths of the forward and reversed
+ *     problems (free starts included) and of Z;
+ *   - free end gaps of the integer DP: exhaustive enumeration with free
+ *     leading runs and free ends (score and traceback, 9 end
+ *     specifications);
  *   - integer DP: exhaustive enumeration of all alignments x nested
  *     arc-match sets on tiny inputs (max score and the traceback's tie rule),
  *     reduction to textbook Gotoh / Smith-Waterman-Gotoh without arcs,
@@ -41,6 +46,9 @@ typedef struct {
     int max_bp_span;       /* --maxBPspan, -1 = unlimited            */
     int band_extended;     /* 1: band PF in long double (locarna), 0: double (locarna_p) */
     int use_ribosum;       /* 1: RIBOSUM85_60, 0: match/mismatch      */
+    int free_endgaps;      /* --free-endgaps: bit 0 left end of the first
+                              sequence, 1 its right end, 2 left end of the
+                              second, 3 its right end (free_endgaps.hh) */
     double min_prob;       /* --min-prob                             */
     double min_trace_prob; /* --min-trace-probability (<=0: no band)  */
     double exp_prob;       /* --exp-prob; <=0: 1/(2 len)              */

@@ -22,8 +22,14 @@ import math
 
 class Problem:
     def __init__(self, lenA, lenB, sigma, gapA, gapB, alpha, arcmatches, am_score,
-                 min_col, max_col, local=False):
+                 min_col, max_col, local=False, free=0):
         self.n, self.m = lenA, lenB
+        # free end gaps (locarna --free-endgaps as bits: 1 left end of the
+        # first sequence, 2 its right end, 4 left end of the second, 8 its
+        # right end): free leading runs of the top level cost nothing and
+        # are not written; with a right end free the alignment may end in
+        # the last column (second sequence) or the last row (first)
+        self.free = free
         self.sigma = sigma                  # dict (i, j) -> score
         self.gapA, self.gapB = gapA, gapB   # lists indexed 1..n / 1..m
         self.alpha = alpha
@@ -57,16 +63,17 @@ def group_alignments(P, r0, c0, i, j, top, memo):
     if key in memo:
         return memo[key]
     out = []
-    zero_init = top and P.local
+    free_row = top and (P.local or P.free & 1)  # insertions before the first position of A
+    free_col = top and (P.local or P.free & 4)  # deletions before the first position of B
     if (i, j) == (r0, c0):
         out.append((0, (), None))
     elif i == r0:
         if all(P.valid(r0, c) for c in range(c0, j + 1)):
-            sc = 0 if zero_init else P.alpha + sum(P.gapB[c] for c in range(c0 + 1, j + 1))
+            sc = 0 if free_row else P.alpha + sum(P.gapB[c] for c in range(c0 + 1, j + 1))
             out.append((sc, (('pad', i, j),), 'I'))
     elif j == c0:
         if all(P.valid(r, c0) for r in range(r0, i + 1)):
-            sc = 0 if zero_init else P.alpha + sum(P.gapA[r] for r in range(r0 + 1, i + 1))
+            sc = 0 if free_col else P.alpha + sum(P.gapA[r] for r in range(r0 + 1, i + 1))
             out.append((sc, (('pad', i, j),), 'D'))
     elif P.valid(i, j):
         if P.valid(i - 1, j - 1):
@@ -122,10 +129,11 @@ def edges(P, steps, r0, c0, top):
     for s in reversed(steps):
         t = s[0]
         if t == 'pad':
-            if not (top and P.local):
-                if s[1] == r0:
+            if s[1] == r0:
+                if not (top and (P.local or P.free & 1)):
                     out += [(-1, c) for c in range(c0 + 1, s[2] + 1)]
-                else:
+            else:
+                if not (top and (P.local or P.free & 4)):
                     out += [(r, -1) for r in range(r0 + 1, s[1] + 1)]
         elif t == 'M':
             out.append((s[1], s[2]))
@@ -146,11 +154,30 @@ def edges(P, steps, r0, c0, top):
 
 def best_global(P):
     """(max score, traced edges, traced arc-match indices, #alignments)."""
-    alns = group_alignments(P, 0, 0, P.n, P.m, True, {})
-    best = max(sc for sc, _, _ in alns)
+    memo = {}
+    ends = [(P.n, P.m)]
+    if P.free & (2 | 8):
+        # aligner.cc:775-816: the last column for the second sequence's
+        # right end (rows ascending), then the last row for the first's
+        # (columns ascending); the first strict maximum
+        ends = []
+        if P.free & 8:
+            ends += [(i, P.m) for i in range(1, P.n + 1) if P.max_col[i] >= P.m]
+        if P.free & 2:
+            ends += [(P.n, j) for j in range(max(1, P.min_col[P.n]), min(P.m, P.max_col[P.n]) + 1)]
+    best, cell, count = None, None, 0
+    for (i, j) in ends:
+        alns = group_alignments(P, 0, 0, i, j, True, memo)
+        count += len(alns)
+        if not alns:
+            continue
+        sc = max(x for x, _, _ in alns)
+        if best is None or sc > best:
+            best, cell = sc, (i, j)
+    alns = group_alignments(P, 0, 0, cell[0], cell[1], True, memo)
     chosen = min((st for sc, st, _ in alns if sc == best), key=trace_key)
     e, a = edges(P, chosen, 0, 0, True)
-    return best, e, a, len(alns)
+    return best, e, a, count
 
 
 def best_local(P):

@@ -153,3 +153,88 @@ def test_enumeration_catches_a_wrong_gap_rule():
     n, m = len(P) - 1, len(P[0]) - 1
     linear = _brute(S, (1.0, w[1] * w[0] ** 0.5, w[2]), n, m)
     assert max(abs(P[i][j] - linear[i][j]) for i in range(n + 1) for j in range(m + 1)) > 1e-6
+
+
+def _prefix_sums(S, w, n, m, fe):
+    """Forward quantities of the band's Gotoh recursion with free end gaps,
+    by enumeration of prefix paths: F[(i, j, X)] sums the weights of paths
+    ending in (i, j) with step X.  A path starts at (0, 0) (weight 1, state
+    M) or, with the second sequence's left end free, at (i, 0) in state A
+    with weight 1 (i >= 1), or, with the first's, at (0, j) in state B; it
+    continues down column 0 / along row 0 only from (0, 0)."""
+    w_open, w_ext, T = w
+    F = {}
+
+    def walk(i, j, last, weight, chain):
+        F[(i, j, last)] = F.get((i, j, last), 0.0) + weight
+        if i < n and j < m:
+            walk(i + 1, j + 1, "M", weight * math.exp(S[i + 1][j + 1] / T), False)
+        if i < n and (j >= 1 or chain):
+            walk(i + 1, j, "A", weight * w_ext * (w_open if last != "A" else 1.0), chain and j == 0)
+        if j < m and (i >= 1 or chain):
+            walk(i, j + 1, "B", weight * w_ext * (w_open if last != "B" else 1.0), chain and i == 0)
+
+    walk(0, 0, "M", 1.0, True)
+    if fe & 4:
+        for i in range(1, n + 1):
+            walk(i, 0, "A", 1.0, False)
+    if fe & 1:
+        for j in range(1, m + 1):
+            walk(0, j, "B", 1.0, False)
+    return F
+
+
+def _free_probs(S, w, n, m, fe):
+    w_open = w[0]
+    F = _prefix_sums(S, w, n, m, fe)
+    # reversed sequences: similarity mirrored, left and right ends swapped
+    Sr = [[0.0] * (m + 1) for _ in range(n + 1)]
+    for i in range(1, n + 1):
+        for j in range(1, m + 1):
+            Sr[i][j] = S[n + 1 - i][m + 1 - j]
+    rev = ((fe & 2) >> 1) | ((fe & 1) << 1) | ((fe & 8) >> 1) | ((fe & 4) << 1)
+    R = _prefix_sums(Sr, w, n, m, rev)
+    f = lambda d, i, j, x: d.get((i, j, x), 0.0)
+    Z = sum(f(F, n, m, x) for x in "MAB")
+    if fe & 4:
+        Z += sum(f(F, i, m, "A") for i in range(n + 1))
+    if fe & 1:
+        Z += sum(f(F, n, j, "B") for j in range(m + 1))
+    P = [[0.0] * (m + 1) for _ in range(n + 1)]
+    for i in range(n + 1):
+        for j in range(m + 1):
+            rM, rA, rB = (f(R, n - i, m - j, x) for x in "MAB")
+            z = f(F, i, j, "M") * (rM + rA + rB) + f(F, i, j, "A") * (rM + rA / w_open + rB) + \
+                f(F, i, j, "B") * (rM + rA + rB / w_open)
+            P[i][j] = z / Z
+    return P
+
+
+@pytest.mark.parametrize("fe", [1, 2, 4, 8, 5, 10, 15])
+def test_trace_probs_free_endgaps_match_enumeration(fe):
+    # edge_probs.icc:76-162 with free end gaps: 1 added to A(i, 0) / B(0, j)
+    # under the left flags, the reverse pass with the flags of the reversed
+    # sequences, and Z with the last column's A and last row's B entries
+    # (:39-53)
+    for seed in range(4):
+        a, b = tiny_pair(9500 + seed, nA=4 + seed % 2, nB=4 + (seed + 1) % 3)
+        params = oracle.locarna_params(min_prob=0.01, max_bp_span=-1, free_endgaps=fe, indel=-150,
+                                       indel_opening=-200)
+        P, S, w = oracle.trace_probs(a, b, params)
+        n, m = len(P) - 1, len(P[0]) - 1
+        ref = _free_probs(S, w, n, m, fe)
+        for i in range(n + 1):
+            for j in range(m + 1):
+                assert abs(P[i][j] - ref[i][j]) <= 1e-12 * max(1.0, abs(ref[i][j])), (fe, seed, i, j)
+
+
+def test_free_enumeration_reduces_to_global():
+    a, b = tiny_pair(9100, nA=5, nB=4)
+    params = oracle.locarna_params(min_prob=0.01, max_bp_span=-1, indel=-150, indel_opening=-200)
+    P, S, w = oracle.trace_probs(a, b, params)
+    n, m = len(P) - 1, len(P[0]) - 1
+    ref = _free_probs(S, w, n, m, 0)
+    glob = _brute(S, w, n, m)
+    for i in range(n + 1):
+        for j in range(m + 1):
+            assert abs(ref[i][j] - glob[i][j]) <= 1e-12 and abs(P[i][j] - ref[i][j]) <= 1e-12

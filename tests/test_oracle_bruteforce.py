@@ -90,3 +90,26 @@ def test_tie_rich_traceback(case, weights):
     best, edges, arcs, _ = bf.best_global(P)
     assert r.score == best
     assert r.edges == edges
+
+
+@pytest.mark.parametrize("free", ["+---", "-+--", "--+-", "---+", "++++", "+-+-", "-+-+", "++--", "--++"])
+@pytest.mark.parametrize("seed", range(6))
+def test_free_endgaps_optimum_and_traceback(free, seed):
+    # locarna --free-endgaps: free leading / trailing gaps per sequence end
+    # (aligner.cc:736-819 top level, :1260-1289 traceback)
+    bits = sum(1 << k for k, c in enumerate(free) if c == "+")
+    ppA, ppB = tiny_pair(4400 + seed * 13 + bits)
+    params = oracle.locarna_params(min_prob=0.01, max_bp_span=-1, min_trace_prob=0.0, indel=-150,
+                                   indel_opening=-200, free_endgaps=bits)
+    r = oracle.align(ppA, ppB, params)
+    P = bf.Problem(r.lenA, r.lenB, r.sigma, r.gapA, r.gapB, params.indel_opening, r.arcmatches,
+                   r.am_scores, r.min_col, r.max_col, free=bits)
+    best, edges, arcs, nal = bf.best_global(P)
+    assert r.score == best, (free, seed)
+    assert r.edges == edges, (free, seed)
+
+
+def test_free_endgaps_rejected_with_local():
+    ppA, ppB = tiny_pair(1)
+    with pytest.raises(RuntimeError):
+        oracle.align(ppA, ppB, oracle.locarna_params(sequ_local=1, free_endgaps=1))
