@@ -42,7 +42,9 @@ struct BandLds {
     double *upA, *dnA, *upB, *dnB;
     int *cA, *cB;
     int *emax;       // [3] largest exponent of a diagonal (ring)
-    double *zsum;    // [3] sum of M of a diagonal (ring; local mode)
+    double *zsum;    // [3] per-diagonal part of Z (ring): M (local mode) or the
+                     // free-end-gap terms of the last column / row (global)
+    double *chain;   // [3] gap chain of column 0, [3] of row 0 (rings)
     int *misc;       // [0] flags, [1] ze
     double *zmisc;   // [0] zm, [1] global Z mantissa
 };
@@ -95,6 +97,8 @@ extern "C" __global__ void __launch_bounds__(BT) lcg_band_pairs(const BandDesc *
         p += 4;
         S.zmisc = p;
         p += 2;
+        S.chain = p;
+        p += 6;
         int *q = reinterpret_cast<int *>(p);
         S.EF = q;
         q += 2 * L1;
@@ -115,6 +119,14 @@ extern "C" __global__ void __launch_bounds__(BT) lcg_band_pairs(const BandDesc *
     const int tid = threadIdx.x;
     const double go = bp.g_open, ge = bp.g_ext, goge = go * ge;
     const int rlim = bp.extended ? 16000 : 1000;
+    // free end gaps: the forward pass takes the flags as given, the reverse
+    // pass those of the reversed sequences (left and right swapped); the
+    // global Z gains the last column's A (second sequence's left flag) and
+    // the last row's B (first sequence's left flag) of the forward pass
+    // (edge_probs.icc:39-53, :120-131)
+    const int fe_fwd = bp.free_endgaps;
+    const int fe_rev = ((fe_fwd & 2) >> 1) | ((fe_fwd & 1) << 1) | ((fe_fwd & 8) >> 1) | ((fe_fwd & 4) << 1);
+    const bool zextra = !bp.local && (fe_fwd & 5);
     for (;;) {
         if (tid == 0) s_pair = atomicAdd(queue, 1);
         __syncthreads();
@@ -151,6 +163,8 @@ extern "C" __global__ void __launch_bounds__(BT) lcg_band_pairs(const BandDesc *
         for (int pass = 0; pass < 2; ++pass) {
             const bool rev = pass == 1;
             int *E = rev ? S.ER : S.EF;
+            const int fe = rev ? fe_rev : fe_fwd;
+            const bool zacc = !rev && (bp.local || zextra);
             // band of this orientation: reversed cell (i', j') is cell
             // (n - i', m - j'), reversed row i' spans m - max_col(n - i')
             // .. m - min_col(n - i')
@@ -170,7 +184,7 @@ extern "C" __global__ void __launch_bounds__(BT) lcg_band_pairs(const BandDesc *
                 }
                 if (tid == 0) {
                     E[d] = Ed;
-                    if (d > 0 && !rev && bp.local) zadd(zm, ze, S.zsum[r1], E[d - 1]);
+                    if (d > 0 && zacc) zadd(zm, ze, S.zsum[r1], E[d - 1]);
                     S.emax[r2] = INT_MIN;  // slot of d + 1
                     S.zsum[r2] = 0.0;
                 }
@@ -219,26 +233,42 @@ extern "C" __global__ void __launch_bounds__(BT) lcg_band_pairs(const BandDesc *
                     } else if (i == 0 && j == 0) {
                         if (bmn(0) <= 0 && 0 <= bmx(0)) vM = bp.local ? 0.0 : s0;
                     } else if (j == 0) {  // i >= 1
+                        // gap chain of column 0 (kept apart from the free
+                        // start, which the reference adds after the chain)
+                        double ch = 0.0;
                         if (i == 1) {
                             if (bmn(1) <= 0 && 0 <= bmx(1)) {
-                                vA = goge * s0;
-                                under(goge, vA);
+                                ch = goge * s0;
+                                under(goge, ch);
                             }
                         } else if (bmn(i) <= 0) {
-                            const double t = S.A[r1][i - 1] * ge;
-                            vA = t * f1;
-                            under(t, vA);
+                            const double t = S.chain[r1] * ge;
+                            ch = t * f1;
+                            under(t, ch);
+                        }
+                        S.chain[r0] = ch;
+                        vA = ch;
+                        if (fe & 4) {  // free left end of the second sequence: 1 at A(i, 0)
+                            under(1.0, s0);
+                            vA += s0;
                         }
                     } else {  // i == 0, j >= 1
+                        double ch = 0.0;
                         if (j == 1) {
                             if (bmn(0) <= 1 && 1 <= bmx(0)) {
-                                vB = goge * s0;
-                                under(goge, vB);
+                                ch = goge * s0;
+                                under(goge, ch);
                             }
                         } else if (j >= (bmn(0) > 2 ? bmn(0) : 2) && j <= (bmx(0) < m ? bmx(0) : m)) {
-                            const double t = S.B[r1][0] * ge;
-                            vB = t * f1;
-                            under(t, vB);
+                            const double t = S.chain[3 + r1] * ge;
+                            ch = t * f1;
+                            under(t, ch);
+                        }
+                        S.chain[3 + r0] = ch;
+                        vB = ch;
+                        if (fe & 1) {  // free left end of the first sequence: 1 at B(0, j)
+                            under(1.0, s0);
+                            vB += s0;
                         }
                     }
                     S.M[r0][i] = vM;
@@ -255,7 +285,11 @@ extern "C" __global__ void __launch_bounds__(BT) lcg_band_pairs(const BandDesc *
                         c[0] = vM;
                         c[1] = vA;
                         c[2] = vB;
-                        zpart += vM;
+                        if (bp.local) zpart += vM;
+                        if (zextra) {
+                            if (j == m && (fe_fwd & 4)) zpart += vA;
+                            if (i == n && (fe_fwd & 1)) zpart += vB;
+                        }
                     } else {
                         // trace probability of cell (n - i, m - j)
                         const int ci = n - i, cj = m - j;
@@ -288,7 +322,7 @@ extern "C" __global__ void __launch_bounds__(BT) lcg_band_pairs(const BandDesc *
                 }
                 if (xmax != INT_MIN) atomicMax(&S.emax[r0], xmax);
                 if (flags) atomicOr(&S.misc[0], flags);
-                if (!rev && bp.local && zpart != 0.0) atomicAdd(&S.zsum[r0], zpart);
+                if (zacc && zpart != 0.0) atomicAdd(&S.zsum[r0], zpart);
                 if (!rev && !bp.local && tid == 0 && d == n + m) {
                     // the thread of cell (n, m) is the one with i = n on the
                     // last diagonal: i0 = i1 = n there, so thread 0
@@ -301,8 +335,8 @@ extern "C" __global__ void __launch_bounds__(BT) lcg_band_pairs(const BandDesc *
                     zadd(zm, ze, S.zsum[(n + m) % 3], E[n + m]);
                     zadd(zm, ze, 1.0, 0);  // the empty alignment
                 } else {
-                    zm = S.zmisc[1];
-                    ze = E[n + m];
+                    if (zextra) zadd(zm, ze, S.zsum[(n + m) % 3], E[n + m]);
+                    zadd(zm, ze, S.zmisc[1], E[n + m]);  // cell (n, m)
                 }
                 S.zmisc[0] = zm;
                 S.misc[1] = ze;
@@ -335,7 +369,7 @@ int set_band_lds_limit() {
 // Dynamic LDS bytes of lcg_band_pairs for sequences of at most lmax.
 extern "C" size_t lcg_band_lds_bytes(int lmax) {
     const size_t L1 = (size_t)lmax + 1;
-    return (13 * L1 + 6) * sizeof(double) + (8 * L1 + 8) * sizeof(int);
+    return (13 * L1 + 12) * sizeof(double) + (8 * L1 + 8) * sizeof(int);
 }
 
 extern "C" int lcg_launch_band(const BandDesc *descs, const int *order, int *queue, int *iarena,

@@ -39,6 +39,8 @@ struct BandParams {
     int32_t local;     // sequence-local partition function
     int32_t extended;  // host precision long double (1) or double (0)
     int32_t lmax;      // largest length in the batch (LDS arrays)
+    int32_t free_endgaps;  // LCG_FREE_* bits (forward orientation)
+    int32_t pad;
 };
 
 #endif

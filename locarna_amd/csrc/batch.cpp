@@ -2561,6 +2561,7 @@ int device_bands(std::vector<std::unique_ptr<lcg_pair>> &P, std::vector<int> tod
     band_similarity(sp, bp.sim);
     bp.local = p.sequ_local != 0;
     bp.extended = p.band_extended != 0;
+    bp.free_endgaps = p.free_endgaps;
     const int64_t budget = (int64_t)1 << 29;  // doubles (4 GiB)
     size_t c0 = 0;
     while (c0 < todo.size()) {
@@ -2729,9 +2730,7 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
         if (dev_band) {
             std::vector<int> todo, recheck;
             for (int k = 0; k < npairs; ++k) {
-                // the kernel has no free end gaps: such pairs are banded on the host
-                if (!p->free_endgaps && P[k]->A->length() <= LCG_BAND_MAX_LEN &&
-                    P[k]->B->length() <= LCG_BAND_MAX_LEN)
+                if (P[k]->A->length() <= LCG_BAND_MAX_LEN && P[k]->B->length() <= LCG_BAND_MAX_LEN)
                     todo.push_back(k);
                 else
                     host_band.push_back(k);

@@ -75,3 +75,13 @@ def test_band_then_device_compaction_aligns_as_oracle():
         assert [B.score(k) for k in range(len(pp))] == list(scores)
     finally:
         B.destroy()
+
+
+@pytest.mark.parametrize("spec", ["+-+-", "-+-+", "++++", "---+", "+---"])
+def test_band_free_endgaps(spec):
+    # the free-end-gap terms of the band kernel (free starts on row 0 /
+    # column 0, reversed flags in the reverse pass, the extra Z terms)
+    fe = api.free_endgaps(spec)
+    pp, pa, dev, st = _check("allpairs64", lambda **kw: api.locarna_params(free_endgaps=fe, **kw),
+                             "allpairs64 free " + spec, limit=200)
+    assert st["device_banded"] >= len(pp) * 0.95
