@@ -278,6 +278,18 @@ int lcg_batch_kernel_times(const lcg_batch *batch, float *fill_ms, float *top_ms
  * align_inside / align_outside, aligner_p.icc:413-438, 1113-1139). */
 #define LCG_NTHETA 275
 int lcg_batch_pf_run(lcg_batch *batch, const double *dtheta, float *device_ms);
+/* Text of the probability files of locarna_p (AlignerP::
+ * write_arcmatch_probabilities / write_basematch_probabilities,
+ * aligner_p.icc:1404-1435; --write-arcmatch-probs / --write-basematch-probs,
+ * src/locarna_p.cc:499-526) for pair k after lcg_batch_pf_run: "al ar bl br
+ * p" per arc match in index order with p >= min_am_prob, "i j p" per base
+ * match in row-major order with p >= min_bm_prob (with or without the arc
+ * ends, --include-am-in-bm); numbers as an ostream writes them (%g).
+ * Returns the text length; writes the text and its terminating 0 into buf
+ * when cap exceeds the length. */
+int64_t lcg_batch_write_am_probs(const lcg_batch *batch, int32_t k, char *buf, int64_t cap);
+int64_t lcg_batch_write_bm_probs(const lcg_batch *batch, int32_t k, int32_t include_am_in_bm, char *buf,
+                                 int64_t cap);
 /* Stochastic traceback (no counterpart in the reference's AlignerP; for
  * fitting score parameters to sampled alignments): draws `nsamples`
  * alignments of pair k from the Boltzmann distribution of the last

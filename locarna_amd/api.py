@@ -114,6 +114,10 @@ def _sig(L):
                                       c.POINTER(c.c_int64)]
     L.lcg_batch_segment.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int64]
     L.lcg_batch_compact_run.argtypes = [c.c_void_p, c.POINTER(c.c_float), c.POINTER(c.c_float)]
+    L.lcg_batch_write_am_probs.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64]
+    L.lcg_batch_write_am_probs.restype = c.c_int64
+    L.lcg_batch_write_bm_probs.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_char_p, c.c_int64]
+    L.lcg_batch_write_bm_probs.restype = c.c_int64
     L.lcg_pair_prepare_scored.argtypes = [c.c_char_p, c.c_char_p, c.POINTER(Params), c.c_char_p, c.c_int32,
                                           c.POINTER(c.c_void_p), c.c_char_p, c.c_int]
     L.lcg_pair_write_arcmatch_scores.argtypes = [c.c_void_p, c.c_char_p, c.c_int64]
@@ -417,6 +421,22 @@ class Batch:
         if rc < 0:
             raise LcgError("lcg_batch_pf_sample (%d)" % rc)
         return list(mean), list(var)
+
+    def _text(self, fn, *args):
+        n = fn(self.h, *args, None, 0)
+        if n < 0:
+            raise LcgError("probability writer (%d)" % n)
+        buf = ctypes.create_string_buffer(n + 1)
+        fn(self.h, *args, buf, n + 1)
+        return buf.value.decode()
+
+    def am_probs_text(self, k):
+        """locarna_p --write-arcmatch-probs text of pair k."""
+        return self._text(lib().lcg_batch_write_am_probs, k)
+
+    def bm_probs_text(self, k, include_am_in_bm=False):
+        """locarna_p --write-basematch-probs text of pair k."""
+        return self._text(lib().lcg_batch_write_bm_probs, k, 1 if include_am_in_bm else 0)
 
     def pf_config(self):
         """(workgroups per split pair, split pairs, workgroups) of LocARNA-P"""

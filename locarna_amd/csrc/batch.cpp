@@ -665,6 +665,7 @@ struct PfHost {
     std::vector<double> am_f;      // by am: f(arcA) + f(arcB), f = 1 - ln p / ln pe
     std::vector<int> am_ends;      // by am: al, ar, bl, br
     std::vector<int> min_col, max_col;
+    double min_am_prob = 0, min_bm_prob = 0;  // cuts of the probability writers
 };
 
 struct lcg_batch {
@@ -765,6 +766,8 @@ static void capture_pf_host(lcg_batch &B, lcg_pair *const *pairs, int npairs) {
         h.T = (double)P.params.temperature_alipf;
         h.ps = P.params.pf_scale;
         h.thr = std::sqrt(P.params.min_am_prob);
+        h.min_am_prob = P.params.min_am_prob;
+        h.min_bm_prob = P.params.min_bm_prob;
         h.tau = P.params.tau;
         h.alpha = P.params.indel_opening;
         h.codeA.assign(h.n + 1, 0);
@@ -1861,6 +1864,53 @@ extern "C" int lcg_batch_bm_probs(const lcg_batch *B, int32_t k, int32_t mode, d
         }
     }
     return (h.n + 1) * W;
+}
+
+namespace {
+int64_t put_text(const std::string &text, char *buf, int64_t cap) {
+    if (buf && cap > (int64_t)text.size()) std::memcpy(buf, text.c_str(), text.size() + 1);
+    return (int64_t)text.size();
+}
+}  // namespace
+
+extern "C" int64_t lcg_batch_write_am_probs(const lcg_batch *B, int32_t k, char *buf, int64_t cap) {
+    if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    if (!B->pf_have) return LCG_E_STATE;
+    const PfHost &h = B->pfh[k];
+    std::vector<double> prob(std::max(h.nam, 1));
+    if (int rc = lcg_batch_am_probs(B, k, prob.data(), (int32_t)prob.size()); rc < 0) return rc;
+    std::string text;
+    char line[128];
+    for (int am = 0; am < h.nam; ++am) {
+        if (!(prob[am] >= h.min_am_prob)) continue;
+        // ostream << double: %g, precision 6
+        std::snprintf(line, sizeof line, "%d %d %d %d %g\n", h.am_ends[4 * am], h.am_ends[4 * am + 1],
+                      h.am_ends[4 * am + 2], h.am_ends[4 * am + 3], prob[am]);
+        text += line;
+    }
+    return put_text(text, buf, cap);
+}
+
+extern "C" int64_t lcg_batch_write_bm_probs(const lcg_batch *B, int32_t k, int32_t include_am_in_bm, char *buf,
+                                            int64_t cap) {
+    if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    if (!B->pf_have) return LCG_E_STATE;
+    const PfHost &h = B->pfh[k];
+    const int W = h.m + 1;
+    std::vector<double> dense((size_t)(h.n + 1) * W);
+    if (int rc = lcg_batch_bm_probs(B, k, include_am_in_bm ? 1 : 0, dense.data(), (int32_t)dense.size()); rc < 0)
+        return rc;
+    std::string text;
+    char line[96];
+    for (int i = 1; i <= h.n; ++i) {
+        for (int j = 1; j <= h.m; ++j) {
+            const double p = dense[(size_t)i * W + j];
+            if (!(p >= h.min_bm_prob)) continue;
+            std::snprintf(line, sizeof line, "%d %d %g\n", i, j, p);
+            text += line;
+        }
+    }
+    return put_text(text, buf, cap);
 }
 
 namespace {

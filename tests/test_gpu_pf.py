@@ -172,3 +172,43 @@ def test_sampled_alignments_workload():
         w = _check_samples(B, k, pa.temperature_alipf, 3000, "pf16x150")
         print("pair", k, "largest deviation / standard error", round(w, 2), flush=True)
     B.destroy()
+
+
+@pytest.mark.parametrize("include_am", [False, True])
+def test_probability_files(include_am):
+    # locarna_p --write-arcmatch-probs / --write-basematch-probs
+    # (aligner_p.icc:1404-1435): lines in the reference's order with p above
+    # the cut, numbers formatted as an ostream does (%g); values against the
+    # oracle within 1e-6 plus the 6-digit rounding
+    kw = dict(min_prob=0.01, max_bp_span=-1, min_trace_prob=0.0, min_am_prob=0.01, min_bm_prob=0.01)
+    pairs = [tiny_pair(7600 + s) for s in range(10)]
+    B, P, ores = _run(pairs, kw, include_am=include_am)
+    pa = api.locarna_p_params(**kw)
+    for k, o in enumerate(ores):
+        am = B.am_probs(k)
+        want = "".join("%d %d %d %d %g\n" % (ends + (p,)) for ends, p in zip(o.arcmatches, am)
+                       if p >= pa.min_am_prob)
+        text = B.am_probs_text(k)
+        assert text == want, k
+        got = {tuple(map(int, l.split()[:4])): float(l.split()[4]) for l in text.splitlines()}
+        for ends, p in zip(o.arcmatches, o.am_probs):
+            if abs(p - pa.min_am_prob) <= 1e-6:
+                continue
+            assert (tuple(ends) in got) == (p >= pa.min_am_prob), (k, ends)
+            if tuple(ends) in got:
+                assert abs(got[tuple(ends)] - p) <= 1e-6 + 5e-6 * p, (k, ends)
+        bm_text = B.bm_probs_text(k, include_am)
+        W = o.lenB + 1
+        got_bm = {tuple(map(int, l.split()[:2])): float(l.split()[2]) for l in bm_text.splitlines()}
+        dense = B.bm_probs(k, 1 if include_am else 0)
+        assert bm_text == "".join("%d %d %g\n" % (i, j, dense[i * W + j]) for i in range(1, o.lenA + 1)
+                                  for j in range(1, o.lenB + 1) if dense[i * W + j] >= pa.min_bm_prob), k
+        for i in range(1, o.lenA + 1):
+            for j in range(1, o.lenB + 1):
+                p = o.bm_probs[i * W + j]
+                if abs(p - pa.min_bm_prob) <= 1e-6:
+                    continue
+                assert ((i, j) in got_bm) == (p >= pa.min_bm_prob), (k, i, j)
+                if (i, j) in got_bm:
+                    assert abs(got_bm[(i, j)] - p) <= 1e-6 + 5e-6 * p, (k, i, j)
+    B.destroy()
