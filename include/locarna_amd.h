@@ -242,6 +242,21 @@ int lcg_batch_score(const lcg_batch *batch, int32_t k, int64_t *score);
  * number of edges (>= 0) or an error. */
 int lcg_batch_alignment(const lcg_batch *batch, int32_t k, int32_t *a, int32_t *b, int32_t cap, char *structA,
                         char *structB, int32_t scap);
+/* Pairwise alignment of pair k as locarna writes it with --clustal
+ * (src/LocARNA/main_helper.icc:557-580, MultipleAlignment::write,
+ * multiple_alignment.cc:994-1085), after a run with traceback: the header
+ * "CLUSTAL W --- LocARNA 2.0.1 --- Score: <score or -inf>" that mlocarna
+ * reads the pair's score from (src/Utils/mlocarna:3516-3527), then blocks
+ * of `width` columns (locarna --width, 120), names padded to 18.  Positions
+ * that a local or free-end-gap alignment leaves out are gap columns
+ * (Alignment::alignment_edges, alignment.cc:121-167) unless
+ * LCG_CLUSTAL_LOCAL_ONLY (--local-file-output); LCG_CLUSTAL_STRUCTURES adds
+ * the structure rows (--write-structure).  Returns the text length; writes
+ * the text and its terminating 0 into buf when cap exceeds the length. */
+#define LCG_CLUSTAL_LOCAL_ONLY 1
+#define LCG_CLUSTAL_STRUCTURES 2
+int64_t lcg_batch_write_clustal(const lcg_batch *batch, int32_t k, int32_t width, int32_t flags, char *buf,
+                                int64_t cap);
 /* D table of pair k in the reference's arc-match order
  * (AlignerImpl::D(am), aligner.cc:574-610).  Returns the count. */
 int lcg_batch_D(const lcg_batch *batch, int32_t k, int64_t *D, int32_t cap);

@@ -114,6 +114,8 @@ def _sig(L):
                                       c.POINTER(c.c_int64)]
     L.lcg_batch_segment.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int64]
     L.lcg_batch_compact_run.argtypes = [c.c_void_p, c.POINTER(c.c_float), c.POINTER(c.c_float)]
+    L.lcg_batch_write_clustal.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_int32, c.c_char_p, c.c_int64]
+    L.lcg_batch_write_clustal.restype = c.c_int64
     L.lcg_batch_write_am_probs.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64]
     L.lcg_batch_write_am_probs.restype = c.c_int64
     L.lcg_batch_write_bm_probs.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_char_p, c.c_int64]
@@ -429,6 +431,10 @@ class Batch:
         buf = ctypes.create_string_buffer(n + 1)
         fn(self.h, *args, buf, n + 1)
         return buf.value.decode()
+
+    def clustal(self, k, width=120, local_only=False, structures=False):
+        """locarna --clustal text of pair k (lcg_batch_write_clustal)."""
+        return self._text(lib().lcg_batch_write_clustal, k, width, (1 if local_only else 0) | (2 if structures else 0))
 
     def am_probs_text(self, k):
         """locarna_p --write-arcmatch-probs text of pair k."""

@@ -710,6 +710,8 @@ struct lcg_batch {
     int32_t *d_res = nullptr, *d_resoff = nullptr;
     // LocARNA-P
     std::vector<PfHost> pfh;
+    // names and sequences of every pair, for the alignment text
+    std::vector<std::string> nameA, nameB, seqA, seqB;
     std::vector<PfDesc> pdescs;
     std::vector<double> pf_arena;
     double *d_pfa = nullptr;
@@ -1287,6 +1289,16 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
     } else {
         capture_pf_host(*B, pairs, npairs);
     }
+    for (int k = 0; k < npairs; ++k) {
+        const lcg_pair &X = *pairs[k];
+        B->nameA.push_back(X.A->name());
+        B->nameB.push_back(X.B->name());
+        std::string a, b;
+        for (int i = 1; i <= X.A->length(); ++i) a += X.A->nt(i);
+        for (int j = 1; j <= X.B->length(); ++j) b += X.B->nt(j);
+        B->seqA.push_back(a);
+        B->seqB.push_back(b);
+    }
     *out = B.release();
     return LCG_OK;
 }
@@ -1495,6 +1507,79 @@ extern "C" int lcg_batch_alignment(const lcg_batch *B, int32_t k, int32_t *a, in
         structB[d.m] = 0;
     }
     return cnt;
+}
+
+extern "C" int64_t lcg_batch_write_clustal(const lcg_batch *B, int32_t k, int32_t width, int32_t flags, char *buf,
+                                           int64_t cap) {
+    if (!B || k < 0 || k >= B->npairs || width < 1) return LCG_E_ARG;
+    if (!B->traced) return LCG_E_STATE;
+    const std::string &sa = B->seqA[k], &sb = B->seqB[k];
+    const int n = (int)sa.size(), m = (int)sb.size();
+    std::vector<int32_t> ea(n + m + 2), eb(n + m + 2);
+    std::string stA(n + 2, 0), stB(m + 2, 0);
+    const int ne = lcg_batch_alignment(B, k, ea.data(), eb.data(), (int32_t)ea.size(), stA.data(), stB.data(),
+                                       (int32_t)std::max(n, m) + 2);
+    if (ne < 0) return ne;
+    int64_t score = 0;
+    if (int rc = lcg_batch_score(B, k, &score)) return rc;
+    // Alignment::alignment_edges (alignment.cc:121-167): the traced edges,
+    // with the positions a local or free-end-gap alignment leaves out as
+    // gap columns (unless only the local part is written)
+    const bool only_local = (flags & LCG_CLUSTAL_LOCAL_ONLY) != 0;
+    std::string rowA, rowB, strA, strB;
+    auto col = [&](int a, int b) {
+        rowA += a > 0 ? sa[a - 1] : '-';
+        rowB += b > 0 ? sb[b - 1] : '-';
+        strA += a > 0 ? stA[a - 1] : '-';
+        strB += b > 0 ? stB[b - 1] : '-';
+    };
+    int lastA = 1, lastB = 1;
+    for (int t = 0; t < ne; ++t) {
+        if (ea[t] > 0) {
+            for (; lastA < ea[t]; ++lastA)
+                if (!only_local) col(lastA, -1);
+        }
+        if (eb[t] > 0) {
+            for (; lastB < eb[t]; ++lastB)
+                if (!only_local) col(-1, lastB);
+        }
+        if (ea[t] > 0) ++lastA;
+        if (eb[t] > 0) ++lastB;
+        col(ea[t], eb[t]);
+    }
+    if (!only_local) {
+        for (; lastA <= n; ++lastA) col(lastA, -1);
+        for (; lastB <= m; ++lastB) col(-1, lastB);
+    }
+    // header (main_helper.icc:557-569) and blocks of `width` columns
+    // (MultipleAlignment::write, multiple_alignment.cc:994-1085)
+    std::string text = "CLUSTAL W --- LocARNA 2.0.1 --- Score: ";
+    text += score < LCG_NEG_LIMIT ? std::string("-inf") : std::to_string(score);
+    text += "\n\n";
+    std::vector<std::pair<std::string, const std::string *>> rows;
+    const bool st = (flags & LCG_CLUSTAL_STRUCTURES) != 0;  // --write-structure
+    if (st) rows.push_back({"", &strA});
+    rows.push_back({B->nameA[k], &rowA});
+    rows.push_back({B->nameB[k], &rowB});
+    if (st) rows.push_back({"", &strB});
+    size_t namew = 18;
+    for (const auto &r : rows) namew = std::max(namew, r.first.size());
+    const size_t len = rowA.size();
+    size_t start = 0;
+    do {
+        const size_t end = std::min(len, start + (size_t)width);
+        for (const auto &r : rows) {
+            text += r.first;
+            text.append(namew - r.first.size(), ' ');
+            text += ' ';
+            text += r.second->substr(start, end - start);
+            text += '\n';
+        }
+        start = end;
+        if (start < len) text += '\n';
+    } while (start < len);
+    if (buf && cap > (int64_t)text.size()) std::memcpy(buf, text.c_str(), text.size() + 1);
+    return (int64_t)text.size();
 }
 
 extern "C" int lcg_batch_D(const lcg_batch *B, int32_t k, int64_t *D, int32_t cap) {
