@@ -227,6 +227,24 @@ void lcg_batch_destroy(lcg_batch *batch);
  * candidates the fill evaluates (lcg_batch_counters; not with mode 2).
  * device_ms (may be NULL): device time of the run from HIP events. */
 int lcg_batch_run(lcg_batch *batch, int32_t mode, int32_t trace, float *device_ms);
+/* Penalized alignment (Aligner::penalized_align, aligner.cc:1603-1622;
+ * locarna --penalized P): fill of D, then the top level with every
+ * position charged P (sigma - 2P, gap costs - P, arc matches - P times both
+ * arc lengths; Scoring::modify_by_parameter, scoring.cc:77-90,
+ * ModifiedScoringView, aligner_impl.hh:197-262), sequence-local or global
+ * as the pairs were prepared, and the traceback; lcg_batch_score then
+ * returns the penalized score. */
+int lcg_batch_run_penalized(lcg_batch *batch, int64_t position_penalty, float *device_ms);
+/* Normalized alignment (Aligner::normalized_align, aligner.cc:1526-1593;
+ * locarna --normalized L; sequence-local pairs only, else LCG_E_ARG): fill
+ * of D once, then Dinkelbach iterations of the top level per pair from
+ * lambda = 0: lambda <- (score + length lambda) / (length + L) with length
+ * = (max_i - min_i + 1) + (max_j - min_j + 1) of the traced alignment, until
+ * lambda repeats; lcg_batch_score returns the normalized score lambda and
+ * the alignment is the last traced one.  LCG_E_STATE if a pair needs more
+ * than max_iterations iterations. */
+int lcg_batch_run_normalized(lcg_batch *batch, int64_t L, int32_t max_iterations, int32_t *iterations,
+                             float *device_ms);
 /* launch configuration chosen for the batch: waves per workgroup,
  * workgroups, LDS bytes per workgroup */
 int lcg_batch_config(const lcg_batch *batch, int32_t *waves, int32_t *workgroups, int32_t *lds_bytes);

@@ -114,6 +114,9 @@ def _sig(L):
                                       c.POINTER(c.c_int64)]
     L.lcg_batch_segment.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int64]
     L.lcg_batch_compact_run.argtypes = [c.c_void_p, c.POINTER(c.c_float), c.POINTER(c.c_float)]
+    L.lcg_batch_run_penalized.argtypes = [c.c_void_p, c.c_int64, c.POINTER(c.c_float)]
+    L.lcg_batch_run_normalized.argtypes = [c.c_void_p, c.c_int64, c.c_int32, c.POINTER(c.c_int32),
+                                           c.POINTER(c.c_float)]
     L.lcg_batch_write_clustal.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_int32, c.c_char_p, c.c_int64]
     L.lcg_batch_write_clustal.restype = c.c_int64
     L.lcg_batch_write_am_probs.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64]
@@ -364,6 +367,23 @@ class Batch:
         if rc != 0:
             raise LcgError("lcg_batch_run failed (%d)" % rc)
         return ms.value
+
+    def run_penalized(self, position_penalty):
+        """locarna --penalized (lcg_batch_run_penalized); device ms."""
+        ms = ctypes.c_float()
+        rc = lib().lcg_batch_run_penalized(self.h, position_penalty, ctypes.byref(ms))
+        if rc != 0:
+            raise LcgError("lcg_batch_run_penalized (%d)" % rc)
+        return ms.value
+
+    def run_normalized(self, L, max_iterations=100):
+        """locarna --normalized L (lcg_batch_run_normalized); returns the
+        Dinkelbach iterations and the device ms."""
+        it, ms = ctypes.c_int32(), ctypes.c_float()
+        rc = lib().lcg_batch_run_normalized(self.h, L, max_iterations, ctypes.byref(it), ctypes.byref(ms))
+        if rc != 0:
+            raise LcgError("lcg_batch_run_normalized (%d)" % rc)
+        return it.value, ms.value
 
     def score(self, k):
         s = ctypes.c_int64()

@@ -130,7 +130,7 @@ struct TopBest {
 // level (values clamped at 0, first maximum tracked).
 template <bool GLOBAL, bool TOP_LOCAL>
 __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W, int *base, bool global_col,
-                           bool global_row, TopBest *best, unsigned long long *counts) {
+                           bool global_row, TopBest *best, unsigned long long *counts, int lam = 0) {
     const int lane = lane_id();
     const int alpha = P.alpha;
     int *Est = base;
@@ -146,7 +146,7 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
         for (int j0 = bl + 1; j0 <= hi0; j0 += 64) {
             const int j = j0 + lane;
             const bool act = j <= hi0;
-            const int g = (act && global_row) ? P.gapB[j] : 0;
+            const int g = (act && global_row) ? P.gapB[j] - lam : 0;
             const int ps = wave_scan_add(g);
             if (act) Mst[j - bl] = carry + ps;
             carry += readlane(ps, 63);
@@ -181,7 +181,7 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
         const int *prow = row - W;
         const int mni = P.mn[i];
         if (colopen && mni <= bl) {
-            if (global_col) colv += P.gapA[i];
+            if (global_col) colv += P.gapA[i] - lam;
         } else {
             colopen = false;
         }
@@ -203,7 +203,8 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
                 const int p = e.x & 0xffff, q = e.x >> 16;
                 const int rl = max(bl, e.w);  // first stored column of row p-1
                 if (p > al && q > bl && q - 1 >= rl) {
-                    const int v = Mst[(p - 1 - al) * W + (q - 1 - rl)] + e.z;
+                    // top-level position penalty: D - lam (arc lengths), aligner_impl.hh:249-253
+                    const int v = Mst[(p - 1 - al) * W + (q - 1 - rl)] + e.z - lam * ((i - p + 1) + (e.y - q + 1));
                     atomicMax(&Srow[e.y - jlo], v);
                     ++cnt_cand;
                 }
@@ -211,7 +212,7 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
             region_sync<GLOBAL>();
             int Fcarry = LCG_NEG;
             int sprev = colopen ? colv : LCG_NEG;  // M(i, jlo - 1)
-            const int gA = P.gapA[i];
+            const int gA = P.gapA[i] - lam;
             const int *sigrow = P.sig + P.codeA[i] * LCG_NCODES;
             for (int j0 = jlo; j0 <= hi; j0 += 64) {
                 const int j = j0 + lane;
@@ -222,9 +223,9 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
                     const int dg = (j - 1 >= lo_p && j - 1 <= hi_p) ? prow[j - 1 - lo_p] : LCG_NEG;
                     const int e = clampneg(max(Est[j - bl] + gA, up + gA + alpha));
                     Est[j - bl] = e;
-                    s = clampneg(max(max(dg + sigrow[P.codeB[j]], e), Srow[j - jlo]));
+                    s = clampneg(max(max(dg + sigrow[P.codeB[j]] - 2 * lam, e), Srow[j - jlo]));
                     if (TOP_LOCAL) s = max(s, 0);
-                    gB = P.gapB[j];
+                    gB = P.gapB[j] - lam;
                     ++cnt_cells;
                 }
                 // F chain
@@ -505,7 +506,7 @@ __device__ __forceinline__ void emit(const Pair &P, int a, int b, int slot) { P.
 // *tcount in LDS).
 template <bool GLOBAL>
 __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W, int al, int bl, int br, int i, int j, bool top,
-                            bool local, int free_endgaps, int *tstack, int *tcount) {
+                            bool local, int free_endgaps, int *tstack, int *tcount, int lam = 0) {
     const int lane = lane_id();
     region_sync<GLOBAL>();
     for (;;) {
@@ -527,7 +528,7 @@ __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W
         }
         // match
         if (valid(P, i - 1, j - 1) &&
-            Mij == Mget(P, Mst, W, al, bl, br, i - 1, j - 1) + P.sig[P.codeA[i] * LCG_NCODES + P.codeB[j]]) {
+            Mij == Mget(P, Mst, W, al, bl, br, i - 1, j - 1) + P.sig[P.codeA[i] * LCG_NCODES + P.codeB[j]] - 2 * lam) {
             if (lane == 0) emit(P, i, j, i + j);
             --i;
             --j;
@@ -541,7 +542,7 @@ __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W
             for (int base = 1; !done && i - base >= al; base += 64) {
                 const int k = base + lane;
                 const bool inr = (i - k >= al) && valid(P, i - k, j);
-                const int g = inr ? P.gapA[i - k + 1] : 0;
+                const int g = inr ? P.gapA[i - k + 1] - lam : 0;
                 const int cost = carry + wave_scan_add(g);
                 const bool hit = inr && Mij == Mget(P, Mst, W, al, bl, br, i - k, j) + cost;
                 const unsigned long long bh = __ballot(hit);
@@ -563,7 +564,7 @@ __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W
             for (int base = 1; !done && j - base >= bl; base += 64) {
                 const int k = base + lane;
                 const bool inr = (j - k >= bl) && valid(P, i, j - k);
-                const int g = inr ? P.gapB[j - k + 1] : 0;
+                const int g = inr ? P.gapB[j - k + 1] - lam : 0;
                 const int cost = carry + wave_scan_add(g);
                 const bool hit = inr && Mij == Mget(P, Mst, W, al, bl, br, i, j - k) + cost;
                 const unsigned long long bh = __ballot(hit);
@@ -582,12 +583,12 @@ __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W
             }
             if (done) continue;
         } else {
-            if (valid(P, i - 1, j) && Mij == Mget(P, Mst, W, al, bl, br, i - 1, j) + P.gapA[i]) {
+            if (valid(P, i - 1, j) && Mij == Mget(P, Mst, W, al, bl, br, i - 1, j) + P.gapA[i] - lam) {
                 if (lane == 0) emit(P, i, 0, i + j);
                 --i;
                 continue;
             }
-            if (valid(P, i, j - 1) && Mij == Mget(P, Mst, W, al, bl, br, i, j - 1) + P.gapB[j]) {
+            if (valid(P, i, j - 1) && Mij == Mget(P, Mst, W, al, bl, br, i, j - 1) + P.gapB[j] - lam) {
                 if (lane == 0) emit(P, 0, j, i + j);
                 --j;
                 continue;
@@ -609,7 +610,8 @@ __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W
                 const int4 e = P.cand[t];
                 p = e.x & 0xffff;
                 q = e.x >> 16;
-                hit = p > al && q > bl && Mij == Mget(P, Mst, W, al, bl, br, p - 1, q - 1) + e.z;
+                hit = p > al && q > bl &&
+                      Mij == Mget(P, Mst, W, al, bl, br, p - 1, q - 1) + e.z - lam * ((i - p + 1) + (j - q + 1));
             }
             const unsigned long long bh = __ballot(hit);
             if (bh) {
@@ -823,7 +825,8 @@ extern "C" __global__ void lcg_reset_barriers(int *barriers, int n) {
 // pair, after kernel 1 has filled D.
 extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc *__restrict__ descs,
                                                                  const int *__restrict__ order, int *queue,
-                                                                 int *arena, int *scratch, BatchParams bp) {
+                                                                 int *arena, int *scratch, BatchParams bp,
+                                                                 const int *__restrict__ lambda) {
     extern __shared__ int lds[];
     __shared__ int s_pair, s_tcount;
     const int nwaves = blockDim.x >> 6;
@@ -838,6 +841,9 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
         if (k >= bp.npairs) break;
         const PairDesc d = descs[order[k]];
         const int n = d.n, m = d.m;
+        // position penalty of the top level (penalized / normalized
+        // alignment, aligner.cc:1526-1622); 0 otherwise
+        const int lam = lambda ? lambda[order[k]] : 0;
         if (bp.trace) {
             for (int t = threadIdx.x; t <= n + m; t += blockDim.x) arena[d.edges + t] = -1;
             for (int t = threadIdx.x; t < n + m; t += blockDim.x) arena[d.structs + t] = 0;
@@ -857,17 +863,17 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
             const int fe = bp.free_endgaps;
             if (bp.local) {
                 if (top_in_lds)
-                    group_fill<false, true>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, false, false, &best, counts);
+                    group_fill<false, true>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, false, false, &best, counts, lam);
                 else
-                    group_fill<true, true>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, false, false, &best, counts);
+                    group_fill<true, true>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, false, false, &best, counts, lam);
             } else {
                 // free end gaps: column 0 free with the second sequence's
                 // left end, row 0 with the first's (aligner.cc:736-745)
                 const bool gcol = !(fe & 4), grow = !(fe & 1);
                 if (top_in_lds)
-                    group_fill<false, false>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, gcol, grow, nullptr, counts);
+                    group_fill<false, false>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, gcol, grow, nullptr, counts, lam);
                 else
-                    group_fill<true, false>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, gcol, grow, nullptr, counts);
+                    group_fill<true, false>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, gcol, grow, nullptr, counts, lam);
                 if (top_in_lds)
                     region_sync<false>();
                 else
@@ -917,10 +923,10 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
             if (bp.trace) {
                 if (top_in_lds)
                     group_trace<false>(P, topM, d.wtop, 0, 0, m + 1, best.i, best.j, true, bp.local != 0,
-                                       bp.free_endgaps, arena + d.tstack, &s_tcount);
+                                       bp.free_endgaps, arena + d.tstack, &s_tcount, lam);
                 else
                     group_trace<true>(P, topM, d.wtop, 0, 0, m + 1, best.i, best.j, true, bp.local != 0,
-                                      bp.free_endgaps, arena + d.tstack, &s_tcount);
+                                      bp.free_endgaps, arena + d.tstack, &s_tcount, lam);
             }
         }
         __threadfence_block();
@@ -989,7 +995,8 @@ extern "C" __global__ void lcg_reset_queue(int *queue) {
 
 extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, int *scratch,
                                 int *barriers, BatchParams bp, int grid, int waves, size_t lds_bytes, BatchParams bp_top,
-                                int top_grid, int top_waves, size_t top_lds_bytes, hipEvent_t mid, hipStream_t stream) {
+                                int top_grid, int top_waves, size_t top_lds_bytes, const int *lambda, int top_only,
+                                hipEvent_t mid, hipStream_t stream) {
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void *)lcg_fill_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1002,14 +1009,16 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
     }
     // two work counters, 64 bytes apart; the per-pair barrier counters of
     // the multi-workgroup fill
-    hipLaunchKernelGGL(lcg_reset_queue, dim3(1), dim3(64), 0, stream, queue);
     hipLaunchKernelGGL(lcg_reset_queue, dim3(1), dim3(64), 0, stream, queue + 16);
-    if (bp.wg_per_pair > 1)
-        hipLaunchKernelGGL(lcg_reset_barriers, dim3(1), dim3(256), 0, stream, barriers, bp.npairs);
-    hipLaunchKernelGGL(lcg_fill_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue, arena,
-                       barriers, bp);
+    if (!top_only) {  // the fill of D (skipped when only the top level is redone)
+        hipLaunchKernelGGL(lcg_reset_queue, dim3(1), dim3(64), 0, stream, queue);
+        if (bp.wg_per_pair > 1)
+            hipLaunchKernelGGL(lcg_reset_barriers, dim3(1), dim3(256), 0, stream, barriers, bp.npairs);
+        hipLaunchKernelGGL(lcg_fill_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue,
+                           arena, barriers, bp);
+    }
     if (mid) hipEventRecord(mid, stream);
     hipLaunchKernelGGL(lcg_top_pairs, dim3(top_grid), dim3(top_waves * 64), top_lds_bytes, stream, descs, order,
-                       queue + 16, arena, scratch, bp_top);
+                       queue + 16, arena, scratch, bp_top, lambda);
     return (int)hipGetLastError();
 }

@@ -39,7 +39,8 @@ extern "C" int lcg_launch_compact_build(const CompactDesc *cds, const PairDesc *
 extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, int *scratch,
                                 int *barriers, BatchParams bp, int grid, int waves, size_t lds_bytes, BatchParams bp_top,
                                 int top_grid,
-                                int top_waves, size_t top_lds_bytes, hipEvent_t mid, hipStream_t stream);
+                                int top_waves, size_t top_lds_bytes, const int *lambda, int top_only, hipEvent_t mid,
+                                hipStream_t stream);
 
 struct lcg_pair {
     lcg_params params;
@@ -710,6 +711,11 @@ struct lcg_batch {
     int32_t *d_res = nullptr, *d_resoff = nullptr;
     // LocARNA-P
     std::vector<PfHost> pfh;
+    // penalized / normalized alignment: per-pair position penalty of the top
+    // level (device), the normalized scores of the last normalized run
+    int32_t *d_lambda = nullptr;
+    std::vector<int64_t> norm_score;
+    bool norm_valid = false;
     // names and sequences of every pair, for the alignment text
     std::vector<std::string> nameA, nameB, seqA, seqB;
     std::vector<PfDesc> pdescs;
@@ -1328,6 +1334,7 @@ extern "C" void lcg_batch_destroy(lcg_batch *B) {
     if (B->d_ws2) hipFree(B->d_ws2);
     if (B->d_cstats) hipFree(B->d_cstats);
     if (B->d_cdescs) hipFree(B->d_cdescs);
+    if (B->d_lambda) hipFree(B->d_lambda);
     if (B->stream) hipStreamDestroy(B->stream);
     delete B;
 }
@@ -1403,7 +1410,7 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
     HIPCHECK(hipEventRecord(B->ev0, B->stream));
     if (mode == 0) {
         int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, B->d_scratch, B->d_barriers, bp, B->grid, B->waves,
-                                  B->lds_bytes, top, B->top_grid, B->top_waves, B->top_lds_bytes, B->ev_mid, B->stream);
+                                  B->lds_bytes, top, B->top_grid, B->top_waves, B->top_lds_bytes, nullptr, 0, B->ev_mid, B->stream);
         if (rc) return hip_fail((hipError_t)rc);
     } else if (mode == 1) {
         BatchParams one = bp, one_top = top;
@@ -1411,7 +1418,7 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
         one_top.npairs = 1;
         for (int k = 0; k < B->npairs; ++k) {
             int rc = lcg_launch_align(B->d_descs, B->d_order + k, B->d_queue, B->d_arena, B->d_scratch, B->d_barriers, one, B->wg_per_pair, B->waves,
-                                      B->lds_bytes, one_top, 1, B->top_waves, B->top_lds_bytes, nullptr, B->stream);
+                                      B->lds_bytes, one_top, 1, B->top_waves, B->top_lds_bytes, nullptr, 0, nullptr, B->stream);
             if (rc) return hip_fail((hipError_t)rc);
         }
     } else {
@@ -1424,7 +1431,7 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
             HIPCHECK(hipStreamCreate(&cs));
             HIPCHECK(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
             int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, B->d_scratch, B->d_barriers, bp, B->grid, B->waves,
-                                      B->lds_bytes, top, B->top_grid, B->top_waves, B->top_lds_bytes, nullptr, cs);
+                                      B->lds_bytes, top, B->top_grid, B->top_waves, B->top_lds_bytes, nullptr, 0, nullptr, cs);
             hipGraph_t g = nullptr;
             hipError_t ec = hipStreamEndCapture(cs, &g);
             hipStreamDestroy(cs);
@@ -1448,7 +1455,114 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
     B->have_results = true;
     B->fetched = false;
     B->traced = bp.trace != 0;
+    B->norm_valid = false;
     return LCG_OK;
+}
+
+namespace {
+int fetch(const lcg_batch *Bc);
+inline int32_t res(const lcg_batch *B, int k, int32_t off);
+
+// One run with the per-pair position penalties in d_lambda (top_only: the
+// top level and traceback again over the D of the last run).
+int run_with_penalty(lcg_batch *B, bool top_only, float *device_ms) {
+    HIPCHECK(hipSetDevice(B->device));
+    BatchParams bp = B->bp, top = B->bp_top;
+    bp.trace = top.trace = 1;
+    bp.count = top.count = 0;
+    HIPCHECK(hipEventRecord(B->ev0, B->stream));
+    int rc = lcg_launch_align(B->d_descs, B->d_order, B->d_queue, B->d_arena, B->d_scratch, B->d_barriers, bp,
+                              B->grid, B->waves, B->lds_bytes, top, B->top_grid, B->top_waves, B->top_lds_bytes,
+                              B->d_lambda, top_only ? 1 : 0, nullptr, B->stream);
+    if (rc) return hip_fail((hipError_t)rc);
+    HIPCHECK(hipEventRecord(B->ev1, B->stream));
+    HIPCHECK(hipEventSynchronize(B->ev1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, B->ev0, B->ev1));
+    if (device_ms) *device_ms += ms;
+    B->fill_ms = B->top_ms = -1;
+    B->have_results = true;
+    B->fetched = false;
+    B->traced = true;
+    B->norm_valid = false;
+    return LCG_OK;
+}
+
+int upload_lambda(lcg_batch *B, const std::vector<int32_t> &lam) {
+    HIPCHECK(hipSetDevice(B->device));
+    if (!B->d_lambda) HIPCHECK(hipMalloc(&B->d_lambda, 4 * (size_t)B->npairs));
+    HIPCHECK(hipMemcpy(B->d_lambda, lam.data(), 4 * (size_t)B->npairs, hipMemcpyHostToDevice));
+    return LCG_OK;
+}
+}  // namespace
+
+extern "C" int lcg_batch_run_penalized(lcg_batch *B, int64_t position_penalty, float *device_ms) {
+    if (!B || position_penalty < -100000 || position_penalty > 100000) return LCG_E_ARG;
+    if (int rc = upload_lambda(B, std::vector<int32_t>(B->npairs, (int32_t)position_penalty))) return rc;
+    if (device_ms) *device_ms = 0;
+    return run_with_penalty(B, false, device_ms);
+}
+
+extern "C" int lcg_batch_run_normalized(lcg_batch *B, int64_t L, int32_t max_iterations, int32_t *iterations,
+                                        float *device_ms) {
+    if (!B || max_iterations < 1 || L < 0) return LCG_E_ARG;
+    if (!B->bp.local) return LCG_E_ARG;  // normalized alignment is sequence-local (src/locarna.cc:428-440)
+    const int np = B->npairs;
+    std::vector<int32_t> lam(np, 0);
+    std::vector<char> done(np, 0);
+    if (device_ms) *device_ms = 0;
+    if (int rc = upload_lambda(B, lam)) return rc;
+    if (int rc = run_with_penalty(B, false, device_ms)) return rc;  // D, then lambda = 0
+    std::vector<int32_t> ea, eb;
+    for (int it = 1;; ++it) {
+        if (int rc = fetch(B)) return rc;
+        bool all = true;
+        for (int k = 0; k < np; ++k) {
+            if (done[k]) continue;
+            const PairDesc &d = B->descs[k];
+            const int64_t score = res(B, k, d.out);
+            const int max_i = res(B, k, d.out + 1), max_j = res(B, k, d.out + 2);
+            // the cell where the local trace stopped (M = 0): before the
+            // first edge, a match edge for the recursion's gap costs; the
+            // empty alignment stops at (0, 0)
+            int min_i = 0, min_j = 0;
+            ea.assign(d.n + d.m + 2, 0);
+            eb.assign(d.n + d.m + 2, 0);
+            const int ne = lcg_batch_alignment(B, k, ea.data(), eb.data(), (int32_t)ea.size(), nullptr, nullptr, 0);
+            if (ne < 0) return ne;
+            if (ne > 0) {
+                for (int t = 0; t < ne; ++t)
+                    if (ea[t] > 0) {
+                        min_i = ea[t] - 1;
+                        break;
+                    }
+                for (int t = 0; t < ne; ++t)
+                    if (eb[t] > 0) {
+                        min_j = eb[t] - 1;
+                        break;
+                    }
+            }
+            // aligner.cc:1562-1575
+            const int64_t length = (int64_t)(max_i - min_i + 1) + (max_j - min_j + 1);
+            const int64_t next = (score + length * lam[k]) / (length + L);
+            if (next == lam[k]) {
+                done[k] = 1;
+                continue;
+            }
+            if (next < -100000 || next > 100000) return LCG_E_STATE;
+            lam[k] = (int32_t)next;
+            all = false;
+        }
+        if (all) {
+            B->norm_score.assign(lam.begin(), lam.end());
+            B->norm_valid = true;
+            if (iterations) *iterations = it;
+            return LCG_OK;
+        }
+        if (it >= max_iterations) return LCG_E_STATE;
+        if (int rc = upload_lambda(B, lam)) return rc;
+        if (int rc = run_with_penalty(B, true, device_ms)) return rc;
+    }
 }
 
 namespace {
@@ -1478,7 +1592,7 @@ inline int32_t res(const lcg_batch *B, int k, int32_t off) {
 extern "C" int lcg_batch_score(const lcg_batch *B, int32_t k, int64_t *score) {
     if (!B || k < 0 || k >= B->npairs || !score) return LCG_E_ARG;
     if (int rc = fetch(B)) return rc;
-    *score = res(B, k, B->descs[k].out);
+    *score = B->norm_valid ? B->norm_score[k] : (int64_t)res(B, k, B->descs[k].out);
     return LCG_OK;
 }
 

@@ -602,6 +602,12 @@ struct Aligner {
     Mat<sc_t> Dmat;
     std::vector<sc_t> am_score;
     int max_i = 0, max_j = 0;
+    int min_i = 1, min_j = 1;  // aligner.cc:111-112, set where a local trace stops
+    // position penalty of the top level (ModifiedScoringView,
+    // aligner_impl.hh:197-262; Scoring::modify_by_parameter, scoring.cc:77-90):
+    // sigma - 2 lam, gap costs - lam, D(am) - lam (arc lengths of A and B)
+    sc_t lam = 0;
+    bool D_done = false;
     std::vector<std::pair<int, int>> edges;
     std::string structA, structB;
     long long cnt_G = 0, cnt_C = 0, cnt_K = 0;
@@ -622,17 +628,18 @@ struct Aligner {
     sc_t &D(const ArcMatch &am) { return Dmat(am.a.idx, am.b.idx); }
 
     // aligner.cc:153-233
-    sc_t align_noex(size_t al, size_t bl, size_t i, size_t j) {
-        sc_t max_score = M(i - 1, j - 1) + sc.sigma_tab(i, j);
-        E[j] = std::max(E[j] + sc.gapA_tab[i], M(i - 1, j) + sc.gapA_tab[i] + alpha);
+    sc_t align_noex(size_t al, size_t bl, size_t i, size_t j, sc_t lm = 0) {
+        sc_t max_score = M(i - 1, j - 1) + (sc.sigma_tab(i, j) - 2 * lm);
+        E[j] = std::max(E[j] + (sc.gapA_tab[i] - lm), M(i - 1, j) + (sc.gapA_tab[i] - lm) + alpha);
         max_score = std::max(max_score, E[j]);
-        F = std::max(F + sc.gapB_tab[j], M(i, j - 1) + sc.gapB_tab[j] + alpha);
+        F = std::max(F + (sc.gapB_tab[j] - lm), M(i, j - 1) + (sc.gapB_tab[j] - lm) + alpha);
         max_score = std::max(max_score, F);
         const auto &adjlA = bpsA.right_adj[i];
         const auto &adjlB = bpsB.right_adj[j];
         for (auto arcA = adjlA.begin(); (size_t)arcA->left > al; ++arcA) {
             for (auto arcB = adjlB.begin(); (size_t)arcB->left > bl; ++arcB) {
-                sc_t ns = M(arcA->left - 1, arcB->left - 1) + D(*arcA, *arcB);
+                sc_t ns = M(arcA->left - 1, arcB->left - 1) + D(*arcA, *arcB) -
+                          lm * (sc_t)((arcA->right - arcA->left + 1) + (arcB->right - arcB->left + 1));
                 if (ns <= max_score) continue;
                 max_score = ns;
             }
@@ -646,21 +653,21 @@ struct Aligner {
     }
 
     // aligner.cc:264-368 (exclA = exclB = false)
-    void init_state(size_t al, size_t ar, size_t bl, size_t br, bool globalA, bool globalB) {
+    void init_state(size_t al, size_t ar, size_t bl, size_t br, bool globalA, bool globalB, sc_t lm = 0) {
         for (size_t j = bl; j < br; j++) E[j] = NEG_INF;
         M(al, bl) = 0;
         sc_t indel_score = globalA ? alpha : 0;
         size_t i;
         for (i = al + 1; i < ar; i++) {
             if (tc.min_col[i] > bl) break;
-            if (!is_neg_inf(indel_score) && globalA) indel_score += sc.gapA_tab[i];
+            if (!is_neg_inf(indel_score) && globalA) indel_score += sc.gapA_tab[i] - lm;
             M(i, bl) = indel_score;
         }
         for (; i < ar; i++) M(i, tc.min_col[i] - 1) = NEG_INF;
         indel_score = globalB ? alpha : 0;
         size_t j;
         for (j = bl + 1; j < std::min(br, tc.max_col[al] + 1); j++) {
-            if (!is_neg_inf(indel_score) && globalB) indel_score += sc.gapB_tab[j];
+            if (!is_neg_inf(indel_score) && globalB) indel_score += sc.gapB_tab[j] - lm;
             M(al, j) = indel_score;
         }
         for (i = al + 1; i < ar; i++)
@@ -712,12 +719,12 @@ struct Aligner {
     // strict maximum (no anchors: the rightmost anchor is (0, 0))
     sc_t align_top_level_global() {
         const int fe = in.p.free_endgaps;
-        init_state(0, lenA + 1, 0, lenB + 1, !fe_left_2(fe), !fe_left_1(fe));
+        init_state(0, lenA + 1, 0, lenB + 1, !fe_left_2(fe), !fe_left_1(fe), lam);
         for (size_t i = 1; i <= (size_t)lenA; i++) {
             F = NEG_INF;
             size_t min_col = std::max((size_t)1, tc.min_col[i]);
             size_t max_col = std::min((size_t)lenB, tc.max_col[i]);
-            for (size_t j = min_col; j <= max_col; j++) M(i, j) = align_noex(0, 0, i, j);
+            for (size_t j = min_col; j <= max_col; j++) M(i, j) = align_noex(0, 0, i, j, lam);
         }
         sc_t max_score = M(lenA, lenB);
         max_i = lenA;
@@ -755,13 +762,13 @@ struct Aligner {
         sc_t max_score = 0;
         max_i = 0;
         max_j = 0;
-        init_state(0, lenA + 1, 0, lenB + 1, false, false);
+        init_state(0, lenA + 1, 0, lenB + 1, false, false, lam);
         for (size_t i = 1; i <= (size_t)lenA; i++) {
             F = NEG_INF;
             size_t min_col = std::max((size_t)1, tc.min_col[i]);
             size_t max_col = std::min((size_t)lenB, tc.max_col[i]);
             for (size_t j = min_col; j <= max_col; j++) {
-                M(i, j) = align_noex(0, 0, i, j);
+                M(i, j) = align_noex(0, 0, i, j, lam);
                 M(i, j) = std::max((sc_t)0, M(i, j));
                 if (max_score < M(i, j)) {
                     max_score = M(i, j);
@@ -775,7 +782,40 @@ struct Aligner {
 
     sc_t align() {
         align_D();
+        D_done = true;
         return in.p.sequ_local ? align_top_level_locally() : align_top_level_global();
+    }
+
+    // aligner.cc:1603-1622: the top level with position penalty P (local, or
+    // global with free end gaps), then the traceback
+    sc_t penalized_align(sc_t P) {
+        if (!D_done) align_D();
+        D_done = true;
+        lam = P;
+        sc_t score = in.p.sequ_local ? align_top_level_locally() : align_top_level_global();
+        trace();
+        return score;
+    }
+
+    // aligner.cc:1526-1593: Dinkelbach iterations over the local top level
+    // with penalty lambda; length = max_i - min_i + 1 + max_j - min_j + 1
+    // of the traced alignment; lambda <- (score + length lambda) /
+    // (length + L) (integer division) until it repeats
+    sc_t normalized_align(sc_t L) {
+        if (!in.p.sequ_local) throw std::runtime_error("normalized alignment needs sequence-local alignment");
+        if (!D_done) align_D();
+        D_done = true;
+        sc_t new_lambda = 0, lambda = -1;
+        while (lambda != new_lambda) {
+            lambda = new_lambda;
+            lam = lambda;
+            sc_t score = align_top_level_locally();
+            trace();
+            const sc_t length = (max_i - min_i + 1) + (max_j - min_j + 1);
+            score += length * lambda;
+            new_lambda = score / (length + L);
+        }
+        return new_lambda;
     }
 
     // ---- traceback: aligner.cc:967-1028, 1082-1229, 1233-1343, 1345-1358
@@ -787,18 +827,19 @@ struct Aligner {
     }
 
     void trace_noex(size_t oal, size_t i, size_t obl, size_t j, bool tl) {
-        if (tc.is_valid(i - 1, j - 1) && M(i, j) == M(i - 1, j - 1) + sc.sigma_tab(i, j)) {
+        const sc_t lm = tl ? lam : 0;
+        if (tc.is_valid(i - 1, j - 1) && M(i, j) == M(i - 1, j - 1) + (sc.sigma_tab(i, j) - 2 * lm)) {
             trace_in_arcmatch(oal, i - 1, obl, j - 1, tl);
             append((int)i, (int)j);
             return;
         }
         if (alpha == 0) {
-            if (tc.is_valid(i - 1, j) && M(i, j) == M(i - 1, j) + sc.gapA_tab[i]) {
+            if (tc.is_valid(i - 1, j) && M(i, j) == M(i - 1, j) + (sc.gapA_tab[i] - lm)) {
                 trace_in_arcmatch(oal, i - 1, obl, j, tl);
                 append((int)i, -1);
                 return;
             }
-            if (tc.is_valid(i, j - 1) && M(i, j) == M(i, j - 1) + sc.gapB_tab[j]) {
+            if (tc.is_valid(i, j - 1) && M(i, j) == M(i, j - 1) + (sc.gapB_tab[j] - lm)) {
                 trace_in_arcmatch(oal, i, obl, j - 1, tl);
                 append(-1, (int)j);
                 return;
@@ -807,7 +848,7 @@ struct Aligner {
             sc_t gap_cost = alpha;
             for (size_t k = 1; i >= oal + k; k++) {
                 if (!tc.is_valid(i - k, j)) break;
-                gap_cost += sc.gapA_tab[i - k + 1];
+                gap_cost += sc.gapA_tab[i - k + 1] - lm;
                 if (M(i, j) == M(i - k, j) + gap_cost) {
                     trace_in_arcmatch(oal, i - k, obl, j, tl);
                     for (size_t l = k; l > 0; l--) append((int)(i - l + 1), -1);
@@ -817,7 +858,7 @@ struct Aligner {
             gap_cost = alpha;
             for (size_t k = 1; j >= obl + k; k++) {
                 if (!tc.is_valid(i, j - k)) break;
-                gap_cost += sc.gapB_tab[j - k + 1];
+                gap_cost += sc.gapB_tab[j - k + 1] - lm;
                 if (M(i, j) == M(i, j - k) + gap_cost) {
                     trace_in_arcmatch(oal, i, obl, j - k, tl);
                     for (size_t l = k; l > 0; l--) append(-1, (int)(j - l + 1));
@@ -830,7 +871,8 @@ struct Aligner {
             const ArcMatch &am = ams.ams[x];
             if ((size_t)am.a.left <= oal || (size_t)am.b.left <= obl) continue;
             size_t al = am.a.left, bl = am.b.left;
-            if (M(i, j) == M(al - 1, bl - 1) + D(am)) {
+            if (M(i, j) == M(al - 1, bl - 1) + D(am) -
+                               lm * (sc_t)((am.a.right - am.a.left + 1) + (am.b.right - am.b.left + 1))) {
                 trace_in_arcmatch(oal, al - 1, obl, bl - 1, tl);
                 structA[al - 1] = '(';
                 structA[i - 1] = ')';
@@ -845,7 +887,11 @@ struct Aligner {
     }
 
     void trace_in_arcmatch(size_t al, size_t i, size_t bl, size_t j, bool tl) {
-        if (tl && in.p.sequ_local && M(i, j) == 0) return;
+        if (tl && in.p.sequ_local && M(i, j) == 0) {
+            min_i = (int)i;
+            min_j = (int)j;
+            return;
+        }
         // aligner.cc:1260-1289: no padding of free leading end gaps
         if (i <= al) {
             if (!(tl && (in.p.sequ_local || fe_left_1(in.p.free_endgaps))))
@@ -860,8 +906,10 @@ struct Aligner {
         trace_noex(al, i, bl, j, tl);
     }
 
-    void trace() {
+    void trace() {  // alignment_.clear() first (aligner.cc:1352-1357)
         edges.clear();
+        structA.assign(lenA, '.');
+        structB.assign(lenB, '.');
         trace_in_arcmatch(0, max_i, 0, max_j, true);
     }
 };
@@ -1168,6 +1216,32 @@ extern "C" oracle_result *oracle_align(const char *ppA, const char *ppB, const o
             r->sA = al.structA;
             r->sB = al.structB;
         }
+        return r;
+    } catch (std::exception &e) {
+        set_err(err, errlen, e.what());
+        return nullptr;
+    }
+}
+
+// Penalized (mode 1, arg = position penalty) and normalized (mode 2, arg =
+// L; sequence-local) alignment: score (normalized: the normalized score)
+// and the traced alignment.
+extern "C" oracle_result *oracle_align_variant(const char *ppA, const char *ppB, const oracle_params *p,
+                                               const char *ribosum_path, int mode, long long arg, char *err,
+                                               int errlen) {
+    try {
+        auto *r = new oracle_result();
+        r->in = orc::prepare(ppA, ppB, *p, ribosum_path);
+        orc::Aligner al(*r->in);
+        if (mode == 1)
+            r->score = al.penalized_align(arg);
+        else if (mode == 2)
+            r->score = al.normalized_align(arg);
+        else
+            throw std::runtime_error("mode: 1 penalized, 2 normalized");
+        r->edges = al.edges;
+        r->sA = al.structA;
+        r->sB = al.structB;
         return r;
     } catch (std::exception &e) {
         set_err(err, errlen, e.what());

@@ -18,6 +18,9 @@
  *   - free end gaps of the integer DP: exhaustive enumeration with free
  *     leading runs and free ends (score and traceback, 9 end
  *     specifications);
+ *   - penalized alignment: exhaustive enumeration over the penalized score
+ *     tables (local and global); normalized alignment: Dinkelbach
+ *     iterations over pinned penalized alignments;
  *   - integer DP: exhaustive enumeration of all alignments x nested
  *     arc-match sets on tiny inputs (max score and the traceback's tie rule),
  *     reduction to textbook Gotoh / Smith-Waterman-Gotoh without arcs,
@@ -69,6 +72,12 @@ oracle_result *oracle_align(const char *ppA, const char *ppB,
 oracle_result *oracle_pf(const char *ppA, const char *ppB,
                          const oracle_params *p, const char *ribosum_path,
                          int include_am_in_bm, char *err, int errlen);
+
+/* Penalized (mode 1, arg = position penalty; locarna --penalized) and
+ * normalized (mode 2, arg = L, sequence-local; --normalized) alignment with
+ * traceback; the score of a normalized alignment is the normalized score. */
+oracle_result *oracle_align_variant(const char *ppA, const char *ppB, const oracle_params *p,
+                                    const char *ribosum_path, int mode, long long arg, char *err, int errlen);
 
 void oracle_free(oracle_result *r);
 /* band step: trace probabilities over the full matrix, the similarity of

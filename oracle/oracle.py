@@ -65,6 +65,10 @@ def load():
         L.oracle_pf.restype = ctypes.c_void_p
         L.oracle_pf.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Params),
                                 ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.oracle_align_variant.restype = ctypes.c_void_p
+        L.oracle_align_variant.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Params),
+                                           ctypes.c_char_p, ctypes.c_int, ctypes.c_longlong, ctypes.c_char_p,
+                                           ctypes.c_int]
         L.oracle_free.argtypes = [ctypes.c_void_p]
         L.oracle_score.restype = ctypes.c_longlong
         L.oracle_score.argtypes = [ctypes.c_void_p]
@@ -145,6 +149,20 @@ def align(ppA, ppB, params=None, trace=True):
                        1 if trace else 0, err, 512)
     if not h:
         raise RuntimeError("oracle_align: " + err.value.decode())
+    return Result(h)
+
+
+def align_variant(ppA, ppB, params=None, penalized=None, normalized=None):
+    """Penalized (position penalty) or normalized (L, sequence-local)
+    alignment with traceback (locarna --penalized / --normalized)."""
+    L = load()
+    params = params or locarna_params()
+    mode, arg = (1, penalized) if penalized is not None else (2, normalized)
+    err = ctypes.create_string_buffer(512)
+    h = L.oracle_align_variant(ppA.encode(), ppB.encode(), ctypes.byref(params), RIBOSUM.encode(), mode, arg,
+                               err, 512)
+    if not h:
+        raise RuntimeError("oracle_align_variant: " + err.value.decode())
     return Result(h)
 
 

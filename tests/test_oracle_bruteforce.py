@@ -113,3 +113,80 @@ def test_free_endgaps_rejected_with_local():
     ppA, ppB = tiny_pair(1)
     with pytest.raises(RuntimeError):
         oracle.align(ppA, ppB, oracle.locarna_params(sequ_local=1, free_endgaps=1))
+
+
+def _penalized_problem(r, alpha, P, local, free=0):
+    # every position charged P (scoring.cc:77-90, aligner_impl.hh:236-253):
+    # match edges -2P, gap positions -P, the four arc-end positions of an arc
+    # match -4P; the inner positions of an arc match are charged through the
+    # tables inside it, which makes D(am) - P (arc lengths) exactly
+    sigma = {k: v - 2 * P for k, v in r.sigma.items()}
+    gapA = [g - P for g in r.gapA]
+    gapB = [g - P for g in r.gapB]
+    am = [s - 4 * P for s in r.am_scores]
+    return bf.Problem(r.lenA, r.lenB, sigma, gapA, gapB, alpha, r.arcmatches, am, r.min_col, r.max_col,
+                      local=local, free=free)
+
+
+@pytest.mark.parametrize("P", [0, 25, 80, -40])
+@pytest.mark.parametrize("seed", range(8))
+def test_penalized_local_matches_enumeration(P, seed):
+    # locarna --penalized (aligner.cc:1603-1622), sequence-local
+    ppA, ppB = tiny_pair(5100 + seed)
+    params = oracle.locarna_params(min_prob=0.01, max_bp_span=-1, sequ_local=1, min_trace_prob=0.0, indel=-150,
+                                   indel_opening=-200)
+    r = oracle.align(ppA, ppB, params)
+    v = oracle.align_variant(ppA, ppB, params, penalized=P)
+    best, ends = bf.best_local(_penalized_problem(r, params.indel_opening, P, True))
+    assert v.score == best, (P, seed)
+    if best > 0:
+        last = [e for e in v.edges if e[0] > 0 and e[1] > 0][-1]
+        assert last == min(ends), (P, seed)
+
+
+@pytest.mark.parametrize("P", [25, -40])
+@pytest.mark.parametrize("seed", range(8))
+def test_penalized_global_matches_enumeration(P, seed):
+    ppA, ppB = tiny_pair(5200 + seed)
+    params = oracle.locarna_params(min_prob=0.01, max_bp_span=-1, min_trace_prob=0.0, indel=-150,
+                                   indel_opening=-200)
+    r = oracle.align(ppA, ppB, params)
+    v = oracle.align_variant(ppA, ppB, params, penalized=P)
+    best, edges, arcs, nal = bf.best_global(_penalized_problem(r, params.indel_opening, P, False))
+    assert v.score == best and v.edges == edges, (P, seed)
+
+
+def _length(edges):
+    # (max_i - min_i + 1) + (max_j - min_j + 1) of a traced local alignment:
+    # it starts after the cell where its trace stopped (a match edge follows)
+    if not edges:
+        return 2
+    a = [x for x, _ in edges if x > 0]
+    b = [y for _, y in edges if y > 0]
+    return (max(a) - (min(a) - 1) + 1) + (max(b) - (min(b) - 1) + 1)
+
+
+@pytest.mark.parametrize("L", [0, 6, 30])
+@pytest.mark.parametrize("seed", range(8))
+def test_normalized_is_dinkelbach_over_penalized(L, seed):
+    # locarna --normalized L (aligner.cc:1526-1593): Dinkelbach iterations,
+    # each a penalized alignment (pinned above)
+    ppA, ppB = tiny_pair(5300 + seed)
+    params = oracle.locarna_params(min_prob=0.01, max_bp_span=-1, sequ_local=1, min_trace_prob=0.0, indel=-150,
+                                   indel_opening=-200)
+    lam, new = -1, 0
+    v = None
+    while lam != new:
+        lam = new
+        v = oracle.align_variant(ppA, ppB, params, penalized=lam)
+        n = _length(v.edges)
+        score = v.score + n * lam
+        new = int(score / (n + L)) if score >= 0 else -((-score) // (n + L))
+    w = oracle.align_variant(ppA, ppB, params, normalized=L)
+    assert w.score == new and w.edges == v.edges, (L, seed)
+
+
+def test_normalized_needs_local():
+    ppA, ppB = tiny_pair(1)
+    with pytest.raises(RuntimeError):
+        oracle.align_variant(ppA, ppB, oracle.locarna_params(), normalized=5)
