@@ -49,6 +49,11 @@ typedef struct {
     int32_t free_endgaps;      /* --free-endgaps "++--" as bits LCG_FREE_*;
                                   not with sequ_local (src/locarna.cc:375),
                                   integer path only                          */
+    int32_t max_diff;          /* --max-diff: band around the diagonal
+                                  (TraceController::constrain_wo_ref,
+                                  trace_controller.cc:361-390); -1: none    */
+    int32_t max_diff_am;       /* --max-diff-am (-1: the longer length)     */
+    int32_t max_diff_at_am;    /* --max-diff-at-am (-1: the longer length)  */
     double min_prob;           /* --min-prob                                */
     double min_trace_prob;     /* --min-trace-probability (<= 0: no band)    */
     double exp_prob;           /* --exp-prob (<= 0: 1/(2 length))            */

@@ -454,7 +454,9 @@ void prepare_compaction_inputs(lcg_pair &P) {
         for (int x = 0; x < 256; ++x)
             img[c.tab + x] = (int32_t)Scoring::ribosum_arc_term((x >> 6) & 3, (x >> 4) & 3, (x >> 2) & 3, x & 3);
     // ArcMatches without max-diff-am / max-diff-at-am (arc_matches.cc:19-34)
-    c.dlen = c.dend = std::max(n, m);
+    // is_valid_arcmatch limits (locarna.cc:606-611: -1 means the longer length)
+    c.dlen = P.params.max_diff_am < 0 ? std::max(n, m) : P.params.max_diff_am;
+    c.dend = P.params.max_diff_at_am < 0 ? std::max(n, m) : P.params.max_diff_at_am;
     while (img.size() & 3) img.push_back(0);
     P.cdesc = c;
 }
@@ -483,6 +485,7 @@ void prepare_parse(const char *ppA, const char *ppB, const lcg_params *p, lcg_pa
     sp.use_ribosum = p->use_ribosum != 0;
     sp.exp_prob = p->exp_prob;
     P.tc.reset(new TraceController(P.A->length(), P.B->length()));
+    if (p->max_diff >= 0) P.tc->constrain_wo_ref(p->max_diff);  // locarna.cc:568-569
 }
 
 // Stage 2 on the host: the trace-probability band in the precision of the
@@ -509,7 +512,7 @@ void prepare_finish(lcg_pair &P, bool deferred) {
         P.desc.nam = P.desc.ngroups = -1;
         prepare_compaction_inputs(P);
     } else {
-        P.am.reset(new ArcMatches(*P.bA, *P.bB, *P.tc));
+        P.am.reset(new ArcMatches(*P.bA, *P.bB, *P.tc, p->max_diff_am, p->max_diff_at_am));
         P.sc.reset(new Scoring(*P.A, *P.B, *P.am, P.sp));
         pack(P);
     }
@@ -569,7 +572,7 @@ extern "C" int lcg_pair_prepare_scored(const char *ppA, const char *ppB, const l
         }
         P->bA.reset(new BasePairs(n, arcsA));
         P->bB.reset(new BasePairs(m, arcsB));
-        P->am.reset(new ArcMatches(*P->bA, *P->bB, *P->tc, lines));
+        P->am.reset(new ArcMatches(*P->bA, *P->bB, *P->tc, lines, p->max_diff_am, p->max_diff_at_am));
         P->sc.reset(new Scoring(*P->A, *P->B, *P->am, P->sp));
         pack(*P);
         *out = P.release();

@@ -195,6 +195,23 @@ std::vector<ExplicitArcMatch> parse_arcmatch_scores(const std::string &text, int
 TraceController::TraceController(int lenA, int lenB)
     : lenA_(lenA), lenB_(lenB), min_col_(lenA + 1, 0), max_col_(lenA + 1, lenB) {}
 
+void TraceController::constrain_wo_ref(int delta) {
+    // unsigned arithmetic as the reference (size_type)
+    const size_t la = lenA_, lb = lenB_, d = (size_t)delta;
+    if (la == 0 || lb == 0) return;
+    for (size_t i = 0; i <= la; i++) {
+        const size_t x = i * lb * (la + lb);
+        size_t y = 2 * d * la * lb;
+        const size_t z = la * (la + lb);
+        if (la > lb)
+            y = std::max(y, (la + lb) * la / 2);
+        else if (lb > la)
+            y = std::max(y, (la + lb) * lb / 2);
+        min_col_[i] = (int)(x > y ? (x - y + z - 1) / z : 0);
+        max_col_[i] = (int)std::min((x + y) / z, lb);
+    }
+}
+
 void TraceController::restrict_by_trace_probabilities(const std::vector<double> &probs, double min_prob) {
     const int W = lenB_ + 1;
     std::vector<int> first(lenA_ + 1, -1), last(lenA_ + 1, -1);
@@ -406,17 +423,18 @@ ArcMatches::ArcMatches(const BasePairs &A, const BasePairs &B, const TraceContro
 }
 
 ArcMatches::ArcMatches(const BasePairs &A, const BasePairs &B, const TraceController &tc,
-                       const std::vector<ExplicitArcMatch> &lines)
+                       const std::vector<ExplicitArcMatch> &lines, int max_diff_am, int max_diff_at_am)
     : A_(A), B_(B), explicit_(true) {
     const int lmax = std::max(A.seqlen(), B.seqlen());
+    const int dlen = max_diff_am < 0 ? lmax : max_diff_am;
+    const int dend = max_diff_at_am < 0 ? lmax : max_diff_at_am;
     for (const ExplicitArcMatch &x : lines) {
         const int a = A.find(x.i, x.j), b = B.find(x.k, x.l);
         const Arc &p = A.arc(a), &q = B.arc(b);
-        // is_valid_arcmatch (arc_matches.cc:19-48) with the default
-        // max-diff-am and max-diff-at-am (the longer length)
+        // is_valid_arcmatch (arc_matches.cc:19-48)
         const bool ok = tc.is_valid_match(p.left, q.left) && tc.is_valid_match(p.right, q.right) &&
-                        std::abs((p.right - p.left) - (q.right - q.left)) <= lmax &&
-                        std::abs(p.left - q.left) <= lmax && std::abs(p.right - q.right) <= lmax;
+                        std::abs((p.right - p.left) - (q.right - q.left)) <= dlen &&
+                        std::abs(p.left - q.left) <= dend && std::abs(p.right - q.right) <= dend;
         if (!ok) continue;
         ams_.push_back(ArcMatch{(int)ams_.size(), a, b});
         scores_.push_back(x.score);

@@ -84,6 +84,9 @@ public:
     bool is_valid_match(int i, int j) const { return is_valid(i, j) && is_valid(i - 1, j - 1); }
     // trace_controller.cc:565-597
     void restrict_by_trace_probabilities(const std::vector<double> &probs, double min_prob);
+    // band of width max-diff around the diagonal without a reference
+    // alignment (constrain_wo_ref, trace_controller.cc:361-390)
+    void constrain_wo_ref(int delta);
     // the same from the first and last column of each row whose trace
     // probability reaches the cut (-1: no column of the row does)
     void restrict_by_rows(const std::vector<int> &first, const std::vector<int> &last);
@@ -160,7 +163,7 @@ public:
     // arc matches of an arc-match scores file, in line order, the invalid
     // ones left out, with their explicit scores (arc_matches.cc:248-273)
     ArcMatches(const BasePairs &A, const BasePairs &B, const TraceController &tc,
-               const std::vector<ExplicitArcMatch> &lines);
+               const std::vector<ExplicitArcMatch> &lines, int max_diff_am = -1, int max_diff_at_am = -1);
     bool explicit_scores() const { return explicit_; }
     score_t explicit_score(int k) const { return scores_[k]; }
     int num_arc_matches() const { return (int)ams_.size(); }

@@ -236,12 +236,28 @@ struct BasePairs {
 };
 
 // ---------------------------------------------------------------- band
-// trace_controller.hh:328-336; trace_controller.cc:406-440 (no max-diff),
-// :565-597 (restrict by trace probabilities)
+// trace_controller.hh:328-336; trace_controller.cc:406-440 (max-diff without
+// a reference alignment), :565-597 (restrict by trace probabilities)
 struct Band {
     int lenA, lenB;
     std::vector<size_t> min_col, max_col;
     Band(int a, int b) : lenA(a), lenB(b), min_col(a + 1, 0), max_col(a + 1, (size_t)b) {}
+    // trace_controller.cc:361-390, unsigned arithmetic as there
+    void constrain_wo_ref(size_t delta) {
+        const size_t la = lenA, lb = lenB;
+        if (la == 0 || lb == 0) return;
+        for (size_t i = 0; i <= la; i++) {
+            size_t x = i * lb * (la + lb);
+            size_t y = 2 * delta * la * lb;
+            size_t z = la * (la + lb);
+            if (la > lb)
+                y = std::max(y, (la + lb) * la / 2);
+            else if (lb > la)
+                y = std::max(y, (la + lb) * lb / 2);
+            min_col[i] = x > y ? (x - y + z - 1) / z : 0;
+            max_col[i] = std::min((x + y) / z, lb);
+        }
+    }
     bool is_valid(size_t i, size_t j) const { return min_col[i] <= j && j <= max_col[i]; }
     bool is_valid_match(size_t i, size_t j) const { return is_valid(i, j) && is_valid(i - 1, j - 1); }
     Band reverse() const {  // trace_controller.cc:320-338
@@ -433,10 +449,13 @@ struct ArcMatches {
     int lenA, lenB;
     std::vector<ArcMatch> ams;
     Mat<std::vector<int>> left_lists, right_lists;
-    ArcMatches(const BasePairs &A, const BasePairs &B, const Band &tc) : lenA(A.len), lenB(B.len) {
+    ArcMatches(const BasePairs &A, const BasePairs &B, const Band &tc, int diff_am = -1, int diff_at_am = -1)
+        : lenA(A.len), lenB(B.len) {
         left_lists.resize(lenA + 1, lenB + 1, {});
         right_lists.resize(lenA + 1, lenB + 1, {});
-        size_t max_length_diff = std::max(lenA, lenB), max_diff_at_am = std::max(lenA, lenB);
+        // locarna.cc:606-611: -1 means the longer length
+        size_t max_length_diff = diff_am != -1 ? (size_t)diff_am : (size_t)std::max(lenA, lenB);
+        size_t max_diff_at_am = diff_at_am != -1 ? (size_t)diff_at_am : (size_t)std::max(lenA, lenB);
         for (const Arc &a : A.arcs) {
             for (const Arc &b : B.arcs) {
                 bool valid = tc.is_valid_match(a.left, b.left) && tc.is_valid_match(a.right, b.right) &&
@@ -558,6 +577,7 @@ std::unique_ptr<Inputs> prepare(const char *ppA, const char *ppB, const oracle_p
     in->B = read_pp(ppB, p.min_prob, p.max_bp_span);
     int lenA = in->A.length(), lenB = in->B.length();
     in->band.reset(new Band(lenA, lenB));
+    if (p.max_diff >= 0) in->band->constrain_wo_ref((size_t)p.max_diff);  // locarna.cc:568-569
     if (p.min_trace_prob > 0.0) {
         // main_helper.icc:371-426: basematch scores = ribosum bm matrix (or
         // match/mismatch); gap costs, struct weight, temperature divided by 100
@@ -580,7 +600,7 @@ std::unique_ptr<Inputs> prepare(const char *ppA, const char *ppB, const oracle_p
     }
     in->bpsA.reset(new BasePairs(in->A, p.min_prob));
     in->bpsB.reset(new BasePairs(in->B, p.min_prob));
-    in->am.reset(new ArcMatches(*in->bpsA, *in->bpsB, *in->band));
+    in->am.reset(new ArcMatches(*in->bpsA, *in->bpsB, *in->band, p.max_diff_am, p.max_diff_at_am));
     in->sc.reset(new Scoring(p, in->rib.get(), in->A, in->B, *in->bpsA, *in->bpsB));
     return in;
 }
@@ -1268,7 +1288,8 @@ extern "C" int oracle_trace_probs(const char *ppA, const char *ppB, const oracle
         double simm[4][4];
         for (int i = 0; i < 4; ++i)
             for (int j = 0; j < 4; ++j) simm[i][j] = rib ? rib->bm[i][j] : (i == j ? p->match : p->mismatch);
-        const Band band(n, m);
+        Band band(n, m);
+        if (p->max_diff >= 0) band.constrain_wo_ref((size_t)p->max_diff);
         const double go = p->indel_opening / 100.0, ge = p->indel / 100.0, sw = p->struct_weight / 100.0;
         const double T = p->temperature_alipf / 100.0;
         Mat<double> pr = p->band_extended

@@ -21,6 +21,11 @@
  *   - penalized alignment: exhaustive enumeration over the penalized score
  *     tables (local and global); normalized alignment: Dinkelbach
  *     iterations over pinned penalized alignments;
+ *   - --max-diff band (Band::constrain_wo_ref, trace_controller.cc:361-390):
+ *     parity unpinned (a restatement of the reference's formula; the
+ *     reference's own TraceController test covers only the band of a
+ *     reference alignment); --max-diff-am / --max-diff-at-am: the predicate
+ *     of arc_matches.cc:19-48, checked through the arc-match lists;
  *   - integer DP: exhaustive enumeration of all alignments x nested
  *     arc-match sets on tiny inputs (max score and the traceback's tie rule),
  *     reduction to textbook Gotoh / Smith-Waterman-Gotoh without arcs,
@@ -52,6 +57,9 @@ typedef struct {
     int free_endgaps;      /* --free-endgaps: bit 0 left end of the first
                               sequence, 1 its right end, 2 left end of the
                               second, 3 its right end (free_endgaps.hh) */
+    int max_diff;          /* --max-diff (-1: no band around the diagonal) */
+    int max_diff_am;       /* --max-diff-am (-1: the longer length)       */
+    int max_diff_at_am;    /* --max-diff-at-am (-1: the longer length)    */
     double min_prob;       /* --min-prob                             */
     double min_trace_prob; /* --min-trace-probability (<=0: no band)  */
     double exp_prob;       /* --exp-prob; <=0: 1/(2 len)              */

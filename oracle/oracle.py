@@ -20,6 +20,7 @@ class Params(ctypes.Structure):
         ("temperature_alipf", ctypes.c_int), ("sequ_local", ctypes.c_int),
         ("max_bp_span", ctypes.c_int), ("band_extended", ctypes.c_int),
         ("use_ribosum", ctypes.c_int), ("free_endgaps", ctypes.c_int),
+        ("max_diff", ctypes.c_int), ("max_diff_am", ctypes.c_int), ("max_diff_at_am", ctypes.c_int),
         ("min_prob", ctypes.c_double), ("min_trace_prob", ctypes.c_double),
         ("exp_prob", ctypes.c_double), ("pf_scale", ctypes.c_double),
         ("min_am_prob", ctypes.c_double), ("min_bm_prob", ctypes.c_double),
@@ -32,7 +33,8 @@ def locarna_params(**kw):
     --min-prob=0.0005 --maxBPspan=100, min-trace-probability 1e-4)."""
     d = dict(indel=-350, indel_opening=-500, struct_weight=200, tau=0, match=50,
              mismatch=0, temperature_alipf=300, sequ_local=0, max_bp_span=100,
-             band_extended=1, use_ribosum=1, free_endgaps=0, min_prob=0.0005, min_trace_prob=1e-4,
+             band_extended=1, use_ribosum=1, free_endgaps=0, max_diff=-1, max_diff_am=-1, max_diff_at_am=-1,
+             min_prob=0.0005, min_trace_prob=1e-4,
              exp_prob=-1.0, pf_scale=1.0, min_am_prob=0.001, min_bm_prob=0.001)
     d.update(kw)
     return Params(**d)
