@@ -22,9 +22,11 @@
  *     tables (local and global); normalized alignment: Dinkelbach
  *     iterations over pinned penalized alignments;
  *   - --max-diff band (Band::constrain_wo_ref, trace_controller.cc:361-390):
- *     parity unpinned (a restatement of the reference's formula; the
- *     reference's own TraceController test covers only the band of a
- *     reference alignment); --max-diff-am / --max-diff-at-am: the predicate
+ *     cell membership against the distance to the scaled diagonal in exact
+ *     integers, |i - j| <= max-diff for equal lengths, non-empty connected
+ *     rows (tests/test_host_max_diff.py; the reference's own TraceController
+ *     test covers only the band of a reference alignment);
+ *     --max-diff-am / --max-diff-at-am: the predicate
  *     of arc_matches.cc:19-48, checked through the arc-match lists;
  *   - integer DP: exhaustive enumeration of all alignments x nested
  *     arc-match sets on tiny inputs (max score and the traceback's tie rule),
