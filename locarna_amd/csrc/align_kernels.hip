@@ -273,7 +273,7 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
 // end in A, left end p descending): a row's candidates with p > al are one
 // prefix, located from the LDS block table once per group; the first 256
 // entries of the next row are fetched while the current row is computed.
-template <bool GLOBAL, int NC, int RT>
+template <bool GLOBAL, int NC, int RT, bool COUNT>
 __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, int W, int *base,
                                  unsigned long long *counts) {
     const int lane = lane_id();
@@ -386,7 +386,7 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
             // branch-free gather (cell 0 of the region when unused), so that
             // the gathers of several chunks are in flight together
             v = Mst[ok ? __mul24(p - 1 - al, W) + (q - 1 - rl) : 0] + e.D;
-            cnt_cand += ok;
+            if (COUNT) cnt_cand += ok;
             return ok ? j - bl - 1 : -1;
         };
         {
@@ -441,7 +441,7 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
                 Ec[c] = e;
                 Srow[j - bl - 1] = LCG_NEG;
                 s = clampneg(max(max(dg + sg, e), sv));
-                ++cnt_cells;
+                if (COUNT) ++cnt_cells;
             }
             const int sl = wave_shift_up(s, sprev);
             const int cc = act ? sl + gBc[c] + alpha : LCG_NEG;
@@ -462,7 +462,7 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
         Mcol_prev = Mcol;
         region_sync<GLOBAL>();
     }
-    if (counts) {
+    if (COUNT && counts) {
         atomicAdd(&counts[0], (unsigned long long)cnt_cells);
         atomicAdd(&counts[1], (unsigned long long)cnt_cand);
     }
@@ -471,19 +471,21 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
 // Fill of an arc-match group (fill-D and trace re-fills), by the fixed-column
 // kernel when the group fits it, else by the general one.  WIDE enables the
 // 4-chunk variant (columns up to 256), left out of the traceback kernel for
-// its register budget.
-template <bool GLOBAL, bool WIDE = true>
+// its register budget.  COUNT keeps the per-lane work counters of the
+// fixed-column kernel (the counting run); without it they cost two vector
+// adds per row chunk and candidate slot.
+template <bool GLOBAL, bool WIDE = true, bool COUNT = false>
 __device__ void fill_group(const Pair &P, int al, int bl, int ar, int br, int W, int *base,
                            unsigned long long *counts, bool fixed_cols) {
     const int cols = br - bl - 1, rows = ar - al - 1;
     if (!fixed_cols)
         group_fill<GLOBAL, false>(P, al, bl, ar, br, W, base, true, true, nullptr, counts);
     else if (cols <= 64 && rows <= 128)
-        group_fill_fixed<GLOBAL, 1, 2>(P, al, bl, ar, br, W, base, counts);
+        group_fill_fixed<GLOBAL, 1, 2, COUNT>(P, al, bl, ar, br, W, base, counts);
     else if (cols <= 128 && rows <= 128)
-        group_fill_fixed<GLOBAL, 2, 2>(P, al, bl, ar, br, W, base, counts);
+        group_fill_fixed<GLOBAL, 2, 2, COUNT>(P, al, bl, ar, br, W, base, counts);
     else if (WIDE && cols <= 256 && rows <= 256)
-        group_fill_fixed<GLOBAL, 4, 4>(P, al, bl, ar, br, W, base, counts);
+        group_fill_fixed<GLOBAL, 4, 4, COUNT>(P, al, bl, ar, br, W, base, counts);
     else
         group_fill<GLOBAL, false>(P, al, bl, ar, br, W, base, true, true, nullptr, counts);
 }
@@ -773,12 +775,19 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc
             for (int rnd = 0; rnd < nr; ++rnd) {
                 for (int g = g0; g < g1; ++g) {
                     const int *G = groups + g * LCG_GROUP_INTS;
+                    // (a wave holds at most one group per round; leaving the
+                    // scan after it measured slower: 53.5 against 49.0 ms on
+                    // 2016 pairs, the compiler then allocates 116 VGPRs
+                    // instead of 96)
                     if (G[5] != (rnd | (wave << 12) | mine)) continue;
                     const int gal = G[0] & 0xffff, gbl = G[0] >> 16;
                     const int gar = G[1] & 0xffff, gbr = G[1] >> 16;
                     const int W = G[2];
                     int *base = pool + G[6];
-                    fill_group<false>(P, gal, gbl, gar, gbr, W, base, counts, bp.fixed_cols != 0);
+                    if (counts)
+                        fill_group<false, true, true>(P, gal, gbl, gar, gbr, W, base, counts, bp.fixed_cols != 0);
+                    else
+                        fill_group<false, true, false>(P, gal, gbl, gar, gbr, W, base, nullptr, bp.fixed_cols != 0);
                     const int *Mst = base + (gbr - gbl) + W;
                     for (int t = G[3] + lane; t < G[4]; t += 64) {
                         const int4 ga = gams[t];
