@@ -50,8 +50,9 @@ def main():
                     help="keep the default launch shapes (no lcg_batch_autotune before timing)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU only (the all-pairs stage would shard "
-              "pairs across GPUs by estimated work; see DESIGN.md)", file=sys.stderr)
+        print("bench.py: runs on one GPU only; --gpus %d is not supported (the all-pairs "
+              "stage would shard the pair list across GPUs by estimated work, one share "
+              "per GPU, and gather the scores)" % args.gpus, file=sys.stderr)
         sys.exit(2)
 
     import torch
