@@ -673,6 +673,12 @@ struct PfHost {
 };
 
 struct lcg_batch {
+    lcg_batch() = default;
+    lcg_batch(const lcg_batch &) = delete;
+    lcg_batch &operator=(const lcg_batch &) = delete;
+    // releases every device resource the batch holds, also for a batch that
+    // lcg_batch_create_shaped abandons on an error return
+    ~lcg_batch();
     int device = 0;
     int npairs = 0;
     std::vector<PairDesc> descs;      // absolute offsets
@@ -1312,8 +1318,10 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
     return LCG_OK;
 }
 
-extern "C" void lcg_batch_destroy(lcg_batch *B) {
-    if (!B) return;
+lcg_batch::~lcg_batch() {
+    lcg_batch *B = this;
+    // the stream is the first resource lcg_batch_create_shaped creates
+    if (!B->stream) return;
     hipSetDevice(B->device);
     if (B->graph_exec) hipGraphExecDestroy(B->graph_exec);
     if (B->graph) hipGraphDestroy(B->graph);
@@ -1339,8 +1347,9 @@ extern "C" void lcg_batch_destroy(lcg_batch *B) {
     if (B->d_cdescs) hipFree(B->d_cdescs);
     if (B->d_lambda) hipFree(B->d_lambda);
     if (B->stream) hipStreamDestroy(B->stream);
-    delete B;
 }
+
+extern "C" void lcg_batch_destroy(lcg_batch *B) { delete B; }
 
 extern "C" int lcg_batch_pair_info(const lcg_batch *B, int32_t k, int32_t *nam, int32_t *ngroups,
                                    int64_t *arena_ints) {
