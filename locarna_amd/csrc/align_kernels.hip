@@ -27,8 +27,10 @@
 // linear recurrence along the row, solved by a prefix sum and a prefix max
 // across the wave (DPP row_shr / row_bcast):
 //   F(j) = P(j) + max_{k<=j} (c(k) - P(k)),  P = inclusive sum of a.
-// Scores are int32 with minus infinity LCG_NEG = -2^29; values below -2^28
-// are normalized to LCG_NEG, which leaves every finite result unchanged.
+// Scores are int32; minus infinity is any value below LCG_NEG_LIMIT = -2^28.
+// The general fill normalizes such values to LCG_NEG = -2^29, the
+// fixed-column fill bounds them below at LCG_NEG (one max instead of a
+// compare and select); neither changes a finite result.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -435,12 +437,16 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
             // both LDS reads unconditional (slot 0 when inactive), issued together
             const int sv = Srow[act ? j - bl - 1 : 0];
             const int sg = sigrow[cBc[c]];
+            // minus infinity: E is bounded below by LCG_NEG, so s and M
+            // (maxima including E) need no clamp; a minus-infinity value
+            // stays below LCG_NEG_LIMIT (it gains at most the positive
+            // scores of one path, far less than 2^28)
             int s = LCG_NEG;
             if (act) {
-                const int e = clampneg(max(Ec[c] + gA, up + gA + alpha));
+                const int e = max(max(Ec[c] + gA, up + gA + alpha), LCG_NEG);
                 Ec[c] = e;
                 Srow[j - bl - 1] = LCG_NEG;
-                s = clampneg(max(max(dg + sg, e), sv));
+                s = max(max(dg + sg, e), sv);
                 if (COUNT) ++cnt_cells;
             }
             const int sl = wave_shift_up(s, sprev);
@@ -451,10 +457,10 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
             const int C = x < 0 ? 0 : readlane(Pre[c], min(x, 63));
             const int px = wave_scan_max(act ? cc - Pre[c] : LCG_NEG);
             const int F = Pre[c] + max(Fcarry - C, px);
-            const int Mv = clampneg(max(s, F));
+            const int Mv = max(s, F);
             if (act) row[j - lo] = Mv;
             Mp[c] = act ? Mv : LCG_NEG;
-            Fcarry = clampneg(readlane(F, 63));
+            Fcarry = max(readlane(F, 63), LCG_NEG);
             sprev = readlane(s, 63);
         }
         cur[0] = nx0;
