@@ -48,6 +48,9 @@ def main():
                     help="skip the secondary configurations (long500, LocARNA-P)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no lcg_batch_autotune before timing)")
+    ap.add_argument("--only", default=None,
+                    choices=["long500_global", "long500_local", "pf16x150", "grad256"],
+                    help="run only this secondary configuration and print its record (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: runs on one GPU only; --gpus %d is not supported (the all-pairs "
@@ -57,6 +60,11 @@ def main():
 
     import torch
     from locarna_amd import api, synth
+
+    if args.only:
+        nthreads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        print(json.dumps(extra_configs(args, api, synth, torch, nthreads, only=args.only)))
+        return
 
     pps, pairs, span = synth.workload(args.workload)
     params = api.locarna_params(max_bp_span=span, sequ_local=1 if args.sequ_local else 0)
@@ -193,10 +201,24 @@ def main():
     kern_ms = sum(dev_ms) / len(dev_ms)
     gpu_scores = [batch.score(k) for k in range(npairs)]
 
-    plain = None
+    # plain baseline: one pair per launch, as mlocarna runs one locarna per
+    # pair (src/Utils/mlocarna:2991-3022).  Each pair is its own one-pair
+    # batch, at the launch shape a one-pair batch takes (its groups dealt to
+    # several workgroups), timed by HIP events around its launch after one
+    # warm-up run; the sum over all pairs is the plain step.  Secondary: mode
+    # 1 of the batch itself (one launch per pair at the batch's shape, one
+    # workgroup per pair for this workload).
+    plain = plain_one_cu = plain_wg = None
     if not args.no_plain:
-        plain_ms = batch.run(mode=1, trace=True)
+        plain_ms = 0.0
+        for p in prepared_host:
+            b1 = api.Batch([p], device=torch.cuda.current_device())
+            b1.run(mode=0, trace=True)
+            plain_ms += b1.run(mode=0, trace=True)
+            plain_wg = b1.workgroups
+            b1.destroy()
         plain = npairs / (plain_ms / 1e3)
+        plain_one_cu = npairs / (batch.run(mode=1, trace=True) / 1e3)
 
     # integer-op roofline of the dominant kernel, lcg_fill_pairs (fill of D
     # over all arc-match groups): W_int = 11 C + 2 K + A over its own cells
@@ -279,7 +301,9 @@ def main():
         "gcups": round(cells / (ms_per_step / 1e3) / 1e9, 3),
         "dp_cells_per_step": cells, "arc_candidates_per_step": cands,
         "plain_pairs_per_sec": round(plain, 2) if plain else None,
+        "plain_workgroups_per_pair": plain_wg,
         "batched_vs_plain": round(value / plain, 2) if plain else None,
+        "plain_one_workgroup_pairs_per_sec": round(plain_one_cu, 2) if plain_one_cu else None,
         "roofline": roofline,
         "end_to_end": end_to_end,
         "arc_match_compaction": comp,
@@ -298,13 +322,15 @@ def main():
 P_FP64_TFLOPS = 78.6
 
 
-def extra_configs(args, api, synth, torch, nthreads):
+def extra_configs(args, api, synth, torch, nthreads, only=None):
     """Secondary figures: the long-pair integer workload (global and
     sequence-local) and the LocARNA-P workloads (inside + outside +
     gradient), each with its CPU oracle timed on a sample of its pairs."""
     from oracle import oracle
     res = {}
     for local in (0, 1):
+        if only and only != "long500_" + ("local" if local else "global"):
+            continue
         pps, pairs, span = synth.workload("long500")
         pa = api.locarna_params(max_bp_span=span, sequ_local=local)
         P = [api.Pair(pps[a], pps[b], pa) for a, b in pairs]
@@ -331,6 +357,8 @@ def extra_configs(args, api, synth, torch, nthreads):
             "scores_bit_exact": "%d/%d" % (sum(1 for x, y in zip(scores, cs) if x == y), len(P)),
             "autotune": tune}
     for wl, nsample in (("pf16x150", 32), ("grad256", 32)):
+        if only and only != wl:
+            continue
         pps, pairs, span = synth.workload(wl)
         pa = api.locarna_p_params(max_bp_span=span)
         P = [api.Pair(pps[a], pps[b], pa) for a, b in pairs]

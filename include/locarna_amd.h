@@ -250,6 +250,21 @@ int lcg_batch_run_penalized(lcg_batch *batch, int64_t position_penalty, float *d
  * than max_iterations iterations. */
 int lcg_batch_run_normalized(lcg_batch *batch, int64_t L, int32_t max_iterations, int32_t *iterations,
                              float *device_ms);
+/* Stream of a batch (a hipStream_t of the batch's device, or NULL for the
+ * batch's own stream; no counterpart in the reference, which runs on the
+ * CPU): every later launch and copy of the batch is ordered on it, after
+ * the work the caller queued there before the call.  Calls stay blocking:
+ * each run returns when its results are complete.  The caller keeps the
+ * stream alive while the batch uses it. */
+int lcg_batch_set_stream(lcg_batch *batch, void *stream);
+/* Bound of the cross-workgroup waits of pairs that several workgroups share
+ * (no counterpart in the reference; for tests of the error path): polls of
+ * the arrival counter before a wait gives up (0: the default, 2^22; -1:
+ * every wait gives up at once without waiting).  A wait that gives up marks
+ * the pair's results invalid; every result request of that run (integer
+ * and LocARNA-P getters alike) then fails with LCG_E_STATE, the first one
+ * and all later ones, until a run completes without such a wait. */
+int lcg_batch_set_spin_limit(lcg_batch *batch, int32_t polls);
 /* launch configuration chosen for the batch: waves per workgroup,
  * workgroups, LDS bytes per workgroup */
 int lcg_batch_config(const lcg_batch *batch, int32_t *waves, int32_t *workgroups, int32_t *lds_bytes);

@@ -94,6 +94,7 @@ def _sig(L):
                                    c.c_char_p, c.c_int]
     L.lcg_batch_destroy.argtypes = [c.c_void_p]
     L.lcg_batch_run.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.POINTER(c.c_float)]
+    L.lcg_batch_set_spin_limit.argtypes = [c.c_void_p, c.c_int32]
     L.lcg_batch_config.argtypes = [c.c_void_p, c.POINTER(c.c_int32), c.POINTER(c.c_int32),
                                    c.POINTER(c.c_int32)]
     L.lcg_batch_score.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_int64)]
@@ -361,6 +362,13 @@ class Batch:
         if rc != 0:
             raise LcgError("lcg_batch_compact_run failed (%d)" % rc)
         return a.value, b.value
+
+    def set_spin_limit(self, polls):
+        """Bound of the cross-workgroup waits (lcg_batch_set_spin_limit):
+        0 default, -1 every wait gives up at once (error-path tests)."""
+        rc = lib().lcg_batch_set_spin_limit(self.h, polls)
+        if rc != 0:
+            raise LcgError("lcg_batch_set_spin_limit (%d)" % rc)
 
     def run(self, mode=0, trace=True, count=False):
         ms = ctypes.c_float()

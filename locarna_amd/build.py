@@ -67,8 +67,13 @@ def build(verbose=False):
     tmp = lib + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
     os.replace(tmp, lib)
+    # locarna_gpu: command-line program on the C++ facade (include/locarna_amd.hpp)
+    exe = os.path.join(OUT, "locarna_gpu")
+    _run([CXX, "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-o", exe + ".tmp", os.path.join(CSRC, "locarna_gpu.cpp"),
+          "-L" + OUT, "-llocarna_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(exe + ".tmp", exe)
     if verbose:
-        print("built", lib)
+        print("built", lib, exe)
     return lib
 
 

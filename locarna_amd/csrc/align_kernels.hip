@@ -735,6 +735,7 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc
     int *pool = lds + bp.hdr_ints;
     const int K = bp.wg_per_pair;
     const int member = K > 1 ? (int)blockIdx.x % K : 0;
+    const int spin_lim = bp.spin_limit ? bp.spin_limit : LCG_SPIN_LIMIT;
     for (int iter = 0;; ++iter) {
         if (K > 1) {
             if (iter > 0) break;
@@ -757,8 +758,8 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc
             if (threadIdx.x == 0) {
                 __threadfence();
                 atomicAdd(&barriers[k], 1);
-                for (int spin = 0; atomicAdd(&barriers[k], 0) < K; ++spin) {
-                    if (spin > LCG_SPIN_LIMIT) {
+                for (int spin = 0; spin_lim < 0 || atomicAdd(&barriers[k], 0) < K; ++spin) {
+                    if (spin > spin_lim) {
                         atomicExch(&arena[d.out + 3], 2);
                         break;
                     }
@@ -816,8 +817,8 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc
                 ++passed;
                 if (threadIdx.x == 0) {
                     atomicAdd(&barriers[k], 1);
-                    for (int spin = 0; atomicAdd(&barriers[k], 0) < K * passed; ++spin) {
-                        if (spin > LCG_SPIN_LIMIT) {
+                    for (int spin = 0; spin_lim < 0 || atomicAdd(&barriers[k], 0) < K * passed; ++spin) {
+                        if (spin > spin_lim) {
                             atomicExch(&arena[d.out + 3], 2);
                             break;
                         }

@@ -496,6 +496,48 @@ void prepare_band_host(lcg_pair &P) {
                                     p->sequ_local != 0, p->free_endgaps);
 }
 
+// Largest magnitude of any alignment score of pair P or partial sum of one
+// (M, E, F, D values).  An alignment of lengths n and m has at most n + m
+// edges, each adding one base-match or gap score and at most one gap
+// opening, and at most min(n, m) / 2 arc matches, each adding its score.
+// The kernels compute in int32 with minus infinity at -2^29 and read values
+// below -2^28 as minus infinity, where the reference's InftyInt is 64-bit
+// (infty_int.hh:56-591), so pairs whose bound exceeds 2^27 are refused
+// rather than narrowed.  For pairs compacted on the device the arc-match
+// scores are not formed on the host; their bound is that of the weights and
+// the tau term (scoring.cc:440-554).
+int64_t score_bound(const lcg_pair &P) {
+    const Scoring &sc = *P.sc;
+    const int64_t n = P.A->length(), m = P.B->length();
+    int64_t smax = 0, gmax = 0, amax = 0;
+    for (int a = 0; a < Scoring::NCODES; ++a)
+        for (int b = 0; b < Scoring::NCODES; ++b) smax = std::max<int64_t>(smax, std::llabs(sc.sigma_code(a, b)));
+    for (int i = 1; i <= n; ++i) gmax = std::max<int64_t>(gmax, std::llabs(sc.gapA(i)));
+    for (int j = 1; j <= m; ++j) gmax = std::max<int64_t>(gmax, std::llabs(sc.gapB(j)));
+    if (P.deferred) {
+        int64_t wa = 0, wb = 0, seq = 2 * smax;
+        for (int k = 0; k < P.bA->num_bps(); ++k) wa = std::max<int64_t>(wa, std::llabs(sc.weightA(k)));
+        for (int k = 0; k < P.bB->num_bps(); ++k) wb = std::max<int64_t>(wb, std::llabs(sc.weightB(k)));
+        for (int x = 0; x < 256; ++x)
+            seq = std::max<int64_t>(seq, std::llabs(Scoring::ribosum_arc_term((x >> 6) & 3, (x >> 4) & 3,
+                                                                                (x >> 2) & 3, x & 3)));
+        amax = wa + wb + (std::llabs((int64_t)P.sp.tau) * seq) / 100 + 1;
+    } else {
+        for (int k = 0; k < P.am->num_arc_matches(); ++k) amax = std::max<int64_t>(amax, std::llabs(sc.arcmatch(k)));
+    }
+    const int64_t alpha = std::llabs((int64_t)P.sp.indel_opening);
+    return (n + m) * (smax + gmax + alpha) + (std::min(n, m) / 2) * amax;
+}
+
+const int64_t LCG_SCORE_BOUND = (int64_t)1 << 27;
+
+void check_score_range(const lcg_pair &P) {
+    const int64_t b = score_bound(P);
+    if (b > LCG_SCORE_BOUND)
+        throw capacity_failure("scores out of range: alignment scores of this pair may reach " + std::to_string(b) +
+                               " in magnitude; the kernels' int32 scores hold at most 2^27");
+}
+
 // Stage 3: base pairs, arc matches (host) or compaction inputs (device),
 // score tables and the device image.
 void prepare_finish(lcg_pair &P, bool deferred) {
@@ -510,10 +552,12 @@ void prepare_finish(lcg_pair &P, bool deferred) {
         P.desc.n = P.A->length();
         P.desc.m = P.B->length();
         P.desc.nam = P.desc.ngroups = -1;
+        check_score_range(P);
         prepare_compaction_inputs(P);
     } else {
         P.am.reset(new ArcMatches(*P.bA, *P.bB, *P.tc, p->max_diff_am, p->max_diff_at_am));
         P.sc.reset(new Scoring(*P.A, *P.B, *P.am, P.sp));
+        check_score_range(P);
         pack(P);
     }
 }
@@ -529,6 +573,9 @@ int prepare(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair **ou
         prepare_finish(*P, deferred);
         *out = P.release();
         return LCG_OK;
+    } catch (capacity_failure &f) {
+        set_err(err, errlen, f.what());
+        return LCG_E_CAPACITY;
     } catch (failure &f) {
         set_err(err, errlen, f.what());
         return LCG_E_INPUT;
@@ -574,9 +621,13 @@ extern "C" int lcg_pair_prepare_scored(const char *ppA, const char *ppB, const l
         P->bB.reset(new BasePairs(m, arcsB));
         P->am.reset(new ArcMatches(*P->bA, *P->bB, *P->tc, lines, p->max_diff_am, p->max_diff_at_am));
         P->sc.reset(new Scoring(*P->A, *P->B, *P->am, P->sp));
+        check_score_range(*P);
         pack(*P);
         *out = P.release();
         return LCG_OK;
+    } catch (capacity_failure &f) {
+        set_err(err, errlen, f.what());
+        return LCG_E_CAPACITY;
     } catch (failure &f) {
         set_err(err, errlen, f.what());
         return LCG_E_INPUT;
@@ -693,7 +744,8 @@ struct lcg_batch {
     int32_t *d_barriers = nullptr;  // per-pair arrival counters (multi-workgroup fill)
     int wg_per_pair = 1;
     int32_t *d_scratch = nullptr;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;      // the stream every launch and copy of the batch goes to
+    hipStream_t own_stream = nullptr;  // the batch's own stream (destroyed with it)
     hipEvent_t ev0 = nullptr, ev1 = nullptr, ev_mid = nullptr;
     float fill_ms = -1, top_ms = -1;  // per-kernel times of the last mode-0 run
     hipGraph_t graph = nullptr;
@@ -712,6 +764,7 @@ struct lcg_batch {
     size_t top_lds_bytes = 0;
     bool have_results = false;
     bool fetched = false;
+    bool run_invalid = false;  // the fetched run's status words reported an expired wait
     bool traced = false;
     // outputs of the last run: per pair the range [edges, out + 12) of its
     // image, packed at res_off[k] (gathered on the device, one copy back)
@@ -727,6 +780,7 @@ struct lcg_batch {
     bool norm_valid = false;
     // names and sequences of every pair, for the alignment text
     std::vector<std::string> nameA, nameB, seqA, seqB;
+    std::vector<int64_t> score_bound;  // per pair, see score_bound()
     std::vector<PfDesc> pdescs;
     std::vector<double> pf_arena;
     double *d_pfa = nullptr;
@@ -1194,6 +1248,7 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
     if ((e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device)) != hipSuccess)
         return hip_fail(e, err, errlen);
     if ((e = hipStreamCreate(&B->stream)) != hipSuccess) return hip_fail(e, err, errlen);
+    B->own_stream = B->stream;
     if ((e = hipEventCreate(&B->ev0)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipEventCreate(&B->ev1)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipEventCreate(&B->ev_mid)) != hipSuccess) return hip_fail(e, err, errlen);
@@ -1306,6 +1361,7 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
     }
     for (int k = 0; k < npairs; ++k) {
         const lcg_pair &X = *pairs[k];
+        B->score_bound.push_back(score_bound(X));
         B->nameA.push_back(X.A->name());
         B->nameB.push_back(X.B->name());
         std::string a, b;
@@ -1346,7 +1402,7 @@ lcg_batch::~lcg_batch() {
     if (B->d_cstats) hipFree(B->d_cstats);
     if (B->d_cdescs) hipFree(B->d_cdescs);
     if (B->d_lambda) hipFree(B->d_lambda);
-    if (B->stream) hipStreamDestroy(B->stream);
+    if (B->own_stream) hipStreamDestroy(B->own_stream);
 }
 
 extern "C" void lcg_batch_destroy(lcg_batch *B) { delete B; }
@@ -1398,6 +1454,21 @@ extern "C" int lcg_batch_compact_run(lcg_batch *B, float *count_ms, float *build
     for (size_t t = 0; t < st.size(); ++t)
         if (st[t] != B->cstats[t]) return LCG_E_STATE;  // a rerun must reproduce the counts and status
     B->have_results = false;
+    return LCG_OK;
+}
+
+extern "C" int lcg_batch_set_stream(lcg_batch *B, void *stream) {
+    if (!B) return LCG_E_ARG;
+    HIPCHECK(hipSetDevice(B->device));
+    HIPCHECK(hipStreamSynchronize(B->stream));  // work queued on the previous stream completes first
+    B->stream = stream ? static_cast<hipStream_t>(stream) : B->own_stream;
+    return LCG_OK;
+}
+
+extern "C" int lcg_batch_set_spin_limit(lcg_batch *B, int32_t polls) {
+    if (!B || polls < -1) return LCG_E_ARG;
+    B->bp.spin_limit = polls;
+    B->bp_top.spin_limit = polls;
     return LCG_OK;
 }
 
@@ -1508,8 +1579,18 @@ int upload_lambda(lcg_batch *B, const std::vector<int32_t> &lam) {
 }
 }  // namespace
 
+// The top level charged a position penalty lam stays in the kernels' score
+// range: lam shifts the score of each aligned position once (a base match
+// twice, scoring.cc:77-90), so by at most 2 |lam| (n + m).
+static bool penalty_in_range(const lcg_batch *B, int k, int64_t lam) {
+    const PairDesc &d = B->descs[k];
+    return B->score_bound[k] + 2 * std::llabs(lam) * (int64_t)(d.n + d.m) <= LCG_SCORE_BOUND;
+}
+
 extern "C" int lcg_batch_run_penalized(lcg_batch *B, int64_t position_penalty, float *device_ms) {
     if (!B || position_penalty < -100000 || position_penalty > 100000) return LCG_E_ARG;
+    for (int k = 0; k < B->npairs; ++k)
+        if (!penalty_in_range(B, k, position_penalty)) return LCG_E_CAPACITY;
     if (int rc = upload_lambda(B, std::vector<int32_t>(B->npairs, (int32_t)position_penalty))) return rc;
     if (device_ms) *device_ms = 0;
     return run_with_penalty(B, false, device_ms);
@@ -1562,6 +1643,7 @@ extern "C" int lcg_batch_run_normalized(lcg_batch *B, int64_t L, int32_t max_ite
                 continue;
             }
             if (next < -100000 || next > 100000) return LCG_E_STATE;
+            if (!penalty_in_range(B, k, next)) return LCG_E_CAPACITY;
             lam[k] = (int32_t)next;
             all = false;
         }
@@ -1582,17 +1664,19 @@ namespace {
 int fetch(const lcg_batch *Bc) {
     lcg_batch *B = const_cast<lcg_batch *>(Bc);
     if (!B->have_results) return LCG_E_STATE;
-    if (B->fetched) return LCG_OK;
+    if (B->fetched) return B->run_invalid ? LCG_E_STATE : LCG_OK;
     HIPCHECK(hipSetDevice(B->device));
     if (int rc = lcg_launch_gather(B->d_descs, B->d_resoff, B->d_arena, B->d_res, B->npairs, B->stream))
         return hip_fail((hipError_t)rc);
     HIPCHECK(hipMemcpyAsync(B->result.data(), B->d_res, B->result.size() * 4, hipMemcpyDeviceToHost, B->stream));
     HIPCHECK(hipStreamSynchronize(B->stream));
-    B->fetched = true;
-    // a bounded wait of the multi-workgroup fill that expired invalidates the run
+    // a bounded wait of the multi-workgroup fill that expired invalidates the
+    // run: every later result request of this run fails too
+    B->run_invalid = false;
     for (int k = 0; k < B->npairs; ++k)
-        if (B->result[B->res_off[k] + (B->descs[k].out + 3 - B->descs[k].edges)] != 0) return LCG_E_STATE;
-    return LCG_OK;
+        if (B->result[B->res_off[k] + (B->descs[k].out + 3 - B->descs[k].edges)] != 0) B->run_invalid = true;
+    B->fetched = true;
+    return B->run_invalid ? LCG_E_STATE : LCG_OK;
 }
 
 // output word at arena offset `off` (in [edges, out + 12)) of pair k
@@ -1978,12 +2062,14 @@ void pf_tables(const lcg_batch &B, int k, const double *dth, double *A) {
 
 extern "C" int lcg_batch_pf_run(lcg_batch *B, const double *dtheta, float *device_ms) {
     if (!B) return LCG_E_ARG;
+    B->pf_have = false;  // the results of an earlier run are overwritten from here on
     if (B->bp.free_endgaps) return LCG_E_ARG;  // free end gaps: integer path only
     if (B->device_built) return LCG_E_STATE;  // LocARNA-P reads the host arc-match lists
     if (int rc = pf_setup(*B)) return rc;
     for (int k = 0; k < B->npairs; ++k) pf_tables(*B, k, dtheta, B->pf_arena.data());
     HIPCHECK(hipSetDevice(B->device));
     HIPCHECK(hipMemcpy(B->d_pfa, B->pf_arena.data(), B->pf_arena.size() * 8, hipMemcpyHostToDevice));
+    B->pbp.spin_limit = B->bp.spin_limit;
     HIPCHECK(hipEventRecord(B->ev0, B->stream));
     int rc = lcg_launch_pf(B->d_descs, B->d_pdescs, B->d_order, B->d_queue, B->d_arena, B->d_pfa, B->d_pfscratch,
                            B->d_pfsched, B->d_barriers, B->pbp, B->pf_grid, B->pf_waves, B->pf_lds, B->stream);
@@ -2749,7 +2835,7 @@ namespace {
 // fn(k) for k in [0, n) on `threads` host threads.  Returns the first k
 // whose call threw (its message in msg), or -1.
 template <class Fn>
-int parallel_for(int n, int threads, Fn fn, std::string &msg) {
+int parallel_for(int n, int threads, Fn fn, std::string &msg, bool *capacity = nullptr) {
     std::atomic<int> next{0}, bad{-1};
     std::mutex mu;
     auto work = [&]() {
@@ -2762,6 +2848,7 @@ int parallel_for(int n, int threads, Fn fn, std::string &msg) {
                 std::lock_guard<std::mutex> g(mu);
                 if (bad.load() < 0) {
                     msg = e.what();
+                    if (capacity) *capacity = dynamic_cast<capacity_failure *>(&e) != nullptr;
                     bad.store(k);
                 }
             }
@@ -3010,8 +3097,12 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
     }
     band_wall_ms = since(t0);
     t0 = clk::now();
-    bad = parallel_for(npairs, threads, [&](int k) { prepare_finish(*P[k], deferred); }, msg);
-    if (bad >= 0) return input_error(bad);
+    bool capacity = false;
+    bad = parallel_for(npairs, threads, [&](int k) { prepare_finish(*P[k], deferred); }, msg, &capacity);
+    if (bad >= 0) {
+        const int rc = input_error(bad);
+        return capacity ? LCG_E_CAPACITY : rc;
+    }
     for (int k = 0; k < npairs; ++k) out[k] = P[k].release();
     if (stats) {
         std::memset(stats, 0, sizeof *stats);

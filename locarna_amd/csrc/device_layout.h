@@ -92,5 +92,7 @@ typedef struct {
     int32_t fblk_lds;
     int32_t wg_per_pair;     // workgroups sharing one pair in the fill of D
     int32_t free_endgaps;    // top level: LCG_FREE_* bits (left_1 1, right_1 2, left_2 4, right_2 8)
-    int32_t pad[2];
+    int32_t spin_limit;      // polls of a bounded cross-workgroup wait (0: default,
+                             // -1: every wait gives up at once, for error-path tests)
+    int32_t pad[1];
 } BatchParams;

@@ -22,6 +22,10 @@ typedef long score_t;  // reference: scoring_fwd.hh:13-17
 struct failure : std::runtime_error {
     explicit failure(const std::string &s) : std::runtime_error(s) {}
 };
+// A well-formed problem beyond what the kernels represent (LCG_E_CAPACITY).
+struct capacity_failure : failure {
+    explicit capacity_failure(const std::string &s) : failure(s) {}
+};
 
 // Sequence plus filtered base-pair probabilities read from #PP 2.0 text.
 // reference: RnaData, rna_data.hh:37; reader rna_data.cc:984-1124

@@ -724,6 +724,7 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
     const bool stat = (int)blockIdx.x < bp.wg_per_pair * bp.nstatic;
     const int K = stat ? bp.wg_per_pair : 1;
     const int member = stat ? (int)blockIdx.x % K : 0;
+    const int spin_lim = bp.spin_limit ? bp.spin_limit : LCG_PF_SPIN_LIMIT;
     for (int iter = 0;; ++iter) {
         if (stat) {
             if (iter > 0) break;
@@ -778,8 +779,8 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
             ++passed;
             if (threadIdx.x == 0) {
                 atomicAdd(&barriers[k], 1);
-                for (int spin = 0; atomicAdd(&barriers[k], 0) < K * passed; ++spin) {
-                    if (spin > LCG_PF_SPIN_LIMIT) {
+                for (int spin = 0; spin_lim < 0 || atomicAdd(&barriers[k], 0) < K * passed; ++spin) {
+                    if (spin > spin_lim) {
                         pfa[f.out + 2] = 1.0;
                         break;
                     }

@@ -42,7 +42,8 @@ typedef struct {
     int32_t nstatic;         // pairs (in queue order) with wg_per_pair
                              // workgroups each, assigned by block index; the
                              // other workgroups take the rest from the queue
-    int32_t pad[1];
+    int32_t spin_limit;      // polls of a bounded cross-workgroup wait (0: default,
+                             // -1: every wait gives up at once, for error-path tests)
 } PfBatchParams;
 
 // score parameters theta: sigma 4x4, arc-match term 16x16, indel,

@@ -6,9 +6,22 @@
  * (paths relative to the LocARNA source tree).
  *
  * Pinning (how the oracle itself is checked, see tests/test_oracle_*.py):
- *   - pp reader, RIBOSUM reader: the reference's own data files
- *     (Data/Examples/example.pp, Data/Matrices/RIBOSUM85_60) with values
- *     checked by hand-derivable properties;
+ *   - pp reader filter (p == cut dropped, span == maxBPspan kept, #BPCUT
+ *     only raising the cut), BasePairs arc order (left end descending, right
+ *     end ascending), arc weights (0 at p = pe, struct_weight at p = 1,
+ *     half of it at sqrt(pe), monotone, --exp-prob), the truncating tau
+ *     division, the band's StralScore similarity (left ends with left ends)
+ *     and the trace cut (the smallest monotone band holding every cell with
+ *     probability >= the cut, ties included): closed forms on hand-built
+ *     pp inputs (tests/test_oracle_tables.py), each failing when a
+ *     plausible mistake is planted in this file;
+ *   - RIBOSUM reader: the data file's printed base-match and arc-match
+ *     matrices are reproduced from its frequency tables, and the band's
+ *     similarity equals the printed base-match matrix at unpaired cells;
+ *   - parity unpinned: the corrected RIBOSUM base-match score sigma
+ *     (ribosum.cc:325-331) and the riboX arc-match term
+ *     (scoring.cc:368-438) follow the reference's formulas, which no check
+ *     available here fixes independently;
  *   - trace band (PFGotoh, PFTraceProbs): enumeration of all alignments
  *     (global) or of all prefixes and suffixes (sequence-local) on tiny
  *     inputs, trace probabilities to 1e-12 in long double and double

@@ -1,0 +1,39 @@
+"""locarna_gpu, the program on the C++ facade (include/locarna_amd.hpp), on
+the CPU: usage errors exit with 2, library failures (LocARNA::failure in the
+reference, a status code here) exit with 1 and print the library's message;
+none of these reaches a GPU call."""
+import os
+import subprocess
+
+from locarna_amd import synth
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "locarna_amd", "_build", "locarna_gpu")
+
+
+def _run(args):
+    return subprocess.run([EXE] + args, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=60)
+
+
+def test_usage_errors():
+    assert _run([]).returncode == 2
+    assert _run(["a.pp"]).returncode == 2
+    assert _run(["a.pp", "b.pp", "--no-such-option"]).returncode == 2
+
+
+def test_failures_report_status_and_message(tmp_path):
+    seqs = synth.sequences(43, 2, 60, 60, 0.5, 0.5)
+    fa, fb, bad = tmp_path / "a.pp", tmp_path / "b.pp", tmp_path / "bad.pp"
+    fa.write_text(synth.make_pp("a", seqs[0], 100))
+    fb.write_text(synth.make_pp("b", seqs[1], 100))
+    bad.write_text("not a pp file\n")
+    # free end gaps with sequence-local alignment (src/locarna.cc:375-378)
+    r = _run([str(fa), str(fb), "--sequ-local", "--free-endgaps=+---"])
+    assert r.returncode == 1 and "cannot be combined with local alignment" in r.stderr
+    r = _run([str(bad), str(fb)])
+    assert r.returncode == 1 and "status -2" in r.stderr and "#PP 2.0" in r.stderr
+    r = _run([str(tmp_path / "missing.pp"), str(fb)])
+    assert r.returncode == 1 and "Cannot read file" in r.stderr
+    # scores beyond the kernels' int32 range: LCG_E_CAPACITY
+    r = _run([str(fa), str(fb), "--indel=-2000000"])
+    assert r.returncode == 1 and "status -4" in r.stderr
