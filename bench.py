@@ -408,8 +408,11 @@ def extra_configs(args, api, synth, torch, nthreads, only=None):
             target = sampled_statistics(B, theta_gen, samples=64, seed=6).sum(0)
             t_sample = time.perf_counter() - t0
             t0 = time.perf_counter()
-            losses, dist, _ = fit_to_target(B, theta_gen, target, 20, 2000.0, None)
+            losses, dist, _ = fit_to_target(B, theta_gen, target, 20, 0.7, None, method="newton", samples=32,
+                                            ridge=1.0 / (30.0 * 30.0 * 64))
             fit = {"steps": 20, "s_per_step": round((time.perf_counter() - t0) / 20, 3),
+                   "method": "Newton steps, diagonal Hessian from 32 sampled alignments per pair and step; "
+                             "Gaussian prior of the generating theta (sd 30) as a ridge term",
                    "target": "64 sampled alignments per pair at the generating theta",
                    "sampling_s": round(t_sample, 3),
                    "loss_first": round(losses[0], 4), "loss_last": round(losses[-1], 4),

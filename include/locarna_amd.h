@@ -60,6 +60,13 @@ typedef struct {
     double pf_scale;           /* --pf-scale                                */
     double min_am_prob;        /* --min-am-prob                             */
     double min_bm_prob;        /* --min-bm-prob                             */
+    int32_t no_lonely_pairs;   /* --noLP: arc matches only in stacks of two
+                                  or more (aligner.cc:612-657, 1030-1078);
+                                  integer path with host arc matches
+                                  (lcg_pair_prepare, lcg_pairs_prepare without
+                                  LCG_PREPARE_DEVICE_COMPACTION); LocARNA-P
+                                  refuses it                                */
+    int32_t reserved;
 } lcg_params;
 
 /* free end gaps (FreeEndgaps, free_endgaps.hh: the characters of the
@@ -180,7 +187,7 @@ typedef struct {
     int32_t wg_per_pair;    /* fill: workgroups per pair, 1..64, with fewer pairs than CUs (CUs / pairs) */
     int32_t top_waves;      /* top level + traceback: waves per workgroup, 1..16 (4) */
     int32_t top_wgs_per_cu; /* top level + traceback: workgroups per CU, 1..4 (4) */
-    int32_t pf_waves;       /* LocARNA-P: waves per workgroup, 1..16 (16) */
+    int32_t pf_waves;       /* LocARNA-P: waves per workgroup, 1..8 (8) */
     int32_t pf_wg_per_pair; /* LocARNA-P: workgroups per pair, 1..64; 0: two for the largest pairs where
                                that shortens the estimated makespan, else CUs / pairs */
     int32_t reserved;
@@ -255,7 +262,9 @@ int lcg_batch_run_normalized(lcg_batch *batch, int64_t L, int32_t max_iterations
  * CPU): every later launch and copy of the batch is ordered on it, after
  * the work the caller queued there before the call.  Calls stay blocking:
  * each run returns when its results are complete.  The caller keeps the
- * stream alive while the batch uses it. */
+ * stream alive while the batch uses it.  The stream belongs to the HIP
+ * runtime the library runs on: the libamdhip64.so.7 the process loaded
+ * first (a PyTorch that is imported before the library brings its own). */
 int lcg_batch_set_stream(lcg_batch *batch, void *stream);
 /* Bound of the cross-workgroup waits of pairs that several workgroups share
  * (no counterpart in the reference; for tests of the error path): polls of

@@ -4,7 +4,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "liblocarna_amd.so")
+# LOCARNA_AMD_LIB selects another build of the same library (the sanitizer
+# build of locarna_amd/build.py)
+LIB_PATH = os.environ.get("LOCARNA_AMD_LIB") or os.path.join(_HERE, "_build", "liblocarna_amd.so")
 _lib = None
 
 

@@ -19,6 +19,7 @@ class Params(ctypes.Structure):
         ("min_prob", ctypes.c_double), ("min_trace_prob", ctypes.c_double),
         ("exp_prob", ctypes.c_double), ("pf_scale", ctypes.c_double),
         ("min_am_prob", ctypes.c_double), ("min_bm_prob", ctypes.c_double),
+        ("no_lonely_pairs", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -95,6 +96,7 @@ def _sig(L):
     L.lcg_batch_destroy.argtypes = [c.c_void_p]
     L.lcg_batch_run.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.POINTER(c.c_float)]
     L.lcg_batch_set_spin_limit.argtypes = [c.c_void_p, c.c_int32]
+    L.lcg_batch_set_stream.argtypes = [c.c_void_p, c.c_void_p]
     L.lcg_batch_config.argtypes = [c.c_void_p, c.POINTER(c.c_int32), c.POINTER(c.c_int32),
                                    c.POINTER(c.c_int32)]
     L.lcg_batch_score.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_int64)]
@@ -369,6 +371,14 @@ class Batch:
         rc = lib().lcg_batch_set_spin_limit(self.h, polls)
         if rc != 0:
             raise LcgError("lcg_batch_set_spin_limit (%d)" % rc)
+
+    def set_stream(self, stream):
+        """Runs on a caller's HIP stream (an integer hipStream_t handle of
+        the process's HIP runtime, e.g. torch.cuda.Stream().cuda_stream when
+        torch was imported before the library; 0 / None: the batch's own)."""
+        rc = lib().lcg_batch_set_stream(self.h, ctypes.c_void_p(stream or None))
+        if rc != 0:
+            raise LcgError("lcg_batch_set_stream (%d)" % rc)
 
     def run(self, mode=0, trace=True, count=False):
         ms = ctypes.c_float()

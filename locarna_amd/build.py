@@ -47,26 +47,57 @@ def _run(cmd):
     return r.stdout
 
 
-def build(verbose=False):
-    os.makedirs(OUT, exist_ok=True)
+# host code with AddressSanitizer and UndefinedBehaviorSanitizer (CPU checks
+# of the packing, scheduling, sampling and input code; the device code is
+# built as usual); loaded by tests run with LOCARNA_AMD_LIB pointing at it
+# and the sanitizer runtimes preloaded (see sanitize_env)
+SAN_OUT = os.path.join(HERE, "_build_san")
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-fno-sanitize-recover=undefined"]
+
+
+def sanitize_env():
+    """Environment for running the CPU tests against the sanitizer build."""
+    libs = [subprocess.run([CXX, "-print-file-name=" + n], stdout=subprocess.PIPE, text=True).stdout.strip()
+            for n in ("libasan.so", "libubsan.so")]
+    env = dict(os.environ)
+    env["LD_PRELOAD"] = ":".join(libs)
+    env["LOCARNA_AMD_LIB"] = os.path.join(SAN_OUT, "liblocarna_amd.so")
+    # leaks of the interpreter and of the HIP runtime are not this library's
+    env["ASAN_OPTIONS"] = "detect_leaks=0:verify_asan_link_order=0:abort_on_error=1"
+    env["UBSAN_OPTIONS"] = "print_stacktrace=1:halt_on_error=1"
+    return env
+
+
+def build(verbose=False, sanitize=False):
+    out = SAN_OUT if sanitize else OUT
+    host_flags = HOST_FLAGS + (SAN_FLAGS if sanitize else [])
+    os.makedirs(out, exist_ok=True)
     objs = []
     for src in HOST_SOURCES:
-        obj = os.path.join(OUT, os.path.splitext(src)[0] + ".o")
-        _run([CXX] + HOST_FLAGS + ["-c", "-o", obj, os.path.join(CSRC, src)])
+        obj = os.path.join(out, os.path.splitext(src)[0] + ".o")
+        _run([CXX] + host_flags + ["-c", "-o", obj, os.path.join(CSRC, src)])
         objs.append(obj)
     for src in HIPAPI_SOURCES:
-        obj = os.path.join(OUT, os.path.splitext(src)[0] + ".o")
-        _run([CXX] + HOST_FLAGS + ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", "-o", obj,
+        obj = os.path.join(out, os.path.splitext(src)[0] + ".o")
+        _run([CXX] + host_flags + ["-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c", "-o", obj,
                                    os.path.join(CSRC, src)])
         objs.append(obj)
     for src in HIP_SOURCES:
-        obj = os.path.join(OUT, os.path.splitext(src)[0] + ".hip.o")
+        obj = os.path.join(out, os.path.splitext(src)[0] + ".hip.o")
         _run([HIPCC] + HIP_FLAGS + ["-c", "-o", obj, os.path.join(CSRC, src)])
         objs.append(obj)
-    lib = os.path.join(OUT, "liblocarna_amd.so")
+    lib = os.path.join(out, "liblocarna_amd.so")
     tmp = lib + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + ["-lpthread"])
+    link_san = []
+    if sanitize:
+        for f in SAN_FLAGS[:1]:
+            link_san += ["-Xarch_host", f]
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs + link_san + ["-lpthread"])
     os.replace(tmp, lib)
+    if sanitize:
+        if verbose:
+            print("built", lib)
+        return lib
     # locarna_gpu: command-line program on the C++ facade (include/locarna_amd.hpp)
     exe = os.path.join(OUT, "locarna_gpu")
     _run([CXX, "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-o", exe + ".tmp", os.path.join(CSRC, "locarna_gpu.cpp"),
@@ -78,4 +109,10 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "sanitize":
+        # python -m locarna_amd.build sanitize [pytest args]: builds the
+        # sanitizer library and runs the CPU tests against it
+        build(verbose=True, sanitize=True)
+        args = sys.argv[2:] or ["tests", "-m", "not gpu", "-q", "-x"]
+        sys.exit(subprocess.run([sys.executable, "-m", "pytest"] + args, cwd=ROOT, env=sanitize_env()).returncode)
     build(verbose=True)

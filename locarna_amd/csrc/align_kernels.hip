@@ -107,8 +107,11 @@ struct Pair {
     int *D;
     int *edges;
     int *structs;
-    int n, m;
+    const int *inner;  // --noLP: arc match -> arc match directly inside it (-1: none)
+    const int *c2a;    // --noLP: candidate entry -> arc match
+    int n, m, nam;
     int alpha, alpha_pos;
+    int nolp;
 };
 
 // Row i's stored column range in the group with origin (al, bl) and
@@ -609,7 +612,7 @@ __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W
         const int cell = P.coff[i] + (j - P.mn[i]);
         const int t0 = P.cptr[cell], t1 = P.cptr[cell + 1];
         bool found = false;
-        int fp = 0, fq = 0;
+        int fp = 0, fq = 0, ft = 0;
         for (int base = t0; base < t1 && !found; base += 64) {
             const int t = base + lane;
             bool hit = false;
@@ -627,6 +630,7 @@ __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W
                 found = true;
                 fp = readlane(p, src);
                 fq = readlane(q, src);
+                ft = readlane(t, src);
             }
         }
         if (!found) break;
@@ -637,9 +641,35 @@ __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W
             P.structs[i - 1] = 2;
             P.structs[P.n + fq - 1] = 1;
             P.structs[P.n + j - 1] = 2;
+            int a = fp, b = fq, c = i, e = j;
+            if (P.nolp) {
+                // aligner.cc:1030-1078: the stack downwards; each inner arc
+                // match continues it while D(x) == D(inner) + s(x), the last
+                // one's region is traced as a task
+                int x = P.c2a[ft];
+                for (int depth = 0; depth < P.n && (unsigned)x < (unsigned)P.nam; ++depth) {
+                    // each step nests deeper (at most n / 2 steps); a traced
+                    // arc match always has an inner one, else its D is minus
+                    // infinity
+                    const int y = P.inner[x];
+                    if ((unsigned)y >= (unsigned)P.nam) break;
+                    ++a;
+                    ++b;
+                    --c;
+                    --e;
+                    emit(P, a, b, a + b);
+                    emit(P, c, e, c + e);
+                    P.structs[a - 1] = 1;
+                    P.structs[c - 1] = 2;
+                    P.structs[P.n + b - 1] = 1;
+                    P.structs[P.n + e - 1] = 2;
+                    if (P.D[x] != P.D[y] + P.amscore[x]) break;
+                    x = y;
+                }
+            }
             const int slot = atomicAdd(tcount, 1);
-            tstack[2 * slot] = fp | (fq << 16);
-            tstack[2 * slot + 1] = i | (j << 16);
+            tstack[2 * slot] = a | (b << 16);
+            tstack[2 * slot + 1] = c | (e << 16);
         }
         i = fp - 1;
         j = fq - 1;
@@ -709,10 +739,14 @@ __device__ __forceinline__ Pair load_pair(const PairDesc &d, int *arena, int *ld
     P.D = arena + d.D;
     P.edges = arena + d.edges;
     P.structs = arena + d.structs;
+    P.inner = arena + d.inner;
+    P.c2a = arena + d.c2a;
     P.n = n;
     P.m = m;
+    P.nam = d.nam;
     P.alpha = bp.alpha;
     P.alpha_pos = bp.alpha > 0 ? bp.alpha : 0;
+    P.nolp = bp.nolp;
     return P;
 }
 
@@ -799,7 +833,16 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc
                     for (int t = G[3] + lane; t < G[4]; t += 64) {
                         const int4 ga = gams[t];
                         const int ar = ga.x & 0xffff, br = ga.x >> 16;
-                        const int v = Mst[(ar - 1 - gal) * W + (br - 1 - row_lo(P, ar - 1, gbl))] + P.amscore[ga.z];
+                        const int mv = Mst[(ar - 1 - gal) * W + (br - 1 - row_lo(P, ar - 1, gbl))];
+                        int v = mv + P.amscore[ga.z];
+                        if (P.nolp) {
+                            // aligner.cc:612-657: the inner arc match (its right
+                            // ends are ar, br) ends the stack or continues it;
+                            // its D comes from a later row, written before
+                            const int y = P.inner[ga.z];  // always one (the host lists no other)
+                            v = (unsigned)y < (unsigned)P.nam ? max(mv + P.amscore[y], P.D[y]) + P.amscore[ga.z]
+                                                              : LCG_NEG;
+                        }
                         P.cand[ga.y].z = v;  // D, not normalized (the reference's InftyInt sum)
                         P.fcand[3 * ga.w + 2] = v;
                         P.D[ga.z] = v;

@@ -82,7 +82,7 @@ int32_t reserve(std::vector<int32_t> &blob, size_t count, int32_t fill = 0) {
 // Offsets of a pair's arena image (device_layout.h) from its sizes, in the
 // order the host packing reserves them; the device compaction lays out its
 // pairs with it, so both produce the same image.  Returns the image size.
-size_t layout_pair(PairDesc &d, int n, int m, int ncells, int ngroups, int nam, int nfblk) {
+size_t layout_pair(PairDesc &d, int n, int m, int ncells, int ngroups, int nam, int nfblk, bool nolp = false) {
     size_t off = 0;
     auto take = [&](size_t count) {
         const int32_t o = (int32_t)off;
@@ -113,8 +113,38 @@ size_t layout_pair(PairDesc &d, int n, int m, int ncells, int ngroups, int nam, 
     d.tstack = take(2 * (size_t)(n + 1));
     align(2);
     d.out = take(12);
+    d.inner = take(nolp ? nam : 0);
+    d.c2a = take(nolp ? nam : 0);
     align(4);
     return off;
+}
+
+// Moves every array offset of a pair's image (PairDesc fields that
+// layout_pair sets) by `base`, the image's place in the batch arena.
+void rebase(PairDesc &d, int32_t base) {
+    int32_t *offs[] = {&d.band, &d.codes, &d.gaps, &d.sigma, &d.rows, &d.groups, &d.gams, &d.amscore,
+                       &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.out, &d.rounds,
+                       &d.fptr, &d.fblk, &d.fcand, &d.f2c, &d.inner, &d.c2a};
+    for (int32_t *o : offs) *o += base;
+}
+
+// A rebased image lies in the arena at [base, base + size): every array
+// offset in that range and, with --noLP, its arc-match indices in range
+// (checked on the host before any launch).
+bool image_in_place(const PairDesc &d, int32_t base, int32_t size, const std::vector<int32_t> &arena) {
+    const int32_t offs[] = {d.band, d.codes, d.gaps, d.sigma, d.rows, d.groups, d.gams, d.amscore, d.cptr, d.cand,
+                            d.D, d.edges, d.structs, d.tstack, d.out, d.rounds, d.fptr, d.fblk, d.fcand, d.f2c,
+                            d.inner, d.c2a};
+    for (int32_t o : offs)
+        if (o < base || o > base + size) return false;
+    if (d.c2a > d.inner) {  // --noLP arrays present
+        if (d.inner + d.nam > base + size || d.c2a + d.nam > base + size) return false;
+        for (int k = 0; k < d.nam; ++k) {
+            if (arena[d.inner + k] < -1 || arena[d.inner + k] >= d.nam) return false;
+            if (arena[d.c2a + k] < 0 || arena[d.c2a + k] >= d.nam) return false;
+        }
+    }
+    return true;
 }
 
 // Top-level region of a pair: stored row width and LDS ints; adds the
@@ -176,6 +206,35 @@ void pack(lcg_pair &P) {
         by_left[(size_t)a.left * (m + 1) + b.left].push_back(k);
         by_right[(size_t)a.right * (m + 1) + b.right].push_back(k);
     }
+    // --noLP: the arc match of the arcs directly inside both arcs, the first
+    // of its left-end list (arc_matches.cc:50-74); the group (al, bl) then
+    // computes the D of the arc matches with left ends (al-1, bl-1) that have
+    // one, over the inner arc match's region (aligner.cc:612-657,
+    // arc_matches.cc:313-355)
+    const bool nolp = P.params.no_lonely_pairs != 0;
+    std::vector<int> inner(nolp ? nam : 0, -1);
+    std::vector<std::vector<int>> by_group;
+    if (nolp) {
+        for (int k = 0; k < nam; ++k) {
+            const ArcMatch &x = ams.arcmatch(k);
+            const Arc &a = bA.arc(x.arcA), &b = bB.arc(x.arcB);
+            if (a.left + 1 > n || b.left + 1 > m) continue;
+            for (int y : by_left[(size_t)(a.left + 1) * (m + 1) + (b.left + 1)]) {
+                const ArcMatch &z = ams.arcmatch(y);
+                if (bA.arc(z.arcA).right == a.right - 1 && bB.arc(z.arcB).right == b.right - 1) {
+                    inner[k] = y;
+                    break;
+                }
+            }
+        }
+        by_group.assign((size_t)(n + 1) * (m + 1), {});
+        for (int k = 0; k < nam; ++k) {
+            if (inner[k] < 0) continue;
+            const ArcMatch &x = ams.arcmatch(k);
+            by_group[(size_t)(bA.arc(x.arcA).left + 1) * (m + 1) + (bB.arc(x.arcB).left + 1)].push_back(k);
+        }
+    }
+    const auto &group_lists = nolp ? by_group : by_left;
     // groups, al descending, bl descending (aligner.cc:659-732)
     struct G {
         int al, bl, ar, br, W, b, e, lds;
@@ -195,13 +254,16 @@ void pack(lcg_pair &P) {
         const int max_bl = std::min(m, tc.max_col(al)), min_bl = std::max(1, tc.min_col(al));
         for (int bl = max_bl; bl >= min_bl; --bl) {
             if (!tc.is_valid_match(al, bl)) continue;
-            const auto &lst = by_left[(size_t)al * (m + 1) + bl];
+            const auto &lst = group_lists[(size_t)al * (m + 1) + bl];
             if (lst.empty()) continue;
+            // right ends of the region's arc match: the arc match itself, or
+            // with --noLP its inner one
+            const int dr = nolp ? 1 : 0;
             int ar = al, br = bl;
             for (int k : lst) {
                 const ArcMatch &x = ams.arcmatch(k);
-                ar = std::max(ar, bA.arc(x.arcA).right);
-                br = std::max(br, bB.arc(x.arcB).right);
+                ar = std::max(ar, bA.arc(x.arcA).right - dr);
+                br = std::max(br, bB.arc(x.arcB).right - dr);
             }
             int W = 0;
             for (int i = al; i < ar; ++i) {
@@ -214,7 +276,7 @@ void pack(lcg_pair &P) {
             G g{al, bl, ar, br, W, (int)gams.size() / 4, 0, lds};
             for (int k : lst) {
                 const ArcMatch &x = ams.arcmatch(k);
-                gams.push_back(bA.arc(x.arcA).right | (bB.arc(x.arcB).right << 16));
+                gams.push_back((bA.arc(x.arcA).right - dr) | ((bB.arc(x.arcB).right - dr) << 16));
                 gams.push_back(0);  // candidate entry, set below
                 gams.push_back(k);
                 gams.push_back(0);
@@ -242,7 +304,7 @@ void pack(lcg_pair &P) {
         G8[7] = groups[g].lds;
     }
     while (blob.size() & 3) blob.push_back(0);
-    d.gams = reserve(blob, gams.size());
+    d.gams = reserve(blob, 4 * (size_t)nam);  // (--noLP: only the arc matches with an inner one have an entry)
     d.amscore = reserve(blob, nam);
     for (int k = 0; k < nam; ++k) blob[d.amscore + k] = (int32_t)sc.arcmatch(k);
     // candidate CSR over band cells, each list sorted (al desc, bl desc);
@@ -328,7 +390,9 @@ void pack(lcg_pair &P) {
     for (int k = 0; k < nam; ++k) blob[d.f2c + fentry_of[k]] = entry_of[k];
     P.entry_of = entry_of;
     std::copy(gams.begin(), gams.end(), blob.begin() + d.gams);
-    d.D = reserve(blob, nam);
+    // every D is written by the fill, except with --noLP those of the arc
+    // matches without an inner one, which stay minus infinity
+    d.D = reserve(blob, nam, nolp ? LCG_NEG : 0);
     d.edges = reserve(blob, (size_t)n + m + 1, -1);
     d.structs = reserve(blob, (size_t)n + m);
     d.tstack = reserve(blob, 2 * (size_t)(n + 1));
@@ -344,11 +408,17 @@ void pack(lcg_pair &P) {
     // blob is a multiple of 4 ints, so blob offsets survive concatenation)
     if (blob.size() & 1) blob.push_back(0);
     d.out = reserve(blob, 12);
+    d.inner = reserve(blob, nolp ? nam : 0);
+    d.c2a = reserve(blob, nolp ? nam : 0);
+    if (nolp) {
+        std::copy(inner.begin(), inner.end(), blob.begin() + d.inner);
+        for (int k = 0; k < nam; ++k) blob[d.c2a + entry_of[k]] = k;
+    }
     while (blob.size() & 3) blob.push_back(0);
     d.gmax = gbuf;
     {
         PairDesc L = d;
-        const size_t size = layout_pair(L, n, m, ncells, d.ngroups, nam, d.nfblk);
+        const size_t size = layout_pair(L, n, m, ncells, d.ngroups, nam, d.nfblk, nolp);
         if (size != blob.size() || std::memcmp(&L, &d, sizeof d) != 0) throw failure("internal: arena layout");
     }
     P.desc = d;
@@ -565,6 +635,10 @@ void prepare_finish(lcg_pair &P, bool deferred) {
 int prepare(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair **out, char *err, int errlen,
             bool deferred) {
     if (!ppA || !ppB || !p || !out) return LCG_E_ARG;
+    if (deferred && p->no_lonely_pairs) {  // the device compaction builds no --noLP groups
+        set_err(err, errlen, "--noLP needs the arc matches enumerated on the host");
+        return LCG_E_ARG;
+    }
     *out = nullptr;
     try {
         auto P = std::make_unique<lcg_pair>();
@@ -1060,11 +1134,7 @@ int device_count_and_layout(lcg_batch &B, lcg_pair *const *pairs, int npairs, st
     B.arena_ints = total;
     size_t base = 0;
     for (int k = 0; k < npairs; ++k) {
-        PairDesc &d = B.descs[k];
-        int32_t *offs[] = {&d.band, &d.codes, &d.gaps, &d.sigma, &d.rows, &d.groups, &d.gams, &d.amscore,
-                           &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.out, &d.rounds,
-                           &d.fptr, &d.fblk, &d.fcand, &d.f2c};
-        for (int32_t *o : offs) *o += (int32_t)base;
+        rebase(B.descs[k], (int32_t)base);
         base += size[k];
     }
     return LCG_OK;
@@ -1233,8 +1303,9 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
     for (int k = 0; k < npairs; ++k) {
         const lcg_pair *P = pairs[k];
         if (P->params.indel_opening != p0.indel_opening || P->params.sequ_local != p0.sequ_local ||
-            P->params.free_endgaps != p0.free_endgaps) {
-            set_err(err, errlen, "all pairs of a batch must share indel_opening, sequ_local and free_endgaps");
+            P->params.free_endgaps != p0.free_endgaps || P->params.no_lonely_pairs != p0.no_lonely_pairs) {
+            set_err(err, errlen,
+                    "all pairs of a batch must share indel_opening, sequ_local, free_endgaps and no_lonely_pairs");
             return LCG_E_ARG;
         }
         if (P->deferred != deferred) {
@@ -1271,10 +1342,11 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
             const int32_t base = (int32_t)B->host_arena.size();
             B->host_arena.insert(B->host_arena.end(), P->blob.begin(), P->blob.end());
             PairDesc d = P->desc;
-            int32_t *offs[] = {&d.band, &d.codes, &d.gaps, &d.sigma, &d.rows, &d.groups, &d.gams, &d.amscore,
-                               &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.out, &d.rounds,
-                               &d.fptr, &d.fblk, &d.fcand, &d.f2c};
-            for (int32_t *o : offs) *o += base;
+            rebase(d, base);
+            if (!image_in_place(d, base, (int32_t)P->blob.size(), B->host_arena)) {
+                set_err(err, errlen, "internal: image offsets");
+                return LCG_E_STATE;
+            }
             B->descs.push_back(d);
             work[k] = P->cells;
         }
@@ -1316,6 +1388,7 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
     B->bp.alpha = p0.indel_opening;
     B->bp.local = p0.sequ_local;
     B->bp.free_endgaps = p0.free_endgaps;
+    B->bp.nolp = p0.no_lonely_pairs != 0;
     B->bp.trace = 1;
     B->bp.npairs = npairs;
     B->bp.hdr_ints = B->hdr;
@@ -1883,7 +1956,7 @@ int pf_setup(lcg_batch &B) {
         // groups are no larger than the top level region
     }
     B.pf_arena.assign(off, 0.0);
-    B.pf_waves = pick(B.req.pf_waves, "LCG_PF_WAVES", 16, 1, 16);
+    B.pf_waves = pick(B.req.pf_waves, "LCG_PF_WAVES", 8, 1, 8);
     int nfblk_max = 1;
     for (int k = 0; k < np; ++k) nfblk_max = std::max(nfblk_max, B.descs[k].nfblk);
     B.pbp.npairs = np;
@@ -2064,6 +2137,7 @@ extern "C" int lcg_batch_pf_run(lcg_batch *B, const double *dtheta, float *devic
     if (!B) return LCG_E_ARG;
     B->pf_have = false;  // the results of an earlier run are overwritten from here on
     if (B->bp.free_endgaps) return LCG_E_ARG;  // free end gaps: integer path only
+    if (B->bp.nolp) return LCG_E_ARG;          // --noLP: integer path only
     if (B->device_built) return LCG_E_STATE;  // LocARNA-P reads the host arc-match lists
     if (int rc = pf_setup(*B)) return rc;
     for (int k = 0; k < B->npairs; ++k) pf_tables(*B, k, dtheta, B->pf_arena.data());
@@ -2586,7 +2660,7 @@ extern "C" int lcg_batch_get_shape(const lcg_batch *B, lcg_shape *shape) {
     if (!B || !shape) return LCG_E_ARG;
     *shape = B->eff;
     if (!B->d_pfa) {  // LocARNA-P not set up yet: what it would take
-        shape->pf_waves = pick(B->req.pf_waves, "LCG_PF_WAVES", 16, 1, 16);
+        shape->pf_waves = pick(B->req.pf_waves, "LCG_PF_WAVES", 8, 1, 8);
         shape->pf_wg_per_pair = B->req.pf_wg_per_pair;
     }
     return LCG_OK;
@@ -2795,7 +2869,7 @@ extern "C" int lcg_batch_autotune(lcg_batch *B, int32_t path, int32_t reps, lcg_
     }
     std::sort(ks.begin(), ks.end());
     ks.erase(std::unique(ks.begin(), ks.end()), ks.end());
-    for (int w : {16, 8})
+    for (int w : {8, 4})
         for (int k : ks) {
             lcg_shape s = B->req;
             s.pf_waves = w;
@@ -3051,6 +3125,10 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
     const double eps = o.band_eps > 0 ? o.band_eps : 1e-9;
     const bool deferred = (o.flags & LCG_PREPARE_DEVICE_COMPACTION) != 0;
     const bool dev_band = (o.flags & LCG_PREPARE_DEVICE_BAND) != 0;
+    if (deferred && p->no_lonely_pairs) {  // the device compaction builds no --noLP groups
+        set_err(err, errlen, "--noLP needs the arc matches enumerated on the host");
+        return LCG_E_ARG;
+    }
     std::vector<std::unique_ptr<lcg_pair>> P(npairs);
     std::string msg;
     using clk = std::chrono::steady_clock;

@@ -47,6 +47,12 @@
 //   out:      [score, max_i, max_j, status (0 ok, 2 a bounded wait of the
 //             multi-workgroup fill expired), fill-D cells, candidates (u64),
 //             top-level cells, candidates (u64)]
+//   inner:    --noLP only: per arc match the index of the arc match of the
+//             arcs directly inside both of its arcs, -1 if none
+//   c2a:      --noLP only: per candidate entry its arc match
+// With --noLP a group (al, bl) computes the D of the arc matches with left
+// ends (al-1, bl-1) that have an inner arc match (their gams entries hold
+// the inner arc match's right ends); the others keep D = minus infinity.
 //
 // A group region in LDS (or scratch) is [E: br-bl][S: W][M: (ar-al) * W]:
 // E the deletion vector over columns bl..br-1, S the arc-candidate maxima of
@@ -74,7 +80,9 @@ typedef struct {
     int32_t fptr, fblk, fcand;
     int32_t nfblk;     // fill blocks including the sentinel
     int32_t f2c;       // fill entry -> candidate entry (cell order)
-    int32_t pad[3];
+    int32_t inner;     // --noLP only: per arc match the arc match directly inside it, -1 if none
+    int32_t c2a;       // --noLP only: candidate entry -> arc match
+    int32_t pad[1];
 } PairDesc;  // 32 ints
 
 typedef struct {
@@ -94,5 +102,5 @@ typedef struct {
     int32_t free_endgaps;    // top level: LCG_FREE_* bits (left_1 1, right_1 2, left_2 4, right_2 8)
     int32_t spin_limit;      // polls of a bounded cross-workgroup wait (0: default,
                              // -1: every wait gives up at once, for error-path tests)
-    int32_t pad[1];
+    int32_t nolp;            // --noLP: D of stacks (groups keyed by the inner left ends)
 } BatchParams;

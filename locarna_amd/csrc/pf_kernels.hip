@@ -238,9 +238,10 @@ __device__ void pf_fill(const PfPair &P, const Region &R) {
 }
 
 // Adjoint of pf_fill for a region whose Mb holds the seeds.  Accumulates
-// Dbar (per candidate entry), match-edge weights into `edge` and the
-// gap-opening weight.  Mb is consumed.
-__device__ void pf_reverse(const PfPair &P, const Region &R, bool want_dbar, double *edge, double *openings) {
+// Dbar (per candidate entry), match-edge weights into `edge` (and into
+// `edge2` when given) and the gap-opening weight.  Mb is consumed.
+__device__ void pf_reverse(const PfPair &P, const Region &R, bool want_dbar, double *edge, double *openings,
+                           double *edge2 = nullptr) {
     const int lane = lane_id();
     const int al = R.al, bl = R.bl, ar = R.ar, br = R.br, W = R.W;
     const double eo = P.eo, ps = P.ps;
@@ -310,6 +311,7 @@ __device__ void pf_reverse(const PfPair &P, const Region &R, bool want_dbar, dou
                     const double dg = *R.at(R.M, P, i - 1, j - 1);
                     atomicAdd(R.at(R.Mb, P, i - 1, j - 1), sb * es);
                     atomicAdd(&edge[P.coff[i] + (j - mni)], sb * dg * es);
+                    if (edge2) atomicAdd(&edge2[P.coff[i] + (j - mni)], sb * dg * es);
                 }
                 // E_j = Eprev gA (1-o) + M(i-1,j) gA o
                 const double Eb = R.Eb[j - bl] + sb;
@@ -372,11 +374,13 @@ __device__ __forceinline__ double shift_up_d(double v, double first) { return dp
 // lane l receives lane l+1's value, lane 63 `last` (DPP wave_shl:1)
 __device__ __forceinline__ double shift_down_d(double v, double last) { return dpp_d<0x130, 0xf>(last, v); }
 
-// candidate range [t0, t1) of rows al+1+r, r = lane + 64 h
-__device__ __forceinline__ void pf_row_table(const PfPair &P, int al, int bl, int br, int nrows, int rt0[2],
-                                             int rt1[2]) {
+// candidate range [t0, t1) of rows al+1+r, r = lane + 64 h (RT tables: at
+// most 64 RT rows)
+template <int RT>
+__device__ __forceinline__ void pf_row_table(const PfPair &P, int al, int bl, int br, int nrows, int rt0[RT],
+                                             int rt1[RT]) {
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
+    for (int h = 0; h < RT; ++h) {
         const int r = lane_id() + 64 * h;
         rt0[h] = rt1[h] = 0;
         if (r < nrows) {
@@ -391,9 +395,15 @@ __device__ __forceinline__ void pf_row_table(const PfPair &P, int al, int bl, in
         }
     }
 }
-__device__ __forceinline__ void pf_row_cands(const int rt0[2], const int rt1[2], int r, int &t0, int &t1) {
-    t0 = (r < 64) ? __builtin_amdgcn_readlane(rt0[0], r & 63) : __builtin_amdgcn_readlane(rt0[1], r & 63);
-    t1 = (r < 64) ? __builtin_amdgcn_readlane(rt1[0], r & 63) : __builtin_amdgcn_readlane(rt1[1], r & 63);
+template <int RT>
+__device__ __forceinline__ void pf_row_cands(const int rt0[RT], const int rt1[RT], int r, int &t0, int &t1) {
+    t0 = t1 = 0;
+#pragma unroll
+    for (int h = 0; h < RT; ++h)
+        if ((r >> 6) == h) {
+            t0 = __builtin_amdgcn_readlane(rt0[h], r & 63);
+            t1 = __builtin_amdgcn_readlane(rt1[h], r & 63);
+        }
 }
 // Unpredicated loads: lanes past the range read its last entry (or entry 0
 // of the pair; a group implies arc matches) and are ignored by the caller's
@@ -415,7 +425,7 @@ __device__ __forceinline__ void lds_fence() { __builtin_amdgcn_fence(__ATOMIC_RE
 // Inside fill of a group (aligner_p.icc:150-312), values as pf_fill.  When
 // Es/Ss are given (global scratch, layout of M), E and s = M - F of every
 // computed cell are stored for the adjoint.
-template <int NC>
+template <int NC, int RT = 2>
 __device__ __forceinline__ void pf_fwd_fixed(const PfPair &P, int al, int bl, int ar, int br, int W, double *base, double *Es,
                              double *Ss) {
     const int lane = lane_id();
@@ -423,8 +433,8 @@ __device__ __forceinline__ void pf_fwd_fixed(const PfPair &P, int al, int bl, in
     double *Srow = base;
     double *Mst = base + (br - bl);
     const int nrows = ar - al - 1;
-    int rt0[2], rt1[2];
-    pf_row_table(P, al, bl, br, nrows, rt0, rt1);
+    int rt0[RT], rt1[RT];
+    pf_row_table<RT>(P, al, bl, br, nrows, rt0, rt1);
     for (int c = lane; c < br - bl; c += 64) Srow[c] = 0.0;
     double Mp[NC], Ep[NC], gBc[NC];
     int cBc[NC];
@@ -453,7 +463,7 @@ __device__ __forceinline__ void pf_fwd_fixed(const PfPair &P, int al, int bl, in
     PfEnt cur[4];
     {
         int u0 = 0, u1 = 0;
-        if (nrows > 0) pf_row_cands(rt0, rt1, 0, u0, u1);
+        if (nrows > 0) pf_row_cands<RT>(rt0, rt1, 0, u0, u1);
 #pragma unroll
         for (int k = 0; k < 4; ++k) cur[k] = pf_fetch(P, u0 + 64 * k + lane, u1);
     }
@@ -472,7 +482,7 @@ __device__ __forceinline__ void pf_fwd_fixed(const PfPair &P, int al, int bl, in
         if (colopen && lane == 0) row[0] = colv;
         const int jlo = max(bl + 1, mni);
         int t0, t1;
-        pf_row_cands(rt0, rt1, r, t0, t1);
+        pf_row_cands<RT>(rt0, rt1, r, t0, t1);
         // arc terms ps * M(p-1, q-1) * D, candidate-parallel, LDS atomic adds
         auto arc = [&](const PfEnt &e, bool in) {
             const int p = e.pq & 0xffff, q = e.pq >> 16, j = e.jm & 0xffff;
@@ -487,7 +497,7 @@ __device__ __forceinline__ void pf_fwd_fixed(const PfPair &P, int al, int bl, in
         for (int t = t0 + 256 + lane; t < t1; t += 64) arc(pf_fetch(P, t, t1), true);
         if (r + 1 < nrows) {  // the next row's chunks
             int u0, u1;
-            pf_row_cands(rt0, rt1, r + 1, u0, u1);
+            pf_row_cands<RT>(rt0, rt1, r + 1, u0, u1);
 #pragma unroll
             for (int k = 0; k < 4; ++k)
                 if (u0 + 64 * k < u1) cur[k] = pf_fetch(P, u0 + 64 * k + lane, u1);
@@ -536,19 +546,26 @@ __device__ __forceinline__ void pf_fwd_fixed(const PfPair &P, int al, int bl, in
 
 // Adjoint of pf_fwd_fixed for a group whose Mbar rows (LDS, after M) hold the
 // seeds; Es/Ss from the forward recompute.  FULL: also Dbar (dZ/dD per
-// candidate entry, global atomics) and the gap-opening weight (returned);
-// edge receives the match-edge weights.  Same terms as pf_reverse.
-template <int NC, bool FULL>
+// candidate entry) and the gap-opening weight (returned); edge receives the
+// match-edge weights.  Same terms as pf_reverse.
+//
+// The sweep (rows ar-1 .. al+1) keeps global memory to loads: once a row is
+// done no later row reads its Mbar, so each computed cell's Mbar is replaced
+// by its sbar there, and the global accumulations (Dbar += sbar M(p-1,q-1)
+// ps per arc candidate, edge += sbar M(i-1,j-1) es per cell) run after the
+// sweep from those values.  Global stores and atomics inside the sweep would
+// hold up every later load of the wave (one in-order counter covers both),
+// i.e. the row chain.  The sweep's Ss/Es loads (FULL) are issued a row ahead.
+template <int NC, bool FULL, int RT = 2>
 __device__ __forceinline__ double pf_rev_fixed(const PfPair &P, int al, int bl, int ar, int br, int W, double *base, const double *Es,
-                               const double *Ss, double *edge) {
+                               const double *Ss, double *edge, double *edge2 = nullptr) {
     const int lane = lane_id();
     const double eo = P.eo, ps = P.ps;
-    double *Sb = base;
     const double *Mst = base + (br - bl);
     double *Mb = base + (br - bl) + (size_t)(ar - al) * W;
     const int nrows = ar - al - 1;
-    int rt0[2], rt1[2];
-    pf_row_table(P, al, bl, br, nrows, rt0, rt1);
+    int rt0[RT], rt1[RT];
+    pf_row_table<RT>(P, al, bl, br, nrows, rt0, rt1);
     double carry[NC], Eb[NC], gBc[NC];
     int cBc[NC];
 #pragma unroll
@@ -559,43 +576,54 @@ __device__ __forceinline__ double pf_rev_fixed(const PfPair &P, int al, int bl, 
         gBc[c] = (j <= br - 1) ? P.egB[j] : 1.0;
         cBc[c] = (j <= br - 1) ? P.codeB[j] : 0;
     }
+    // first stored column of row i of the region (row al from bl)
+    auto lo_of = [&](int i) { return (i == al) ? bl : max(bl, P.mn[i]); };
+    // Ss of row i at column j - 1 and Es of row i - 1 at column j (clamped
+    // into the stored rows; the caller's flags decide whether they are used)
+    double ssc[NC], esc[NC];
+    auto load_se = [&](int i, double *ss, double *es) {
+        const int r = i - al - 1;
+        const int lo = lo_of(i), lo_u = lo_of(i - 1);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const int j = bl + 1 + 64 * c + lane;
+            ss[c] = Ss[(size_t)(r + 1) * W + min(max(j - 1 - lo, 0), W - 1)];
+            es[c] = Es[(size_t)r * W + min(max(j - lo_u, 0), W - 1)];
+        }
+    };
+    if (FULL && nrows > 0) load_se(ar - 1, ssc, esc);
     double colcarry = 0.0;  // contribution of row i+1 to Mbar(i, bl)
     double open_acc = 0.0;
-    // first 64 fill entries of the row (and their candidate entries),
-    // fetched one row ahead; rows are visited downwards
+    // first 64 fill entries of the row, fetched one row ahead; rows are
+    // visited downwards
     PfEnt ce;
-    int cx;
     {
         int u0 = 0, u1 = 0;
-        if (nrows > 0) pf_row_cands(rt0, rt1, nrows - 1, u0, u1);
+        if (nrows > 0) pf_row_cands<RT>(rt0, rt1, nrows - 1, u0, u1);
         ce = pf_fetch(P, u0 + lane, u1);
-        cx = FULL ? P.f2c[u0 + lane < u1 ? u0 + lane : max(u1 - 1, 0)] : 0;
     }
     lds_fence();
     for (int i = ar - 1; i > al; --i) {
         const int r = i - al - 1;
         PfEnt ne = ce;
-        int nx = cx;
+        double ssn[NC], esn[NC];
         if (r > 0) {
             int u0, u1;
-            pf_row_cands(rt0, rt1, r - 1, u0, u1);
-            if (u1 > u0) {
-                ne = pf_fetch(P, u0 + lane, u1);
-                if (FULL) nx = P.f2c[u0 + lane < u1 ? u0 + lane : u1 - 1];
-            }
+            pf_row_cands<RT>(rt0, rt1, r - 1, u0, u1);
+            if (u1 > u0) ne = pf_fetch(P, u0 + lane, u1);
+            if (FULL) load_se(i - 1, ssn, esn);
         }
         const int mni = P.mn[i];
         const int lo = max(bl, mni), hi = min(br - 1, P.mx[i]), jlo = max(bl + 1, mni);
         const bool colopen_i = mni <= bl;
-        const int lo_u = (i - 1 == al) ? bl : max(bl, P.mn[i - 1]), hi_u = min(br - 1, P.mx[i - 1]);
+        const int lo_u = lo_of(i - 1), hi_u = min(br - 1, P.mx[i - 1]);
         const int jlo_u = max(bl + 1, P.mn[i - 1]);
         const bool upc_row = i - 1 > al;  // row i-1 computed (has E)
         const double *rowM = Mst + (size_t)(r + 1) * W;
         const double *rowU = Mst + (size_t)r * W;
-        const double *rowB = Mb + (size_t)(r + 1) * W;
+        double *rowB = Mb + (size_t)(r + 1) * W;
         const double gA = P.egA[i];
         const double *esrow = P.es + P.codeA[i] * LCG_NCODES;
-        const int ecell = P.coff[i] - mni;
         double Fb_in = 0.0, gB_in = 0.0, d_in = 0.0;
         double newcarry[NC];
         double colcarry_next = 0.0, colb = 0.0;
@@ -611,10 +639,10 @@ __device__ __forceinline__ double pf_rev_fixed(const PfPair &P, int al, int bl, 
             const double Fb = cm * Fb_in + gg;  // Fbar_j
             const double Fb_next = shift_down_d(Fb, Fb_in);
             const double sb = act ? g + Fb_next * gB_next * eo : 0.0;
-            if (act) Sb[j - bl - 1] = sb;
+            if (act) rowB[j - lo] = sb;  // this row's Mbar is not read again
             const bool bcol = act && j == jlo && jlo == bl + 1 && colopen_i;  // s_{j-1} = M(i, bl)
             if (FULL && act) {
-                const double sl = (j == jlo) ? (bcol ? rowM[0] : 0.0) : Ss[(size_t)(r + 1) * W + (j - 1 - lo)];
+                const double sl = (j == jlo) ? (bcol ? rowM[0] : 0.0) : ssc[cc];
                 open_acc += Fb * sl * gB * eo;
             }
             if (cc == 0) colb = readlane_d(bcol ? Fb * gB * eo : 0.0, 0);
@@ -624,15 +652,13 @@ __device__ __forceinline__ double pf_rev_fixed(const PfPair &P, int al, int bl, 
             if (FULL && act) {
                 const double up = upst ? rowU[j - lo_u] : 0.0;
                 const bool upc = upc_row && j >= jlo_u && j <= hi_u;
-                const double Ep = upc ? Es[(size_t)r * W + (j - lo_u)] : 0.0;
+                const double Ep = upc ? esc[cc] : 0.0;
                 open_acc += eb * (up - Ep) * gA * eo;
             }
             Eb[cc] = act ? eb * gA * (1.0 - eo) : 0.0;
             // match: s_j uses M(i-1, j-1) es
             const bool dgst = act && j - 1 >= lo_u && j - 1 <= hi_u;
-            const double es = esrow[cBc[cc]];
-            if (dgst) atomicAdd(&edge[ecell + j], sb * rowU[j - 1 - lo_u] * es);
-            const double d = dgst ? sb * es : 0.0;  // to Mbar(i-1, j-1)
+            const double d = dgst ? sb * esrow[cBc[cc]] : 0.0;  // to Mbar(i-1, j-1)
             const double d_right = shift_down_d(d, d_in);
             newcarry[cc] = upst ? (act ? eb * gA * eo : 0.0) + d_right : 0.0;
             if (cc == 0) colcarry_next = readlane_d(d, 0);
@@ -643,23 +669,25 @@ __device__ __forceinline__ double pf_rev_fixed(const PfPair &P, int al, int bl, 
         // the init column cell (i, bl): one opening per value
         if (FULL && colopen_i && lane == 0) open_acc += (rowB[0] + colcarry + colb) * rowM[0];
         lds_fence();
-        // arc terms: Mbar(p-1, q-1) += sbar D ps; Dbar += sbar M(p-1, q-1) ps
+        // arc terms: Mbar(p-1, q-1) += sbar D ps
         int t0, t1;
-        pf_row_cands(rt0, rt1, r, t0, t1);
-        auto arc = [&](const PfEnt &e, int x, bool in) {
+        pf_row_cands<RT>(rt0, rt1, r, t0, t1);
+        auto arc = [&](const PfEnt &e, bool in) {
             const int p = e.pq & 0xffff, q = e.pq >> 16, j = e.jm & 0xffff;
             const int rl = max(bl, e.jm >> 16);
-            if (in && q > bl && j >= jlo && j <= hi && q - 1 >= rl && e.D != 0.0) {
-                const double sb = Sb[j - bl - 1];
-                const int idx = (p - 1 - al) * W + (q - 1 - rl);
-                atomicAdd(&Mb[idx], sb * e.D * ps);
-                if (FULL) atomicAdd(&P.Dbar[x], sb * Mst[idx] * ps);
-            }
+            if (in && q > bl && j >= jlo && j <= hi && q - 1 >= rl && e.D != 0.0)
+                atomicAdd(&Mb[(p - 1 - al) * W + (q - 1 - rl)], rowB[j - lo] * e.D * ps);
         };
-        if (t1 > t0) arc(ce, cx, t0 + lane < t1);
-        for (int t = t0 + 64 + lane; t < t1; t += 64) arc(pf_fetch(P, t, t1), FULL ? P.f2c[t] : 0, true);
+        if (t1 > t0) arc(ce, t0 + lane < t1);
+        for (int t = t0 + 64 + lane; t < t1; t += 64) arc(pf_fetch(P, t, t1), true);
         ce = ne;
-        cx = nx;
+        if (FULL && r > 0) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                ssc[c] = ssn[c];
+                esc[c] = esn[c];
+            }
+        }
         lds_fence();
 #pragma unroll
         for (int c = 0; c < NC; ++c) carry[c] = newcarry[c];
@@ -672,6 +700,35 @@ __device__ __forceinline__ double pf_rev_fixed(const PfPair &P, int al, int bl, 
         for (int c = 0; c < NC; ++c) {
             const int j = bl + 1 + 64 * c + lane;
             if (j <= hi0) open_acc += (Mb[j - bl] + carry[c]) * Mst[j - bl];
+        }
+    }
+    // global accumulations from sbar (Mb of the computed cells) and M
+    for (int i = al + 1; i < ar; ++i) {
+        const int r = i - al - 1;
+        const int mni = P.mn[i];
+        const int lo = max(bl, mni), hi = min(br - 1, P.mx[i]), jlo = max(bl + 1, mni);
+        if (jlo > hi) continue;
+        const int lo_u = lo_of(i - 1), hi_u = min(br - 1, P.mx[i - 1]);
+        const double *rowB = Mb + (size_t)(r + 1) * W;
+        const double *rowU = Mst + (size_t)r * W;
+        const double *esrow = P.es + P.codeA[i] * LCG_NCODES;
+        const int ecell = P.coff[i] - mni;
+        for (int j = jlo + lane; j <= hi; j += 64)
+            if (j - 1 >= lo_u && j - 1 <= hi_u) {
+                const double w = rowB[j - lo] * rowU[j - 1 - lo_u] * esrow[P.codeB[j]];
+                atomicAdd(&edge[ecell + j], w);
+                if (edge2) atomicAdd(&edge2[ecell + j], w);
+            }
+        if (FULL) {
+            int t0, t1;
+            pf_row_cands<RT>(rt0, rt1, r, t0, t1);
+            for (int t = t0 + lane; t < t1; t += 64) {
+                const PfEnt e = pf_fetch(P, t, t1);
+                const int p = e.pq & 0xffff, q = e.pq >> 16, j = e.jm & 0xffff;
+                const int rl = max(bl, e.jm >> 16);
+                if (q > bl && j >= jlo && j <= hi && q - 1 >= rl && e.D != 0.0)
+                    atomicAdd(&P.Dbar[P.f2c[t]], rowB[j - lo] * Mst[(p - 1 - al) * W + (q - 1 - rl)] * ps);
+            }
         }
     }
     return wave_sum(open_acc);
@@ -696,7 +753,7 @@ __device__ __forceinline__ void zero_region_bar(const Region &R, const PfPair &P
 // matches has P(am) > sqrt(min_am_prob), the restricted one.
 #define LCG_PF_SPIN_LIMIT (1 << 22)
 
-extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *__restrict__ descs,
+extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *__restrict__ descs,
                                                                 const PfDesc *__restrict__ pdescs,
                                                                 const int *__restrict__ order, int *queue,
                                                                 int *arena, double *pfa, double *scratch,
@@ -893,19 +950,39 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_pf_pairs(const PairDesc *
             if (K > 1) pair_barrier();
         }
         // ---- top level and its adjoint (wave 0 of member 0)
+        // (the restricted base-match weights count every top-level match edge,
+        // aligner_p.icc:1201-1399: the seeds are the same, so one adjoint
+        // sweep gives both)
         if (wave == 0 && member == 0) {
-            const Region R = region(0, 0, n + 1, m + 1, d.wtop);
-            pf_fill(P, R);
-            const double Z = *R.at(R.M, P, n, m);
-            if (lane == 0) pfa[f.out + 0] = Z;
-            zero_region_bar(R, P);
-            if (lane == 0) *R.at(R.Mb, P, n, m) = 1.0;
-            scratch_sync();
-            pf_reverse(P, R, true, P.edge_full, &s_open);
-            zero_region_bar(R, P);
-            if (lane == 0) *R.at(R.Mb, P, n, m) = 1.0;
-            scratch_sync();
-            pf_reverse(P, R, false, P.edge_ref, nullptr);
+            const int W = d.wtop;
+            const size_t cells = (size_t)(n + 1) * W;
+            if (n <= 192 && m <= 192 && (size_t)(m + 1) + 2 * cells <= (size_t)bp.pool) {
+                // the whole LDS pool is free while the other waves wait: the
+                // fixed-column kernels (E, s of every cell in the wave's scratch)
+                double *base = pool;
+                double *Es = wscr, *Ss = wscr + cells;
+                pf_fwd_fixed<3, 3>(P, 0, 0, n + 1, m + 1, W, base, Es, Ss);
+                const int xz = fixed_at(0, 0, m + 1, W, n, m);
+                const double Z = xz >= 0 ? base[(m + 1) + xz] : 0.0;
+                if (lane == 0) pfa[f.out + 0] = Z;
+                double *Mb = base + (m + 1) + cells;
+                for (size_t c = lane; c < cells; c += 64) Mb[c] = 0.0;
+                lds_fence();
+                if (lane == 0 && xz >= 0) Mb[xz] = 1.0;
+                scratch_sync();  // Es/Ss (global) and the seed (LDS) before the sweep
+                const double o = pf_rev_fixed<3, true, 3>(P, 0, 0, n + 1, m + 1, W, base, Es, Ss, P.edge_full,
+                                                          P.edge_ref);
+                if (lane == 0) atomicAdd(&s_open, o);
+            } else {
+                const Region R = region(0, 0, n + 1, m + 1, W);
+                pf_fill(P, R);
+                const double Z = *R.at(R.M, P, n, m);
+                if (lane == 0) pfa[f.out + 0] = Z;
+                zero_region_bar(R, P);
+                if (lane == 0) *R.at(R.Mb, P, n, m) = 1.0;
+                scratch_sync();
+                pf_reverse(P, R, true, P.edge_full, &s_open, P.edge_ref);
+            }
         }
         if (K > 1) {
             pair_barrier();

@@ -484,6 +484,34 @@ struct ArcMatches {
             *max_br = std::max(*max_br, ams[x].b.right);
         }
     }
+    // arc_matches.cc:50-74: the arc match of the arcs directly inside both
+    // arcs (left ends + 1, right ends - 1), the first in its left-end list;
+    // -1 when there is none
+    std::vector<int> inner;
+    void init_inner() {
+        inner.assign(ams.size(), -1);
+        for (const ArcMatch &am : ams) {
+            if (am.a.left + 1 > lenA || am.b.left + 1 > lenB) continue;
+            for (int y : left_lists(am.a.left + 1, am.b.left + 1))
+                if (ams[y].a.right == am.a.right - 1 && ams[y].b.right == am.b.right - 1) {
+                    inner[am.idx] = y;
+                    break;
+                }
+        }
+    }
+    // arc_matches.cc:313-355 with no_lonely_pairs: the group (al, bl) holds
+    // the inner region of the arc matches with left ends (al-1, bl-1) that
+    // have an inner arc match
+    void max_right_ends_noLP(int al, int bl, int *max_ar, int *max_br) const {
+        int ar = *max_ar + 1, br = *max_br + 1;
+        for (int x : left_lists(al - 1, bl - 1)) {
+            if (inner[x] < 0) continue;
+            ar = std::max(ar, ams[x].a.right);
+            br = std::max(br, ams[x].b.right);
+        }
+        *max_ar = ar - 1;
+        *max_br = br - 1;
+    }
 };
 
 // ---------------------------------------------------------------- Scoring
@@ -601,13 +629,14 @@ std::unique_ptr<Inputs> prepare(const char *ppA, const char *ppB, const oracle_p
     in->bpsA.reset(new BasePairs(in->A, p.min_prob));
     in->bpsB.reset(new BasePairs(in->B, p.min_prob));
     in->am.reset(new ArcMatches(*in->bpsA, *in->bpsB, *in->band, p.max_diff_am, p.max_diff_at_am));
+    in->am->init_inner();
     in->sc.reset(new Scoring(p, in->rib.get(), in->A, in->B, *in->bpsA, *in->bpsB));
     return in;
 }
 
 // ---------------------------------------------------------------- Aligner
 // aligner.cc (state E_NO_NO only: no structure locality, no anchors, no
-// lonely-pair or stacking variants; free end gaps at the top level)
+// stacking; --noLP D-fill and traceback; free end gaps at the top level)
 struct Aligner {
     Inputs &in;
     const Band &tc;
@@ -713,8 +742,23 @@ struct Aligner {
         }
     }
 
+    // aligner.cc:612-657 (no stacking): D of the arc matches with left ends
+    // (al-1, bl-1) that have an inner arc match, from the group (al, bl)
+    // over the inner region: the inner arc match ends the stack (its region
+    // aligned freely) or continues it (its own D)
+    void fill_D_entries_noLP(int al, int bl) {
+        for (int x : ams.left_lists(al - 1, bl - 1)) {
+            const int y = ams.inner[x];
+            if (y < 0) continue;
+            const ArcMatch &am = ams.ams[x], &in_am = ams.ams[y];
+            const sc_t m = M(in_am.a.right - 1, in_am.b.right - 1);
+            D(am) = std::max(m + am_score[y], D(in_am)) + am_score[x];
+        }
+    }
+
     // aligner.cc:659-732
     void align_D() {
+        const bool noLP = in.p.no_lonely_pairs != 0;
         for (size_t al = lenA + 1; al > 1;) {
             al--;
             size_t max_bl = std::min((size_t)lenB, tc.max_col[al]);
@@ -723,11 +767,17 @@ struct Aligner {
                 bl--;
                 if (!tc.is_valid_match(al, bl)) continue;
                 int max_ar = (int)al, max_br = (int)bl;
-                ams.max_right_ends((int)al, (int)bl, &max_ar, &max_br);
+                if (noLP)
+                    ams.max_right_ends_noLP((int)al, (int)bl, &max_ar, &max_br);
+                else
+                    ams.max_right_ends((int)al, (int)bl, &max_ar, &max_br);
                 if ((int)al == max_ar || (int)bl == max_br) continue;
                 if (counting) cnt_G++;
                 align_in_arcmatch(al, max_ar, bl, max_br);
-                fill_D_entries((int)al, (int)bl);
+                if (noLP)
+                    fill_D_entries_noLP((int)al, (int)bl);
+                else
+                    fill_D_entries((int)al, (int)bl);
             }
         }
     }
@@ -846,6 +896,25 @@ struct Aligner {
         trace_in_arcmatch(am.a.left, am.a.right - 1, am.b.left, am.b.right - 1, false);
     }
 
+    // aligner.cc:1030-1078 (no stacking, not structure-local): the inner
+    // arc match of the stack, which continues the stack when D(am) ==
+    // D(inner) + s(am), else ends it with its region traced
+    void trace_arcmatch_noLP(const ArcMatch &am) {
+        const ArcMatch &y = ams.ams[ams.inner[am.idx]];
+        structA[y.a.left - 1] = '(';
+        structA[y.a.right - 1] = ')';
+        structB[y.b.left - 1] = '(';
+        structB[y.b.right - 1] = ')';
+        append(y.a.left, y.b.left);
+        if (D(am) == D(y) + am_score[am.idx]) {
+            trace_arcmatch_noLP(y);
+        } else {
+            align_in_arcmatch(y.a.left, y.a.right, y.b.left, y.b.right);
+            trace_in_arcmatch(y.a.left, y.a.right - 1, y.b.left, y.b.right - 1, false);
+        }
+        append(y.a.right, y.b.right);
+    }
+
     void trace_noex(size_t oal, size_t i, size_t obl, size_t j, bool tl) {
         const sc_t lm = tl ? lam : 0;
         if (tc.is_valid(i - 1, j - 1) && M(i, j) == M(i - 1, j - 1) + (sc.sigma_tab(i, j) - 2 * lm)) {
@@ -899,7 +968,10 @@ struct Aligner {
                 structB[bl - 1] = '(';
                 structB[j - 1] = ')';
                 append((int)al, (int)bl);
-                trace_arcmatch(am);
+                if (in.p.no_lonely_pairs)
+                    trace_arcmatch_noLP(am);
+                else
+                    trace_arcmatch(am);
                 append((int)i, (int)j);
                 return;
             }

@@ -81,6 +81,9 @@ typedef struct {
     double pf_scale;       /* --pf-scale                             */
     double min_am_prob;    /* --min-am-prob                          */
     double min_bm_prob;    /* --min-bm-prob                          */
+    int no_lonely_pairs;   /* --noLP (integer path): arc matches only in
+                              stacks of two or more                   */
+    int reserved;
 } oracle_params;
 
 typedef struct oracle_result oracle_result;

@@ -24,6 +24,7 @@ class Params(ctypes.Structure):
         ("min_prob", ctypes.c_double), ("min_trace_prob", ctypes.c_double),
         ("exp_prob", ctypes.c_double), ("pf_scale", ctypes.c_double),
         ("min_am_prob", ctypes.c_double), ("min_bm_prob", ctypes.c_double),
+        ("no_lonely_pairs", ctypes.c_int), ("reserved", ctypes.c_int),
     ]
 
 
@@ -35,7 +36,7 @@ def locarna_params(**kw):
              mismatch=0, temperature_alipf=300, sequ_local=0, max_bp_span=100,
              band_extended=1, use_ribosum=1, free_endgaps=0, max_diff=-1, max_diff_am=-1, max_diff_at_am=-1,
              min_prob=0.0005, min_trace_prob=1e-4,
-             exp_prob=-1.0, pf_scale=1.0, min_am_prob=0.001, min_bm_prob=0.001)
+             exp_prob=-1.0, pf_scale=1.0, min_am_prob=0.001, min_bm_prob=0.001, no_lonely_pairs=0)
     d.update(kw)
     return Params(**d)
 

@@ -22,8 +22,12 @@ import math
 
 class Problem:
     def __init__(self, lenA, lenB, sigma, gapA, gapB, alpha, arcmatches, am_score,
-                 min_col, max_col, local=False, free=0):
+                 min_col, max_col, local=False, free=0, no_lonely_pairs=False):
         self.n, self.m = lenA, lenB
+        # --noLP: only structure alignments whose every matched arc pair has
+        # its directly inner or directly outer arc pair matched as well
+        # (stacks of two or more, aligner.cc:612-657)
+        self.no_lonely_pairs = no_lonely_pairs
         # free end gaps (locarna --free-endgaps as bits: 1 left end of the
         # first sequence, 2 its right end, 4 left end of the second, 8 its
         # right end): free leading runs of the top level cost nothing and
@@ -152,6 +156,27 @@ def edges(P, steps, r0, c0, top):
     return out, arcs
 
 
+def matched_arcs(P, steps):
+    """Arc-match indices of a structure alignment (all nesting levels)."""
+    out = []
+    for s in steps:
+        if s[0] == 'A':
+            out.append(s[2])
+            out += matched_arcs(P, s[3])
+    return out
+
+
+def stacked(P, steps):
+    """True unless --noLP forbids the alignment: every matched arc pair
+    (al, ar, bl, br) needs (al+1, ar-1, bl+1, br-1) or (al-1, ar+1, bl-1,
+    br+1) matched too."""
+    if not P.no_lonely_pairs:
+        return True
+    S = {P.arcmatches[k] for k in matched_arcs(P, steps)}
+    return all((al + 1, ar - 1, bl + 1, br - 1) in S or (al - 1, ar + 1, bl - 1, br + 1) in S
+               for (al, ar, bl, br) in S)
+
+
 def best_global(P):
     """(max score, traced edges, traced arc-match indices, #alignments)."""
     memo = {}
@@ -167,14 +192,14 @@ def best_global(P):
             ends += [(P.n, j) for j in range(max(1, P.min_col[P.n]), min(P.m, P.max_col[P.n]) + 1)]
     best, cell, count = None, None, 0
     for (i, j) in ends:
-        alns = group_alignments(P, 0, 0, i, j, True, memo)
+        alns = [a for a in group_alignments(P, 0, 0, i, j, True, memo) if stacked(P, a[1])]
         count += len(alns)
         if not alns:
             continue
         sc = max(x for x, _, _ in alns)
         if best is None or sc > best:
             best, cell = sc, (i, j)
-    alns = group_alignments(P, 0, 0, cell[0], cell[1], True, memo)
+    alns = [a for a in group_alignments(P, 0, 0, cell[0], cell[1], True, memo) if stacked(P, a[1])]
     chosen = min((st for sc, st, _ in alns if sc == best), key=trace_key)
     e, a = edges(P, chosen, 0, 0, True)
     return best, e, a, count
@@ -198,6 +223,8 @@ def best_local(P):
                     if (r0, c0) != (0, 0) and (r0 == 0 or c0 == 0):
                         continue  # starts on the zero init row / column are covered by (0,0)
                     for sc, st, _ in group_alignments(P, r0, c0, i, j, True, memo):
+                        if not stacked(P, st):
+                            continue
                         if sc > best:
                             best, ends = sc, [(i, j)]
                         elif sc == best and (i, j) not in ends:
@@ -280,3 +307,11 @@ def pf_details(P, T, min_am_prob):
             if kind == 'M' and (owner is None or am[owner] > thr):
                 ref[x] = ref.get(x, 0.0) + w
     return (Z, am, {k: v / Z for k, v in full.items()}, {k: v / Z for k, v in ref.items()}, open_w / Z)
+
+
+def optimal_global_edges(P):
+    """Edge lists of every optimal global structure alignment (top level
+    ending in (n, m))."""
+    alns = [a for a in group_alignments(P, 0, 0, P.n, P.m, True, {}) if stacked(P, a[1])]
+    best = max(sc for sc, _, _ in alns)
+    return best, [edges(P, st, 0, 0, True)[0] for sc, st, _ in alns if sc == best]

@@ -20,3 +20,29 @@ def tiny_pair(seed, nA=None, nB=None, arcs=(1, 3)):
     nB = nB or rng.randint(4, 7)
     return (tiny_pp(rng, nA, rng.randint(*arcs), "a"),
             tiny_pp(rng, nB, rng.randint(*arcs), "b"))
+
+
+def tiny_stacked_pp(rng, n, name="t"):
+    """A tiny sequence whose arcs include stacks: one or two stacks of two
+    or three arcs (i, j), (i+1, j-1), ... and possibly a lonely arc."""
+    seq = "".join(rng.choice("ACGU") for _ in range(n))
+    arcs = set()
+    for _ in range(rng.randint(1, 2)):
+        depth = rng.choice((2, 3)) if n >= 9 else 2
+        i = rng.randint(1, n - 3 - 2 * (depth - 1))
+        j = rng.randint(i + 3 + 2 * (depth - 1), n)
+        for d in range(depth):
+            arcs.add((i + d, j - d))
+    if rng.random() < 0.7:
+        i = rng.randint(1, n - 3)
+        j = rng.randint(i + 3, n)
+        arcs.add((i, j))
+    pairs = [(i, j, round(rng.uniform(0.05, 0.95), 4)) for (i, j) in sorted(arcs)]
+    return pp_text(name, seq, pairs, bpcut=0.0005)
+
+
+def tiny_stacked_pair(seed, nA=None, nB=None):
+    rng = random.Random(seed)
+    nA = nA or rng.randint(6, 9)
+    nB = nB or rng.randint(6, 9)
+    return tiny_stacked_pp(rng, nA, "a"), tiny_stacked_pp(rng, nB, "b")
