@@ -257,6 +257,20 @@ int lcg_batch_run_penalized(lcg_batch *batch, int64_t position_penalty, float *d
  * than max_iterations iterations. */
 int lcg_batch_run_normalized(lcg_batch *batch, int64_t L, int32_t max_iterations, int32_t *iterations,
                              float *device_ms);
+/* Allocation-free run (no counterpart in the reference; for callers that
+ * cannot allocate device memory: code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc): the fill of D, the top level and,
+ * with trace bit 0, the traceback of every pair of the batch, as
+ * lcg_batch_run mode 0, with the work queues, the cross-workgroup arrival
+ * counters and the top-level per-wave regions in a static 64 MiB device
+ * array of the library instead of the batch's buffers.  Calls no hipMalloc
+ * or hipFree, records no event and does not wait: it enqueues kernels on
+ * the batch's stream (lcg_batch_set_stream) and returns; the result getters
+ * synchronize.  LCG_E_CAPACITY when the batch needs more than the array
+ * holds (one top-level workgroup's regions plus one counter per pair).
+ * Every call on a GPU shares that GPU's array: calls that overlap on one GPU
+ * (different batches, streams or threads) race. */
+int lcg_batch_run_noalloc(lcg_batch *batch, int32_t trace);
 /* Stream of a batch (a hipStream_t of the batch's device, or NULL for the
  * batch's own stream; no counterpart in the reference, which runs on the
  * CPU): every later launch and copy of the batch is ordered on it, after

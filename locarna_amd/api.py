@@ -96,6 +96,7 @@ def _sig(L):
     L.lcg_batch_destroy.argtypes = [c.c_void_p]
     L.lcg_batch_run.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.POINTER(c.c_float)]
     L.lcg_batch_set_spin_limit.argtypes = [c.c_void_p, c.c_int32]
+    L.lcg_batch_run_noalloc.argtypes = [c.c_void_p, c.c_int32]
     L.lcg_batch_set_stream.argtypes = [c.c_void_p, c.c_void_p]
     L.lcg_batch_config.argtypes = [c.c_void_p, c.POINTER(c.c_int32), c.POINTER(c.c_int32),
                                    c.POINTER(c.c_int32)]
@@ -371,6 +372,13 @@ class Batch:
         rc = lib().lcg_batch_set_spin_limit(self.h, polls)
         if rc != 0:
             raise LcgError("lcg_batch_set_spin_limit (%d)" % rc)
+
+    def run_noalloc(self, trace=True):
+        """lcg_batch_run_noalloc: enqueues the run with the library's static
+        scratch; the getters synchronize."""
+        rc = lib().lcg_batch_run_noalloc(self.h, 1 if trace else 0)
+        if rc != 0:
+            raise LcgError("lcg_batch_run_noalloc (%d)" % rc)
 
     def set_stream(self, stream):
         """Runs on a caller's HIP stream (an integer hipStream_t handle of

@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include <climits>
+#include <mutex>
 
 #include "device_layout.h"
 
@@ -1050,6 +1051,41 @@ extern "C" int lcg_launch_gather(const PairDesc *descs, const int *res_off, cons
 
 extern "C" __global__ void lcg_reset_queue(int *queue) {
     if (threadIdx.x == 0) *queue = 0;
+}
+
+// Scratch of the allocation-free run (lcg_batch_run_noalloc in batch.cpp):
+// the two work-queue counters, the arrival counters of pairs that several
+// workgroups share and the per-wave regions of the top-level kernel.  Every
+// GPU the library runs on holds its own copy until the process ends; host
+// code reaches it only through the per-device address that
+// lcg_noalloc_scratch looks up (the array's name is not a device address
+// in host code).
+#define LCG_NOALLOC_SCRATCH_INTS (16 << 20) /* 64 MiB */
+static __device__ __align__(256) int scratch[LCG_NOALLOC_SCRATCH_INTS];
+
+// Device address and size (ints) of `scratch` on `device`, looked up once
+// per device with that device current.
+extern "C" int lcg_noalloc_scratch(int device, int **ptr, long long *ints) {
+    static std::mutex mu;
+    static int *addr[64] = {nullptr};
+    if (device < 0 || device >= 64 || !ptr || !ints) return (int)hipErrorInvalidValue;
+    std::lock_guard<std::mutex> guard(mu);
+    if (!addr[device]) {
+        int cur = -1;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return (int)e;
+        if (cur != device && (e = hipSetDevice(device)) != hipSuccess) return (int)e;
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        const hipError_t e2 = (cur != device) ? hipSetDevice(cur) : hipSuccess;
+        if (e != hipSuccess) return (int)e;
+        if (e2 != hipSuccess) return (int)e2;
+        if (!p) return (int)hipErrorInvalidSymbol;
+        addr[device] = static_cast<int *>(p);
+    }
+    *ptr = addr[device];
+    *ints = LCG_NOALLOC_SCRATCH_INTS;
+    return (int)hipSuccess;
 }
 
 extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, int *scratch,

@@ -1615,6 +1615,42 @@ extern "C" int lcg_batch_run(lcg_batch *B, int32_t mode, int32_t trace, float *d
     return LCG_OK;
 }
 
+extern "C" int lcg_noalloc_scratch(int device, int **ptr, long long *ints);
+
+// The fill of D, top level and traceback of every pair (as lcg_batch_run
+// mode 0), enqueued on the batch's stream with the work-queue counters, the
+// arrival counters and the top-level per-wave regions in the library's
+// static device array instead of the batch's own buffers.  Calls nothing
+// that allocates, records no event and does not wait: the getters
+// synchronize with the stream.  Fewer top-level workgroups than the batch's
+// own launch when the array holds fewer per-wave regions (results are the
+// same for any number).
+extern "C" int lcg_batch_run_noalloc(lcg_batch *B, int32_t trace) {
+    if (!B || (trace & ~1)) return LCG_E_ARG;
+    HIPCHECK(hipSetDevice(B->device));
+    int *S = nullptr;
+    long long cap = 0;
+    if (int rc = lcg_noalloc_scratch(B->device, &S, &cap)) return hip_fail((hipError_t)rc);
+    const long long bar_off = 64;                                 // queues at 0 and 16
+    const long long top_off = (bar_off + B->npairs + 63) & ~63LL;  // 256-byte aligned
+    const long long per_wg = (long long)B->top_waves * B->top_max;
+    if (top_off + per_wg > cap) return LCG_E_CAPACITY;
+    const int top_grid = (int)std::min<long long>(B->top_grid, (cap - top_off) / per_wg);
+    BatchParams bp = B->bp, top = B->bp_top;
+    bp.trace = top.trace = trace & 1;
+    bp.count = top.count = 0;
+    if (int rc = lcg_launch_align(B->d_descs, B->d_order, S, B->d_arena, S + top_off, S + bar_off, bp, B->grid,
+                                  B->waves, B->lds_bytes, top, top_grid, B->top_waves, B->top_lds_bytes, nullptr, 0,
+                                  nullptr, B->stream))
+        return hip_fail((hipError_t)rc);
+    B->fill_ms = B->top_ms = -1;
+    B->have_results = true;
+    B->fetched = false;
+    B->traced = (trace & 1) != 0;
+    B->norm_valid = false;
+    return LCG_OK;
+}
+
 namespace {
 int fetch(const lcg_batch *Bc);
 inline int32_t res(const lcg_batch *B, int k, int32_t off);
