@@ -66,7 +66,14 @@ typedef struct {
                                   (lcg_pair_prepare, lcg_pairs_prepare without
                                   LCG_PREPARE_DEVICE_COMPACTION); LocARNA-P
                                   refuses it                                */
-    int32_t reserved;
+    int32_t stacking;          /* bit 0 --stacking, bit 1 --new-stacking:
+                                  the #STACK joint probabilities of the pp
+                                  input give an arc match whose inner arc
+                                  match exists a stacked alternative of D
+                                  (aligner.cc:574-610; weights
+                                  scoring.cc:200-236); integer path with host
+                                  arc matches and pp input (not arc-match
+                                  score files); LocARNA-P refuses it        */
 } lcg_params;
 
 /* free end gaps (FreeEndgaps, free_endgaps.hh: the characters of the
@@ -257,6 +264,25 @@ int lcg_batch_run_penalized(lcg_batch *batch, int64_t position_penalty, float *d
  * than max_iterations iterations. */
 int lcg_batch_run_normalized(lcg_batch *batch, int64_t L, int32_t max_iterations, int32_t *iterations,
                              float *device_ms);
+/* k-best suboptimal alignments by interval splitting (Aligner::suboptimal,
+ * aligner.cc:1382-1514; locarna --kbest k / --better t, locarna.cc:203-206,
+ * 740-746; not normalized): the best alignment of pair k, then the best of
+ * each half of its restriction, split at the middle of the alignment's
+ * extent in the longer sequence (both halves keep that position), best
+ * first, until kbest alignments (kbest < 0: no limit) or until the best
+ * queued score is at most threshold; empty alignments are skipped.  Runs
+ * the batch's fill of D once (lcg_batch_run) and then one top-level launch
+ * of pair k per restriction (AlignerRestriction, aligner_restriction.hh:
+ * 26-120); afterwards the batch's results of pair k are those of its last
+ * restricted launch.  Writes the first cap scores; returns the number of
+ * alignments. */
+int lcg_batch_suboptimal(lcg_batch *batch, int32_t k, int32_t kbest, int64_t threshold, int64_t *scores,
+                         int32_t cap);
+/* Alignment t of the last lcg_batch_suboptimal: edges a[e], b[e] (-1 for a
+ * gap) and structures as lcg_batch_alignment, and its restriction (startA,
+ * endA, startB, endB; may be NULL).  Returns the number of edges. */
+int lcg_batch_suboptimal_alignment(const lcg_batch *batch, int32_t t, int32_t *a, int32_t *b, int32_t cap,
+                                   char *structA, char *structB, int32_t scap, int32_t *restriction);
 /* Allocation-free run (no counterpart in the reference; for callers that
  * cannot allocate device memory: code under HIP graph capture, serving
  * loops, harnesses that forbid hipMalloc): the fill of D, the top level and,
@@ -398,6 +424,20 @@ int lcg_batch_pf_D(const lcg_batch *batch, int32_t k, double *D, double *Dprime,
 int lcg_batch_bm_probs(const lcg_batch *batch, int32_t k, int32_t mode, double *dense, int32_t cap);
 /* d log Z / d theta of pair k (no counterpart in the reference) */
 int lcg_batch_pf_grad(const lcg_batch *batch, int32_t k, double *grad, int32_t cap);
+/* Fragment match probabilities (AlignerP::compute_fragment_match_prob,
+ * aligner_p.hh:605-609, aligner_p.icc:1440-1479; locarna_p
+ * --fragment-match-probs, locarna_p.cc:537-561): the fragments of pair k
+ * whose probabilities the next lcg_batch_pf_run computes, A[i..j] matched
+ * with B[k..l] as one block for each ijkl[4q .. 4q+3] (1-based, i <= j <= n,
+ * k <= l <= m; LCG_E_ARG otherwise); nq = 0 clears them.  A list longer
+ * than the one the LocARNA-P buffers were laid out for discards the results
+ * of the last run (the next run lays them out again).  A block with a
+ * corner cell, (i-1, k-1) or (j, l), outside the trace band has
+ * probability 0. */
+int lcg_batch_pf_set_fragments(lcg_batch *batch, int32_t k, int32_t nq, const int32_t *ijkl);
+/* The fragment match probabilities of pair k from the last lcg_batch_pf_run,
+ * in the order they were set; returns their count. */
+int lcg_batch_pf_fragment_probs(const lcg_batch *batch, int32_t k, double *prob, int32_t cap);
 
 /* Synthetic inputs (no counterpart in the reference; they replace
  * RNAfold -p, which the reference calls through ViennaRNA,

@@ -7,12 +7,14 @@
 //   Aligner(const AlignerParams &)              Aligner(const AlignerParams &)       aligner.hh:94
 //     align(), trace(), get_alignment()           same                               aligner.hh:101-120
 //     normalized_align(L), penalized_align(P)     same                               aligner.hh:166-171
+//     suboptimal(k, threshold, ...)               suboptimal(k, threshold) -> alignments aligner.hh:146
 //   AlignerP<T>(const AlignerPParams<T> &)      AlignerP<double>                      aligner_p.hh:501
 //     align_inside(), align_outside()             same                               aligner_p.hh:528-536
 //     compute_arcmatch_probabilities()            same                               aligner_p.hh:546
 //     compute_basematch_probabilities(bool)       same                               aligner_p.hh:541
 //     write_arcmatch_probabilities(ostream &)     same                               aligner_p.hh:555
 //     write_basematch_probabilities(ostream &)    same                               aligner_p.hh:564
+//     compute_fragment_match_prob(i, j, k, l)     same                               aligner_p.hh:605
 //   LocARNA::failure (aux.hh:229-255)           locarna_amd::failure (status code kept)
 //
 // Parameters use the reference's named-argument style
@@ -22,13 +24,15 @@
 // Scoring, ArcMatches, TraceController), this facade passes what they are
 // built from -- the #PP 2.0 texts (RnaData), the ScoringParams fields and
 // the band options -- and the library builds them (lcg_pair_prepare).
-// Features the library does not implement (no_lonely_pairs, struct_local,
-// stacking, anchor constraints) fail with failure(LCG_E_ARG) when asked for.
+// Features the library does not implement (struct_local, anchor
+// constraints; no_lonely_pairs and stacking for AlignerP) fail with
+// failure(LCG_E_ARG) when asked for.
 //
 // Header only, over the C ABI of locarna_amd.h; link -llocarna_amd.
 #ifndef LOCARNA_AMD_HPP
 #define LOCARNA_AMD_HPP
 
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -101,6 +105,12 @@ struct ScoringParams {
     int tau_factor = 50;
     double exp_prob = -1.0;  // <= 0: 1 / (2 length)
     int temperature_alipf = 300;
+    // stacking terms (scoring.hh:116-120): stacking derives stacked arc
+    // weights from the joint probabilities of #STACK files, new_stacking
+    // adds them to the plain weights; the DP follows these flags
+    // (aligner.cc:600, :629, :987)
+    bool stacking = false;
+    bool new_stacking = false;
 };
 
 // Base-pair and band options (the RnaData / TraceController arguments of
@@ -144,7 +154,7 @@ public:
     const RnaData *seqA_ = nullptr, *seqB_ = nullptr;
     ScoringParams scoring_;
     BandParams band_;
-    bool sequ_local_ = false, do_trace_ = true;
+    bool sequ_local_ = false, do_trace_ = true, no_lonely_pairs_ = false, stacking_ = false;
     std::string free_endgaps_ = "----";
     int max_diff_am_ = -1, max_diff_at_am_ = -1, device_ = 0;
 
@@ -176,6 +186,8 @@ public:
         p.pf_scale = 1.0;
         p.min_am_prob = 0.0;
         p.min_bm_prob = 0.0;
+        p.no_lonely_pairs = no_lonely_pairs_ ? 1 : 0;
+        p.stacking = (scoring_.stacking ? 1 : 0) | (scoring_.new_stacking ? 2 : 0);
         return p;
     }
 
@@ -185,9 +197,11 @@ protected:
     void set(const seqB &a) { seqB_ = a.value; }
     void set(const scoring &a) { scoring_ = *a.value; }
     void set(const trace_controller &a) { band_ = *a.value; }
-    void set(const no_lonely_pairs &a) { unsupported(a.value, "no_lonely_pairs"); }
+    void set(const no_lonely_pairs &a) { no_lonely_pairs_ = a.value; }
     void set(const struct_local &a) { unsupported(a.value, "struct_local"); }
-    void set(const stacking &a) { unsupported(a.value, "stacking"); }
+    // kept as the reference keeps it (aligner_params.hh:63); the stacked
+    // recursion follows ScoringParams::stacking / new_stacking
+    void set(const stacking &a) { stacking_ = a.value; }
     void set(const sequ_local &a) { sequ_local_ = a.value; }
     void set(const free_endgaps &a) { free_endgaps_ = a.value; }
     void set(const DO_TRACE &a) { do_trace_ = a.value; }
@@ -339,6 +353,32 @@ public:
         alignment_.reset();
         return score();
     }
+    // k-best suboptimal alignments by interval splitting (Aligner::suboptimal,
+    // aligner.hh:146-164, aligner.cc:1382-1514, not normalized; the
+    // reference writes each as it goes, here they are returned, best first).
+    std::vector<Alignment> suboptimal(int k, int64_t threshold = -1000000) {
+        std::vector<int64_t> scores(4096);
+        const int n = lcg_batch_suboptimal(batch_.get(), 0, k, threshold, scores.data(), (int32_t)scores.size());
+        detail::check(n, "lcg_batch_suboptimal");
+        std::vector<Alignment> out;
+        const int cap = lenA_ + lenB_ + 2;
+        for (int t = 0; t < n; ++t) {
+            std::vector<int32_t> ea(cap), eb(cap);
+            std::string sA(cap, '\0'), sB(cap, '\0');
+            const int ne = lcg_batch_suboptimal_alignment(batch_.get(), t, ea.data(), eb.data(), cap, &sA[0], &sB[0],
+                                                          cap, nullptr);
+            detail::check(ne, "lcg_batch_suboptimal_alignment");
+            Alignment a;
+            for (int e = 0; e < ne; ++e) a.edges_.push_back({ea[e], eb[e]});
+            a.structA_ = sA.c_str();
+            a.structB_ = sB.c_str();
+            a.score_ = t < (int)scores.size() ? scores[t] : 0;
+            out.push_back(std::move(a));
+        }
+        traced_ = false;  // the batch holds the last restricted run now
+        alignment_.reset();
+        return out;
+    }
     // The traced alignment (Aligner::get_alignment, aligner.hh:101).
     const Alignment &get_alignment() {
         if (!traced_) throw failure(LCG_E_STATE, "get_alignment before trace()");
@@ -391,7 +431,11 @@ template <typename T = double>
 class AlignerP {
 public:
     explicit AlignerP(const AlignerPParams<T> &ap) : ap_(ap) {
+        if (ap.no_lonely_pairs_) throw failure(LCG_E_ARG, "no_lonely_pairs: integer alignment only");
         lcg_params p = ap.c_params();
+        // locarna_p scores without stacking terms (src/locarna_p.cc:310
+        // reads the joint probabilities only)
+        if (p.stacking) throw failure(LCG_E_ARG, "stacking: integer alignment only");
         auto pb = detail::make_pair_batch(ap, p);
         pair_ = std::move(pb.first);
         batch_ = std::move(pb.second);
@@ -439,6 +483,27 @@ public:
         const int inc = include_am_in_bm_ ? 1 : 0;
         return out << detail::text_of(
                    [b, inc](char *buf, int64_t cap) { return lcg_batch_write_bm_probs(b, 0, inc, buf, cap); });
+    }
+    // Fragment match probability (AlignerP::compute_fragment_match_prob,
+    // aligner_p.hh:605-609, aligner_p.icc:1440-1479): the probability that
+    // A[i..j] and B[k..l] are matched as one block.  Like the reference,
+    // computes it anew: one inside + outside run with the fragment queued.
+    double compute_fragment_match_prob(int i, int j, int k, int l) {
+        return compute_fragment_match_probs({{i, j, k, l}})[0];
+    }
+    // Several fragments in one run (no counterpart in the reference).
+    std::vector<double> compute_fragment_match_probs(const std::vector<std::array<int, 4>> &frags) {
+        std::vector<int32_t> q;
+        for (const auto &f : frags) q.insert(q.end(), f.begin(), f.end());
+        const int32_t n = (int32_t)frags.size();
+        detail::check(lcg_batch_pf_set_fragments(batch_.get(), 0, n, q.data()), "lcg_batch_pf_set_fragments");
+        detail::check(lcg_batch_pf_run(batch_.get(), nullptr, nullptr), "lcg_batch_pf_run");
+        inside_ = true;
+        std::vector<double> p(std::max(n, 1));
+        const int got = lcg_batch_pf_fragment_probs(batch_.get(), 0, p.data(), (int32_t)p.size());
+        detail::check(got, "lcg_batch_pf_fragment_probs");
+        p.resize(got);
+        return p;
     }
     // inside D(am) and outside D'(am) (AlignerP::D / Dprime)
     std::pair<std::vector<double>, std::vector<double>> D_Dprime() const {

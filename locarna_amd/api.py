@@ -19,7 +19,7 @@ class Params(ctypes.Structure):
         ("min_prob", ctypes.c_double), ("min_trace_prob", ctypes.c_double),
         ("exp_prob", ctypes.c_double), ("pf_scale", ctypes.c_double),
         ("min_am_prob", ctypes.c_double), ("min_bm_prob", ctypes.c_double),
-        ("no_lonely_pairs", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("no_lonely_pairs", ctypes.c_int32), ("stacking", ctypes.c_int32),
     ]
 
 
@@ -116,6 +116,11 @@ def _sig(L):
     L.lcg_batch_pf_sample.argtypes = [c.c_void_p, c.c_int32, c.c_uint64, c.c_int32, c.c_void_p, c.c_void_p,
                                       c.c_int32]
     L.lcg_batch_pf_config.argtypes = [c.c_void_p] + [c.POINTER(c.c_int32)] * 3
+    L.lcg_batch_suboptimal.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_int64, c.c_void_p, c.c_int32]
+    L.lcg_batch_suboptimal_alignment.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_void_p, c.c_int32,
+                                                 c.c_char_p, c.c_char_p, c.c_int32, c.c_void_p]
+    L.lcg_batch_pf_set_fragments.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_void_p]
+    L.lcg_batch_pf_fragment_probs.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int32]
     L.lcg_batch_pair_info.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_int32), c.POINTER(c.c_int32),
                                       c.POINTER(c.c_int64)]
     L.lcg_batch_segment.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_int64]
@@ -413,6 +418,30 @@ class Batch:
             raise LcgError("lcg_batch_run_normalized (%d)" % rc)
         return it.value, ms.value
 
+    def suboptimal(self, k, kbest, threshold=-1000000):
+        """k-best suboptimal alignments of pair k (locarna --kbest kbest
+        --better threshold): list of (score, edges, (startA, endA, startB,
+        endB), structA, structB)."""
+        cap = 4096
+        scores = (ctypes.c_int64 * cap)()
+        n = lib().lcg_batch_suboptimal(self.h, k, kbest, threshold, scores, cap)
+        if n < 0:
+            raise LcgError("lcg_batch_suboptimal (%d)" % n)
+        p = self.pairs[k]
+        ec = p.lenA + p.lenB + 2
+        out = []
+        for t in range(n):
+            a, b = (ctypes.c_int32 * ec)(), (ctypes.c_int32 * ec)()
+            sa = ctypes.create_string_buffer(max(p.lenA, p.lenB) + 2)
+            sb = ctypes.create_string_buffer(max(p.lenA, p.lenB) + 2)
+            r = (ctypes.c_int32 * 4)()
+            ne = lib().lcg_batch_suboptimal_alignment(self.h, t, a, b, ec, sa, sb, len(sa), r)
+            if ne < 0:
+                raise LcgError("lcg_batch_suboptimal_alignment (%d)" % ne)
+            out.append((scores[t] if t < cap else None, [(a[e], b[e]) for e in range(ne)], tuple(r),
+                        sa.value.decode(), sb.value.decode()))
+        return out
+
     def score(self, k):
         s = ctypes.c_int64()
         rc = lib().lcg_batch_score(self.h, k, ctypes.byref(s))
@@ -546,6 +575,24 @@ class Batch:
         if cnt < 0:
             raise LcgError("lcg_batch_bm_probs (%d)" % cnt)
         return list(out)
+
+    def pf_set_fragments(self, k, frags):
+        """Fragments (i, j, k, l) of pair k, A[i..j] matched with B[k..l],
+        whose probabilities the next pf_run computes (locarna_p
+        --fragment-match-probs)."""
+        arr = (ctypes.c_int32 * max(1, 4 * len(frags)))(*[int(x) for f in frags for x in f])
+        rc = lib().lcg_batch_pf_set_fragments(self.h, k, len(frags), arr)
+        if rc != 0:
+            raise LcgError("lcg_batch_pf_set_fragments (%d)" % rc)
+
+    def pf_fragment_probs(self, k):
+        """Fragment match probabilities of pair k from the last pf_run."""
+        cap = 4096
+        out = (ctypes.c_double * cap)()
+        cnt = lib().lcg_batch_pf_fragment_probs(self.h, k, out, cap)
+        if cnt < 0:
+            raise LcgError("lcg_batch_pf_fragment_probs (%d)" % cnt)
+        return list(out)[:cnt]
 
     def pf_grad(self, k):
         out = (ctypes.c_double * NTHETA)()

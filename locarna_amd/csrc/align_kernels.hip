@@ -58,9 +58,11 @@ __device__ __forceinline__ int wave_scan_add(int v) {
     return v;
 }
 
-// inclusive prefix max over the 64 lanes
+// inclusive prefix max over the 64 lanes.  With INT_MIN (the identity of
+// max) as the value of lanes without a source, each step compiles to one
+// v_max_i32_dpp instead of a constant move, a DPP move and a max.
 __device__ __forceinline__ int wave_scan_max(int v) {
-    const int id = -0x7fffffff;
+    const int id = INT_MIN;
     v = max(v, dpp<0x111, 0xf>(id, v));
     v = max(v, dpp<0x112, 0xf>(id, v));
     v = max(v, dpp<0x114, 0xf>(id, v));
@@ -108,11 +110,12 @@ struct Pair {
     int *D;
     int *edges;
     int *structs;
-    const int *inner;  // --noLP: arc match -> arc match directly inside it (-1: none)
-    const int *c2a;    // --noLP: candidate entry -> arc match
+    const int *inner;  // --noLP / --stacking: arc match -> arc match directly inside it (-1: none)
+    const int *c2a;    // --noLP / --stacking: candidate entry -> arc match
+    const int *amst;   // --stacking: stacked score per arc match, LCG_NEG if not stackable
     int n, m, nam;
     int alpha, alpha_pos;
-    int nolp;
+    int nolp, stacking;
 };
 
 // Row i's stored column range in the group with origin (al, bl) and
@@ -163,6 +166,12 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
     int colv = global_col ? alpha : 0;
     bool colopen = true;
     int lo_p = bl, hi_p = hi0;
+    // M(i-1, bl) of the init column, also where row i-1 stores no column bl
+    // (a restricted top level whose origin lies right of the row's band;
+    // the reference's init column runs while min_col <= bl,
+    // aligner.cc:264-368); row al: the origin
+    int cprev = 0;
+    bool cprev_ok = true;
     // Candidate entries of row i are fetched one row ahead: the loads only
     // need the band, the M values they index are read after row i-1 is done.
     auto row_range = [&](int r, int &t0, int &t1) {
@@ -207,7 +216,10 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
             for (int t = t0 + lane; t < t1; t += 64) {
                 const int4 e = (t - t0 < 64) ? e0 : P.cand[t];
                 const int p = e.x & 0xffff, q = e.x >> 16;
-                const int rl = max(bl, e.w);  // first stored column of row p-1
+                // first stored column of row p-1: the region's first row is
+                // stored from bl, which lies left of the band for a
+                // restricted top level whose origin the band excludes
+                const int rl = (p - 1 == al) ? bl : max(bl, e.w);
                 if (p > al && q > bl && q - 1 >= rl) {
                     // top-level position penalty: D - lam (arc lengths), aligner_impl.hh:249-253
                     const int v = Mst[(p - 1 - al) * W + (q - 1 - rl)] + e.z - lam * ((i - p + 1) + (e.y - q + 1));
@@ -226,7 +238,9 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
                 int s = LCG_NEG, gB = 0;
                 if (act) {
                     const int up = (j >= lo_p && j <= hi_p) ? prow[j - lo_p] : LCG_NEG;
-                    const int dg = (j - 1 >= lo_p && j - 1 <= hi_p) ? prow[j - 1 - lo_p] : LCG_NEG;
+                    const int dg = (j - 1 >= lo_p && j - 1 <= hi_p) ? prow[j - 1 - lo_p]
+                                   : (j - 1 == bl && cprev_ok)      ? cprev
+                                                                    : LCG_NEG;
                     const int e = clampneg(max(Est[j - bl] + gA, up + gA + alpha));
                     Est[j - bl] = e;
                     s = clampneg(max(max(dg + sigrow[P.codeB[j]] - 2 * lam, e), Srow[j - jlo]));
@@ -260,6 +274,8 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
         }
         lo_p = lo;
         hi_p = hi;
+        cprev = colv;
+        cprev_ok = colopen;
         region_sync<GLOBAL>();
     }
     if (counts) {
@@ -504,7 +520,12 @@ __device__ void fill_group(const Pair &P, int al, int bl, int ar, int br, int W,
 __device__ __forceinline__ int Mget(const Pair &P, const int *Mst, int W, int al, int bl, int br, int i, int j) {
     const int lo = (i == al) ? bl : row_lo(P, i, bl);
     const int hi = row_hi(P, i, br);
-    if (j < lo || j > hi) return LCG_NEG;
+    if (j < lo || j > hi) {
+        // the init column where the row's band ends left of it (a restricted
+        // top level): group_fill keeps its value in the row's first slot
+        if (j == bl && i > al && hi < bl && P.mn[i] <= bl) return Mst[(i - al) * W];
+        return LCG_NEG;
+    }
     return Mst[(i - al) * W + (j - lo)];
 }
 
@@ -664,7 +685,29 @@ __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W
                     P.structs[c - 1] = 2;
                     P.structs[P.n + b - 1] = 1;
                     P.structs[P.n + e - 1] = 2;
-                    if (P.D[x] != P.D[y] + P.amscore[x]) break;
+                    if (P.D[x] != P.D[y] + (P.stacking ? P.amst[x] : P.amscore[x])) break;
+                    x = y;
+                }
+            } else if (P.stacking) {
+                // aligner.cc:967-1028: while the arc match's D is its inner
+                // one's plus its stacked score, the inner arc match is traced
+                // directly (only stackable arc matches have a stacked score)
+                int x = P.c2a[ft];
+                for (int depth = 0; depth < P.n && (unsigned)x < (unsigned)P.nam; ++depth) {
+                    const int y = P.inner[x];
+                    if ((unsigned)y >= (unsigned)P.nam) break;
+                    const int st = P.amst[x];
+                    if (st == LCG_NEG || P.D[x] != P.D[y] + st) break;
+                    ++a;
+                    ++b;
+                    --c;
+                    --e;
+                    emit(P, a, b, a + b);
+                    emit(P, c, e, c + e);
+                    P.structs[a - 1] = 1;
+                    P.structs[c - 1] = 2;
+                    P.structs[P.n + b - 1] = 1;
+                    P.structs[P.n + e - 1] = 2;
                     x = y;
                 }
             }
@@ -742,12 +785,14 @@ __device__ __forceinline__ Pair load_pair(const PairDesc &d, int *arena, int *ld
     P.structs = arena + d.structs;
     P.inner = arena + d.inner;
     P.c2a = arena + d.c2a;
+    P.amst = arena + d.amst;
     P.n = n;
     P.m = m;
     P.nam = d.nam;
     P.alpha = bp.alpha;
     P.alpha_pos = bp.alpha > 0 ? bp.alpha : 0;
     P.nolp = bp.nolp;
+    P.stacking = bp.stacking;
     return P;
 }
 
@@ -839,10 +884,21 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc
                         if (P.nolp) {
                             // aligner.cc:612-657: the inner arc match (its right
                             // ends are ar, br) ends the stack or continues it;
-                            // its D comes from a later row, written before
+                            // its D comes from a later row, written before.  With
+                            // stacking only a stackable arc match has a D, with
+                            // its stacked score
                             const int y = P.inner[ga.z];  // always one (the host lists no other)
-                            v = (unsigned)y < (unsigned)P.nam ? max(mv + P.amscore[y], P.D[y]) + P.amscore[ga.z]
-                                                              : LCG_NEG;
+                            const int sx = P.stacking ? P.amst[ga.z] : P.amscore[ga.z];
+                            v = (unsigned)y < (unsigned)P.nam && sx != LCG_NEG
+                                    ? max(mv + P.amscore[y], P.D[y]) + sx
+                                    : LCG_NEG;
+                        } else if (P.stacking) {
+                            // aligner.cc:574-610: a stackable arc match with an
+                            // inner one may take the inner one's D (a later row,
+                            // written before) plus its stacked score
+                            const int y = P.inner[ga.z];
+                            const int st = P.amst[ga.z];
+                            if ((unsigned)y < (unsigned)P.nam && st != LCG_NEG) v = max(v, P.D[y] + st);
                         }
                         P.cand[ga.y].z = v;  // D, not normalized (the reference's InftyInt sum)
                         P.fcand[3 * ga.w + 2] = v;
@@ -886,7 +942,8 @@ extern "C" __global__ void lcg_reset_barriers(int *barriers, int n) {
 extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc *__restrict__ descs,
                                                                  const int *__restrict__ order, int *queue,
                                                                  int *arena, int *scratch, BatchParams bp,
-                                                                 const int *__restrict__ lambda) {
+                                                                 const int *__restrict__ lambda,
+                                                                 const int4 *__restrict__ restr) {
     extern __shared__ int lds[];
     __shared__ int s_pair, s_tcount;
     const int nwaves = blockDim.x >> 6;
@@ -915,61 +972,82 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");  // the pair is this workgroup's alone
         __syncthreads();
         // ---- top level (wave 0)
-        const bool top_in_lds = d.top_ints <= bp.pool_ints;
+        // the top level aligns A[sA..eA] with B[sB..eB]: the whole sequences,
+        // or a restriction (AlignerRestriction, aligner_restriction.hh:26-120;
+        // the k-best enumeration, aligner.cc:1382-1514), whose region (origin
+        // (sA-1, sB-1), bounds (eA+1, eB+1)) the host sized the scratch for
+        int sA = 1, eA = n, sB = 1, eB = m;
+        if (restr) {
+            const int4 rr = restr[order[k]];
+            sA = rr.x;
+            eA = rr.y;
+            sB = rr.z;
+            eB = rr.w;
+        }
+        const int tal = sA - 1, tbl = sB - 1, tar = eA + 1, tbr = eB + 1;
+        const int tW = restr ? region_width(P, tal, tbl, tar, tbr) : d.wtop;
+        const int t_ints = restr ? (((tbr - tbl) + tW + (tar - tal) * tW + 3) & ~3) : d.top_ints;
+        const bool top_in_lds = t_ints <= bp.pool_ints;
         int *topbase = top_in_lds ? pool : wave_scratch;
-        int *topM = topbase + (m + 1) + d.wtop;
+        int *topM = topbase + (tbr - tbl) + tW;
         if (wave == 0) {
-            TopBest best{0, 0, 0};
+            TopBest best{0, tal, tbl};
             const int fe = bp.free_endgaps;
             if (bp.local) {
                 if (top_in_lds)
-                    group_fill<false, true>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, false, false, &best, counts, lam);
+                    group_fill<false, true>(P, tal, tbl, tar, tbr, tW, topbase, false, false, &best, counts, lam);
                 else
-                    group_fill<true, true>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, false, false, &best, counts, lam);
+                    group_fill<true, true>(P, tal, tbl, tar, tbr, tW, topbase, false, false, &best, counts, lam);
             } else {
                 // free end gaps: column 0 free with the second sequence's
                 // left end, row 0 with the first's (aligner.cc:736-745)
                 const bool gcol = !(fe & 4), grow = !(fe & 1);
                 if (top_in_lds)
-                    group_fill<false, false>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, gcol, grow, nullptr, counts, lam);
+                    group_fill<false, false>(P, tal, tbl, tar, tbr, tW, topbase, gcol, grow, nullptr, counts, lam);
                 else
-                    group_fill<true, false>(P, 0, 0, n + 1, m + 1, d.wtop, topbase, gcol, grow, nullptr, counts, lam);
+                    group_fill<true, false>(P, tal, tbl, tar, tbr, tW, topbase, gcol, grow, nullptr, counts, lam);
                 if (top_in_lds)
                     region_sync<false>();
                 else
                     region_sync<true>();
-                // M(n, m) of the reference's matrix; outside the band (free
-                // end gaps can cut it out) its never-written initial 0
-                best.score = (P.mn[n] <= m && m <= P.mx[n]) ? Mget(P, topM, d.wtop, 0, 0, m + 1, n, m) : 0;
-                best.i = n;
-                best.j = m;
+                // M(eA, eB) of the reference's matrix; outside the band (free
+                // end gaps can cut it out) its never-written initial 0, for a
+                // proper restriction minus infinity (the reference reads what
+                // an earlier computation left there)
+                const bool whole = sA == 1 && sB == 1 && eA == n && eB == m;
+                best.score = (P.mn[eA] <= eB && eB <= P.mx[eA]) ? Mget(P, topM, tW, tal, tbl, tbr, eA, eB)
+                             : whole                             ? 0
+                                                                 : LCG_NEG;
+                best.i = eA;
+                best.j = eB;
                 if (fe & (2 | 8)) {
                     // first strict maximum of the last column (second
-                    // sequence's right end; rows with max_col >= m,
+                    // sequence's right end; rows with max_col >= eB,
                     // ascending), then of the last row (first sequence's
                     // right end, band columns ascending) (aligner.cc:775-816)
                     best.score = LCG_NEG;
-                    best.i = best.j = 0;
+                    best.i = tal;
+                    best.j = tbl;
                     auto scan = [&](int cnt, bool col, int base) {
                         for (int t0 = 0; t0 < cnt; t0 += 64) {
                             const int t = t0 + lane;
-                            const int ci = col ? base + t : n, cj = col ? m : base + t;
-                            const bool ok = t < cnt && (!col || P.mx[ci] >= m);
-                            const int v = ok ? Mget(P, topM, d.wtop, 0, 0, m + 1, ci, cj) : INT_MIN;
+                            const int ci = col ? base + t : eA, cj = col ? eB : base + t;
+                            const bool ok = t < cnt && (!col || P.mx[ci] >= eB);
+                            const int v = ok ? Mget(P, topM, tW, tal, tbl, tbr, ci, cj) : INT_MIN;
                             int wmax = v;
                             for (int o = 32; o >= 1; o >>= 1) wmax = max(wmax, __shfl_xor(wmax, o));
                             if (wmax > best.score) {
                                 const unsigned long long bal = __ballot(ok && v == wmax);
                                 const int l = (int)__ffsll((long long)bal) - 1;
                                 best.score = wmax;
-                                best.i = col ? base + t0 + l : n;
-                                best.j = col ? m : base + t0 + l;
+                                best.i = col ? base + t0 + l : eA;
+                                best.j = col ? eB : base + t0 + l;
                             }
                         }
                     };
-                    if (fe & 8) scan(n, true, 1);
+                    if (fe & 8) scan(eA - sA + 1, true, sA);
                     if (fe & 2) {
-                        const int j0 = max(1, P.mn[n]), j1 = min(m, P.mx[n]);
+                        const int j0 = max(sB, P.mn[eA]), j1 = min(eB, P.mx[eA]);
                         if (j1 >= j0) scan(j1 - j0 + 1, false, j0);
                     }
                 }
@@ -982,10 +1060,10 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
             // ---- trace the top level
             if (bp.trace) {
                 if (top_in_lds)
-                    group_trace<false>(P, topM, d.wtop, 0, 0, m + 1, best.i, best.j, true, bp.local != 0,
+                    group_trace<false>(P, topM, tW, tal, tbl, tbr, best.i, best.j, true, bp.local != 0,
                                        bp.free_endgaps, arena + d.tstack, &s_tcount, lam);
                 else
-                    group_trace<true>(P, topM, d.wtop, 0, 0, m + 1, best.i, best.j, true, bp.local != 0,
+                    group_trace<true>(P, topM, tW, tal, tbl, tbr, best.i, best.j, true, bp.local != 0,
                                       bp.free_endgaps, arena + d.tstack, &s_tcount, lam);
             }
         }
@@ -1091,7 +1169,7 @@ extern "C" int lcg_noalloc_scratch(int device, int **ptr, long long *ints) {
 extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *queue, int *arena, int *scratch,
                                 int *barriers, BatchParams bp, int grid, int waves, size_t lds_bytes, BatchParams bp_top,
                                 int top_grid, int top_waves, size_t top_lds_bytes, const int *lambda, int top_only,
-                                hipEvent_t mid, hipStream_t stream) {
+                                hipEvent_t mid, hipStream_t stream, const int4 *restr) {
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void *)lcg_fill_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1114,6 +1192,6 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
     }
     if (mid) hipEventRecord(mid, stream);
     hipLaunchKernelGGL(lcg_top_pairs, dim3(top_grid), dim3(top_waves * 64), top_lds_bytes, stream, descs, order,
-                       queue + 16, arena, scratch, bp_top, lambda);
+                       queue + 16, arena, scratch, bp_top, lambda, restr);
     return (int)hipGetLastError();
 }

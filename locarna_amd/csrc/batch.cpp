@@ -3,6 +3,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <atomic>
 #include <climits>
@@ -15,6 +16,7 @@
 #include <memory>
 #include <mutex>
 #include <numeric>
+#include <queue>
 #include <string>
 #include <thread>
 #include <vector>
@@ -40,7 +42,7 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
                                 int *barriers, BatchParams bp, int grid, int waves, size_t lds_bytes, BatchParams bp_top,
                                 int top_grid,
                                 int top_waves, size_t top_lds_bytes, const int *lambda, int top_only, hipEvent_t mid,
-                                hipStream_t stream);
+                                hipStream_t stream, const int32_t *restr = nullptr);  // int4 per pair
 
 struct lcg_pair {
     lcg_params params;
@@ -82,7 +84,8 @@ int32_t reserve(std::vector<int32_t> &blob, size_t count, int32_t fill = 0) {
 // Offsets of a pair's arena image (device_layout.h) from its sizes, in the
 // order the host packing reserves them; the device compaction lays out its
 // pairs with it, so both produce the same image.  Returns the image size.
-size_t layout_pair(PairDesc &d, int n, int m, int ncells, int ngroups, int nam, int nfblk, bool nolp = false) {
+size_t layout_pair(PairDesc &d, int n, int m, int ncells, int ngroups, int nam, int nfblk, bool nolp = false,
+                   bool stacking = false) {
     size_t off = 0;
     auto take = [&](size_t count) {
         const int32_t o = (int32_t)off;
@@ -113,8 +116,9 @@ size_t layout_pair(PairDesc &d, int n, int m, int ncells, int ngroups, int nam, 
     d.tstack = take(2 * (size_t)(n + 1));
     align(2);
     d.out = take(12);
-    d.inner = take(nolp ? nam : 0);
-    d.c2a = take(nolp ? nam : 0);
+    d.inner = take(nolp || stacking ? nam : 0);
+    d.c2a = take(nolp || stacking ? nam : 0);
+    d.amst = take(stacking ? nam : 0);
     align(4);
     return off;
 }
@@ -124,20 +128,22 @@ size_t layout_pair(PairDesc &d, int n, int m, int ncells, int ngroups, int nam, 
 void rebase(PairDesc &d, int32_t base) {
     int32_t *offs[] = {&d.band, &d.codes, &d.gaps, &d.sigma, &d.rows, &d.groups, &d.gams, &d.amscore,
                        &d.cptr, &d.cand, &d.D, &d.edges, &d.structs, &d.tstack, &d.out, &d.rounds,
-                       &d.fptr, &d.fblk, &d.fcand, &d.f2c, &d.inner, &d.c2a};
+                       &d.fptr, &d.fblk, &d.fcand, &d.f2c, &d.inner, &d.c2a, &d.amst};
     for (int32_t *o : offs) *o += base;
 }
 
 // A rebased image lies in the arena at [base, base + size): every array
 // offset in that range and, with --noLP, its arc-match indices in range
 // (checked on the host before any launch).
-bool image_in_place(const PairDesc &d, int32_t base, int32_t size, const std::vector<int32_t> &arena) {
+bool image_in_place(const PairDesc &d, int32_t base, int32_t size, const std::vector<int32_t> &arena,
+                    bool stacking = false) {
     const int32_t offs[] = {d.band, d.codes, d.gaps, d.sigma, d.rows, d.groups, d.gams, d.amscore, d.cptr, d.cand,
                             d.D, d.edges, d.structs, d.tstack, d.out, d.rounds, d.fptr, d.fblk, d.fcand, d.f2c,
-                            d.inner, d.c2a};
+                            d.inner, d.c2a, d.amst};
     for (int32_t o : offs)
         if (o < base || o > base + size) return false;
-    if (d.c2a > d.inner) {  // --noLP arrays present
+    if (stacking && d.amst + d.nam > base + size) return false;  // --stacking scores present
+    if (d.c2a > d.inner) {  // --noLP / --stacking arrays present
         if (d.inner + d.nam > base + size || d.c2a + d.nam > base + size) return false;
         for (int k = 0; k < d.nam; ++k) {
             if (arena[d.inner + k] < -1 || arena[d.inner + k] >= d.nam) return false;
@@ -212,9 +218,12 @@ void pack(lcg_pair &P) {
     // one, over the inner arc match's region (aligner.cc:612-657,
     // arc_matches.cc:313-355)
     const bool nolp = P.params.no_lonely_pairs != 0;
-    std::vector<int> inner(nolp ? nam : 0, -1);
+    // --stacking: the inner arc match of every arc match too, for the
+    // stacked alternative of its D (aligner.cc:574-610)
+    const bool stacking = P.params.stacking != 0;
+    std::vector<int> inner(nolp || stacking ? nam : 0, -1);
     std::vector<std::vector<int>> by_group;
-    if (nolp) {
+    if (nolp || stacking) {
         for (int k = 0; k < nam; ++k) {
             const ArcMatch &x = ams.arcmatch(k);
             const Arc &a = bA.arc(x.arcA), &b = bB.arc(x.arcB);
@@ -227,6 +236,8 @@ void pack(lcg_pair &P) {
                 }
             }
         }
+    }
+    if (nolp) {
         by_group.assign((size_t)(n + 1) * (m + 1), {});
         for (int k = 0; k < nam; ++k) {
             if (inner[k] < 0) continue;
@@ -408,17 +419,20 @@ void pack(lcg_pair &P) {
     // blob is a multiple of 4 ints, so blob offsets survive concatenation)
     if (blob.size() & 1) blob.push_back(0);
     d.out = reserve(blob, 12);
-    d.inner = reserve(blob, nolp ? nam : 0);
-    d.c2a = reserve(blob, nolp ? nam : 0);
-    if (nolp) {
+    d.inner = reserve(blob, nolp || stacking ? nam : 0);
+    d.c2a = reserve(blob, nolp || stacking ? nam : 0);
+    if (nolp || stacking) {
         std::copy(inner.begin(), inner.end(), blob.begin() + d.inner);
         for (int k = 0; k < nam; ++k) blob[d.c2a + entry_of[k]] = k;
     }
+    d.amst = reserve(blob, stacking ? nam : 0);
+    if (stacking)
+        for (int k = 0; k < nam; ++k) blob[d.amst + k] = sc.stackable(k) ? (int32_t)sc.arcmatch_stacked(k) : LCG_NEG;
     while (blob.size() & 3) blob.push_back(0);
     d.gmax = gbuf;
     {
         PairDesc L = d;
-        const size_t size = layout_pair(L, n, m, ncells, d.ngroups, nam, d.nfblk, nolp);
+        const size_t size = layout_pair(L, n, m, ncells, d.ngroups, nam, d.nfblk, nolp, stacking);
         if (size != blob.size() || std::memcmp(&L, &d, sizeof d) != 0) throw failure("internal: arena layout");
     }
     P.desc = d;
@@ -541,8 +555,9 @@ void prepare_parse(const char *ppA, const char *ppB, const lcg_params *p, lcg_pa
     if (p->sequ_local && p->free_endgaps)  // src/locarna.cc:375-378
         throw failure("Free endgaps cannot be combined with local alignment.");
     if (p->free_endgaps & ~15) throw failure("free end gaps: bits 1, 2, 4, 8 only");
-    P.A.reset(new RnaData(ppA, p->min_prob, p->max_bp_span));
-    P.B.reset(new RnaData(ppB, p->min_prob, p->max_bp_span));
+    if (p->stacking & ~3) throw failure("stacking: bits 1 (--stacking) and 2 (--new-stacking) only");
+    P.A.reset(new RnaData(ppA, p->min_prob, p->max_bp_span, p->stacking != 0));
+    P.B.reset(new RnaData(ppB, p->min_prob, p->max_bp_span, p->stacking != 0));
     if (P.A->length() >= 0xffff || P.B->length() >= 0xffff) throw failure("sequence too long");
     ScoringParams &sp = P.sp;
     sp.indel = p->indel;
@@ -554,6 +569,7 @@ void prepare_parse(const char *ppA, const char *ppB, const lcg_params *p, lcg_pa
     sp.temperature_alipf = p->temperature_alipf;
     sp.use_ribosum = p->use_ribosum != 0;
     sp.exp_prob = p->exp_prob;
+    sp.stacking = p->stacking;
     P.tc.reset(new TraceController(P.A->length(), P.B->length()));
     if (p->max_diff >= 0) P.tc->constrain_wo_ref(p->max_diff);  // locarna.cc:568-569
 }
@@ -593,7 +609,10 @@ int64_t score_bound(const lcg_pair &P) {
                                                                                 (x >> 2) & 3, x & 3)));
         amax = wa + wb + (std::llabs((int64_t)P.sp.tau) * seq) / 100 + 1;
     } else {
-        for (int k = 0; k < P.am->num_arc_matches(); ++k) amax = std::max<int64_t>(amax, std::llabs(sc.arcmatch(k)));
+        for (int k = 0; k < P.am->num_arc_matches(); ++k) {
+            amax = std::max<int64_t>(amax, std::llabs(sc.arcmatch(k)));
+            if (sc.stackable(k)) amax = std::max<int64_t>(amax, std::llabs(sc.arcmatch_stacked(k)));
+        }
     }
     const int64_t alpha = std::llabs((int64_t)P.sp.indel_opening);
     return (n + m) * (smax + gmax + alpha) + (std::min(n, m) / 2) * amax;
@@ -639,6 +658,10 @@ int prepare(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair **ou
         set_err(err, errlen, "--noLP needs the arc matches enumerated on the host");
         return LCG_E_ARG;
     }
+    if (deferred && p->stacking) {  // nor the inner arc matches and stacked scores
+        set_err(err, errlen, "--stacking needs the arc matches enumerated on the host");
+        return LCG_E_ARG;
+    }
     *out = nullptr;
     try {
         auto P = std::make_unique<lcg_pair>();
@@ -675,6 +698,12 @@ extern "C" int lcg_pair_prepare_scored(const char *ppA, const char *ppB, const l
                                        char *err, int errlen) {
     if (!ppA || !ppB || !p || !arcmatch_scores || !out) return LCG_E_ARG;
     *out = nullptr;
+    // explicit arc-match scores take no stacked scores (the reference's
+    // Scoring::arcmatch ignores stacking for them, scoring.cc:541-546)
+    if (p->stacking) {
+        set_err(err, errlen, "--stacking needs arc matches from the pp input, not an arc-match scores file");
+        return LCG_E_ARG;
+    }
     try {
         auto P = std::make_unique<lcg_pair>();
         prepare_parse(ppA, ppB, p, *P);
@@ -850,6 +879,18 @@ struct lcg_batch {
     // penalized / normalized alignment: per-pair position penalty of the top
     // level (device), the normalized scores of the last normalized run
     int32_t *d_lambda = nullptr;
+    // k-best suboptimal alignments of the last lcg_batch_suboptimal call;
+    // the per-pair restrictions (int4) and the top-level scratch of its runs
+    struct SuboptAln {
+        int64_t score;
+        std::vector<int32_t> a, b;
+        std::string sA, sB;
+        int32_t restr[4];
+    };
+    std::vector<SuboptAln> subopt;
+    int32_t *d_restr = nullptr;
+    int32_t *d_kscratch = nullptr;
+    size_t kscratch_ints = 0;
     std::vector<int64_t> norm_score;
     bool norm_valid = false;
     // names and sequences of every pair, for the alignment text
@@ -866,6 +907,12 @@ struct lcg_batch {
     PfBatchParams pbp;
     bool pf_have = false;
     std::vector<double> pf_result;
+    // fragment match queries per pair (set for the next pf run), the
+    // capacity of each pair's query block in the pf arena, and the
+    // probabilities of the last run
+    std::vector<std::vector<std::array<int32_t, 4>>> frags;
+    std::vector<int32_t> frag_cap;
+    std::vector<std::vector<double>> frag_prob;
     // device compaction (pairs from lcg_pair_prepare_device)
     bool device_built = false;
     std::vector<int32_t> cstats;      // per pair LCG_CS_INTS (counts, status)
@@ -1303,9 +1350,11 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
     for (int k = 0; k < npairs; ++k) {
         const lcg_pair *P = pairs[k];
         if (P->params.indel_opening != p0.indel_opening || P->params.sequ_local != p0.sequ_local ||
-            P->params.free_endgaps != p0.free_endgaps || P->params.no_lonely_pairs != p0.no_lonely_pairs) {
+            P->params.free_endgaps != p0.free_endgaps || P->params.no_lonely_pairs != p0.no_lonely_pairs ||
+            (P->params.stacking != 0) != (p0.stacking != 0)) {
             set_err(err, errlen,
-                    "all pairs of a batch must share indel_opening, sequ_local, free_endgaps and no_lonely_pairs");
+                    "all pairs of a batch must share indel_opening, sequ_local, free_endgaps, no_lonely_pairs "
+                    "and stacking");
             return LCG_E_ARG;
         }
         if (P->deferred != deferred) {
@@ -1343,7 +1392,7 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
             B->host_arena.insert(B->host_arena.end(), P->blob.begin(), P->blob.end());
             PairDesc d = P->desc;
             rebase(d, base);
-            if (!image_in_place(d, base, (int32_t)P->blob.size(), B->host_arena)) {
+            if (!image_in_place(d, base, (int32_t)P->blob.size(), B->host_arena, P->params.stacking != 0)) {
                 set_err(err, errlen, "internal: image offsets");
                 return LCG_E_STATE;
             }
@@ -1389,6 +1438,7 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
     B->bp.local = p0.sequ_local;
     B->bp.free_endgaps = p0.free_endgaps;
     B->bp.nolp = p0.no_lonely_pairs != 0;
+    B->bp.stacking = p0.stacking != 0;
     B->bp.trace = 1;
     B->bp.npairs = npairs;
     B->bp.hdr_ints = B->hdr;
@@ -1460,6 +1510,8 @@ lcg_batch::~lcg_batch() {
     if (B->d_queue) hipFree(B->d_queue);
     if (B->d_barriers) hipFree(B->d_barriers);
     if (B->d_scratch) hipFree(B->d_scratch);
+    if (B->d_restr) hipFree(B->d_restr);
+    if (B->d_kscratch) hipFree(B->d_kscratch);
     if (B->d_pfa) hipFree(B->d_pfa);
     if (B->d_pdescs) hipFree(B->d_pdescs);
     if (B->d_pfscratch) hipFree(B->d_pfscratch);
@@ -1794,6 +1846,156 @@ inline int32_t res(const lcg_batch *B, int k, int32_t off) {
 }
 }  // namespace
 
+// k-best suboptimal alignments by interval splitting (Aligner::suboptimal,
+// aligner.cc:1382-1514, not normalized).  The fill of D runs once for the
+// batch; every restriction of pair k is then one top-level launch over that
+// pair only (one workgroup), with the traceback for the enumerated ones.
+// Tasks are ordered as the reference orders them: a max-heap on the score
+// (greater_second, aligner.hh:34-45, compares with <; std::priority_queue).
+namespace {
+int restricted_top(lcg_batch *B, int k, const int32_t r[4], bool trace, int64_t *score) {
+    HIPCHECK(hipSetDevice(B->device));
+    const PairDesc &d = B->descs[k];
+    // the stored width of a restricted region is at most m + 1
+    const size_t need = ((size_t)(d.m + 1) * 2 + (size_t)(d.n + 1) * (d.m + 1) + 3) & ~(size_t)3;
+    if (need > B->kscratch_ints) {
+        if (B->d_kscratch) HIPCHECK(hipFree(B->d_kscratch));
+        B->d_kscratch = nullptr;
+        HIPCHECK(hipMalloc(&B->d_kscratch, need * B->top_waves * 4));
+        B->kscratch_ints = need;
+    }
+    if (!B->d_restr) HIPCHECK(hipMalloc(&B->d_restr, 16 * (size_t)B->npairs));
+    // the restriction goes into the pair's slot (indexed by pair, like the
+    // position penalties)
+    HIPCHECK(hipMemcpyAsync(B->d_restr + 4 * (size_t)k, r, 16, hipMemcpyHostToDevice, B->stream));
+    // queue position of pair k (order[pos] == k)
+    int pos = -1;
+    for (int t = 0; t < B->npairs && pos < 0; ++t)
+        if (B->order[t] == k) pos = t;
+    if (pos < 0) return LCG_E_STATE;
+    BatchParams bp = B->bp, top = B->bp_top;
+    bp.npairs = top.npairs = 1;
+    bp.trace = top.trace = trace ? 1 : 0;
+    bp.count = top.count = 0;
+    top.scratch_stride = (int32_t)B->kscratch_ints;
+    int rc = lcg_launch_align(B->d_descs, B->d_order + pos, B->d_queue, B->d_arena, B->d_kscratch, B->d_barriers, bp,
+                              1, B->waves, B->lds_bytes, top, 1, B->top_waves, B->top_lds_bytes, nullptr, 1, nullptr,
+                              B->stream, B->d_restr);
+    if (rc) return hip_fail((hipError_t)rc);
+    HIPCHECK(hipStreamSynchronize(B->stream));
+    B->fill_ms = B->top_ms = -1;
+    B->have_results = true;
+    B->fetched = false;
+    B->traced = trace;
+    B->norm_valid = false;
+    if (int rc2 = fetch(B)) return rc2;
+    *score = (int64_t)res(B, k, B->descs[k].out);
+    return LCG_OK;
+}
+}  // namespace
+
+extern "C" int lcg_batch_suboptimal(lcg_batch *B, int32_t k, int32_t kbest, int64_t threshold, int64_t *scores,
+                                    int32_t cap) {
+    if (!B || k < 0 || k >= B->npairs || kbest == 0 || cap < 0 || (cap > 0 && !scores)) return LCG_E_ARG;
+    B->subopt.clear();
+    // the fill of D of every pair (and the unrestricted alignment)
+    if (int rc = lcg_batch_run(B, 0, 1, nullptr)) return rc;
+    if (int rc = fetch(B)) return rc;
+    const PairDesc &d = B->descs[k];
+    const int n = d.n, m = d.m;
+    struct Task {
+        int32_t r[4];  // startA, endA, startB, endB
+        int64_t score;
+    };
+    struct Less {
+        bool operator()(const Task &x, const Task &y) const { return x.score < y.score; }
+    };
+    std::priority_queue<Task, std::vector<Task>, Less> tasks;
+    Task whole{{1, n, 1, m}, 0};
+    if (int rc = restricted_top(B, k, whole.r, false, &whole.score)) return rc;
+    tasks.push(whole);
+    std::vector<int32_t> ea(n + m + 2), eb(n + m + 2);
+    std::string sa(std::max(n, m) + 2, '\0'), sb(std::max(n, m) + 2, '\0');
+    size_t i = 1;
+    while (kbest < 0 || i <= (size_t)kbest) {
+        if (tasks.empty()) break;
+        const Task task = tasks.top();
+        tasks.pop();
+        // a restriction without an alignment (minus infinity) ends the
+        // enumeration like one below the threshold
+        if (task.score < threshold + 1 || task.score < LCG_NEG_LIMIT) break;
+        int64_t sc = 0;
+        if (int rc = restricted_top(B, k, task.r, true, &sc)) return rc;
+        const int ne = lcg_batch_alignment(B, k, ea.data(), eb.data(), (int32_t)ea.size(), &sa[0], &sb[0],
+                                           (int32_t)sa.size());
+        if (ne < 0) return ne;
+        if (ne == 0) continue;  // empty alignment
+        lcg_batch::SuboptAln al;
+        al.score = task.score;
+        al.a.assign(ea.begin(), ea.begin() + ne);
+        al.b.assign(eb.begin(), eb.begin() + ne);
+        al.sA = sa.c_str();
+        al.sB = sb.c_str();
+        std::copy(task.r, task.r + 4, al.restr);
+        B->subopt.push_back(std::move(al));
+        if (kbest >= 0 && i == (size_t)kbest) break;
+        // start / end positions (Alignment::start_positions / end_positions,
+        // alignment.cc:170-195) and the split of the longer side
+        int fa = -1, fb = -1, la = 0, lb = 0;
+        for (int t = 0; t < ne; ++t) {
+            if (fa < 0 && ea[t] > 0) fa = ea[t];
+            if (fb < 0 && eb[t] > 0) fb = eb[t];
+            if (ea[t] > 0) la = ea[t];
+            if (eb[t] > 0) lb = eb[t];
+        }
+        if (fa < 0) fa = n;
+        if (fb < 0) fb = m;
+        Task r1 = task, r2 = task;
+        if (task.r[1] - task.r[0] > task.r[3] - task.r[2]) {
+            const int split = (fa + la) / 2;
+            r1.r[1] = split;
+            r2.r[0] = split;
+        } else {
+            const int split = (fb + lb) / 2;
+            r1.r[3] = split;
+            r2.r[2] = split;
+        }
+        // a restriction the split leaves empty aligns nothing (its region
+        // would have no rows or columns): score 0 locally, minus infinity
+        // globally, without a launch
+        for (Task *t : {&r1, &r2}) {
+            if (t->r[1] < t->r[0] || t->r[3] < t->r[2] || t->r[0] < 1 || t->r[2] < 1 || t->r[1] > n ||
+                t->r[3] > m) {
+                t->score = B->bp.local ? 0 : LCG_NEG;
+                continue;
+            }
+            if (int rc = restricted_top(B, k, t->r, false, &t->score)) return rc;
+        }
+        tasks.push(r1);
+        tasks.push(r2);
+        ++i;
+    }
+    const int cnt = (int)std::min<size_t>(B->subopt.size(), (size_t)cap);
+    for (int t = 0; t < cnt; ++t) scores[t] = B->subopt[t].score;
+    return (int)B->subopt.size();
+}
+
+extern "C" int lcg_batch_suboptimal_alignment(const lcg_batch *B, int32_t t, int32_t *a, int32_t *b, int32_t cap,
+                                              char *structA, char *structB, int32_t scap, int32_t *restriction) {
+    if (!B || t < 0 || t >= (int32_t)B->subopt.size()) return LCG_E_ARG;
+    const auto &al = B->subopt[t];
+    if (cap < (int32_t)al.a.size()) return LCG_E_ARG;
+    std::copy(al.a.begin(), al.a.end(), a);
+    std::copy(al.b.begin(), al.b.end(), b);
+    if (structA && structB) {
+        if (scap <= (int32_t)al.sA.size() || scap <= (int32_t)al.sB.size()) return LCG_E_ARG;
+        std::memcpy(structA, al.sA.c_str(), al.sA.size() + 1);
+        std::memcpy(structB, al.sB.c_str(), al.sB.size() + 1);
+    }
+    if (restriction) std::copy(al.restr, al.restr + 4, restriction);
+    return (int32_t)al.a.size();
+}
+
 extern "C" int lcg_batch_score(const lcg_batch *B, int32_t k, int64_t *score) {
     if (!B || k < 0 || k >= B->npairs || !score) return LCG_E_ARG;
     if (int rc = fetch(B)) return rc;
@@ -1985,6 +2187,9 @@ int pf_setup(lcg_batch &B) {
         f.scal = take(4);
         f.out = take(4);
         f.fD = take(h.nam);
+        const int cap = B.frags.empty() ? 0 : (int)B.frags[k].size();
+        f.frag = take(8 * (size_t)cap);
+        f.nfrag = 0;
         max_len = std::max(max_len, std::max(h.n, h.m));
         const PairDesc &d = B.descs[k];
         const size_t top_cells = (size_t)(d.n + 1) * d.wtop;
@@ -1992,6 +2197,8 @@ int pf_setup(lcg_batch &B) {
         // groups are no larger than the top level region
     }
     B.pf_arena.assign(off, 0.0);
+    B.frag_cap.assign(np, 0);
+    for (int k = 0; k < np && !B.frags.empty(); ++k) B.frag_cap[k] = (int32_t)B.frags[k].size();
     B.pf_waves = pick(B.req.pf_waves, "LCG_PF_WAVES", 8, 1, 8);
     int nfblk_max = 1;
     for (int k = 0; k < np; ++k) nfblk_max = std::max(nfblk_max, B.descs[k].nfblk);
@@ -2145,6 +2352,42 @@ int pf_setup(lcg_batch &B) {
     return LCG_OK;
 }
 
+// Fragment match queries of every pair into the pf arena: i, j, k, l, the
+// stored row width W of the block's region (i-1, k-1; j+1, l+1), and in /
+// out cleared.  A block whose corner cell (i-1, k-1) or (j, l) lies outside
+// the band has probability 0 (no alignment in the band passes through it):
+// W = 0, the kernels skip it.
+int frag_write(lcg_batch &B) {
+    for (int k = 0; k < B.npairs; ++k) {
+        PfDesc &f = B.pdescs[k];
+        const int nq = B.frags.empty() ? 0 : (int)B.frags[k].size();
+        if (nq > B.frag_cap[k]) return LCG_E_STATE;  // set_fragments drops the arena first
+        f.nfrag = nq;
+        const PfHost &h = B.pfh[k];
+        for (int q = 0; q < nq; ++q) {
+            const auto &x = B.frags[k][q];
+            const int i = x[0], j = x[1], kk = x[2], l = x[3];
+            auto valid = [&](int r, int c) { return h.min_col[r] <= c && c <= h.max_col[r]; };
+            int W = 0;
+            if (valid(i - 1, kk - 1) && valid(j, l)) {
+                W = std::min(l, h.max_col[i - 1]) - (kk - 1) + 1;
+                for (int r = i; r <= j; ++r)
+                    W = std::max(W, std::min(l, h.max_col[r]) - std::max(kk - 1, h.min_col[r]) + 1);
+                const size_t need = 4 * (size_t)(j - i + 2) * W + 2 * (size_t)(l - kk + 2);
+                if (need > (size_t)B.pbp.scratch_stride) return LCG_E_CAPACITY;
+            }
+            double *d = B.pf_arena.data() + f.frag + 8 * q;
+            d[0] = i;
+            d[1] = j;
+            d[2] = kk;
+            d[3] = l;
+            d[4] = W;
+            d[5] = d[6] = d[7] = 0.0;
+        }
+    }
+    return LCG_OK;
+}
+
 // Boltzmann tables of pair k for parameters theta0 + dtheta.
 void pf_tables(const lcg_batch &B, int k, const double *dth, double *A) {
     const PfHost &h = B.pfh[k];
@@ -2174,11 +2417,14 @@ extern "C" int lcg_batch_pf_run(lcg_batch *B, const double *dtheta, float *devic
     B->pf_have = false;  // the results of an earlier run are overwritten from here on
     if (B->bp.free_endgaps) return LCG_E_ARG;  // free end gaps: integer path only
     if (B->bp.nolp) return LCG_E_ARG;          // --noLP: integer path only
+    if (B->bp.stacking) return LCG_E_ARG;      // --stacking: integer path only
     if (B->device_built) return LCG_E_STATE;  // LocARNA-P reads the host arc-match lists
     if (int rc = pf_setup(*B)) return rc;
     for (int k = 0; k < B->npairs; ++k) pf_tables(*B, k, dtheta, B->pf_arena.data());
+    if (int rc = frag_write(*B)) return rc;
     HIPCHECK(hipSetDevice(B->device));
     HIPCHECK(hipMemcpy(B->d_pfa, B->pf_arena.data(), B->pf_arena.size() * 8, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(B->d_pdescs, B->pdescs.data(), sizeof(PfDesc) * B->npairs, hipMemcpyHostToDevice));
     B->pbp.spin_limit = B->bp.spin_limit;
     HIPCHECK(hipEventRecord(B->ev0, B->stream));
     int rc = lcg_launch_pf(B->d_descs, B->d_pdescs, B->d_order, B->d_queue, B->d_arena, B->d_pfa, B->d_pfscratch,
@@ -2192,8 +2438,47 @@ extern "C" int lcg_batch_pf_run(lcg_batch *B, const double *dtheta, float *devic
     // a bounded wait of the multi-workgroup path that expired invalidates the run
     for (const PfDesc &f : B->pdescs)
         if (B->pf_result[f.out + 2] != 0.0) return LCG_E_STATE;
+    B->frag_prob.assign(B->npairs, {});
+    for (int k = 0; k < B->npairs; ++k) {
+        const PfDesc &f = B->pdescs[k];
+        const double Z = B->pf_result[f.out], ps = B->pfh[k].ps;
+        for (int q = 0; q < f.nfrag; ++q) {
+            const double *x = B->pf_result.data() + f.frag + 8 * q;
+            B->frag_prob[k].push_back(x[4] > 0.0 ? x[5] * x[6] * ps / Z : 0.0);
+        }
+    }
     B->pf_have = true;
     return LCG_OK;
+}
+
+namespace {
+void drop_pf_state(lcg_batch &B);
+}
+
+extern "C" int lcg_batch_pf_set_fragments(lcg_batch *B, int32_t k, int32_t nq, const int32_t *ijkl) {
+    if (!B || k < 0 || k >= B->npairs || nq < 0 || (nq > 0 && !ijkl)) return LCG_E_ARG;
+    if (B->device_built) return LCG_E_STATE;  // LocARNA-P needs host-packed pairs
+    const PfHost &h = B->pfh[k];
+    std::vector<std::array<int32_t, 4>> q((size_t)nq);
+    for (int t = 0; t < nq; ++t) {
+        const int32_t *x = ijkl + 4 * t;
+        if (x[0] < 1 || x[1] < x[0] || x[1] > h.n || x[2] < 1 || x[3] < x[2] || x[3] > h.m) return LCG_E_ARG;
+        q[t] = {x[0], x[1], x[2], x[3]};
+    }
+    if (B->frags.empty()) B->frags.resize(B->npairs);
+    // a larger query block than the pf arena holds: lay it out again
+    if (B->d_pfa && (B->frag_cap.empty() || nq > B->frag_cap[k])) drop_pf_state(*B);
+    B->frags[k] = std::move(q);
+    return LCG_OK;
+}
+
+extern "C" int lcg_batch_pf_fragment_probs(const lcg_batch *B, int32_t k, double *prob, int32_t cap) {
+    if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    if (!B->pf_have) return LCG_E_STATE;
+    const std::vector<double> &p = B->frag_prob[k];
+    if (cap < (int32_t)p.size() || (!prob && !p.empty())) return LCG_E_ARG;
+    std::copy(p.begin(), p.end(), prob);
+    return (int)p.size();
 }
 
 extern "C" int lcg_batch_pf_config(lcg_batch *B, int32_t *wg_per_pair, int32_t *nstatic, int32_t *workgroups) {
@@ -3163,6 +3448,10 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
     const bool dev_band = (o.flags & LCG_PREPARE_DEVICE_BAND) != 0;
     if (deferred && p->no_lonely_pairs) {  // the device compaction builds no --noLP groups
         set_err(err, errlen, "--noLP needs the arc matches enumerated on the host");
+        return LCG_E_ARG;
+    }
+    if (deferred && p->stacking) {  // nor the inner arc matches and stacked scores
+        set_err(err, errlen, "--stacking needs the arc matches enumerated on the host");
         return LCG_E_ARG;
     }
     std::vector<std::unique_ptr<lcg_pair>> P(npairs);

@@ -80,9 +80,9 @@ typedef struct {
     int32_t fptr, fblk, fcand;
     int32_t nfblk;     // fill blocks including the sentinel
     int32_t f2c;       // fill entry -> candidate entry (cell order)
-    int32_t inner;     // --noLP only: per arc match the arc match directly inside it, -1 if none
-    int32_t c2a;       // --noLP only: candidate entry -> arc match
-    int32_t pad[1];
+    int32_t inner;     // --noLP / --stacking only: per arc match the arc match directly inside it, -1 if none
+    int32_t c2a;       // --noLP / --stacking only: candidate entry -> arc match
+    int32_t amst;      // --stacking only: per arc match its stacked score, LCG_NEG if not stackable
 } PairDesc;  // 32 ints
 
 typedef struct {
@@ -103,4 +103,5 @@ typedef struct {
     int32_t spin_limit;      // polls of a bounded cross-workgroup wait (0: default,
                              // -1: every wait gives up at once, for error-path tests)
     int32_t nolp;            // --noLP: D of stacks (groups keyed by the inner left ends)
+    int32_t stacking;        // --stacking / --new-stacking: stacked alternative of D
 } BatchParams;

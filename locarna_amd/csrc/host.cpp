@@ -31,7 +31,7 @@ bool next_content_line(std::istream &in, std::string &line) {
 bool starts_with(const std::string &s, const char *p) { return s.compare(0, std::strlen(p), p) == 0; }
 }  // namespace
 
-RnaData::RnaData(const std::string &pp_text, double p_bpcut, long max_bp_span) : p_bpcut_(p_bpcut) {
+RnaData::RnaData(const std::string &pp_text, double p_bpcut, long max_bp_span, bool stacking) : p_bpcut_(p_bpcut) {
     std::istringstream in(pp_text);
     std::string line;
     std::getline(in, line);
@@ -57,9 +57,11 @@ RnaData::RnaData(const std::string &pp_text, double p_bpcut, long max_bp_span) :
     if (!next_content_line(in, line) || line != "#SECTION BASEPAIRS")
         throw failure("Syntax error: Expected base pair section header.");
     const size_t span_max = max_bp_span < 0 ? std::numeric_limits<size_t>::max() : (size_t)max_bp_span;
+    bool stack_keyword = false;
     while (next_content_line(in, line)) {
         if (line[0] == '#') {
             if (starts_with(line, "#END")) break;
+            if (starts_with(line, "#STACK")) stack_keyword = true;
             if (starts_with(line, "#BPCUT")) {
                 std::istringstream ls(line);
                 std::string kw;
@@ -79,8 +81,19 @@ RnaData::RnaData(const std::string &pp_text, double p_bpcut, long max_bp_span) :
             throw failure("Syntax error: Invalid indices in PP input line \"" + line + "\".");
         if (p <= p_bpcut_ || j - i + 1 > span_max) continue;
         probs_[{(int)i, (int)j}] = p;
+        // with stacking a fourth column: the joint probability of (i, j) and
+        // (i+1, j-1), kept above the cut (rna_data.cc:1083-1092)
+        double p2;
+        if (stacking && (ls >> p2) && p2 > p_bpcut_) joint_[{(int)i, (int)j}] = p2;
     }
+    if (!stack_keyword && !joint_.empty())  // rna_data.cc:1097-1100
+        throw failure("Syntax error: Stacking probabilties found but stack keyword is missing.");
     for (const auto &x : probs_) pairs_.push_back(x);
+}
+
+double RnaData::joint_arc_prob(int i, int j) const {
+    auto it = joint_.find({i, j});
+    return it == joint_.end() ? 0.0 : it->second;
 }
 
 double RnaData::arc_prob(int i, int j) const {
@@ -519,6 +532,36 @@ Scoring::Scoring(const RnaData &A, const RnaData &B, const ArcMatches &am, const
         }
         am_seq_[k] = seq;
         am_score_[k] = ((score_t)p.tau * seq) / 100 + (wA_[x.arcA] + wB_[x.arcB]);
+    }
+    if (!p.stacking) return;
+    // stacked weights (scoring.cc:200-236): --stacking the weight of the
+    // stacked probability p2 / p(i+1, j-1), --new-stacking the arc's weight
+    // plus that of p2, both: the sum of the two terms; for arcs whose inner
+    // arc exists and that have a joint probability (only arc matches of such
+    // arcs on both sides use them; elsewhere the reference's weight of a zero
+    // probability is not finite)
+    auto stack_w = [&](const RnaData &R, const BasePairs &bp, const std::vector<score_t> &w, double pe) {
+        std::vector<score_t> sw(bp.num_bps(), 0);
+        for (int k = 0; k < bp.num_bps(); ++k) {
+            const Arc &a = bp.arc(k);
+            const double pin = R.arc_prob(a.left + 1, a.right - 1), p2 = R.joint_arc_prob(a.left, a.right);
+            const bool ok = pin > 0 && p2 > 0;
+            if ((p.stacking & 1) && ok) sw[k] = weight(p2 / pin, pe);
+            if (p.stacking & 2) {
+                if (!(p.stacking & 1)) sw[k] = w[k];
+                if (ok) sw[k] += weight(p2, pe);
+            }
+        }
+        return sw;
+    };
+    const std::vector<score_t> swA = stack_w(A, bA, wA_, peA), swB = stack_w(B, bB, wB_, peB);
+    am_score_st_.assign(na, 0);
+    am_stackable_.assign(na, 0);
+    for (int k = 0; k < na; ++k) {
+        const ArcMatch &x = am.arcmatch(k);
+        const Arc &a = bA.arc(x.arcA), &b = bB.arc(x.arcB);
+        am_stackable_[k] = A.joint_arc_prob(a.left, a.right) > 0 && B.joint_arc_prob(b.left, b.right) > 0;
+        if (am_stackable_[k]) am_score_st_[k] = ((score_t)p.tau * am_seq_[k]) / 100 + (swA[x.arcA] + swB[x.arcB]);
     }
 }
 

@@ -31,12 +31,15 @@ struct capacity_failure : failure {
 // reference: RnaData, rna_data.hh:37; reader rna_data.cc:984-1124
 class RnaData {
 public:
-    RnaData(const std::string &pp_text, double p_bpcut, long max_bp_span);
+    // stacking: read the joint probabilities of the #STACK column
+    RnaData(const std::string &pp_text, double p_bpcut, long max_bp_span, bool stacking = false);
     int length() const { return (int)seq_.size(); }
     const std::string &sequence() const { return seq_; }  // 0-based chars
     char nt(int i) const { return seq_[i - 1]; }           // 1-based
     const std::string &name() const { return name_; }
     double arc_prob(int i, int j) const;
+    // joint probability of (i, j) and (i+1, j-1); 0 without (rna_data.cc:700-710)
+    double joint_arc_prob(int i, int j) const;
     double prob_paired_upstream(int i) const;    // rna_data.cc:713-722
     double prob_paired_downstream(int i) const;  // rna_data.cc:724-732
     // base pairs kept after filtering, ordered i ascending, j ascending
@@ -44,7 +47,7 @@ public:
 
 private:
     std::string name_, seq_;
-    std::map<std::pair<int, int>, double> probs_;
+    std::map<std::pair<int, int>, double> probs_, joint_;
     std::vector<std::pair<std::pair<int, int>, double>> pairs_;
     double p_bpcut_;
 };
@@ -115,6 +118,7 @@ struct ScoringParams {
     int temperature_alipf = 300;
     bool use_ribosum = true;
     double exp_prob = -1.0;  // <= 0: 1/(2 len)  (aux.hh:216-220)
+    int stacking = 0;        // bit 0 --stacking, bit 1 --new-stacking
 };
 
 // RIBOSUM nucleotide code: 0..3 for A, C, G, U; -1 for any other symbol
@@ -192,6 +196,12 @@ public:
     score_t gapB(int j) const { return gapB_[j]; }
     score_t indel_opening() const { return p_.indel_opening; }
     score_t arcmatch(int am) const { return am_score_[am]; }
+    // --stacking: whether both arcs of the arc match have a joint
+    // probability with their inner arcs (scoring.cc:556-569), and its
+    // stacked score (scoring.cc:440-554 with the stacked weights of
+    // scoring.cc:200-236)
+    bool stackable(int am) const { return !am_stackable_.empty() && am_stackable_[am]; }
+    score_t arcmatch_stacked(int am) const { return am_score_st_[am]; }
     score_t weightA(int arc) const { return wA_[arc]; }
     score_t weightB(int arc) const { return wB_[arc]; }
     const ScoringParams &params() const { return p_; }
@@ -212,7 +222,8 @@ private:
     std::vector<int> codeA_, codeB_;
     std::vector<char> symbols_;
     std::vector<score_t> sigma_codes_;
-    std::vector<score_t> gapA_, gapB_, wA_, wB_, am_score_, am_seq_;
+    std::vector<score_t> gapA_, gapB_, wA_, wB_, am_score_, am_seq_, am_score_st_;
+    std::vector<char> am_stackable_;
 };
 
 }  // namespace locarna_amd

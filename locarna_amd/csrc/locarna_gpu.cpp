@@ -3,14 +3,17 @@
 // reference's command-line programs: src/locarna.cc:727-785 (Aligner:
 // align, trace, get_alignment; "Score: N" and the --clustal text) and, with
 // --pf, src/locarna_p.cc:466-527 (AlignerP: align_inside, align_outside,
-// compute_*_probabilities, the probability files).
+// compute_*_probabilities, the probability files; src/locarna_p.cc:537-561,
+// the FRAGMENT_MATCH_PROB lines).
 //
 //   locarna_gpu A.pp B.pp [--indel=N] [--indel-opening=N] [--struct-weight=N]
 //       [--tau=N] [--min-prob=P] [--maxBPspan=N] [--min-trace-probability=P]
 //       [--sequ-local] [--free-endgaps=XXXX] [--max-diff-am=N] [--max-diff-at-am=N]
-//       [--penalized=P] [--normalized=L] [--clustal] [--write-structure]
+//       [--penalized=P] [--normalized=L] [--noLP] [--stacking] [--new-stacking]
+//       [--exp-prob=P] [--kbest=K] [--better=T] [--clustal] [--write-structure]
 //   locarna_gpu --pf A.pp B.pp [options] [--min-am-prob=P] [--min-bm-prob=P]
 //       [--include-am-in-bm] [--write-arcmatch-probs=FILE] [--write-basematch-probs=FILE]
+//       [--fragment-match-probs="i j k l; ..."]
 //
 // Defaults are the reference's (locarna / locarna_p).  Exit status 0 on
 // success, 1 on a library failure (the message on stderr), 2 on usage.
@@ -18,7 +21,9 @@
 #include <cstdlib>
 #include <fstream>
 #include <iostream>
+#include <sstream>
 #include <string>
+#include <vector>
 
 #include "locarna_amd.hpp"
 
@@ -49,7 +54,10 @@ int main(int argc, char **argv) {
     double min_am_prob = 0.001, min_bm_prob = 0.001;  // src/locarna_p.cc:143-146
     bool penalized = false, normalized = false;
     long long penalty = 0, norm_L = 0;
-    bool trace_given = false;
+    bool trace_given = false, nolp = false, subopt = false, exp_prob_given = false;
+    int kbest = -1;
+    long long better = -1000000;  // src/locarna.cc:203-206
+    std::string fragments;
     for (int a = 1; a < argc; ++a) {
         const std::string arg = argv[a];
         std::string v;
@@ -58,6 +66,12 @@ int main(int argc, char **argv) {
         else if (arg == "--write-structure") structures = true;
         else if (arg == "--sequ-local") local = true;
         else if (arg == "--include-am-in-bm") include_am = true;
+        else if (arg == "--noLP") nolp = true;
+        else if (arg == "--stacking") sp.stacking = true;
+        else if (arg == "--new-stacking") sp.new_stacking = true;
+        else if (opt(arg, "fragment-match-probs", v)) fragments = v;
+        else if (opt(arg, "kbest", v)) subopt = true, kbest = std::atoi(v.c_str());
+        else if (opt(arg, "better", v)) subopt = true, better = std::atoll(v.c_str());
         else if (opt(arg, "indel", v)) sp.indel = std::atoi(v.c_str());
         else if (opt(arg, "indel-opening", v)) sp.indel_opening = std::atoi(v.c_str());
         else if (opt(arg, "struct-weight", v)) sp.struct_weight = std::atoi(v.c_str());
@@ -65,7 +79,7 @@ int main(int argc, char **argv) {
         else if (opt(arg, "match", v)) sp.match = std::atoi(v.c_str());
         else if (opt(arg, "mismatch", v)) sp.mismatch = std::atoi(v.c_str());
         else if (opt(arg, "temperature-alipf", v)) sp.temperature_alipf = std::atoi(v.c_str());
-        else if (opt(arg, "exp-prob", v)) sp.exp_prob = std::atof(v.c_str());
+        else if (opt(arg, "exp-prob", v)) sp.exp_prob = std::atof(v.c_str()), exp_prob_given = true;
         else if (opt(arg, "min-prob", v)) bp.min_prob = std::atof(v.c_str());
         else if (opt(arg, "maxBPspan", v)) bp.max_bp_span = std::atoi(v.c_str());
         else if (opt(arg, "min-trace-probability", v)) bp.min_trace_probability = std::atof(v.c_str()), trace_given = true;
@@ -84,14 +98,38 @@ int main(int argc, char **argv) {
         else return usage();
     }
     if (nfiles != 2) return usage();
+    // src/locarna.cc:407-414
+    if (sp.stacking && !exp_prob_given) {
+        std::cerr << "WARNING: stacking turned off. Stacking requires setting a background probability "
+                     "explicitely (option --exp-prob).\n";
+        sp.stacking = false;
+    }
     try {
         const RnaData A = RnaData::from_file(files[0]), B = RnaData::from_file(files[1]);
         if (!pf) {
             const AlignerParams ap{AlignerParams::seqA(&A), AlignerParams::seqB(&B), AlignerParams::scoring(&sp),
                                    AlignerParams::trace_controller(&bp), AlignerParams::sequ_local(local),
                                    AlignerParams::free_endgaps(free_endgaps), AlignerParams::max_diff_am(max_diff_am),
-                                   AlignerParams::max_diff_at_am(max_diff_at_am)};
+                                   AlignerParams::max_diff_at_am(max_diff_at_am),
+                                   AlignerParams::no_lonely_pairs(nolp),
+                                   AlignerParams::stacking(sp.stacking || sp.new_stacking)};
             Aligner aligner(ap);
+            if (subopt) {
+                // src/locarna.cc:740-746: the enumeration replaces the
+                // single alignment; one block per alignment
+                for (const Alignment &a : aligner.suboptimal(kbest, better)) {
+                    std::cout << "Score: " << a.score() << "\n";
+                    std::cout << "EDGES";
+                    for (const auto &e : a.alignment_edges()) std::cout << " " << e.first << ":" << e.second;
+                    std::cout << "\n";
+                    if (structures) {
+                        std::cout << "STRUCTURE_A " << a.dot_bracket_structureA() << "\n";
+                        std::cout << "STRUCTURE_B " << a.dot_bracket_structureB() << "\n";
+                    }
+                    std::cout << "\n";
+                }
+                return 0;
+            }
             int64_t score;
             if (normalized) score = aligner.normalized_align(norm_L);
             else if (penalized) score = aligner.penalized_align(penalty);
@@ -115,7 +153,8 @@ int main(int argc, char **argv) {
             const AlignerPParams<double> ap{AlignerParams::seqA(&A), AlignerParams::seqB(&B),
                                             AlignerParams::scoring(&sp), AlignerParams::trace_controller(&bp),
                                             AlignerPParams<double>::min_am_prob(min_am_prob),
-                                            AlignerPParams<double>::min_bm_prob(min_bm_prob)};
+                                            AlignerPParams<double>::min_bm_prob(min_bm_prob),
+                                            AlignerParams::no_lonely_pairs(nolp)};
             AlignerP<double> aligner(ap);
             std::cout << "Run inside algorithm.\n";
             const double z = aligner.align_inside();
@@ -132,6 +171,29 @@ int main(int argc, char **argv) {
             if (!bm_file.empty()) {
                 std::ofstream out(bm_file);
                 aligner.write_basematch_probabilities(out);
+            }
+            // fragment_list ::= empty | pos pos pos pos {; pos pos pos pos}
+            std::vector<std::array<int, 4>> frags;
+            {
+                std::istringstream in(fragments);
+                std::string item;
+                while (std::getline(in, item, ';')) {
+                    std::istringstream one(item);
+                    std::array<int, 4> f;
+                    if (!(one >> f[0] >> f[1] >> f[2] >> f[3])) {
+                        if (item.find_first_not_of(" \t") == std::string::npos) continue;
+                        std::cerr << "WARNING: expected 4 positions or end when parsing argument "
+                                     "fragment-match-probs\"" << fragments << "\"\n";
+                        break;
+                    }
+                    frags.push_back(f);
+                }
+            }
+            if (!frags.empty()) {
+                const std::vector<double> p = aligner.compute_fragment_match_probs(frags);
+                for (size_t q = 0; q < frags.size(); ++q)
+                    std::cout << "FRAGMENT_MATCH_PROB " << frags[q][0] << " " << frags[q][1] << " " << frags[q][2]
+                              << " " << frags[q][3] << " : " << p[q] << "\n";
             }
             char buf[64];
             std::snprintf(buf, sizeof buf, "%.17g", z);

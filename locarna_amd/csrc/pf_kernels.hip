@@ -117,7 +117,22 @@ struct PfPair {
     const int *fcand;      // int arena: fill entries (p | q<<16, j | min_col(p-1)<<16, -)
     const int *f2c;        // fill entry -> candidate entry
     double *fD;            // inside D per fill entry
+    double *fq;            // fragment match queries (8 doubles each)
+    int nfq;
 };
+
+// Fragment match queries (A[i..j] with B[k..l]): the reverse sweep of every
+// region holding the cells (i-1, k-1) and (j, l) adds M(i-1, k-1) times the
+// adjoint of the summand s(j, l) (the weight of every completion from (j,
+// l), the block being a summand of s that is not part of E) to the query's
+// out; the block's inside is a separate fill.  P = in out ps / Z.
+__device__ __forceinline__ void frag_query(const PfPair &P, int q, int &i, int &j, int &k, int &l) {
+    const double *x = P.fq + 8 * q;
+    i = (int)x[0];
+    j = (int)x[1];
+    k = (int)x[2];
+    l = (int)x[3];
+}
 
 __device__ __forceinline__ int row_lo(const PfPair &P, int i, int bl) { return max(bl, P.mn[i]); }
 __device__ __forceinline__ int row_hi(const PfPair &P, int i, int br) { return min(br - 1, P.mx[i]); }
@@ -293,6 +308,14 @@ __device__ void pf_reverse(const PfPair &P, const Region &R, bool want_dbar, dou
                 gB_next = 0.0;
             }
             const double sb = g + Fb_next * gB_next * eo;
+            if (act && want_dbar) {
+                for (int q = 0; q < P.nfq; ++q) {
+                    int fi, fj, fk, fl;
+                    frag_query(P, q, fi, fj, fk, fl);
+                    if (fj == i && fl == j && P.fq[8 * q + 4] > 0.0 && R.stored(P, fi - 1, fk - 1))
+                        atomicAdd(&P.fq[8 * q + 6], sb * *R.at(R.M, P, fi - 1, fk - 1));
+                }
+            }
             if (act) {
                 const double F = R.F[idx];
                 // F_j = F_{j-1} gB (1-o) + s_{j-1} gB o ; opening part = F_j - F_{j-1} gB
@@ -549,13 +572,13 @@ __device__ __forceinline__ void pf_fwd_fixed(const PfPair &P, int al, int bl, in
 // candidate entry) and the gap-opening weight (returned); edge receives the
 // match-edge weights.  Same terms as pf_reverse.
 //
-// The sweep (rows ar-1 .. al+1) keeps global memory to loads: once a row is
-// done no later row reads its Mbar, so each computed cell's Mbar is replaced
-// by its sbar there, and the global accumulations (Dbar += sbar M(p-1,q-1)
-// ps per arc candidate, edge += sbar M(i-1,j-1) es per cell) run after the
-// sweep from those values.  Global stores and atomics inside the sweep would
-// hold up every later load of the wave (one in-order counter covers both),
-// i.e. the row chain.  The sweep's Ss/Es loads (FULL) are issued a row ahead.
+// The sweep visits rows ar-1 .. al+1.  Once a row is done no later row reads
+// its Mbar, so each computed cell's Mbar is replaced by its sbar there; the
+// arc terms of the row read sbar from LDS.  The global accumulations (Dbar +=
+// sbar M(p-1,q-1) ps per arc candidate, edge += sbar M(i-1,j-1) es per cell)
+// are issued inside the sweep, next to the values they use (a separate pass
+// after the sweep that re-reads sbar and M measured slower).  The sweep's
+// Ss/Es loads (FULL) are issued a row ahead.
 template <int NC, bool FULL, int RT = 2>
 __device__ __forceinline__ double pf_rev_fixed(const PfPair &P, int al, int bl, int ar, int br, int W, double *base, const double *Es,
                                const double *Ss, double *edge, double *edge2 = nullptr) {
@@ -659,6 +682,11 @@ __device__ __forceinline__ double pf_rev_fixed(const PfPair &P, int al, int bl, 
             // match: s_j uses M(i-1, j-1) es
             const bool dgst = act && j - 1 >= lo_u && j - 1 <= hi_u;
             const double d = dgst ? sb * esrow[cBc[cc]] : 0.0;  // to Mbar(i-1, j-1)
+            if (dgst) {  // match-edge weight sbar M(i-1, j-1) es
+                const double w = d * rowU[j - 1 - lo_u];
+                atomicAdd(&edge[P.coff[i] - mni + j], w);
+                if (edge2) atomicAdd(&edge2[P.coff[i] - mni + j], w);
+            }
             const double d_right = shift_down_d(d, d_in);
             newcarry[cc] = upst ? (act ? eb * gA * eo : 0.0) + d_right : 0.0;
             if (cc == 0) colcarry_next = readlane_d(d, 0);
@@ -672,14 +700,18 @@ __device__ __forceinline__ double pf_rev_fixed(const PfPair &P, int al, int bl, 
         // arc terms: Mbar(p-1, q-1) += sbar D ps
         int t0, t1;
         pf_row_cands<RT>(rt0, rt1, r, t0, t1);
-        auto arc = [&](const PfEnt &e, bool in) {
+        // arc terms: Mbar(p-1, q-1) += sbar D ps and (FULL) Dbar += sbar
+        // M(p-1, q-1) ps of the candidate entry tt
+        auto arc = [&](const PfEnt &e, int tt, bool in) {
             const int p = e.pq & 0xffff, q = e.pq >> 16, j = e.jm & 0xffff;
             const int rl = max(bl, e.jm >> 16);
-            if (in && q > bl && j >= jlo && j <= hi && q - 1 >= rl && e.D != 0.0)
+            if (in && q > bl && j >= jlo && j <= hi && q - 1 >= rl && e.D != 0.0) {
                 atomicAdd(&Mb[(p - 1 - al) * W + (q - 1 - rl)], rowB[j - lo] * e.D * ps);
+                if (FULL) atomicAdd(&P.Dbar[P.f2c[tt]], rowB[j - lo] * Mst[(p - 1 - al) * W + (q - 1 - rl)] * ps);
+            }
         };
-        if (t1 > t0) arc(ce, t0 + lane < t1);
-        for (int t = t0 + 64 + lane; t < t1; t += 64) arc(pf_fetch(P, t, t1), true);
+        if (t1 > t0) arc(ce, t0 + lane, t0 + lane < t1);
+        for (int t = t0 + 64 + lane; t < t1; t += 64) arc(pf_fetch(P, t, t1), t, true);
         ce = ne;
         if (FULL && r > 0) {
 #pragma unroll
@@ -702,33 +734,19 @@ __device__ __forceinline__ double pf_rev_fixed(const PfPair &P, int al, int bl, 
             if (j <= hi0) open_acc += (Mb[j - bl] + carry[c]) * Mst[j - bl];
         }
     }
-    // global accumulations from sbar (Mb of the computed cells) and M
-    for (int i = al + 1; i < ar; ++i) {
-        const int r = i - al - 1;
-        const int mni = P.mn[i];
-        const int lo = max(bl, mni), hi = min(br - 1, P.mx[i]), jlo = max(bl + 1, mni);
-        if (jlo > hi) continue;
-        const int lo_u = lo_of(i - 1), hi_u = min(br - 1, P.mx[i - 1]);
-        const double *rowB = Mb + (size_t)(r + 1) * W;
-        const double *rowU = Mst + (size_t)r * W;
-        const double *esrow = P.es + P.codeA[i] * LCG_NCODES;
-        const int ecell = P.coff[i] - mni;
-        for (int j = jlo + lane; j <= hi; j += 64)
-            if (j - 1 >= lo_u && j - 1 <= hi_u) {
-                const double w = rowB[j - lo] * rowU[j - 1 - lo_u] * esrow[P.codeB[j]];
-                atomicAdd(&edge[ecell + j], w);
-                if (edge2) atomicAdd(&edge2[ecell + j], w);
-            }
-        if (FULL) {
-            int t0, t1;
-            pf_row_cands<RT>(rt0, rt1, r, t0, t1);
-            for (int t = t0 + lane; t < t1; t += 64) {
-                const PfEnt e = pf_fetch(P, t, t1);
-                const int p = e.pq & 0xffff, q = e.pq >> 16, j = e.jm & 0xffff;
-                const int rl = max(bl, e.jm >> 16);
-                if (q > bl && j >= jlo && j <= hi && q - 1 >= rl && e.D != 0.0)
-                    atomicAdd(&P.Dbar[P.f2c[t]], rowB[j - lo] * Mst[(p - 1 - al) * W + (q - 1 - rl)] * ps);
-            }
+    // fragment queries: the computed cells hold sbar now (lanes over queries)
+    if (FULL && P.nfq > 0) {
+        lds_fence();
+        for (int q = lane; q < P.nfq; q += 64) {
+            int fi, fj, fk, fl;
+            frag_query(P, q, fi, fj, fk, fl);
+            if (!(P.fq[8 * q + 4] > 0.0) || fj <= al || fj >= ar || fi - 1 < al || fk - 1 < bl) continue;
+            const int clo = lo_of(fi - 1), chi = min(br - 1, P.mx[fi - 1]);
+            const int jlo = max(bl + 1, P.mn[fj]), jhi = min(br - 1, P.mx[fj]);
+            if (fk - 1 < clo || fk - 1 > chi || fl < jlo || fl > jhi) continue;
+            const double mv = Mst[(size_t)(fi - 1 - al) * W + (fk - 1 - clo)];
+            const double sbv = Mb[(size_t)(fj - al) * W + (fl - lo_of(fj))];
+            atomicAdd(&P.fq[8 * q + 6], mv * sbv);
         }
     }
     return wave_sum(open_acc);
@@ -872,6 +890,8 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
         P.fcand = arena + d.fcand;
         P.f2c = arena + d.f2c;
         P.fD = pfa + f.fD;
+        P.fq = pfa + f.frag;
+        P.nfq = f.nfrag;
         if (K > 1) {
             pair_barrier();
         } else {
@@ -1073,6 +1093,18 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
                 __syncthreads();
             }
             if (K > 1) pair_barrier();
+        }
+        // fragment queries: the inside of each block (the virtual arc match
+        // (i-1, j+1; k-1, l+1), AlignerP::compute_fragment_match_prob,
+        // aligner_p.icc:1440-1479), one wave per query in its global scratch
+        for (int q = member * nwaves + wave; q < P.nfq; q += K * nwaves) {
+            int fi, fj, fk, fl;
+            frag_query(P, q, fi, fj, fk, fl);
+            const int Wq = (int)P.fq[8 * q + 4];
+            if (Wq <= 0) continue;
+            const Region R = region(fi - 1, fk - 1, fj + 1, fl + 1, Wq);
+            pf_fill(P, R);
+            if (lane == 0) P.fq[8 * q + 5] = R.get(R.M, P, fj, fl);
         }
         if (threadIdx.x == 0) {
             if (K > 1)

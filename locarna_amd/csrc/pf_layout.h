@@ -15,6 +15,11 @@
 //   out[4]              Z, expected-openings weight
 //   fD[nam]             inside D per fill entry (the fill list of the int
 //                       arena, for the candidate gathers of the group kernels)
+//   frag[8 cap]         fragment match queries, 8 doubles each: i, j, k, l,
+//                       W (the stored row width of the block's region; 0:
+//                       a corner outside the band, probability 0), in (the
+//                       block's inside), out (the weight of its outside),
+//                       unused
 //
 // Per pair in the int schedule buffer (offsets in ints): per group 4 ints
 // (inside round | wave<<12, inside LDS offset in doubles, outside round |
@@ -29,7 +34,7 @@ typedef struct {
     int32_t scal, out;
     int32_t fD;
     int32_t sched, rin, rout;  // int schedule buffer
-    int32_t pad[2];
+    int32_t frag, nfrag;       // fragment match queries of the run
 } PfDesc;  // 16 ints
 
 typedef struct {

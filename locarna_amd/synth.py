@@ -6,6 +6,7 @@ library's McCaskill-style folding (``lcg_synth_bpp``), written with
 (reference: src/LocARNA/rna_data.cc:984-1124).
 """
 import ctypes
+import random
 
 from . import _lib
 
@@ -39,11 +40,31 @@ def base_pair_probs(seq, max_span, cutoff=BPCUT):
 
 
 def pp_text(name, seq, pairs, bpcut=BPCUT):
+    """#PP 2.0 text; pairs (i, j, p), or (i, j, p, p2) with the joint
+    probability p2 of (i, j) and (i+1, j-1) (then the section carries the
+    #STACK keyword, rna_data.cc:1055-1057)."""
+    stack = any(len(t) > 3 for t in pairs)
     lines = ["#PP 2.0", "", "%s %s" % (name, seq), "", "#END", "",
-             "#SECTION BASEPAIRS", "", "#BPCUT %g" % bpcut, ""]
-    lines += ["%d %d %.8g" % (i, j, p) for (i, j, p) in pairs]
+             "#SECTION BASEPAIRS", "", "#BPCUT %g" % bpcut] + (["#STACK"] if stack else []) + [""]
+    lines += [("%d %d %.8g %.8g" % t) if len(t) > 3 else ("%d %d %.8g" % t[:3]) for t in pairs]
     lines += ["", "#END", ""]
     return "\n".join(lines)
+
+
+def with_stacking(pairs, seed=0):
+    """The pairs (i, j, p) with a joint probability of (i, j) and its inner
+    pair (i+1, j-1) where that pair is listed: a seeded fraction in [0.2, 1]
+    of min(p, p_inner), so that it never exceeds either."""
+    rng = random.Random(seed)
+    prob = {(i, j): p for (i, j, p) in pairs}
+    out = []
+    for (i, j, p) in pairs:
+        q = prob.get((i + 1, j - 1))
+        if q is None:
+            out.append((i, j, p))
+        else:
+            out.append((i, j, p, min(p, q) * rng.uniform(0.2, 1.0)))
+    return out
 
 
 def make_pp(name, seq, max_span, bpcut=BPCUT):

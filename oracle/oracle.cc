@@ -21,6 +21,7 @@
 #include <string>
 #include <thread>
 #include <atomic>
+#include <queue>
 #include <vector>
 
 namespace orc {
@@ -118,12 +119,20 @@ struct RnaData {
     std::string name;
     std::string seq;  // 1-based via seq[i-1]
     std::map<std::pair<int, int>, double> arc_probs;
+    // joint probability of (i, j) and (i+1, j-1) (the #STACK column)
+    std::map<std::pair<int, int>, double> arc_2_probs;
     double p_bpcut;
     int length() const { return (int)seq.size(); }
     double arc_prob(int i, int j) const {
         auto it = arc_probs.find({i, j});
         return it == arc_probs.end() ? 0.0 : it->second;
     }
+    // rna_data.cc:700-710
+    double joint_arc_prob(int i, int j) const {
+        auto it = arc_2_probs.find({i, j});
+        return it == arc_2_probs.end() ? 0.0 : it->second;
+    }
+    double stacked_arc_prob(int i, int j) const { return joint_arc_prob(i, j) / arc_prob(i + 1, j - 1); }
     // rna_data.cc:713-732
     double prob_paired_upstream(int i) const {
         double p = 0.0;
@@ -148,7 +157,7 @@ static bool get_nonempty_line(std::istream &in, std::string &line) {
     return false;
 }
 
-RnaData read_pp(const std::string &text, double min_prob, long long max_bp_span) {
+RnaData read_pp(const std::string &text, double min_prob, long long max_bp_span, bool stacking = false) {
     std::istringstream in(text);
     RnaData r;
     r.p_bpcut = min_prob;
@@ -177,9 +186,11 @@ RnaData read_pp(const std::string &text, double min_prob, long long max_bp_span)
     if (!get_nonempty_line(in, line) || line != "#SECTION BASEPAIRS")
         throw failure("Expected base pair section header.");
     size_t max_span = max_bp_span < 0 ? std::numeric_limits<size_t>::max() : (size_t)max_bp_span;
+    bool contains_stacking = false;
     while (get_nonempty_line(in, line)) {
         if (line[0] == '#') {
             if (line.compare(0, 4, "#END") == 0) break;
+            if (line.compare(0, 6, "#STACK") == 0) contains_stacking = true;
             if (line.compare(0, 6, "#BPCUT") == 0) {
                 std::istringstream ls(line);
                 std::string dummy;
@@ -199,7 +210,13 @@ RnaData read_pp(const std::string &text, double min_prob, long long max_bp_span)
             throw failure("Invalid indices in PP input line \"" + line + "\".");
         if (p <= r.p_bpcut || (j - i + 1) > max_span) continue;  // rna_data.cc:1078
         r.arc_probs[{(int)i, (int)j}] = p;
+        // rna_data.cc:1083-1092: with stacking, a fourth column is the joint
+        // probability of (i, j) and (i+1, j-1)
+        double p2;
+        if (stacking && (ls >> p2) && p2 > r.p_bpcut) r.arc_2_probs[{(int)i, (int)j}] = p2;
     }
+    if (!contains_stacking && !r.arc_2_probs.empty())  // rna_data.cc:1097-1100
+        throw failure("Stacking probabilties found but stack keyword is missing.");
     return r;
 }
 
@@ -524,6 +541,7 @@ struct Scoring {
     const BasePairs &bpsA, &bpsB;
     Mat<sc_t> sigma_tab;
     std::vector<sc_t> gapA_tab, gapB_tab, weightsA, weightsB;
+    std::vector<sc_t> stack_weightsA, stack_weightsB;  // with --stacking / --new-stacking
     static sc_t round2score(double d) { return (sc_t)((d < 0) ? (d - 0.5) : (d + 0.5)); }  // scoring.hh:384-387
     Scoring(const oracle_params &p_, const Ribosum *r, const RnaData &a, const RnaData &b, const BasePairs &ba,
             const BasePairs &bb)
@@ -540,7 +558,10 @@ struct Scoring {
         double peB = p.exp_prob > 0 ? p.exp_prob : 1.0 / (2.0 * lenB);
         weights(A, bpsA, peA, weightsA);
         weights(B, bpsB, peB, weightsB);
+        stack_weights(A, bpsA, peA, weightsA, stack_weightsA);
+        stack_weights(B, bpsB, peB, weightsB, stack_weightsB);
     }
+    bool stacking() const { return p.stacking != 0; }
     sc_t sigma_(int ia, int ib) const {
         char cA = A.seq[ia - 1], cB = B.seq[ib - 1];
         sc_t score = 0;
@@ -559,6 +580,30 @@ struct Scoring {
         for (size_t i = 0; i < bps.arcs.size(); ++i)
             w[i] = probToWeight(r.arc_prob(bps.arcs[i].left, bps.arcs[i].right), pe);
     }
+    // scoring.cc:200-236: the stacked weight of an arc whose inner arc
+    // exists (probability > 0); others keep 0.  Only arc matches of
+    // stackable arcs (joint probability > 0 in both) use these weights, and
+    // a stackable arc's joint probability exceeds the cut, so the logarithm
+    // is finite wherever it is used.
+    void stack_weights(const RnaData &r, const BasePairs &bps, double pe, const std::vector<sc_t> &w,
+                       std::vector<sc_t> &sw) {
+        sw.assign(bps.arcs.size(), 0);
+        const bool old_st = (p.stacking & 1) != 0, new_st = (p.stacking & 2) != 0;
+        for (size_t i = 0; i < bps.arcs.size(); ++i) {
+            const Arc &a = bps.arcs[i];
+            const bool inner = r.arc_prob(a.left + 1, a.right - 1) > 0;
+            const bool joint = r.joint_arc_prob(a.left, a.right) > 0;
+            if (old_st && inner && joint) sw[i] = probToWeight(r.stacked_arc_prob(a.left, a.right), pe);
+            if (new_st) {
+                if (!old_st) sw[i] = w[i];
+                if (inner && joint) sw[i] += probToWeight(r.joint_arc_prob(a.left, a.right), pe);
+            }
+        }
+    }
+    // scoring.cc:556-569
+    bool stackable(const ArcMatch &am) const {
+        return A.joint_arc_prob(am.a.left, am.a.right) > 0 && B.joint_arc_prob(am.b.left, am.b.right) > 0;
+    }
     sc_t riboX_arcmatch_score(const Arc &a, const Arc &b) const {
         char al = A.seq[a.left - 1], ar = A.seq[a.right - 1], bl = B.seq[b.left - 1], br = B.seq[b.right - 1];
         double score = 0;
@@ -573,13 +618,14 @@ struct Scoring {
         if (considered == 0) return 0;
         return round2score(100.0 * score / considered);
     }
-    sc_t arcmatch(const ArcMatch &am) const {
+    sc_t arcmatch(const ArcMatch &am, bool stacked = false) const {
         sc_t seq_contrib = 0;
         if (p.tau != 0) {
             if (rib) seq_contrib = riboX_arcmatch_score(am.a, am.b);
             else seq_contrib = sigma_tab(am.a.left, am.b.left) + sigma_tab(am.a.right, am.b.right);
         }
-        return ((sc_t)p.tau * seq_contrib) / 100 + (weightsA[am.a.idx] + weightsB[am.b.idx]);
+        return ((sc_t)p.tau * seq_contrib) / 100 +
+               (stacked ? stack_weightsA[am.a.idx] + stack_weightsB[am.b.idx] : weightsA[am.a.idx] + weightsB[am.b.idx]);
     }
 };
 
@@ -601,8 +647,8 @@ std::unique_ptr<Inputs> prepare(const char *ppA, const char *ppB, const oracle_p
     auto in = std::make_unique<Inputs>();
     in->p = p;
     if (p.use_ribosum) in->rib.reset(new Ribosum(read_ribosum(rib_path)));
-    in->A = read_pp(ppA, p.min_prob, p.max_bp_span);
-    in->B = read_pp(ppB, p.min_prob, p.max_bp_span);
+    in->A = read_pp(ppA, p.min_prob, p.max_bp_span, p.stacking != 0);
+    in->B = read_pp(ppB, p.min_prob, p.max_bp_span, p.stacking != 0);
     int lenA = in->A.length(), lenB = in->B.length();
     in->band.reset(new Band(lenA, lenB));
     if (p.max_diff >= 0) in->band->constrain_wo_ref((size_t)p.max_diff);  // locarna.cc:568-569
@@ -650,12 +696,17 @@ struct Aligner {
     sc_t F;
     Mat<sc_t> Dmat;
     std::vector<sc_t> am_score;
+    std::vector<sc_t> am_score_st;  // stacked score (--stacking)
+    std::vector<char> am_stackable;
     int max_i = 0, max_j = 0;
     int min_i = 1, min_j = 1;  // aligner.cc:111-112, set where a local trace stops
     // position penalty of the top level (ModifiedScoringView,
     // aligner_impl.hh:197-262; Scoring::modify_by_parameter, scoring.cc:77-90):
     // sigma - 2 lam, gap costs - lam, D(am) - lam (arc lengths of A and B)
     sc_t lam = 0;
+    // AlignerRestriction (aligner_restriction.hh:26-120): the top level
+    // aligns A[rsA..reA] with B[rsB..reB] (the whole sequences by default)
+    int rsA = 1, reA = 0, rsB = 1, reB = 0;
     bool D_done = false;
     std::vector<std::pair<int, int>> edges;
     std::string structA, structB;
@@ -670,8 +721,17 @@ struct Aligner {
         Dmat.resize(bpsA.arcs.size(), bpsB.arcs.size(), NEG_INF);  // aligner.cc:122-123
         am_score.resize(ams.ams.size());
         for (size_t k = 0; k < ams.ams.size(); ++k) am_score[k] = sc.arcmatch(ams.ams[k]);
+        am_score_st.assign(ams.ams.size(), 0);
+        am_stackable.assign(ams.ams.size(), 0);
+        if (sc.stacking())
+            for (size_t k = 0; k < ams.ams.size(); ++k) {
+                am_stackable[k] = sc.stackable(ams.ams[k]);
+                if (am_stackable[k]) am_score_st[k] = sc.arcmatch(ams.ams[k], true);
+            }
         structA.assign(lenA, '.');
         structB.assign(lenB, '.');
+        reA = lenA;
+        reB = lenB;
     }
     sc_t D(const Arc &a, const Arc &b) const { return Dmat(a.idx, b.idx); }
     sc_t &D(const ArcMatch &am) { return Dmat(am.a.idx, am.b.idx); }
@@ -735,10 +795,14 @@ struct Aligner {
     }
 
     // aligner.cc:574-610
+    // with stacking, an arc match whose inner arc match exists and whose
+    // arcs are stackable may take the inner one's D plus its stacked score
     void fill_D_entries(int al, int bl) {
         for (int x : ams.left_lists(al, bl)) {
             const ArcMatch &am = ams.ams[x];
             D(am) = M(am.a.right - 1, am.b.right - 1) + am_score[x];
+            if (sc.stacking() && ams.inner[x] >= 0 && am_stackable[x])
+                D(am) = std::max(D(am), D(ams.ams[ams.inner[x]]) + am_score_st[x]);
         }
     }
 
@@ -750,9 +814,12 @@ struct Aligner {
         for (int x : ams.left_lists(al - 1, bl - 1)) {
             const int y = ams.inner[x];
             if (y < 0) continue;
+            // with stacking only stackable arc matches, with their stacked
+            // score (aligner.cc:624-650)
+            if (sc.stacking() && !am_stackable[x]) continue;
             const ArcMatch &am = ams.ams[x], &in_am = ams.ams[y];
             const sc_t m = M(in_am.a.right - 1, in_am.b.right - 1);
-            D(am) = std::max(m + am_score[y], D(in_am)) + am_score[x];
+            D(am) = std::max(m + am_score[y], D(in_am)) + (sc.stacking() ? am_score_st[x] : am_score[x]);
         }
     }
 
@@ -789,37 +856,42 @@ struct Aligner {
     // strict maximum (no anchors: the rightmost anchor is (0, 0))
     sc_t align_top_level_global() {
         const int fe = in.p.free_endgaps;
-        init_state(0, lenA + 1, 0, lenB + 1, !fe_left_2(fe), !fe_left_1(fe), lam);
-        for (size_t i = 1; i <= (size_t)lenA; i++) {
+        const size_t sA = rsA, eA = reA, sB = rsB, eB = reB;
+        init_state(sA - 1, eA + 1, sB - 1, eB + 1, !fe_left_2(fe), !fe_left_1(fe), lam);
+        for (size_t i = sA; i <= eA; i++) {
             F = NEG_INF;
-            size_t min_col = std::max((size_t)1, tc.min_col[i]);
-            size_t max_col = std::min((size_t)lenB, tc.max_col[i]);
-            for (size_t j = min_col; j <= max_col; j++) M(i, j) = align_noex(0, 0, i, j, lam);
+            size_t min_col = std::max(sB, tc.min_col[i]);
+            size_t max_col = std::min(eB, tc.max_col[i]);
+            for (size_t j = min_col; j <= max_col; j++) M(i, j) = align_noex(sA - 1, sB - 1, i, j, lam);
         }
-        sc_t max_score = M(lenA, lenB);
-        max_i = lenA;
-        max_j = lenB;
+        // the end cell of a proper restriction outside the band: minus
+        // infinity (the reference reads whatever an earlier computation left
+        // in its matrix there); the whole sequences keep the matrix's value
+        const bool whole = sA == 1 && sB == 1 && eA == (size_t)lenA && eB == (size_t)lenB;
+        sc_t max_score = (whole || tc.is_valid(eA, eB)) ? M(eA, eB) : NEG_INF;
+        max_i = (int)eA;
+        max_j = (int)eB;
         if (fe_right_1(fe) || fe_right_2(fe)) {
             max_score = NEG_INF;
-            max_i = 0;
-            max_j = 0;
+            max_i = (int)sA - 1;
+            max_j = (int)sB - 1;
         }
         if (fe_right_2(fe)) {
-            for (size_t i = 1; i <= (size_t)lenA; i++) {
-                if (tc.max_col[i] >= (size_t)lenB && M(i, lenB) > max_score) {
-                    max_score = M(i, lenB);
+            for (size_t i = sA; i <= eA; i++) {
+                if (tc.max_col[i] >= eB && M(i, eB) > max_score) {
+                    max_score = M(i, eB);
                     max_i = (int)i;
-                    max_j = lenB;
+                    max_j = (int)eB;
                 }
             }
         }
         if (fe_right_1(fe)) {
-            size_t min_col = std::max((size_t)1, tc.min_col[lenA]);
-            size_t max_col = std::min((size_t)lenB, tc.max_col[lenA]);
+            size_t min_col = std::max(sB, tc.min_col[eA]);
+            size_t max_col = std::min(eB, tc.max_col[eA]);
             for (size_t j = min_col; j <= max_col; j++) {
-                if (M(lenA, j) > max_score) {
-                    max_score = M(lenA, j);
-                    max_i = lenA;
+                if (M(eA, j) > max_score) {
+                    max_score = M(eA, j);
+                    max_i = (int)eA;
                     max_j = (int)j;
                 }
             }
@@ -829,16 +901,17 @@ struct Aligner {
 
     // aligner.cc:823-880 (no anchors: left anchor (lenA+1,lenB+1), right (0,0))
     sc_t align_top_level_locally() {
+        const size_t sA = rsA, eA = reA, sB = rsB, eB = reB;
         sc_t max_score = 0;
-        max_i = 0;
-        max_j = 0;
-        init_state(0, lenA + 1, 0, lenB + 1, false, false, lam);
-        for (size_t i = 1; i <= (size_t)lenA; i++) {
+        max_i = (int)sA - 1;
+        max_j = (int)sB - 1;
+        init_state(sA - 1, eA + 1, sB - 1, eB + 1, false, false, lam);
+        for (size_t i = sA; i <= eA; i++) {
             F = NEG_INF;
-            size_t min_col = std::max((size_t)1, tc.min_col[i]);
-            size_t max_col = std::min((size_t)lenB, tc.max_col[i]);
+            size_t min_col = std::max(sB, tc.min_col[i]);
+            size_t max_col = std::min(eB, tc.max_col[i]);
             for (size_t j = min_col; j <= max_col; j++) {
-                M(i, j) = align_noex(0, 0, i, j, lam);
+                M(i, j) = align_noex(sA - 1, sB - 1, i, j, lam);
                 M(i, j) = std::max((sc_t)0, M(i, j));
                 if (max_score < M(i, j)) {
                     max_score = M(i, j);
@@ -891,7 +964,24 @@ struct Aligner {
     // ---- traceback: aligner.cc:967-1028, 1082-1229, 1233-1343, 1345-1358
     void append(int a, int b) { edges.push_back({a, b}); }
 
+    // aligner.cc:967-1028: with stacking, an arc match whose D is the inner
+    // one's plus its stacked score traces the inner arc match directly (the
+    // reference compares with the stacked score of any arc match with an
+    // inner one; for arcs that are not stackable that score is undefined
+    // there and never matches, so only stackable ones are compared here)
     void trace_arcmatch(const ArcMatch &am) {
+        const int y = ams.inner[am.idx];
+        if (sc.stacking() && y >= 0 && am_stackable[am.idx] && D(am) == D(ams.ams[y]) + am_score_st[am.idx]) {
+            const ArcMatch &in_am = ams.ams[y];
+            structA[in_am.a.left - 1] = '(';
+            structA[in_am.a.right - 1] = ')';
+            structB[in_am.b.left - 1] = '(';
+            structB[in_am.b.right - 1] = ')';
+            append(in_am.a.left, in_am.b.left);
+            trace_arcmatch(in_am);
+            append(in_am.a.right, in_am.b.right);
+            return;
+        }
         align_in_arcmatch(am.a.left, am.a.right, am.b.left, am.b.right);
         trace_in_arcmatch(am.a.left, am.a.right - 1, am.b.left, am.b.right - 1, false);
     }
@@ -906,7 +996,7 @@ struct Aligner {
         structB[y.b.left - 1] = '(';
         structB[y.b.right - 1] = ')';
         append(y.a.left, y.b.left);
-        if (D(am) == D(y) + am_score[am.idx]) {
+        if (D(am) == D(y) + (sc.stacking() ? am_score_st[am.idx] : am_score[am.idx])) {
             trace_arcmatch_noLP(y);
         } else {
             align_in_arcmatch(y.a.left, y.a.right, y.b.left, y.b.right);
@@ -998,11 +1088,99 @@ struct Aligner {
         trace_noex(al, i, bl, j, tl);
     }
 
-    void trace() {  // alignment_.clear() first (aligner.cc:1352-1357)
+    void trace() {  // alignment_.clear() first (aligner.cc:1345-1358)
         edges.clear();
         structA.assign(lenA, '.');
         structB.assign(lenB, '.');
-        trace_in_arcmatch(0, max_i, 0, max_j, true);
+        trace_in_arcmatch(rsA - 1, max_i, rsB - 1, max_j, true);
+    }
+
+    // Aligner::suboptimal (aligner.cc:1382-1514), not normalized: the best
+    // alignment of a restriction, then the restriction split at the middle
+    // of the traced alignment's extent in the longer sequence (both halves
+    // keep the middle position), both halves queued with their scores; a
+    // max-heap on the score (greater_second, aligner.hh:34-45, compares
+    // with <), k alignments or until the best queued score is at most the
+    // threshold.  Empty alignments are skipped.
+    struct Subopt {
+        sc_t score;
+        std::vector<std::pair<int, int>> edges;
+        std::string structA, structB;
+        int sA, sB, eA, eB;  // its restriction
+    };
+    std::vector<Subopt> suboptimal(int k, sc_t threshold) {
+        struct Restr {
+            int sA, sB, eA, eB;
+        };
+        typedef std::pair<Restr, sc_t> task_t;
+        struct Less {
+            bool operator()(const task_t &a, const task_t &b) const { return a.second < b.second; }
+        };
+        std::vector<Subopt> out;
+        if (!D_done) align_D();
+        D_done = true;
+        auto top = [&](const Restr &r) {
+            rsA = r.sA;
+            rsB = r.sB;
+            reA = r.eA;
+            reB = r.eB;
+            return in.p.sequ_local ? align_top_level_locally() : align_top_level_global();
+        };
+        std::priority_queue<task_t, std::vector<task_t>, Less> tasks;
+        const Restr whole{1, 1, lenA, lenB};
+        tasks.push(task_t(whole, top(whole)));
+        size_t i = 1;
+        while (k < 0 || i <= (size_t)k) {
+            if (tasks.empty()) break;
+            task_t task = tasks.top();
+            tasks.pop();
+            const Restr r = task.first;
+            const sc_t task_score = task.second;
+            // a restriction without an alignment (minus infinity) ends the
+            // enumeration like one below the threshold
+            if (task_score < threshold + 1 || is_neg_inf(task_score)) break;
+            top(r);
+            trace();
+            if (edges.empty()) continue;
+            out.push_back({task_score, edges, structA, structB, r.sA, r.sB, r.eA, r.eB});
+            if (k >= 0 && i == (size_t)k) break;
+            // start / end positions (alignment.cc:170-195)
+            int fa = -1, fb = -1, la = 0, lb = 0;
+            for (auto &e : edges) {
+                if (fa < 0 && e.first > 0) fa = e.first;
+                if (fb < 0 && e.second > 0) fb = e.second;
+                if (e.first > 0) la = e.first;
+                if (e.second > 0) lb = e.second;
+            }
+            if (fa < 0) fa = lenA;
+            if (fb < 0) fb = lenB;
+            Restr r1 = r, r2 = r;
+            if (r.eA - r.sA > r.eB - r.sB) {
+                const int split = (fa + la) / 2;
+                r1.eA = split;
+                r2.sA = split;
+            } else {
+                const int split = (fb + lb) / 2;
+                r1.eB = split;
+                r2.sB = split;
+            }
+            // a restriction left without rows or columns aligns nothing
+            auto score_of = [&](const Restr &x) -> sc_t {
+                if (x.eA < x.sA || x.eB < x.sB || x.sA < 1 || x.sB < 1 || x.eA > lenA || x.eB > lenB)
+                    return in.p.sequ_local ? 0 : NEG_INF;
+                return top(x);
+            };
+            const sc_t s1 = score_of(r1);
+            const sc_t s2 = score_of(r2);
+            tasks.push(task_t(r1, s1));
+            tasks.push(task_t(r2, s2));
+            ++i;
+        }
+        rsA = 1;
+        rsB = 1;
+        reA = lenA;
+        reB = lenB;
+        return out;
     }
 };
 
@@ -1146,9 +1324,23 @@ struct AlignerP {
             }
         }
     }
+    // fragment capture of reverse_group: at the visit of cell (j, l), adds
+    // M(i-1, k-1) of the group times the cell's adjoint to *out
+    struct FragCap {
+        size_t i, j, k, l;
+        double *out;
+    };
+    // cell (x, y) holds a value of the group's fill: its init row and column
+    // (init_M) or a computed cell
+    bool live(const Fwd &f, size_t x, size_t y) const {
+        if (x < f.al || x >= f.ar || y < f.bl || y >= f.br) return false;
+        if (x == f.al) return y <= std::min(f.br - 1, tc.max_col[f.al]);
+        if (y == f.bl) return tc.min_col[x] <= f.bl;
+        return y >= tc.min_col[x] && y <= tc.max_col[x];
+    }
     // Mbar: seeds on entry (indexed like M), consumed
     void reverse_group(const Fwd &f, Mat<double> &Mbar, std::vector<double> *Dbar, Mat<double> *edge,
-                       double *openings) {
+                       double *openings, const FragCap *cap = nullptr) {
         const size_t al = f.al, ar = f.ar, bl = f.bl, br = f.br;
         std::vector<double> Ebar(lenB + 1, 0.0);
         for (size_t i = ar - 1; i > al; --i) {
@@ -1161,6 +1353,8 @@ struct AlignerP {
             }
             for (size_t j = max_col; j >= min_col; --j) {
                 const double g = Mbar(i, j);
+                if (cap && i == cap->j && j == cap->l && live(f, cap->i - 1, cap->k - 1))
+                    *cap->out += g * f.Mv(cap->i - 1, cap->k - 1);
                 // M(i,j) = M(i-1,j-1) esig + E + F + arcs
                 Mbar(i - 1, j - 1) += g * esig(i, j);
                 if (edge) (*edge)(i, j) += g * f.Mv(i - 1, j - 1) * esig(i, j);
@@ -1214,6 +1408,60 @@ struct AlignerP {
             for (size_t k = 0; k < ams.ams.size(); ++k) am_of[ams.ams[k].a.idx * bpsB.arcs.size() + ams.ams[k].b.idx] = (int)k;
         }
         return am_of[(size_t)a * bpsB.arcs.size() + b];
+    }
+
+    // Fragment match probability, AlignerP::compute_fragment_match_prob
+    // (aligner_p.icc:1440-1479): the weight of the structure alignments that
+    // align A[i..j] with B[k..l] as one block, over Z.  The block's own
+    // alignments are the inside of the virtual arc match (i-1, j+1; k-1,
+    // l+1): in = M(j, l) of that region.  The rest: the alignment passes
+    // through the cells (i-1, k-1) and (j, l) of one group (an arc match's
+    // interior or the top level), with no arc match crossing either; its
+    // weight is out = sum over the groups R holding both cells of
+    // M_R(i-1, k-1) times the adjoint of M_R(j, l) at the visit of the cell
+    // in the reverse sweep (the weight of every completion from (j, l) to
+    // the group's seeds), which the reference computes as the outside of
+    // the hole [i..j] x [k..l] (Mprime, virtual_Mprime; aligner_p.icc:
+    // 786-1046).  P = in out pf_scale / Z.  The block starts and ends its
+    // own gap runs: a run through either cell counts as two, as in the
+    // reference's decomposition.  Dbar: the outside pass's adjoints.
+    double fragment(const std::vector<double> &Dbar, size_t i, size_t j, size_t k, size_t l) {
+        if (i < 1 || k < 1 || i > j || k > l || j > (size_t)lenA || l > (size_t)lenB)
+            throw std::runtime_error("fragment positions out of range");
+        double fin = 0.0;
+        {
+            align_inside_arcmatch(i - 1, j + 1, k - 1, l + 1);
+            Fwd v;
+            v.al = i - 1; v.ar = j + 1; v.bl = k - 1; v.br = l + 1;
+            if (j > i - 1 && l > k - 1 && live(v, j, l)) fin = M(j, l);
+        }
+        double out = 0.0;
+        const FragCap cap{i, j, k, l, &out};
+        Fwd f;
+        Mat<double> Mbar;
+        forward_group(0, lenA + 1, 0, lenB + 1, f);
+        Mbar.resize(lenA + 1, lenB + 1, 0.0);
+        Mbar(lenA, lenB) = 1.0;
+        reverse_group(f, Mbar, nullptr, nullptr, nullptr, &cap);
+        for (size_t al = 1; al + 1 <= i; ++al) {
+            size_t min_bl = std::max((size_t)1, tc.min_col[al]);
+            size_t max_bl = std::min((size_t)lenB, tc.max_col[al]);
+            for (size_t bl = min_bl; bl <= max_bl && bl + 1 <= k; ++bl) {
+                const auto &lst = ams.left_lists(al, bl);
+                if (lst.empty()) continue;
+                int max_ar = (int)al, max_br = (int)bl;
+                max_right_ends((int)al, (int)bl, &max_ar, &max_br);
+                if ((size_t)max_ar <= j || (size_t)max_br <= l) continue;
+                forward_group(al, max_ar, bl, max_br, f);
+                Mbar.resize(lenA + 1, lenB + 1, 0.0);
+                for (int x : lst) {
+                    const ArcMatch &am = ams.ams[x];
+                    Mbar(am.a.right - 1, am.b.right - 1) += Dbar[x] * eam[x];
+                }
+                reverse_group(f, Mbar, nullptr, nullptr, nullptr, &cap);
+            }
+        }
+        return fin * out * ps / Z;
     }
 
     // Outside pass; returns Dbar (per am), full match-edge weights, restricted
@@ -1273,6 +1521,7 @@ struct oracle_result {
     std::vector<std::pair<int, int>> edges;
     std::string sA, sB;
     std::vector<long long> D;
+    std::vector<long long> am_st;  // stacked arc-match score, or NEG_INF if not stackable
     std::vector<double> am_prob, bm_prob, bm_full, pfD, pfDprime;
     double openings = 0;
     long long counters[6] = {0, 0, 0, 0, 0, 0};
@@ -1302,6 +1551,9 @@ extern "C" oracle_result *oracle_align(const char *ppA, const char *ppB, const o
         al.counting = false;
         r->D.resize(r->in->am->ams.size());
         for (size_t k = 0; k < r->D.size(); ++k) r->D[k] = al.D(r->in->am->ams[k]);
+        r->am_st.assign(r->D.size(), orc::NEG_INF);
+        for (size_t k = 0; k < r->D.size(); ++k)
+            if (al.am_stackable[k]) r->am_st[k] = al.am_score_st[k];
         if (do_trace) {
             al.trace();
             r->edges = al.edges;
@@ -1415,6 +1667,18 @@ extern "C" int oracle_band(const oracle_result *r, int *mn, int *mx, int cap) {
     }
     return n;
 }
+// stacked arc-match scores (--stacking) in arc-match order; not stackable:
+// stackable[k] = 0
+extern "C" int oracle_stacked_scores(const oracle_result *r, long long *score, int *stackable, int cap) {
+    const int n = (int)r->am_st.size();
+    if (cap < n) return -1;
+    for (int k = 0; k < n; ++k) {
+        stackable[k] = r->am_st[k] != orc::NEG_INF;
+        score[k] = stackable[k] ? r->am_st[k] : 0;
+    }
+    return n;
+}
+
 extern "C" int oracle_arcmatches(const oracle_result *r, int *al, int *ar, int *bl, int *br, long long *score,
                                  long long *D, int cap) {
     const auto &ams = r->in->am->ams;
@@ -1483,6 +1747,63 @@ extern "C" int oracle_align_many(const char **ppA, const char **ppB, int npairs,
     for (int t = 0; t < nthreads; ++t) th.emplace_back(work);
     for (auto &t : th) t.join();
     return failed ? -1 : npairs;
+}
+
+// k-best suboptimal alignments (Aligner::suboptimal): scores[t] and, per
+// alignment, edge_count[t] edges at edge_a / edge_b + t * cap_edges.
+// Returns the number of alignments or -1 (message in err).
+extern "C" int oracle_suboptimal(const char *ppA, const char *ppB, const oracle_params *p, const char *ribosum_path,
+                                 int k, long long threshold, long long *scores, int *restr, int *edge_a,
+                                 int *edge_b, int *edge_count, int cap_alns, int cap_edges, char *err,
+                                 int errlen) {
+    try {
+        auto in = orc::prepare(ppA, ppB, *p, ribosum_path);
+        orc::Aligner al(*in);
+        const auto res = al.suboptimal(k, threshold);
+        const int n = (int)std::min<size_t>(res.size(), (size_t)cap_alns);
+        for (int t = 0; t < n; ++t) {
+            scores[t] = res[t].score;
+            if (restr) {
+                restr[4 * t] = res[t].sA;
+                restr[4 * t + 1] = res[t].eA;
+                restr[4 * t + 2] = res[t].sB;
+                restr[4 * t + 3] = res[t].eB;
+            }
+            const int ne = (int)std::min<size_t>(res[t].edges.size(), (size_t)cap_edges);
+            edge_count[t] = ne;
+            for (int e = 0; e < ne; ++e) {
+                edge_a[(size_t)t * cap_edges + e] = res[t].edges[e].first;
+                edge_b[(size_t)t * cap_edges + e] = res[t].edges[e].second;
+            }
+        }
+        return n;
+    } catch (std::exception &e) {
+        set_err(err, errlen, e.what());
+        return -1;
+    }
+}
+
+// fragment match probabilities of the LocARNA-P problem of r (an
+// oracle_pf result): prob[q] for the fragments ijkl[4q .. 4q+3] = i, j, k, l
+extern "C" int oracle_pf_fragments(oracle_result *r, int nq, const int *ijkl, double *prob, char *err, int errlen) {
+    try {
+        if (!r || nq < 0 || (nq > 0 && (!ijkl || !prob))) throw std::runtime_error("bad arguments");
+        orc::AlignerP ap(*r->in);
+        ap.inside();
+        std::vector<double> Dbar;
+        orc::Mat<double> edge_full, edge_ref;
+        double openings = 0;
+        ap.outside(Dbar, edge_full, edge_ref, openings);
+        for (int q = 0; q < nq; ++q) {
+            const int *x = ijkl + 4 * q;
+            if (x[0] < 1 || x[1] < x[0] || x[2] < 1 || x[3] < x[2]) throw std::runtime_error("fragment positions out of range");
+            prob[q] = ap.fragment(Dbar, (size_t)x[0], (size_t)x[1], (size_t)x[2], (size_t)x[3]);
+        }
+        return nq;
+    } catch (std::exception &e) {
+        set_err(err, errlen, e.what());
+        return -1;
+    }
 }
 
 extern "C" oracle_result *oracle_pf(const char *ppA, const char *ppB, const oracle_params *p,

@@ -83,7 +83,9 @@ typedef struct {
     double min_bm_prob;    /* --min-bm-prob                          */
     int no_lonely_pairs;   /* --noLP (integer path): arc matches only in
                               stacks of two or more                   */
-    int reserved;
+    int stacking;          /* integer path: bit 0 --stacking, bit 1
+                              --new-stacking (stacked arc-match scores from
+                              the #STACK joint probabilities)         */
 } oracle_params;
 
 typedef struct oracle_result oracle_result;
@@ -98,6 +100,21 @@ oracle_result *oracle_align(const char *ppA, const char *ppB,
 oracle_result *oracle_pf(const char *ppA, const char *ppB,
                          const oracle_params *p, const char *ribosum_path,
                          int include_am_in_bm, char *err, int errlen);
+
+/* Fragment match probabilities (AlignerP::compute_fragment_match_prob,
+ * locarna_p --fragment-match-probs) of the problem of an oracle_pf result:
+ * prob[q] for fragments A[i..j], B[k..l] given as ijkl[4q..4q+3].  Returns
+ * nq or -1 (message in err). */
+int oracle_pf_fragments(oracle_result *r, int nq, const int *ijkl, double *prob, char *err, int errlen);
+
+/* k-best suboptimal alignments by interval splitting (Aligner::suboptimal,
+ * locarna --kbest / --better): at most cap_alns alignments; scores[t], the
+ * restriction restr[4t..4t+3] = startA, endA, startB, endB (may be NULL) and
+ * edge_count[t] edges of alignment t at edge_a / edge_b + t * cap_edges
+ * (-1 for a gap).  Returns the number of alignments or -1. */
+int oracle_suboptimal(const char *ppA, const char *ppB, const oracle_params *p, const char *ribosum_path, int k,
+                      long long threshold, long long *scores, int *restr, int *edge_a, int *edge_b,
+                      int *edge_count, int cap_alns, int cap_edges, char *err, int errlen);
 
 /* Penalized (mode 1, arg = position penalty; locarna --penalized) and
  * normalized (mode 2, arg = L, sequence-local; --normalized) alignment with
@@ -124,6 +141,9 @@ int oracle_band(const oracle_result *r, int *min_col, int *max_col, int cap);
  * path) */
 int oracle_arcmatches(const oracle_result *r, int *al, int *ar, int *bl,
                       int *br, long long *score, long long *D, int cap);
+/* stacked arc-match scores (--stacking) in arc-match order, stackable[k] = 0
+ * where the arc match has no stacked score; returns the count */
+int oracle_stacked_scores(const oracle_result *r, long long *score, int *stackable, int cap);
 /* score tables: sigma (lenA+1)*(lenB+1) row major, gapA[lenA+1], gapB[lenB+1] */
 int oracle_tables(const oracle_result *r, long long *sigma, long long *gapA, long long *gapB);
 /* work counters: [N_A, N_B, A, G, C, K] */

@@ -24,7 +24,7 @@ class Params(ctypes.Structure):
         ("min_prob", ctypes.c_double), ("min_trace_prob", ctypes.c_double),
         ("exp_prob", ctypes.c_double), ("pf_scale", ctypes.c_double),
         ("min_am_prob", ctypes.c_double), ("min_bm_prob", ctypes.c_double),
-        ("no_lonely_pairs", ctypes.c_int), ("reserved", ctypes.c_int),
+        ("no_lonely_pairs", ctypes.c_int), ("stacking", ctypes.c_int),
     ]
 
 
@@ -117,6 +117,9 @@ class Result:
         self.arcmatches = [(arrs[0][k], arrs[1][k], arrs[2][k], arrs[3][k]) for k in range(na)]
         self.am_scores = list(sc)[:na]
         self.D = list(D)[:na]
+        st, stk = (ctypes.c_longlong * max(na, 1))(), (ctypes.c_int * max(na, 1))()
+        ns = L.oracle_stacked_scores(self._h, st, stk, max(na, 1))
+        self.am_scores_stacked = [st[k] if stk[k] else None for k in range(max(ns, 0))]
         nb = (self.lenA + 1) * (self.lenB + 1)
         sig = (ctypes.c_longlong * nb)()
         gA = (ctypes.c_longlong * (self.lenA + 1))()
@@ -178,6 +181,54 @@ def pf(ppA, ppB, params=None, include_am_in_bm=False):
     if not h:
         raise RuntimeError("oracle_pf: " + err.value.decode())
     return Result(h)
+
+
+def suboptimal(ppA, ppB, k, threshold=-1000000, params=None, cap=64):
+    """k-best suboptimal alignments (locarna --kbest k --better threshold):
+    list of (score, edges, (startA, endA, startB, endB))."""
+    L = load()
+    params = params or locarna_params()
+    ce = len(ppA) + len(ppB) + 2  # above n + m
+    scores = (ctypes.c_longlong * cap)()
+    ea, eb = (ctypes.c_int * (cap * ce))(), (ctypes.c_int * (cap * ce))()
+    cnt = (ctypes.c_int * cap)()
+    restr = (ctypes.c_int * (4 * cap))()
+    err = ctypes.create_string_buffer(512)
+    L.oracle_suboptimal.restype = ctypes.c_int
+    L.oracle_suboptimal.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Params), ctypes.c_char_p,
+                                    ctypes.c_int, ctypes.c_longlong, ctypes.c_void_p, ctypes.c_void_p,
+                                    ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                    ctypes.c_char_p, ctypes.c_int]
+    n = L.oracle_suboptimal(ppA.encode(), ppB.encode(), ctypes.byref(params), RIBOSUM.encode(), k, threshold,
+                            scores, restr, ea, eb, cnt, cap, ce, err, 512)
+    if n < 0:
+        raise RuntimeError("oracle_suboptimal: " + err.value.decode())
+    return [(scores[t], [(ea[t * ce + e], eb[t * ce + e]) for e in range(cnt[t])],
+             tuple(restr[4 * t + x] for x in range(4))) for t in range(n)]
+
+
+def pf_fragments(ppA, ppB, frags, params=None):
+    """Fragment match probabilities (locarna_p --fragment-match-probs) of
+    the fragments (i, j, k, l): A[i..j] matched with B[k..l]."""
+    L = load()
+    params = params or locarna_p_params()
+    err = ctypes.create_string_buffer(512)
+    h = L.oracle_pf(ppA.encode(), ppB.encode(), ctypes.byref(params), RIBOSUM.encode(), 0, err, 512)
+    if not h:
+        raise RuntimeError("oracle_pf: " + err.value.decode())
+    try:
+        nq = len(frags)
+        q = (ctypes.c_int * max(1, 4 * nq))(*[x for f in frags for x in f])
+        prob = (ctypes.c_double * max(1, nq))()
+        L.oracle_pf_fragments.restype = ctypes.c_int
+        L.oracle_pf_fragments.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                          ctypes.c_char_p, ctypes.c_int]
+        rc = L.oracle_pf_fragments(h, nq, q, prob, err, 512)
+        if rc < 0:
+            raise RuntimeError("oracle_pf_fragments: " + err.value.decode())
+        return list(prob)[:nq]
+    finally:
+        L.oracle_free(ctypes.c_void_p(h))
 
 
 def align_many(pp_pairs, params=None, nthreads=1):

@@ -22,8 +22,13 @@ import math
 
 class Problem:
     def __init__(self, lenA, lenB, sigma, gapA, gapB, alpha, arcmatches, am_score,
-                 min_col, max_col, local=False, free=0, no_lonely_pairs=False):
+                 min_col, max_col, local=False, free=0, no_lonely_pairs=False, am_score_stacked=None):
         self.n, self.m = lenA, lenB
+        # --stacking: the stacked score of each stackable arc match (None
+        # elsewhere); an arc match whose interior is exactly its inner arc
+        # match (al+1, ar-1, bl+1, br-1) scores the better of its two scores
+        # (aligner.cc:574-610)
+        self.am_score_stacked = am_score_stacked
         # --noLP: only structure alignments whose every matched arc pair has
         # its directly inner or directly outer arc pair matched as well
         # (stacks of two or more, aligner.cc:612-657)
@@ -60,6 +65,22 @@ class Problem:
 #                        inner alignment is `inner`
 #   ('pad', i, j)        the group's init run from its origin to (i, j)
 
+def _live(P, r0, c0, i, j):
+    """A cell whose M the group with origin (r0, c0) defines: a band cell,
+    or a cell of its init row (columns up to the row's max_col) or init
+    column (rows while min_col <= c0), which the reference's init_state
+    sets whether or not the band holds them (aligner.cc:264-368).  They
+    differ from band cells only for a restricted top level whose origin the
+    band excludes."""
+    if P.valid(i, j) or (i, j) == (r0, c0):
+        return True
+    if i == r0:
+        return c0 <= j <= P.max_col[r0]
+    if j == c0:
+        return i > r0 and all(P.min_col[x] <= c0 for x in range(r0 + 1, i + 1))
+    return False
+
+
 def group_alignments(P, r0, c0, i, j, top, memo):
     """All structure alignments of the group with origin (r0, c0) ending at
     cell (i, j): list of (score, steps, last_edge_type)."""
@@ -72,30 +93,35 @@ def group_alignments(P, r0, c0, i, j, top, memo):
     if (i, j) == (r0, c0):
         out.append((0, (), None))
     elif i == r0:
-        if all(P.valid(r0, c) for c in range(c0, j + 1)):
+        if all(_live(P, r0, c0, r0, c) for c in range(c0, j + 1)):
             sc = 0 if free_row else P.alpha + sum(P.gapB[c] for c in range(c0 + 1, j + 1))
             out.append((sc, (('pad', i, j),), 'I'))
     elif j == c0:
-        if all(P.valid(r, c0) for r in range(r0, i + 1)):
+        if all(_live(P, r0, c0, r, c0) for r in range(r0, i + 1)):
             sc = 0 if free_col else P.alpha + sum(P.gapA[r] for r in range(r0 + 1, i + 1))
             out.append((sc, (('pad', i, j),), 'D'))
     elif P.valid(i, j):
-        if P.valid(i - 1, j - 1):
+        if _live(P, r0, c0, i - 1, j - 1):
             for sc, st, _ in group_alignments(P, r0, c0, i - 1, j - 1, top, memo):
                 out.append((sc + P.sigma[(i, j)], (('M', i, j),) + st, 'M'))
+        # a gap run starts at any cell with a value and runs through band cells
         k = 1
-        while i - k >= r0 and P.valid(i - k, j):
+        while i - k >= r0 and _live(P, r0, c0, i - k, j):
             cost = P.alpha + sum(P.gapA[r] for r in range(i - k + 1, i + 1))
             for sc, st, last in group_alignments(P, r0, c0, i - k, j, top, memo):
                 if last != 'D':
                     out.append((sc + cost, (('D', k, i, j),) + st, 'D'))
+            if not P.valid(i - k, j):
+                break
             k += 1
         k = 1
-        while j - k >= c0 and P.valid(i, j - k):
+        while j - k >= c0 and _live(P, r0, c0, i, j - k):
             cost = P.alpha + sum(P.gapB[c] for c in range(j - k + 1, j + 1))
             for sc, st, last in group_alignments(P, r0, c0, i, j - k, top, memo):
                 if last != 'I':
                     out.append((sc + cost, (('I', k, i, j),) + st, 'I'))
+            if not P.valid(i, j - k):
+                break
             k += 1
         for rank, (al, bl, kk) in enumerate(P.by_right.get((i, j), [])):
             if al <= r0 or bl <= c0:
@@ -103,8 +129,21 @@ def group_alignments(P, r0, c0, i, j, top, memo):
             inner = group_alignments(P, al, bl, i - 1, j - 1, False, memo)
             prefix = group_alignments(P, r0, c0, al - 1, bl - 1, top, memo)
             for isc, ist, _ in inner:
+                s_am = P.am_score[kk]
+                if P.am_score_stacked is not None and len(ist) == 1 and ist[0][0] == 'A' and \
+                        P.arcmatches[ist[0][2]] == (al + 1, i - 1, bl + 1, j - 1):
+                    st = P.am_score_stacked[kk]
+                    if P.no_lonely_pairs:
+                        # --noLP with stacking (aligner.cc:612-657): an arc
+                        # match around its inner one always scores stacked,
+                        # and only a stackable one can
+                        if st is None:
+                            continue
+                        s_am = st
+                    elif st is not None:
+                        s_am = max(s_am, st)
                 for psc, pst, _ in prefix:
-                    out.append((psc + isc + P.am_score[kk], (('A', rank, kk, ist),) + pst, 'A'))
+                    out.append((psc + isc + s_am, (('A', rank, kk, ist),) + pst, 'A'))
     memo[key] = out
     return out
 
@@ -315,3 +354,99 @@ def optimal_global_edges(P):
     alns = [a for a in group_alignments(P, 0, 0, P.n, P.m, True, {}) if stacked(P, a[1])]
     best = max(sc for sc, _, _ in alns)
     return best, [edges(P, st, 0, 0, True)[0] for sc, st, _ in alns if sc == best]
+
+
+def _group_paths(P, steps, r0, c0, out):
+    """Collects, for the group with origin (r0, c0) and every group nested in
+    it, the cells its path visits in order and the interior cells of its gap
+    runs: out gets (cells, run_interior) per group."""
+    cells = [(r0, c0)]
+    inner_cells = set()
+    for s in reversed(steps):
+        t = s[0]
+        if t == 'pad':
+            i, j = s[1], s[2]
+            if i == r0:
+                path = [(r0, c) for c in range(c0 + 1, j + 1)]
+            else:
+                path = [(r, c0) for r in range(r0 + 1, i + 1)]
+            inner_cells.update(path[:-1])
+            cells.extend(path)
+        elif t == 'M':
+            cells.append((s[1], s[2]))
+        elif t == 'D':
+            k, i, j = s[1], s[2], s[3]
+            path = [(r, j) for r in range(i - k + 1, i + 1)]
+            inner_cells.update(path[:-1])
+            cells.extend(path)
+        elif t == 'I':
+            k, i, j = s[1], s[2], s[3]
+            path = [(i, c) for c in range(j - k + 1, j + 1)]
+            inner_cells.update(path[:-1])
+            cells.extend(path)
+        elif t == 'A':
+            al, ar, bl, br = P.arcmatches[s[2]]
+            cells.append((ar, br))
+            _group_paths(P, s[3], al, bl, out)
+    out.append((cells, inner_cells))
+
+
+def fragment_probabilities(P, T, frags, split=True):
+    """Probabilities that A[i..j] and B[k..l] are matched as one block, for
+    each (i, j, k, l) of frags (AlignerP::compute_fragment_match_prob,
+    aligner_p.icc:1440-1479), by enumeration: the weight of the structure
+    alignments one of whose groups (the top level or an arc match's
+    interior) passes through the cells (i-1, k-1) and (j, l), over Z.  The
+    block starts and ends its own gap runs, as in the reference's
+    decomposition into the block's inside and its outside: a run through
+    either cell (the cell inside the run) counts twice, its opening score is
+    added once more (split=False leaves that out, for a check that the rule
+    is observed)."""
+    alns = group_alignments(P, 0, 0, P.n, P.m, True, {})
+    Z = 0.0
+    hit = [0.0] * len(frags)
+    for sc, st, _ in alns:
+        Z += math.exp(sc / T)
+        groups = []
+        _group_paths(P, st, 0, 0, groups)
+        for cells, run_inner in groups:
+            cs = set(cells)
+            for q, (i, j, k, l) in enumerate(frags):
+                a, b = (i - 1, k - 1), (j, l)
+                if a in cs and b in cs:
+                    splits = ((a in run_inner) + (b in run_inner)) if split else 0
+                    hit[q] += math.exp((sc + splits * P.alpha) / T)
+    return [h / Z for h in hit]
+
+
+def best_restricted(P, sA, eA, sB, eB):
+    """Top level restricted to A[sA..eA] x B[sB..eB] (AlignerRestriction,
+    aligner.cc:736-880 with r_): global, the best alignment from the
+    origin (sA-1, sB-1) to (eA, eB) and its traced edges (the trace's tie
+    rule); sequence-local, alignments that start at any valid cell of the
+    rectangle with score 0 (the origin's row and column are the zero init)
+    and end at any cell, and the optimum's first end cell in row-major
+    order.  Returns (best, edges or None, end cell or None)."""
+    memo = {}
+    if not P.local:
+        alns = group_alignments(P, sA - 1, sB - 1, eA, eB, True, memo)
+        if not alns:
+            return None, None, None
+        best = max(sc for sc, _, _ in alns)
+        chosen = min((st for sc, st, _ in alns if sc == best), key=trace_key)
+        return best, edges(P, chosen, sA - 1, sB - 1, True)[0], (eA, eB)
+    best, end = 0, None
+    for i in range(sA - 1, eA + 1):
+        for j in range(sB - 1, eB + 1):
+            if not P.valid(i, j):
+                continue
+            for r0 in range(sA - 1, i + 1):
+                for c0 in range(sB - 1, j + 1):
+                    if (r0, c0) != (sA - 1, sB - 1) and not P.valid(r0, c0):
+                        continue
+                    if (r0, c0) != (sA - 1, sB - 1) and (r0 == sA - 1 or c0 == sB - 1):
+                        continue  # starts on the zero init row / column are covered by the origin
+                    for sc, st, _ in group_alignments(P, r0, c0, i, j, True, memo):
+                        if sc > best:
+                            best, end = sc, (i, j)
+    return best, None, end

@@ -37,3 +37,10 @@ def test_failures_report_status_and_message(tmp_path):
     # scores beyond the kernels' int32 range: LCG_E_CAPACITY
     r = _run([str(fa), str(fb), "--indel=-2000000"])
     assert r.returncode == 1 and "status -4" in r.stderr
+    # --noLP is an option of the integer path only
+    r = _run(["--pf", str(fa), str(fb), "--noLP"])
+    assert r.returncode == 1 and "no_lonely_pairs" in r.stderr
+    # stacking terms are scored by the integer path only (locarna_p reads
+    # the joint probabilities but scores without them)
+    r = _run(["--pf", str(fa), str(fb), "--new-stacking"])
+    assert r.returncode == 1 and "stacking" in r.stderr

@@ -46,3 +46,40 @@ def tiny_stacked_pair(seed, nA=None, nB=None):
     nA = nA or rng.randint(6, 9)
     nB = nB or rng.randint(6, 9)
     return tiny_stacked_pp(rng, nA, "a"), tiny_stacked_pp(rng, nB, "b")
+
+
+def tiny_stacking_pp(rng, n, name="t"):
+    """A tiny stacked sequence (tiny_stacked_pp's arcs) whose stacked arcs
+    carry joint probabilities with their inner arcs (#STACK), some of them
+    at or below the cut."""
+    seq = "".join(rng.choice("ACGU") for _ in range(n))
+    arcs = set()
+    for _ in range(rng.randint(1, 2)):
+        depth = rng.choice((2, 3)) if n >= 9 else 2
+        i = rng.randint(1, n - 3 - 2 * (depth - 1))
+        j = rng.randint(i + 3 + 2 * (depth - 1), n)
+        for d in range(depth):
+            arcs.add((i + d, j - d))
+    if rng.random() < 0.7:
+        i = rng.randint(1, n - 3)
+        j = rng.randint(i + 3, n)
+        arcs.add((i, j))
+    prob = {a: round(rng.uniform(0.05, 0.95), 4) for a in sorted(arcs)}
+    pairs = []
+    for (i, j) in sorted(arcs):
+        q = prob.get((i + 1, j - 1))
+        if q is not None and rng.random() < 0.85:
+            p2 = round(min(prob[(i, j)], q) * rng.uniform(0.2, 1.0), 4)
+            if rng.random() < 0.1:
+                p2 = 0.0004  # at the cut: not a joint probability
+            pairs.append((i, j, prob[(i, j)], p2))
+        else:
+            pairs.append((i, j, prob[(i, j)]))
+    return pp_text(name, seq, pairs, bpcut=0.0005)
+
+
+def tiny_stacking_pair(seed, nA=None, nB=None):
+    rng = random.Random(seed)
+    nA = nA or rng.randint(6, 9)
+    nB = nB or rng.randint(6, 9)
+    return tiny_stacking_pp(rng, nA, "a"), tiny_stacking_pp(rng, nB, "b")
