@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no lcg_batch_autotune before timing)")
     ap.add_argument("--only", default=None,
-                    choices=["long500_global", "long500_local", "pf16x150", "grad256"],
+                    choices=["long500_global", "long500_local", "pf16x150", "grad256", "progressive64"],
                     help="run only this secondary configuration and print its record (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -324,10 +324,13 @@ P_FP64_TFLOPS = 78.6
 
 def extra_configs(args, api, synth, torch, nthreads, only=None):
     """Secondary figures: the long-pair integer workload (global and
-    sequence-local) and the LocARNA-P workloads (inside + outside +
-    gradient), each with its CPU oracle timed on a sample of its pairs."""
+    sequence-local), mlocarna's progressive stage over the 64 all-pairs
+    sequences, and the LocARNA-P workloads (inside + outside + gradient),
+    each with its CPU oracle timed on a sample of its work."""
     from oracle import oracle
     res = {}
+    if not only or only == "progressive64":
+        res["progressive64"] = progressive_stage(api, synth, oracle)
     for local in (0, 1):
         if only and only != "long500_" + ("local" if local else "global"):
             continue
@@ -430,6 +433,57 @@ def extra_configs(args, api, synth, torch, nthreads, only=None):
         if fit:
             res[wl]["fit"] = fit
     return res
+
+
+def progressive_stage(api, synth, oracle, cpu_budget_s=15.0):
+    """mlocarna after the all-pairs stage: UPGMA guide tree from the 2016
+    scores, then the 63 merges of the progressive alignment, each a
+    profile-profile alignment whose consensus pp feeds the next; merges
+    whose children are done run as one batch.  The CPU oracle runs the
+    merges in tree order for about cpu_budget_s seconds on one core; each
+    of them is checked against the GPU's consensus text."""
+    pps, pairs, span = synth.workload("allpairs64")
+    n = 64
+    texts = [pps[x] for x in range(n)]
+    pa = api.locarna_params(max_bp_span=span)
+    B = api.Batch([api.Pair(texts[a], texts[b], pa) for a, b in pairs])
+    B.run(mode=2)
+    S = [[0] * n for _ in range(n)]
+    for k, (a, b) in enumerate(pairs):
+        S[a][b] = S[b][a] = B.score(k)
+    B.destroy()
+    t0 = time.perf_counter()
+    merges = api.guide_tree_upgma(S)
+    t_tree = time.perf_counter() - t0
+    api.Progressive(texts, merges, pa).destroy()  # warm-up (kernel load, first allocations)
+    t0 = time.perf_counter()
+    R = api.Progressive(texts, merges, pa)
+    wall = time.perf_counter() - t0
+    st = R.stats()
+    po = oracle.locarna_params(max_bp_span=span)
+    text = list(texts)
+    done, ok = 0, 0
+    t0 = time.perf_counter()
+    for t, (a, b) in enumerate(merges):
+        if text[a] is None or text[b] is None:
+            text.append(None)
+            continue
+        r = oracle.align(text[a], text[b], po)
+        text.append(oracle.consensus_pp(text[a], text[b], r.edges, po))
+        done += 1
+        ok += int(R.pp(n + t) == text[-1])
+        if time.perf_counter() - t0 > cpu_budget_s:
+            break
+    dt = time.perf_counter() - t0
+    rows = len([l for l in R.pp(2 * n - 2).split("#END")[0].split("\n")[2:] if l.strip()])
+    R.destroy()
+    return {"sequences": n, "merges": n - 1, "rows_final": rows, "rounds": st["rounds"], "batches": st["batches"],
+            "guide_tree_ms": round(t_tree * 1e3, 3), "wall_ms": round(wall * 1e3, 2),
+            "device_ms": round(st["device_ms"], 3), "prepare_ms": round(st["prepare_ms"], 2),
+            "merges_per_sec": round((n - 1) / wall, 2),
+            "cpu_baseline_merges_per_sec": round(done / dt, 3), "cpu_cores": 1,
+            "cpu_sample": "%d of %d merges in tree order (CPU oracle, one core)" % (done, n - 1),
+            "consensus_bit_exact": "%d/%d" % (ok, done)}
 
 
 if __name__ == "__main__":

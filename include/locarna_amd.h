@@ -101,6 +101,14 @@ int lcg_pair_num_arcs(const lcg_pair *pair, int32_t *num_arcsA, int32_t *num_arc
 /* trace band: min_col[0..lenA], max_col[0..lenA]
  * (TraceController::min_col / max_col, trace_controller.hh) */
 int lcg_pair_band(const lcg_pair *pair, int32_t *min_col, int32_t *max_col, int32_t cap);
+/* score tables of a prepared pair: sigma[(i)(lenB+1) + j] = base-match
+ * score of positions (columns) i, j (Scoring::basematch, scoring.cc:140-198:
+ * the sum-of-pairs average for alignment profiles), gapA[0..lenA] and
+ * gapB[0..lenB] the position-specific gap costs (Scoring::gapA / gapB,
+ * scoring.cc:271-311); rowsA / rowsB the rows of each side (1 for a single
+ * sequence).  Any output may be NULL. */
+int lcg_pair_score_tables(const lcg_pair *pair, int64_t *sigma, int64_t *gapA, int64_t *gapB, int32_t *rowsA,
+                          int32_t *rowsB);
 /* arc matches in the reference's index order (ArcMatches::arcmatch(idx)),
  * with their score (Scoring::arcmatch, scoring.cc:440-554) */
 int lcg_pair_arcmatches(const lcg_pair *pair, int32_t *al, int32_t *ar, int32_t *bl, int32_t *br, int64_t *score,
@@ -333,8 +341,11 @@ int lcg_batch_alignment(const lcg_batch *batch, int32_t k, int32_t *a, int32_t *
  * (src/LocARNA/main_helper.icc:557-580, MultipleAlignment::write,
  * multiple_alignment.cc:994-1085), after a run with traceback: the header
  * "CLUSTAL W --- LocARNA 2.0.1 --- Score: <score or -inf>" that mlocarna
- * reads the pair's score from (src/Utils/mlocarna:3516-3527), then blocks
- * of `width` columns (locarna --width, 120), names padded to 18.  Positions
+ * reads the pair's score from (src/Utils/mlocarna:3516-3527; without the
+ * score when a side is an alignment profile), then the rows of A and of B
+ * ("A." / "B." name prefixes on a name clash, multiple_alignment.cc:
+ * 186-247) in blocks of `width` columns (locarna --width, 120), names
+ * padded to 18.  Positions
  * that a local or free-end-gap alignment leaves out are gap columns
  * (Alignment::alignment_edges, alignment.cc:121-167) unless
  * LCG_CLUSTAL_LOCAL_ONLY (--local-file-output); LCG_CLUSTAL_STRUCTURES adds
@@ -438,6 +449,50 @@ int lcg_batch_pf_set_fragments(lcg_batch *batch, int32_t k, int32_t nq, const in
 /* The fragment match probabilities of pair k from the last lcg_batch_pf_run,
  * in the order they were set; returns their count. */
 int lcg_batch_pf_fragment_probs(const lcg_batch *batch, int32_t k, double *prob, int32_t cap);
+
+/* Consensus pp text of pair k's alignment, after a run with traceback
+ * (locarna --pp, main_helper.icc:617-632; RnaData consensus constructor,
+ * rna_data.cc:75-122, 1475-1570; write_pp, :1242-1339): the rows of both
+ * inputs with the alignment's gaps and the consensus base-pair
+ * probabilities, the input of the next step of mlocarna's progressive
+ * alignment.  Base-pair lines in (i, j) order (the reference writes them
+ * in hash order).  Returns the text length; writes the text and its
+ * terminating 0 into buf when cap exceeds the length. */
+int64_t lcg_batch_write_consensus_pp(const lcg_batch *batch, int32_t k, char *buf, int64_t cap);
+
+/* mlocarna's guide tree (MLocarna::Tree UPGMA on the similarity matrix,
+ * lib/perl/MLocarna/Tree.pm:164-315; mlocarna:2352-2390): scores[i n + j]
+ * the pairwise alignment scores (diagonal ignored), distances max - score,
+ * average linkage, ties to the first pair in the reference's cluster-list
+ * order.  merges[2t], merges[2t+1] receive the two children of merge t
+ * (leaves 0..n-1, merge t is node n + t), in the order the reference's
+ * tree lists them. */
+int lcg_guide_tree_upgma(const int64_t *scores, int32_t n, int32_t *merges);
+
+/* mlocarna's progressive alignment (src/Utils/mlocarna:3660-3707,
+ * call_locarna :3462-3500) along a guide tree: each merge aligns the pp
+ * texts of its two children (leaves: pp[0..n-1]; merge t: the consensus pp
+ * of its result, lcg_batch_write_consensus_pp) with `params`.  Merges
+ * whose children are done run in one round: their pairs are prepared
+ * together (lcg_pairs_prepare with the device band for pairs of single
+ * sequences) and aligned in one batch per sigma kind.  The reference runs
+ * one locarna process per merge; the results are the same. */
+typedef struct lcg_progressive lcg_progressive;
+int lcg_progressive_run(const char *const *pp, int32_t n, const int32_t *merges, const lcg_params *params,
+                        int32_t device, lcg_progressive **out, char *err, int errlen);
+/* pp text of a node (leaf i or merge n + t); returns its length, writes it
+ * with its terminating 0 when cap exceeds the length */
+int64_t lcg_progressive_pp(const lcg_progressive *r, int32_t node, char *buf, int64_t cap);
+/* CLUSTAL text of merge t (locarna --clustal of that step; the last merge is
+ * mlocarna's result alignment) */
+int64_t lcg_progressive_clustal(const lcg_progressive *r, int32_t t, int32_t width, char *buf, int64_t cap);
+/* alignment score of every merge; returns n - 1 */
+int lcg_progressive_scores(const lcg_progressive *r, int64_t *scores, int32_t cap);
+/* rounds, batches, summed device time of the alignment runs, host time of
+ * the pair preparation, wall time of lcg_progressive_run */
+int lcg_progressive_stats(const lcg_progressive *r, int32_t *rounds, int32_t *batches, float *device_ms,
+                          float *prepare_ms, float *wall_ms);
+void lcg_progressive_free(lcg_progressive *r);
 
 /* Synthetic inputs (no counterpart in the reference; they replace
  * RNAfold -p, which the reference calls through ViennaRNA,

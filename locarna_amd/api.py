@@ -90,6 +90,7 @@ def _sig(L):
     L.lcg_pair_num_arcs.argtypes = [c.c_void_p, c.POINTER(c.c_int32), c.POINTER(c.c_int32)]
     L.lcg_pair_band.argtypes = [c.c_void_p, c.c_void_p, c.c_void_p, c.c_int32]
     L.lcg_pair_arcmatches.argtypes = [c.c_void_p] + [c.c_void_p] * 5 + [c.c_int32]
+    L.lcg_pair_score_tables.argtypes = [c.c_void_p] + [c.c_void_p] * 5
     L.lcg_pair_schedule.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_int32, c.c_void_p, c.c_void_p, c.c_int32]
     L.lcg_batch_create.argtypes = [c.POINTER(c.c_void_p), c.c_int32, c.c_int32, c.POINTER(c.c_void_p),
                                    c.c_char_p, c.c_int]
@@ -130,6 +131,19 @@ def _sig(L):
                                            c.POINTER(c.c_float)]
     L.lcg_batch_write_clustal.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_int32, c.c_char_p, c.c_int64]
     L.lcg_batch_write_clustal.restype = c.c_int64
+    L.lcg_batch_write_consensus_pp.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64]
+    L.lcg_batch_write_consensus_pp.restype = c.c_int64
+    L.lcg_guide_tree_upgma.argtypes = [c.c_void_p, c.c_int32, c.c_void_p]
+    L.lcg_progressive_run.argtypes = [c.POINTER(c.c_char_p), c.c_int32, c.c_void_p, c.POINTER(Params), c.c_int32,
+                                      c.POINTER(c.c_void_p), c.c_char_p, c.c_int]
+    L.lcg_progressive_pp.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64]
+    L.lcg_progressive_pp.restype = c.c_int64
+    L.lcg_progressive_clustal.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_char_p, c.c_int64]
+    L.lcg_progressive_clustal.restype = c.c_int64
+    L.lcg_progressive_scores.argtypes = [c.c_void_p, c.c_void_p, c.c_int32]
+    L.lcg_progressive_stats.argtypes = [c.c_void_p] + [c.c_void_p] * 5
+    L.lcg_progressive_free.argtypes = [c.c_void_p]
+    L.lcg_progressive_free.restype = None
     L.lcg_batch_write_am_probs.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64]
     L.lcg_batch_write_am_probs.restype = c.c_int64
     L.lcg_batch_write_bm_probs.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_char_p, c.c_int64]
@@ -235,6 +249,18 @@ class Pair:
         mn, mx = (ctypes.c_int32 * n)(), (ctypes.c_int32 * n)()
         lib().lcg_pair_band(self.h, mn, mx, n)
         return list(mn), list(mx)
+
+    def score_tables(self):
+        """(sigma dict (i, j) -> score, gapA list, gapB list, rowsA, rowsB)."""
+        n, m = self.lenA, self.lenB
+        sig = (ctypes.c_int64 * ((n + 1) * (m + 1)))()
+        gA, gB = (ctypes.c_int64 * (n + 1))(), (ctypes.c_int64 * (m + 1))()
+        rA, rB = ctypes.c_int32(), ctypes.c_int32()
+        rc = lib().lcg_pair_score_tables(self.h, sig, gA, gB, ctypes.byref(rA), ctypes.byref(rB))
+        if rc != 0:
+            raise LcgError("lcg_pair_score_tables (%d)" % rc)
+        return ({(i, j): sig[i * (m + 1) + j] for i in range(1, n + 1) for j in range(1, m + 1)},
+                list(gA), list(gB), rA.value, rB.value)
 
     def arcmatches(self):
         n = max(self.num_arcmatches, 1)
@@ -513,6 +539,10 @@ class Batch:
         """locarna --clustal text of pair k (lcg_batch_write_clustal)."""
         return self._text(lib().lcg_batch_write_clustal, k, width, (1 if local_only else 0) | (2 if structures else 0))
 
+    def consensus_pp(self, k):
+        """locarna --pp text of pair k (lcg_batch_write_consensus_pp)."""
+        return self._text(lib().lcg_batch_write_consensus_pp, k)
+
     def am_probs_text(self, k):
         """locarna_p --write-arcmatch-probs text of pair k."""
         return self._text(lib().lcg_batch_write_am_probs, k)
@@ -604,6 +634,74 @@ class Batch:
     def destroy(self):
         if self.h:
             lib().lcg_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+def guide_tree_upgma(scores):
+    """mlocarna's UPGMA guide tree of a symmetric score matrix (list of
+    lists): list of merges (child0, child1), leaves 0..n-1, merge t = node
+    n + t (lcg_guide_tree_upgma)."""
+    n = len(scores)
+    flat = (ctypes.c_int64 * (n * n))(*[int(scores[i][j]) if i != j else 0 for i in range(n) for j in range(n)])
+    m = (ctypes.c_int32 * max(2 * (n - 1), 1))()
+    rc = lib().lcg_guide_tree_upgma(flat, n, m)
+    if rc != 0:
+        raise LcgError("lcg_guide_tree_upgma (%d)" % rc)
+    return [(m[2 * t], m[2 * t + 1]) for t in range(n - 1)]
+
+
+class Progressive:
+    """mlocarna's progressive alignment along a guide tree on one GPU
+    (lcg_progressive_run): pp(node), clustal(t), scores, stats."""
+
+    def __init__(self, pps, merges, params=None, device=0):
+        params = params or locarna_params()
+        n = len(pps)
+        arr = (ctypes.c_char_p * n)(*[p.encode() for p in pps])
+        flat = (ctypes.c_int32 * max(2 * len(merges), 1))(*[x for m in merges for x in m])
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        self.n = n
+        rc = lib().lcg_progressive_run(arr, n, flat, ctypes.byref(params), device, ctypes.byref(h), err, 1024)
+        if rc != 0:
+            raise LcgError("lcg_progressive_run (%d): %s" % (rc, err.value.decode()))
+        self.h = h
+
+    def _text(self, fn, *args):
+        need = fn(self.h, *args, None, 0)
+        if need < 0:
+            raise LcgError("%s (%d)" % (fn.__name__, need))
+        buf = ctypes.create_string_buffer(need + 1)
+        fn(self.h, *args, buf, need + 1)
+        return buf.value.decode()
+
+    def pp(self, node):
+        return self._text(lib().lcg_progressive_pp, node)
+
+    def clustal(self, t=None, width=120):
+        return self._text(lib().lcg_progressive_clustal, self.n - 2 if t is None else t, width)
+
+    def scores(self):
+        out = (ctypes.c_int64 * max(self.n - 1, 1))()
+        lib().lcg_progressive_scores(self.h, out, max(self.n - 1, 1))
+        return list(out)[:self.n - 1]
+
+    def stats(self):
+        r, b = ctypes.c_int32(), ctypes.c_int32()
+        d, p, w = ctypes.c_float(), ctypes.c_float(), ctypes.c_float()
+        lib().lcg_progressive_stats(self.h, ctypes.byref(r), ctypes.byref(b), ctypes.byref(d), ctypes.byref(p),
+                                    ctypes.byref(w))
+        return dict(rounds=r.value, batches=b.value, device_ms=d.value, prepare_ms=p.value, wall_ms=w.value)
+
+    def destroy(self):
+        if self.h:
+            lib().lcg_progressive_free(self.h)
             self.h = None
 
     def __del__(self):

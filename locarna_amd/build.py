@@ -18,6 +18,7 @@ INCLUDE = os.path.join(ROOT, "include")
 HOST_SOURCES = [
     "synth.cpp",
     "host.cpp",
+    "progressive.cpp",
 ]
 # host code that calls the HIP runtime API (no kernels): compiled by g++
 HIPAPI_SOURCES = [

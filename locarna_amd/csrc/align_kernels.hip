@@ -102,6 +102,7 @@ struct Pair {
     const int *__restrict__ gapA;   // LDS
     const int *__restrict__ gapB;   // LDS
     const int *__restrict__ sig;    // LDS: 8x8 sigma
+    const int *__restrict__ gsig;   // global: (n+1) x (m+1) sigma of a pair of profiles
     const int *__restrict__ cptr;   // global
     int4 *cand;                     // global: (al|bl<<16, j, D, min_col(al-1))
     const int *fbp, *fbk;           // LDS: fill blocks of row i, (p, first entry)
@@ -117,6 +118,19 @@ struct Pair {
     int alpha, alpha_pos;
     int nolp, stacking;
 };
+
+// Base-match score sigma(i, j): by symbol codes from the LDS table, or (TAB,
+// pairs of alignment profiles) from the pair's column-pair table in global
+// memory, row i at gsig + i * (m + 1).  sig_row gives the row, sig_col the
+// index of column j in it.
+template <bool TAB>
+__device__ __forceinline__ const int *sig_row(const Pair &P, int i) {
+    return TAB ? P.gsig + i * (P.m + 1) : P.sig + P.codeA[i] * LCG_NCODES;
+}
+template <bool TAB>
+__device__ __forceinline__ int sig_col(const Pair &P, int j) {
+    return TAB ? j : P.codeB[j];
+}
 
 // Row i's stored column range in the group with origin (al, bl) and
 // column bound br (exclusive): [lo, hi].
@@ -137,7 +151,7 @@ struct TopBest {
 // M row r = i - al at M + r*W, column j at j - row_lo(i)).
 // GLOBAL: the region is global scratch; TOP_LOCAL: the sequence-local top
 // level (values clamped at 0, first maximum tracked).
-template <bool GLOBAL, bool TOP_LOCAL>
+template <bool TAB, bool GLOBAL, bool TOP_LOCAL>
 __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar, int br, int W, int *base, bool global_col,
                            bool global_row, TopBest *best, unsigned long long *counts, int lam = 0) {
     const int lane = lane_id();
@@ -231,7 +245,7 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
             int Fcarry = LCG_NEG;
             int sprev = colopen ? colv : LCG_NEG;  // M(i, jlo - 1)
             const int gA = P.gapA[i] - lam;
-            const int *sigrow = P.sig + P.codeA[i] * LCG_NCODES;
+            const int *sigrow = sig_row<TAB>(P, i);
             for (int j0 = jlo; j0 <= hi; j0 += 64) {
                 const int j = j0 + lane;
                 const bool act = j <= hi;
@@ -243,7 +257,7 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
                                                                     : LCG_NEG;
                     const int e = clampneg(max(Est[j - bl] + gA, up + gA + alpha));
                     Est[j - bl] = e;
-                    s = clampneg(max(max(dg + sigrow[P.codeB[j]] - 2 * lam, e), Srow[j - jlo]));
+                    s = clampneg(max(max(dg + sigrow[sig_col<TAB>(P, j)] - 2 * lam, e), Srow[j - jlo]));
                     if (TOP_LOCAL) s = max(s, 0);
                     gB = P.gapB[j] - lam;
                     ++cnt_cells;
@@ -295,7 +309,7 @@ __device__ __forceinline__ void group_fill(const Pair &P, int al, int bl, int ar
 // end in A, left end p descending): a row's candidates with p > al are one
 // prefix, located from the LDS block table once per group; the first 256
 // entries of the next row are fetched while the current row is computed.
-template <bool GLOBAL, int NC, int RT, bool COUNT>
+template <bool TAB, bool GLOBAL, int NC, int RT, bool COUNT>
 __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, int W, int *base,
                                  unsigned long long *counts) {
     const int lane = lane_id();
@@ -359,7 +373,7 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
             carry += readlane(ps, 63);
             Ec[c] = LCG_NEG;
             gBc[c] = col ? P.gapB[j] : 0;
-            cBc[c] = col ? P.codeB[j] : 0;
+            cBc[c] = col ? sig_col<TAB>(P, j) : 0;
             Pre[c] = wave_scan_add(col ? gBc[c] + P.alpha_pos : 0);
         }
         if (lane == 0) Mst[0] = 0;
@@ -446,7 +460,7 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
         int sprev = Mcol;      // s(i, bl) entering the F chain: M(i, bl)
         int dgin = Mcol_prev;  // M(i-1, bl)
         const int gA = P.gapA[i];
-        const int *sigrow = P.sig + P.codeA[i] * LCG_NCODES;
+        const int *sigrow = sig_row<TAB>(P, i);
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             const int j = bl + 1 + 64 * c + lane;
@@ -500,20 +514,20 @@ __device__ void group_fill_fixed(const Pair &P, int al, int bl, int ar, int br, 
 // its register budget.  COUNT keeps the per-lane work counters of the
 // fixed-column kernel (the counting run); without it they cost two vector
 // adds per row chunk and candidate slot.
-template <bool GLOBAL, bool WIDE = true, bool COUNT = false>
+template <bool TAB, bool GLOBAL, bool WIDE = true, bool COUNT = false>
 __device__ void fill_group(const Pair &P, int al, int bl, int ar, int br, int W, int *base,
                            unsigned long long *counts, bool fixed_cols) {
     const int cols = br - bl - 1, rows = ar - al - 1;
     if (!fixed_cols)
-        group_fill<GLOBAL, false>(P, al, bl, ar, br, W, base, true, true, nullptr, counts);
+        group_fill<TAB, GLOBAL, false>(P, al, bl, ar, br, W, base, true, true, nullptr, counts);
     else if (cols <= 64 && rows <= 128)
-        group_fill_fixed<GLOBAL, 1, 2, COUNT>(P, al, bl, ar, br, W, base, counts);
+        group_fill_fixed<TAB, GLOBAL, 1, 2, COUNT>(P, al, bl, ar, br, W, base, counts);
     else if (cols <= 128 && rows <= 128)
-        group_fill_fixed<GLOBAL, 2, 2, COUNT>(P, al, bl, ar, br, W, base, counts);
+        group_fill_fixed<TAB, GLOBAL, 2, 2, COUNT>(P, al, bl, ar, br, W, base, counts);
     else if (WIDE && cols <= 256 && rows <= 256)
-        group_fill_fixed<GLOBAL, 4, 4, COUNT>(P, al, bl, ar, br, W, base, counts);
+        group_fill_fixed<TAB, GLOBAL, 4, 4, COUNT>(P, al, bl, ar, br, W, base, counts);
     else
-        group_fill<GLOBAL, false>(P, al, bl, ar, br, W, base, true, true, nullptr, counts);
+        group_fill<TAB, GLOBAL, false>(P, al, bl, ar, br, W, base, true, true, nullptr, counts);
 }
 
 // stored value of cell (i, j) of the group (origin al, bl; bound br)
@@ -537,7 +551,7 @@ __device__ __forceinline__ void emit(const Pair &P, int a, int b, int slot) { P.
 // trace_in_arcmatch / trace_noex, state E_NO_NO).  Mst is the group's
 // matrix.  Arc matches taken are appended to the work list (tstack, counter
 // *tcount in LDS).
-template <bool GLOBAL>
+template <bool TAB, bool GLOBAL>
 __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W, int al, int bl, int br, int i, int j, bool top,
                             bool local, int free_endgaps, int *tstack, int *tcount, int lam = 0) {
     const int lane = lane_id();
@@ -561,7 +575,7 @@ __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W
         }
         // match
         if (valid(P, i - 1, j - 1) &&
-            Mij == Mget(P, Mst, W, al, bl, br, i - 1, j - 1) + P.sig[P.codeA[i] * LCG_NCODES + P.codeB[j]] - 2 * lam) {
+            Mij == Mget(P, Mst, W, al, bl, br, i - 1, j - 1) + sig_row<TAB>(P, i)[sig_col<TAB>(P, j)] - 2 * lam) {
             if (lane == 0) emit(P, i, j, i + j);
             --i;
             --j;
@@ -730,11 +744,11 @@ __device__ __forceinline__ int region_width(const Pair &P, int al, int bl, int a
     return w;
 }
 
-template <bool GLOBAL>
+template <bool TAB, bool GLOBAL>
 __device__ __forceinline__ void fill_and_trace(const Pair &P, int al, int bl, int ar, int br, int W, int *base, int *tstack,
                                int *tcount) {
-    fill_group<GLOBAL, false>(P, al, bl, ar, br, W, base, nullptr, true);
-    group_trace<GLOBAL>(P, base + (br - bl) + W, W, al, bl, br, ar - 1, br - 1, false, false, 0, tstack, tcount);
+    fill_group<TAB, GLOBAL, false>(P, al, bl, ar, br, W, base, nullptr, true);
+    group_trace<TAB, GLOBAL>(P, base + (br - bl) + W, W, al, bl, br, ar - 1, br - 1, false, false, 0, tstack, tcount);
 }
 
 }  // namespace
@@ -763,7 +777,8 @@ __device__ __forceinline__ Pair load_pair(const PairDesc &d, int *arena, int *ld
         coff[t] = arena[d.band + 2 * (n + 1) + t];
         fbp[t] = arena[d.fptr + t];
     }
-    for (int t = threadIdx.x; t < LCG_NCODES * LCG_NCODES; t += blockDim.x) sig[t] = arena[d.sigma + t];
+    if (!bp.sigma_table)
+        for (int t = threadIdx.x; t < LCG_NCODES * LCG_NCODES; t += blockDim.x) sig[t] = arena[d.sigma + t];
     for (int t = threadIdx.x; t < 2 * d.nfblk; t += blockDim.x) fbk[t] = arena[d.fblk + t];
     Pair P;
     P.mn = mn;
@@ -774,6 +789,7 @@ __device__ __forceinline__ Pair load_pair(const PairDesc &d, int *arena, int *ld
     P.gapA = gapA;
     P.gapB = gapB;
     P.sig = sig;
+    P.gsig = arena + d.sigma;
     P.cptr = arena + d.cptr;
     P.cand = reinterpret_cast<int4 *>(arena + d.cand);
     P.fbp = fbp;
@@ -805,9 +821,9 @@ __device__ __forceinline__ Pair load_pair(const PairDesc &d, int *arena, int *ld
 // Group schedule word G[5] = round | wave << 12 | member << 16.
 #define LCG_SPIN_LIMIT (1 << 22)
 
-extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc *__restrict__ descs,
-                                                                  const int *__restrict__ order, int *queue,
-                                                                  int *arena, int *barriers, BatchParams bp) {
+template <bool TAB>
+__device__ __forceinline__ void fill_pairs(const PairDesc *__restrict__ descs, const int *__restrict__ order,
+                                           int *queue, int *arena, int *barriers, BatchParams bp) {
     extern __shared__ int lds[];
     __shared__ int s_pair;
     const int wave = threadIdx.x >> 6;
@@ -872,9 +888,9 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc
                     const int W = G[2];
                     int *base = pool + G[6];
                     if (counts)
-                        fill_group<false, true, true>(P, gal, gbl, gar, gbr, W, base, counts, bp.fixed_cols != 0);
+                        fill_group<TAB, false, true, true>(P, gal, gbl, gar, gbr, W, base, counts, bp.fixed_cols != 0);
                     else
-                        fill_group<false, true, false>(P, gal, gbl, gar, gbr, W, base, nullptr, bp.fixed_cols != 0);
+                        fill_group<TAB, false, true, false>(P, gal, gbl, gar, gbr, W, base, nullptr, bp.fixed_cols != 0);
                     const int *Mst = base + (gbr - gbl) + W;
                     for (int t = G[3] + lane; t < G[4]; t += 64) {
                         const int4 ga = gams[t];
@@ -933,17 +949,29 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc
     }
 }
 
+// One instance per sigma kind: symbol codes, and the column-pair tables of
+// pairs of alignment profiles.
+extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc *__restrict__ descs,
+                                                                  const int *__restrict__ order, int *queue,
+                                                                  int *arena, int *barriers, BatchParams bp) {
+    fill_pairs<false>(descs, order, queue, arena, barriers, bp);
+}
+extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs_tab(const PairDesc *__restrict__ descs,
+                                                                      const int *__restrict__ order, int *queue,
+                                                                      int *arena, int *barriers, BatchParams bp) {
+    fill_pairs<true>(descs, order, queue, arena, barriers, bp);
+}
+
 extern "C" __global__ void lcg_reset_barriers(int *barriers, int n) {
     for (int t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) barriers[t] = 0;
 }
 
 // Kernel 2: top level (global or sequence-local) and traceback of every
 // pair, after kernel 1 has filled D.
-extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc *__restrict__ descs,
-                                                                 const int *__restrict__ order, int *queue,
-                                                                 int *arena, int *scratch, BatchParams bp,
-                                                                 const int *__restrict__ lambda,
-                                                                 const int4 *__restrict__ restr) {
+template <bool TAB>
+__device__ __forceinline__ void top_pairs(const PairDesc *__restrict__ descs, const int *__restrict__ order,
+                                          int *queue, int *arena, int *scratch, BatchParams bp,
+                                          const int *__restrict__ lambda, const int4 *__restrict__ restr) {
     extern __shared__ int lds[];
     __shared__ int s_pair, s_tcount;
     const int nwaves = blockDim.x >> 6;
@@ -995,17 +1023,17 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
             const int fe = bp.free_endgaps;
             if (bp.local) {
                 if (top_in_lds)
-                    group_fill<false, true>(P, tal, tbl, tar, tbr, tW, topbase, false, false, &best, counts, lam);
+                    group_fill<TAB, false, true>(P, tal, tbl, tar, tbr, tW, topbase, false, false, &best, counts, lam);
                 else
-                    group_fill<true, true>(P, tal, tbl, tar, tbr, tW, topbase, false, false, &best, counts, lam);
+                    group_fill<TAB, true, true>(P, tal, tbl, tar, tbr, tW, topbase, false, false, &best, counts, lam);
             } else {
                 // free end gaps: column 0 free with the second sequence's
                 // left end, row 0 with the first's (aligner.cc:736-745)
                 const bool gcol = !(fe & 4), grow = !(fe & 1);
                 if (top_in_lds)
-                    group_fill<false, false>(P, tal, tbl, tar, tbr, tW, topbase, gcol, grow, nullptr, counts, lam);
+                    group_fill<TAB, false, false>(P, tal, tbl, tar, tbr, tW, topbase, gcol, grow, nullptr, counts, lam);
                 else
-                    group_fill<true, false>(P, tal, tbl, tar, tbr, tW, topbase, gcol, grow, nullptr, counts, lam);
+                    group_fill<TAB, true, false>(P, tal, tbl, tar, tbr, tW, topbase, gcol, grow, nullptr, counts, lam);
                 if (top_in_lds)
                     region_sync<false>();
                 else
@@ -1060,10 +1088,10 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
             // ---- trace the top level
             if (bp.trace) {
                 if (top_in_lds)
-                    group_trace<false>(P, topM, tW, tal, tbl, tbr, best.i, best.j, true, bp.local != 0,
+                    group_trace<TAB, false>(P, topM, tW, tal, tbl, tbr, best.i, best.j, true, bp.local != 0,
                                        bp.free_endgaps, arena + d.tstack, &s_tcount, lam);
                 else
-                    group_trace<true>(P, topM, tW, tal, tbl, tbr, best.i, best.j, true, bp.local != 0,
+                    group_trace<TAB, true>(P, topM, tW, tal, tbl, tbr, best.i, best.j, true, bp.local != 0,
                                       bp.free_endgaps, arena + d.tstack, &s_tcount, lam);
             }
         }
@@ -1086,7 +1114,7 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
                     const int ar = ts[1] & 0xffff, br = ts[1] >> 16;
                     const int W = region_width(P, al, bl, ar, br);
                     if ((br - bl) + W + (ar - al) * W <= slot)
-                        fill_and_trace<false>(P, al, bl, ar, br, W, pool + wave * slot, arena + d.tstack, &s_tcount);
+                        fill_and_trace<TAB, false>(P, al, bl, ar, br, W, pool + wave * slot, arena + d.tstack, &s_tcount);
                 }
                 __threadfence_block();
                 __syncthreads();
@@ -1097,7 +1125,7 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
                         const int ar = ts[1] & 0xffff, br = ts[1] >> 16;
                         const int W = region_width(P, al, bl, ar, br);
                         if ((br - bl) + W + (ar - al) * W > slot)
-                            fill_and_trace<false>(P, al, bl, ar, br, W, pool, arena + d.tstack, &s_tcount);
+                            fill_and_trace<TAB, false>(P, al, bl, ar, br, W, pool, arena + d.tstack, &s_tcount);
                     }
                 }
                 __threadfence_block();
@@ -1109,6 +1137,21 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
         }
         __syncthreads();
     }
+}
+
+extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc *__restrict__ descs,
+                                                                 const int *__restrict__ order, int *queue,
+                                                                 int *arena, int *scratch, BatchParams bp,
+                                                                 const int *__restrict__ lambda,
+                                                                 const int4 *__restrict__ restr) {
+    top_pairs<false>(descs, order, queue, arena, scratch, bp, lambda, restr);
+}
+extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs_tab(const PairDesc *__restrict__ descs,
+                                                                     const int *__restrict__ order, int *queue,
+                                                                     int *arena, int *scratch, BatchParams bp,
+                                                                     const int *__restrict__ lambda,
+                                                                     const int4 *__restrict__ restr) {
+    top_pairs<true>(descs, order, queue, arena, scratch, bp, lambda, restr);
 }
 
 // Copies the outputs of every pair (edges, structure strings, traceback
@@ -1172,12 +1215,12 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
                                 hipEvent_t mid, hipStream_t stream, const int4 *restr) {
     static bool attr_set = false;
     if (!attr_set) {
-        hipError_t e = hipFuncSetAttribute((const void *)lcg_fill_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           160 * 1024 - 256);
-        if (e == hipSuccess)
-            e = hipFuncSetAttribute((const void *)lcg_top_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    160 * 1024 - 256);
-        if (e != hipSuccess) return (int)e;
+        const void *kernels[] = {(const void *)lcg_fill_pairs, (const void *)lcg_top_pairs,
+                                 (const void *)lcg_fill_pairs_tab, (const void *)lcg_top_pairs_tab};
+        for (const void *k : kernels) {
+            hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
+            if (e != hipSuccess) return (int)e;
+        }
         attr_set = true;
     }
     // two work counters, 64 bytes apart; the per-pair barrier counters of
@@ -1187,11 +1230,19 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
         hipLaunchKernelGGL(lcg_reset_queue, dim3(1), dim3(64), 0, stream, queue);
         if (bp.wg_per_pair > 1)
             hipLaunchKernelGGL(lcg_reset_barriers, dim3(1), dim3(256), 0, stream, barriers, bp.npairs);
-        hipLaunchKernelGGL(lcg_fill_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue,
-                           arena, barriers, bp);
+        if (bp.sigma_table)
+            hipLaunchKernelGGL(lcg_fill_pairs_tab, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order,
+                               queue, arena, barriers, bp);
+        else
+            hipLaunchKernelGGL(lcg_fill_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue,
+                               arena, barriers, bp);
     }
     if (mid) hipEventRecord(mid, stream);
-    hipLaunchKernelGGL(lcg_top_pairs, dim3(top_grid), dim3(top_waves * 64), top_lds_bytes, stream, descs, order,
-                       queue + 16, arena, scratch, bp_top, lambda, restr);
+    if (bp_top.sigma_table)
+        hipLaunchKernelGGL(lcg_top_pairs_tab, dim3(top_grid), dim3(top_waves * 64), top_lds_bytes, stream, descs,
+                           order, queue + 16, arena, scratch, bp_top, lambda, restr);
+    else
+        hipLaunchKernelGGL(lcg_top_pairs, dim3(top_grid), dim3(top_waves * 64), top_lds_bytes, stream, descs, order,
+                           queue + 16, arena, scratch, bp_top, lambda, restr);
     return (int)hipGetLastError();
 }

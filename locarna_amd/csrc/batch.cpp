@@ -47,7 +47,7 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
 struct lcg_pair {
     lcg_params params;
     ScoringParams sp;
-    std::unique_ptr<RnaData> A, B;
+    std::shared_ptr<const RnaData> A, B;  // shared with the batches made of the pair (output writers)
     std::unique_ptr<BasePairs> bA, bB;
     std::unique_ptr<TraceController> tc;
     std::unique_ptr<ArcMatches> am;
@@ -85,7 +85,7 @@ int32_t reserve(std::vector<int32_t> &blob, size_t count, int32_t fill = 0) {
 // order the host packing reserves them; the device compaction lays out its
 // pairs with it, so both produce the same image.  Returns the image size.
 size_t layout_pair(PairDesc &d, int n, int m, int ncells, int ngroups, int nam, int nfblk, bool nolp = false,
-                   bool stacking = false) {
+                   bool stacking = false, bool table = false) {
     size_t off = 0;
     auto take = [&](size_t count) {
         const int32_t o = (int32_t)off;
@@ -96,7 +96,7 @@ size_t layout_pair(PairDesc &d, int n, int m, int ncells, int ngroups, int nam, 
     d.band = take(3 * (size_t)(n + 1) + 1);
     d.codes = take((size_t)n + m + 2);
     d.gaps = take((size_t)n + m + 2);
-    d.sigma = take(LCG_NCODES * LCG_NCODES);
+    d.sigma = take(table ? (size_t)(n + 1) * (m + 1) : LCG_NCODES * LCG_NCODES);
     d.rows = take((size_t)n + 2);
     d.rounds = take((size_t)n + 2);
     d.groups = take((size_t)LCG_GROUP_INTS * ngroups);
@@ -199,9 +199,18 @@ void pack(lcg_pair &P) {
     d.gaps = reserve(blob, (size_t)n + m + 2);
     for (int i = 1; i <= n; ++i) blob[d.gaps + i] = (int32_t)sc.gapA(i);
     for (int j = 1; j <= m; ++j) blob[d.gaps + n + 1 + j] = (int32_t)sc.gapB(j);
-    d.sigma = reserve(blob, LCG_NCODES * LCG_NCODES);
-    for (int a = 0; a < LCG_NCODES; ++a)
-        for (int b = 0; b < LCG_NCODES; ++b) blob[d.sigma + a * LCG_NCODES + b] = (int32_t)sc.sigma_code(a, b);
+    // sigma: by symbol codes, or for profiles by column pair (row i at
+    // sigma + i * (m + 1))
+    const bool table = sc.profile();
+    if (table) {
+        d.sigma = reserve(blob, (size_t)(n + 1) * (m + 1));
+        const std::vector<score_t> &st = sc.sigma_table();
+        for (size_t x = 0; x < st.size(); ++x) blob[d.sigma + x] = (int32_t)st[x];
+    } else {
+        d.sigma = reserve(blob, LCG_NCODES * LCG_NCODES);
+        for (int a = 0; a < LCG_NCODES; ++a)
+            for (int b = 0; b < LCG_NCODES; ++b) blob[d.sigma + a * LCG_NCODES + b] = (int32_t)sc.sigma_code(a, b);
+    }
 
     // arc matches by left ends (groups) and by right ends (candidates)
     std::vector<std::vector<int>> by_left((size_t)(n + 1) * (m + 1));
@@ -432,7 +441,7 @@ void pack(lcg_pair &P) {
     d.gmax = gbuf;
     {
         PairDesc L = d;
-        const size_t size = layout_pair(L, n, m, ncells, d.ngroups, nam, d.nfblk, nolp, stacking);
+        const size_t size = layout_pair(L, n, m, ncells, d.ngroups, nam, d.nfblk, nolp, stacking, table);
         if (size != blob.size() || std::memcmp(&L, &d, sizeof d) != 0) throw failure("internal: arena layout");
     }
     P.desc = d;
@@ -596,8 +605,7 @@ int64_t score_bound(const lcg_pair &P) {
     const Scoring &sc = *P.sc;
     const int64_t n = P.A->length(), m = P.B->length();
     int64_t smax = 0, gmax = 0, amax = 0;
-    for (int a = 0; a < Scoring::NCODES; ++a)
-        for (int b = 0; b < Scoring::NCODES; ++b) smax = std::max<int64_t>(smax, std::llabs(sc.sigma_code(a, b)));
+    smax = sc.max_abs_sigma();
     for (int i = 1; i <= n; ++i) gmax = std::max<int64_t>(gmax, std::llabs(sc.gapA(i)));
     for (int j = 1; j <= m; ++j) gmax = std::max<int64_t>(gmax, std::llabs(sc.gapB(j)));
     if (P.deferred) {
@@ -634,6 +642,8 @@ void prepare_finish(lcg_pair &P, bool deferred) {
     P.bA.reset(new BasePairs(P.A.get(), p->min_prob));
     P.bB.reset(new BasePairs(P.B.get(), p->min_prob));
     if (deferred) {
+        if (P.A->num_rows() > 1 || P.B->num_rows() > 1)  // the compaction reads symbol codes
+            throw failure("alignment profiles need the arc matches enumerated on the host");
         P.am.reset(new ArcMatches(*P.bA, *P.bB));
         P.sc.reset(new Scoring(*P.A, *P.B, *P.am, P.sp));
         P.deferred = true;
@@ -786,6 +796,23 @@ extern "C" int lcg_pair_band(const lcg_pair *p, int32_t *mn, int32_t *mx, int32_
     return p->desc.n + 1;
 }
 
+extern "C" int lcg_pair_score_tables(const lcg_pair *p, int64_t *sigma, int64_t *gapA, int64_t *gapB,
+                                     int32_t *rowsA, int32_t *rowsB) {
+    if (!p || !p->sc) return LCG_E_ARG;
+    const int n = p->A->length(), m = p->B->length();
+    if (sigma)
+        for (int i = 0; i <= n; ++i)
+            for (int j = 0; j <= m; ++j)
+                sigma[(size_t)i * (m + 1) + j] = (i == 0 || j == 0) ? 0 : p->sc->basematch(i, j);
+    if (gapA)
+        for (int i = 0; i <= n; ++i) gapA[i] = i ? p->sc->gapA(i) : 0;
+    if (gapB)
+        for (int j = 0; j <= m; ++j) gapB[j] = j ? p->sc->gapB(j) : 0;
+    if (rowsA) *rowsA = p->A->num_rows();
+    if (rowsB) *rowsB = p->B->num_rows();
+    return LCG_OK;
+}
+
 extern "C" int lcg_pair_arcmatches(const lcg_pair *p, int32_t *al, int32_t *ar, int32_t *bl, int32_t *br,
                                    int64_t *score, int32_t cap) {
     if (!p) return LCG_E_ARG;
@@ -895,6 +922,10 @@ struct lcg_batch {
     bool norm_valid = false;
     // names and sequences of every pair, for the alignment text
     std::vector<std::string> nameA, nameB, seqA, seqB;
+    // inputs of every pair for the alignment and consensus writers (rows of
+    // profiles, base-pair probabilities) and their expected probabilities
+    std::vector<std::shared_ptr<const RnaData>> rdA, rdB;
+    std::vector<double> pexpA, pexpB;
     std::vector<int64_t> score_bound;  // per pair, see score_bound()
     std::vector<PfDesc> pdescs;
     std::vector<double> pf_arena;
@@ -1361,6 +1392,12 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
             set_err(err, errlen, "a batch takes either host-packed or device-compacted pairs, not both");
             return LCG_E_ARG;
         }
+        // the kernels read sigma either by symbol codes or, for profiles,
+        // from a column-pair table: one kind per batch
+        if (P->sc->profile() != pairs[0]->sc->profile()) {
+            set_err(err, errlen, "a batch takes either pairs of alignment profiles or pairs of single sequences");
+            return LCG_E_ARG;
+        }
     }
     hipError_t e;
     int cus = 0;
@@ -1439,6 +1476,7 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
     B->bp.free_endgaps = p0.free_endgaps;
     B->bp.nolp = p0.no_lonely_pairs != 0;
     B->bp.stacking = p0.stacking != 0;
+    B->bp.sigma_table = pairs[0]->sc->profile() ? 1 : 0;
     B->bp.trace = 1;
     B->bp.npairs = npairs;
     B->bp.hdr_ints = B->hdr;
@@ -1492,6 +1530,11 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
         for (int j = 1; j <= X.B->length(); ++j) b += X.B->nt(j);
         B->seqA.push_back(a);
         B->seqB.push_back(b);
+        B->rdA.push_back(X.A);
+        B->rdB.push_back(X.B);
+        // locarna.cc:662-663
+        B->pexpA.push_back(X.params.exp_prob > 0 ? X.params.exp_prob : 1.0 / (2.0 * X.A->length()));
+        B->pexpB.push_back(X.params.exp_prob > 0 ? X.params.exp_prob : 1.0 / (2.0 * X.B->length()));
     }
     *out = B.release();
     return LCG_OK;
@@ -2030,29 +2073,24 @@ extern "C" int lcg_batch_alignment(const lcg_batch *B, int32_t k, int32_t *a, in
     return cnt;
 }
 
-extern "C" int64_t lcg_batch_write_clustal(const lcg_batch *B, int32_t k, int32_t width, int32_t flags, char *buf,
-                                           int64_t cap) {
-    if (!B || k < 0 || k >= B->npairs || width < 1) return LCG_E_ARG;
-    if (!B->traced) return LCG_E_STATE;
-    const std::string &sa = B->seqA[k], &sb = B->seqB[k];
-    const int n = (int)sa.size(), m = (int)sb.size();
+namespace {
+// Alignment::alignment_edges (alignment.cc:121-167): the traced edges of
+// pair k as (a, b) columns (-1 for a gap), with the positions a local or
+// free-end-gap alignment leaves out as gap columns unless only the local
+// part is asked for.  Returns LCG_OK or a status.
+int full_edges(const lcg_batch *B, int32_t k, bool only_local, std::vector<std::pair<int, int>> &cols,
+               std::string *strA = nullptr, std::string *strB = nullptr) {
+    const int n = (int)B->seqA[k].size(), m = (int)B->seqB[k].size();
     std::vector<int32_t> ea(n + m + 2), eb(n + m + 2);
     std::string stA(n + 2, 0), stB(m + 2, 0);
     const int ne = lcg_batch_alignment(B, k, ea.data(), eb.data(), (int32_t)ea.size(), stA.data(), stB.data(),
                                        (int32_t)std::max(n, m) + 2);
     if (ne < 0) return ne;
-    int64_t score = 0;
-    if (int rc = lcg_batch_score(B, k, &score)) return rc;
-    // Alignment::alignment_edges (alignment.cc:121-167): the traced edges,
-    // with the positions a local or free-end-gap alignment leaves out as
-    // gap columns (unless only the local part is written)
-    const bool only_local = (flags & LCG_CLUSTAL_LOCAL_ONLY) != 0;
-    std::string rowA, rowB, strA, strB;
+    cols.clear();
     auto col = [&](int a, int b) {
-        rowA += a > 0 ? sa[a - 1] : '-';
-        rowB += b > 0 ? sb[b - 1] : '-';
-        strA += a > 0 ? stA[a - 1] : '-';
-        strB += b > 0 ? stB[b - 1] : '-';
+        cols.push_back({a > 0 ? a : -1, b > 0 ? b : -1});
+        if (strA) *strA += a > 0 ? stA[a - 1] : '-';
+        if (strB) *strB += b > 0 ? stB[b - 1] : '-';
     };
     int lastA = 1, lastB = 1;
     for (int t = 0; t < ne; ++t) {
@@ -2072,33 +2110,49 @@ extern "C" int64_t lcg_batch_write_clustal(const lcg_batch *B, int32_t k, int32_
         for (; lastA <= n; ++lastA) col(lastA, -1);
         for (; lastB <= m; ++lastB) col(-1, lastB);
     }
-    // header (main_helper.icc:557-569) and blocks of `width` columns
-    // (MultipleAlignment::write, multiple_alignment.cc:994-1085)
-    std::string text = "CLUSTAL W --- LocARNA 2.0.1 --- Score: ";
-    text += score < LCG_NEG_LIMIT ? std::string("-inf") : std::to_string(score);
+    return LCG_OK;
+}
+}  // namespace
+
+extern "C" int64_t lcg_batch_write_clustal(const lcg_batch *B, int32_t k, int32_t width, int32_t flags, char *buf,
+                                           int64_t cap) {
+    if (!B || k < 0 || k >= B->npairs || width < 1) return LCG_E_ARG;
+    if (!B->traced) return LCG_E_STATE;
+    int64_t score = 0;
+    if (int rc = lcg_batch_score(B, k, &score)) return rc;
+    std::vector<std::pair<int, int>> cols;
+    std::string strA, strB;
+    if (int rc = full_edges(B, k, (flags & LCG_CLUSTAL_LOCAL_ONLY) != 0, cols, &strA, &strB)) return rc;
+    // rows of both inputs with the alignment's gaps, "A." / "B." prefixes
+    // on a name clash (MultipleAlignment::init, multiple_alignment.cc:186-247)
+    const std::vector<std::pair<std::string, std::string>> ali = aligned_rows(*B->rdA[k], *B->rdB[k], cols);
+    // header (main_helper.icc:557-569: the score for pairs of single
+    // sequences only) and blocks of `width` columns (MultipleAlignment::write,
+    // multiple_alignment.cc:994-1085), with --write-structure the structure
+    // of A first and that of B last
+    std::string text = "CLUSTAL W --- LocARNA 2.0.1";
+    if (B->rdA[k]->num_rows() == 1 && B->rdB[k]->num_rows() == 1) {
+        text += " --- Score: ";
+        text += score < LCG_NEG_LIMIT ? std::string("-inf") : std::to_string(score);
+    }
     text += "\n\n";
     std::vector<std::pair<std::string, const std::string *>> rows;
     const bool st = (flags & LCG_CLUSTAL_STRUCTURES) != 0;  // --write-structure
     if (st) rows.push_back({"", &strA});
-    rows.push_back({B->nameA[k], &rowA});
-    rows.push_back({B->nameB[k], &rowB});
+    for (const auto &r : ali) rows.push_back({r.first, &r.second});
     if (st) rows.push_back({"", &strB});
-    size_t namew = 18;
-    for (const auto &r : rows) namew = std::max(namew, r.first.size());
-    const size_t len = rowA.size();
-    size_t start = 0;
-    do {
-        const size_t end = std::min(len, start + (size_t)width);
-        for (const auto &r : rows) {
-            text += r.first;
-            text.append(namew - r.first.size(), ' ');
-            text += ' ';
-            text += r.second->substr(start, end - start);
-            text += '\n';
-        }
-        start = end;
-        if (start < len) text += '\n';
-    } while (start < len);
+    text += clustal_blocks(rows, (size_t)width);
+    if (buf && cap > (int64_t)text.size()) std::memcpy(buf, text.c_str(), text.size() + 1);
+    return (int64_t)text.size();
+}
+
+extern "C" int64_t lcg_batch_write_consensus_pp(const lcg_batch *B, int32_t k, char *buf, int64_t cap) {
+    if (!B || k < 0 || k >= B->npairs) return LCG_E_ARG;
+    if (!B->traced) return LCG_E_STATE;
+    std::vector<std::pair<int, int>> cols;
+    if (int rc = full_edges(B, k, false, cols)) return rc;
+    const std::string text =
+        consensus_pp_text(*B->rdA[k], *B->rdB[k], cols, B->pexpA[k], B->pexpB[k], B->bp.stacking != 0);
     if (buf && cap > (int64_t)text.size()) std::memcpy(buf, text.c_str(), text.size() + 1);
     return (int64_t)text.size();
 }
@@ -2418,6 +2472,7 @@ extern "C" int lcg_batch_pf_run(lcg_batch *B, const double *dtheta, float *devic
     if (B->bp.free_endgaps) return LCG_E_ARG;  // free end gaps: integer path only
     if (B->bp.nolp) return LCG_E_ARG;          // --noLP: integer path only
     if (B->bp.stacking) return LCG_E_ARG;      // --stacking: integer path only
+    if (B->bp.sigma_table) return LCG_E_ARG;   // profiles: integer path only
     if (B->device_built) return LCG_E_STATE;  // LocARNA-P reads the host arc-match lists
     if (int rc = pf_setup(*B)) return rc;
     for (int k = 0; k < B->npairs; ++k) pf_tables(*B, k, dtheta, B->pf_arena.data());
@@ -3481,7 +3536,10 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
         if (dev_band) {
             std::vector<int> todo, recheck;
             for (int k = 0; k < npairs; ++k) {
-                if (P[k]->A->length() <= LCG_BAND_MAX_LEN && P[k]->B->length() <= LCG_BAND_MAX_LEN)
+                // the band kernel reads one symbol per position: profiles
+                // are banded on the host
+                if (P[k]->A->length() <= LCG_BAND_MAX_LEN && P[k]->B->length() <= LCG_BAND_MAX_LEN &&
+                    P[k]->A->num_rows() == 1 && P[k]->B->num_rows() == 1)
                     todo.push_back(k);
                 else
                     host_band.push_back(k);

@@ -9,7 +9,8 @@
 //             are cell_off[i] + (j - min_col[i]); rows may be empty)
 //   codes:    codeA[n+1], codeB[m+1]    symbol codes (0..3 = ACGU)
 //   gaps:     gapA[n+1], gapB[m+1]      position-specific gap scores
-//   sigma:    8 x 8 base-match scores by symbol code
+//   sigma:    8 x 8 base-match scores by symbol code; for a pair of
+//             alignment profiles (n+1) x (m+1) scores by column pair
 //   rows:     row_begin[n+2]            groups of left end al are
 //                                       [row_begin[al+1], row_begin[al])
 //                                       (groups are stored al descending)
@@ -104,4 +105,7 @@ typedef struct {
                              // -1: every wait gives up at once, for error-path tests)
     int32_t nolp;            // --noLP: D of stacks (groups keyed by the inner left ends)
     int32_t stacking;        // --stacking / --new-stacking: stacked alternative of D
+    int32_t sigma_table;     // pairs of alignment profiles: sigma is a per-pair
+                             // (n+1) x (m+1) table at PairDesc.sigma (row i at
+                             // sigma + i * (m+1)), read from global memory
 } BatchParams;

@@ -47,13 +47,20 @@ RnaData::RnaData(const std::string &pp_text, double p_bpcut, long max_bp_span, b
         if (!rows.count(nm)) names.push_back(nm);
         rows[nm] += s;
     }
-    if (names.size() != 1) throw failure("RnaData: expected exactly one sequence in the pp alignment block");
-    name_ = names[0];
-    seq_ = rows[name_];
-    for (auto &c : seq_) {  // aux.cc:65-70
-        c = (char)std::toupper((unsigned char)c);
-        if (c == 'T') c = 'U';
+    if (names.empty()) throw failure("RnaData: no sequence in the pp alignment block");
+    for (const std::string &nm : names) {
+        std::string r = rows[nm];
+        for (auto &c : r) {  // aux.cc:65-70
+            c = (char)std::toupper((unsigned char)c);
+            if (c == 'T') c = 'U';
+        }
+        if (!rows_.empty() && r.size() != rows_[0].size())
+            throw failure("RnaData: rows of the pp alignment differ in length");
+        rows_.push_back(r);
     }
+    names_ = names;
+    name_ = names[0];
+    seq_ = rows_[0];
     if (!next_content_line(in, line) || line != "#SECTION BASEPAIRS")
         throw failure("Syntax error: Expected base pair section header.");
     const size_t span_max = max_bp_span < 0 ? std::numeric_limits<size_t>::max() : (size_t)max_bp_span;
@@ -344,17 +351,29 @@ std::vector<double> trace_probs(const RnaData &RA, const RnaData &RB, const Trac
         upB[j] = RB.prob_paired_upstream(j);
         dnB[j] = RB.prob_paired_downstream(j);
     }
-    // Boltzmann weights of the similarity, forward orientation
+    // Boltzmann weights of the similarity, forward orientation; the
+    // sequence term averages over the row pairs of the two columns whose
+    // symbols are both in ACGU, summed in row order (stral_score.cc:29-60)
+    const int rA = RA.num_rows(), rB = RB.num_rows();
+    std::vector<int> cA((size_t)rA * (n + 1)), cB((size_t)rB * (m + 1));
+    for (int k = 0; k < rA; ++k)
+        for (int i = 1; i <= n; ++i) cA[(size_t)k * (n + 1) + i] = nt_code(RA.row(k)[i - 1]);
+    for (int l = 0; l < rB; ++l)
+        for (int j = 1; j <= m; ++j) cB[(size_t)l * (m + 1) + j] = nt_code(RB.row(l)[j - 1]);
     std::vector<double> fw((size_t)(n + 1) * (m + 1), 0.0);
     for (int i = 1; i <= n; ++i) {
-        int a = nt_code(RA.nt(i));
         for (int j = 1; j <= m; ++j) {
-            int b = nt_code(RB.nt(j));
             double seq = 0;
             int cnt = 0;
-            if (a >= 0 && b >= 0) {
-                seq += sim(a, b);
-                cnt++;
+            for (int k = 0; k < rA; ++k) {
+                const int a = cA[(size_t)k * (n + 1) + i];
+                for (int l = 0; l < rB; ++l) {
+                    const int b = cB[(size_t)l * (m + 1) + j];
+                    if (a >= 0 && b >= 0) {
+                        seq += sim(a, b);
+                        cnt++;
+                    }
+                }
             }
             if (cnt != 0) seq /= cnt;
             double s = sw * (std::sqrt(dnA[i] * dnB[j]) + std::sqrt(upA[i] * upB[j])) + seq;
@@ -468,32 +487,62 @@ Scoring::Scoring(const RnaData &A, const RnaData &B, const ArcMatches &am, const
         return (int)symbols_.size() - 1;
     };
     const int n = A.length(), m = B.length();
+    lenB_ = m;
+    profile_ = A.num_rows() > 1 || B.num_rows() > 1;
+    // score of one pair of row symbols (scoring.cc:162-190): corrected
+    // RIBOSUM log-odds for two nucleotides, otherwise match / mismatch
+    // unless one of them is N (gap symbols compare as symbols)
+    auto symbol_score = [&](char ca, char cb) -> score_t {
+        const int xa = nt_code(ca), xb = nt_code(cb);
+        if (p.use_ribosum && xa >= 0 && xb >= 0) {
+            double bg = ribosum85_60::base_nonstruct[xa] * ribosum85_60::base_nonstruct[xb];
+            double corrected = std::log(ribosum85_60::basematch[xa * 4 + xb] / bg) / std::log(2);
+            return round2score(100.0 * corrected);
+        }
+        if (ca != 'N' && cb != 'N') return (ca == cb) ? p.match : p.mismatch;
+        return 0;
+    };
     codeA_.assign(n + 1, 0);
     codeB_.assign(m + 1, 0);
-    for (int i = 1; i <= n; ++i) codeA_[i] = code_of(A.nt(i));
-    for (int j = 1; j <= m; ++j) codeB_[j] = code_of(B.nt(j));
-    // sigma per symbol pair (scoring.cc:140-198, single-row alignments)
     sigma_codes_.assign(NCODES * NCODES, 0);
-    for (size_t a = 0; a < symbols_.size(); ++a) {
-        for (size_t b = 0; b < symbols_.size(); ++b) {
-            char ca = symbols_[a], cb = symbols_[b];
-            score_t s = 0;
-            int xa = nt_code(ca), xb = nt_code(cb);
-            if (p.use_ribosum && xa >= 0 && xb >= 0) {
-                double bg = ribosum85_60::base_nonstruct[xa] * ribosum85_60::base_nonstruct[xb];
-                double corrected = std::log(ribosum85_60::basematch[xa * 4 + xb] / bg) / std::log(2);
-                s += round2score(100.0 * corrected);
-            } else if (ca != 'N' && cb != 'N') {
-                s += (ca == cb) ? p.match : p.mismatch;
+    if (profile_) {
+        // sum of pairs over the rows of the two columns, integer quotient by
+        // the number of row pairs (scoring.cc:140-198)
+        const int rA = A.num_rows(), rB = B.num_rows();
+        sigma_tab_.assign((size_t)(n + 1) * (m + 1), 0);
+        std::vector<score_t> tab(256 * 256);
+        for (int x = 0; x < 256; ++x)
+            for (int y = 0; y < 256; ++y) tab[x * 256 + y] = symbol_score((char)x, (char)y);
+        for (int i = 1; i <= n; ++i)
+            for (int j = 1; j <= m; ++j) {
+                score_t sum = 0;
+                for (int k = 0; k < rA; ++k) {
+                    const unsigned char ca = (unsigned char)A.row(k)[i - 1];
+                    for (int l = 0; l < rB; ++l) sum += tab[ca * 256 + (unsigned char)B.row(l)[j - 1]];
+                }
+                sigma_tab_[(size_t)i * (m + 1) + j] = round2score((double)(sum / (int)(rA * rB)));
             }
-            sigma_codes_[a * NCODES + b] = round2score(s / (int)1);
-        }
+    } else {
+        for (int i = 1; i <= n; ++i) codeA_[i] = code_of(A.nt(i));
+        for (int j = 1; j <= m; ++j) codeB_[j] = code_of(B.nt(j));
+        // sigma per symbol pair (scoring.cc:140-198, single-row alignments)
+        for (size_t a = 0; a < symbols_.size(); ++a)
+            for (size_t b = 0; b < symbols_.size(); ++b)
+                sigma_codes_[a * NCODES + b] = round2score((double)(symbol_score(symbols_[a], symbols_[b]) / (int)1));
     }
-    // position specific gap cost of single sequences (scoring.cc:271-311)
-    gapA_.assign(n + 1, 0);
-    gapB_.assign(m + 1, 0);
-    for (int i = 1; i <= n; ++i) gapA_[i] = round2score((1 - 0.0f) * (score_t)p.indel);
-    for (int j = 1; j <= m; ++j) gapB_[j] = round2score((1 - 0.0f) * (score_t)p.indel);
+    // position-specific gap cost from the column's gap frequency, in single
+    // precision (scoring.cc:271-311)
+    auto gapcost = [&](const RnaData &R, std::vector<score_t> &g) {
+        g.assign(R.length() + 1, 0);
+        for (int i = 1; i <= R.length(); ++i) {
+            float freq = 0;
+            for (int k = 0; k < R.num_rows(); ++k) freq += (R.row(k)[i - 1] == '-') ? 1 : 0;
+            freq /= (size_t)R.num_rows();
+            g[i] = round2score((1 - freq) * (score_t)p.indel);
+        }
+    };
+    gapcost(A, gapA_);
+    gapcost(B, gapB_);
     // arc weights (scoring.cc:200-265)
     const double peA = p.exp_prob > 0 ? p.exp_prob : 1.0 / (2.0 * n);
     const double peB = p.exp_prob > 0 ? p.exp_prob : 1.0 / (2.0 * m);
@@ -522,7 +571,9 @@ Scoring::Scoring(const RnaData &A, const RnaData &B, const ArcMatches &am, const
         const Arc &a = bA.arc(x.arcA), &b = bB.arc(x.arcB);
         score_t seq = 0;
         if (p.tau != 0) {
-            if (p.use_ribosum) {
+            if (p.use_ribosum && profile_) {
+                seq = ribosum_arc_term_rows(A, a, B, b);
+            } else if (p.use_ribosum) {
                 int al = nt_code(A.nt(a.left)), ar = nt_code(A.nt(a.right));
                 int bl = nt_code(B.nt(b.left)), br = nt_code(B.nt(b.right));
                 if (al >= 0 && ar >= 0 && bl >= 0 && br >= 0) seq = ribosum_arc_term(al, ar, bl, br);
@@ -573,6 +624,151 @@ score_t Scoring::ribosum_arc_term(int al, int ar, int bl, int br) {
     return round2score(100.0 * sc / 1);
 }
 
-score_t Scoring::basematch(int i, int j) const { return sigma_codes_[codeA_[i] * NCODES + codeB_[j]]; }
+score_t Scoring::ribosum_arc_term_rows(const RnaData &A, const Arc &a, const RnaData &B, const Arc &b) {
+    double sc = 0;
+    int considered = 0;
+    for (int k = 0; k < A.num_rows(); ++k) {
+        const int al = nt_code(A.row(k)[a.left - 1]), ar = nt_code(A.row(k)[a.right - 1]);
+        for (int l = 0; l < B.num_rows(); ++l) {
+            const int bl = nt_code(B.row(l)[b.left - 1]), br = nt_code(B.row(l)[b.right - 1]);
+            if (al < 0 || ar < 0 || bl < 0 || br < 0) continue;
+            ++considered;
+            const double pam = ribosum85_60::arcmatch[(al * 4 + ar) * 16 + (bl * 4 + br)];
+            const double pa = ribosum85_60::basepair[al * 4 + ar], pb = ribosum85_60::basepair[bl * 4 + br];
+            sc += std::log(pam / (pa * pb)) / std::log(2);
+        }
+    }
+    if (considered == 0) return 0;
+    return round2score(100.0 * sc / considered);
+}
+
+score_t Scoring::max_abs_sigma() const {
+    score_t mx = 0;
+    for (score_t v : sigma_codes_) mx = std::max(mx, v < 0 ? -v : v);
+    for (score_t v : sigma_tab_) mx = std::max(mx, v < 0 ? -v : v);
+    return mx;
+}
+
+score_t Scoring::basematch(int i, int j) const {
+    if (profile_) return sigma_tab_[(size_t)i * (lenB_ + 1) + j];
+    return sigma_codes_[codeA_[i] * NCODES + codeB_[j]];
+}
+
+// ------------------------------------------------------------ writers
+
+std::vector<std::pair<std::string, std::string>> aligned_rows(const RnaData &A, const RnaData &B,
+                                                              const std::vector<std::pair<int, int>> &cols) {
+    bool clash = false;
+    for (int k = 0; k < A.num_rows() && !clash; ++k)
+        for (int l = 0; l < B.num_rows() && !clash; ++l) clash = A.row_name(k) == B.row_name(l);
+    std::vector<std::pair<std::string, std::string>> out;
+    for (int k = 0; k < A.num_rows(); ++k) {
+        std::string r;
+        r.reserve(cols.size());
+        for (const auto &c : cols) r += c.first > 0 ? A.row(k)[c.first - 1] : '-';
+        out.push_back({(clash ? "A." : "") + A.row_name(k), r});
+    }
+    for (int l = 0; l < B.num_rows(); ++l) {
+        std::string r;
+        r.reserve(cols.size());
+        for (const auto &c : cols) r += c.second > 0 ? B.row(l)[c.second - 1] : '-';
+        out.push_back({(clash ? "B." : "") + B.row_name(l), r});
+    }
+    return out;
+}
+
+std::string clustal_blocks(const std::vector<std::pair<std::string, const std::string *>> &rows, size_t width) {
+    size_t namew = 18, len = 0;
+    for (const auto &r : rows) {
+        namew = std::max(namew, r.first.size());
+        len = std::max(len, r.second->size());
+    }
+    std::string text;
+    size_t start = 0;
+    do {
+        const size_t end = std::min(len, start + width);
+        for (const auto &r : rows) {
+            text += r.first;
+            text.append(namew - r.first.size(), ' ');
+            text += ' ';
+            text += r.second->substr(std::min(start, r.second->size()), end - start);
+            text += '\n';
+        }
+        start = end;
+        if (start < len) text += '\n';
+    } while (start < len);
+    return text;
+}
+
+std::string format_prob(double p) {
+    std::ostringstream fixed;
+    fixed.precision(4);
+    fixed << p;
+    std::string txt = fixed.str();
+    if (txt.size() > 8) {  // longer than precision + 4 characters
+        std::ostringstream sci;
+        sci.setf(std::ios::scientific, std::ios::floatfield);
+        sci.precision(3);
+        sci << p;
+        txt = sci.str();
+    }
+    const size_t e = txt.find("e-0");
+    if (e != std::string::npos) txt.replace(e, 3, "e-");
+    return txt;
+}
+
+std::string consensus_pp_text(const RnaData &A, const RnaData &B, const std::vector<std::pair<int, int>> &cols,
+                              double p_expA, double p_expB, bool stacking) {
+    const double wA = A.num_rows(), wB = B.num_rows();
+    const double cut = std::exp((std::log(A.cutoff()) * wA + std::log(B.cutoff()) * wB) / (wA + wB));
+    const double floor_pen = cut * 0.1;  // penalty factor of rna_data.cc:116-117
+    const double floorA = std::min(p_expA, floor_pen), floorB = std::min(p_expB, floor_pen);
+    auto mean = [&](double pa, double pb) {
+        pa = std::max(floorA, pa);
+        pb = std::max(floorB, pb);
+        return std::exp((std::log(pa) * wA + std::log(pb) * wB) / (wA + wB));
+    };
+    // probabilities of the column pairs: only pairs with a base pair on
+    // some side can exceed the cut (the floors' mean is below it), so the
+    // loop runs over the base pairs of A and of B mapped to columns
+    const int ncols = (int)cols.size();
+    std::vector<int> colA(A.length() + 1, -1), colB(B.length() + 1, -1);
+    for (int c = 0; c < ncols; ++c) {
+        if (cols[c].first > 0) colA[cols[c].first] = c;
+        if (cols[c].second > 0) colB[cols[c].second] = c;
+    }
+    std::map<std::pair<int, int>, double> out;
+    auto consider = [&](int ci, int cj) {
+        if (ci < 0 || cj < 0 || ci >= cj || out.count({ci + 1, cj + 1})) return;
+        const auto &ei = cols[ci], &ej = cols[cj];
+        const bool okA = ei.first > 0 && ej.first > 0, okB = ei.second > 0 && ej.second > 0;
+        const double pc = mean(okA ? A.arc_prob(ei.first, ej.first) : 0, okB ? B.arc_prob(ei.second, ej.second) : 0);
+        bool keep = pc > cut;
+        if (stacking && !keep) {
+            const double sc = mean(okA ? A.joint_arc_prob(ei.first, ej.first) : 0,
+                                   okB ? B.joint_arc_prob(ei.second, ej.second) : 0);
+            keep = sc > cut;
+        }
+        if (keep) out[{ci + 1, cj + 1}] = pc;
+    };
+    for (const auto &x : A.pairs()) consider(colA[x.first.first], colA[x.first.second]);
+    for (const auto &x : B.pairs()) consider(colB[x.first.first], colB[x.first.second]);
+    std::vector<std::pair<std::string, std::string>> rows = aligned_rows(A, B, cols);
+    size_t namew = 18;
+    for (const auto &r : rows) namew = std::max(namew, r.first.size());
+    std::string text = "#PP 2.0\n\n";
+    for (const auto &r : rows) {
+        text += r.first;
+        text.append(namew - std::min(namew, r.first.size()), ' ');
+        text += ' ';
+        text += r.second;
+        text += '\n';
+    }
+    text += "\n#END\n\n#SECTION BASEPAIRS\n\n#BPCUT " + format_prob(cut) + "\n\n";
+    for (const auto &x : out)
+        text += std::to_string(x.first.first) + " " + std::to_string(x.first.second) + " " + format_prob(x.second) + "\n";
+    text += "\n#END\n";
+    return text;
+}
 
 }  // namespace locarna_amd

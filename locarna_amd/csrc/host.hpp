@@ -27,16 +27,23 @@ struct capacity_failure : failure {
     explicit capacity_failure(const std::string &s) : failure(s) {}
 };
 
-// Sequence plus filtered base-pair probabilities read from #PP 2.0 text.
-// reference: RnaData, rna_data.hh:37; reader rna_data.cc:984-1124
+// Sequence, or multiple alignment (a profile: mlocarna's intermediate
+// alignments), plus filtered base-pair probabilities read from #PP 2.0 text.
+// reference: RnaData, rna_data.hh:37; reader rna_data.cc:984-1124, rows
+// multiple_alignment.cc:270-380
 class RnaData {
 public:
     // stacking: read the joint probabilities of the #STACK column
     RnaData(const std::string &pp_text, double p_bpcut, long max_bp_span, bool stacking = false);
     int length() const { return (int)seq_.size(); }
-    const std::string &sequence() const { return seq_; }  // 0-based chars
-    char nt(int i) const { return seq_[i - 1]; }           // 1-based
+    const std::string &sequence() const { return seq_; }  // first row, 0-based chars
+    char nt(int i) const { return seq_[i - 1]; }           // first row, 1-based
     const std::string &name() const { return name_; }
+    // rows of the alignment, in first-seen name order (one for a sequence)
+    int num_rows() const { return (int)rows_.size(); }
+    const std::string &row(int k) const { return rows_[k]; }
+    const std::string &row_name(int k) const { return names_[k]; }
+    double cutoff() const { return p_bpcut_; }  // arc_cutoff_prob (rna_data.cc:686-688)
     double arc_prob(int i, int j) const;
     // joint probability of (i, j) and (i+1, j-1); 0 without (rna_data.cc:700-710)
     double joint_arc_prob(int i, int j) const;
@@ -47,6 +54,7 @@ public:
 
 private:
     std::string name_, seq_;
+    std::vector<std::string> names_, rows_;
     std::map<std::pair<int, int>, double> probs_, joint_;
     std::vector<std::pair<std::pair<int, int>, double>> pairs_;
     double p_bpcut_;
@@ -205,25 +213,70 @@ public:
     score_t weightA(int arc) const { return wA_[arc]; }
     score_t weightB(int arc) const { return wB_[arc]; }
     const ScoringParams &params() const { return p_; }
-    // single-sequence symbol code (0..3 = ACGU, 4.. other symbols)
+    // a pair of profiles (more than one row on a side): sigma as a table
+    // over all column pairs instead of symbol codes
+    bool profile() const { return profile_; }
+    // single-sequence symbol code (0..3 = ACGU, 4.. other symbols; 0 for
+    // profiles)
     int codeA(int i) const { return codeA_[i]; }
     int codeB(int j) const { return codeB_[j]; }
     static const int NCODES = 8;
     score_t sigma_code(int a, int b) const { return sigma_codes_[a * NCODES + b]; }
+    // largest |sigma| over the code table (sequences) or the column table
+    // (profiles)
+    score_t max_abs_sigma() const;
+    // profiles: sigma of columns (i, j), 1-based
+    const std::vector<score_t> &sigma_table() const { return sigma_tab_; }
     // ribosum arc-match term riboX (scoring.cc:368-438) before the tau factor
     score_t arc_sequence_term(int am) const { return am_seq_[am]; }
     // riboX term of one arc match from the nucleotide codes (0..3 = ACGU) of
     // its four ends, RIBOSUM85_60 (scoring.cc:368-438, single sequences)
     static score_t ribosum_arc_term(int al, int ar, int bl, int br);
+    // the same averaged over the row pairs of two profiles whose four ends
+    // are all in ACGU (scoring.cc:368-438)
+    static score_t ribosum_arc_term_rows(const RnaData &A, const Arc &a, const RnaData &B, const Arc &b);
 
 private:
     ScoringParams p_;
     const ArcMatches &am_;
+    bool profile_ = false;
+    int lenB_ = 0;
+    std::vector<score_t> sigma_tab_;  // profiles: (n+1) x (m+1)
     std::vector<int> codeA_, codeB_;
     std::vector<char> symbols_;
     std::vector<score_t> sigma_codes_;
     std::vector<score_t> gapA_, gapB_, wA_, wB_, am_score_, am_seq_, am_score_st_;
     std::vector<char> am_stackable_;
 };
+
+// ------------------------------------------------------------ writers
+// Rows of an aligned pair: the rows of A, then those of B, with '-' in the
+// gap columns of `cols` ((a, b) per column, -1 for a gap); on a name clash
+// between the two sides the names get "A." / "B." prefixes.
+// reference: MultipleAlignment::init, multiple_alignment.cc:186-247
+std::vector<std::pair<std::string, std::string>> aligned_rows(const RnaData &A, const RnaData &B,
+                                                              const std::vector<std::pair<int, int>> &cols);
+// Name / sequence lines in blocks of `width` columns, names padded to
+// max(18, longest name) and one blank, a blank line between blocks.
+// reference: MultipleAlignment::write, multiple_alignment.cc:994-1085
+std::string clustal_blocks(const std::vector<std::pair<std::string, const std::string *>> &rows, size_t width);
+// A probability as pp files write it: 4 significant digits, or scientific
+// with 3 decimals when that is shorter, "e-0" as "e-".
+// reference: format_prob, rna_data.cc:1274-1303
+std::string format_prob(double p);
+// pp 2.0 text of an aligned pair (locarna --pp): the aligned rows, and the
+// consensus base-pair probabilities of every pair of columns as weighted
+// geometric means of the two sides' probabilities, each raised to at
+// least min(expected probability, 0.1 x cut), kept above the weighted
+// geometric mean of the two cuts, which is written as #BPCUT.  With
+// stacking (both sides read with joint probabilities) a pair is kept also
+// when the consensus of the joint probabilities exceeds the cut; the text
+// carries no joint column (the reference's consensus object has no
+// stacking flag).  Lines of base pairs in (i, j) order.
+// reference: RnaData consensus constructor, rna_data.cc:75-122,
+// init_as_consensus_dot_plot / consensus_probability :1475-1570, write_pp
+// :1242-1339
+std::string consensus_pp_text(const RnaData &A, const RnaData &B, const std::vector<std::pair<int, int>> &cols,
+                              double p_expA, double p_expB, bool stacking);
 
 }  // namespace locarna_amd

@@ -51,6 +51,18 @@ def pp_text(name, seq, pairs, bpcut=BPCUT):
     return "\n".join(lines)
 
 
+def pp_text_rows(rows, pairs, bpcut=BPCUT):
+    """#PP 2.0 text of a multiple alignment (a profile): rows (name,
+    aligned sequence) of one length, base pairs (i, j, p) on its columns."""
+    width = max([18] + [len(n) for n, _ in rows])
+    lines = ["#PP 2.0", ""] + ["%-*s %s" % (width, n, r) for n, r in rows] + ["", "#END", "",
+                                                                             "#SECTION BASEPAIRS", "",
+                                                                             "#BPCUT %g" % bpcut, ""]
+    lines += ["%d %d %.8g" % t[:3] for t in pairs]
+    lines += ["", "#END", ""]
+    return "\n".join(lines)
+
+
 def with_stacking(pairs, seed=0):
     """The pairs (i, j, p) with a joint probability of (i, j) and its inner
     pair (i+1, j-1) where that pair is listed: a seeded fraction in [0.2, 1]

@@ -117,7 +117,11 @@ Ribosum read_ribosum(const std::string &path) {
 // rna_data.cc:984-1124 (pp 2.0 reader), aux.cc:65-70 (normalize symbols)
 struct RnaData {
     std::string name;
-    std::string seq;  // 1-based via seq[i-1]
+    std::string seq;  // the first row; 1-based via seq[i-1]
+    // rows of a multiple alignment (a profile), first-seen name order
+    // (multiple_alignment.cc:270-380); one row for a single sequence
+    std::vector<std::string> names, rows;
+    int num_rows() const { return (int)rows.size(); }
     std::map<std::pair<int, int>, double> arc_probs;
     // joint probability of (i, j) and (i+1, j-1) (the #STACK column)
     std::map<std::pair<int, int>, double> arc_2_probs;
@@ -176,13 +180,20 @@ RnaData read_pp(const std::string &text, double min_prob, long long max_bp_span,
         if (!seqs.count(name)) names.push_back(name);
         seqs[name] += s;
     }
-    if (names.size() != 1) throw failure("oracle supports single-sequence pp input");
-    r.name = names[0];
-    r.seq = seqs[r.name];
-    for (auto &c : r.seq) {
-        c = (char)toupper((unsigned char)c);
-        if (c == 'T') c = 'U';
+    if (names.empty()) throw failure("no sequence in the pp alignment block");
+    for (const std::string &nm : names) {
+        std::string row = seqs[nm];
+        for (auto &c : row) {  // aux.cc:65-70
+            c = (char)toupper((unsigned char)c);
+            if (c == 'T') c = 'U';
+        }
+        if (!r.rows.empty() && row.size() != r.rows[0].size())
+            throw failure("rows of the pp alignment differ in length");
+        r.rows.push_back(row);
     }
+    r.names = names;
+    r.name = names[0];
+    r.seq = r.rows[0];
     if (!get_nonempty_line(in, line) || line != "#SECTION BASEPAIRS")
         throw failure("Expected base pair section header.");
     size_t max_span = max_bp_span < 0 ? std::numeric_limits<size_t>::max() : (size_t)max_bp_span;
@@ -289,12 +300,12 @@ struct Band {
 
 // stral_score.cc:10-98
 struct StralScore {
-    std::string seqA, seqB;
+    std::vector<std::string> seqA, seqB;  // rows
     const double (*sim)[4];
     double struct_weight, indel_opening, indel;
     std::vector<double> p_upA, p_downA, p_upB, p_downB;
     StralScore(const RnaData &a, const RnaData &b, const double (*sim_)[4], double sw, double io, double ind)
-        : seqA(a.seq), seqB(b.seq), sim(sim_), struct_weight(sw), indel_opening(io), indel(ind) {
+        : seqA(a.rows), seqB(b.rows), sim(sim_), struct_weight(sw), indel_opening(io), indel(ind) {
         init(a, p_upA, p_downA);
         init(b, p_upB, p_downB);
     }
@@ -308,19 +319,23 @@ struct StralScore {
         }
     }
     double sigma(size_t i, size_t j) const {
+        // average over the row pairs whose symbols are both in ACGU
         int pairs = 0;
         double seq_score = 0;
-        int a = Ribosum::idx(seqA[i - 1]), b = Ribosum::idx(seqB[j - 1]);
-        if (a >= 0 && b >= 0) {
-            seq_score += sim[a][b];
-            pairs++;
-        }
+        for (size_t k = 0; k < seqA.size(); k++)
+            for (size_t l = 0; l < seqB.size(); l++) {
+                int a = Ribosum::idx(seqA[k][i - 1]), b = Ribosum::idx(seqB[l][j - 1]);
+                if (a >= 0 && b >= 0) {
+                    seq_score += sim[a][b];
+                    pairs++;
+                }
+            }
         if (pairs != 0) seq_score /= pairs;
         return struct_weight * (sqrt(p_downA[i] * p_downB[j]) + sqrt(p_upA[i] * p_upB[j])) + seq_score;
     }
     void reverse() {
-        std::reverse(seqA.begin(), seqA.end());
-        std::reverse(seqB.begin(), seqB.end());
+        for (auto &r : seqA) std::reverse(r.begin(), r.end());
+        for (auto &r : seqB) std::reverse(r.begin(), r.end());
         std::reverse(p_upA.begin() + 1, p_upA.end());
         std::reverse(p_downA.begin() + 1, p_downA.end());
         std::reverse(p_upB.begin() + 1, p_upB.end());
@@ -552,8 +567,18 @@ struct Scoring {
             for (int j = 1; j <= lenB; ++j) sigma_tab(i, j) = sigma_(i, j);
         gapA_tab.assign(lenA + 1, 0);
         gapB_tab.assign(lenB + 1, 0);
-        for (int i = 1; i <= lenA; i++) gapA_tab[i] = round2score((1 - 0.0f) * (sc_t)p.indel);
-        for (int i = 1; i <= lenB; i++) gapB_tab[i] = round2score((1 - 0.0f) * (sc_t)p.indel);
+        // scoring.cc:271-311: gap frequencies of the columns in float, the
+        // gap cost round((1 - freq) * indel)
+        auto gapcost = [&](const RnaData &r, std::vector<sc_t> &tab) {
+            for (int i = 1; i <= r.length(); i++) {
+                float freq = 0;
+                for (const std::string &row : r.rows) freq += (row[i - 1] == '-') ? 1 : 0;
+                freq /= r.rows.size();
+                tab[i] = round2score((1 - freq) * p.indel);
+            }
+        };
+        gapcost(A, gapA_tab);
+        gapcost(B, gapB_tab);
         double peA = p.exp_prob > 0 ? p.exp_prob : 1.0 / (2.0 * lenA);  // aux.hh:216-220
         double peB = p.exp_prob > 0 ? p.exp_prob : 1.0 / (2.0 * lenB);
         weights(A, bpsA, peA, weightsA);
@@ -562,15 +587,20 @@ struct Scoring {
         stack_weights(B, bpsB, peB, weightsB, stack_weightsB);
     }
     bool stacking() const { return p.stacking != 0; }
+    // sum of pairs over the rows of the two columns, then the integer
+    // quotient by the number of row pairs (C++ division, toward zero)
     sc_t sigma_(int ia, int ib) const {
-        char cA = A.seq[ia - 1], cB = B.seq[ib - 1];
         sc_t score = 0;
-        if (rib && Ribosum::idx(cA) >= 0 && Ribosum::idx(cB) >= 0) {
-            score += round2score(100.0 * rib->basematch_score_corrected(cA, cB));
-        } else if (cA != 'N' && cB != 'N') {
-            score += (cA == cB) ? p.match : p.mismatch;
-        }
-        return round2score(score / (int)(1 * 1));
+        for (const std::string &ra : A.rows)
+            for (const std::string &rb : B.rows) {
+                char cA = ra[ia - 1], cB = rb[ib - 1];
+                if (rib && Ribosum::idx(cA) >= 0 && Ribosum::idx(cB) >= 0) {
+                    score += round2score(100.0 * rib->basematch_score_corrected(cA, cB));
+                } else if (cA != 'N' && cB != 'N') {
+                    score += (cA == cB) ? p.match : p.mismatch;
+                }
+            }
+        return round2score(score / (int)(A.rows.size() * B.rows.size()));
     }
     sc_t probToWeight(double prob, double pe) const {
         return round2score(round((sc_t)p.struct_weight * (1 - log(prob) / log(pe))));
@@ -604,17 +634,24 @@ struct Scoring {
     bool stackable(const ArcMatch &am) const {
         return A.joint_arc_prob(am.a.left, am.a.right) > 0 && B.joint_arc_prob(am.b.left, am.b.right) > 0;
     }
+    // the log-odds averaged over the row pairs whose four end symbols are
+    // all in ACGU (rows with a gap at an end take no part)
     sc_t riboX_arcmatch_score(const Arc &a, const Arc &b) const {
-        char al = A.seq[a.left - 1], ar = A.seq[a.right - 1], bl = B.seq[b.left - 1], br = B.seq[b.right - 1];
         double score = 0;
         int considered = 0;
-        if (Ribosum::idx(al) >= 0 && Ribosum::idx(ar) >= 0 && Ribosum::idx(bl) >= 0 && Ribosum::idx(br) >= 0) {
-            considered++;
-            double pam = rib->arcmatch[Ribosum::idx(al) * 4 + Ribosum::idx(ar)][Ribosum::idx(bl) * 4 + Ribosum::idx(br)];
-            double pa = rib->basepair[Ribosum::idx(al)][Ribosum::idx(ar)];
-            double pb = rib->basepair[Ribosum::idx(bl)][Ribosum::idx(br)];
-            score += log(pam / (pa * pb)) / log(2);
-        }
+        for (const std::string &ra : A.rows)
+            for (const std::string &rb : B.rows) {
+                char al = ra[a.left - 1], ar = ra[a.right - 1], bl = rb[b.left - 1], br = rb[b.right - 1];
+                if (Ribosum::idx(al) >= 0 && Ribosum::idx(ar) >= 0 && Ribosum::idx(bl) >= 0 &&
+                    Ribosum::idx(br) >= 0) {
+                    considered++;
+                    double pam = rib->arcmatch[Ribosum::idx(al) * 4 + Ribosum::idx(ar)]
+                                              [Ribosum::idx(bl) * 4 + Ribosum::idx(br)];
+                    double pa = rib->basepair[Ribosum::idx(al)][Ribosum::idx(ar)];
+                    double pb = rib->basepair[Ribosum::idx(bl)][Ribosum::idx(br)];
+                    score += log(pam / (pa * pb)) / log(2);
+                }
+            }
         if (considered == 0) return 0;
         return round2score(100.0 * score / considered);
     }
@@ -628,6 +665,91 @@ struct Scoring {
                (stacked ? stack_weightsA[am.a.idx] + stack_weightsB[am.b.idx] : weightsA[am.a.idx] + weightsB[am.b.idx]);
     }
 };
+
+// ------------------------------------------------------------- consensus
+// The pp file of an aligned pair, as locarna --pp writes it for the next
+// step of mlocarna's progressive alignment: rows from the alignment edges
+// (multiple_alignment.cc:186-247, "A." / "B." name prefixes on a name
+// clash), consensus base-pair probabilities as weighted geometric means
+// (rna_data.cc:1475-1570, penalty factor 0.1 from :116-120), written by
+// RnaData::write_pp (rna_data.cc:1242-1339, probabilities by format_prob,
+// name / sequence lines by multiple_alignment.cc:995-1030).  The reference
+// iterates a hash map for the base-pair lines; they are written here in
+// (i, j) order, which the reader does not see.
+std::string format_prob(double prob) {
+    const int precision = 4;
+    std::ostringstream outd;
+    outd.precision(precision);
+    outd << prob;
+    std::string str = outd.str();
+    if (str.length() > (size_t)precision + 4) {
+        std::ostringstream outs;
+        outs.setf(std::ios::scientific, std::ios::floatfield);
+        outs.precision(precision - 1);
+        outs << prob;
+        str = outs.str();
+    }
+    size_t pos = str.find("e-0");
+    if (pos != std::string::npos) str.replace(pos, 3, "e-");
+    return str;
+}
+
+std::string consensus_pp(const RnaData &A, const RnaData &B, const std::vector<std::pair<int, int>> &edges,
+                         double p_expA, double p_expB, bool stacking) {
+    const size_t rowsA = A.rows.size(), rowsB = B.rows.size();
+    std::vector<std::string> aliA(rowsA), aliB(rowsB);
+    for (const auto &e : edges) {
+        for (size_t k = 0; k < rowsA; ++k) aliA[k] += e.first < 0 ? '-' : A.rows[k][e.first - 1];
+        for (size_t k = 0; k < rowsB; ++k) aliB[k] += e.second < 0 ? '-' : B.rows[k][e.second - 1];
+    }
+    bool clash = false;
+    for (const std::string &x : A.names)
+        for (const std::string &y : B.names) clash = clash || x == y;
+    std::vector<std::pair<std::string, std::string>> rows;
+    for (size_t k = 0; k < rowsA; ++k) rows.push_back({(clash ? "A." : "") + A.names[k], aliA[k]});
+    for (size_t k = 0; k < rowsB; ++k) rows.push_back({(clash ? "B." : "") + B.names[k], aliB[k]});
+    const double p_minMean = exp((log(A.p_bpcut) * rowsA + log(B.p_bpcut) * rowsB) / (rowsA + rowsB));
+    const double p_penalty = p_minMean * 0.1;
+    auto cons = [&](double pA, double pB) {
+        pA = std::max(std::min(p_expA, p_penalty), pA);
+        pB = std::max(std::min(p_expB, p_penalty), pB);
+        return exp((log(pA) * rowsA + log(pB) * rowsB) / (rowsA + rowsB));
+    };
+    std::map<std::pair<int, int>, double> probs;
+    for (size_t i = 0; i < edges.size(); i++)
+        for (size_t j = i + 1; j < edges.size(); j++) {
+            const auto &ei = edges[i], &ej = edges[j];
+            const double pA = (ei.first < 0 || ej.first < 0) ? 0 : A.arc_prob(ei.first, ej.first);
+            const double pB = (ei.second < 0 || ej.second < 0) ? 0 : B.arc_prob(ei.second, ej.second);
+            const double pc = cons(pA, pB);
+            if (stacking) {
+                const double sA = (ei.first < 0 || ej.first < 0) ? 0 : A.joint_arc_prob(ei.first, ej.first);
+                const double sB = (ei.second < 0 || ej.second < 0) ? 0 : B.joint_arc_prob(ei.second, ej.second);
+                // the joint consensus decides membership; the written file
+                // carries no #STACK column (the consensus object is built
+                // with has_stacking_ false)
+                if (pc > p_minMean || cons(sA, sB) > p_minMean) probs[{(int)i + 1, (int)j + 1}] = pc;
+            } else if (pc > p_minMean) {
+                probs[{(int)i + 1, (int)j + 1}] = pc;
+            }
+        }
+    size_t namewidth = 18;
+    for (const auto &r : rows) namewidth = std::max(namewidth, r.first.length());
+    std::ostringstream out;
+    out << "#PP 2.0" << std::endl << std::endl;
+    for (const auto &r : rows) {
+        std::streamsize ow = out.width(namewidth);
+        out << std::left << r.first << " ";
+        out.width(ow);
+        out << r.second << std::endl;
+    }
+    out << std::endl << "#END" << std::endl;
+    out << std::endl << "#SECTION BASEPAIRS" << std::endl << std::endl << "#BPCUT " << format_prob(p_minMean)
+        << std::endl << std::endl;
+    for (const auto &x : probs) out << x.first.first << " " << x.first.second << " " << format_prob(x.second) << std::endl;
+    out << std::endl << "#END" << std::endl;
+    return out.str();
+}
 
 // ---------------------------------------------------------------- inputs
 struct Inputs {
@@ -1860,4 +1982,37 @@ extern "C" int oracle_pf_extra(const oracle_result *r, double *bm_full, int cap,
     for (int k = 0; k < n && k < cap; ++k) bm_full[k] = r->bm_full[k];
     if (openings) *openings = r->openings;
     return n;
+}
+
+extern "C" long long oracle_consensus_pp(const char *ppA, const char *ppB, const oracle_params *p, const int *ea,
+                                         const int *eb, int nedges, char *buf, long long cap, char *err, int errlen) {
+    try {
+        const orc::RnaData A = orc::read_pp(ppA, p->min_prob, p->max_bp_span, p->stacking != 0);
+        const orc::RnaData B = orc::read_pp(ppB, p->min_prob, p->max_bp_span, p->stacking != 0);
+        std::vector<std::pair<int, int>> edges;
+        for (int k = 0; k < nedges; ++k) {
+            if ((ea[k] < 0 && eb[k] < 0) || ea[k] > A.length() || eb[k] > B.length() || ea[k] == 0 || eb[k] == 0)
+                throw orc::failure("invalid alignment edge");
+            edges.push_back({ea[k], eb[k]});
+        }
+        // locarna.cc:662-663
+        const double peA = p->exp_prob > 0 ? p->exp_prob : 1.0 / (2.0 * A.length());
+        const double peB = p->exp_prob > 0 ? p->exp_prob : 1.0 / (2.0 * B.length());
+        const std::string text = orc::consensus_pp(A, B, edges, peA, peB, p->stacking != 0);
+        if ((long long)text.size() + 1 <= cap) std::memcpy(buf, text.c_str(), text.size() + 1);
+        return (long long)text.size();
+    } catch (const std::exception &e) {
+        if (err && errlen > 0) {
+            std::strncpy(err, e.what(), errlen - 1);
+            err[errlen - 1] = 0;
+        }
+        return -1;
+    }
+}
+
+extern "C" int oracle_format_prob(double prob, char *buf, int cap) {
+    const std::string s = orc::format_prob(prob);
+    if ((int)s.size() + 1 > cap) return -1;
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    return (int)s.size();
 }

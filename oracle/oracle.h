@@ -158,6 +158,15 @@ int oracle_pf_D(const oracle_result *r, double *D, double *Dprime, int cap);
 /* pf extras: unrestricted base-match (match edge, not arc end) marginals,
  * dense (lenA+1)*(lenB+1), and the expected number of gap openings */
 int oracle_pf_extra(const oracle_result *r, double *bm_full, int cap, double *openings);
+/* pp text of the aligned pair given by its edges (ea[k], eb[k]: 1-based
+ * positions, -1 for a gap): rows of both inputs with the alignment's gaps,
+ * consensus base-pair probabilities (locarna --pp; rna_data.cc:1475-1570,
+ * :1242-1339).  Returns the text length (written with its terminating zero
+ * when it fits cap), -1 on an error (message in err). */
+long long oracle_consensus_pp(const char *ppA, const char *ppB, const oracle_params *p, const int *ea,
+                              const int *eb, int nedges, char *buf, long long cap, char *err, int errlen);
+/* probability text of pp files (format_prob, rna_data.cc:1274-1303) */
+int oracle_format_prob(double prob, char *buf, int cap);
 
 /* CPU baseline: align many pairs with nthreads worker threads (scores
  * only, with traceback).  Returns number of pairs done or <0. */

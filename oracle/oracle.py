@@ -265,3 +265,93 @@ def trace_probs(ppA, ppB, params=None):
     P = [[probs[i * (m + 1) + j] for j in range(m + 1)] for i in range(n + 1)]
     S = [[sim[i * (m + 1) + j] for j in range(m + 1)] for i in range(n + 1)]
     return P, S, (w[0], w[1], w[2])
+
+
+def consensus_pp(ppA, ppB, edges, params=None):
+    """pp text of the pair aligned by `edges` ((a, b), -1 for a gap), as
+    locarna --pp writes it (consensus base-pair probabilities)."""
+    L = load()
+    params = params or locarna_params()
+    n = len(edges)
+    ea = (ctypes.c_int * max(n, 1))(*[e[0] for e in edges])
+    eb = (ctypes.c_int * max(n, 1))(*[e[1] for e in edges])
+    err = ctypes.create_string_buffer(512)
+    L.oracle_consensus_pp.restype = ctypes.c_longlong
+    L.oracle_consensus_pp.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Params), ctypes.c_void_p,
+                                      ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_longlong,
+                                      ctypes.c_char_p, ctypes.c_int]
+    need = L.oracle_consensus_pp(ppA.encode(), ppB.encode(), ctypes.byref(params), ea, eb, n, None, 0, err, 512)
+    if need < 0:
+        raise RuntimeError("oracle_consensus_pp: " + err.value.decode())
+    buf = ctypes.create_string_buffer(need + 1)
+    L.oracle_consensus_pp(ppA.encode(), ppB.encode(), ctypes.byref(params), ea, eb, n, buf, need + 1, err, 512)
+    return buf.value.decode()
+
+
+def format_prob(p):
+    L = load()
+    buf = ctypes.create_string_buffer(64)
+    L.oracle_format_prob.restype = ctypes.c_int
+    L.oracle_format_prob.argtypes = [ctypes.c_double, ctypes.c_char_p, ctypes.c_int]
+    L.oracle_format_prob(p, buf, 64)
+    return buf.value.decode()
+
+
+def upgma(scores):
+    """Guide tree of mlocarna from a symmetric similarity matrix (lists of
+    lists; the diagonal is ignored and read as 0): MLocarna::Tree::_upgma
+    (lib/perl/MLocarna/Tree.pm:164-315).  Distances are max - score, max
+    over the upper triangle and the (zeroed) [0][0] entry; the closest pair
+    is the first strict minimum in the scan over the current cluster list,
+    whose order follows the Perl list updates.  Returns the merges as a
+    list of (child0, child1): leaves 0..n-1, merge t is node n + t."""
+    n = len(scores)
+    if n < 2:
+        return []
+    mx = 0
+    for i in range(n):
+        for j in range(i + 1, n):
+            mx = max(scores[i][j], mx)
+    dist = [[mx - (0 if i == j else scores[i][j]) for j in range(n)] for i in range(n)]
+    clusters = list(range(n))
+    size = [1] * n
+    node = list(range(n))  # cluster id -> tree node
+    merges = []
+    while len(clusters) > 1:
+        min_i = min_j = None
+        min_d = 1e10
+        for i in range(len(clusters)):
+            for j in range(i + 1, len(clusters)):
+                d = dist[clusters[i]][clusters[j]]
+                if d < min_d:
+                    min_i, min_j, min_d = i, j, d
+        ci, cj = clusters[min_i], clusters[min_j]
+        last = len(clusters) - 1
+        clusters[min_j] = clusters[last]
+        clusters[min_i] = clusters[0]
+        clusters[0] = ci
+        for i in range(1, last):
+            dist[clusters[0]][clusters[i]] = ((size[ci] * dist[ci][clusters[i]] + size[cj] * dist[cj][clusters[i]])
+                                              / (size[ci] + size[cj]))
+            dist[clusters[i]][clusters[0]] = dist[clusters[0]][clusters[i]]
+        merges.append((node[ci], node[cj]))
+        node[clusters[0]] = n + len(merges) - 1
+        size[clusters[0]] = size[ci] + size[cj]
+        clusters.pop()
+    return merges
+
+
+def progressive(pps, merges, params=None):
+    """mlocarna's progressive stage over the guide tree `merges`
+    (src/Utils/mlocarna:3660-3707, call_locarna): every merge aligns the
+    pp files of its two children and writes the consensus pp of the result
+    (locarna --pp); returns (final pp text, per-merge (score, edges))."""
+    params = params or locarna_params()
+    n = len(pps)
+    text = list(pps) + [None] * len(merges)
+    out = []
+    for t, (a, b) in enumerate(merges):
+        r = align(text[a], text[b], params)
+        out.append((r.score, r.edges))
+        text[n + t] = consensus_pp(text[a], text[b], r.edges, params)
+    return (text[-1] if merges else pps[0]), out

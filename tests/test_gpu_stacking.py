@@ -23,7 +23,12 @@ def _compare(pp_pairs, kw, shape=None):
         edges, sA, sB = B.alignment(k)
         assert edges == o.edges, k
         assert (sA, sB) == (o.structA, o.structB), k
-        assert B.D(k) == o.D, k
+        assert len(B.D(k)) == len(o.D), k
+        for t, (x, y) in enumerate(zip(B.D(k), o.D)):
+            if y < -(1 << 60):  # the oracle's minus infinity (64-bit)
+                assert x < -(1 << 28), (k, t)
+            else:
+                assert x == y, (k, t)
     B.destroy()
 
 

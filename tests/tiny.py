@@ -2,7 +2,7 @@
 arcs) written as #PP 2.0 text, for exhaustive checks."""
 import random
 
-from locarna_amd.synth import pp_text
+from locarna_amd.synth import pp_text, pp_text_rows
 
 
 def tiny_pp(rng, n, narcs, name="t"):
@@ -83,3 +83,37 @@ def tiny_stacking_pair(seed, nA=None, nB=None):
     nA = nA or rng.randint(6, 9)
     nB = nB or rng.randint(6, 9)
     return tiny_stacking_pp(rng, nA, "a"), tiny_stacking_pp(rng, nB, "b")
+
+
+def tiny_profile_pp(rng, n, nrows, narcs, name="p", gap_rate=0.25, n_rate=0.05):
+    """A tiny alignment of nrows rows over n columns (gaps '-', now and then
+    an N) with arcs on its columns; rows are named name0, name1, ..."""
+    rows = []
+    for r in range(nrows):
+        row = "".join("-" if rng.random() < gap_rate else ("N" if rng.random() < n_rate else rng.choice("ACGU"))
+                      for _ in range(n))
+        rows.append(("%s%d" % (name, r), row))
+    cand = [(i, j) for i in range(1, n + 1) for j in range(i + 3, n + 1)]
+    rng.shuffle(cand)
+    arcs = sorted(cand[:narcs])
+    pairs = [(i, j, round(rng.uniform(0.02, 0.95), 4)) for (i, j) in arcs]
+    return pp_text_rows(rows, pairs, bpcut=0.0005)
+
+
+def tiny_profile_pair(seed, rows=(1, 3), arcs=(1, 3)):
+    """Two tiny profiles (a single-row side now and then)."""
+    rng = random.Random(seed)
+    return (tiny_profile_pp(rng, rng.randint(4, 7), rng.randint(*rows), rng.randint(*arcs), "a"),
+            tiny_profile_pp(rng, rng.randint(4, 7), rng.randint(*rows), rng.randint(*arcs), "b"))
+
+
+def replicate_rows(pp, k):
+    """The pp text with its single sequence row repeated k times (rows
+    named name.0 .. name.k-1): a profile that scores as the sequence."""
+    lines = pp.split("\n")
+    end = lines.index("#END")
+    seqlines = [i for i in range(end) if lines[i] and not lines[i].startswith("#")]
+    assert len(seqlines) == 1
+    name, seq = lines[seqlines[0]].split()
+    rows = ["%s.%d %s" % (name, t, seq) for t in range(k)]
+    return "\n".join(lines[:seqlines[0]] + rows + lines[seqlines[0] + 1:])
