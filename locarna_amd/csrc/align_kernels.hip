@@ -876,13 +876,17 @@ __device__ __forceinline__ void fill_pairs(const PairDesc *__restrict__ descs, c
             if (g0 == g1) continue;
             const int nr = rounds[al];
             for (int rnd = 0; rnd < nr; ++rnd) {
-                for (int g = g0; g < g1; ++g) {
-                    const int *G = groups + g * LCG_GROUP_INTS;
-                    // (a wave holds at most one group per round; leaving the
-                    // scan after it measured slower: 53.5 against 49.0 ms on
-                    // 2016 pairs, the compiler then allocates 116 VGPRs
-                    // instead of 96)
-                    if (G[5] != (rnd | (wave << 12) | mine)) continue;
+                // the wave's group of this round (at most one): the lanes
+                // compare the schedule words of 64 groups at a time
+                const int want = rnd | (wave << 12) | mine;
+                int mg = -1;
+                for (int gb = g0; gb < g1 && mg < 0; gb += 64) {
+                    const int g = gb + lane;
+                    const unsigned long long hit = __ballot(g < g1 && groups[g * LCG_GROUP_INTS + 5] == want);
+                    if (hit) mg = gb + __ffsll((long long)hit) - 1;
+                }
+                if (mg >= 0) {
+                    const int *G = groups + mg * LCG_GROUP_INTS;
                     const int gal = G[0] & 0xffff, gbl = G[0] >> 16;
                     const int gar = G[1] & 0xffff, gbr = G[1] >> 16;
                     const int W = G[2];

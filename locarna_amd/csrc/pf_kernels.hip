@@ -771,6 +771,18 @@ __device__ __forceinline__ void zero_region_bar(const Region &R, const PfPair &P
 // matches has P(am) > sqrt(min_am_prob), the restricted one.
 #define LCG_PF_SPIN_LIMIT (1 << 22)
 
+// The group of rows g0..g1-1 whose schedule word (S[4 g + slot]) is `want`,
+// -1 if none: the lanes compare 64 words at a time (a wave holds at most
+// one group per round).
+__device__ __forceinline__ int find_group(const int *S, int slot, int g0, int g1, int want) {
+    for (int gb = g0; gb < g1; gb += 64) {
+        const int g = gb + lane_id();
+        const unsigned long long hit = __ballot(g < g1 && S[4 * g + slot] == want);
+        if (hit) return gb + __ffsll((long long)hit) - 1;
+    }
+    return -1;
+}
+
 extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *__restrict__ descs,
                                                                 const PfDesc *__restrict__ pdescs,
                                                                 const int *__restrict__ order, int *queue,
@@ -931,8 +943,8 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
             if (g0 == g1) continue;
             const int nr = sched[f.rin + al];
             for (int rnd = 0; rnd < nr; ++rnd) {
-                for (int g = g0; g < g1; ++g) {
-                    if (S[4 * g] != (rnd | (wave << 12) | mine)) continue;
+                const int g = find_group(S, 0, g0, g1, rnd | (wave << 12) | mine);
+                if (g >= 0) {
                     const int *G = groups + g * LCG_GROUP_INTS;
                     const int gal = G[0] & 0xffff, gbl = G[0] >> 16, gar = G[1] & 0xffff, gbr = G[1] >> 16;
                     const int W = G[2];
@@ -1018,8 +1030,8 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
             if (g0 == g1) continue;
             const int nr = sched[f.rout + al];
             for (int rnd = 0; rnd < nr; ++rnd) {
-                for (int g = g0; g < g1; ++g) {
-                    if (S[4 * g + 2] != (rnd | (wave << 12) | mine)) continue;
+                const int g = find_group(S, 2, g0, g1, rnd | (wave << 12) | mine);
+                if (g >= 0) {
                     const int *G = groups + g * LCG_GROUP_INTS;
                     const int gal = G[0] & 0xffff, gbl = G[0] >> 16, gar = G[1] & 0xffff, gbr = G[1] >> 16;
                     const int W = G[2];
