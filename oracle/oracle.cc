@@ -766,6 +766,8 @@ struct Inputs {
 std::unique_ptr<Inputs> prepare(const char *ppA, const char *ppB, const oracle_params &p, const char *rib_path) {
     if (p.sequ_local && p.free_endgaps)  // locarna.cc:375-378
         throw std::runtime_error("Free endgaps cannot be combined with local alignment.");
+    if (p.struct_local && (p.no_lonely_pairs || p.stacking))  // restated without these combinations
+        throw std::runtime_error("--struct-local with --noLP or stacking is not restated");
     auto in = std::make_unique<Inputs>();
     in->p = p;
     if (p.use_ribosum) in->rib.reset(new Ribosum(read_ribosum(rib_path)));
@@ -834,12 +836,27 @@ struct Aligner {
     std::string structA, structB;
     long long cnt_G = 0, cnt_C = 0, cnt_K = 0;
     bool counting = false;
+    // --struct-local (aligner.cc:118-128; states aligner_impl.hh:108-122):
+    // E = exclusion, NO = none, X = one (closed), OP = open; the first part
+    // refers to A, the second to B.  State 0 is M / E / F above.
+    enum { E_NO_NO, E_X_NO, E_NO_X, E_X_X, E_OP_NO, E_NO_OP, E_OP_X, E_X_OP };
+    Mat<sc_t> Mx[8];
+    std::vector<sc_t> Ex[4];
+    sc_t Fx[4] = {0, 0, 0, 0};
+    Mat<sc_t> &Ms(int k) { return k ? Mx[k] : M; }
+    std::vector<sc_t> &Es(int k) { return k ? Ex[k] : E; }
+    sc_t &Fs(int k) { return k ? Fx[k] : F; }
+    bool struct_local() const { return in.p.struct_local != 0; }
 
     explicit Aligner(Inputs &in_)
         : in(in_), tc(*in_.band), ams(*in_.am), sc(*in_.sc), bpsA(*in_.bpsA), bpsB(*in_.bpsB),
           lenA(in_.A.length()), lenB(in_.B.length()), alpha(in_.p.indel_opening) {
         M.resize(lenA + 1, lenB + 1, 0);
         E.assign(lenB + 1, 0);
+        if (in_.p.struct_local) {
+            for (int k = 1; k < 8; ++k) Mx[k].resize(lenA + 1, lenB + 1, 0);
+            for (int k = 1; k < 4; ++k) Ex[k].assign(lenB + 1, 0);
+        }
         Dmat.resize(bpsA.arcs.size(), bpsB.arcs.size(), NEG_INF);  // aligner.cc:122-123
         am_score.resize(ams.ams.size());
         for (size_t k = 0; k < ams.ams.size(); ++k) am_score[k] = sc.arcmatch(ams.ams[k]);
@@ -916,13 +933,101 @@ struct Aligner {
         }
     }
 
+    // aligner.cc:153-233 in state k < 4 (--struct-local), no position penalty
+    sc_t align_noex_s(int k, size_t al, size_t bl, size_t i, size_t j) {
+        Mat<sc_t> &Mk = Ms(k);
+        std::vector<sc_t> &Ek = Es(k);
+        sc_t &Fk = Fs(k);
+        sc_t max_score = Mk(i - 1, j - 1) + sc.sigma_tab(i, j);
+        Ek[j] = std::max(Ek[j] + sc.gapA_tab[i], Mk(i - 1, j) + sc.gapA_tab[i] + alpha);
+        max_score = std::max(max_score, Ek[j]);
+        Fk = std::max(Fk + sc.gapB_tab[j], Mk(i, j - 1) + sc.gapB_tab[j] + alpha);
+        max_score = std::max(max_score, Fk);
+        const auto &adjlA = bpsA.right_adj[i];
+        const auto &adjlB = bpsB.right_adj[j];
+        for (auto arcA = adjlA.begin(); (size_t)arcA->left > al; ++arcA)
+            for (auto arcB = adjlB.begin(); (size_t)arcB->left > bl; ++arcB) {
+                sc_t ns = Mk(arcA->left - 1, arcB->left - 1) + D(*arcA, *arcB);
+                if (ns <= max_score) continue;
+                max_score = ns;
+            }
+        return normalize(max_score);
+    }
+
+    // aligner.cc:264-368 in state k: the first column and row start at 0 (a
+    // free open exclusion) without the global flag of their sequence, at the
+    // exclusion score (a closed exclusion of the prefix) with its exclusion
+    // flag, else at the indel opening plus gap costs
+    void init_state_s(int k, size_t al, size_t ar, size_t bl, size_t br, bool globalA, bool exclA, bool globalB,
+                      bool exclB) {
+        if (k < 4)
+            for (size_t j = bl; j < br; j++) Es(k)[j] = NEG_INF;
+        Mat<sc_t> &Mk = Ms(k);
+        Mk(al, bl) = 0;
+        sc_t indel_score = exclA ? (sc_t)in.p.exclusion : (!globalA ? 0 : alpha);
+        size_t i;
+        for (i = al + 1; i < ar; i++) {
+            if (tc.min_col[i] > bl) break;
+            if (!is_neg_inf(indel_score) && !exclA && globalA) indel_score += sc.gapA_tab[i];
+            Mk(i, bl) = indel_score;
+        }
+        for (; i < ar; i++) Mk(i, tc.min_col[i] - 1) = NEG_INF;
+        indel_score = exclB ? (sc_t)in.p.exclusion : (!globalB ? 0 : alpha);
+        size_t j;
+        for (j = bl + 1; j < std::min(br, tc.max_col[al] + 1); j++) {
+            if (!is_neg_inf(indel_score) && !exclB && globalB) indel_score += sc.gapB_tab[j];
+            Mk(al, j) = indel_score;
+        }
+        for (i = al + 1; i < ar; i++)
+            for (; j < std::min(br, tc.max_col[i] + 1); ++j) Mk(i - 1, j) = NEG_INF;
+    }
+
+    // aligner.cc:373-565 with allow_exclusion: all eight states, in the
+    // reference's order, each over the whole region
+    void align_in_arcmatch_sl(size_t al, size_t ar, size_t bl, size_t br) {
+        init_state_s(E_NO_NO, al, ar, bl, br, true, false, true, false);
+        init_state_s(E_X_NO, al, ar, bl, br, true, true, true, false);
+        init_state_s(E_NO_X, al, ar, bl, br, true, false, true, true);
+        init_state_s(E_X_X, al, ar, bl, br, true, true, true, true);
+        init_state_s(E_OP_NO, al, ar, bl, br, false, false, true, false);
+        init_state_s(E_NO_OP, al, ar, bl, br, true, false, false, false);
+        init_state_s(E_X_OP, al, ar, bl, br, true, true, false, false);
+        init_state_s(E_OP_X, al, ar, bl, br, false, false, true, true);
+        const sc_t ex = in.p.exclusion;
+        auto rows = [&](auto cell, bool resetF, int k) {
+            for (size_t i = al + 1; i < ar; i++) {
+                if (resetF) Fs(k) = NEG_INF;
+                const size_t lo = std::max(bl + 1, tc.min_col[i]), hi = std::min(br - 1, tc.max_col[i]);
+                for (size_t j = lo; j <= hi; j++) cell(i, j);
+            }
+        };
+        rows([&](size_t i, size_t j) { M(i, j) = align_noex_s(E_NO_NO, al, bl, i, j); }, true, E_NO_NO);
+        rows([&](size_t i, size_t j) { Ms(E_OP_NO)(i, j) = std::max(Ms(E_OP_NO)(i - 1, j), M(i, j)); }, false, 0);
+        rows([&](size_t i, size_t j) { Ms(E_NO_OP)(i, j) = std::max(Ms(E_NO_OP)(i, j - 1), M(i, j)); }, false, 0);
+        rows([&](size_t i, size_t j) {
+            Ms(E_NO_X)(i, j) = normalize(std::max(align_noex_s(E_NO_X, al, bl, i, j), Ms(E_NO_OP)(i, j) + ex));
+        }, true, E_NO_X);
+        rows([&](size_t i, size_t j) { Ms(E_OP_X)(i, j) = std::max(Ms(E_OP_X)(i - 1, j), Ms(E_NO_X)(i, j)); }, false, 0);
+        rows([&](size_t i, size_t j) {
+            Ms(E_X_NO)(i, j) = normalize(std::max(align_noex_s(E_X_NO, al, bl, i, j), Ms(E_OP_NO)(i, j) + ex));
+        }, true, E_X_NO);
+        rows([&](size_t i, size_t j) { Ms(E_X_OP)(i, j) = std::max(Ms(E_X_OP)(i, j - 1), Ms(E_X_NO)(i, j)); }, false, 0);
+        rows([&](size_t i, size_t j) {
+            Ms(E_X_X)(i, j) = normalize(std::max(align_noex_s(E_X_X, al, bl, i, j),
+                                                 std::max(Ms(E_OP_X)(i, j) + ex, Ms(E_X_OP)(i, j) + ex)));
+        }, true, E_X_X);
+    }
+
     // aligner.cc:574-610
     // with stacking, an arc match whose inner arc match exists and whose
     // arcs are stackable may take the inner one's D plus its stacked score
     void fill_D_entries(int al, int bl) {
         for (int x : ams.left_lists(al, bl)) {
             const ArcMatch &am = ams.ams[x];
-            D(am) = M(am.a.right - 1, am.b.right - 1) + am_score[x];
+            sc_t m = M(am.a.right - 1, am.b.right - 1);
+            if (struct_local())  // the best of the non-open states (aligner.cc:586-592)
+                for (int k = 1; k < 4; k++) m = std::max(m, Ms(k)(am.a.right - 1, am.b.right - 1));
+            D(am) = m + am_score[x];
             if (sc.stacking() && ams.inner[x] >= 0 && am_stackable[x])
                 D(am) = std::max(D(am), D(ams.ams[ams.inner[x]]) + am_score_st[x]);
         }
@@ -962,7 +1067,10 @@ struct Aligner {
                     ams.max_right_ends((int)al, (int)bl, &max_ar, &max_br);
                 if ((int)al == max_ar || (int)bl == max_br) continue;
                 if (counting) cnt_G++;
-                align_in_arcmatch(al, max_ar, bl, max_br);
+                if (struct_local())
+                    align_in_arcmatch_sl(al, max_ar, bl, max_br);
+                else
+                    align_in_arcmatch(al, max_ar, bl, max_br);
                 if (noLP)
                     fill_D_entries_noLP((int)al, (int)bl);
                 else
@@ -1104,8 +1212,130 @@ struct Aligner {
             append(in_am.a.right, in_am.b.right);
             return;
         }
+        if (struct_local()) {
+            // aligner.cc:1012-1025: the first non-open state whose value
+            // gives D(am)
+            align_in_arcmatch_sl(am.a.left, am.a.right, am.b.left, am.b.right);
+            for (int k = 0; k < 4; k++)
+                if (D(am) == Ms(k)(am.a.right - 1, am.b.right - 1) + am_score[am.idx]) {
+                    trace_in_arcmatch_s(k, am.a.left, am.a.right - 1, am.b.left, am.b.right - 1);
+                    break;
+                }
+            return;
+        }
         align_in_arcmatch(am.a.left, am.a.right, am.b.left, am.b.right);
         trace_in_arcmatch(am.a.left, am.a.right - 1, am.b.left, am.b.right - 1, false);
+    }
+
+    // aligner.cc:1233-1343 inside an arc match in state k (--struct-local):
+    // excluded positions get no edge; a prefix run out in a sequence with an
+    // exclusion (open or closed) is excluded rather than padded with gaps
+    void trace_in_arcmatch_s(int k, size_t al, size_t i, size_t bl, size_t j) {
+        Mat<sc_t> &Mk = Ms(k);
+        const sc_t ex = in.p.exclusion;
+        if (i <= al) {
+            if (k == E_NO_NO || k == E_OP_NO || k == E_X_NO)
+                for (size_t t = bl + 1; t <= j; t++) append(-1, (int)t);
+            return;
+        }
+        if (j <= bl) {
+            if (k == E_NO_NO || k == E_NO_OP || k == E_NO_X)
+                for (size_t t = al + 1; t <= i; t++) append((int)t, -1);
+            return;
+        }
+        switch (k) {
+            case E_NO_NO: trace_noex_s(k, al, i, bl, j); break;
+            case E_OP_NO:
+                if (Mk(i, j) == Mk(i - 1, j)) trace_in_arcmatch_s(k, al, i - 1, bl, j);
+                else if (Mk(i, j) == M(i, j)) trace_in_arcmatch_s(E_NO_NO, al, i, bl, j);
+                break;
+            case E_NO_OP:
+                if (Mk(i, j) == Mk(i, j - 1)) trace_in_arcmatch_s(k, al, i, bl, j - 1);
+                else if (Mk(i, j) == M(i, j)) trace_in_arcmatch_s(E_NO_NO, al, i, bl, j);
+                break;
+            case E_NO_X:
+                if (Mk(i, j) == Ms(E_NO_OP)(i, j) + ex) trace_in_arcmatch_s(E_NO_OP, al, i, bl, j);
+                else trace_noex_s(k, al, i, bl, j);
+                break;
+            case E_OP_X:
+                if (Mk(i, j) == Mk(i - 1, j)) trace_in_arcmatch_s(k, al, i - 1, bl, j);
+                else if (Mk(i, j) == Ms(E_NO_X)(i, j)) trace_in_arcmatch_s(E_NO_X, al, i, bl, j);
+                break;
+            case E_X_NO:
+                if (Mk(i, j) == Ms(E_OP_NO)(i, j) + ex) trace_in_arcmatch_s(E_OP_NO, al, i, bl, j);
+                else trace_noex_s(k, al, i, bl, j);
+                break;
+            case E_X_OP:
+                if (Mk(i, j) == Mk(i, j - 1)) trace_in_arcmatch_s(k, al, i, bl, j - 1);
+                else if (Mk(i, j) == Ms(E_X_NO)(i, j)) trace_in_arcmatch_s(E_X_NO, al, i, bl, j);
+                break;
+            case E_X_X:
+                if (Mk(i, j) == Ms(E_OP_X)(i, j) + ex) trace_in_arcmatch_s(E_OP_X, al, i, bl, j);
+                else if (Mk(i, j) == Ms(E_X_OP)(i, j) + ex) trace_in_arcmatch_s(E_X_OP, al, i, bl, j);
+                else trace_noex_s(k, al, i, bl, j);
+                break;
+        }
+    }
+
+    // aligner.cc:1082-1229 in state k < 4 inside an arc match (no position
+    // penalty, not the top level)
+    void trace_noex_s(int k, size_t oal, size_t i, size_t obl, size_t j) {
+        Mat<sc_t> &Mk = Ms(k);
+        if (tc.is_valid(i - 1, j - 1) && Mk(i, j) == Mk(i - 1, j - 1) + sc.sigma_tab(i, j)) {
+            trace_in_arcmatch_s(k, oal, i - 1, obl, j - 1);
+            append((int)i, (int)j);
+            return;
+        }
+        if (alpha == 0) {
+            if (tc.is_valid(i - 1, j) && Mk(i, j) == Mk(i - 1, j) + sc.gapA_tab[i]) {
+                trace_in_arcmatch_s(k, oal, i - 1, obl, j);
+                append((int)i, -1);
+                return;
+            }
+            if (tc.is_valid(i, j - 1) && Mk(i, j) == Mk(i, j - 1) + sc.gapB_tab[j]) {
+                trace_in_arcmatch_s(k, oal, i, obl, j - 1);
+                append(-1, (int)j);
+                return;
+            }
+        } else {
+            sc_t gap_cost = alpha;
+            for (size_t t = 1; i >= oal + t; t++) {
+                if (!tc.is_valid(i - t, j)) break;
+                gap_cost += sc.gapA_tab[i - t + 1];
+                if (Mk(i, j) == Mk(i - t, j) + gap_cost) {
+                    trace_in_arcmatch_s(k, oal, i - t, obl, j);
+                    for (size_t l = t; l > 0; l--) append((int)(i - l + 1), -1);
+                    return;
+                }
+            }
+            gap_cost = alpha;
+            for (size_t t = 1; j >= obl + t; t++) {
+                if (!tc.is_valid(i, j - t)) break;
+                gap_cost += sc.gapB_tab[j - t + 1];
+                if (Mk(i, j) == Mk(i, j - t) + gap_cost) {
+                    trace_in_arcmatch_s(k, oal, i, obl, j - t);
+                    for (size_t l = t; l > 0; l--) append(-1, (int)(j - l + 1));
+                    return;
+                }
+            }
+        }
+        if (!tc.is_valid(i - 1, j - 1)) return;
+        for (int x : ams.right_lists(i, j)) {
+            const ArcMatch &am = ams.ams[x];
+            if ((size_t)am.a.left <= oal || (size_t)am.b.left <= obl) continue;
+            size_t al = am.a.left, bl = am.b.left;
+            if (Mk(i, j) == Mk(al - 1, bl - 1) + D(am)) {
+                trace_in_arcmatch_s(k, oal, al - 1, obl, bl - 1);
+                structA[al - 1] = '(';
+                structA[i - 1] = ')';
+                structB[bl - 1] = '(';
+                structB[j - 1] = ')';
+                append((int)al, (int)bl);
+                trace_arcmatch(am);
+                append((int)i, (int)j);
+                return;
+            }
+        }
     }
 
     // aligner.cc:1030-1078 (no stacking, not structure-local): the inner

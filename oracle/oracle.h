@@ -86,6 +86,10 @@ typedef struct {
     int stacking;          /* integer path: bit 0 --stacking, bit 1
                               --new-stacking (stacked arc-match scores from
                               the #STACK joint probabilities)         */
+    int struct_local;      /* --struct-local (integer path): at most one
+                              excluded block per sequence inside every
+                              arc match                               */
+    int exclusion;         /* --exclusion: score of one exclusion      */
 } oracle_params;
 
 typedef struct oracle_result oracle_result;

@@ -25,6 +25,7 @@ class Params(ctypes.Structure):
         ("exp_prob", ctypes.c_double), ("pf_scale", ctypes.c_double),
         ("min_am_prob", ctypes.c_double), ("min_bm_prob", ctypes.c_double),
         ("no_lonely_pairs", ctypes.c_int), ("stacking", ctypes.c_int),
+        ("struct_local", ctypes.c_int), ("exclusion", ctypes.c_int),
     ]
 
 

@@ -22,8 +22,17 @@ import math
 
 class Problem:
     def __init__(self, lenA, lenB, sigma, gapA, gapB, alpha, arcmatches, am_score,
-                 min_col, max_col, local=False, free=0, no_lonely_pairs=False, am_score_stacked=None):
+                 min_col, max_col, local=False, free=0, no_lonely_pairs=False, am_score_stacked=None,
+                 struct_local=False, exclusion=0):
         self.n, self.m = lenA, lenB
+        # --struct-local: inside every arc match (not at the top level) the
+        # alignment may exclude one contiguous block of A positions and one
+        # of B positions; an excluded block costs `exclusion`, pays no gap
+        # costs, ends any gap run it touches and takes no edges.  (Modelled
+        # for gap costs <= 0 and exclusion <= 0, where excluding a deleted
+        # prefix is never worse and an empty block never helps.)
+        self.struct_local = struct_local
+        self.exclusion = exclusion
         # --stacking: the stacked score of each stackable arc match (None
         # elsewhere); an arc match whose interior is exactly its inner arc
         # match (al+1, ar-1, bl+1, br-1) scores the better of its two scores
@@ -81,11 +90,62 @@ def _live(P, r0, c0, i, j):
     return False
 
 
+def _sl_alignments(P, r0, c0, i, j, memo):
+    """Structure-local alignments of the arc-match region with origin (r0,
+    c0) ending at (i, j): list of (score, steps, last, usedA, usedB); steps
+    as group_alignments plus ('XA', k, i, j) / ('XB', k, i, j) for an
+    excluded block of k positions of A / B ending at (i, j)."""
+    key = ('sl', r0, c0, i, j)
+    if key in memo:
+        return memo[key]
+    out = []
+    ok = lambda x, y: (x, y) == (r0, c0) or P.valid(x, y)
+    if (i, j) == (r0, c0):
+        out.append((0, (), None, False, False))
+    elif ok(i, j):
+        if i > r0 and j > c0 and ok(i - 1, j - 1):
+            for sc, st, _, ua, ub in _sl_alignments(P, r0, c0, i - 1, j - 1, memo):
+                out.append((sc + P.sigma[(i, j)], (('M', i, j),) + st, 'M', ua, ub))
+        for kind, di, dj in (('D', 1, 0), ('I', 0, 1)):
+            k = 1
+            while i - k * di >= r0 and j - k * dj >= c0 and ok(i - k * di, j - k * dj):
+                x, y = i - k * di, j - k * dj
+                cost = P.alpha + (sum(P.gapA[r] for r in range(x + 1, i + 1)) if di else
+                                  sum(P.gapB[c] for c in range(y + 1, j + 1)))
+                for sc, st, last, ua, ub in _sl_alignments(P, r0, c0, x, y, memo):
+                    if last != kind:
+                        out.append((sc + cost, ((kind, k, i, j),) + st, kind, ua, ub))
+                k += 1
+        for kind, di, dj in (('XA', 1, 0), ('XB', 0, 1)):
+            k = 1
+            while i - k * di >= r0 and j - k * dj >= c0 and ok(i - k * di, j - k * dj):
+                x, y = i - k * di, j - k * dj
+                for sc, st, last, ua, ub in _sl_alignments(P, r0, c0, x, y, memo):
+                    if (ua if kind == 'XA' else ub):
+                        continue
+                    out.append((sc + P.exclusion, ((kind, k, i, j),) + st, kind,
+                                ua or kind == 'XA', ub or kind == 'XB'))
+                k += 1
+        if i > r0 and j > c0:
+            for rank, (al, bl, kk) in enumerate(P.by_right.get((i, j), [])):
+                if al <= r0 or bl <= c0:
+                    continue
+                inner = group_alignments(P, al, bl, i - 1, j - 1, False, memo)
+                for isc, ist, _ in inner:
+                    for psc, pst, _, ua, ub in _sl_alignments(P, r0, c0, al - 1, bl - 1, memo):
+                        out.append((psc + isc + P.am_score[kk], (('A', rank, kk, ist),) + pst, 'A', ua, ub))
+    memo[key] = out
+    return out
+
+
 def group_alignments(P, r0, c0, i, j, top, memo):
     """All structure alignments of the group with origin (r0, c0) ending at
     cell (i, j): list of (score, steps, last_edge_type)."""
     key = (r0, c0, i, j, top)
     if key in memo:
+        return memo[key]
+    if P.struct_local and not top:
+        memo[key] = [(sc, st, last) for sc, st, last, _, _ in _sl_alignments(P, r0, c0, i, j, memo)]
         return memo[key]
     out = []
     free_row = top and (P.local or P.free & 1)  # insertions before the first position of A
