@@ -74,6 +74,14 @@ typedef struct {
                                   scoring.cc:200-236); integer path with host
                                   arc matches and pp input (not arc-match
                                   score files); LocARNA-P refuses it        */
+    int32_t struct_local;      /* --struct-local: inside every arc match one
+                                  block of A and one of B positions may be
+                                  excluded (aligner.cc:373-565, eight
+                                  states; trace :1233-1343); integer path
+                                  with host arc matches, single sequences,
+                                  not with --noLP or stacking             */
+    int32_t exclusion;         /* --exclusion: score of one excluded block
+                                  (ScoringParams::exclusion, scoring.hh:107) */
 } lcg_params;
 
 /* free end gaps (FreeEndgaps, free_endgaps.hh: the characters of the

@@ -24,8 +24,8 @@
 // Scoring, ArcMatches, TraceController), this facade passes what they are
 // built from -- the #PP 2.0 texts (RnaData), the ScoringParams fields and
 // the band options -- and the library builds them (lcg_pair_prepare).
-// Features the library does not implement (struct_local, anchor
-// constraints; no_lonely_pairs and stacking for AlignerP) fail with
+// Features the library does not implement (anchor constraints;
+// no_lonely_pairs, stacking and struct_local for AlignerP) fail with
 // failure(LCG_E_ARG) when asked for.
 //
 // Header only, over the C ABI of locarna_amd.h; link -llocarna_amd.
@@ -111,6 +111,7 @@ struct ScoringParams {
     // (aligner.cc:600, :629, :987)
     bool stacking = false;
     bool new_stacking = false;
+    int exclusion = 0;  // score of one excluded block with struct_local (scoring.hh:107)
 };
 
 // Base-pair and band options (the RnaData / TraceController arguments of
@@ -154,7 +155,7 @@ public:
     const RnaData *seqA_ = nullptr, *seqB_ = nullptr;
     ScoringParams scoring_;
     BandParams band_;
-    bool sequ_local_ = false, do_trace_ = true, no_lonely_pairs_ = false, stacking_ = false;
+    bool sequ_local_ = false, do_trace_ = true, no_lonely_pairs_ = false, stacking_ = false, struct_local_ = false;
     std::string free_endgaps_ = "----";
     int max_diff_am_ = -1, max_diff_at_am_ = -1, device_ = 0;
 
@@ -188,6 +189,8 @@ public:
         p.min_bm_prob = 0.0;
         p.no_lonely_pairs = no_lonely_pairs_ ? 1 : 0;
         p.stacking = (scoring_.stacking ? 1 : 0) | (scoring_.new_stacking ? 2 : 0);
+        p.struct_local = struct_local_ ? 1 : 0;
+        p.exclusion = scoring_.exclusion;
         return p;
     }
 
@@ -198,7 +201,7 @@ protected:
     void set(const scoring &a) { scoring_ = *a.value; }
     void set(const trace_controller &a) { band_ = *a.value; }
     void set(const no_lonely_pairs &a) { no_lonely_pairs_ = a.value; }
-    void set(const struct_local &a) { unsupported(a.value, "struct_local"); }
+    void set(const struct_local &a) { struct_local_ = a.value; }
     // kept as the reference keeps it (aligner_params.hh:63); the stacked
     // recursion follows ScoringParams::stacking / new_stacking
     void set(const stacking &a) { stacking_ = a.value; }
@@ -436,6 +439,7 @@ public:
         // locarna_p scores without stacking terms (src/locarna_p.cc:310
         // reads the joint probabilities only)
         if (p.stacking) throw failure(LCG_E_ARG, "stacking: integer alignment only");
+        if (p.struct_local) throw failure(LCG_E_ARG, "struct_local: integer alignment only");
         auto pb = detail::make_pair_batch(ap, p);
         pair_ = std::move(pb.first);
         batch_ = std::move(pb.second);

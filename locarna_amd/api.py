@@ -20,6 +20,7 @@ class Params(ctypes.Structure):
         ("exp_prob", ctypes.c_double), ("pf_scale", ctypes.c_double),
         ("min_am_prob", ctypes.c_double), ("min_bm_prob", ctypes.c_double),
         ("no_lonely_pairs", ctypes.c_int32), ("stacking", ctypes.c_int32),
+        ("struct_local", ctypes.c_int32), ("exclusion", ctypes.c_int32),
     ]
 
 

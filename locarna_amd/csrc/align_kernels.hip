@@ -117,6 +117,7 @@ struct Pair {
     int n, m, nam;
     int alpha, alpha_pos;
     int nolp, stacking;
+    int struct_local, exclusion;  // --struct-local, --exclusion
 };
 
 // Base-match score sigma(i, j): by symbol codes from the LDS table, or (TAB,
@@ -734,6 +735,360 @@ __device__ __forceinline__ void group_trace(const Pair &P, const int *Mst, int W
     }
 }
 
+
+// ---- structure-local alignment (--struct-local; aligner.cc:118-128,
+// 264-565, 586-592, 1012-1025, 1233-1343).  Inside an arc match the
+// alignment may exclude one block of A and one of B positions, each at the
+// exclusion score: eight matrices, E = exclusion, NO = none, X = one
+// (closed), OP = one open; the first part refers to A, the second to B.
+// D takes the best non-open state at (ar-1, br-1); the top level is the
+// plain recursion.  A group's region lives in global scratch (one region
+// per wave): [E: br-bl] [S: W] [M of state 0 .. 7: (ar-al) W each], rows
+// stored as in group_fill.
+enum { SL_NO_NO, SL_X_NO, SL_NO_X, SL_X_X, SL_OP_NO, SL_NO_OP, SL_OP_X, SL_X_OP };
+
+__device__ __forceinline__ int *sl_M(int *base, int al, int bl, int ar, int br, int W, int k) {
+    return base + (br - bl) + W + k * (ar - al) * W;
+}
+
+// init_state (aligner.cc:264-368) of one state's matrix: origin 0; the
+// first row and column by mode: 0 zero (an exclusion open from the origin,
+// no global flag), 1 indel opening plus gap costs, 2 the exclusion score
+// (the prefix excluded)
+__device__ void sl_init(const Pair &P, int *Mst, int al, int bl, int ar, int br, int W, int colmode, int rowmode) {
+    const int lane = lane_id();
+    const int hi0 = row_hi(P, al, br);
+    int carry = rowmode == 1 ? P.alpha : rowmode == 2 ? P.exclusion : 0;
+    for (int j0 = bl + 1; j0 <= hi0; j0 += 64) {
+        const int j = j0 + lane;
+        const bool act = j <= hi0;
+        const int ps = wave_scan_add((act && rowmode == 1) ? P.gapB[j] : 0);
+        if (act) Mst[j - bl] = carry + ps;
+        carry += readlane(ps, 63);
+    }
+    if (lane == 0) {
+        Mst[0] = 0;
+        int colv = colmode == 1 ? P.alpha : colmode == 2 ? P.exclusion : 0;
+        for (int i = al + 1; i < ar && P.mn[i] <= bl; ++i) {
+            if (colmode == 1) colv += P.gapA[i];
+            Mst[(i - al) * W] = colv;  // rows reaching column bl are stored from bl
+        }
+    }
+}
+
+// value of the init column at row i (its mode), valid where mn(i) <= bl
+__device__ __forceinline__ int sl_colv(const Pair &P, int al, int i, int colmode) {
+    if (colmode == 0) return 0;
+    if (colmode == 2) return P.exclusion;
+    int v = P.alpha;
+    for (int x = al + 1; x <= i; ++x) v += P.gapA[x];
+    return v;
+}
+
+// One state k < 4: the recursion of group_fill over M_k (align_noex,
+// aligner.cc:153-233), raised to X1 + exclusion and X2 + exclusion where
+// given (the closing of an open exclusion, aligner.cc:476-563)
+template <bool TAB>
+__device__ void sl_fill_state(const Pair &P, int al, int bl, int ar, int br, int W, int *base, int k, int colmode,
+                              int rowmode, const int *X1, const int *X2) {
+    const int lane = lane_id();
+    const int alpha = P.alpha, ex = P.exclusion;
+    int *Est = base;
+    int *Srow = base + (br - bl);
+    int *Mst = sl_M(base, al, bl, ar, br, W, k);
+    for (int c = lane; c < br - bl; c += 64) Est[c] = LCG_NEG;
+    sl_init(P, Mst, al, bl, ar, br, W, colmode, rowmode);
+    region_sync<true>();
+    const int hi0 = row_hi(P, al, br);
+    int lo_p = bl, hi_p = hi0;
+    int colv = sl_colv(P, al, al, colmode);
+    bool colopen = true;
+    int cprev = 0;  // M(i-1, bl) where row i-1 reaches column bl; row al: the origin
+    bool cprev_ok = true;
+    for (int i = al + 1; i < ar; ++i) {
+        const int r = i - al;
+        int *row = Mst + r * W;
+        const int *prow = row - W;
+        const int mni = P.mn[i];
+        if (colopen && mni <= bl) {
+            if (colmode == 1) colv += P.gapA[i];
+        } else {
+            colopen = false;
+        }
+        const int lo = max(bl, mni), hi = min(br - 1, P.mx[i]);
+        const int jlo = max(bl + 1, mni);
+        if (jlo <= hi) {
+            for (int c = lane; c <= hi - jlo; c += 64) Srow[c] = LCG_NEG;
+            region_sync<true>();
+            const int cb0 = P.coff[i] - P.mn[i];
+            const int t0 = P.cptr[cb0 + jlo], t1 = P.cptr[cb0 + hi + 1];
+            for (int t = t0 + lane; t < t1; t += 64) {
+                const int4 e = P.cand[t];
+                const int p = e.x & 0xffff, q = e.x >> 16;
+                const int rl = (p - 1 == al) ? bl : max(bl, e.w);
+                if (p > al && q > bl && q - 1 >= rl) atomicMax(&Srow[e.y - jlo], Mst[(p - 1 - al) * W + (q - 1 - rl)] + e.z);
+            }
+            region_sync<true>();
+            int Fcarry = LCG_NEG;
+            int sprev = colopen ? colv : LCG_NEG;  // M(i, jlo - 1)
+            const int gA = P.gapA[i];
+            const int *sigrow = sig_row<TAB>(P, i);
+            for (int j0 = jlo; j0 <= hi; j0 += 64) {
+                const int j = j0 + lane;
+                const bool act = j <= hi;
+                int s = LCG_NEG, gB = 0;
+                if (act) {
+                    const int up = (j >= lo_p && j <= hi_p) ? prow[j - lo_p] : LCG_NEG;
+                    const int dg = (j - 1 >= lo_p && j - 1 <= hi_p) ? prow[j - 1 - lo_p]
+                                   : (j - 1 == bl && cprev_ok)      ? cprev
+                                                                    : LCG_NEG;
+                    const int e = clampneg(max(Est[j - bl] + gA, up + gA + alpha));
+                    Est[j - bl] = e;
+                    s = clampneg(max(max(dg + sigrow[sig_col<TAB>(P, j)], e), Srow[j - jlo]));
+                    if (X1) s = clampneg(max(s, X1[r * W + (j - lo)] + ex));
+                    if (X2) s = clampneg(max(s, X2[r * W + (j - lo)] + ex));
+                    gB = P.gapB[j];
+                }
+                const int sl = wave_shift_up(s, sprev);
+                const int a = act ? gB + P.alpha_pos : 0;
+                const int c = act ? sl + gB + alpha : LCG_NEG;
+                const int Pa = wave_scan_add(a);
+                const int px = wave_scan_max(act ? c - Pa : LCG_NEG);
+                const int F = Pa + max(Fcarry, px);
+                if (act) row[j - lo] = clampneg(max(s, F));
+                Fcarry = clampneg(readlane(F, 63));
+                sprev = readlane(s, 63);
+            }
+        }
+        lo_p = lo;
+        hi_p = hi;
+        cprev = colv;
+        cprev_ok = colopen;
+        region_sync<true>();
+    }
+}
+
+// An open state: M(i, j) = max(M(i-1, j), Src(i, j)) (vertical: an open
+// exclusion of A, aligner.cc:443-460) or max(M(i, j-1), Src(i, j))
+// (horizontal: of B, :462-479)
+__device__ void sl_chain(const Pair &P, int al, int bl, int ar, int br, int W, int *Mst, const int *Src, bool vertical,
+                         int colmode, int rowmode) {
+    const int lane = lane_id();
+    sl_init(P, Mst, al, bl, ar, br, W, colmode, rowmode);
+    region_sync<true>();
+    const int hi0 = row_hi(P, al, br);
+    int lo_p = bl, hi_p = hi0;
+    int colv = sl_colv(P, al, al, colmode);
+    for (int i = al + 1; i < ar; ++i) {
+        const int r = i - al;
+        int *row = Mst + r * W;
+        const int *prow = row - W;
+        const int *srow = Src + r * W;
+        const int mni = P.mn[i];
+        if (mni <= bl && colmode == 1) colv += P.gapA[i];
+        const int lo = max(bl, mni), hi = min(br - 1, P.mx[i]);
+        const int jlo = max(bl + 1, mni);
+        int carry = mni <= bl ? colv : LCG_NEG;  // M(i, jlo - 1)
+        for (int j0 = jlo; j0 <= hi; j0 += 64) {
+            const int j = j0 + lane;
+            const bool act = j <= hi;
+            const int sv = act ? srow[j - lo] : LCG_NEG;
+            if (vertical) {
+                const int up = (act && j >= lo_p && j <= hi_p) ? prow[j - lo_p] : LCG_NEG;
+                if (act) row[j - lo] = max(up, sv);
+            } else {
+                const int v = max(carry, wave_scan_max(sv));
+                if (act) row[j - lo] = v;
+                carry = readlane(v, 63);
+            }
+        }
+        lo_p = lo;
+        hi_p = hi;
+        region_sync<true>();
+    }
+}
+
+// all eight states of a group, in the reference's order (aligner.cc:409-563)
+template <bool TAB>
+__device__ void group_fill_sl(const Pair &P, int al, int bl, int ar, int br, int W, int *base) {
+    int *M0 = sl_M(base, al, bl, ar, br, W, SL_NO_NO);
+    int *Mxn = sl_M(base, al, bl, ar, br, W, SL_X_NO), *Mnx = sl_M(base, al, bl, ar, br, W, SL_NO_X);
+    int *Mon = sl_M(base, al, bl, ar, br, W, SL_OP_NO), *Mno = sl_M(base, al, bl, ar, br, W, SL_NO_OP);
+    int *Mox = sl_M(base, al, bl, ar, br, W, SL_OP_X), *Mxo = sl_M(base, al, bl, ar, br, W, SL_X_OP);
+    sl_fill_state<TAB>(P, al, bl, ar, br, W, base, SL_NO_NO, 1, 1, nullptr, nullptr);
+    sl_chain(P, al, bl, ar, br, W, Mon, M0, true, 0, 1);
+    sl_chain(P, al, bl, ar, br, W, Mno, M0, false, 1, 0);
+    sl_fill_state<TAB>(P, al, bl, ar, br, W, base, SL_NO_X, 1, 2, Mno, nullptr);
+    sl_chain(P, al, bl, ar, br, W, Mox, Mnx, true, 0, 2);
+    sl_fill_state<TAB>(P, al, bl, ar, br, W, base, SL_X_NO, 2, 1, Mon, nullptr);
+    sl_chain(P, al, bl, ar, br, W, Mxo, Mxn, false, 2, 0);
+    sl_fill_state<TAB>(P, al, bl, ar, br, W, base, SL_X_X, 2, 2, Mox, Mxo);
+}
+
+// best non-open state value at (i, j) and the first state holding it
+// (fill_D_entries, aligner.cc:586-592; trace_arcmatch, :1015-1024)
+__device__ __forceinline__ int sl_best(const Pair &P, int *base, int al, int bl, int ar, int br, int W, int i, int j,
+                                       int *state) {
+    int best = INT_MIN, k0 = 0;
+    for (int k = 0; k < 4; ++k) {
+        const int v = Mget(P, sl_M(base, al, bl, ar, br, W, k), W, al, bl, br, i, j);
+        if (v > best) {
+            best = v;
+            k0 = k;
+        }
+    }
+    if (state) *state = k0;
+    return best;
+}
+
+// Traceback of a structure-local arc match's region from (ar-1, br-1)
+// (trace_in_arcmatch / trace_noex in states, aligner.cc:1082-1343):
+// excluded positions emit no edge; arc matches found go to the work list
+template <bool TAB>
+__device__ void group_trace_sl(const Pair &P, int *base, int W, int al, int bl, int ar, int br, int *tstack,
+                               int *tcount) {
+    const int lane = lane_id();
+    const int ex = P.exclusion;
+    int i = ar - 1, j = br - 1, k = 0;
+    sl_best(P, base, al, bl, ar, br, W, i, j, &k);
+    region_sync<true>();
+    auto Mk = [&](int s, int x, int y) { return Mget(P, sl_M(base, al, bl, ar, br, W, s), W, al, bl, br, x, y); };
+    for (int step = 0; step < 4 * (ar - al + br - bl) + 16; ++step) {
+        if (i <= al) {
+            if (k == SL_NO_NO || k == SL_OP_NO || k == SL_X_NO)
+                for (int t = bl + 1 + lane; t <= j; t += 64) emit(P, 0, t, al + t);
+            break;
+        }
+        if (j <= bl) {
+            if (k == SL_NO_NO || k == SL_NO_OP || k == SL_NO_X)
+                for (int t = al + 1 + lane; t <= i; t += 64) emit(P, t, 0, t + bl);
+            break;
+        }
+        const int Mij = Mk(k, i, j);
+        if (k == SL_OP_NO || k == SL_OP_X) {  // open exclusion of A
+            if (Mij == Mk(k, i - 1, j)) {
+                --i;
+            } else {
+                k = (k == SL_OP_NO) ? SL_NO_NO : SL_NO_X;
+            }
+            continue;
+        }
+        if (k == SL_NO_OP || k == SL_X_OP) {  // open exclusion of B
+            if (Mij == Mk(k, i, j - 1)) {
+                --j;
+            } else {
+                k = (k == SL_NO_OP) ? SL_NO_NO : SL_X_NO;
+            }
+            continue;
+        }
+        if (k == SL_NO_X && Mij == Mk(SL_NO_OP, i, j) + ex) {
+            k = SL_NO_OP;
+            continue;
+        }
+        if (k == SL_X_NO && Mij == Mk(SL_OP_NO, i, j) + ex) {
+            k = SL_OP_NO;
+            continue;
+        }
+        if (k == SL_X_X) {
+            if (Mij == Mk(SL_OP_X, i, j) + ex) {
+                k = SL_OP_X;
+                continue;
+            }
+            if (Mij == Mk(SL_X_OP, i, j) + ex) {
+                k = SL_X_OP;
+                continue;
+            }
+        }
+        // trace_noex in state k
+        if (valid(P, i - 1, j - 1) && Mij == Mk(k, i - 1, j - 1) + sig_row<TAB>(P, i)[sig_col<TAB>(P, j)]) {
+            if (lane == 0) emit(P, i, j, i + j);
+            --i;
+            --j;
+            continue;
+        }
+        bool done = false;
+        if (P.alpha != 0) {
+            for (int dir = 0; dir < 2 && !done; ++dir) {
+                int carry = P.alpha;
+                const int lim = dir == 0 ? i - al : j - bl;
+                for (int b0 = 1; !done && b0 <= lim; b0 += 64) {
+                    const int t = b0 + lane;
+                    const bool inr = t <= lim && (dir == 0 ? valid(P, i - t, j) : valid(P, i, j - t));
+                    const int g = inr ? (dir == 0 ? P.gapA[i - t + 1] : P.gapB[j - t + 1]) : 0;
+                    const int cost = carry + wave_scan_add(g);
+                    const bool hit = inr && Mij == (dir == 0 ? Mk(k, i - t, j) : Mk(k, i, j - t)) + cost;
+                    const unsigned long long bh = __ballot(hit);
+                    const unsigned long long nv = ~__ballot(inr);
+                    const unsigned long long valid_prefix = nv ? (nv & (0ull - nv)) - 1 : ~0ull;
+                    const unsigned long long h = bh & valid_prefix;
+                    if (h) {
+                        const int kk = b0 + (int)__ffsll((long long)h) - 1;
+                        if (dir == 0) {
+                            for (int u = i - kk + 1 + lane; u <= i; u += 64) emit(P, u, 0, u + j);
+                            i -= kk;
+                        } else {
+                            for (int u = j - kk + 1 + lane; u <= j; u += 64) emit(P, 0, u, i + u);
+                            j -= kk;
+                        }
+                        done = true;
+                    } else if (valid_prefix != ~0ull) {
+                        break;
+                    }
+                    carry = readlane(cost, 63);
+                }
+            }
+        } else {
+            if (valid(P, i - 1, j) && Mij == Mk(k, i - 1, j) + P.gapA[i]) {
+                if (lane == 0) emit(P, i, 0, i + j);
+                --i;
+                done = true;
+            } else if (valid(P, i, j - 1) && Mij == Mk(k, i, j - 1) + P.gapB[j]) {
+                if (lane == 0) emit(P, 0, j, i + j);
+                --j;
+                done = true;
+            }
+        }
+        if (done) continue;
+        if (!valid(P, i - 1, j - 1) || !valid(P, i, j)) break;
+        const int cell = P.coff[i] + (j - P.mn[i]);
+        const int t0 = P.cptr[cell], t1 = P.cptr[cell + 1];
+        bool found = false;
+        int fp = 0, fq = 0;
+        for (int b0 = t0; b0 < t1 && !found; b0 += 64) {
+            const int t = b0 + lane;
+            bool hit = false;
+            int p = 0, q = 0;
+            if (t < t1) {
+                const int4 e = P.cand[t];
+                p = e.x & 0xffff;
+                q = e.x >> 16;
+                hit = p > al && q > bl && Mij == Mk(k, p - 1, q - 1) + e.z;
+            }
+            const unsigned long long bh = __ballot(hit);
+            if (bh) {
+                const int src = (int)__ffsll((long long)bh) - 1;
+                found = true;
+                fp = readlane(p, src);
+                fq = readlane(q, src);
+            }
+        }
+        if (!found) break;
+        if (lane == 0) {
+            emit(P, fp, fq, fp + fq);
+            emit(P, i, j, i + j);
+            P.structs[fp - 1] = 1;
+            P.structs[i - 1] = 2;
+            P.structs[P.n + fq - 1] = 1;
+            P.structs[P.n + j - 1] = 2;
+            const int slot = atomicAdd(tcount, 1);
+            tstack[2 * slot] = fp | (fq << 16);
+            tstack[2 * slot + 1] = i | (j << 16);
+        }
+        i = fp - 1;
+        j = fq - 1;
+    }
+}
+
 // stored width of the rows al..ar-1 of a region (origin al, bl; bound br)
 __device__ __forceinline__ int region_width(const Pair &P, int al, int bl, int ar, int br) {
     int w = 0;
@@ -809,6 +1164,8 @@ __device__ __forceinline__ Pair load_pair(const PairDesc &d, int *arena, int *ld
     P.alpha_pos = bp.alpha > 0 ? bp.alpha : 0;
     P.nolp = bp.nolp;
     P.stacking = bp.stacking;
+    P.struct_local = bp.struct_local;
+    P.exclusion = bp.exclusion;
     return P;
 }
 
@@ -821,9 +1178,9 @@ __device__ __forceinline__ Pair load_pair(const PairDesc &d, int *arena, int *ld
 // Group schedule word G[5] = round | wave << 12 | member << 16.
 #define LCG_SPIN_LIMIT (1 << 22)
 
-template <bool TAB>
+template <bool TAB, bool SL>
 __device__ __forceinline__ void fill_pairs(const PairDesc *__restrict__ descs, const int *__restrict__ order,
-                                           int *queue, int *arena, int *barriers, BatchParams bp) {
+                                           int *queue, int *arena, int *barriers, int *scratch, BatchParams bp) {
     extern __shared__ int lds[];
     __shared__ int s_pair;
     const int wave = threadIdx.x >> 6;
@@ -885,7 +1242,24 @@ __device__ __forceinline__ void fill_pairs(const PairDesc *__restrict__ descs, c
                     const unsigned long long hit = __ballot(g < g1 && groups[g * LCG_GROUP_INTS + 5] == want);
                     if (hit) mg = gb + __ffsll((long long)hit) - 1;
                 }
-                if (mg >= 0) {
+                if (SL && mg >= 0) {
+                    // structure-local: eight states in the wave's global scratch
+                    const int *G = groups + mg * LCG_GROUP_INTS;
+                    const int gal = G[0] & 0xffff, gbl = G[0] >> 16;
+                    const int gar = G[1] & 0xffff, gbr = G[1] >> 16;
+                    const int W = G[2];
+                    int *base = scratch + (size_t)(blockIdx.x * (blockDim.x >> 6) + wave) * bp.sl_stride;
+                    group_fill_sl<TAB>(P, gal, gbl, gar, gbr, W, base);
+                    for (int t = G[3] + lane; t < G[4]; t += 64) {
+                        const int4 ga = gams[t];
+                        const int ar = ga.x & 0xffff, br = ga.x >> 16;
+                        const int v = sl_best(P, base, gal, gbl, gar, gbr, W, ar - 1, br - 1, nullptr) + P.amscore[ga.z];
+                        P.cand[ga.y].z = v;
+                        P.fcand[3 * ga.w + 2] = v;
+                        P.D[ga.z] = v;
+                    }
+                    region_sync<true>();
+                } else if (!SL && mg >= 0) {
                     const int *G = groups + mg * LCG_GROUP_INTS;
                     const int gal = G[0] & 0xffff, gbl = G[0] >> 16;
                     const int gar = G[1] & 0xffff, gbr = G[1] >> 16;
@@ -957,13 +1331,22 @@ __device__ __forceinline__ void fill_pairs(const PairDesc *__restrict__ descs, c
 // pairs of alignment profiles.
 extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs(const PairDesc *__restrict__ descs,
                                                                   const int *__restrict__ order, int *queue,
-                                                                  int *arena, int *barriers, BatchParams bp) {
-    fill_pairs<false>(descs, order, queue, arena, barriers, bp);
+                                                                  int *arena, int *barriers, int *scratch,
+                                                                  BatchParams bp) {
+    fill_pairs<false, false>(descs, order, queue, arena, barriers, scratch, bp);
 }
 extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs_tab(const PairDesc *__restrict__ descs,
                                                                       const int *__restrict__ order, int *queue,
-                                                                      int *arena, int *barriers, BatchParams bp) {
-    fill_pairs<true>(descs, order, queue, arena, barriers, bp);
+                                                                      int *arena, int *barriers, int *scratch,
+                                                                      BatchParams bp) {
+    fill_pairs<true, false>(descs, order, queue, arena, barriers, scratch, bp);
+}
+// structure-local (symbol codes only)
+extern "C" __global__ void __launch_bounds__(1024) lcg_fill_pairs_sl(const PairDesc *__restrict__ descs,
+                                                                     const int *__restrict__ order, int *queue,
+                                                                     int *arena, int *barriers, int *scratch,
+                                                                     BatchParams bp) {
+    fill_pairs<false, true>(descs, order, queue, arena, barriers, scratch, bp);
 }
 
 extern "C" __global__ void lcg_reset_barriers(int *barriers, int n) {
@@ -972,7 +1355,7 @@ extern "C" __global__ void lcg_reset_barriers(int *barriers, int n) {
 
 // Kernel 2: top level (global or sequence-local) and traceback of every
 // pair, after kernel 1 has filled D.
-template <bool TAB>
+template <bool TAB, bool SL>
 __device__ __forceinline__ void top_pairs(const PairDesc *__restrict__ descs, const int *__restrict__ order,
                                           int *queue, int *arena, int *scratch, BatchParams bp,
                                           const int *__restrict__ lambda, const int4 *__restrict__ restr) {
@@ -1105,7 +1488,26 @@ __device__ __forceinline__ void top_pairs(const PairDesc *__restrict__ descs, co
         // tasks share the LDS pool in equal slots; a task whose region does
         // not fit its slot waits for a pass in which wave 0 has the whole
         // pool (every group region fits the pool, checked at batch creation).
-        if (bp.trace) {
+        if (SL && bp.trace) {
+            // structure-local regions in each wave's global scratch
+            int lo = 0, hi = s_tcount;
+            __syncthreads();
+            while (lo < hi) {
+                for (int e = lo + wave; e < hi; e += nwaves) {
+                    const int *ts = arena + d.tstack + 2 * e;
+                    const int al = ts[0] & 0xffff, bl = ts[0] >> 16;
+                    const int ar = ts[1] & 0xffff, br = ts[1] >> 16;
+                    const int W = region_width(P, al, bl, ar, br);
+                    group_fill_sl<TAB>(P, al, bl, ar, br, W, wave_scratch);
+                    group_trace_sl<TAB>(P, wave_scratch, W, al, bl, ar, br, arena + d.tstack, &s_tcount);
+                }
+                __threadfence_block();
+                __syncthreads();
+                lo = hi;
+                hi = s_tcount;
+                __syncthreads();
+            }
+        } else if (!SL && bp.trace) {
             int lo = 0, hi = s_tcount;
             __syncthreads();
             while (lo < hi) {
@@ -1148,14 +1550,21 @@ extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs(const PairDesc 
                                                                  int *arena, int *scratch, BatchParams bp,
                                                                  const int *__restrict__ lambda,
                                                                  const int4 *__restrict__ restr) {
-    top_pairs<false>(descs, order, queue, arena, scratch, bp, lambda, restr);
+    top_pairs<false, false>(descs, order, queue, arena, scratch, bp, lambda, restr);
 }
 extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs_tab(const PairDesc *__restrict__ descs,
                                                                      const int *__restrict__ order, int *queue,
                                                                      int *arena, int *scratch, BatchParams bp,
                                                                      const int *__restrict__ lambda,
                                                                      const int4 *__restrict__ restr) {
-    top_pairs<true>(descs, order, queue, arena, scratch, bp, lambda, restr);
+    top_pairs<true, false>(descs, order, queue, arena, scratch, bp, lambda, restr);
+}
+extern "C" __global__ void __launch_bounds__(1024) lcg_top_pairs_sl(const PairDesc *__restrict__ descs,
+                                                                    const int *__restrict__ order, int *queue,
+                                                                    int *arena, int *scratch, BatchParams bp,
+                                                                    const int *__restrict__ lambda,
+                                                                    const int4 *__restrict__ restr) {
+    top_pairs<false, true>(descs, order, queue, arena, scratch, bp, lambda, restr);
 }
 
 // Copies the outputs of every pair (edges, structure strings, traceback
@@ -1219,8 +1628,9 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
                                 hipEvent_t mid, hipStream_t stream, const int4 *restr) {
     static bool attr_set = false;
     if (!attr_set) {
-        const void *kernels[] = {(const void *)lcg_fill_pairs, (const void *)lcg_top_pairs,
-                                 (const void *)lcg_fill_pairs_tab, (const void *)lcg_top_pairs_tab};
+        const void *kernels[] = {(const void *)lcg_fill_pairs,    (const void *)lcg_top_pairs,
+                                 (const void *)lcg_fill_pairs_tab, (const void *)lcg_top_pairs_tab,
+                                 (const void *)lcg_fill_pairs_sl,  (const void *)lcg_top_pairs_sl};
         for (const void *k : kernels) {
             hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
             if (e != hipSuccess) return (int)e;
@@ -1234,15 +1644,21 @@ extern "C" int lcg_launch_align(const PairDesc *descs, const int *order, int *qu
         hipLaunchKernelGGL(lcg_reset_queue, dim3(1), dim3(64), 0, stream, queue);
         if (bp.wg_per_pair > 1)
             hipLaunchKernelGGL(lcg_reset_barriers, dim3(1), dim3(256), 0, stream, barriers, bp.npairs);
-        if (bp.sigma_table)
+        if (bp.struct_local)
+            hipLaunchKernelGGL(lcg_fill_pairs_sl, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order,
+                               queue, arena, barriers, scratch, bp);
+        else if (bp.sigma_table)
             hipLaunchKernelGGL(lcg_fill_pairs_tab, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order,
-                               queue, arena, barriers, bp);
+                               queue, arena, barriers, scratch, bp);
         else
             hipLaunchKernelGGL(lcg_fill_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, order, queue,
-                               arena, barriers, bp);
+                               arena, barriers, scratch, bp);
     }
     if (mid) hipEventRecord(mid, stream);
-    if (bp_top.sigma_table)
+    if (bp_top.struct_local)
+        hipLaunchKernelGGL(lcg_top_pairs_sl, dim3(top_grid), dim3(top_waves * 64), top_lds_bytes, stream, descs,
+                           order, queue + 16, arena, scratch, bp_top, lambda, restr);
+    else if (bp_top.sigma_table)
         hipLaunchKernelGGL(lcg_top_pairs_tab, dim3(top_grid), dim3(top_waves * 64), top_lds_bytes, stream, descs,
                            order, queue + 16, arena, scratch, bp_top, lambda, restr);
     else

@@ -263,7 +263,8 @@ void pack(lcg_pair &P) {
     std::vector<int32_t> gams;
     std::vector<int32_t> rows(n + 2, 0);
     int64_t cells = 0;
-    int ecap = 0, gbuf = 0;
+    int ecap = 0, gbuf = 0, slmax = 0;
+    const bool sl = P.params.struct_local != 0;
     auto row_width = [&](int i, int al, int bl, int br) {
         int lo = (i == al) ? bl : std::max(bl, tc.min_col(i));
         int hi = std::min(br - 1, tc.max_col(i));
@@ -291,8 +292,10 @@ void pack(lcg_pair &P) {
                 W = std::max(W, w);
                 if (i > al) cells += std::max(0, std::min(br - 1, tc.max_col(i)) - std::max(bl + 1, tc.min_col(i)) + 1);
             }
-            // LDS region E | S | M, rounded to 4 ints
-            const int lds = ((br - bl) + W + (ar - al) * W + 3) & ~3;
+            // LDS region E | S | M, rounded to 4 ints; --struct-local: no LDS,
+            // eight matrices in global scratch
+            const int lds = sl ? 0 : ((br - bl) + W + (ar - al) * W + 3) & ~3;
+            if (sl) slmax = std::max(slmax, ((br - bl) + W + 8 * (ar - al) * W + 3) & ~3);
             G g{al, bl, ar, br, W, (int)gams.size() / 4, 0, lds};
             for (int k : lst) {
                 const ArcMatch &x = ams.arcmatch(k);
@@ -439,6 +442,7 @@ void pack(lcg_pair &P) {
         for (int k = 0; k < nam; ++k) blob[d.amst + k] = sc.stackable(k) ? (int32_t)sc.arcmatch_stacked(k) : LCG_NEG;
     while (blob.size() & 3) blob.push_back(0);
     d.gmax = gbuf;
+    d.slmax = slmax;
     {
         PairDesc L = d;
         const size_t size = layout_pair(L, n, m, ncells, d.ngroups, nam, d.nfblk, nolp, stacking, table);
@@ -565,6 +569,8 @@ void prepare_parse(const char *ppA, const char *ppB, const lcg_params *p, lcg_pa
         throw failure("Free endgaps cannot be combined with local alignment.");
     if (p->free_endgaps & ~15) throw failure("free end gaps: bits 1, 2, 4, 8 only");
     if (p->stacking & ~3) throw failure("stacking: bits 1 (--stacking) and 2 (--new-stacking) only");
+    if (p->struct_local && (p->no_lonely_pairs || p->stacking))
+        throw failure("--struct-local with --noLP or stacking is not supported");
     P.A.reset(new RnaData(ppA, p->min_prob, p->max_bp_span, p->stacking != 0));
     P.B.reset(new RnaData(ppB, p->min_prob, p->max_bp_span, p->stacking != 0));
     if (P.A->length() >= 0xffff || P.B->length() >= 0xffff) throw failure("sequence too long");
@@ -623,6 +629,9 @@ int64_t score_bound(const lcg_pair &P) {
         }
     }
     const int64_t alpha = std::llabs((int64_t)P.sp.indel_opening);
+    // --struct-local: two excluded blocks inside every arc match and the
+    // exclusion score in the first row and column of the regions
+    if (P.params.struct_local) amax += 3 * std::llabs((int64_t)P.params.exclusion);
     return (n + m) * (smax + gmax + alpha) + (std::min(n, m) / 2) * amax;
 }
 
@@ -670,6 +679,10 @@ int prepare(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair **ou
     }
     if (deferred && p->stacking) {  // nor the inner arc matches and stacked scores
         set_err(err, errlen, "--stacking needs the arc matches enumerated on the host");
+        return LCG_E_ARG;
+    }
+    if (deferred && p->struct_local) {  // nor the eight-state regions
+        set_err(err, errlen, "--struct-local needs the arc matches enumerated on the host");
         return LCG_E_ARG;
     }
     *out = nullptr;
@@ -886,6 +899,7 @@ struct lcg_batch {
     lcg_shape req{};
     lcg_shape eff{};
     int cus = 1, hdr = 0, gmax = 4, top_max = 4, max_len = 1;
+    int sl_max = 0;  // --struct-local: ints of the largest group region (global scratch per wave)
     BatchParams bp;      // fill-D kernel
     int waves = 1, grid = 1;
     size_t lds_bytes = 0;
@@ -1350,8 +1364,11 @@ void apply_shape(lcg_batch &B, const FillShape &s, const TopShape &t) {
     B.bp.pool_ints = s.pool;
     B.bp.wg_per_pair = s.K;
     B.grid = s.K > 1 ? np * s.K : std::min(np, B.cus * s.wgs_per_cu);
+    B.bp.sl_stride = B.sl_max;
     B.bp_top = B.bp;
     B.bp_top.pool_ints = t.pool;
+    // the traceback's structure-local regions use the per-wave scratch too
+    B.bp_top.scratch_stride = std::max(B.top_max, B.sl_max);
     B.top_waves = t.waves;
     B.top_lds_bytes = (size_t)(B.hdr + t.pool) * 4;
     B.top_grid = std::min(np, B.cus * t.wgs_per_cu);
@@ -1360,6 +1377,14 @@ void apply_shape(lcg_batch &B, const FillShape &s, const TopShape &t) {
     B.eff.wg_per_pair = s.K;
     B.eff.top_waves = t.waves;
     B.eff.top_wgs_per_cu = t.wgs_per_cu;
+}
+
+// global scratch of a batch: the top level's per-wave regions and, with
+// --struct-local, the fill's and traceback's eight-state regions
+size_t scratch_ints(const lcg_batch &B) {
+    const size_t top = (size_t)B.top_grid * B.top_waves * std::max(B.top_max, B.sl_max);
+    const size_t fill = (size_t)B.grid * B.waves * B.sl_max;
+    return std::max(top, fill);
 }
 }  // namespace
 
@@ -1382,10 +1407,11 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
         const lcg_pair *P = pairs[k];
         if (P->params.indel_opening != p0.indel_opening || P->params.sequ_local != p0.sequ_local ||
             P->params.free_endgaps != p0.free_endgaps || P->params.no_lonely_pairs != p0.no_lonely_pairs ||
-            (P->params.stacking != 0) != (p0.stacking != 0)) {
+            (P->params.stacking != 0) != (p0.stacking != 0) || P->params.struct_local != p0.struct_local ||
+            (p0.struct_local && P->params.exclusion != p0.exclusion)) {
             set_err(err, errlen,
-                    "all pairs of a batch must share indel_opening, sequ_local, free_endgaps, no_lonely_pairs "
-                    "and stacking");
+                    "all pairs of a batch must share indel_opening, sequ_local, free_endgaps, no_lonely_pairs, "
+                    "stacking, struct_local and exclusion");
             return LCG_E_ARG;
         }
         if (P->deferred != deferred) {
@@ -1438,6 +1464,7 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
         }
     }
     int max_len = 1, gmax = 4, top_max = 4, nfblk_max = 1;
+    for (const PairDesc &d : B->descs) B->sl_max = std::max(B->sl_max, d.slmax);
     for (const PairDesc &d : B->descs) {
         max_len = std::max(max_len, std::max(d.n, d.m));
         gmax = std::max(gmax, d.gmax);
@@ -1477,6 +1504,12 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
     B->bp.nolp = p0.no_lonely_pairs != 0;
     B->bp.stacking = p0.stacking != 0;
     B->bp.sigma_table = pairs[0]->sc->profile() ? 1 : 0;
+    B->bp.struct_local = p0.struct_local != 0;
+    B->bp.exclusion = p0.exclusion;
+    if (B->bp.struct_local && B->bp.sigma_table) {
+        set_err(err, errlen, "--struct-local: pairs of single sequences only");
+        return LCG_E_ARG;
+    }
     B->bp.trace = 1;
     B->bp.npairs = npairs;
     B->bp.hdr_ints = B->hdr;
@@ -1493,8 +1526,7 @@ extern "C" int lcg_batch_create_shaped(lcg_pair *const *pairs, int32_t npairs, i
     if ((e = hipMalloc(&B->d_order, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_queue, 256)) != hipSuccess) return hip_fail(e, err, errlen);
     if ((e = hipMalloc(&B->d_barriers, 4 * (size_t)npairs)) != hipSuccess) return hip_fail(e, err, errlen);
-    const size_t scratch_ints = (size_t)B->top_grid * B->top_waves * top_max;
-    if ((e = hipMalloc(&B->d_scratch, scratch_ints * 4)) != hipSuccess) return hip_fail(e, err, errlen);
+    if ((e = hipMalloc(&B->d_scratch, scratch_ints(*B) * 4)) != hipSuccess) return hip_fail(e, err, errlen);
     if (deferred)  // the compaction writes the image over a cleared arena
         e = hipMemsetAsync(B->d_arena, 0, B->arena_ints * 4, B->stream);
     else
@@ -1722,6 +1754,7 @@ extern "C" int lcg_noalloc_scratch(int device, int **ptr, long long *ints);
 // same for any number).
 extern "C" int lcg_batch_run_noalloc(lcg_batch *B, int32_t trace) {
     if (!B || (trace & ~1)) return LCG_E_ARG;
+    if (B->bp.struct_local) return LCG_E_ARG;  // its regions need the batch's own scratch
     HIPCHECK(hipSetDevice(B->device));
     int *S = nullptr;
     long long cap = 0;
@@ -1900,7 +1933,8 @@ int restricted_top(lcg_batch *B, int k, const int32_t r[4], bool trace, int64_t 
     HIPCHECK(hipSetDevice(B->device));
     const PairDesc &d = B->descs[k];
     // the stored width of a restricted region is at most m + 1
-    const size_t need = ((size_t)(d.m + 1) * 2 + (size_t)(d.n + 1) * (d.m + 1) + 3) & ~(size_t)3;
+    const size_t need = std::max<size_t>(((size_t)(d.m + 1) * 2 + (size_t)(d.n + 1) * (d.m + 1) + 3) & ~(size_t)3,
+                                         (size_t)B->sl_max);
     if (need > B->kscratch_ints) {
         if (B->d_kscratch) HIPCHECK(hipFree(B->d_kscratch));
         B->d_kscratch = nullptr;
@@ -2473,6 +2507,7 @@ extern "C" int lcg_batch_pf_run(lcg_batch *B, const double *dtheta, float *devic
     if (B->bp.nolp) return LCG_E_ARG;          // --noLP: integer path only
     if (B->bp.stacking) return LCG_E_ARG;      // --stacking: integer path only
     if (B->bp.sigma_table) return LCG_E_ARG;   // profiles: integer path only
+    if (B->bp.struct_local) return LCG_E_ARG;  // --struct-local: integer path only
     if (B->device_built) return LCG_E_STATE;  // LocARNA-P reads the host arc-match lists
     if (int rc = pf_setup(*B)) return rc;
     for (int k = 0; k < B->npairs; ++k) pf_tables(*B, k, dtheta, B->pf_arena.data());
@@ -3084,13 +3119,13 @@ extern "C" int lcg_batch_reshape(lcg_batch *B, const lcg_shape *shape, char *err
             HIPCHECK(hipMemcpy(B->d_arena, B->host_arena.data(), B->arena_ints * 4, hipMemcpyHostToDevice));
         }
     }
-    const size_t old_scratch = (size_t)B->top_grid * B->top_waves * B->top_max;
+    const size_t old_scratch = scratch_ints(*B);
     apply_shape(*B, fs, ts);
-    const size_t scratch_ints = (size_t)B->top_grid * B->top_waves * B->top_max;
-    if (scratch_ints > old_scratch) {
+    const size_t new_scratch = scratch_ints(*B);
+    if (new_scratch > old_scratch) {
         HIPCHECK(hipFree(B->d_scratch));
         B->d_scratch = nullptr;
-        HIPCHECK(hipMalloc(&B->d_scratch, scratch_ints * 4));
+        HIPCHECK(hipMalloc(&B->d_scratch, new_scratch * 4));
     }
     if (B->graph_exec) hipGraphExecDestroy(B->graph_exec);
     if (B->graph) hipGraphDestroy(B->graph);
@@ -3507,6 +3542,10 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
     }
     if (deferred && p->stacking) {  // nor the inner arc matches and stacked scores
         set_err(err, errlen, "--stacking needs the arc matches enumerated on the host");
+        return LCG_E_ARG;
+    }
+    if (deferred && p->struct_local) {  // nor the eight-state regions
+        set_err(err, errlen, "--struct-local needs the arc matches enumerated on the host");
         return LCG_E_ARG;
     }
     std::vector<std::unique_ptr<lcg_pair>> P(npairs);

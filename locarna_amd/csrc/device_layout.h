@@ -84,7 +84,8 @@ typedef struct {
     int32_t inner;     // --noLP / --stacking only: per arc match the arc match directly inside it, -1 if none
     int32_t c2a;       // --noLP / --stacking only: candidate entry -> arc match
     int32_t amst;      // --stacking only: per arc match its stacked score, LCG_NEG if not stackable
-} PairDesc;  // 32 ints
+    int32_t slmax;     // --struct-local: ints of the largest group region (eight states)
+} PairDesc;  // 33 ints
 
 typedef struct {
     int32_t alpha;           // indel opening
@@ -108,4 +109,9 @@ typedef struct {
     int32_t sigma_table;     // pairs of alignment profiles: sigma is a per-pair
                              // (n+1) x (m+1) table at PairDesc.sigma (row i at
                              // sigma + i * (m+1)), read from global memory
+    int32_t struct_local;    // --struct-local: eight states inside arc matches,
+                             // group regions in global scratch
+    int32_t exclusion;       // --exclusion: score of one excluded block
+    int32_t sl_stride;       // ints of global scratch per (workgroup, wave) of the
+                             // fill with --struct-local
 } BatchParams;

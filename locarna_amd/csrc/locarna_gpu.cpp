@@ -10,7 +10,8 @@
 //       [--tau=N] [--min-prob=P] [--maxBPspan=N] [--min-trace-probability=P]
 //       [--sequ-local] [--free-endgaps=XXXX] [--max-diff-am=N] [--max-diff-at-am=N]
 //       [--penalized=P] [--normalized=L] [--noLP] [--stacking] [--new-stacking]
-//       [--exp-prob=P] [--kbest=K] [--better=T] [--clustal] [--write-structure]
+//       [--exp-prob=P] [--struct-local] [--exclusion=N] [--kbest=K] [--better=T] [--clustal]
+//       [--write-structure]
 //   locarna_gpu --pf A.pp B.pp [options] [--min-am-prob=P] [--min-bm-prob=P]
 //       [--include-am-in-bm] [--write-arcmatch-probs=FILE] [--write-basematch-probs=FILE]
 //       [--fragment-match-probs="i j k l; ..."]
@@ -45,7 +46,7 @@ int usage() {
 
 int main(int argc, char **argv) {
     using namespace locarna_amd;
-    bool pf = false, clustal = false, structures = false, include_am = false, local = false;
+    bool pf = false, clustal = false, structures = false, include_am = false, local = false, slocal = false;
     std::string files[2], free_endgaps = "----", am_file, bm_file;
     int nfiles = 0;
     ScoringParams sp;
@@ -68,6 +69,8 @@ int main(int argc, char **argv) {
         else if (arg == "--include-am-in-bm") include_am = true;
         else if (arg == "--noLP") nolp = true;
         else if (arg == "--stacking") sp.stacking = true;
+        else if (arg == "--struct-local") slocal = true;
+        else if (opt(arg, "exclusion", v)) sp.exclusion = std::atoi(v.c_str());
         else if (arg == "--new-stacking") sp.new_stacking = true;
         else if (opt(arg, "fragment-match-probs", v)) fragments = v;
         else if (opt(arg, "kbest", v)) subopt = true, kbest = std::atoi(v.c_str());
@@ -112,7 +115,8 @@ int main(int argc, char **argv) {
                                    AlignerParams::free_endgaps(free_endgaps), AlignerParams::max_diff_am(max_diff_am),
                                    AlignerParams::max_diff_at_am(max_diff_at_am),
                                    AlignerParams::no_lonely_pairs(nolp),
-                                   AlignerParams::stacking(sp.stacking || sp.new_stacking)};
+                                   AlignerParams::stacking(sp.stacking || sp.new_stacking),
+                                   AlignerParams::struct_local(slocal)};
             Aligner aligner(ap);
             if (subopt) {
                 // src/locarna.cc:740-746: the enumeration replaces the
@@ -154,7 +158,8 @@ int main(int argc, char **argv) {
                                             AlignerParams::scoring(&sp), AlignerParams::trace_controller(&bp),
                                             AlignerPParams<double>::min_am_prob(min_am_prob),
                                             AlignerPParams<double>::min_bm_prob(min_bm_prob),
-                                            AlignerParams::no_lonely_pairs(nolp)};
+                                            AlignerParams::no_lonely_pairs(nolp),
+                                            AlignerParams::struct_local(slocal)};
             AlignerP<double> aligner(ap);
             std::cout << "Run inside algorithm.\n";
             const double z = aligner.align_inside();

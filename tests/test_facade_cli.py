@@ -44,3 +44,5 @@ def test_failures_report_status_and_message(tmp_path):
     # the joint probabilities but scores without them)
     r = _run(["--pf", str(fa), str(fb), "--new-stacking"])
     assert r.returncode == 1 and "stacking" in r.stderr
+    r = _run(["--pf", str(fa), str(fb), "--struct-local"])
+    assert r.returncode == 1 and "struct_local" in r.stderr

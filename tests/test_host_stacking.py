@@ -42,3 +42,15 @@ def test_refused_combinations():
         api.Pair.from_arcmatch_scores(pp, pp, "2 9 2 9 100\n", _params(1))
     with pytest.raises(api.LcgError):
         api.Pair(pp, pp, _params(4))  # unknown bit
+
+
+def test_struct_local_refusals_on_the_host():
+    """--struct-local with --noLP or stacking, or with the device
+    compaction, is refused before any device work."""
+    from tiny import tiny_pair
+    a, b = tiny_pair(4600)
+    for kw in (dict(no_lonely_pairs=1), dict(stacking=1)):
+        with pytest.raises(api.LcgError):
+            api.Pair(a, b, api.locarna_params(struct_local=1, **kw))
+    with pytest.raises(api.LcgError, match="host"):
+        api.Pair(a, b, api.locarna_params(struct_local=1), device_compaction=True)
