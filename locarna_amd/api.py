@@ -306,8 +306,10 @@ def prepare_pairs(pp_pairs, params=None, device_compaction=False, device_band=Fa
     L = lib()
     params = params or locarna_params()
     n = len(pp_pairs)
-    A = (ctypes.c_char_p * n)(*[a.encode() for a, _ in pp_pairs])
-    B = (ctypes.c_char_p * n)(*[b.encode() for _, b in pp_pairs])
+    # one buffer per distinct text, so that the library parses it once
+    enc = {}
+    A = (ctypes.c_char_p * n)(*[enc.setdefault(a, a.encode()) for a, _ in pp_pairs])
+    B = (ctypes.c_char_p * n)(*[enc.setdefault(b, b.encode()) for _, b in pp_pairs])
     out = (ctypes.c_void_p * max(n, 1))()
     flags = (1 if device_compaction else 0) | (2 if device_band else 0)
     opts = PrepareOpts(flags=flags, device=device, threads=threads, reserved=0, band_eps=band_eps)

@@ -17,6 +17,7 @@
 #include <mutex>
 #include <numeric>
 #include <queue>
+#include <unordered_map>
 #include <string>
 #include <thread>
 #include <vector>
@@ -563,7 +564,10 @@ void prepare_compaction_inputs(lcg_pair &P) {
 namespace {
 // Stage 1 of the preparation of a pair: both RnaData (pp reader), the
 // score parameters and the unrestricted band.
-void prepare_parse(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair &P) {
+// A and B, when given, are the already parsed inputs (shared between the
+// pairs of one lcg_pairs_prepare call that name the same text)
+void prepare_parse(const char *ppA, const char *ppB, const lcg_params *p, lcg_pair &P,
+                   std::shared_ptr<const RnaData> A = nullptr, std::shared_ptr<const RnaData> B = nullptr) {
     P.params = *p;
     if (p->sequ_local && p->free_endgaps)  // src/locarna.cc:375-378
         throw failure("Free endgaps cannot be combined with local alignment.");
@@ -571,8 +575,8 @@ void prepare_parse(const char *ppA, const char *ppB, const lcg_params *p, lcg_pa
     if (p->stacking & ~3) throw failure("stacking: bits 1 (--stacking) and 2 (--new-stacking) only");
     if (p->struct_local && (p->no_lonely_pairs || p->stacking))
         throw failure("--struct-local with --noLP or stacking is not supported");
-    P.A.reset(new RnaData(ppA, p->min_prob, p->max_bp_span, p->stacking != 0));
-    P.B.reset(new RnaData(ppB, p->min_prob, p->max_bp_span, p->stacking != 0));
+    P.A = A ? A : std::make_shared<const RnaData>(ppA, p->min_prob, p->max_bp_span, p->stacking != 0);
+    P.B = B ? B : std::make_shared<const RnaData>(ppB, p->min_prob, p->max_bp_span, p->stacking != 0);
     if (P.A->length() >= 0xffff || P.B->length() >= 0xffff) throw failure("sequence too long");
     ScoringParams &sp = P.sp;
     sp.indel = p->indel;
@@ -3558,11 +3562,38 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
         set_err(err, errlen, "pair " + std::to_string(k) + ": " + msg);
         return LCG_E_INPUT;
     };
+    // every distinct input text is parsed once (all-pairs batches name each
+    // sequence's text n - 1 times; texts are told apart by address)
+    std::unordered_map<const char *, int> uid;
+    std::vector<const char *> texts;
+    std::vector<int> ua(npairs), ub(npairs), first_pair;
+    for (int k = 0; k < npairs; ++k) {
+        for (int side = 0; side < 2; ++side) {
+            const char *t = side ? ppB[k] : ppA[k];
+            auto it = uid.find(t);
+            int u;
+            if (it == uid.end()) {
+                u = (int)texts.size();
+                uid.emplace(t, u);
+                texts.push_back(t);
+                first_pair.push_back(k);
+            } else {
+                u = it->second;
+            }
+            (side ? ub : ua)[k] = u;
+        }
+    }
+    std::vector<std::shared_ptr<const RnaData>> rd(texts.size());
     int bad = parallel_for(
+        (int)texts.size(), threads,
+        [&](int u) { rd[u] = std::make_shared<const RnaData>(texts[u], p->min_prob, p->max_bp_span, p->stacking != 0); },
+        msg);
+    if (bad >= 0) return input_error(first_pair[bad]);
+    bad = parallel_for(
         npairs, threads,
         [&](int k) {
             P[k].reset(new lcg_pair);
-            prepare_parse(ppA[k], ppB[k], p, *P[k]);
+            prepare_parse(ppA[k], ppB[k], p, *P[k], rd[ua[k]], rd[ub[k]]);
         },
         msg);
     if (bad >= 0) return input_error(bad);

@@ -96,6 +96,15 @@ RnaData::RnaData(const std::string &pp_text, double p_bpcut, long max_bp_span, b
     if (!stack_keyword && !joint_.empty())  // rna_data.cc:1097-1100
         throw failure("Syntax error: Stacking probabilties found but stack keyword is missing.");
     for (const auto &x : probs_) pairs_.push_back(x);
+    // paired probabilities per position, summed in the reference's orders
+    // (upstream: j ascending; downstream: i ascending): both are the order
+    // of the map, (i, j) ascending
+    up_.assign(seq_.size() + 2, 0.0);
+    down_.assign(seq_.size() + 2, 0.0);
+    for (const auto &x : probs_) {
+        up_[x.first.first] += x.second;
+        down_[x.first.second] += x.second;
+    }
 }
 
 double RnaData::joint_arc_prob(int i, int j) const {
@@ -108,20 +117,9 @@ double RnaData::arc_prob(int i, int j) const {
     return it == probs_.end() ? 0.0 : it->second;
 }
 
-double RnaData::prob_paired_upstream(int i) const {
-    double p = 0.0;
-    for (auto it = probs_.lower_bound({i, 0}); it != probs_.end() && it->first.first == i; ++it) p += it->second;
-    return p;
-}
+double RnaData::prob_paired_upstream(int i) const { return up_[i]; }
 
-double RnaData::prob_paired_downstream(int j) const {
-    double p = 0.0;  // summed over i ascending, as the reference does
-    for (int i = 1; i < j; ++i) {
-        auto it = probs_.find({i, j});
-        if (it != probs_.end()) p += it->second;
-    }
-    return p;
-}
+double RnaData::prob_paired_downstream(int j) const { return down_[j]; }
 
 // ---------------------------------------------------------------- BasePairs
 

@@ -56,6 +56,7 @@ private:
     std::string name_, seq_;
     std::vector<std::string> names_, rows_;
     std::map<std::pair<int, int>, double> probs_, joint_;
+    std::vector<double> up_, down_;  // prob_paired_upstream / downstream per position
     std::vector<std::pair<std::pair<int, int>, double>> pairs_;
     double p_bpcut_;
 };
