@@ -951,6 +951,7 @@ struct lcg_batch {
     int32_t *d_pfsched = nullptr;  // LocARNA-P group schedule (rounds, LDS offsets)
     PfDesc *d_pdescs = nullptr;
     double *d_pfscratch = nullptr;
+    double *d_pfstore = nullptr;  // LocARNA-P forward values of the fixed-column groups (inside -> outside)
     int pf_grid = 0, pf_waves = 8;
     size_t pf_lds = 0;
     PfBatchParams pbp;
@@ -1595,6 +1596,7 @@ lcg_batch::~lcg_batch() {
     if (B->d_pdescs) hipFree(B->d_pdescs);
     if (B->d_pfscratch) hipFree(B->d_pfscratch);
     if (B->d_pfsched) hipFree(B->d_pfsched);
+    if (B->d_pfstore) hipFree(B->d_pfstore);
     if (B->ev0) hipEventDestroy(B->ev0);
     if (B->ev1) hipEventDestroy(B->ev1);
     if (B->ev_mid) hipEventDestroy(B->ev_mid);
@@ -2248,8 +2250,8 @@ extern "C" int lcg_batch_kernel_times(const lcg_batch *B, float *fill_ms, float 
 // ------------------------------------------------------------ LocARNA-P
 
 extern "C" int lcg_launch_pf(const PairDesc *descs, const PfDesc *pdescs, const int *order, int *queue, int *arena,
-                             double *pfa, double *scratch, const int *sched, int *barriers, PfBatchParams bp, int grid,
-                             int waves, size_t lds_bytes, hipStream_t stream);
+                             double *pfa, double *scratch, const int *sched, int *barriers, double *fstore,
+                             PfBatchParams bp, int grid, int waves, size_t lds_bytes, hipStream_t stream);
 
 namespace {
 int pf_setup(lcg_batch &B) {
@@ -2427,6 +2429,27 @@ int pf_setup(lcg_batch &B) {
         // group indices in sched are relative to the pair's group array
         (void)gbase;
     }
+    // forward-value store: every group that runs the fixed-column kernels in
+    // both passes gets 3 (ar - al) W doubles (E, s, M) while the store stays
+    // within its budget; the others are recomputed by the outside pass
+    const size_t store_budget = (size_t)1 << 31;  // doubles (16 GiB)
+    size_t store = 0;
+    for (int k = 0; k < np; ++k) {
+        const PairDesc &d = B.descs[k];
+        PfDesc &f = B.pdescs[k];
+        const int32_t *A = B.host_arena.data();
+        f.fso = (int32_t)sched.size();
+        sched.resize(sched.size() + (size_t)d.ngroups, -1);
+        for (int g = 0; g < d.ngroups; ++g) {
+            const int32_t *S = &sched[f.sched + 4 * (size_t)g];
+            if (S[1] < 0 || S[3] < 0) continue;
+            const int32_t *G = A + d.groups + g * LCG_GROUP_INTS;
+            const size_t need = (3 * (size_t)((G[1] & 0xffff) - (G[0] & 0xffff)) * G[2] + 3) & ~(size_t)3;
+            if (store + need > store_budget) continue;
+            sched[f.fso + (size_t)g] = (int32_t)(store / 4);
+            store += need;
+        }
+    }
     stride = std::max(stride, max_fixed);
     B.pbp.scratch_stride = (int32_t)((stride + 31) & ~(size_t)31);
     hipError_t e;
@@ -2438,6 +2461,7 @@ int pf_setup(lcg_batch &B) {
     HIPCHECK(hipMalloc(&B.d_pdescs, sizeof(PfDesc) * np));
     HIPCHECK(hipMalloc(&B.d_pfscratch, (size_t)B.pf_grid * B.pf_waves * B.pbp.scratch_stride * 8));
     HIPCHECK(hipMalloc(&B.d_pfsched, std::max<size_t>(1, sched.size()) * 4));
+    HIPCHECK(hipMalloc(&B.d_pfstore, std::max<size_t>(4, store) * 8));
     HIPCHECK(hipMemcpy(B.d_pfsched, sched.data(), sched.size() * 4, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(B.d_pdescs, B.pdescs.data(), sizeof(PfDesc) * np, hipMemcpyHostToDevice));
     (void)e;
@@ -2522,7 +2546,8 @@ extern "C" int lcg_batch_pf_run(lcg_batch *B, const double *dtheta, float *devic
     B->pbp.spin_limit = B->bp.spin_limit;
     HIPCHECK(hipEventRecord(B->ev0, B->stream));
     int rc = lcg_launch_pf(B->d_descs, B->d_pdescs, B->d_order, B->d_queue, B->d_arena, B->d_pfa, B->d_pfscratch,
-                           B->d_pfsched, B->d_barriers, B->pbp, B->pf_grid, B->pf_waves, B->pf_lds, B->stream);
+                           B->d_pfsched, B->d_barriers, B->d_pfstore, B->pbp, B->pf_grid, B->pf_waves, B->pf_lds,
+                           B->stream);
     if (rc) return hip_fail((hipError_t)rc);
     HIPCHECK(hipEventRecord(B->ev1, B->stream));
     HIPCHECK(hipEventSynchronize(B->ev1));
@@ -3056,10 +3081,12 @@ void drop_pf_state(lcg_batch &B) {
     if (B.d_pdescs) hipFree(B.d_pdescs);
     if (B.d_pfscratch) hipFree(B.d_pfscratch);
     if (B.d_pfsched) hipFree(B.d_pfsched);
+    if (B.d_pfstore) hipFree(B.d_pfstore);
     B.d_pfa = nullptr;
     B.d_pdescs = nullptr;
     B.d_pfscratch = nullptr;
     B.d_pfsched = nullptr;
+    B.d_pfstore = nullptr;
     B.pf_have = false;
 }
 

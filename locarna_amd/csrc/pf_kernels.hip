@@ -788,7 +788,7 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
                                                                 const int *__restrict__ order, int *queue,
                                                                 int *arena, double *pfa, double *scratch,
                                                                 const int *__restrict__ sched, int *barriers,
-                                                                PfBatchParams bp) {
+                                                                double *fstore, PfBatchParams bp) {
     extern __shared__ int lds[];
     __shared__ int s_pair;
     __shared__ double s_open;
@@ -951,11 +951,20 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
                     const int off = S[4 * g + 1];
                     if (off >= 0) {
                         double *base = pool + off;
+                        // the forward values the outside pass reads (E, s, M), if stored
+                        const int so = sched[f.fso + g];
+                        const size_t cells = (size_t)(gar - gal) * W;
+                        double *Es = so >= 0 ? fstore + 4 * (size_t)so : nullptr;
+                        double *Ss = so >= 0 ? Es + cells : nullptr;
                         if (gbr - gbl - 1 <= 64)
-                            pf_fwd_fixed<1>(P, gal, gbl, gar, gbr, W, base, nullptr, nullptr);
+                            pf_fwd_fixed<1>(P, gal, gbl, gar, gbr, W, base, Es, Ss);
                         else
-                            pf_fwd_fixed<2>(P, gal, gbl, gar, gbr, W, base, nullptr, nullptr);
+                            pf_fwd_fixed<2>(P, gal, gbl, gar, gbr, W, base, Es, Ss);
                         const double *Mst = base + (gbr - gbl);
+                        if (so >= 0) {
+                            double *Ms = Ss + cells;
+                            for (size_t c = lane; c < cells; c += 64) Ms[c] = Mst[c];
+                        }
                         for (int t = G[3] + lane; t < G[4]; t += 64) {
                             const int4 ga = gams[t];
                             const int ar = ga.x & 0xffff, br = ga.x >> 16;
@@ -1046,12 +1055,22 @@ extern "C" __global__ void __launch_bounds__(512) lcg_pf_pairs(const PairDesc *_
                         double *base = pool + off;
                         const size_t cells = (size_t)(gar - gal) * W;
                         double *Mb = base + (gbr - gbl) + cells;
-                        double *Es = wscr, *Ss = wscr + cells;
                         const bool nc1 = gbr - gbl - 1 <= 64;
-                        if (nc1)
+                        const int so = sched[f.fso + g];
+                        double *Es = wscr, *Ss = wscr + cells;
+                        if (so >= 0) {
+                            // the inside pass's forward values: M into the region
+                            Es = fstore + 4 * (size_t)so;
+                            Ss = Es + cells;
+                            const double *Ms = Ss + cells;
+                            double *Mst = base + (gbr - gbl);
+                            for (size_t c = lane; c < cells; c += 64) Mst[c] = Ms[c];
+                            lds_fence();
+                        } else if (nc1) {
                             pf_fwd_fixed<1>(P, gal, gbl, gar, gbr, W, base, Es, Ss);
-                        else
+                        } else {
                             pf_fwd_fixed<2>(P, gal, gbl, gar, gbr, W, base, Es, Ss);
+                        }
                         for (int pass = 0; pass < (sig ? 2 : 1); ++pass) {
                             for (size_t c = lane; c < cells; c += 64) Mb[c] = 0.0;
                             lds_fence();
@@ -1137,8 +1156,8 @@ extern "C" __global__ void lcg_pf_reset_queue(int *queue) {
 }
 
 extern "C" int lcg_launch_pf(const PairDesc *descs, const PfDesc *pdescs, const int *order, int *queue, int *arena,
-                             double *pfa, double *scratch, const int *sched, int *barriers, PfBatchParams bp, int grid,
-                             int waves, size_t lds_bytes, hipStream_t stream) {
+                             double *pfa, double *scratch, const int *sched, int *barriers, double *fstore,
+                             PfBatchParams bp, int grid, int waves, size_t lds_bytes, hipStream_t stream) {
     static bool attr_set = false;
     if (!attr_set) {
         hipError_t e = hipFuncSetAttribute((const void *)lcg_pf_pairs, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -1150,6 +1169,6 @@ extern "C" int lcg_launch_pf(const PairDesc *descs, const PfDesc *pdescs, const 
     if (bp.wg_per_pair > 1)
         hipLaunchKernelGGL(lcg_pf_reset_barriers, dim3(1), dim3(256), 0, stream, barriers, bp.npairs);
     hipLaunchKernelGGL(lcg_pf_pairs, dim3(grid), dim3(waves * 64), lds_bytes, stream, descs, pdescs, order, queue,
-                       arena, pfa, scratch, sched, barriers, bp);
+                       arena, pfa, scratch, sched, barriers, fstore, bp);
     return (int)hipGetLastError();
 }

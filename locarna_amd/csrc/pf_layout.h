@@ -24,7 +24,12 @@
 // Per pair in the int schedule buffer (offsets in ints): per group 4 ints
 // (inside round | wave<<12, inside LDS offset in doubles, outside round |
 // wave<<12, outside LDS offset; offset -1: the group runs in global
-// scratch), and the number of rounds of every row for both passes.
+// scratch), the number of rounds of every row for both passes, and per
+// group the offset of its forward values in the store (units of 4 doubles;
+// -1: none, the outside pass recomputes them): E and s = M - F of every
+// computed cell and the region's M rows, (ar - al) W doubles each, written
+// by the inside pass and read by the outside pass instead of a second
+// forward fill.
 #pragma once
 #include <stdint.h>
 
@@ -35,7 +40,8 @@ typedef struct {
     int32_t fD;
     int32_t sched, rin, rout;  // int schedule buffer
     int32_t frag, nfrag;       // fragment match queries of the run
-} PfDesc;  // 16 ints
+    int32_t fso;               // int schedule buffer: forward-store offset per group
+} PfDesc;  // 17 ints
 
 typedef struct {
     int32_t npairs;
