@@ -49,7 +49,7 @@ struct lcg_pair {
     lcg_params params;
     ScoringParams sp;
     std::shared_ptr<const RnaData> A, B;  // shared with the batches made of the pair (output writers)
-    std::unique_ptr<BasePairs> bA, bB;
+    std::shared_ptr<const BasePairs> bA, bB;  // shared by the pairs of one text (lcg_pairs_prepare)
     std::unique_ptr<TraceController> tc;
     std::unique_ptr<ArcMatches> am;
     std::unique_ptr<Scoring> sc;
@@ -650,10 +650,13 @@ void check_score_range(const lcg_pair &P) {
 
 // Stage 3: base pairs, arc matches (host) or compaction inputs (device),
 // score tables and the device image.
-void prepare_finish(lcg_pair &P, bool deferred) {
+// bA and bB, when given, are the base pairs of the inputs (shared between
+// the pairs of one lcg_pairs_prepare call that name the same text)
+void prepare_finish(lcg_pair &P, bool deferred, std::shared_ptr<const BasePairs> bA = nullptr,
+                    std::shared_ptr<const BasePairs> bB = nullptr) {
     const lcg_params *p = &P.params;
-    P.bA.reset(new BasePairs(P.A.get(), p->min_prob));
-    P.bB.reset(new BasePairs(P.B.get(), p->min_prob));
+    P.bA = bA ? bA : std::make_shared<const BasePairs>(P.A.get(), p->min_prob);
+    P.bB = bB ? bB : std::make_shared<const BasePairs>(P.B.get(), p->min_prob);
     if (deferred) {
         if (P.A->num_rows() > 1 || P.B->num_rows() > 1)  // the compaction reads symbol codes
             throw failure("alignment profiles need the arc matches enumerated on the host");
@@ -3611,9 +3614,13 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
         }
     }
     std::vector<std::shared_ptr<const RnaData>> rd(texts.size());
+    std::vector<std::shared_ptr<const BasePairs>> bps(texts.size());
     int bad = parallel_for(
         (int)texts.size(), threads,
-        [&](int u) { rd[u] = std::make_shared<const RnaData>(texts[u], p->min_prob, p->max_bp_span, p->stacking != 0); },
+        [&](int u) {
+            rd[u] = std::make_shared<const RnaData>(texts[u], p->min_prob, p->max_bp_span, p->stacking != 0);
+            bps[u] = std::make_shared<const BasePairs>(rd[u].get(), p->min_prob);
+        },
         msg);
     if (bad >= 0) return input_error(first_pair[bad]);
     bad = parallel_for(
@@ -3656,7 +3663,8 @@ extern "C" int lcg_pairs_prepare(const char *const *ppA, const char *const *ppB,
     band_wall_ms = since(t0);
     t0 = clk::now();
     bool capacity = false;
-    bad = parallel_for(npairs, threads, [&](int k) { prepare_finish(*P[k], deferred); }, msg, &capacity);
+    bad = parallel_for(
+        npairs, threads, [&](int k) { prepare_finish(*P[k], deferred, bps[ua[k]], bps[ub[k]]); }, msg, &capacity);
     if (bad >= 0) {
         const int rc = input_error(bad);
         return capacity ? LCG_E_CAPACITY : rc;
