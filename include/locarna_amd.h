@@ -502,6 +502,43 @@ int lcg_progressive_stats(const lcg_progressive *r, int32_t *rounds, int32_t *ba
                           float *prepare_ms, float *wall_ms);
 void lcg_progressive_free(lcg_progressive *r);
 
+/* Base-pair probabilities of single sequences on the device: the input
+ * step mlocarna runs per sequence (compute_dotplot_rnafold_pp,
+ * mlocarna:3148-3188 -> locarna_rnafold_pp, locarna_rnafold_pp.cc:209-271:
+ * RnaEnsemble's partition function folding, rna_ensemble.cc:141-193, the
+ * p > min_prob filter of rna_data.cc:509-547 and RnaData::write_pp,
+ * rna_data.cc:1242-1350).  The partition function is McCaskill's, over this
+ * library's reduced nearest-neighbour energy model (canonical pairs,
+ * Turner 2004 Watson-Crick stacks, size-dependent hairpin, bulge and
+ * interior loops up to 30 unpaired bases, linear multiloop, terminal AU/GU
+ * penalties; no dangles), not ViennaRNA's full Turner 2004 set: the
+ * probabilities are McCaskill probabilities of that model.  Sequences are
+ * upper-cased and T read as U (normalize_rna_sequence, aux.cc:65-70);
+ * other symbols do not pair.  LCG_E_CAPACITY for sequences longer than
+ * 4096 or whose partition function exceeds the double range. */
+typedef struct lcg_fold lcg_fold;
+typedef struct {
+    int32_t max_bp_span; /* --maxBPspan: pairs with j - i + 1 <= span; -1: no limit */
+    int32_t stacking;    /* --stacking: joint probabilities of (i,j) and (i+1,j-1) (#STACK) */
+    double min_prob;     /* -p / --min-prob: pairs with p > min_prob are written
+                            (locarna_rnafold_pp 0.001; mlocarna passes its --min-prob) */
+} lcg_fold_params;
+int lcg_fold_run(const char *const *seqs, int32_t count, const lcg_fold_params *params, int32_t device,
+                 lcg_fold **out, char *err, int errlen);
+/* pp text of sequence k under `name` (RnaData::write_pp): pairs with
+ * p > min_prob in order i, j ascending; returns the length, writes the text
+ * with its terminating 0 when cap exceeds it */
+int64_t lcg_fold_write_pp(const lcg_fold *f, int32_t k, const char *name, char *buf, int64_t cap);
+/* pair probabilities P (RnaEnsemble::arc_prob) and joint probabilities P2
+ * of (i,j) and (i+1,j-1) (RnaEnsemble::arc_2_prob, rna_ensemble.cc:415-436)
+ * as dense (n+1) x (n+1) row-major matrices (either may be NULL); returns n */
+int lcg_fold_probs(const lcg_fold *f, int32_t k, double *P, double *P2, int64_t cap);
+/* natural logarithm of the partition function of sequence k */
+int lcg_fold_log_z(const lcg_fold *f, int32_t k, double *log_z);
+/* device time of the folding kernel, wall time of lcg_fold_run */
+int lcg_fold_stats(const lcg_fold *f, float *device_ms, float *wall_ms);
+void lcg_fold_free(lcg_fold *f);
+
 /* Synthetic inputs (no counterpart in the reference; they replace
  * RNAfold -p, which the reference calls through ViennaRNA,
  * rna_data.cc:52-61). */

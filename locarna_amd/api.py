@@ -80,6 +80,12 @@ class LcgError(RuntimeError):
     pass
 
 
+class FoldParams(ctypes.Structure):
+    """lcg_fold_params (include/locarna_amd.h): locarna_rnafold_pp's
+    --maxBPspan, --stacking and -p/--min-prob."""
+    _fields_ = [("max_bp_span", ctypes.c_int32), ("stacking", ctypes.c_int32), ("min_prob", ctypes.c_double)]
+
+
 def _sig(L):
     c = ctypes
     L.lcg_pair_prepare.argtypes = [c.c_char_p, c.c_char_p, c.POINTER(Params), c.POINTER(c.c_void_p),
@@ -145,6 +151,15 @@ def _sig(L):
     L.lcg_progressive_stats.argtypes = [c.c_void_p] + [c.c_void_p] * 5
     L.lcg_progressive_free.argtypes = [c.c_void_p]
     L.lcg_progressive_free.restype = None
+    L.lcg_fold_run.argtypes = [c.POINTER(c.c_char_p), c.c_int32, c.POINTER(FoldParams), c.c_int32,
+                               c.POINTER(c.c_void_p), c.c_char_p, c.c_int]
+    L.lcg_fold_write_pp.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_char_p, c.c_int64]
+    L.lcg_fold_write_pp.restype = c.c_int64
+    L.lcg_fold_probs.argtypes = [c.c_void_p, c.c_int32, c.c_void_p, c.c_void_p, c.c_int64]
+    L.lcg_fold_log_z.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_double)]
+    L.lcg_fold_stats.argtypes = [c.c_void_p, c.POINTER(c.c_float), c.POINTER(c.c_float)]
+    L.lcg_fold_free.argtypes = [c.c_void_p]
+    L.lcg_fold_free.restype = None
     L.lcg_batch_write_am_probs.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64]
     L.lcg_batch_write_am_probs.restype = c.c_int64
     L.lcg_batch_write_bm_probs.argtypes = [c.c_void_p, c.c_int32, c.c_int32, c.c_char_p, c.c_int64]
@@ -705,6 +720,68 @@ class Progressive:
     def destroy(self):
         if self.h:
             lib().lcg_progressive_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Fold:
+    """Base-pair probabilities of single sequences on one GPU
+    (lcg_fold_run; the step locarna_rnafold_pp performs for mlocarna,
+    src/locarna_rnafold_pp.cc:209-271): pp(k, name), probs(k), log_z(k),
+    stats()."""
+
+    def __init__(self, seqs, max_bp_span=-1, min_prob=0.001, stacking=False, device=0):
+        n = len(seqs)
+        arr = (ctypes.c_char_p * n)(*[s.encode() for s in seqs])
+        fp = FoldParams(max_bp_span, 1 if stacking else 0, min_prob)
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        self.h = None
+        self.lens = [len(s) for s in seqs]
+        rc = lib().lcg_fold_run(arr, n, ctypes.byref(fp), device, ctypes.byref(h), err, 1024)
+        if rc != 0:
+            raise LcgError("lcg_fold_run (%d): %s" % (rc, err.value.decode()))
+        self.h = h
+
+    def pp(self, k, name):
+        need = lib().lcg_fold_write_pp(self.h, k, name.encode(), None, 0)
+        if need < 0:
+            raise LcgError("lcg_fold_write_pp (%d)" % need)
+        buf = ctypes.create_string_buffer(need + 1)
+        lib().lcg_fold_write_pp(self.h, k, name.encode(), buf, need + 1)
+        return buf.value.decode()
+
+    def probs(self, k):
+        """(P, P2) as dense (n+1) x (n+1) numpy arrays."""
+        import numpy as np
+        n = self.lens[k]
+        P = np.zeros((n + 1, n + 1))
+        P2 = np.zeros((n + 1, n + 1))
+        rc = lib().lcg_fold_probs(self.h, k, P.ctypes.data, P2.ctypes.data, (n + 1) * (n + 1))
+        if rc != n:
+            raise LcgError("lcg_fold_probs (%d)" % rc)
+        return P, P2
+
+    def log_z(self, k):
+        v = ctypes.c_double()
+        rc = lib().lcg_fold_log_z(self.h, k, ctypes.byref(v))
+        if rc != 0:
+            raise LcgError("lcg_fold_log_z (%d)" % rc)
+        return v.value
+
+    def stats(self):
+        d, w = ctypes.c_float(), ctypes.c_float()
+        lib().lcg_fold_stats(self.h, ctypes.byref(d), ctypes.byref(w))
+        return dict(device_ms=d.value, wall_ms=w.value)
+
+    def destroy(self):
+        if self.h:
+            lib().lcg_fold_free(self.h)
             self.h = None
 
     def __del__(self):

@@ -23,12 +23,14 @@ HOST_SOURCES = [
 # host code that calls the HIP runtime API (no kernels): compiled by g++
 HIPAPI_SOURCES = [
     "batch.cpp",
+    "fold.cpp",
 ]
 HIP_SOURCES = [
     "align_kernels.hip",
     "pf_kernels.hip",
     "compact_kernels.hip",
     "band_kernels.hip",
+    "fold_kernels.hip",
 ]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

@@ -172,6 +172,12 @@ long long oracle_consensus_pp(const char *ppA, const char *ppB, const oracle_par
 /* probability text of pp files (format_prob, rna_data.cc:1274-1303) */
 int oracle_format_prob(double prob, char *buf, int cap);
 
+/* Base-pair probabilities of one sequence (oracle/fold.cc: McCaskill over
+ * the library's energy model): P and P2 dense (n+1)*(n+1) row major (P2:
+ * joint probability of (i,j) and (i+1,j-1); either may be NULL), log Z.
+ * max_span < 0: no span limit.  Returns n or -1. */
+int oracle_fold(const char *seq, int max_span, double *P, double *P2, double *log_z);
+
 /* CPU baseline: align many pairs with nthreads worker threads (scores
  * only, with traceback).  Returns number of pairs done or <0. */
 int oracle_align_many(const char **ppA, const char **ppB, int npairs,

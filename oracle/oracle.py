@@ -356,3 +356,51 @@ def progressive(pps, merges, params=None):
         out.append((r.score, r.edges))
         text[n + t] = consensus_pp(text[a], text[b], r.edges, params)
     return (text[-1] if merges else pps[0]), out
+
+
+def fold(seq, max_span=-1):
+    """Base-pair probabilities of one sequence (McCaskill over the library's
+    energy model, oracle/fold.cc): (P, P2, log Z) with P and P2 dense
+    (n+1) x (n+1) numpy arrays (P2: joint probability of (i,j) and
+    (i+1,j-1))."""
+    import numpy as np
+    L = load()
+    L.oracle_fold.restype = ctypes.c_int
+    L.oracle_fold.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                              ctypes.POINTER(ctypes.c_double)]
+    n = len(seq)
+    P = np.zeros((n + 1, n + 1))
+    P2 = np.zeros((n + 1, n + 1))
+    lz = ctypes.c_double()
+    r = L.oracle_fold(seq.encode(), max_span, P.ctypes.data, P2.ctypes.data, ctypes.byref(lz))
+    if r != n:
+        raise ValueError("oracle_fold failed")
+    return P, P2, lz.value
+
+
+def fold_many(seqs, max_span=-1, nthreads=1):
+    """fold() of every sequence on nthreads host threads (the C function
+    runs without the interpreter lock)."""
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=max(1, nthreads)) as ex:
+        return list(ex.map(lambda s: fold(s, max_span), seqs))
+
+
+def fold_pp(name, seq, P, P2, min_prob, stacking=False):
+    """The pp text locarna_rnafold_pp writes for these probabilities
+    (RnaData::write_pp, rna_data.cc:1242-1350; pairs with p > min_prob,
+    i and j ascending, the joint probability where stacking and above the
+    cut)."""
+    seq = seq.upper().replace("T", "U")
+    lines = ["#PP 2.0", "", "%-18s %s" % (name, seq), "", "#END", "", "#SECTION BASEPAIRS", "",
+             "#BPCUT " + format_prob(min_prob)] + (["#STACK"] if stacking else []) + [""]
+    n = len(seq)
+    for i in range(1, n + 1):
+        for j in range(i + 1, n + 1):
+            p = P[i][j]
+            if p > min_prob:
+                s = "%d %d %s" % (i, j, format_prob(p))
+                if stacking and P2[i][j] > min_prob:
+                    s += " " + format_prob(P2[i][j])
+                lines.append(s)
+    return "\n".join(lines + ["", "#END", ""])
