@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no lcg_batch_autotune before timing)")
     ap.add_argument("--only", default=None,
-                    choices=["long500_global", "long500_local", "pf16x150", "grad256", "progressive64"],
+                    choices=["long500_global", "long500_local", "pf16x150", "grad256", "progressive64", "fold64"],
                     help="run only this secondary configuration and print its record (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -331,6 +331,8 @@ def extra_configs(args, api, synth, torch, nthreads, only=None):
     res = {}
     if not only or only == "progressive64":
         res["progressive64"] = progressive_stage(api, synth, oracle)
+    if not only or only == "fold64":
+        res["fold64"] = fold_stage(api, synth, oracle, cpu_threads=nthreads)
     for local in (0, 1):
         if only and only != "long500_" + ("local" if local else "global"):
             continue
@@ -484,6 +486,40 @@ def progressive_stage(api, synth, oracle, cpu_budget_s=15.0):
             "cpu_baseline_merges_per_sec": round(done / dt, 3), "cpu_cores": 1,
             "cpu_sample": "%d of %d merges in tree order (CPU oracle, one core)" % (done, n - 1),
             "consensus_bit_exact": "%d/%d" % (ok, done)}
+
+
+def fold_stage(api, synth, oracle, reps=5, cpu_threads=16):
+    """mlocarna's input step before the all-pairs stage: base-pair
+    probabilities of the 64 allpairs64 sequences (60-200 nt, pairs of span
+    <= 100, pairs with p > 0.0005 written as pp texts), one kernel launch for
+    all of them.  The CPU oracle folds the same 64 sequences on
+    cpu_threads host threads; every GPU probability is checked against it."""
+    seqs = synth.sequences(2, 64, 60, 200, 0.35, 0.65)
+    api.Fold(seqs, max_bp_span=100, min_prob=synth.BPCUT).destroy()  # warm-up
+    dev, wall = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        F = api.Fold(seqs, max_bp_span=100, min_prob=synth.BPCUT)
+        texts = [F.pp(k, "s%03d" % k) for k in range(len(seqs))]
+        wall.append(time.perf_counter() - t0)
+        dev.append(F.stats()["device_ms"])
+        if _ < reps - 1:
+            F.destroy()
+    t0 = time.perf_counter()
+    ref = oracle.fold_many(seqs, 100, cpu_threads)
+    dt = time.perf_counter() - t0
+    err = 0.0
+    for k in range(len(seqs)):
+        P, P2 = F.probs(k)
+        err = max(err, float(abs(P - ref[k][0]).max()), float(abs(P2 - ref[k][1]).max()))
+    F.destroy()
+    dms = sorted(dev)[len(dev) // 2]
+    return {"sequences": len(seqs), "nt": sum(len(x) for x in seqs), "max_bp_span": 100,
+            "kernel_ms": round(dms, 3), "wall_ms_with_pp_texts": round(sorted(wall)[len(wall) // 2] * 1e3, 2),
+            "sequences_per_sec": round(len(seqs) / (dms / 1e3), 1),
+            "cpu_baseline_sequences_per_sec": round(len(seqs) / dt, 2), "cpu_cores": cpu_threads,
+            "cpu_sample": "all 64 sequences (CPU oracle, %d threads)" % cpu_threads,
+            "max_abs_diff_vs_oracle": err, "pp_text_bytes": sum(len(t) for t in texts)}
 
 
 if __name__ == "__main__":

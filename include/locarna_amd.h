@@ -527,7 +527,9 @@ int lcg_fold_run(const char *const *seqs, int32_t count, const lcg_fold_params *
                  lcg_fold **out, char *err, int errlen);
 /* pp text of sequence k under `name` (RnaData::write_pp): pairs with
  * p > min_prob in order i, j ascending; returns the length, writes the text
- * with its terminating 0 when cap exceeds it */
+ * with its terminating 0 when cap exceeds it.  The handle keeps the last
+ * text per sequence (a length query followed by the copy builds it once),
+ * so calls on one handle must not overlap. */
 int64_t lcg_fold_write_pp(const lcg_fold *f, int32_t k, const char *name, char *buf, int64_t cap);
 /* pair probabilities P (RnaEnsemble::arc_prob) and joint probabilities P2
  * of (i,j) and (i+1,j-1) (RnaEnsemble::arc_2_prob, rna_ensemble.cc:415-436)

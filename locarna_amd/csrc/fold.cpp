@@ -34,6 +34,9 @@ struct lcg_fold {
     std::vector<double> bands;     // P and P2 bands of every sequence
     std::vector<double> Z;
     float device_ms = 0, wall_ms = 0;
+    // the last pp text written per sequence and its name: a caller asks for
+    // the length first and then for the text
+    mutable std::vector<std::string> pp_name, pp_text;
 };
 
 namespace {
@@ -236,6 +239,15 @@ extern "C" int lcg_fold_log_z(const lcg_fold *f, int32_t k, double *log_z) {
 
 extern "C" int64_t lcg_fold_write_pp(const lcg_fold *f, int32_t k, const char *name, char *buf, int64_t cap) {
     if (!f || !name || k < 0 || k >= (int)f->seq.size()) return LCG_E_ARG;
+    if (f->pp_name.size() != f->seq.size()) {
+        f->pp_name.assign(f->seq.size(), std::string());
+        f->pp_text.assign(f->seq.size(), std::string());
+    }
+    if (!f->pp_text[k].empty() && f->pp_name[k] == name) {
+        const std::string &text = f->pp_text[k];
+        if (buf && cap > (int64_t)text.size()) std::memcpy(buf, text.c_str(), text.size() + 1);
+        return (int64_t)text.size();
+    }
     const std::string &s = f->seq[k];
     const int n = (int)s.size(), W = f->W[k];
     const size_t sz = (size_t)(n + 2) * W;
@@ -267,7 +279,9 @@ extern "C" int64_t lcg_fold_write_pp(const lcg_fold *f, int32_t k, const char *n
         }
     text += "\n#END\n";
     if (buf && cap > (int64_t)text.size()) std::memcpy(buf, text.c_str(), text.size() + 1);
-    return (int64_t)text.size();
+    f->pp_name[k] = name;
+    f->pp_text[k] = std::move(text);
+    return (int64_t)f->pp_text[k].size();
 }
 
 extern "C" int lcg_fold_stats(const lcg_fold *f, float *device_ms, float *wall_ms) {

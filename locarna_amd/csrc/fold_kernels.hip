@@ -4,9 +4,11 @@
 // RnaEnsemble, rna_ensemble.cc:141-193; mlocarna:3148-3188).
 //
 // One workgroup of 16 waves folds one sequence.  The inside pass walks the
-// diagonals d = j - i upward; the cells of a diagonal are independent, one
-// wave per cell, and the lanes of the wave split the cell's terms (interior
-// loops, multiloop splits) and sum them with a butterfly.  The exterior
+// diagonals d = j - i upward; the cells of a diagonal are independent, a
+// quarter wave (16 lanes) per cell, and the lanes of a cell split its terms
+// (interior loops, multiloop splits) and sum them with a butterfly; 16
+// lanes per cell measured faster than 32 or 64 on 64 sequences of 60-200 nt
+// (the work is latency-bound: more cells in flight per wave).  The exterior
 // prefix sums Z5 (forward) and Z5b (their adjoints) run on one wave.  The
 // outside pass walks the diagonals downward and gathers, for every cell, the
 // terms of the enclosing cells that use it (the pull form of the adjoint of
@@ -28,10 +30,16 @@
 namespace {
 
 constexpr int FT = LCG_FOLD_THREADS;
+#ifndef LCG_FOLD_CELL_LANES
+#define LCG_FOLD_CELL_LANES 16
+#endif
+// lanes per cell: a wave works on 64 / CL cells of a diagonal at once
+constexpr int CL = LCG_FOLD_CELL_LANES;
 
-__device__ __forceinline__ double wave_sum(double v) {
+// sum over the CL lanes of a cell
+__device__ __forceinline__ double cell_sum(double v) {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    for (int o = CL / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
 
@@ -75,6 +83,7 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
     unsigned short *LL = (unsigned short *)(ptab + 28);  // LCG_FOLD_NLOOP: l1 | l2 << 8
     signed char *cs = (signed char *)(LL + LCG_FOLD_NLOOP + 8);  // n + 2
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = FT / 64;
+    const int cl = lane % CL, cg = wave * (64 / CL) + lane / CL, ncg = nw * (64 / CL);
 
     for (int x = tid; x < LCG_FOLD_TAB_FIXED; x += FT) T[x] = tabs[x];
     for (int x = tid; x < n + 2; x += FT) cs[x] = (x >= 1 && x <= n) ? codes[D.codes + x] : (signed char)4;
@@ -108,7 +117,7 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
 
     // ---------------- inside
     for (int d = 0; d < W; ++d) {
-        for (int c = wave; c < n - d; c += nw) {
+        for (int c = cg; c < n - d; c += ncg) {
             const int i = 1 + c, j = i + d;
             double qb = 0, qm1 = 0, qm = 0;
             if (d >= 4) {
@@ -116,33 +125,33 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                 if (t) {
                     double s = 0;
                     const int cnt = loop_count(min(LCG_FOLD_MAXLOOP, d - 6));
-                    for (int e = lane; e < cnt; e += 64) {
+                    for (int e = cl; e < cnt; e += CL) {
                         const int l1 = LL[e] & 255, l2 = LL[e] >> 8;
                         const int k = i + 1 + l1, l = j - 1 - l2;
                         const int ti = S.pt(k, l);
                         if (ti) s += loopf(T, l1, l2, t, ti) * Qb[(size_t)k * W + (l - k)];
                     }
                     double mq = 0;
-                    for (int u = i + 6 + lane; u <= j - 5; u += 64)
+                    for (int u = i + 6 + cl; u <= j - 5; u += CL)
                         mq += Qm[(size_t)(i + 1) * W + (u - 2 - i)] * Qm1[(size_t)u * W + (j - 1 - u)];
-                    s = wave_sum(s);
-                    mq = wave_sum(mq);
+                    s = cell_sum(s);
+                    mq = cell_sum(mq);
                     qb = hp[(d - 1) * 2 + weak(t)] + s + mq * mlc * T[LCG_FOLD_TAB_TP + weak(t)];
                 }
                 double a1 = 0;
-                for (int l = i + 4 + lane; l < j; l += 64) {
+                for (int l = i + 4 + cl; l < j; l += CL) {
                     const int tl = S.pt(i, l);
                     if (tl) a1 += Qb[(size_t)i * W + (l - i)] * T[LCG_FOLD_TAB_BRANCH + weak(tl)];
                 }
-                a1 = wave_sum(a1);
+                a1 = cell_sum(a1);
                 if (t) a1 += qb * T[LCG_FOLD_TAB_BRANCH + weak(t)];
                 qm1 = a1;
                 double a2 = 0;
-                for (int u = i + 1 + lane; u <= j - 4; u += 64)
+                for (int u = i + 1 + cl; u <= j - 4; u += CL)
                     a2 += (1.0 + Qm[(size_t)i * W + (u - 1 - i)]) * Qm1[(size_t)u * W + (j - u)];
-                qm = wave_sum(a2) + qm1;
+                qm = cell_sum(a2) + qm1;
             }
-            if (lane == 0) {
+            if (cl == 0) {
                 const size_t x = (size_t)i * W + d;
                 Qb[x] = qb;
                 Qm1[x] = qm1;
@@ -152,30 +161,58 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
         __syncthreads();
     }
 
-    // ---------------- exterior loop: Z5 forward, then its adjoints
+    // ---------------- exterior loop: Z5 forward, then its adjoints.  The
+    // sum of column j reads Z5 up to j - 5 only, so four consecutive columns
+    // are summed at once (a quarter wave each) and chained in order.
     if (wave == 0) {
-        double zprev = 1.0;
+        const int g = lane >> 4, gl = lane & 15;
+        double zlast = 1.0;
         if (lane == 0) Z5[0] = 1.0;
-        for (int j = 1; j <= n; ++j) {
-            double z = 0;
-            for (int k = max(1, j - W + 1) + lane; k <= j - 4; k += 64) {
-                const int t = S.pt(k, j);
-                if (t) z += Z5[k - 1] * Qb[(size_t)k * W + (j - k)] * T[LCG_FOLD_TAB_EXT + weak(t)];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        for (int j0 = 1; j0 <= n; j0 += 4) {
+            const int j = j0 + g;
+            double f = 0;
+            if (j <= n)
+                for (int k = max(1, j - W + 1) + gl; k <= j - 4; k += 16) {
+                    const int t = S.pt(k, j);
+                    if (t) f += Z5[k - 1] * Qb[(size_t)k * W + (j - k)] * T[LCG_FOLD_TAB_EXT + weak(t)];
+                }
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) f += __shfl_xor(f, o);
+            const double z0 = zlast + __shfl(f, 0), z1 = z0 + __shfl(f, 16), z2 = z1 + __shfl(f, 32),
+                         z3 = z2 + __shfl(f, 48);
+            if (lane == 0) {
+                Z5[j0] = z0;
+                if (j0 + 1 <= n) Z5[j0 + 1] = z1;
+                if (j0 + 2 <= n) Z5[j0 + 2] = z2;
+                if (j0 + 3 <= n) Z5[j0 + 3] = z3;
             }
-            zprev += wave_sum(z);
-            if (lane == 0) Z5[j] = zprev;
+            zlast = j0 + 3 <= n ? z3 : j0 + 2 <= n ? z2 : j0 + 1 <= n ? z1 : z0;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         }
-        double gprev = 1.0;
+        // Z5b(m) = Z5b(m+1) + sum_j Z5b(j) Qb(m+1,j) EXT reads Z5b from m + 5 up
+        double glast = 1.0;
         if (lane == 0) Z5b[n] = 1.0;
-        for (int m = n - 1; m >= 0; --m) {
-            double g = 0;
-            for (int j = m + 5 + lane; j <= min(n, m + W); j += 64) {
-                const int t = S.pt(m + 1, j);
-                if (t) g += Z5b[j] * Qb[(size_t)(m + 1) * W + (j - m - 1)] * T[LCG_FOLD_TAB_EXT + weak(t)];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+        for (int m0 = n - 1; m0 >= 0; m0 -= 4) {
+            const int m = m0 - g;
+            double f = 0;
+            if (m >= 0)
+                for (int j = m + 5 + gl; j <= min(n, m + W); j += 16) {
+                    const int t = S.pt(m + 1, j);
+                    if (t) f += Z5b[j] * Qb[(size_t)(m + 1) * W + (j - m - 1)] * T[LCG_FOLD_TAB_EXT + weak(t)];
+                }
+#pragma unroll
+            for (int o = 8; o > 0; o >>= 1) f += __shfl_xor(f, o);
+            const double z0 = glast + __shfl(f, 0), z1 = z0 + __shfl(f, 16), z2 = z1 + __shfl(f, 32),
+                         z3 = z2 + __shfl(f, 48);
+            if (lane == 0) {
+                Z5b[m0] = z0;
+                if (m0 - 1 >= 0) Z5b[m0 - 1] = z1;
+                if (m0 - 2 >= 0) Z5b[m0 - 2] = z2;
+                if (m0 - 3 >= 0) Z5b[m0 - 3] = z3;
             }
-            gprev += wave_sum(g);
-            if (lane == 0) Z5b[m] = gprev;
+            glast = m0 - 3 >= 0 ? z3 : m0 - 2 >= 0 ? z2 : m0 - 1 >= 0 ? z1 : z0;
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
         }
         if (lane == 0) zout[sq] = Z5[n];
@@ -185,40 +222,40 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
 
     // ---------------- outside (gathered adjoints), probabilities
     for (int d = W - 1; d >= 0; --d) {
-        for (int c = wave; c < n - d; c += nw) {
+        for (int c = cg; c < n - d; c += ncg) {
             const int a = 1 + c, e = a + d;
             double qmb = 0, qm1b = 0, qbb = 0, p = 0, p2 = 0;
             if (d >= 4) {
                 // Qm(a, e): inside Qb(a-1, j) as Qm(i+1, u-1), inside Qm(a, j) as Qm(i, u-1)
                 double s = 0;
                 if (a >= 2)
-                    for (int j = e + 6 + lane; j <= min(n, a + W - 2); j += 64) {
+                    for (int j = e + 6 + cl; j <= min(n, a + W - 2); j += CL) {
                         const int t = S.pt(a - 1, j);
                         if (t)
                             s += Qbb[(size_t)(a - 1) * W + (j - a + 1)] * mlc * T[LCG_FOLD_TAB_TP + weak(t)] *
                                  Qm1[(size_t)(e + 1) * W + (j - e - 2)];
                     }
-                for (int j = e + 5 + lane; j <= min(n, a + W - 1); j += 64)
+                for (int j = e + 5 + cl; j <= min(n, a + W - 1); j += CL)
                     s += Qmb[(size_t)a * W + (j - a)] * Qm1[(size_t)(e + 1) * W + (j - e - 1)];
-                qmb = wave_sum(s);
+                qmb = cell_sum(s);
                 // Qm1(a, e): inside Qb(i, e+1) as Qm1(u, j-1), inside Qm(i, e) as Qm1(u, j)
                 s = 0;
                 if (e + 1 <= n)
-                    for (int i = max(1, e + 2 - W) + lane; i <= a - 6; i += 64) {
+                    for (int i = max(1, e + 2 - W) + cl; i <= a - 6; i += CL) {
                         const int t = S.pt(i, e + 1);
                         if (t)
                             s += Qbb[(size_t)i * W + (e + 1 - i)] * mlc * T[LCG_FOLD_TAB_TP + weak(t)] *
                                  Qm[(size_t)(i + 1) * W + (a - 2 - i)];
                     }
-                for (int i = max(1, e - W + 1) + lane; i <= a - 1; i += 64)
+                for (int i = max(1, e - W + 1) + cl; i <= a - 1; i += CL)
                     s += Qmb[(size_t)i * W + (e - i)] * (1.0 + Qm[(size_t)i * W + (a - 1 - i)]);
-                qm1b = wave_sum(s) + qmb;
+                qm1b = cell_sum(s) + qmb;
                 // Qb(a, e): exterior loop, enclosing pairs, multiloop branches
                 const int t = S.pt(a, e);
                 if (t) {
                     double si = 0, sb = 0;
                     const int cnt = loop_count(min(LCG_FOLD_MAXLOOP, W - 3 - d));
-                    for (int q = lane; q < cnt; q += 64) {
+                    for (int q = cl; q < cnt; q += CL) {
                         const int l1 = LL[q] & 255, l2 = LL[q] >> 8;
                         const int i = a - 1 - l1, j = e + 1 + l2;
                         if (i >= 1 && j <= n) {
@@ -226,9 +263,9 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                             if (to) si += Qbb[(size_t)i * W + (j - i)] * loopf(T, l1, l2, to, t);
                         }
                     }
-                    for (int j = e + 1 + lane; j <= min(n, a + W - 1); j += 64) sb += Qm1b[(size_t)a * W + (j - a)];
-                    si = wave_sum(si);
-                    sb = wave_sum(sb);
+                    for (int j = e + 1 + cl; j <= min(n, a + W - 1); j += CL) sb += Qm1b[(size_t)a * W + (j - a)];
+                    si = cell_sum(si);
+                    sb = cell_sum(sb);
                     qbb = Z5b[e] * Z5[a - 1] * T[LCG_FOLD_TAB_EXT + weak(t)] + si +
                           T[LCG_FOLD_TAB_BRANCH + weak(t)] * (sb + qm1b);
                     const double qb = Qb[(size_t)a * W + d];
@@ -240,7 +277,7 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                     }
                 }
             }
-            if (lane == 0) {
+            if (cl == 0) {
                 const size_t x = (size_t)a * W + d;
                 Qmb[x] = qmb;
                 Qm1b[x] = qm1b;
