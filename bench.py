@@ -49,7 +49,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no lcg_batch_autotune before timing)")
     ap.add_argument("--only", default=None,
-                    choices=["long500_global", "long500_local", "pf16x150", "grad256", "progressive64", "fold64"],
+                    choices=["long500_global", "long500_local", "pf16x150", "grad256", "progressive64", "fold64", "mlocarna64"],
                     help="run only this secondary configuration and print its record (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -333,6 +333,8 @@ def extra_configs(args, api, synth, torch, nthreads, only=None):
         res["progressive64"] = progressive_stage(api, synth, oracle)
     if not only or only == "fold64":
         res["fold64"] = fold_stage(api, synth, oracle, cpu_threads=nthreads)
+    if not only or only == "mlocarna64":
+        res["mlocarna64"] = mlocarna_stage(api, synth, oracle, cpu_threads=nthreads)
     for local in (0, 1):
         if only and only != "long500_" + ("local" if local else "global"):
             continue
@@ -520,6 +522,47 @@ def fold_stage(api, synth, oracle, reps=5, cpu_threads=16):
             "cpu_baseline_sequences_per_sec": round(len(seqs) / dt, 2), "cpu_cores": cpu_threads,
             "cpu_sample": "all 64 sequences (CPU oracle, %d threads)" % cpu_threads,
             "max_abs_diff_vs_oracle": err, "pp_text_bytes": sum(len(t) for t in texts)}
+
+
+def mlocarna_stage(api, synth, oracle, cpu_threads=16):
+    """mlocarna's whole default run from the 64 allpairs64 sequences to the
+    final multiple alignment on one GPU (lcg_mlocarna_run: folding, the
+    2016 pairs, UPGMA, 63 progressive merges).  The CPU oracle runs the same
+    pipeline (folding and all pairs on cpu_threads threads, the merges on
+    one core, as mlocarna runs them in tree order) and its final consensus
+    text is compared with the GPU's."""
+    seqs = synth.sequences(2, 64, 60, 200, 0.35, 0.65)
+    names = ["s%03d" % k for k in range(len(seqs))]
+    n = len(seqs)
+    api.Mlocarna(names, seqs, max_bp_span=100).destroy()  # warm-up
+    t0 = time.perf_counter()
+    M = api.Mlocarna(names, seqs, max_bp_span=100)
+    wall = time.perf_counter() - t0
+    st = M.stats()
+    final = M.pp(2 * n - 2)
+    merges = M.tree()
+    M.destroy()
+    po = oracle.locarna_params(max_bp_span=100)
+    t0 = time.perf_counter()
+    folded = oracle.fold_many(seqs, 100, cpu_threads)
+    pps = [oracle.fold_pp(names[k], seqs[k], folded[k][0], folded[k][1], 0.0005) for k in range(n)]
+    pairs = [(a, b) for a in range(n) for b in range(a)]
+    sc = oracle.align_many([(pps[a], pps[b]) for a, b in pairs], po, cpu_threads)
+    S = [[0] * n for _ in range(n)]
+    for (a, b), v in zip(pairs, sc):
+        S[a][b] = S[b][a] = v
+    ref_merges = oracle.upgma(S)
+    ref_final, _ = oracle.progressive(pps, ref_merges, po)
+    dt = time.perf_counter() - t0
+    return {"sequences": n, "pairs": len(pairs), "merges": n - 1, "wall_ms": round(wall * 1e3, 2),
+            "fold_ms": round(st["fold_ms"], 2), "pairs_ms": round(st["pairs_ms"], 2),
+            "progressive_ms": round(st["progressive_ms"], 2), "runs_per_sec": round(1.0 / wall, 3),
+            "cpu_baseline_s": round(dt, 2), "cpu_cores": cpu_threads,
+            "cpu_sample": "the whole pipeline (CPU oracle: folding and pairs on %d threads, merges on one core)"
+                          % cpu_threads,
+            "speedup_vs_cpu": round(dt / wall, 1),
+            "same_guide_tree": [tuple(m) for m in ref_merges] == merges,
+            "final_alignment_equal": ref_final == final}
 
 
 if __name__ == "__main__":

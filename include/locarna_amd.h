@@ -541,6 +541,31 @@ int lcg_fold_log_z(const lcg_fold *f, int32_t k, double *log_z);
 int lcg_fold_stats(const lcg_fold *f, float *device_ms, float *wall_ms);
 void lcg_fold_free(lcg_fold *f);
 
+/* mlocarna's default pipeline from sequences to the multiple alignment
+ * (src/Utils/mlocarna: compute_all_dotplots :3099-3110, the all-pairs
+ * stage :3547-3640, the UPGMA guide tree :2352-2420 and the progressive
+ * steps :3660-3707) on one GPU: lcg_fold_run on all inputs (pp texts under
+ * names[k]), every pair a > b prepared together (device band; device
+ * arc-match compaction unless --noLP, stacking or --struct-local) and
+ * aligned in one batch, lcg_guide_tree_upgma on the scores, then
+ * lcg_progressive_run along the tree.  n >= 2. */
+typedef struct lcg_mlocarna lcg_mlocarna;
+int lcg_mlocarna_run(const char *const *names, const char *const *seqs, int32_t n, const lcg_fold_params *fold,
+                     const lcg_params *params, int32_t device, lcg_mlocarna **out, char *err, int errlen);
+/* the result alignment (CLUSTAL text of the last merge) */
+int64_t lcg_mlocarna_clustal(const lcg_mlocarna *r, int32_t width, char *buf, int64_t cap);
+/* pp text of node k: an input (k < n, as lcg_fold_write_pp wrote it) or a
+ * merge (n + t) */
+int64_t lcg_mlocarna_pp(const lcg_mlocarna *r, int32_t node, char *buf, int64_t cap);
+/* all-pairs scores, n x n row major (0 on the diagonal); returns n */
+int lcg_mlocarna_scores(const lcg_mlocarna *r, int64_t *scores, int64_t cap);
+/* the guide tree as lcg_guide_tree_upgma returns it; returns n - 1 */
+int lcg_mlocarna_tree(const lcg_mlocarna *r, int32_t *merges, int32_t cap);
+/* wall times of the three stages and of the whole run */
+int lcg_mlocarna_stats(const lcg_mlocarna *r, float *fold_ms, float *pairs_ms, float *progressive_ms,
+                       float *wall_ms);
+void lcg_mlocarna_free(lcg_mlocarna *r);
+
 /* Synthetic inputs (no counterpart in the reference; they replace
  * RNAfold -p, which the reference calls through ViennaRNA,
  * rna_data.cc:52-61). */

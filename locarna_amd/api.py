@@ -159,6 +159,17 @@ def _sig(L):
     L.lcg_fold_log_z.argtypes = [c.c_void_p, c.c_int32, c.POINTER(c.c_double)]
     L.lcg_fold_stats.argtypes = [c.c_void_p, c.POINTER(c.c_float), c.POINTER(c.c_float)]
     L.lcg_fold_free.argtypes = [c.c_void_p]
+    L.lcg_mlocarna_run.argtypes = [c.POINTER(c.c_char_p), c.POINTER(c.c_char_p), c.c_int32, c.POINTER(FoldParams),
+                                   c.POINTER(Params), c.c_int32, c.POINTER(c.c_void_p), c.c_char_p, c.c_int]
+    L.lcg_mlocarna_clustal.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64]
+    L.lcg_mlocarna_clustal.restype = c.c_int64
+    L.lcg_mlocarna_pp.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64]
+    L.lcg_mlocarna_pp.restype = c.c_int64
+    L.lcg_mlocarna_scores.argtypes = [c.c_void_p, c.c_void_p, c.c_int64]
+    L.lcg_mlocarna_tree.argtypes = [c.c_void_p, c.c_void_p, c.c_int32]
+    L.lcg_mlocarna_stats.argtypes = [c.c_void_p] + [c.POINTER(c.c_float)] * 4
+    L.lcg_mlocarna_free.argtypes = [c.c_void_p]
+    L.lcg_mlocarna_free.restype = None
     L.lcg_fold_free.restype = None
     L.lcg_batch_write_am_probs.argtypes = [c.c_void_p, c.c_int32, c.c_char_p, c.c_int64]
     L.lcg_batch_write_am_probs.restype = c.c_int64
@@ -782,6 +793,69 @@ class Fold:
     def destroy(self):
         if self.h:
             lib().lcg_fold_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+class Mlocarna:
+    """mlocarna's default pipeline from sequences on one GPU
+    (lcg_mlocarna_run): base-pair probabilities, all pairs, UPGMA guide
+    tree, progressive alignment.  clustal(), pp(node), scores(), tree(),
+    stats()."""
+
+    def __init__(self, names, seqs, params=None, max_bp_span=-1, min_prob=0.0005, device=0):
+        params = params or locarna_params(max_bp_span=max_bp_span, min_prob=min_prob)
+        n = len(seqs)
+        self.n = n
+        nm = (ctypes.c_char_p * n)(*[x.encode() for x in names])
+        sq = (ctypes.c_char_p * n)(*[x.encode() for x in seqs])
+        fp = FoldParams(max_bp_span, 0, min_prob)
+        h = ctypes.c_void_p()
+        err = ctypes.create_string_buffer(1024)
+        self.h = None
+        rc = lib().lcg_mlocarna_run(nm, sq, n, ctypes.byref(fp), ctypes.byref(params), device, ctypes.byref(h),
+                                    err, 1024)
+        if rc != 0:
+            raise LcgError("lcg_mlocarna_run (%d): %s" % (rc, err.value.decode()))
+        self.h = h
+
+    def _text(self, fn, *args):
+        need = fn(self.h, *args, None, 0)
+        if need < 0:
+            raise LcgError("%s (%d)" % (fn.__name__, need))
+        buf = ctypes.create_string_buffer(need + 1)
+        fn(self.h, *args, buf, need + 1)
+        return buf.value.decode()
+
+    def clustal(self, width=120):
+        return self._text(lib().lcg_mlocarna_clustal, width)
+
+    def pp(self, node):
+        return self._text(lib().lcg_mlocarna_pp, node)
+
+    def scores(self):
+        out = (ctypes.c_int64 * (self.n * self.n))()
+        lib().lcg_mlocarna_scores(self.h, out, self.n * self.n)
+        return [list(out[a * self.n:(a + 1) * self.n]) for a in range(self.n)]
+
+    def tree(self):
+        out = (ctypes.c_int32 * (2 * (self.n - 1)))()
+        lib().lcg_mlocarna_tree(self.h, out, 2 * (self.n - 1))
+        return [(out[2 * t], out[2 * t + 1]) for t in range(self.n - 1)]
+
+    def stats(self):
+        v = [ctypes.c_float() for _ in range(4)]
+        lib().lcg_mlocarna_stats(self.h, *[ctypes.byref(x) for x in v])
+        return dict(zip(("fold_ms", "pairs_ms", "progressive_ms", "wall_ms"), [x.value for x in v]))
+
+    def destroy(self):
+        if self.h:
+            lib().lcg_mlocarna_free(self.h)
             self.h = None
 
     def __del__(self):

@@ -19,6 +19,7 @@ HOST_SOURCES = [
     "synth.cpp",
     "host.cpp",
     "progressive.cpp",
+    "mlocarna.cpp",
 ]
 # host code that calls the HIP runtime API (no kernels): compiled by g++
 HIPAPI_SOURCES = [
