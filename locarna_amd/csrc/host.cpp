@@ -8,6 +8,7 @@
 #include <cstring>
 #include <limits>
 #include <sstream>
+#include <thread>
 
 #include "ribosum85_60.h"
 
@@ -359,7 +360,14 @@ std::vector<double> trace_probs(const RnaData &RA, const RnaData &RB, const Trac
     for (int l = 0; l < rB; ++l)
         for (int j = 1; j <= m; ++j) cB[(size_t)l * (m + 1) + j] = nt_code(RB.row(l)[j - 1]);
     std::vector<double> fw((size_t)(n + 1) * (m + 1), 0.0);
-    for (int i = 1; i <= n; ++i) {
+    // rows of the similarity on several threads for large profile pairs
+    // (each cell's sum keeps its row order, so the values do not change)
+    const double work = (double)rA * rB * n * m;
+    const int nth = (rA * rB > 1 && work >= 4e6)
+                        ? (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()))
+                        : 1;
+    auto rows = [&](int t) {
+    for (int i = 1 + t; i <= n; i += nth) {
         for (int j = 1; j <= m; ++j) {
             double seq = 0;
             int cnt = 0;
@@ -377,6 +385,14 @@ std::vector<double> trace_probs(const RnaData &RA, const RnaData &RB, const Trac
             double s = sw * (std::sqrt(dnA[i] * dnB[j]) + std::sqrt(upA[i] * upB[j])) + seq;
             fw[(size_t)i * (m + 1) + j] = std::exp(s / temp);
         }
+    }
+    };
+    if (nth == 1) {
+        rows(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (int t = 0; t < nth; ++t) pool.emplace_back(rows, t);
+        for (auto &th : pool) th.join();
     }
     GotohPF<T> F{n, m, local, std::exp(gap_open / temp), std::exp(gap_ext / temp), fe};
     gotoh_forward(F, fw, tc.min_cols(), tc.max_cols());
@@ -511,13 +527,26 @@ Scoring::Scoring(const RnaData &A, const RnaData &B, const ArcMatches &am, const
         std::vector<score_t> tab(256 * 256);
         for (int x = 0; x < 256; ++x)
             for (int y = 0; y < 256; ++y) tab[x * 256 + y] = symbol_score((char)x, (char)y);
+        // the integer sum over row pairs equals the sum over pairs of
+        // distinct symbols weighted by their counts in the two columns
+        auto histogram = [](const RnaData &R, int len) {
+            std::vector<std::vector<std::pair<unsigned char, score_t>>> h(len + 1);
+            for (int i = 1; i <= len; ++i)
+                for (int k = 0; k < R.num_rows(); ++k) {
+                    const unsigned char c = (unsigned char)R.row(k)[i - 1];
+                    bool found = false;
+                    for (auto &e : h[i])
+                        if (e.first == c) ++e.second, found = true;
+                    if (!found) h[i].push_back({c, 1});
+                }
+            return h;
+        };
+        const auto hA = histogram(A, n), hB = histogram(B, m);
         for (int i = 1; i <= n; ++i)
             for (int j = 1; j <= m; ++j) {
                 score_t sum = 0;
-                for (int k = 0; k < rA; ++k) {
-                    const unsigned char ca = (unsigned char)A.row(k)[i - 1];
-                    for (int l = 0; l < rB; ++l) sum += tab[ca * 256 + (unsigned char)B.row(l)[j - 1]];
-                }
+                for (const auto &x : hA[i])
+                    for (const auto &y : hB[j]) sum += x.second * y.second * tab[x.first * 256 + y.first];
                 sigma_tab_[(size_t)i * (m + 1) + j] = round2score((double)(sum / (int)(rA * rB)));
             }
     } else {

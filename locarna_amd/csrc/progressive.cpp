@@ -163,15 +163,33 @@ extern "C" int lcg_progressive_run(const char *const *pp, int32_t n, const int32
             A[x] = R->text[merges[2 * ready[x]]].c_str();
             B[x] = R->text[merges[2 * ready[x] + 1]].c_str();
         }
-        lcg_prepare_opts o;
-        std::memset(&o, 0, sizeof o);
-        o.flags = LCG_PREPARE_DEVICE_BAND;  // pairs with a profile are banded on the host
-        o.device = device;
+        // pairs of two single sequences are prepared with the device band
+        // and, where the parameters allow it, the device arc-match
+        // compaction; pairs with a profile on a side with the device band
+        // call, which bands them on the host
+        const bool compaction = !params->no_lonely_pairs && !params->stacking && !params->struct_local;
         PairGuard P;
         P.p.assign(k, nullptr);
         const auto tp = clk::now();
-        if (int rc = lcg_pairs_prepare(A.data(), B.data(), k, params, &o, P.p.data(), nullptr, err, errlen))
-            return rc;
+        for (int pass = 0; pass < 2; ++pass) {
+            std::vector<int> idx;
+            for (int x = 0; x < k; ++x) {
+                const bool single = pp_rows(A[x]).size() == 1 && pp_rows(B[x]).size() == 1;
+                if (single == (pass == 0)) idx.push_back(x);
+            }
+            if (idx.empty()) continue;
+            std::vector<const char *> a, b;
+            for (int x : idx) a.push_back(A[x]), b.push_back(B[x]);
+            std::vector<lcg_pair *> got(idx.size(), nullptr);
+            lcg_prepare_opts o;
+            std::memset(&o, 0, sizeof o);
+            o.flags = LCG_PREPARE_DEVICE_BAND | (pass == 0 && compaction ? LCG_PREPARE_DEVICE_COMPACTION : 0);
+            o.device = device;
+            const int rc = lcg_pairs_prepare(a.data(), b.data(), (int)idx.size(), params, &o, got.data(), nullptr,
+                                             err, errlen);
+            for (size_t y = 0; y < idx.size(); ++y) P.p[idx[y]] = got[y];
+            if (rc) return rc;
+        }
         R->prepare_ms += std::chrono::duration<float, std::milli>(clk::now() - tp).count();
         // one batch per kind of sigma table
         std::vector<int> kind(k);
