@@ -107,6 +107,11 @@ def build(verbose=False, sanitize=False):
     _run([CXX, "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-o", exe + ".tmp", os.path.join(CSRC, "locarna_gpu.cpp"),
           "-L" + OUT, "-llocarna_amd", "-Wl,-rpath,$ORIGIN"])
     os.replace(exe + ".tmp", exe)
+    # locarna_rnafold_pp: the folding program mlocarna calls per sequence
+    exe2 = os.path.join(OUT, "locarna_rnafold_pp")
+    _run([CXX, "-O2", "-std=c++17", "-Wall", "-I" + INCLUDE, "-o", exe2 + ".tmp", os.path.join(CSRC, "rnafold_pp.cpp"),
+          "-L" + OUT, "-llocarna_amd", "-Wl,-rpath,$ORIGIN"])
+    os.replace(exe2 + ".tmp", exe2)
     if verbose:
         print("built", lib, exe)
     return lib

@@ -113,3 +113,26 @@ def test_fold_pp_feeds_the_aligner():
     B.run(mode=0, trace=True)
     for k, (a, b) in enumerate(pairs):
         assert B.score(k) == oracle.align(a, b, po).score
+
+
+def test_rnafold_pp_program(tmp_path):
+    """The locarna_rnafold_pp replacement reads the "name sequence" line
+    mlocarna pipes into it (mlocarna:3172) or a FASTA file and writes the
+    same text as lcg_fold_write_pp."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(api.__file__), "_build", "locarna_rnafold_pp")
+    seq = synth.sequences(2, 3, 60, 200, 0.35, 0.65)[2]
+    out = tmp_path / "s2.pp"
+    r = subprocess.run([exe, "-p", "0.0005", "--stacking", "--maxBPspan", "100", "-o", str(out)],
+                       input="s2 %s\n#A1 ....\n" % seq, text=True, capture_output=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    F = api.Fold([seq], max_bp_span=100, min_prob=0.0005, stacking=True)
+    assert out.read_text() == F.pp(0, "s2")
+    fa = tmp_path / "in.fa"
+    fa.write_text(">s2 description\n%s\n%s\n" % (seq[:50], seq[50:]))
+    r = subprocess.run([exe, "--min-prob=0.0005", "--maxBPspan=100", "--stacking", str(fa)], text=True,
+                       capture_output=True, timeout=120)
+    assert r.returncode == 0 and r.stdout == F.pp(0, "s2")
+    r = subprocess.run([exe, "--noLP"], input="x ACGU\n", text=True, capture_output=True, timeout=60)
+    assert r.returncode != 0 and "not supported" in r.stderr
