@@ -515,13 +515,19 @@ void lcg_progressive_free(lcg_progressive *r);
  * probabilities are McCaskill probabilities of that model.  Sequences are
  * upper-cased and T read as U (normalize_rna_sequence, aux.cc:65-70);
  * other symbols do not pair.  LCG_E_CAPACITY for sequences longer than
- * 4096 or whose partition function exceeds the double range. */
+ * 4096, or when no scale keeps a sequence's values in the double range. */
 typedef struct lcg_fold lcg_fold;
 typedef struct {
     int32_t max_bp_span; /* --maxBPspan: pairs with j - i + 1 <= span; -1: no limit */
     int32_t stacking;    /* --stacking: joint probabilities of (i,j) and (i+1,j-1) (#STACK) */
     double min_prob;     /* -p / --min-prob: pairs with p > min_prob are written
                             (locarna_rnafold_pp 0.001; mlocarna passes its --min-prob) */
+    double scale_kcal_per_nt; /* free energy per base (kcal/mol) the partition
+                            function is scaled by (ViennaRNA's pf_scale: a
+                            segment of L bases is stored times
+                            exp(L g / kT)); 0: chosen per sequence, retried
+                            by bisection when a value leaves the double
+                            range.  The probabilities do not depend on it. */
 } lcg_fold_params;
 int lcg_fold_run(const char *const *seqs, int32_t count, const lcg_fold_params *params, int32_t device,
                  lcg_fold **out, char *err, int errlen);

@@ -83,7 +83,8 @@ class LcgError(RuntimeError):
 class FoldParams(ctypes.Structure):
     """lcg_fold_params (include/locarna_amd.h): locarna_rnafold_pp's
     --maxBPspan, --stacking and -p/--min-prob."""
-    _fields_ = [("max_bp_span", ctypes.c_int32), ("stacking", ctypes.c_int32), ("min_prob", ctypes.c_double)]
+    _fields_ = [("max_bp_span", ctypes.c_int32), ("stacking", ctypes.c_int32), ("min_prob", ctypes.c_double),
+                ("scale_kcal_per_nt", ctypes.c_double)]
 
 
 def _sig(L):
@@ -746,10 +747,10 @@ class Fold:
     src/locarna_rnafold_pp.cc:209-271): pp(k, name), probs(k), log_z(k),
     stats()."""
 
-    def __init__(self, seqs, max_bp_span=-1, min_prob=0.001, stacking=False, device=0):
+    def __init__(self, seqs, max_bp_span=-1, min_prob=0.001, stacking=False, device=0, scale_kcal_per_nt=0.0):
         n = len(seqs)
         arr = (ctypes.c_char_p * n)(*[s.encode() for s in seqs])
-        fp = FoldParams(max_bp_span, 1 if stacking else 0, min_prob)
+        fp = FoldParams(max_bp_span, 1 if stacking else 0, min_prob, scale_kcal_per_nt)
         h = ctypes.c_void_p()
         err = ctypes.create_string_buffer(1024)
         self.h = None
@@ -814,7 +815,7 @@ class Mlocarna:
         self.n = n
         nm = (ctypes.c_char_p * n)(*[x.encode() for x in names])
         sq = (ctypes.c_char_p * n)(*[x.encode() for x in seqs])
-        fp = FoldParams(max_bp_span, 0, min_prob)
+        fp = FoldParams(max_bp_span, 0, min_prob, 0.0)
         h = ctypes.c_void_p()
         err = ctypes.create_string_buffer(1024)
         self.h = None

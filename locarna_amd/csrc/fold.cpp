@@ -32,8 +32,10 @@ struct lcg_fold {
     std::vector<int> W;
     std::vector<size_t> out;       // offset of sequence k's P band in `bands`
     std::vector<double> bands;     // P and P2 bands of every sequence
-    std::vector<double> Z;
+    std::vector<double> Z;     // scaled partition functions
+    std::vector<double> logz;  // natural logarithms of the partition functions
     float device_ms = 0, wall_ms = 0;
+    int launches = 0;
     // the last pp text written per sequence and its name: a caller asks for
     // the length first and then for the text
     mutable std::vector<std::string> pp_name, pp_text;
@@ -123,6 +125,7 @@ extern "C" int lcg_fold_run(const char *const *seqs, int32_t count, const lcg_fo
     F->W.resize(count);
     F->out.resize(count);
     F->Z.assign(count, 1.0);
+    F->logz.assign(count, 0.0);
     std::vector<FoldDesc> descs(count);
     std::vector<signed char> codes;
     size_t mat = 0, outd = 0;
@@ -172,8 +175,7 @@ extern "C" int lcg_fold_run(const char *const *seqs, int32_t count, const lcg_fo
         (e = hipMalloc(&dt.p, tabs.size() * sizeof(double))) || (e = hipMalloc(&dm.p, mat * sizeof(double))) ||
         (e = hipMalloc(&dout.p, outd * sizeof(double))) || (e = hipMalloc(&dz.p, count * sizeof(double))))
         return hip_fail(e);
-    if ((e = hipMemcpy(dd.p, descs.data(), sizeof(FoldDesc) * count, hipMemcpyHostToDevice)) ||
-        (e = hipMemcpy(dc.p, codes.data(), codes.size(), hipMemcpyHostToDevice)) ||
+    if ((e = hipMemcpy(dc.p, codes.data(), codes.size(), hipMemcpyHostToDevice)) ||
         (e = hipMemcpy(dt.p, tabs.data(), tabs.size() * sizeof(double), hipMemcpyHostToDevice)))
         return hip_fail(e);
     hipEvent_t ev0, ev1;
@@ -182,33 +184,82 @@ extern "C" int lcg_fold_run(const char *const *seqs, int32_t count, const lcg_fo
         (void)hipEventDestroy(ev0);
         return hip_fail(e);
     }
+    // scale per base: a free energy g per base (kcal/mol) makes the stored
+    // values exp(L g / kT) times the Boltzmann sums of L bases.  Without a
+    // given g, short sequences start unscaled and longer ones at a typical
+    // ensemble free energy per base; a sequence whose values leave the
+    // double range is folded again with g bisected in [-2, 1] (the
+    // direction from overflow or underflow), as ViennaRNA rescales with
+    // its MFE-derived pf_scale.
+    using locarna_amd::fold_model::kT;
+    const bool fixed = params->scale_kcal_per_nt != 0;
+    std::vector<double> g(count), lo(count, -2.0), hi(count, 1.0);
+    for (int k = 0; k < count; ++k)
+        g[k] = fixed ? params->scale_kcal_per_nt : (F->seq[k].size() <= 400 ? 0.0 : -0.35);
+    std::vector<char> ok(count, 0);
     int rc = LCG_OK;
-    (void)hipEventRecord(ev0, 0);
-    int lr = lcg_launch_fold((const FoldDesc *)dd.p, (const signed char *)dc.p, (const double *)dt.p,
-                             (double *)dm.p, (double *)dout.p, (double *)dz.p, count, nmax, 0);
-    (void)hipEventRecord(ev1, 0);
-    if (lr) {
-        rc = hip_fail((hipError_t)lr);
-    } else if ((e = hipEventSynchronize(ev1))) {
-        rc = hip_fail(e);
-    } else {
-        (void)hipEventElapsedTime(&F->device_ms, ev0, ev1);
-        F->bands.resize(outd);
-        if ((e = hipMemcpy(F->bands.data(), dout.p, outd * sizeof(double), hipMemcpyDeviceToHost)) ||
-            (e = hipMemcpy(F->Z.data(), dz.p, count * sizeof(double), hipMemcpyDeviceToHost)))
+    for (int attempt = 0; attempt < 24 && rc == LCG_OK; ++attempt) {
+        for (int k = 0; k < count; ++k) descs[k].log_sigma = -g[k] / kT;
+        if ((e = hipMemcpy(dd.p, descs.data(), sizeof(FoldDesc) * count, hipMemcpyHostToDevice))) {
             rc = hip_fail(e);
+            break;
+        }
+        (void)hipEventRecord(ev0, 0);
+        int lr = lcg_launch_fold((const FoldDesc *)dd.p, (const signed char *)dc.p, (const double *)dt.p,
+                                 (double *)dm.p, (double *)dout.p, (double *)dz.p, count, nmax, 0);
+        (void)hipEventRecord(ev1, 0);
+        if (lr) {
+            rc = hip_fail((hipError_t)lr);
+            break;
+        }
+        if ((e = hipEventSynchronize(ev1))) {
+            rc = hip_fail(e);
+            break;
+        }
+        float t = 0;
+        (void)hipEventElapsedTime(&t, ev0, ev1);
+        F->device_ms += t;
+        ++F->launches;
+        if ((e = hipMemcpy(F->Z.data(), dz.p, count * sizeof(double), hipMemcpyDeviceToHost))) {
+            rc = hip_fail(e);
+            break;
+        }
+        bool all = true;
+        for (int k = 0; k < count; ++k) {
+            const double z = F->Z[k];
+            const bool over = !(z < 1e250);  // also NaN (an infinity times 0)
+            const bool under = z < 1e-250;
+            ok[k] = !over && !under;
+            if (ok[k]) continue;
+            all = false;
+            if (fixed || hi[k] - lo[k] < 1e-6) {
+                set_err(err, errlen, "sequence " + std::to_string(k) + " (length " +
+                                         std::to_string(F->seq[k].size()) +
+                                         "): the partition function leaves the range of double at this scale");
+                rc = LCG_E_CAPACITY;
+                break;
+            }
+            if (over) hi[k] = g[k];
+            else lo[k] = g[k];
+            g[k] = 0.5 * (lo[k] + hi[k]);
+        }
+        if (all || rc) break;
+    }
+    if (rc == LCG_OK)
+        for (int k = 0; k < count; ++k)
+            if (!ok[k]) {
+                set_err(err, errlen, "sequence " + std::to_string(k) + ": no scale keeps the partition function in range");
+                rc = LCG_E_CAPACITY;
+                break;
+            }
+    if (rc == LCG_OK) {
+        F->bands.resize(outd);
+        if ((e = hipMemcpy(F->bands.data(), dout.p, outd * sizeof(double), hipMemcpyDeviceToHost))) rc = hip_fail(e);
+        for (int k = 0; k < count; ++k) F->logz[k] = std::log(F->Z[k]) + (double)F->seq[k].size() * (-g[k] / kT);
     }
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     if (rc) return rc;
-    for (int k = 0; k < count; ++k)
-        if (!(std::isfinite(F->Z[k]) && F->Z[k] >= 1.0)) {
-            // the recursion is unscaled: its values are Boltzmann sums of
-            // double range, exp(709) at most
-            set_err(err, errlen, "sequence " + std::to_string(k) + " (length " + std::to_string(F->seq[k].size()) +
-                                     "): the partition function exceeds the range of double");
-            return LCG_E_CAPACITY;
-        }
     F->wall_ms = std::chrono::duration<float, std::milli>(clk::now() - t0).count();
     *out = F.release();
     return LCG_OK;
@@ -233,7 +284,7 @@ extern "C" int lcg_fold_probs(const lcg_fold *f, int32_t k, double *P, double *P
 
 extern "C" int lcg_fold_log_z(const lcg_fold *f, int32_t k, double *log_z) {
     if (!f || !log_z || k < 0 || k >= (int)f->seq.size()) return LCG_E_ARG;
-    *log_z = std::log(f->Z[k]);
+    *log_z = f->logz[k];
     return LCG_OK;
 }
 

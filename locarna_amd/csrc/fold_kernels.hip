@@ -23,6 +23,12 @@
 //   Qm1(i,j) = sum_{l <= j} Qb(i,l) BR(i,l)
 //   Qm(i,j)  = sum_{u >= i} (1 + Qm(i,u-1)) Qm1(u,j)
 //   Z5(j)    = Z5(j-1) + sum_k Z5(k-1) Qb(k,j) EXT(k,j),  Z = Z5(n)
+// Every value of a segment of L bases is stored times sigma^-L (sigma per
+// sequence, FoldDesc.log_sigma; ViennaRNA's per-nucleotide pf_scale), so
+// each term carries sigma^-(bases it adds): SC[l1 + l2 + 2] on an interior
+// loop, SC[2] on a multiloop's closing pair, SC[d + 1] on a hairpin, SC[j - l]
+// on the unpaired tail of a branch, SC[1] per exterior base.  The
+// probabilities do not depend on sigma; log Z = log Z5(n) + n log sigma.
 #include <hip/hip_runtime.h>
 
 #include "fold_layout.h"
@@ -79,7 +85,8 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
     double *T = lds;                       // fixed factor table
     double *Z5 = T + LCG_FOLD_TAB_FIXED;   // n + 1
     double *Z5b = Z5 + (n + 1);            // n + 1
-    int *ptab = (int *)(Z5b + (n + 1));    // 25
+    double *SC = Z5b + (n + 1);            // n + 2: SC[L] = sigma^-L
+    int *ptab = (int *)(SC + (n + 2));     // 25
     unsigned short *LL = (unsigned short *)(ptab + 28);  // LCG_FOLD_NLOOP: l1 | l2 << 8
     signed char *cs = (signed char *)(LL + LCG_FOLD_NLOOP + 8);  // n + 2
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nw = FT / 64;
@@ -87,6 +94,7 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
 
     for (int x = tid; x < LCG_FOLD_TAB_FIXED; x += FT) T[x] = tabs[x];
     for (int x = tid; x < n + 2; x += FT) cs[x] = (x >= 1 && x <= n) ? codes[D.codes + x] : (signed char)4;
+    for (int x = tid; x < n + 2; x += FT) SC[x] = exp(-(double)x * D.log_sigma);
     if (tid < 25) {
         // canonical pairs AU, CG, GC, UA, GU, UG as types 1..6
         const int a = tid / 5, b = tid % 5;
@@ -129,26 +137,26 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                         const int l1 = LL[e] & 255, l2 = LL[e] >> 8;
                         const int k = i + 1 + l1, l = j - 1 - l2;
                         const int ti = S.pt(k, l);
-                        if (ti) s += loopf(T, l1, l2, t, ti) * Qb[(size_t)k * W + (l - k)];
+                        if (ti) s += loopf(T, l1, l2, t, ti) * SC[l1 + l2 + 2] * Qb[(size_t)k * W + (l - k)];
                     }
                     double mq = 0;
                     for (int u = i + 6 + cl; u <= j - 5; u += CL)
                         mq += Qm[(size_t)(i + 1) * W + (u - 2 - i)] * Qm1[(size_t)u * W + (j - 1 - u)];
                     s = cell_sum(s);
                     mq = cell_sum(mq);
-                    qb = hp[(d - 1) * 2 + weak(t)] + s + mq * mlc * T[LCG_FOLD_TAB_TP + weak(t)];
+                    qb = hp[(d - 1) * 2 + weak(t)] * SC[d + 1] + s + mq * mlc * T[LCG_FOLD_TAB_TP + weak(t)] * SC[2];
                 }
                 double a1 = 0;
                 for (int l = i + 4 + cl; l < j; l += CL) {
                     const int tl = S.pt(i, l);
-                    if (tl) a1 += Qb[(size_t)i * W + (l - i)] * T[LCG_FOLD_TAB_BRANCH + weak(tl)];
+                    if (tl) a1 += Qb[(size_t)i * W + (l - i)] * T[LCG_FOLD_TAB_BRANCH + weak(tl)] * SC[j - l];
                 }
                 a1 = cell_sum(a1);
                 if (t) a1 += qb * T[LCG_FOLD_TAB_BRANCH + weak(t)];
                 qm1 = a1;
                 double a2 = 0;
                 for (int u = i + 1 + cl; u <= j - 4; u += CL)
-                    a2 += (1.0 + Qm[(size_t)i * W + (u - 1 - i)]) * Qm1[(size_t)u * W + (j - u)];
+                    a2 += (SC[u - i] + Qm[(size_t)i * W + (u - 1 - i)]) * Qm1[(size_t)u * W + (j - u)];
                 qm = cell_sum(a2) + qm1;
             }
             if (cl == 0) {
@@ -179,8 +187,8 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                 }
 #pragma unroll
             for (int o = 8; o > 0; o >>= 1) f += __shfl_xor(f, o);
-            const double z0 = zlast + __shfl(f, 0), z1 = z0 + __shfl(f, 16), z2 = z1 + __shfl(f, 32),
-                         z3 = z2 + __shfl(f, 48);
+            const double z0 = zlast * SC[1] + __shfl(f, 0), z1 = z0 * SC[1] + __shfl(f, 16),
+                         z2 = z1 * SC[1] + __shfl(f, 32), z3 = z2 * SC[1] + __shfl(f, 48);
             if (lane == 0) {
                 Z5[j0] = z0;
                 if (j0 + 1 <= n) Z5[j0 + 1] = z1;
@@ -204,8 +212,8 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                 }
 #pragma unroll
             for (int o = 8; o > 0; o >>= 1) f += __shfl_xor(f, o);
-            const double z0 = glast + __shfl(f, 0), z1 = z0 + __shfl(f, 16), z2 = z1 + __shfl(f, 32),
-                         z3 = z2 + __shfl(f, 48);
+            const double z0 = glast * SC[1] + __shfl(f, 0), z1 = z0 * SC[1] + __shfl(f, 16),
+                         z2 = z1 * SC[1] + __shfl(f, 32), z3 = z2 * SC[1] + __shfl(f, 48);
             if (lane == 0) {
                 Z5b[m0] = z0;
                 if (m0 - 1 >= 0) Z5b[m0 - 1] = z1;
@@ -233,7 +241,7 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                         const int t = S.pt(a - 1, j);
                         if (t)
                             s += Qbb[(size_t)(a - 1) * W + (j - a + 1)] * mlc * T[LCG_FOLD_TAB_TP + weak(t)] *
-                                 Qm1[(size_t)(e + 1) * W + (j - e - 2)];
+                                 SC[2] * Qm1[(size_t)(e + 1) * W + (j - e - 2)];
                     }
                 for (int j = e + 5 + cl; j <= min(n, a + W - 1); j += CL)
                     s += Qmb[(size_t)a * W + (j - a)] * Qm1[(size_t)(e + 1) * W + (j - e - 1)];
@@ -245,10 +253,10 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                         const int t = S.pt(i, e + 1);
                         if (t)
                             s += Qbb[(size_t)i * W + (e + 1 - i)] * mlc * T[LCG_FOLD_TAB_TP + weak(t)] *
-                                 Qm[(size_t)(i + 1) * W + (a - 2 - i)];
+                                 SC[2] * Qm[(size_t)(i + 1) * W + (a - 2 - i)];
                     }
                 for (int i = max(1, e - W + 1) + cl; i <= a - 1; i += CL)
-                    s += Qmb[(size_t)i * W + (e - i)] * (1.0 + Qm[(size_t)i * W + (a - 1 - i)]);
+                    s += Qmb[(size_t)i * W + (e - i)] * (SC[a - i] + Qm[(size_t)i * W + (a - 1 - i)]);
                 qm1b = cell_sum(s) + qmb;
                 // Qb(a, e): exterior loop, enclosing pairs, multiloop branches
                 const int t = S.pt(a, e);
@@ -260,10 +268,11 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                         const int i = a - 1 - l1, j = e + 1 + l2;
                         if (i >= 1 && j <= n) {
                             const int to = S.pt(i, j);
-                            if (to) si += Qbb[(size_t)i * W + (j - i)] * loopf(T, l1, l2, to, t);
+                            if (to) si += Qbb[(size_t)i * W + (j - i)] * loopf(T, l1, l2, to, t) * SC[l1 + l2 + 2];
                         }
                     }
-                    for (int j = e + 1 + cl; j <= min(n, a + W - 1); j += CL) sb += Qm1b[(size_t)a * W + (j - a)];
+                    for (int j = e + 1 + cl; j <= min(n, a + W - 1); j += CL)
+                        sb += Qm1b[(size_t)a * W + (j - a)] * SC[j - e];
                     si = cell_sum(si);
                     sb = cell_sum(sb);
                     qbb = Z5b[e] * Z5[a - 1] * T[LCG_FOLD_TAB_EXT + weak(t)] + si +
@@ -273,7 +282,7 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                     if (d >= 6) {
                         const int ti = S.pt(a + 1, e - 1);
                         if (ti)
-                            p2 = qbb * T[LCG_FOLD_TAB_STACK + t * 8 + ti] * Qb[(size_t)(a + 1) * W + (d - 2)] / Z;
+                            p2 = qbb * T[LCG_FOLD_TAB_STACK + t * 8 + ti] * SC[2] * Qb[(size_t)(a + 1) * W + (d - 2)] / Z;
                     }
                 }
             }
@@ -291,7 +300,7 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
 }
 
 extern "C" size_t lcg_fold_lds_bytes(int nmax) {
-    return (size_t)(LCG_FOLD_TAB_FIXED + 2 * (nmax + 1)) * sizeof(double) + 28 * sizeof(int) +
+    return (size_t)(LCG_FOLD_TAB_FIXED + 2 * (nmax + 1) + (nmax + 2)) * sizeof(double) + 28 * sizeof(int) +
            (LCG_FOLD_NLOOP + 8) * sizeof(unsigned short) + (size_t)(nmax + 2);
 }
 

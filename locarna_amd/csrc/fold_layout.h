@@ -26,7 +26,7 @@
 #define LCG_FOLD_TAB_MLC 3978     // multiloop closing (closing + one branch)
 #define LCG_FOLD_TAB_FIXED 3980
 #define LCG_FOLD_TAB_HP 3980      // [L * 2 + weak]: hairpin of L unpaired bases
-#define LCG_FOLD_MAX_LEN 4096     // longest sequence (Z5 and Z5b live in LDS)
+#define LCG_FOLD_MAX_LEN 4096     // longest sequence (Z5, Z5b and the scale powers live in LDS)
 #define LCG_FOLD_THREADS 1024
 
 struct FoldDesc {
@@ -35,6 +35,7 @@ struct FoldDesc {
     long long codes; // offset of the codes: position i at codes + i (0 A, 1 C, 2 G, 3 U, 4 other)
     long long mat;   // offset (doubles) of the six bands Qb, Qm, Qm1, Qbb, Qmb, Qm1b
     long long out;   // offset (doubles) of the two bands P, P2
+    double log_sigma; // scale per base: values of L bases are stored times exp(-L log_sigma)
 };
 
 #endif

@@ -69,6 +69,7 @@ int main(int argc, char **argv) {
     fp.max_bp_span = -1;
     fp.stacking = 0;
     fp.min_prob = 0.001;
+    fp.scale_kcal_per_nt = 0;
     std::string input = "-", output;
     for (int a = 1; a < argc; ++a) {
         const std::string o = argv[a];

@@ -96,11 +96,45 @@ def test_fold_edge_cases():
     assert F.probs(5)[0].max() > 0.1  # T read as U: the hairpin forms
 
 
-def test_fold_overflow_is_reported():
-    # a perfect 200-pair GC helix: log Z ~ 5 per pair exceeds the double range
-    seq = "G" * 200 + "AAAA" + "C" * 200
+def test_fold_scaled_matches_oracle():
+    """A forced scale per base changes how values are stored, not the
+    probabilities or log Z."""
+    seqs = synth.sequences(2, 64, 60, 200, 0.35, 0.65)[:8]
+    for g in (-0.45, 0.2):
+        print("fold scale", g, flush=True)
+        F = api.Fold(seqs, max_bp_span=100, stacking=True, scale_kcal_per_nt=g)
+        for k, seq in enumerate(seqs):
+            P, P2 = F.probs(k)
+            Po, P2o, lz = oracle.fold(seq, 100)
+            assert np.abs(P - Po).max() <= 1e-10
+            assert np.abs(P2 - P2o).max() <= 1e-10
+            assert abs(F.log_z(k) - lz) <= 1e-11 * abs(lz)
+
+
+def test_fold_beyond_double_range():
+    """A perfect 200-pair GC helix (log Z above 709, exp(709) being the
+    double range) and two random 2,500-nt sequences fold with the scale
+    chosen by bisection."""
+    helix = "G" * 200 + "AAAA" + "C" * 200
+    rnd = synth.sequences(9, 2, 2500, 2500, 0.5, 0.5)
+    print("fold helix and 2 x 2500 nt", flush=True)
+    F = api.Fold([helix], max_bp_span=-1)
+    assert F.log_z(0) > 709
+    P, P2 = F.probs(0)
+    n = len(helix)
+    assert min(P[i][n + 1 - i] for i in range(5, 195)) > 0.9
+    assert P2[10][n - 9] > 0.9
+    G = api.Fold(rnd, max_bp_span=150)
+    for k in range(2):
+        assert G.log_z(k) > 709
+        P, P2 = G.probs(k)
+        occ = P.sum(axis=1) + P.sum(axis=0)
+        assert occ.max() <= 1.0 + 1e-9 and P.min() >= 0
+        inner = np.zeros_like(P)
+        inner[:-1, 1:] = P[1:, :-1]
+        assert (P2 <= np.minimum(P, inner) + 1e-9).all()
     with pytest.raises(api.LcgError, match="range of double"):
-        api.Fold([seq])
+        api.Fold([helix], scale_kcal_per_nt=0.5)
 
 
 def test_fold_pp_feeds_the_aligner():
