@@ -133,11 +133,15 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                 if (t) {
                     double s = 0;
                     const int cnt = loop_count(min(LCG_FOLD_MAXLOOP, d - 6));
+#pragma unroll 4
                     for (int e = cl; e < cnt; e += CL) {
                         const int l1 = LL[e] & 255, l2 = LL[e] >> 8;
                         const int k = i + 1 + l1, l = j - 1 - l2;
                         const int ti = S.pt(k, l);
-                        if (ti) s += loopf(T, l1, l2, t, ti) * SC[l1 + l2 + 2] * Qb[(size_t)k * W + (l - k)];
+                        // the load does not wait for the pair test: (k, l)
+                        // lies in the band and holds 0 where it cannot pair
+                        const double q = Qb[(size_t)k * W + (l - k)];
+                        s += (ti ? loopf(T, l1, l2, t, ti) * SC[l1 + l2 + 2] : 0.0) * q;
                     }
                     double mq = 0;
                     for (int u = i + 6 + cl; u <= j - 5; u += CL)
@@ -263,13 +267,16 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                 if (t) {
                     double si = 0, sb = 0;
                     const int cnt = loop_count(min(LCG_FOLD_MAXLOOP, W - 3 - d));
+#pragma unroll 4
                     for (int q = cl; q < cnt; q += CL) {
                         const int l1 = LL[q] & 255, l2 = LL[q] >> 8;
                         const int i = a - 1 - l1, j = e + 1 + l2;
-                        if (i >= 1 && j <= n) {
-                            const int to = S.pt(i, j);
-                            if (to) si += Qbb[(size_t)i * W + (j - i)] * loopf(T, l1, l2, to, t) * SC[l1 + l2 + 2];
-                        }
+                        const bool in = i >= 1 && j <= n;
+                        // (i, j) in range lies in the band (cnt bounds l1 + l2); the load
+                        // does not wait for the pair test, cells that cannot pair hold 0
+                        const double o = in ? Qbb[(size_t)i * W + (j - i)] : 0.0;
+                        const int to = in ? S.pt(i, j) : 0;
+                        si += (to ? loopf(T, l1, l2, to, t) * SC[l1 + l2 + 2] : 0.0) * o;
                     }
                     for (int j = e + 1 + cl; j <= min(n, a + W - 1); j += CL)
                         sb += Qm1b[(size_t)a * W + (j - a)] * SC[j - e];
