@@ -153,7 +153,8 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                 double a1 = 0;
                 for (int l = i + 4 + cl; l < j; l += CL) {
                     const int tl = S.pt(i, l);
-                    if (tl) a1 += Qb[(size_t)i * W + (l - i)] * T[LCG_FOLD_TAB_BRANCH + weak(tl)] * SC[j - l];
+                    const double q = Qb[(size_t)i * W + (l - i)];  // 0 where (i, l) cannot pair
+                    a1 += (tl ? T[LCG_FOLD_TAB_BRANCH + weak(tl)] * SC[j - l] : 0.0) * q;
                 }
                 a1 = cell_sum(a1);
                 if (t) a1 += qb * T[LCG_FOLD_TAB_BRANCH + weak(t)];
@@ -187,7 +188,8 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
             if (j <= n)
                 for (int k = max(1, j - W + 1) + gl; k <= j - 4; k += 16) {
                     const int t = S.pt(k, j);
-                    if (t) f += Z5[k - 1] * Qb[(size_t)k * W + (j - k)] * T[LCG_FOLD_TAB_EXT + weak(t)];
+                    const double q = Qb[(size_t)k * W + (j - k)];
+                    f += (t ? Z5[k - 1] * T[LCG_FOLD_TAB_EXT + weak(t)] : 0.0) * q;
                 }
 #pragma unroll
             for (int o = 8; o > 0; o >>= 1) f += __shfl_xor(f, o);
@@ -212,7 +214,8 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
             if (m >= 0)
                 for (int j = m + 5 + gl; j <= min(n, m + W); j += 16) {
                     const int t = S.pt(m + 1, j);
-                    if (t) f += Z5b[j] * Qb[(size_t)(m + 1) * W + (j - m - 1)] * T[LCG_FOLD_TAB_EXT + weak(t)];
+                    const double q = Qb[(size_t)(m + 1) * W + (j - m - 1)];
+                    f += (t ? Z5b[j] * T[LCG_FOLD_TAB_EXT + weak(t)] : 0.0) * q;
                 }
 #pragma unroll
             for (int o = 8; o > 0; o >>= 1) f += __shfl_xor(f, o);
@@ -243,9 +246,8 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                 if (a >= 2)
                     for (int j = e + 6 + cl; j <= min(n, a + W - 2); j += CL) {
                         const int t = S.pt(a - 1, j);
-                        if (t)
-                            s += Qbb[(size_t)(a - 1) * W + (j - a + 1)] * mlc * T[LCG_FOLD_TAB_TP + weak(t)] *
-                                 SC[2] * Qm1[(size_t)(e + 1) * W + (j - e - 2)];
+                        const double o = Qbb[(size_t)(a - 1) * W + (j - a + 1)], q = Qm1[(size_t)(e + 1) * W + (j - e - 2)];
+                        s += (t ? mlc * T[LCG_FOLD_TAB_TP + weak(t)] * SC[2] : 0.0) * o * q;
                     }
                 for (int j = e + 5 + cl; j <= min(n, a + W - 1); j += CL)
                     s += Qmb[(size_t)a * W + (j - a)] * Qm1[(size_t)(e + 1) * W + (j - e - 1)];
@@ -255,9 +257,8 @@ extern "C" __global__ __launch_bounds__(FT) void lcg_fold_seqs(const FoldDesc *d
                 if (e + 1 <= n)
                     for (int i = max(1, e + 2 - W) + cl; i <= a - 6; i += CL) {
                         const int t = S.pt(i, e + 1);
-                        if (t)
-                            s += Qbb[(size_t)i * W + (e + 1 - i)] * mlc * T[LCG_FOLD_TAB_TP + weak(t)] *
-                                 SC[2] * Qm[(size_t)(i + 1) * W + (a - 2 - i)];
+                        const double o = Qbb[(size_t)i * W + (e + 1 - i)], q = Qm[(size_t)(i + 1) * W + (a - 2 - i)];
+                        s += (t ? mlc * T[LCG_FOLD_TAB_TP + weak(t)] * SC[2] : 0.0) * o * q;
                     }
                 for (int i = max(1, e - W + 1) + cl; i <= a - 1; i += CL)
                     s += Qmb[(size_t)i * W + (e - i)] * (SC[a - i] + Qm[(size_t)i * W + (a - 1 - i)]);
