@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cctype>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -728,16 +729,15 @@ std::string clustal_blocks(const std::vector<std::pair<std::string, const std::s
 }
 
 std::string format_prob(double p) {
-    std::ostringstream fixed;
-    fixed.precision(4);
-    fixed << p;
-    std::string txt = fixed.str();
+    // a stream with precision 4 and the default float field writes what
+    // printf's %.4g writes, the scientific field with precision 3 what %.3e
+    // writes (the streams' num_put is specified by those conversions)
+    char buf[64];
+    std::snprintf(buf, sizeof buf, "%.4g", p);
+    std::string txt = buf;
     if (txt.size() > 8) {  // longer than precision + 4 characters
-        std::ostringstream sci;
-        sci.setf(std::ios::scientific, std::ios::floatfield);
-        sci.precision(3);
-        sci << p;
-        txt = sci.str();
+        std::snprintf(buf, sizeof buf, "%.3e", p);
+        txt = buf;
     }
     const size_t e = txt.find("e-0");
     if (e != std::string::npos) txt.replace(e, 3, "e-");
