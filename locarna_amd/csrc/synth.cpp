@@ -11,7 +11,8 @@
 //    (AU, CG, GU), minimum hairpin of 3, stacking energies, size-dependent
 //    hairpin / bulge / interior loop initiation (loops up to 30 unpaired),
 //    a linear multiloop model and terminal AU/GU penalties, with a maximum
-//    base-pair span.
+//    base-pair span (the model is fold_model.h, shared with the device
+//    folding of fold_kernels.hip, which computes the same probabilities).
 //
 // The energies are a simplification of the Turner 2004 set; the probabilities
 // are therefore McCaskill-like, not RNAfold-identical.  They only serve as
