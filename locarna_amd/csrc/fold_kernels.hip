@@ -7,7 +7,7 @@
 // diagonals d = j - i upward; the cells of a diagonal are independent, a
 // quarter wave (16 lanes) per cell, and the lanes of a cell split its terms
 // (interior loops, multiloop splits) and sum them with a butterfly; 16
-// lanes per cell measured faster than 32 or 64 on 64 sequences of 60-200 nt
+// lanes per cell measured faster than 8, 32 or 64 on 64 sequences of 60-200 nt
 // (the work is latency-bound: more cells in flight per wave).  The exterior
 // prefix sums Z5 (forward) and Z5b (their adjoints) run on one wave.  The
 // outside pass walks the diagonals downward and gathers, for every cell, the
